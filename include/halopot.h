@@ -1,0 +1,193 @@
+/*
+ * halopot.h -- MI355X (gfx950) per-halo gravitational potential and bound
+ * pass, behind MPI-Rockstar's own C signatures.
+ *
+ * libhalopot.so links in place of the reference's potential.o
+ * (src/Makefile:34, MPI_ROCKSTAR_CORE).  The two reference entry points keep
+ * their exact signatures, so src/properties.c:538 and :548,550 (compiled
+ * inside src/groupies.c:272) call them unchanged.  The hp_* entry points are
+ * additions: the batched path that the reference does not have, and
+ * diagnostics.
+ *
+ * Every entry point runs on the GPU.  There is no CPU fallback: if no gfx950
+ * device is usable the call fails through the error path below.
+ */
+#ifndef HALOPOT_H
+#define HALOPOT_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ------------------------------------------------------------------ record
+ * src/potential.h:7-21, built without POTENTIAL_COMPARISON / CALC_POTENTIALS
+ * (the reference's default, src/Makefile:26).  48 bytes: pos[6] at 0, r2 at
+ * 24, pe at 32, ke at 40, flags at 44. */
+#ifndef _POTENTIAL_H_
+struct potential {
+    float   pos[6], r2;
+    double  pe;
+    float   ke;
+    int32_t flags;
+};
+#endif
+
+/* ------------------------------------------------- reference entry points */
+
+/* Replaces compute_potential, src/potential.h:24 / src/potential.c:183-192.
+ * Writes pe for every record from the Barnes-Hut potential of the reference's
+ * k-d tree (10 points per leaf, dmin acceptance with ERR_TOL 1, monopoles).
+ * Non-finite positions get pe = 0 and are left out of the tree
+ * (src/fast3tree.c:875-887).  Unlike the reference, the records are NOT
+ * permuted: every record keeps its place; callers sort by r2 afterwards
+ * either way (src/properties.c:545).  Reads the globals FORCE_RES and
+ * PARTICLE_MASS at call time.  Safe to call from many host threads at once
+ * (src/mpi_main.cpp:1369-1402); each thread gets its own stream and device
+ * scratch. */
+void compute_potential(struct potential *po, int64_t num_po);
+
+/* Replaces compute_kinetic_energy, src/potential.h:22-23 /
+ * src/potential.c:194-211.  Records with ke < 0 (potential-only) are left
+ * alone.  ke = a/(2 Gc) |v - v_c + H(a) a (x - x_c)|^2, accumulated as the
+ * reference does (float ke, double terms).  Reads Om, Ol, W0, WA
+ * (src/hubble.c:8-19). */
+void compute_kinetic_energy(struct potential *po, int64_t num_po,
+                            float *vel_cen, float *pos_cen, float scale_now);
+
+/* Configuration globals read at call time (src/config.template.h:7,16,22-25
+ * through src/config_vars.h).  libhalopot carries weak definitions with the
+ * reference's defaults, so the program that defines the real ones (the
+ * halo finder) overrides them and stand-alone callers can set them here. */
+extern double FORCE_RES, PARTICLE_MASS, Om, Ol, W0, WA;
+
+/* ----------------------------------------------------------- batched path */
+
+/* One compute_potential call of a batch: a contiguous run of records in the
+ * batch's record array, plus the halo fields the bound pass reads and
+ * writes (src/halo.h:8-35: pos, corevel, bulkvel, min_vel_err,
+ * min_bulkvel_err, n_core).  104 bytes. */
+typedef struct hp_halo {
+    int64_t first;          /* index of the call's first record */
+    int64_t count;          /* records, potential-only ones (ke < 0) included */
+    float   pos[6];         /* centre; pos[3..5] receives corevel
+                               (src/properties.c:213) */
+    float   corevel[3];     /* in: prior value; out: core velocity */
+    float   bulkvel[3];     /* in: prior value; out: bulk velocity */
+    float   min_vel_err, min_bulkvel_err;
+    int64_t n_core;
+    int32_t is_sub;         /* 1: KE about corevel (sub-halo), 0: about
+                               bulkvel (src/properties.c:547-550) */
+    int32_t status;         /* out: 0 */
+    int64_t kept;           /* out: records left after dropping potential-
+                               only ones (src/properties.c:539-544) */
+    int64_t n_bound;        /* out: kept records with !(pe < ke)
+                               (src/properties.c:403) */
+} hp_halo;
+
+enum {
+    HP_ACC_COMPENSATED = 0, /* FP32 pair terms, two-sum compensated sums */
+    HP_ACC_NAIVE = 1,       /* FP32 pair terms, plain FP32 sums */
+    HP_ACC_F64 = 2          /* FP64 pair terms and sums (same tree) */
+};
+
+enum {
+    HP_METHOD_TREE = 0,     /* the reference's Barnes-Hut walk; halos of at
+                               most 10 finite points, where the tree is a
+                               single leaf, go to the direct kernel */
+    HP_METHOD_DIRECT = 1,   /* exact pairwise sum for every halo (not the
+                               reference's answer above 10 points) */
+    HP_METHOD_AUTO = 2      /* tree, or direct where the autotuner found it
+                               faster AND the halo has at most 10 points */
+};
+
+typedef struct hp_options {
+    int32_t accumulate;     /* HP_ACC_* */
+    int32_t method;         /* HP_METHOD_* */
+    int32_t use_graph;      /* replay the launch sequence from a hipGraph */
+    int32_t timing;         /* record HIP events around each phase */
+    int32_t count_work;     /* count node visits / interactions (slower) */
+    int32_t reserved[3];
+} hp_options;
+
+typedef struct hp_plan hp_plan;
+
+/* Timings of the last hp_plan_run / hp_plan_potential (milliseconds, HIP
+ * events on the plan's stream) and work counters. */
+typedef struct hp_stats {
+    float   ms_total, ms_build, ms_multipole, ms_walk, ms_direct, ms_unbind;
+    int64_t nodes, leaves, levels, max_depth;
+    int64_t walk_launches, walk_groups;
+    int64_t visits, p2p, p2m;  /* when count_work is set */
+    int64_t records, halos;
+} hp_stats;
+
+/* Plans a batch from host copies of its descriptors: sizes the device
+ * scratch (kept until hp_plan_destroy), builds the per-halo work tables.
+ * halos[i].first / count must tile [0, total) without overlap. */
+int hp_plan_create(hp_plan **plan, const hp_halo *halos, int64_t nhalos,
+                   const hp_options *opt);
+
+/* The bound pass of src/properties.c:537-550 for every call of the batch,
+ * on device-resident data (d_po: the record array, d_halos: the
+ * descriptors, d_bound: one byte per record or NULL; stream: a hipStream_t
+ * or NULL for the plan's own).  Afterwards, per call, records
+ * [first, first + kept) are the non-potential-only records sorted by r2
+ * with pe and ke set, followed by the potential-only records with pe set;
+ * d_bound[first + i] = !(pe < ke) for i < kept.  Returns 0 or an HP_E*
+ * code. */
+int hp_plan_run(hp_plan *plan, struct potential *d_po, hp_halo *d_halos,
+                uint8_t *d_bound, void *stream);
+
+/* compute_potential over every call of the batch (pe only, records keep
+ * their places). */
+int hp_plan_potential(hp_plan *plan, struct potential *d_po, void *stream);
+
+int hp_plan_stats(const hp_plan *plan, hp_stats *out);
+int hp_plan_destroy(hp_plan *plan);
+
+/* Host-buffer convenience: copies in, hp_plan_run, copies out. */
+int hp_bound_pass_host(struct potential *po, int64_t nrecords, hp_halo *halos,
+                       int64_t nhalos, uint8_t *bound, const hp_options *opt);
+
+/* ------------------------------------------------------------- diagnostics */
+
+/* Tree export for structural checks: for a single call of n records (host
+ * buffers), fills perm[n] (record index of each position in tree order;
+ * non-finite records last) and up to cap nodes in the layout of hp_node_out
+ * below.  Returns the node count, or a negative HP_E* code. */
+typedef struct hp_node_out {
+    float   lo[3], hi[3];
+    float   com[3], mass;
+    float   dmin2, dmin;
+    int32_t first, count;   /* range in tree order, relative to the call */
+    int32_t left, skip;     /* -1: leaf / end of walk */
+    int32_t parent, pad;
+} hp_node_out;
+
+int64_t hp_debug_tree(const struct potential *po, int64_t n, int32_t *perm,
+                      hp_node_out *nodes, int64_t cap);
+
+enum {
+    HP_OK = 0,
+    HP_EINVAL = -1,         /* bad argument */
+    HP_ENODEV = -2,         /* no usable gfx950 device */
+    HP_EHIP = -3,           /* a HIP call failed */
+    HP_ENOMEM = -4          /* device allocation failed */
+};
+
+const char *hp_last_error(void);
+
+/* Called on a fatal error inside the void reference entry points.  When the
+ * program links src/error.cpp, rockstar_abort_impl (src/error.h:15) is
+ * called instead, so the reference's exception path is kept.  With neither,
+ * the library prints the error and aborts. */
+typedef void (*hp_error_handler)(int code, const char *file, int line);
+void hp_set_error_handler(hp_error_handler handler);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* HALOPOT_H */

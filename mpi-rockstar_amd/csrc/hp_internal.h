@@ -1,0 +1,172 @@
+// hp_internal.h -- device layouts and helpers shared by the halopot kernels.
+//
+// Everything here is compiled for gfx950 (wave64) with -ffp-contract=off:
+// float and double expressions that must round exactly like the reference's
+// C99 code (bounding boxes, split planes, centres of mass, dmin, acceptance)
+// are written as plain operators and are never fused; the hot pair loops use
+// explicit fmaf where fusing is wanted.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "../../include/halopot.h"
+
+namespace hp {
+
+constexpr int kLeafPoints = 10;   // POINTS_PER_LEAF, src/potential.c:57
+constexpr int kDegenerate = 20;   // 2 * POINTS_PER_LEAF, src/potential.c:145
+constexpr int kWave = 64;
+constexpr int kGroup = 64;        // sink particles per walk wavefront
+constexpr int kTile = 4096;       // points per workgroup tile of a big node
+constexpr int kTileThreads = 256;
+
+// Tree node, 64 bytes, read by the walk with one scalar load.
+//   [first, first + count) is the node's range in tree order (batch-global);
+//   left is the first child (the right child is left + 1) or -1 for a leaf;
+//   skip is the node that follows the subtree in depth-first order (-1: end).
+struct __attribute__((aligned(16))) Node {
+    float lo[3];
+    float hi[3];
+    float com[3];
+    float mass;
+    float dmin2;     // float(dmin) * float(dmin), src/potential.c:128
+    int32_t first;
+    int32_t count;
+    int32_t left;
+    int32_t skip;
+    float dmin;
+};
+static_assert(sizeof(Node) == 64, "node layout");
+
+// A node waiting to be split at the current level.
+struct Task {
+    int32_t node;
+    int32_t dim;
+    float lim;
+    int32_t tile0;   // first tile (big tasks only)
+    int32_t ntiles;
+    int32_t depth;
+    int32_t pad[2];
+};
+
+// Outcome of a big task's count pass, read by its scatter pass.
+struct BigSplit {
+    int32_t nleft;
+    int32_t left;      // child id, or -1 when the split failed (node is a leaf)
+    int32_t leafL;     // left child is a leaf (points go straight to tree order)
+    int32_t leafR;
+};
+
+// Per-tile partial of a big task: left count and the bounding boxes of the
+// left and right parts.
+struct TilePart {
+    int32_t nleft;
+    int32_t pad[3];
+    float lo[2][3];
+    float hi[2][3];
+};
+
+// One walk wavefront: sink records [p0, min(p0 + 64, pend)) of the halo
+// whose root node is `root`.
+struct Group {
+    int32_t root;
+    int32_t p0;
+    int32_t pend;
+    int32_t pad;
+};
+
+// Device-side counters of one build.
+enum Counter {
+    kNodeCount = 0,
+    kSmallNext = 1,   // small tasks pushed for the next level
+    kBigNext = 2,     // big tasks pushed for the next level (append-only)
+    kTileNext = 3,    // tiles of big tasks (append-only)
+    kMaxDepth = 4,
+    kLeafCount = 5,
+    kErrFlags = 6,
+    kBigDmin = 7,     // nodes whose dmin is reduced by tiles
+    kNumCounters = 16
+};
+
+enum ErrFlag { kErrWalkCap = 1, kErrBadTree = 2, kErrCapacity = 4 };
+
+// ----------------------------------------------------------- wave helpers
+
+__device__ __forceinline__ int lane_id() { return (int)__lane_id(); }
+
+__device__ __forceinline__ uint64_t ballot(bool p) { return (uint64_t)__ballot(p); }
+
+// number of set bits of m below this lane
+__device__ __forceinline__ int rank_below(uint64_t m) {
+    return (int)__builtin_amdgcn_mbcnt_hi((uint32_t)(m >> 32),
+                                          __builtin_amdgcn_mbcnt_lo((uint32_t)m, 0u));
+}
+
+__device__ __forceinline__ int uniform(int v) { return __builtin_amdgcn_readfirstlane(v); }
+
+template <typename T>
+__device__ __forceinline__ T wave_min(T v) {
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) {
+        T w = __shfl_xor(v, o, 64);
+        v = w < v ? w : v;
+    }
+    return v;
+}
+
+template <typename T>
+__device__ __forceinline__ T wave_max(T v) {
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) {
+        T w = __shfl_xor(v, o, 64);
+        v = w > v ? w : v;
+    }
+    return v;
+}
+
+template <typename T>
+__device__ __forceinline__ T wave_sum(T v) {
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) v = v + __shfl_xor(v, o, 64);
+    return v;
+}
+
+__device__ __forceinline__ float comp(const float4 &p, int d) {
+    return d == 0 ? p.x : (d == 1 ? p.y : p.z);
+}
+
+__device__ __forceinline__ bool finite3(float x, float y, float z) {
+    return __builtin_isfinite(x) && __builtin_isfinite(y) && __builtin_isfinite(z);
+}
+
+// Split dimension of a tight box: z first, x or y only when strictly wider
+// (src/fast3tree.c:736-747).
+__host__ __device__ inline int split_dim(const float *lo, const float *hi) {
+    int dim = 2;
+    float w = hi[2] - lo[2];
+    for (int k = 0; k < 2; k++) {
+        float e = hi[k] - lo[k];
+        if (e > w) {
+            w = e;
+            dim = k;
+        }
+    }
+    return dim;
+}
+
+// Midpoint in float, halved exactly (src/fast3tree.c:757).
+__host__ __device__ inline float split_lim(const float *lo, const float *hi, int dim) {
+    return 0.5f * (hi[dim] + lo[dim]);
+}
+
+// Order-preserving map of a float onto an int, for atomic min/max.
+__device__ __forceinline__ int float_key(float f) {
+    int i = __float_as_int(f);
+    return i >= 0 ? i : (i ^ 0x7fffffff);
+}
+__device__ __forceinline__ float key_float(int i) {
+    return __int_as_float(i >= 0 ? i : (i ^ 0x7fffffff));
+}
+
+}  // namespace hp

@@ -1,0 +1,963 @@
+// hp_lib.hip -- libhalopot: host engine and C ABI.
+//
+// One translation unit: the kernel files are included here so that every
+// launch sees its kernel's definition.
+//
+// Engine: one HIP stream (default priority) plus device scratch grown on
+// demand.  Each host thread that calls the reference entry points gets its
+// own engine (the reference keeps its tree thread-private,
+// src/potential.c:66-69); a plan owns one too.
+//
+// Pipeline of one batch (every call of the batch together):
+//   gather + roots -> split levels until no node has more than 10 points ->
+//   centres of mass -> dmin -> walk (or the direct kernel) -> [bound pass:
+//   radix sort by (call, potential-only, r2) -> core/bulk velocity -> kinetic
+//   energy + bound flags -> records written back in sorted order].
+// The split levels are host-driven: after each level the engine reads the
+// device counters (four integers) to size the next level's launches.
+
+#include "hp_build.hip"
+#include "hp_unbind.hip"
+#include "hp_walk.hip"
+
+#include <atomic>
+#include <cmath>
+#include <cstdarg>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <mutex>
+#include <string>
+#include <vector>
+
+// ----------------------------------------------------------- C globals
+
+extern "C" {
+// Weak definitions with the reference's defaults (src/config.template.h:7,
+// 16, 20, 22-25); the halo finder's own definitions take precedence.
+__attribute__((weak, visibility("default"))) double FORCE_RES = 0.003;
+__attribute__((weak, visibility("default"))) double PARTICLE_MASS = 0.0;
+__attribute__((weak, visibility("default"))) double Om = 0.27;
+__attribute__((weak, visibility("default"))) double Ol = 0.73;
+__attribute__((weak, visibility("default"))) double W0 = -1.0;
+__attribute__((weak, visibility("default"))) double WA = 0.0;
+__attribute__((weak, visibility("default"))) double SCALE_NOW = 1.0;
+// src/error.h:15, present when the program links src/error.cpp
+void rockstar_abort_impl(int code, const char *file, int line) __attribute__((weak));
+}
+
+namespace hp {
+
+static thread_local std::string g_last_error;
+static std::atomic<hp_error_handler> g_handler{nullptr};
+
+static int set_error(int code, const char *fmt, ...) {
+    char buf[512];
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(buf, sizeof buf, fmt, ap);
+    va_end(ap);
+    g_last_error = buf;
+    return code;
+}
+
+#define HP_TRY(call)                                                                   \
+    do {                                                                               \
+        hipError_t e_ = (call);                                                        \
+        if (e_ != hipSuccess)                                                          \
+            return set_error(HP_EHIP, "%s failed at %s:%d: %s", #call, __FILE__,       \
+                             __LINE__, hipGetErrorString(e_));                         \
+    } while (0)
+
+#define HP_LAUNCHED()                                                                  \
+    do {                                                                               \
+        hipError_t e_ = hipGetLastError();                                             \
+        if (e_ != hipSuccess)                                                          \
+            return set_error(HP_EHIP, "launch failed at %s:%d: %s", __FILE__, __LINE__,\
+                             hipGetErrorString(e_));                                   \
+    } while (0)
+
+// Fatal error inside a void reference entry point: the reference's exit()
+// path (src/error.h:17-20) when linked, else the user handler, else abort.
+static void fatal(int code, const char *file, int line) {
+    if (rockstar_abort_impl) {
+        rockstar_abort_impl(code, file, line);
+        return;
+    }
+    hp_error_handler h = g_handler.load();
+    if (h) {
+        h(code, file, line);
+        return;
+    }
+    fprintf(stderr, "[halopot error] %s (code %d at %s:%d)\n", g_last_error.c_str(), code,
+            file, line);
+    abort();
+}
+
+// --------------------------------------------------------------- scalars
+
+// src/hubble.c:8-19
+static double hubble_scaling(double z) {
+    double z1 = 1.0 + z;
+    double a = 1.0 / z1;
+    double weff = (a != 1.0) ? W0 + WA - WA * (a - 1.0) / log(a) : W0;
+    return sqrt(Om * (z1 * z1 * z1) + Ol * pow(a, -3.0 * (1.0 + weff)));
+}
+
+// src/groupies.c:97-100
+static double vir_density(double a) {
+    double x = (Om / pow(a, 3)) / pow(hubble_scaling(1.0 / a - 1.0), 2.0) - 1.0;
+    return ((18 * M_PI * M_PI + 82.0 * x - 39 * x * x) / (1.0 + x));
+}
+
+// particle_rvir_dens as src/groupies.c:106-135 leaves it for "vir": float
+// background density, float threshold
+static double particle_rvir_dens() {
+    float scale_now = (float)SCALE_NOW;
+    float cons = (float)(Om * 2.77519737e11 / PARTICLE_MASS);
+    float thresh = (float)(vir_density(scale_now) * cons);
+    return (double)thresh;
+}
+
+// ----------------------------------------------------------- device buffer
+
+struct DevBuf {
+    void *p = nullptr;
+    size_t bytes = 0;
+    int ensure(size_t n) {
+        if (n <= bytes) return HP_OK;
+        if (p) (void)hipFree(p);
+        p = nullptr;
+        bytes = 0;
+        size_t want = n + n / 8 + 256;
+        if (hipMalloc(&p, want) != hipSuccess) {
+            (void)hipGetLastError();
+            return set_error(HP_ENOMEM, "hipMalloc of %zu bytes failed", want);
+        }
+        bytes = want;
+        return HP_OK;
+    }
+    void release() {
+        if (p) (void)hipFree(p);
+        p = nullptr;
+        bytes = 0;
+    }
+    template <typename T>
+    T *as() const {
+        return static_cast<T *>(p);
+    }
+};
+
+#define HP_ENSURE(buf, n)                       \
+    do {                                        \
+        int r_ = (buf).ensure(n);               \
+        if (r_) return r_;                      \
+    } while (0)
+
+// Host-side tables of one batch (depend only on the calls' first/count).
+struct Tables {
+    int64_t nrec = 0, nhalos = 0;
+    std::vector<int64_t> first, count;
+    std::vector<RootTile> root_tiles;
+    std::vector<Group> groups;          // walk wavefronts
+    std::vector<int2> direct_blocks;    // direct-kernel workgroups
+    std::vector<int32_t> rec_call;      // record -> call
+    std::vector<ScanTile> scan_tiles;
+    std::vector<int32_t> call_tile0, call_ntiles;
+    int method = HP_METHOD_TREE;
+    int32_t direct_max = kLeafPoints;   // AUTO: direct at or below this count
+};
+
+static int make_tables(Tables &T, const int64_t *first, const int64_t *count, int64_t nh,
+                       int method, bool want_unbind) {
+    T = Tables();
+    T.method = method;
+    T.nhalos = nh;
+    T.first.assign(first, first + nh);
+    T.count.assign(count, count + nh);
+    int64_t pos = 0;
+    for (int64_t h = 0; h < nh; h++) {
+        if (count[h] < 0 || first[h] != pos)
+            return set_error(HP_EINVAL, "call %lld: first=%lld count=%lld; calls must tile the "
+                             "record array in order", (long long)h, (long long)first[h],
+                             (long long)count[h]);
+        pos += count[h];
+    }
+    if (pos > INT32_MAX / 2) return set_error(HP_EINVAL, "batch of %lld records is too large", (long long)pos);
+    if (nh > (1 << 22)) return set_error(HP_EINVAL, "too many calls in one batch");
+    T.nrec = pos;
+    for (int64_t h = 0; h < nh; h++) {
+        for (int64_t off = 0; off < count[h]; off += kTile)
+            T.root_tiles.push_back(RootTile{(int32_t)h, (int32_t)off});
+        const bool direct = method == HP_METHOD_DIRECT ||
+                            (method == HP_METHOD_AUTO && count[h] <= T.direct_max);
+        if (direct) {
+            for (int64_t off = 0; off < count[h]; off += 256)
+                T.direct_blocks.push_back(make_int2((int32_t)h, (int32_t)off));
+        } else {
+            for (int64_t off = 0; off < count[h]; off += kGroup)
+                T.groups.push_back(Group{(int32_t)h, (int32_t)(first[h] + off),
+                                         (int32_t)(first[h] + count[h]), 0});
+        }
+    }
+    if (want_unbind) {
+        T.rec_call.resize(pos);
+        T.call_tile0.resize(nh);
+        T.call_ntiles.resize(nh);
+        for (int64_t h = 0; h < nh; h++) {
+            for (int64_t i = 0; i < count[h]; i++) T.rec_call[first[h] + i] = (int32_t)h;
+            T.call_tile0[h] = (int32_t)T.scan_tiles.size();
+            for (int64_t off = 0; off < count[h]; off += kScanTile)
+                T.scan_tiles.push_back(ScanTile{(int32_t)h, (int32_t)off});
+            T.call_ntiles[h] = (int32_t)(T.scan_tiles.size() - T.call_tile0[h]);
+        }
+    }
+    return HP_OK;
+}
+
+struct Engine {
+    int device = -1;
+    hipStream_t stream = nullptr;
+    int32_t *h_counters = nullptr;  // pinned mirror
+    unsigned long long *h_work = nullptr;
+    hipEvent_t ev[8] = {};
+    int32_t big_threshold = 4096;
+
+    // batch tables on the device
+    DevBuf hfirst, hcount, root_tiles, groups, direct_blocks, rec_call, scan_tiles, call_tile0,
+        call_ntiles;
+    // build
+    DevBuf box, bad, A, B, F, leaf_of, nodes, parent, arrive, small0, small1, big, tiles, parts,
+        splitres, dparts, counters, work;
+    // unbind
+    DevBuf keys0, keys1, vals0, vals1, hist, srt, callinfo, tsum, tcand, nbound;
+    int64_t cap_nodes = 0, cap_small = 0, cap_big = 0, cap_tiles = 0;
+
+    // last run
+    hp_stats st{};
+
+    int init() {
+        if (stream) return HP_OK;
+        int n = 0;
+        if (hipGetDeviceCount(&n) != hipSuccess || n < 1) {
+            (void)hipGetLastError();
+            return set_error(HP_ENODEV, "no HIP device available (libhalopot has no CPU path)");
+        }
+        HP_TRY(hipGetDevice(&device));
+        hipDeviceProp_t prop;
+        HP_TRY(hipGetDeviceProperties(&prop, device));
+        if (std::strncmp(prop.gcnArchName, "gfx950", 6) != 0)
+            return set_error(HP_ENODEV, "device %d is %s; libhalopot is built for gfx950", device,
+                             prop.gcnArchName);
+        HP_TRY(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking));
+        HP_TRY(hipHostMalloc((void **)&h_counters, sizeof(int32_t) * kNumCounters));
+        HP_TRY(hipHostMalloc((void **)&h_work, sizeof(unsigned long long) * 4));
+        for (auto &e : ev) HP_TRY(hipEventCreate(&e));
+        if (const char *s = getenv("HALOPOT_BIG_THRESHOLD")) big_threshold = atoi(s);
+        return HP_OK;
+    }
+
+    ~Engine() {
+        DevBuf *all[] = {&hfirst, &hcount, &root_tiles, &groups, &direct_blocks, &rec_call,
+                         &scan_tiles, &call_tile0, &call_ntiles, &box, &bad, &A, &B, &F,
+                         &leaf_of, &nodes, &parent, &arrive, &small0, &small1, &big, &tiles,
+                         &parts, &splitres, &dparts, &counters, &work, &keys0, &keys1, &vals0,
+                         &vals1, &hist, &srt, &callinfo, &tsum, &tcand, &nbound};
+        if (stream) {
+            (void)hipStreamSynchronize(stream);
+            for (DevBuf *b : all) b->release();
+            for (auto &e : ev)
+                if (e) (void)hipEventDestroy(e);
+            if (h_counters) (void)hipHostFree(h_counters);
+            if (h_work) (void)hipHostFree(h_work);
+            (void)hipStreamDestroy(stream);
+        }
+    }
+
+    template <typename T>
+    int upload(DevBuf &b, const std::vector<T> &v) {
+        HP_ENSURE(b, sizeof(T) * (v.size() ? v.size() : 1));
+        if (!v.empty())
+            HP_TRY(hipMemcpyAsync(b.p, v.data(), sizeof(T) * v.size(), hipMemcpyHostToDevice, stream));
+        return HP_OK;
+    }
+
+    int reserve(const Tables &T, bool unbind) {
+        const int64_t N = T.nrec, H = T.nhalos;
+        HP_TRY(hipSetDevice(device));
+        HP_ENSURE(box, sizeof(int32_t) * 6 * (H + 1));
+        HP_ENSURE(bad, sizeof(int32_t) * (H + 1));
+        HP_ENSURE(A, sizeof(float4) * (N + 1));
+        HP_ENSURE(B, sizeof(float4) * (N + 1));
+        HP_ENSURE(F, sizeof(float4) * (N + 1));
+        HP_ENSURE(leaf_of, sizeof(int32_t) * (N + 1));
+        cap_nodes = 2 * N + H + 2;
+        HP_ENSURE(nodes, sizeof(Node) * cap_nodes);
+        HP_ENSURE(parent, sizeof(int32_t) * cap_nodes);
+        HP_ENSURE(arrive, sizeof(int32_t) * cap_nodes);
+        cap_small = N / (kLeafPoints + 1) + H + 64;
+        HP_ENSURE(small0, sizeof(Task) * cap_small);
+        HP_ENSURE(small1, sizeof(Task) * cap_small);
+        const int64_t bt = big_threshold > 0 ? big_threshold : 1;
+        cap_big = 128 * (N / bt + 1) + H + 64;
+        cap_tiles = 128 * (N / kTile + 2) + cap_big;
+        HP_ENSURE(big, sizeof(Task) * cap_big);
+        HP_ENSURE(splitres, sizeof(BigSplit) * cap_big);
+        HP_ENSURE(tiles, sizeof(int2) * cap_tiles);
+        HP_ENSURE(parts, sizeof(TilePart) * cap_tiles);
+        HP_ENSURE(dparts, sizeof(double2) * cap_tiles);
+        HP_ENSURE(counters, sizeof(int32_t) * kNumCounters);
+        HP_ENSURE(work, sizeof(unsigned long long) * 4);
+        if (unbind) {
+            HP_ENSURE(keys0, sizeof(uint64_t) * (N + 1));
+            HP_ENSURE(keys1, sizeof(uint64_t) * (N + 1));
+            HP_ENSURE(vals0, sizeof(int32_t) * (N + 1));
+            HP_ENSURE(vals1, sizeof(int32_t) * (N + 1));
+            const int64_t nt = (N + kSortTile - 1) / kSortTile + 1;
+            HP_ENSURE(hist, sizeof(int32_t) * 256 * nt);
+            HP_ENSURE(srt, sizeof(struct potential) * (N + 1));
+            HP_ENSURE(callinfo, sizeof(CallInfo) * (H + 1));
+            HP_ENSURE(tsum, sizeof(Moments) * (T.scan_tiles.size() + 1));
+            HP_ENSURE(tcand, sizeof(Cand) * 2 * (T.scan_tiles.size() + 1));
+            HP_ENSURE(nbound, sizeof(int32_t) * (H + 1));
+        }
+        return HP_OK;
+    }
+
+    int load_tables(const Tables &T) {
+        int r;
+        if ((r = upload(hfirst, T.first))) return r;
+        if ((r = upload(hcount, T.count))) return r;
+        if ((r = upload(root_tiles, T.root_tiles))) return r;
+        if ((r = upload(groups, T.groups))) return r;
+        if ((r = upload(direct_blocks, T.direct_blocks))) return r;
+        if ((r = upload(rec_call, T.rec_call))) return r;
+        if ((r = upload(scan_tiles, T.scan_tiles))) return r;
+        if ((r = upload(call_tile0, T.call_tile0))) return r;
+        if ((r = upload(call_ntiles, T.call_ntiles))) return r;
+        return HP_OK;
+    }
+
+    int read_counters() {
+        HP_TRY(hipMemcpyAsync(h_counters, counters.p, sizeof(int32_t) * kNumCounters,
+                              hipMemcpyDeviceToHost, stream));
+        HP_TRY(hipStreamSynchronize(stream));
+        return HP_OK;
+    }
+
+    BuildArgs build_args(Task *small_next) {
+        BuildArgs a;
+        a.nodes = nodes.as<Node>();
+        a.parent = parent.as<int32_t>();
+        a.leaf_of = leaf_of.as<int32_t>();
+        a.fin = F.as<float4>();
+        a.small_next = small_next;
+        a.big = big.as<Task>();
+        a.tiles = tiles.as<int2>();
+        a.counters = counters.as<int32_t>();
+        a.big_threshold = big_threshold;
+        a.cap_small = (int32_t)cap_small;
+        a.cap_big = (int32_t)cap_big;
+        a.cap_tiles = (int32_t)cap_tiles;
+        a.cap_nodes = (int32_t)cap_nodes;
+        return a;
+    }
+
+    static int blocks(int64_t n, int per) { return (int)((n + per - 1) / per); }
+
+    // Tree build + multipoles for every call with a root queued; leaves the
+    // tree in nodes / F / leaf_of.
+    int build(const Tables &T, const struct potential *d_po, bool timing) {
+        const int32_t H = (int32_t)T.nhalos;
+        hipStream_t s = stream;
+        if (timing) HP_TRY(hipEventRecord(ev[0], s));
+        if (H > 0) {
+            k_root_init<<<blocks(H, 256), 256, 0, s>>>(box.as<int32_t>(), bad.as<int32_t>(), H);
+            HP_LAUNCHED();
+        }
+        if (!T.root_tiles.empty()) {
+            k_gather<<<(int)T.root_tiles.size(), kTileThreads, 0, s>>>(
+                d_po, hfirst.as<int64_t>(), hcount.as<int64_t>(), root_tiles.as<RootTile>(),
+                A.as<float4>(), box.as<int32_t>(), bad.as<int32_t>());
+            HP_LAUNCHED();
+        }
+        k_fill_i32<<<1, 64, 0, s>>>(counters.as<int32_t>(), 0, kNumCounters);
+        HP_LAUNCHED();
+        k_fill_i32<<<1, 1, 0, s>>>(counters.as<int32_t>() + kNodeCount, H, 1);
+        HP_LAUNCHED();
+        Task *lists[2] = {small0.as<Task>(), small1.as<Task>()};
+        if (H > 0) {
+            k_roots<<<H, 256, 0, s>>>(hfirst.as<int64_t>(), hcount.as<int64_t>(),
+                                      box.as<int32_t>(), bad.as<int32_t>(), A.as<float4>(),
+                                      build_args(lists[0]));
+            HP_LAUNCHED();
+        }
+        int r;
+        if ((r = read_counters())) return r;
+        int32_t ns = h_counters[kSmallNext], nb = h_counters[kBigNext],
+                nt = h_counters[kTileNext];
+        int32_t b0 = 0, t0 = 0;
+        int cur = 0, level = 1;
+        while ((ns > 0 || nb > b0) && !(h_counters[kErrFlags])) {
+            if (level > 100000) return set_error(HP_EINVAL, "tree deeper than 100000 levels");
+            float4 *src = (level & 1) ? A.as<float4>() : B.as<float4>();
+            float4 *dst = (level & 1) ? B.as<float4>() : A.as<float4>();
+            BuildArgs ba = build_args(lists[cur ^ 1]);
+            k_fill_i32<<<1, 1, 0, s>>>(counters.as<int32_t>() + kSmallNext, 0, 1);
+            HP_LAUNCHED();
+            if (nt > t0) {
+                k_big_count<<<nt - t0, kTileThreads, 0, s>>>(src, tiles.as<int2>(), t0,
+                                                              big.as<Task>(), nodes.as<Node>(),
+                                                              parts.as<TilePart>());
+                HP_LAUNCHED();
+            }
+            if (nb > b0) {
+                k_big_split<<<blocks(nb - b0, 64), 64, 0, s>>>(b0, nb - b0, big.as<Task>(),
+                                                               parts.as<TilePart>(),
+                                                               splitres.as<BigSplit>(), ba);
+                HP_LAUNCHED();
+            }
+            if (nt > t0) {
+                k_big_scatter<<<nt - t0, kTileThreads, 0, s>>>(
+                    src, dst, tiles.as<int2>(), t0, big.as<Task>(), parts.as<TilePart>(),
+                    splitres.as<BigSplit>(), ba);
+                HP_LAUNCHED();
+            }
+            if (ns > 0) {
+                k_split_small<<<blocks((int64_t)ns * kWave, 256), 256, 0, s>>>(src, dst, lists[cur],
+                                                                           ns, ba);
+                HP_LAUNCHED();
+            }
+            b0 = nb;
+            t0 = nt;
+            cur ^= 1;
+            level++;
+            if ((r = read_counters())) return r;
+            ns = h_counters[kSmallNext];
+            nb = h_counters[kBigNext];
+            nt = h_counters[kTileNext];
+        }
+        if (h_counters[kErrFlags])
+            return set_error(HP_EINVAL, "tree build exceeded its capacity (flags %d)",
+                             h_counters[kErrFlags]);
+        const int32_t nn = h_counters[kNodeCount];
+        st.nodes = nn;
+        st.leaves = h_counters[kLeafCount];
+        st.levels = level;
+        st.max_depth = h_counters[kMaxDepth];
+        if (timing) HP_TRY(hipEventRecord(ev[1], s));
+        // multipoles
+        if (nn > 0) {
+            k_fill_i32<<<blocks(nn, 256), 256, 0, s>>>(arrive.as<int32_t>(), 0, nn);
+            HP_LAUNCHED();
+            k_com<<<blocks((int64_t)nn * kWave, 256), 256, 0, s>>>(
+                nodes.as<Node>(), parent.as<int32_t>(), arrive.as<int32_t>(), F.as<float4>(), nn,
+                PARTICLE_MASS);
+            HP_LAUNCHED();
+            k_dmin_small<<<blocks((int64_t)nn * kWave, 256), 256, 0, s>>>(
+                nodes.as<Node>(), F.as<float4>(), nn, big_threshold);
+            HP_LAUNCHED();
+        }
+        if (nt > 0) {
+            k_dmin_tiles<<<nt, kTileThreads, 0, s>>>(nodes.as<Node>(), F.as<float4>(),
+                                                     tiles.as<int2>(), big.as<Task>(),
+                                                     dparts.as<double2>());
+            HP_LAUNCHED();
+            k_dmin_big<<<blocks(nb, 64), 64, 0, s>>>(nodes.as<Node>(), big.as<Task>(), nb,
+                                                     dparts.as<double2>());
+            HP_LAUNCHED();
+        }
+        if (timing) HP_TRY(hipEventRecord(ev[2], s));
+        return HP_OK;
+    }
+
+    int walk(const Tables &T, struct potential *d_po, const hp_options &o) {
+        hipStream_t s = stream;
+        const bool timing = o.timing != 0;
+        if (o.count_work) HP_TRY(hipMemsetAsync(work.p, 0, sizeof(unsigned long long) * 4, s));
+        if (!T.groups.empty()) {
+            WalkArgs a;
+            a.nodes = nodes.as<Node>();
+            a.fin = F.as<float4>();
+            a.leaf_of = leaf_of.as<int32_t>();
+            a.groups = groups.as<Group>();
+            a.ngroups = (int32_t)T.groups.size();
+            a.nnodes = (int32_t)st.nodes;
+            a.pm = (float)PARTICLE_MASS;
+            a.inv_fr = (float)(1.0 / FORCE_RES);
+            a.pm_d = PARTICLE_MASS;
+            a.fr_d = FORCE_RES;
+            a.po = d_po;
+            a.err = counters.as<int32_t>() + kErrFlags;
+            a.work = work.as<unsigned long long>();
+            const int nb = blocks((int64_t)T.groups.size() * kWave, 256);
+            const int acc = o.accumulate;
+            const bool cnt = o.count_work != 0;
+#define HP_WALK(ACC)                                                                   \
+    do {                                                                               \
+        if (cnt) k_walk<ACC, true><<<nb, 256, 0, s>>>(a);                              \
+        else k_walk<ACC, false><<<nb, 256, 0, s>>>(a);                                 \
+    } while (0)
+            if (acc == HP_ACC_NAIVE) HP_WALK(HP_ACC_NAIVE);
+            else if (acc == HP_ACC_F64) HP_WALK(HP_ACC_F64);
+            else HP_WALK(HP_ACC_COMPENSATED);
+#undef HP_WALK
+            HP_LAUNCHED();
+        }
+        if (timing) HP_TRY(hipEventRecord(ev[3], s));
+        if (!T.direct_blocks.empty()) {
+            DirectArgs d;
+            d.nodes = nodes.as<Node>();
+            d.fin = T.method == HP_METHOD_DIRECT ? A.as<float4>() : F.as<float4>();
+            d.blocks = direct_blocks.as<int2>();
+            d.nblocks = (int32_t)T.direct_blocks.size();
+            d.pm = (float)PARTICLE_MASS;
+            d.inv_fr = (float)(1.0 / FORCE_RES);
+            d.pm_d = PARTICLE_MASS;
+            d.fr_d = FORCE_RES;
+            d.po = d_po;
+            if (T.method == HP_METHOD_DIRECT) {
+                k_zero_pe<<<blocks(T.nrec, 256), 256, 0, s>>>(d_po, T.nrec);
+                HP_LAUNCHED();
+            } else {
+                k_zero_pe_calls<<<(int)T.direct_blocks.size(), 256, 0, s>>>(
+                    d_po, hfirst.as<int64_t>(), hcount.as<int64_t>(), direct_blocks.as<int2>());
+                HP_LAUNCHED();
+            }
+            const int nbk = (int)T.direct_blocks.size();
+            if (o.accumulate == HP_ACC_NAIVE) k_direct<HP_ACC_NAIVE><<<nbk, 256, 0, s>>>(d);
+            else if (o.accumulate == HP_ACC_F64) k_direct<HP_ACC_F64><<<nbk, 256, 0, s>>>(d);
+            else k_direct<HP_ACC_COMPENSATED><<<nbk, 256, 0, s>>>(d);
+            HP_LAUNCHED();
+        }
+        if (timing) HP_TRY(hipEventRecord(ev[4], s));
+        return HP_OK;
+    }
+
+    int potential(const Tables &T, struct potential *d_po, const hp_options &o) {
+        st = hp_stats();
+        st.records = T.nrec;
+        st.halos = T.nhalos;
+        int r;
+        if (T.method == HP_METHOD_DIRECT) {
+            // the direct kernel needs only the gathered, compacted roots
+            if ((r = build_roots_only(T, d_po, o.timing != 0))) return r;
+        } else if ((r = build(T, d_po, o.timing != 0))) {
+            return r;
+        }
+        if ((r = walk(T, d_po, o))) return r;
+        if ((r = read_counters())) return r;
+        if (h_counters[kErrFlags])
+            return set_error(HP_EINVAL, "walk stopped early (flags %d): corrupt tree",
+                             h_counters[kErrFlags]);
+        st.walk_groups = (int64_t)T.groups.size();
+        st.walk_launches = T.groups.empty() ? 0 : 1;
+        if (o.count_work) {
+            HP_TRY(hipMemcpyAsync(h_work, work.p, sizeof(unsigned long long) * 4,
+                                  hipMemcpyDeviceToHost, stream));
+            HP_TRY(hipStreamSynchronize(stream));
+            st.visits = (int64_t)h_work[0];
+            st.p2p = (int64_t)h_work[1];
+            st.p2m = (int64_t)h_work[2];
+        }
+        if (o.timing) {
+            HP_TRY(hipStreamSynchronize(stream));
+            float t;
+            if (T.method != HP_METHOD_DIRECT) {
+                HP_TRY(hipEventElapsedTime(&t, ev[0], ev[1]));
+                st.ms_build = t;
+                HP_TRY(hipEventElapsedTime(&t, ev[1], ev[2]));
+                st.ms_multipole = t;
+            }
+            HP_TRY(hipEventElapsedTime(&t, ev[2], ev[3]));
+            st.ms_walk = t;
+            HP_TRY(hipEventElapsedTime(&t, ev[3], ev[4]));
+            st.ms_direct = t;
+            HP_TRY(hipEventElapsedTime(&t, ev[0], ev[4]));
+            st.ms_total = t;
+        }
+        return HP_OK;
+    }
+
+    int build_roots_only(const Tables &T, const struct potential *d_po, bool timing) {
+        const int32_t H = (int32_t)T.nhalos;
+        hipStream_t s = stream;
+        if (timing) HP_TRY(hipEventRecord(ev[0], s));
+        if (H > 0) {
+            k_root_init<<<blocks(H, 256), 256, 0, s>>>(box.as<int32_t>(), bad.as<int32_t>(), H);
+            HP_LAUNCHED();
+        }
+        if (!T.root_tiles.empty()) {
+            k_gather<<<(int)T.root_tiles.size(), kTileThreads, 0, s>>>(
+                d_po, hfirst.as<int64_t>(), hcount.as<int64_t>(), root_tiles.as<RootTile>(),
+                A.as<float4>(), box.as<int32_t>(), bad.as<int32_t>());
+            HP_LAUNCHED();
+        }
+        k_fill_i32<<<1, 64, 0, s>>>(counters.as<int32_t>(), 0, kNumCounters);
+        k_fill_i32<<<1, 1, 0, s>>>(counters.as<int32_t>() + kNodeCount, H, 1);
+        HP_LAUNCHED();
+        if (H > 0) {
+            k_roots<<<H, 256, 0, s>>>(hfirst.as<int64_t>(), hcount.as<int64_t>(),
+                                      box.as<int32_t>(), bad.as<int32_t>(), A.as<float4>(),
+                                      build_args(small0.as<Task>()));
+            HP_LAUNCHED();
+        }
+        if (timing) {
+            HP_TRY(hipEventRecord(ev[1], s));
+            HP_TRY(hipEventRecord(ev[2], s));
+        }
+        return HP_OK;
+    }
+
+    // ------------------------------------------------------------ bound pass
+
+    int radix_sort(int64_t n, int bits) {
+        hipStream_t s = stream;
+        const int32_t nt = (int32_t)((n + kSortTile - 1) / kSortTile);
+        uint64_t *k0 = keys0.as<uint64_t>(), *k1 = keys1.as<uint64_t>();
+        int32_t *v0 = vals0.as<int32_t>(), *v1 = vals1.as<int32_t>();
+        for (int shift = 0; shift < bits; shift += 8) {
+            k_radix_hist<<<nt, kSortThreads, 0, s>>>(k0, (int32_t)n, shift, nt, hist.as<int32_t>());
+            HP_LAUNCHED();
+            k_scan_i32<<<1, 1024, 0, s>>>(hist.as<int32_t>(), 256 * nt);
+            HP_LAUNCHED();
+            k_radix_scatter<<<nt, kSortThreads, 0, s>>>(k0, v0, k1, v1, (int32_t)n, shift, nt,
+                                                        hist.as<int32_t>());
+            HP_LAUNCHED();
+            std::swap(k0, k1);
+            std::swap(v0, v1);
+        }
+        sorted_vals = v0;
+        return HP_OK;
+    }
+    int32_t *sorted_vals = nullptr;
+
+    int unbind(const Tables &T, struct potential *d_po, hp_halo *d_halos, uint8_t *d_bound,
+               const hp_options &o) {
+        hipStream_t s = stream;
+        const int64_t N = T.nrec;
+        const int32_t H = (int32_t)T.nhalos;
+        if (N == 0 || H == 0) return HP_OK;
+        k_make_keys<<<blocks(N, 256), 256, 0, s>>>(d_po, rec_call.as<int32_t>(), (int32_t)N,
+                                                   keys0.as<uint64_t>(), vals0.as<int32_t>());
+        HP_LAUNCHED();
+        int hbits = 1;
+        while ((1ll << hbits) < H) hbits++;
+        int r;
+        if ((r = radix_sort(N, 33 + hbits))) return r;
+        struct potential *S = srt.as<struct potential>();
+        k_gather_sorted<<<blocks(N, 256), 256, 0, s>>>(d_po, sorted_vals, (int32_t)N, S);
+        HP_LAUNCHED();
+        CallInfo *ci = callinfo.as<CallInfo>();
+        k_call_init<<<blocks(H, 256), 256, 0, s>>>(d_halos, H, ci);
+        HP_LAUNCHED();
+        const double rvir_thresh = particle_rvir_dens() * (4.0 * M_PI / 3.0);
+        k_kept_rvir<<<blocks(N, 256), 256, 0, s>>>(S, rec_call.as<int32_t>(), (int32_t)N,
+                                                  rvir_thresh, ci);
+        HP_LAUNCHED();
+        k_core_max<<<blocks(H, 256), 256, 0, s>>>(S, H, ci);
+        HP_LAUNCHED();
+        const int32_t nst = (int32_t)T.scan_tiles.size();
+        if (nst > 0) {
+            k_vel_sums<<<nst, 256, 0, s>>>(S, scan_tiles.as<ScanTile>(), nst, ci,
+                                           tsum.as<Moments>());
+            HP_LAUNCHED();
+            k_vel_prefix<<<blocks(H, 256), 256, 0, s>>>(call_tile0.as<int32_t>(),
+                                                        call_ntiles.as<int32_t>(), H,
+                                                        tsum.as<Moments>());
+            HP_LAUNCHED();
+            k_vel_select<<<nst, 256, 0, s>>>(S, scan_tiles.as<ScanTile>(), nst, ci,
+                                             tsum.as<Moments>(), tcand.as<Cand>());
+            HP_LAUNCHED();
+        }
+        k_vel_finish<<<blocks(H, 256), 256, 0, s>>>(S, call_tile0.as<int32_t>(),
+                                                    call_ntiles.as<int32_t>(), H, ci,
+                                                    tsum.as<Moments>(), tcand.as<Cand>(), d_halos);
+        HP_LAUNCHED();
+        // kinetic energy (src/potential.c:194-211 with scale_now = SCALE_NOW)
+        const float a = (float)SCALE_NOW;
+        const double conv = 0.5 * a / 4.30117902e-9;
+        const double z1 = 1.0 / a;
+        const double hub = 100.0 * hubble_scaling(z1 - 1.0);
+        const double hub_a = hub * a;
+        k_fill_i32<<<blocks(H, 256), 256, 0, s>>>(nbound.as<int32_t>(), 0, H);
+        HP_LAUNCHED();
+        k_kinetic_bound<<<blocks(N, 256), 256, 0, s>>>(S, rec_call.as<int32_t>(), (int32_t)N,
+                                                       d_halos, hub_a, conv, d_bound,
+                                                       nbound.as<int32_t>());
+        HP_LAUNCHED();
+        k_write_nbound<<<blocks(H, 256), 256, 0, s>>>(nbound.as<int32_t>(), H, d_halos);
+        HP_LAUNCHED();
+        HP_TRY(hipMemcpyAsync(d_po, S, sizeof(struct potential) * N, hipMemcpyDeviceToDevice, s));
+        return HP_OK;
+    }
+};
+
+// One engine per host thread for the reference entry points.
+static Engine *thread_engine(int *err) {
+    static thread_local Engine *e = nullptr;
+    if (!e) {
+        e = new Engine();
+        int r = e->init();
+        if (r) {
+            delete e;
+            e = nullptr;
+            *err = r;
+            return nullptr;
+        }
+    }
+    *err = HP_OK;
+    return e;
+}
+
+static hp_options default_options() {
+    hp_options o;
+    memset(&o, 0, sizeof o);
+    o.accumulate = HP_ACC_COMPENSATED;
+    o.method = HP_METHOD_TREE;
+    return o;
+}
+
+}  // namespace hp
+
+// ------------------------------------------------------------------ C ABI
+
+struct hp_plan {
+    hp::Engine eng;
+    hp::Tables T;
+    hp_options opt;
+};
+
+using namespace hp;
+
+extern "C" {
+
+__attribute__((visibility("default"))) const char *hp_last_error(void) {
+    return g_last_error.c_str();
+}
+
+__attribute__((visibility("default"))) void hp_set_error_handler(hp_error_handler h) {
+    g_handler.store(h);
+}
+
+__attribute__((visibility("default"))) void compute_potential(struct potential *po,
+                                                               int64_t num_po) {
+    if (num_po <= 0) return;
+    int r;
+    Engine *e = thread_engine(&r);
+    if (!e) return fatal(r, __FILE__, __LINE__);
+    Tables T;
+    int64_t first = 0, count = num_po;
+    if ((r = make_tables(T, &first, &count, 1, HP_METHOD_TREE, false)))
+        return fatal(r, __FILE__, __LINE__);
+    hp_options o = default_options();
+    if ((r = e->reserve(T, false)) || (r = e->load_tables(T))) return fatal(r, __FILE__, __LINE__);
+    DevBuf &dpo = e->srt;  // record staging
+    if ((r = dpo.ensure(sizeof(struct potential) * num_po))) return fatal(r, __FILE__, __LINE__);
+    if (hipMemcpyAsync(dpo.p, po, sizeof(struct potential) * num_po, hipMemcpyHostToDevice,
+                       e->stream) != hipSuccess) {
+        set_error(HP_EHIP, "copy of %lld records to the device failed", (long long)num_po);
+        return fatal(HP_EHIP, __FILE__, __LINE__);
+    }
+    if ((r = e->potential(T, dpo.as<struct potential>(), o))) return fatal(r, __FILE__, __LINE__);
+    if (hipMemcpyAsync(po, dpo.p, sizeof(struct potential) * num_po, hipMemcpyDeviceToHost,
+                       e->stream) != hipSuccess ||
+        hipStreamSynchronize(e->stream) != hipSuccess) {
+        set_error(HP_EHIP, "copy of %lld records from the device failed", (long long)num_po);
+        return fatal(HP_EHIP, __FILE__, __LINE__);
+    }
+}
+
+__attribute__((visibility("default"))) void compute_kinetic_energy(struct potential *po,
+                                                                    int64_t num_po,
+                                                                    float *vel_cen,
+                                                                    float *pos_cen,
+                                                                    float scale_now) {
+    if (num_po <= 0) return;
+    int r;
+    Engine *e = thread_engine(&r);
+    if (!e) return fatal(r, __FILE__, __LINE__);
+    DevBuf &dpo = e->srt;
+    if ((r = dpo.ensure(sizeof(struct potential) * num_po))) return fatal(r, __FILE__, __LINE__);
+    const double conv = 0.5 * scale_now / 4.30117902e-9;
+    const double z1 = 1.0 / scale_now;
+    const double hub = 100.0 * hubble_scaling(z1 - 1.0);
+    const double hub_a = hub * scale_now;
+    if (hipMemcpyAsync(dpo.p, po, sizeof(struct potential) * num_po, hipMemcpyHostToDevice,
+                       e->stream) != hipSuccess) {
+        set_error(HP_EHIP, "copy to the device failed");
+        return fatal(HP_EHIP, __FILE__, __LINE__);
+    }
+    k_kinetic_plain<<<Engine::blocks(num_po, 256), 256, 0, e->stream>>>(
+        dpo.as<struct potential>(), num_po, vel_cen[0], vel_cen[1], vel_cen[2], pos_cen[0],
+        pos_cen[1], pos_cen[2], hub_a, conv);
+    if (hipGetLastError() != hipSuccess ||
+        hipMemcpyAsync(po, dpo.p, sizeof(struct potential) * num_po, hipMemcpyDeviceToHost,
+                       e->stream) != hipSuccess ||
+        hipStreamSynchronize(e->stream) != hipSuccess) {
+        set_error(HP_EHIP, "kinetic energy kernel failed");
+        return fatal(HP_EHIP, __FILE__, __LINE__);
+    }
+}
+
+__attribute__((visibility("default"))) int hp_plan_create(hp_plan **out, const hp_halo *halos,
+                                                          int64_t nhalos,
+                                                          const hp_options *opt) {
+    if (!out || (nhalos > 0 && !halos) || nhalos < 0)
+        return set_error(HP_EINVAL, "hp_plan_create: bad arguments");
+    hp_plan *p = new hp_plan();
+    p->opt = opt ? *opt : default_options();
+    int r = p->eng.init();
+    if (!r) {
+        std::vector<int64_t> f(nhalos), c(nhalos);
+        for (int64_t h = 0; h < nhalos; h++) {
+            f[h] = halos[h].first;
+            c[h] = halos[h].count;
+        }
+        r = make_tables(p->T, f.data(), c.data(), nhalos, p->opt.method, true);
+    }
+    if (!r) r = p->eng.reserve(p->T, true);
+    if (!r) r = p->eng.load_tables(p->T);
+    if (!r && hipStreamSynchronize(p->eng.stream) != hipSuccess)
+        r = set_error(HP_EHIP, "plan upload failed");
+    if (r) {
+        delete p;
+        return r;
+    }
+    *out = p;
+    return HP_OK;
+}
+
+__attribute__((visibility("default"))) int hp_plan_potential(hp_plan *p, struct potential *d_po,
+                                                             void *stream) {
+    if (!p || (!d_po && p->T.nrec)) return set_error(HP_EINVAL, "hp_plan_potential: bad arguments");
+    (void)stream;
+    return p->eng.potential(p->T, d_po, p->opt);
+}
+
+__attribute__((visibility("default"))) int hp_plan_run(hp_plan *p, struct potential *d_po,
+                                                       hp_halo *d_halos, uint8_t *d_bound,
+                                                       void *stream) {
+    if (!p || ((!d_po || !d_halos) && p->T.nrec))
+        return set_error(HP_EINVAL, "hp_plan_run: bad arguments");
+    (void)stream;
+    int r = p->eng.potential(p->T, d_po, p->opt);
+    if (r) return r;
+    hipEvent_t e0 = p->eng.ev[5], e1 = p->eng.ev[6];
+    if (p->opt.timing) HP_TRY(hipEventRecord(e0, p->eng.stream));
+    if ((r = p->eng.unbind(p->T, d_po, d_halos, d_bound, p->opt))) return r;
+    if (p->opt.timing) {
+        HP_TRY(hipEventRecord(e1, p->eng.stream));
+        HP_TRY(hipEventSynchronize(e1));
+        float t;
+        HP_TRY(hipEventElapsedTime(&t, e0, e1));
+        p->eng.st.ms_unbind = t;
+        HP_TRY(hipEventElapsedTime(&t, p->eng.ev[0], e1));
+        p->eng.st.ms_total = t;
+    }
+    HP_TRY(hipStreamSynchronize(p->eng.stream));
+    return HP_OK;
+}
+
+__attribute__((visibility("default"))) int hp_plan_stats(const hp_plan *p, hp_stats *out) {
+    if (!p || !out) return set_error(HP_EINVAL, "hp_plan_stats: bad arguments");
+    *out = p->eng.st;
+    return HP_OK;
+}
+
+__attribute__((visibility("default"))) int hp_plan_destroy(hp_plan *p) {
+    delete p;
+    return HP_OK;
+}
+
+__attribute__((visibility("default"))) int hp_bound_pass_host(struct potential *po,
+                                                              int64_t nrecords, hp_halo *halos,
+                                                              int64_t nhalos, uint8_t *bound,
+                                                              const hp_options *opt) {
+    hp_plan *p = nullptr;
+    int r = hp_plan_create(&p, halos, nhalos, opt);
+    if (r) return r;
+    if (p->T.nrec != nrecords) {
+        hp_plan_destroy(p);
+        return set_error(HP_EINVAL, "halos cover %lld records, not %lld", (long long)p->T.nrec,
+                         (long long)nrecords);
+    }
+    DevBuf dpo, dh, db;
+    r = dpo.ensure(sizeof(struct potential) * (nrecords + 1));
+    if (!r) r = dh.ensure(sizeof(hp_halo) * (nhalos + 1));
+    if (!r) r = db.ensure(nrecords + 1);
+    hipStream_t s = p->eng.stream;
+    if (!r && (hipMemcpyAsync(dpo.p, po, sizeof(struct potential) * nrecords,
+                              hipMemcpyHostToDevice, s) != hipSuccess ||
+               hipMemcpyAsync(dh.p, halos, sizeof(hp_halo) * nhalos, hipMemcpyHostToDevice, s) !=
+                   hipSuccess))
+        r = set_error(HP_EHIP, "copy to the device failed");
+    if (!r)
+        r = hp_plan_run(p, dpo.as<struct potential>(), dh.as<hp_halo>(),
+                        bound ? db.as<uint8_t>() : nullptr, nullptr);
+    if (!r && (hipMemcpyAsync(po, dpo.p, sizeof(struct potential) * nrecords,
+                              hipMemcpyDeviceToHost, s) != hipSuccess ||
+               hipMemcpyAsync(halos, dh.p, sizeof(hp_halo) * nhalos, hipMemcpyDeviceToHost, s) !=
+                   hipSuccess ||
+               (bound && hipMemcpyAsync(bound, db.p, nrecords, hipMemcpyDeviceToHost, s) !=
+                             hipSuccess) ||
+               hipStreamSynchronize(s) != hipSuccess))
+        r = set_error(HP_EHIP, "copy from the device failed");
+    dpo.release();
+    dh.release();
+    db.release();
+    hp_plan_destroy(p);
+    return r;
+}
+
+__attribute__((visibility("default"))) int64_t hp_debug_tree(const struct potential *po,
+                                                             int64_t n, int32_t *perm,
+                                                             hp_node_out *out, int64_t cap) {
+    if (n < 0 || (n > 0 && !po)) return set_error(HP_EINVAL, "hp_debug_tree: bad arguments");
+    int r;
+    Engine *e = thread_engine(&r);
+    if (!e) return r;
+    Tables T;
+    int64_t first = 0, count = n;
+    if ((r = make_tables(T, &first, &count, 1, HP_METHOD_TREE, false))) return r;
+    if ((r = e->reserve(T, false)) || (r = e->load_tables(T))) return r;
+    DevBuf &dpo = e->srt;
+    if ((r = dpo.ensure(sizeof(struct potential) * (n + 1)))) return r;
+    HP_TRY(hipMemcpyAsync(dpo.p, po, sizeof(struct potential) * n, hipMemcpyHostToDevice, e->stream));
+    if ((r = e->build(T, dpo.as<struct potential>(), false))) return r;
+    const int64_t nn = e->st.nodes;
+    std::vector<Node> nodes(nn);
+    std::vector<int32_t> par(nn);
+    std::vector<float4> fin(n);
+    HP_TRY(hipMemcpyAsync(nodes.data(), e->nodes.p, sizeof(Node) * nn, hipMemcpyDeviceToHost, e->stream));
+    HP_TRY(hipMemcpyAsync(par.data(), e->parent.p, sizeof(int32_t) * nn, hipMemcpyDeviceToHost, e->stream));
+    HP_TRY(hipMemcpyAsync(fin.data(), e->F.p, sizeof(float4) * n, hipMemcpyDeviceToHost, e->stream));
+    HP_TRY(hipStreamSynchronize(e->stream));
+    if (perm)
+        for (int64_t i = 0; i < n; i++) {
+            int32_t v;
+            std::memcpy(&v, &fin[i].w, 4);
+            perm[i] = v;
+        }
+    for (int64_t k = 0; k < nn && k < cap && out; k++) {
+        hp_node_out &o = out[k];
+        const Node &d = nodes[k];
+        for (int j = 0; j < 3; j++) {
+            o.lo[j] = d.lo[j];
+            o.hi[j] = d.hi[j];
+            o.com[j] = d.com[j];
+        }
+        o.mass = d.mass;
+        o.dmin2 = d.dmin2;
+        o.dmin = d.dmin;
+        o.first = d.first;
+        o.count = d.count;
+        o.left = d.left;
+        o.skip = d.skip;
+        o.parent = par[k];
+        o.pad = 0;
+    }
+    return nn;
+}
+
+}  // extern "C"

@@ -1,0 +1,214 @@
+"""Python access to libhalopot (the MI355X potential + bound pass).
+
+Mirrors the reference's operator interface for this path:
+  * ``compute_potential(po)`` -- src/potential.h:24, on an array of
+    ``struct potential`` records (``catalog.POTENTIAL_DTYPE``); pe is written in
+    place, records keep their places.
+  * ``compute_kinetic_energy(po, vel_cen, pos_cen, scale_now)`` --
+    src/potential.h:22-23.
+  * ``set_config(params)`` -- the configuration globals the path reads
+    (FORCE_RES, PARTICLE_MASS, Om, Ol, W0, WA, SCALE_NOW).
+plus the batched path (``Plan``) and ``bound_pass_host``.
+
+There is no CPU fallback: if libhalopot.so is missing or no gfx950 device is
+usable, every call raises.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import pathlib
+
+import numpy as np
+
+from . import catalog  # noqa: F401  (re-exported)
+from .catalog import HALO_DTYPE, POTENTIAL_DTYPE, Params
+
+PKG = pathlib.Path(__file__).resolve().parent
+LIB_PATH = PKG / "libhalopot.so"
+
+ACC_COMPENSATED, ACC_NAIVE, ACC_F64 = 0, 1, 2
+METHOD_TREE, METHOD_DIRECT, METHOD_AUTO = 0, 1, 2
+
+NODE_OUT_DTYPE = np.dtype(
+    {
+        "names": ["lo", "hi", "com", "mass", "dmin2", "dmin", "first", "count", "left", "skip",
+                  "parent", "pad"],
+        "formats": [("<f4", (3,)), ("<f4", (3,)), ("<f4", (3,)), "<f4", "<f4", "<f4", "<i4",
+                    "<i4", "<i4", "<i4", "<i4", "<i4"],
+        "offsets": [0, 12, 24, 36, 40, 44, 48, 52, 56, 60, 64, 68],
+        "itemsize": 72,
+    }
+)
+
+
+class HalopotError(RuntimeError):
+    pass
+
+
+class Options(C.Structure):
+    _fields_ = [("accumulate", C.c_int32), ("method", C.c_int32), ("use_graph", C.c_int32),
+                ("timing", C.c_int32), ("count_work", C.c_int32), ("reserved", C.c_int32 * 3)]
+
+
+class Stats(C.Structure):
+    _fields_ = [("ms_total", C.c_float), ("ms_build", C.c_float), ("ms_multipole", C.c_float),
+                ("ms_walk", C.c_float), ("ms_direct", C.c_float), ("ms_unbind", C.c_float),
+                ("nodes", C.c_int64), ("leaves", C.c_int64), ("levels", C.c_int64),
+                ("max_depth", C.c_int64), ("walk_launches", C.c_int64),
+                ("walk_groups", C.c_int64), ("visits", C.c_int64), ("p2p", C.c_int64),
+                ("p2m", C.c_int64), ("records", C.c_int64), ("halos", C.c_int64)]
+
+    def as_dict(self):
+        return {f: getattr(self, f) for f, _ in self._fields_}
+
+
+_LIB = None
+
+EXPORTS = ("compute_potential", "compute_kinetic_energy", "hp_plan_create", "hp_plan_run",
+           "hp_plan_potential", "hp_plan_stats", "hp_plan_destroy", "hp_bound_pass_host",
+           "hp_debug_tree", "hp_last_error", "hp_set_error_handler")
+GLOBALS = ("FORCE_RES", "PARTICLE_MASS", "Om", "Ol", "W0", "WA", "SCALE_NOW")
+
+
+def lib() -> C.CDLL:
+    """Loads libhalopot.so (built in-tree by build.py); raises if absent."""
+    global _LIB
+    if _LIB is None:
+        if not LIB_PATH.exists():
+            raise HalopotError(f"{LIB_PATH} is missing: build it (python {PKG / 'build.py'})")
+        L = C.CDLL(str(LIB_PATH))
+        L.compute_potential.argtypes = [C.c_void_p, C.c_int64]
+        L.compute_potential.restype = None
+        L.compute_kinetic_energy.argtypes = [C.c_void_p, C.c_int64, C.c_void_p, C.c_void_p,
+                                             C.c_float]
+        L.compute_kinetic_energy.restype = None
+        L.hp_plan_create.argtypes = [C.POINTER(C.c_void_p), C.c_void_p, C.c_int64,
+                                     C.POINTER(Options)]
+        L.hp_plan_run.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]
+        L.hp_plan_potential.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p]
+        L.hp_plan_stats.argtypes = [C.c_void_p, C.POINTER(Stats)]
+        L.hp_plan_destroy.argtypes = [C.c_void_p]
+        L.hp_bound_pass_host.argtypes = [C.c_void_p, C.c_int64, C.c_void_p, C.c_int64,
+                                         C.c_void_p, C.POINTER(Options)]
+        L.hp_debug_tree.argtypes = [C.c_void_p, C.c_int64, C.c_void_p, C.c_void_p, C.c_int64]
+        L.hp_debug_tree.restype = C.c_int64
+        L.hp_last_error.restype = C.c_char_p
+        _LIB = L
+    return _LIB
+
+
+def _check(code: int) -> None:
+    if code != 0:
+        raise HalopotError(f"halopot error {code}: {lib().hp_last_error().decode()}")
+
+
+def _ptr(a: np.ndarray):
+    return a.ctypes.data_as(C.c_void_p)
+
+
+def set_config(prm: Params) -> None:
+    """Sets the configuration globals that libhalopot reads at call time."""
+    L = lib()
+    for name, val in (("FORCE_RES", prm.force_res), ("PARTICLE_MASS", prm.particle_mass),
+                      ("Om", prm.Om), ("Ol", prm.Ol), ("W0", prm.W0), ("WA", prm.WA),
+                      ("SCALE_NOW", prm.scale_now)):
+        C.c_double.in_dll(L, name).value = val
+
+
+def get_config() -> dict:
+    L = lib()
+    return {n: C.c_double.in_dll(L, n).value for n in GLOBALS}
+
+
+def _records(po: np.ndarray) -> np.ndarray:
+    if po.dtype != POTENTIAL_DTYPE or not po.flags.c_contiguous:
+        raise TypeError("po must be a C-contiguous array of POTENTIAL_DTYPE records")
+    return po
+
+
+def compute_potential(po: np.ndarray) -> None:
+    """src/potential.c:183-192 on the GPU; pe written in place."""
+    _records(po)
+    lib().compute_potential(_ptr(po), po.shape[0])
+
+
+def compute_kinetic_energy(po: np.ndarray, vel_cen, pos_cen, scale_now: float) -> None:
+    """src/potential.c:194-211 on the GPU; ke written in place where ke >= 0."""
+    _records(po)
+    vc = np.ascontiguousarray(vel_cen, np.float32)
+    pc = np.ascontiguousarray(pos_cen, np.float32)
+    lib().compute_kinetic_energy(_ptr(po), po.shape[0], _ptr(vc), _ptr(pc), float(scale_now))
+
+
+def options(accumulate=ACC_COMPENSATED, method=METHOD_TREE, use_graph=0, timing=0,
+            count_work=0) -> Options:
+    o = Options()
+    o.accumulate, o.method, o.use_graph, o.timing, o.count_work = (
+        accumulate, method, use_graph, timing, count_work)
+    return o
+
+
+def bound_pass_host(po: np.ndarray, halos: np.ndarray, opts: Options | None = None,
+                    want_bound: bool = True):
+    """Batched bound pass on host arrays (copied in and out).  Returns the
+    bound flags (one byte per record) or None."""
+    _records(po)
+    if halos.dtype != HALO_DTYPE or not halos.flags.c_contiguous:
+        raise TypeError("halos must be a C-contiguous HALO_DTYPE array")
+    bound = np.zeros(po.shape[0], np.uint8) if want_bound else None
+    o = opts or options()
+    _check(lib().hp_bound_pass_host(_ptr(po), po.shape[0], _ptr(halos), halos.shape[0],
+                                    _ptr(bound) if want_bound else None, C.byref(o)))
+    return bound
+
+
+def debug_tree(po: np.ndarray):
+    """GPU build of one call's tree: (nodes, perm) where perm[i] is the record
+    at tree position i."""
+    _records(po)
+    n = po.shape[0]
+    perm = np.zeros(n, np.int32)
+    cap = 2 * n + 2
+    nodes = np.zeros(cap, NODE_OUT_DTYPE)
+    nn = lib().hp_debug_tree(_ptr(po), n, _ptr(perm), _ptr(nodes), cap)
+    if nn < 0:
+        _check(int(nn))
+    return nodes[:nn], perm
+
+
+class Plan:
+    """A planned batch (hp_plan_create) run on device-resident data.
+
+    Device memory comes from torch (plumbing only); records and halos are
+    passed as raw device pointers to the C ABI."""
+
+    def __init__(self, halos: np.ndarray, opts: Options | None = None):
+        if halos.dtype != HALO_DTYPE or not halos.flags.c_contiguous:
+            raise TypeError("halos must be a C-contiguous HALO_DTYPE array")
+        self.opts = opts or options()
+        self._h = C.c_void_p()
+        _check(lib().hp_plan_create(C.byref(self._h), _ptr(halos), halos.shape[0],
+                                    C.byref(self.opts)))
+
+    def run(self, d_po: int, d_halos: int, d_bound: int | None = None) -> None:
+        _check(lib().hp_plan_run(self._h, C.c_void_p(d_po), C.c_void_p(d_halos),
+                                 C.c_void_p(d_bound) if d_bound else None, None))
+
+    def potential(self, d_po: int) -> None:
+        _check(lib().hp_plan_potential(self._h, C.c_void_p(d_po), None))
+
+    def stats(self) -> dict:
+        s = Stats()
+        _check(lib().hp_plan_stats(self._h, C.byref(s)))
+        return s.as_dict()
+
+    def close(self) -> None:
+        if self._h:
+            lib().hp_plan_destroy(self._h)
+            self._h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

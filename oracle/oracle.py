@@ -1,0 +1,210 @@
+"""ctypes access to the CPU checkers.
+
+TEST INFRASTRUCTURE ONLY: imported by tests/, ``__graft_entry__.smoke()`` and
+bench.py's ``cpu_baseline`` leg, never by the product.
+
+* ``Restatement`` wraps liboracle.so (restate.c): the clean-room restatement of
+  src/potential.c, the POTENTIAL fast3tree build, compute_kinetic_energy,
+  calculate_corevel (src/properties.c:165-214) and the bound-pass driver
+  (src/properties.c:537-550), plus the FP64 direct sum.
+* ``Reference`` wraps _ref/libref_potential.so, the reference's own
+  src/potential.c + src/hubble.c + src/config_vars.c + src/error.cpp compiled
+  from its sources by oracle/Makefile.  It reads the configuration globals
+  FORCE_RES, PARTICLE_MASS, Om, Ol, W0, WA (src/config_vars.h) at call time.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+import pathlib
+import subprocess
+
+import numpy as np
+
+HERE = pathlib.Path(__file__).resolve().parent
+
+NODE_DTYPE = np.dtype(
+    {
+        "names": ["lo", "hi", "first", "n", "left", "dim", "depth", "com", "m", "dmin"],
+        "formats": [("<f4", (3,)), ("<f4", (3,)), "<i8", "<i8", "<i8", "<i4", "<i4",
+                    ("<f4", (3,)), "<f4", "<f4"],
+        "offsets": [0, 12, 24, 32, 40, 48, 52, 56, 68, 72],
+        "itemsize": 80,
+    }
+)
+
+
+class _Params(C.Structure):
+    _fields_ = [("force_res", C.c_double), ("particle_mass", C.c_double),
+                ("Om", C.c_double), ("Ol", C.c_double), ("W0", C.c_double),
+                ("WA", C.c_double), ("scale_now", C.c_double),
+                ("particle_rvir_dens", C.c_double)]
+
+
+class _Stats(C.Structure):
+    _fields_ = [("p2p", C.c_int64), ("p2m", C.c_int64), ("visits", C.c_int64),
+                ("nodes", C.c_int64), ("leaves", C.c_int64), ("max_depth", C.c_int64),
+                ("nfinite", C.c_int64)]
+
+
+def build(quiet: bool = True) -> None:
+    """Compile liboracle.so and, where the reference sources exist, _ref/."""
+    out = subprocess.run(["make", "-C", str(HERE)], capture_output=True, text=True)
+    if out.returncode != 0:
+        raise RuntimeError("oracle build failed:\n" + out.stdout + out.stderr)
+    if not quiet:
+        print(out.stdout)
+
+
+def _p(a):
+    return a.ctypes.data_as(C.c_void_p)
+
+
+def _check_records(po):
+    if po.dtype.itemsize != 48 or po.dtype.names != ("pos", "r2", "pe", "ke", "flags") \
+            or not po.flags.c_contiguous:
+        raise TypeError("records must be contiguous 48-byte struct potential")
+    return po
+
+
+def _params(prm) -> _Params:
+    return _Params(prm.force_res, prm.particle_mass, prm.Om, prm.Ol, prm.W0, prm.WA,
+                   prm.scale_now, prm.particle_rvir_dens)
+
+
+class Restatement:
+    def __init__(self, path: str | os.PathLike | None = None):
+        path = path or HERE / "liboracle.so"
+        if not pathlib.Path(path).exists():
+            build()
+        self.lib = L = C.CDLL(str(path))
+        L.orc_compute_potential.restype = C.c_int64
+        L.orc_compute_potential.argtypes = [C.c_void_p, C.c_int64, C.POINTER(_Params), C.c_void_p,
+                                            C.c_void_p, C.POINTER(_Stats), C.c_void_p, C.c_int64]
+        L.orc_build_tree.restype = C.c_int64
+        L.orc_build_tree.argtypes = [C.c_void_p, C.c_int64, C.POINTER(_Params), C.c_void_p,
+                                     C.c_int64]
+        L.orc_compute_potential_bh64.argtypes = [C.c_void_p, C.c_int64, C.POINTER(_Params)]
+        L.orc_direct64.argtypes = [C.c_void_p, C.c_int64, C.c_int64, C.c_double, C.c_double,
+                                   C.c_void_p]
+        L.orc_kinetic.argtypes = [C.c_void_p, C.c_int64, C.c_void_p, C.c_void_p, C.c_float,
+                                  C.POINTER(_Params)]
+        L.orc_corevel.argtypes = [C.c_void_p, C.c_void_p, C.c_int64, C.POINTER(_Params)]
+        L.orc_bound_pass.restype = C.c_int64
+        L.orc_bound_pass.argtypes = [C.c_void_p, C.c_int64, C.c_void_p, C.POINTER(_Params),
+                                     C.c_void_p]
+        L.orc_bound_pass_batch.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p,
+                                           C.c_int64, C.POINTER(_Params), C.c_void_p,
+                                           C.c_void_p, C.c_int]
+        L.orc_hubble_scaling.restype = C.c_double
+        L.orc_hubble_scaling.argtypes = [C.POINTER(_Params), C.c_double]
+        L.orc_max_threads.restype = C.c_int
+        for f in ("orc_sizeof_node", "orc_sizeof_halo", "orc_sizeof_potential"):
+            getattr(L, f).restype = C.c_int64
+        assert L.orc_sizeof_node() == NODE_DTYPE.itemsize
+        assert L.orc_sizeof_potential() == 48
+
+    def compute_potential(self, po: np.ndarray, prm, counts: bool = False, tree: bool = False):
+        """Barnes-Hut potential; po (POTENTIAL_DTYPE) is permuted in place.
+        Returns a dict with stats and, on request, per-record counters and the tree."""
+        _check_records(po)
+        n = po.shape[0]
+        pp = _params(prm)
+        st = _Stats()
+        np2p = np.zeros(n, np.int32) if counts else None
+        np2m = np.zeros(n, np.int32) if counts else None
+        cap = 2 * n + 1 if tree else 0
+        nodes = np.zeros(cap, NODE_DTYPE) if tree else None
+        nn = self.lib.orc_compute_potential(_p(po), n, C.byref(pp),
+                                            _p(np2p) if counts else None,
+                                            _p(np2m) if counts else None, C.byref(st),
+                                            _p(nodes) if tree else None, cap)
+        out = {f: getattr(st, f) for f, _ in _Stats._fields_}
+        if counts:
+            out["np2p"], out["np2m"] = np2p, np2m
+        if tree:
+            out["tree"] = nodes[:nn]
+        return out
+
+    def build_tree(self, po: np.ndarray, prm) -> np.ndarray:
+        _check_records(po)
+        n = po.shape[0]
+        nodes = np.zeros(2 * n + 1, NODE_DTYPE)
+        nn = self.lib.orc_build_tree(_p(po), n, C.byref(_params(prm)), _p(nodes), nodes.shape[0])
+        return nodes[:nn]
+
+    def compute_potential_bh64(self, po: np.ndarray, prm) -> None:
+        _check_records(po)
+        self.lib.orc_compute_potential_bh64(_p(po), po.shape[0], C.byref(_params(prm)))
+
+    def direct64(self, po: np.ndarray, prm) -> np.ndarray:
+        pos = np.ascontiguousarray(po["pos"], dtype=np.float32)
+        pe = np.zeros(po.shape[0], np.float64)
+        self.lib.orc_direct64(_p(pos), 6, po.shape[0], prm.force_res, prm.particle_mass, _p(pe))
+        return pe
+
+    def kinetic(self, po, vel_cen, pos_cen, scale_now, prm):
+        vc = np.ascontiguousarray(vel_cen, np.float32)
+        pc = np.ascontiguousarray(pos_cen, np.float32)
+        self.lib.orc_kinetic(_p(po), po.shape[0], _p(vc), _p(pc), scale_now, C.byref(_params(prm)))
+
+    def corevel(self, halo_row: np.ndarray, po: np.ndarray, prm):
+        self.lib.orc_corevel(_p(halo_row), _p(po), po.shape[0], C.byref(_params(prm)))
+
+    def bound_pass(self, po, halo_row, prm):
+        _check_records(po)
+        bound = np.zeros(po.shape[0], np.uint8)
+        kept = self.lib.orc_bound_pass(_p(po), po.shape[0], _p(halo_row), C.byref(_params(prm)),
+                                       _p(bound))
+        return kept, bound[:kept]
+
+    def bound_pass_batch(self, po, halos, prm, nthreads=0):
+        """In place over a whole catalogue: OpenMP dynamic schedule over halos."""
+        _check_records(po)
+        first = np.ascontiguousarray(halos["first"])
+        count = np.ascontiguousarray(halos["count"])
+        bound = np.zeros(po.shape[0], np.uint8)
+        kept = np.zeros(halos.shape[0], np.int64)
+        self.lib.orc_bound_pass_batch(_p(po), _p(first), _p(count), _p(halos), halos.shape[0],
+                                      C.byref(_params(prm)), _p(bound), _p(kept), nthreads)
+        return kept, bound
+
+    def hubble_scaling(self, prm, z):
+        return self.lib.orc_hubble_scaling(C.byref(_params(prm)), z)
+
+    def max_threads(self):
+        return self.lib.orc_max_threads()
+
+
+class Reference:
+    """The compiled reference (src/potential.c et al.)."""
+
+    def __init__(self, path: str | os.PathLike | None = None):
+        path = pathlib.Path(path or HERE / "_ref" / "libref_potential.so")
+        if not path.exists():
+            raise FileNotFoundError(f"{path} missing: run make -C oracle where the reference exists")
+        self.lib = L = C.CDLL(str(path))
+        L.compute_potential.argtypes = [C.c_void_p, C.c_int64]
+        L.compute_kinetic_energy.argtypes = [C.c_void_p, C.c_int64, C.c_void_p, C.c_void_p,
+                                             C.c_float]
+
+    @staticmethod
+    def available(path=None) -> bool:
+        return pathlib.Path(path or HERE / "_ref" / "libref_potential.so").exists()
+
+    def set_globals(self, prm) -> None:
+        for name, val in (("FORCE_RES", prm.force_res), ("PARTICLE_MASS", prm.particle_mass),
+                          ("Om", prm.Om), ("Ol", prm.Ol), ("W0", prm.W0), ("WA", prm.WA),
+                          ("SCALE_NOW", prm.scale_now)):
+            C.c_double.in_dll(self.lib, name).value = val
+
+    def compute_potential(self, po: np.ndarray, prm) -> None:
+        _check_records(po)
+        self.set_globals(prm)
+        self.lib.compute_potential(_p(po), po.shape[0])
+
+    def compute_kinetic_energy(self, po, vel_cen, pos_cen, scale_now, prm) -> None:
+        self.set_globals(prm)
+        vc = np.ascontiguousarray(vel_cen, np.float32)
+        pc = np.ascontiguousarray(pos_cen, np.float32)
+        self.lib.compute_kinetic_energy(_p(po), po.shape[0], _p(vc), _p(pc), scale_now)

@@ -1,0 +1,642 @@
+/*
+ * oracle/restate.c -- CPU restatement of MPI-Rockstar's per-halo Barnes-Hut
+ * potential and bound-particle pass, written from the reference's semantics.
+ *
+ * TEST INFRASTRUCTURE ONLY.  This library is the checker: only tests/, the
+ * smoke() entry and bench.py's cpu_baseline leg load it, and only to check or
+ * to time the CPU path beside the GPU one.  The product (libhalopot) never
+ * links, loads or calls anything here.
+ *
+ * What it restates (reference file:line):
+ *   - struct potential record ............ src/potential.h:7-21
+ *   - k-d tree build (POTENTIAL instance,
+ *     10 points per leaf) ................. src/fast3tree.c:736-774 (split dim,
+ *                                           midpoint, partition), 778-794 (tight
+ *                                           bbox), 798-846 (recursion, failed
+ *                                           splits), 865-908 (non-finite removal)
+ *   - multipoles ......................... src/potential.c:71-82 (dmin),
+ *                                           98-124 (mass centres)
+ *   - group walk ......................... src/potential.c:126-168
+ *   - pair terms ......................... src/potential.c:22-55
+ *   - compute_potential .................. src/potential.c:183-192
+ *   - compute_kinetic_energy ............. src/potential.c:194-211
+ *   - hubble_scaling ..................... src/hubble.c:8-19
+ *   - calculate_corevel .................. src/properties.c:165-214
+ *   - bound-pass driver .................. src/properties.c:537-550 and the
+ *                                           bound test at src/properties.c:403
+ *   - vir_density / particle_rvir_dens ... src/groupies.c:97-142
+ *
+ * Deliberate differences from the reference, none of which changes a node's
+ * membership, a walk decision or a value beyond f64 summation order:
+ *   - the partition inside a node is stable (left keeps input order, then
+ *     right), where the reference swaps in place; node membership is the set
+ *     {pos[dim] <= lim} either way;
+ *   - non-finite points are moved to the tail stably;
+ *   - nodes are numbered breadth-first, not depth-first.  The walk still
+ *     visits left before right, so every sink leaf sees the same sequence of
+ *     decisions.
+ * Pinned by: the compiled reference (oracle/_ref) on the fixtures, the FP64
+ * direct sum (N <= 10 is exact; the walk's mass bookkeeping is checked by an
+ * interaction-count invariant) and closed forms for the kinetic energy.
+ *
+ * Built with -std=c99 -O3 -fno-math-errno (src/Makefile:8,23): no FP
+ * contraction, so every float/double expression rounds as the reference's.
+ */
+#include <math.h>
+#include <stdint.h>
+#include <stdlib.h>
+#include <string.h>
+#ifdef _OPENMP
+#include <omp.h>
+#endif
+
+struct potential {
+    float   pos[6], r2;
+    double  pe;
+    float   ke;
+    int32_t flags;
+};
+
+typedef struct orc_params {
+    double force_res;          /* FORCE_RES, Mpc/h comoving */
+    double particle_mass;      /* PARTICLE_MASS, Msun/h */
+    double Om, Ol, W0, WA;     /* cosmology used by hubble_scaling */
+    double scale_now;          /* SCALE_NOW */
+    double particle_rvir_dens; /* groupies.c:135; particles per (Mpc/h)^3 */
+} orc_params;
+
+typedef struct orc_node {
+    float   lo[3], hi[3];
+    int64_t first, n;
+    int64_t left; /* right child is left + 1; -1 for a leaf */
+    int32_t dim;  /* split dimension; -1 for a leaf */
+    int32_t depth;
+    float   com[3], m, dmin;
+} orc_node;
+
+/* same layout as the product's per-call descriptor (104 bytes) */
+typedef struct orc_halo {
+    int64_t first, count;
+    float   pos[6];
+    float   corevel[3], bulkvel[3];
+    float   min_vel_err, min_bulkvel_err;
+    int64_t n_core;
+    int32_t is_sub, status;
+    int64_t kept, n_bound;
+} orc_halo;
+
+typedef struct orc_tree {
+    orc_node *nodes;
+    int64_t   nnodes, cap;
+    int64_t   nfinite;
+    int64_t   max_depth;
+} orc_tree;
+
+#define ORC_LEAF_POINTS 10 /* src/potential.c:57 */
+#define ORC_DEGENERATE  20 /* src/potential.c:145: 2 * POINTS_PER_LEAF */
+#define ORC_ERR_TOL     1.0 /* src/potential.c:10 */
+#define ORC_GC          (4.30117902e-9) /* src/universal_constants.h:4 */
+
+/* ---------------------------------------------------------------- helpers */
+
+static void *orc_grow(void *p, size_t bytes) {
+    void *q = realloc(p, bytes);
+    if (!q && bytes) abort();
+    return q;
+}
+
+/* squared distance with float differences and double squares
+   (src/potential.c:22-29) */
+static double orc_d2(const float *a, const float *b) {
+    double s = 0.0;
+    for (int k = 0; k < 3; k++) {
+        double d = (float)(a[k] - b[k]);
+        s += d * d;
+    }
+    return s;
+}
+
+static int orc_finite3(const float *p) {
+    return isfinite(p[0]) && isfinite(p[1]) && isfinite(p[2]);
+}
+
+static void orc_bbox(const struct potential *po, int64_t first, int64_t n,
+                     float lo[3], float hi[3]) {
+    for (int k = 0; k < 3; k++) lo[k] = hi[k] = po[first].pos[k];
+    for (int64_t i = first + 1; i < first + n; i++)
+        for (int k = 0; k < 3; k++) {
+            float x = po[i].pos[k];
+            if (x < lo[k]) lo[k] = x;
+            if (x > hi[k]) hi[k] = x;
+        }
+}
+
+/* ------------------------------------------------------------------ build */
+
+static int64_t orc_new_node(orc_tree *t) {
+    if (t->nnodes == t->cap) {
+        t->cap = t->cap ? 2 * t->cap : 64;
+        t->nodes = orc_grow(t->nodes, sizeof(orc_node) * (size_t)t->cap);
+    }
+    memset(t->nodes + t->nnodes, 0, sizeof(orc_node));
+    t->nodes[t->nnodes].left = -1;
+    t->nodes[t->nnodes].dim = -1;
+    return t->nnodes++;
+}
+
+/* Split dimension: z first, x or y only if strictly wider
+   (src/fast3tree.c:736-747). */
+static int orc_split_dim(const float lo[3], const float hi[3]) {
+    int dim = 2;
+    float w = hi[2] - lo[2];
+    for (int k = 0; k < 2; k++) {
+        float e = hi[k] - lo[k];
+        if (e > w) {
+            w = e;
+            dim = k;
+        }
+    }
+    return dim;
+}
+
+/* Builds the tree over po[0..n); po is permuted so that every node owns a
+   contiguous range.  Returns the number of finite points. */
+static int64_t orc_build(orc_tree *t, struct potential *po, int64_t n,
+                         struct potential *scratch) {
+    /* stable move of non-finite points to the tail (src/fast3tree.c:875-887
+       drops them from the tree) */
+    int64_t nf = 0, nb = 0;
+    for (int64_t i = 0; i < n; i++) {
+        if (orc_finite3(po[i].pos)) po[nf++] = po[i];
+        else scratch[nb++] = po[i];
+    }
+    for (int64_t i = 0; i < nb; i++) po[nf + i] = scratch[i];
+
+    t->nnodes = 0;
+    t->nfinite = nf;
+    t->max_depth = 0;
+    int64_t r = orc_new_node(t);
+    orc_node *root = t->nodes + r;
+    root->first = 0;
+    root->n = nf;
+    if (nf) orc_bbox(po, 0, nf, root->lo, root->hi);
+
+    /* breadth-first: nodes are appended in creation order, so a cursor over
+       the node array visits every node exactly once */
+    for (int64_t c = 0; c < t->nnodes; c++) {
+        orc_node *nd = t->nodes + c;
+        if (nd->depth > t->max_depth) t->max_depth = nd->depth;
+        if (nd->n <= ORC_LEAF_POINTS) continue;
+        int dim = orc_split_dim(nd->lo, nd->hi);
+        if (nd->hi[dim] == nd->lo[dim]) continue; /* src/fast3tree.c:760-761 */
+        /* float sum, halved exactly (src/fast3tree.c:757) */
+        float lim = 0.5f * (nd->hi[dim] + nd->lo[dim]);
+        int64_t nl = 0, nr = 0;
+        for (int64_t i = nd->first; i < nd->first + nd->n; i++) {
+            if (po[i].pos[dim] <= lim) po[nd->first + nl++] = po[i];
+            else scratch[nr++] = po[i];
+        }
+        for (int64_t i = 0; i < nr; i++) po[nd->first + nl + i] = scratch[i];
+        if (nl == 0 || nl == nd->n) continue; /* src/fast3tree.c:805-809 */
+        int64_t a = orc_new_node(t);
+        int64_t b = orc_new_node(t);
+        nd = t->nodes + c; /* array may have moved */
+        nd->dim = dim;
+        nd->left = a;
+        orc_node *L = t->nodes + a, *R = t->nodes + b;
+        L->first = nd->first;
+        L->n = nl;
+        R->first = nd->first + nl;
+        R->n = nd->n - nl;
+        L->depth = R->depth = nd->depth + 1;
+        orc_bbox(po, L->first, L->n, L->lo, L->hi);
+        orc_bbox(po, R->first, R->n, R->lo, R->hi);
+    }
+    return nf;
+}
+
+/* ------------------------------------------------------------- multipoles */
+
+/* dmin over every point below the node (src/potential.c:71-82) */
+static void orc_dmin(orc_node *nd, const struct potential *po) {
+    double sum = 0.0, big = 0.0;
+    for (int64_t i = nd->first; i < nd->first + nd->n; i++) {
+        double d2 = orc_d2(nd->com, po[i].pos);
+        if (d2 > big) big = d2;
+        sum += d2;
+    }
+    double b = sqrt(big) / 2.0;
+    nd->dmin = (float)(b + sqrt(b * b + sum / ((double)nd->n * ORC_ERR_TOL)));
+}
+
+/* Leaves: f64 position sums; internal nodes: float combination of the two
+   children, right term first (src/potential.c:98-124).  Children always have
+   larger indices than their parent, so a reverse sweep is a post-order. */
+static void orc_multipoles(orc_tree *t, const struct potential *po,
+                           double particle_mass) {
+    for (int64_t c = t->nnodes - 1; c >= 0; c--) {
+        orc_node *nd = t->nodes + c;
+        if (nd->n == 0) { /* empty root: nothing to compute */
+            nd->m = 0;
+            nd->com[0] = nd->com[1] = nd->com[2] = 0;
+            nd->dmin = 0;
+            continue;
+        }
+        if (nd->left < 0) {
+            double s[3] = {0, 0, 0};
+            nd->m = (float)(particle_mass * (double)nd->n);
+            for (int64_t i = nd->first; i < nd->first + nd->n; i++)
+                for (int k = 0; k < 3; k++) s[k] += po[i].pos[k];
+            for (int k = 0; k < 3; k++)
+                nd->com[k] = s[k] ? (float)(s[k] / (double)nd->n) : 0.0f;
+        } else {
+            const orc_node *L = t->nodes + nd->left, *R = L + 1;
+            nd->m = L->m + R->m;
+            for (int k = 0; k < 3; k++) {
+                float rr = R->com[k] * R->m;
+                float ll = L->com[k] * L->m;
+                nd->com[k] = nd->m ? (rr + ll) / nd->m : 0.0f;
+            }
+        }
+        orc_dmin(nd, po);
+    }
+}
+
+/* ------------------------------------------------------------------- walk */
+
+typedef struct orc_ctx {
+    const orc_tree *t;
+    struct potential *po;
+    double force_res, particle_mass;
+    int32_t *np2p, *np2m; /* optional per-particle interaction counters */
+    int64_t p2p, p2m, visits;
+} orc_ctx;
+
+static double orc_pair(const orc_ctx *x, const float *a, const float *b) {
+    double r = sqrt(orc_d2(a, b));
+    if (r < x->force_res) r = x->force_res; /* src/potential.c:31-36 */
+    return x->particle_mass * (1.0 / r);
+}
+
+static void orc_self(orc_ctx *x, const orc_node *s) {
+    struct potential *p = x->po + s->first;
+    for (int64_t i = 0; i < s->n; i++)
+        for (int64_t j = i + 1; j < s->n; j++) {
+            double v = orc_pair(x, p[i].pos, p[j].pos);
+            p[i].pe += v;
+            p[j].pe += v;
+        }
+    x->p2p += s->n * (s->n - 1);
+    if (x->np2p)
+        for (int64_t i = 0; i < s->n; i++)
+            x->np2p[s->first + i] += (int32_t)(s->n - 1);
+}
+
+static void orc_leaf_leaf(orc_ctx *x, const orc_node *s, const orc_node *b) {
+    struct potential *p = x->po + s->first;
+    const struct potential *q = x->po + b->first;
+    for (int64_t i = 0; i < s->n; i++)
+        for (int64_t j = 0; j < b->n; j++) p[i].pe += orc_pair(x, p[i].pos, q[j].pos);
+    x->p2p += s->n * b->n;
+    if (x->np2p)
+        for (int64_t i = 0; i < s->n; i++) x->np2p[s->first + i] += (int32_t)b->n;
+}
+
+static void orc_monopole(orc_ctx *x, const orc_node *s, const orc_node *b) {
+    struct potential *p = x->po + s->first;
+    for (int64_t i = 0; i < s->n; i++) {
+        float r = (float)sqrt(orc_d2(p[i].pos, b->com));
+        p[i].pe += b->m / r; /* float quotient, unsoftened (potential.c:163-166) */
+    }
+    x->p2m += s->n;
+    if (x->np2m)
+        for (int64_t i = 0; i < s->n; i++) x->np2m[s->first + i] += 1;
+}
+
+/* src/potential.c:126-131: float distance of the two centres against the
+   float square of the source's dmin */
+static int orc_accept(const orc_node *s, const orc_node *b) {
+    float r2 = (float)orc_d2(s->com, b->com);
+    return r2 > b->dmin * b->dmin;
+}
+
+static int orc_inside(const orc_node *s, const orc_node *b) {
+    for (int k = 0; k < 3; k++)
+        if (s->lo[k] < b->lo[k] || s->hi[k] > b->hi[k]) return 0;
+    return 1;
+}
+
+static void orc_visit(orc_ctx *x, const orc_node *s, const orc_node *b) {
+    x->visits++;
+    if (s == b) {
+        if (s->n <= ORC_DEGENERATE) orc_self(x, s);
+        return;
+    }
+    if (orc_inside(s, b) || !orc_accept(s, b)) {
+        if (b->left < 0) orc_leaf_leaf(x, s, b);
+        else {
+            orc_visit(x, s, x->t->nodes + b->left);
+            orc_visit(x, s, x->t->nodes + b->left + 1);
+        }
+    } else
+        orc_monopole(x, s, b);
+}
+
+/* --------------------------------------------------------- exported: tree */
+
+typedef struct orc_walk_stats {
+    int64_t p2p, p2m, visits, nodes, leaves, max_depth, nfinite;
+} orc_walk_stats;
+
+/* compute_potential (src/potential.c:183-192).  po is permuted in place into
+   tree order.  np2p / np2m, when given, receive per-record interaction counts
+   in the permuted order.  nodes_out, when given with room for cap_nodes,
+   receives the tree.  Returns the node count. */
+int64_t orc_compute_potential(struct potential *po, int64_t n,
+                              const orc_params *prm, int32_t *np2p,
+                              int32_t *np2m, orc_walk_stats *st,
+                              orc_node *nodes_out, int64_t cap_nodes) {
+    for (int64_t i = 0; i < n; i++) po[i].pe = 0;
+    if (np2p) memset(np2p, 0, sizeof(int32_t) * (size_t)(n > 0 ? n : 0));
+    if (np2m) memset(np2m, 0, sizeof(int32_t) * (size_t)(n > 0 ? n : 0));
+    if (n <= 0) return 0;
+    orc_tree t = {0};
+    struct potential *scratch = orc_grow(NULL, sizeof(struct potential) * (size_t)n);
+    orc_build(&t, po, n, scratch);
+    free(scratch);
+    orc_multipoles(&t, po, prm->particle_mass);
+    orc_ctx x = {&t, po, prm->force_res, prm->particle_mass, np2p, np2m, 0, 0, 0};
+    int64_t leaves = 0;
+    if (t.nfinite > 0)
+        for (int64_t c = 0; c < t.nnodes; c++)
+            if (t.nodes[c].left < 0) {
+                leaves++;
+                orc_visit(&x, t.nodes + c, t.nodes);
+            }
+    if (st) {
+        st->p2p = x.p2p;
+        st->p2m = x.p2m;
+        st->visits = x.visits;
+        st->nodes = t.nnodes;
+        st->leaves = leaves;
+        st->max_depth = t.max_depth;
+        st->nfinite = t.nfinite;
+    }
+    if (nodes_out)
+        memcpy(nodes_out, t.nodes,
+               sizeof(orc_node) * (size_t)(t.nnodes < cap_nodes ? t.nnodes : cap_nodes));
+    int64_t nn = t.nnodes;
+    free(t.nodes);
+    return nn;
+}
+
+/* Tree only (build + multipoles), for structural comparison with the GPU
+   build.  po is permuted into tree order. */
+int64_t orc_build_tree(struct potential *po, int64_t n, const orc_params *prm,
+                       orc_node *nodes_out, int64_t cap_nodes) {
+    if (n <= 0) return 0;
+    orc_tree t = {0};
+    struct potential *scratch = orc_grow(NULL, sizeof(struct potential) * (size_t)n);
+    orc_build(&t, po, n, scratch);
+    free(scratch);
+    orc_multipoles(&t, po, prm->particle_mass);
+    if (nodes_out)
+        memcpy(nodes_out, t.nodes,
+               sizeof(orc_node) * (size_t)(t.nnodes < cap_nodes ? t.nnodes : cap_nodes));
+    int64_t nn = t.nnodes;
+    free(t.nodes);
+    return nn;
+}
+
+/* The same tree and the same walk decisions, every term evaluated in f64:
+   P2P as the reference, P2M as m / r in double.  This is the "FP64
+   evaluation of the same Barnes-Hut tree" used to separate truncation error
+   from arithmetic error.  po is permuted as by orc_compute_potential. */
+static void orc_visit64(orc_ctx *x, const orc_node *s, const orc_node *b) {
+    if (s == b) {
+        if (s->n <= ORC_DEGENERATE) orc_self(x, s);
+        return;
+    }
+    if (orc_inside(s, b) || !orc_accept(s, b)) {
+        if (b->left < 0) orc_leaf_leaf(x, s, b);
+        else {
+            orc_visit64(x, s, x->t->nodes + b->left);
+            orc_visit64(x, s, x->t->nodes + b->left + 1);
+        }
+    } else {
+        struct potential *p = x->po + s->first;
+        for (int64_t i = 0; i < s->n; i++) {
+            double dd = 0.0;
+            for (int k = 0; k < 3; k++) {
+                double d = (double)p[i].pos[k] - (double)b->com[k];
+                dd += d * d;
+            }
+            p[i].pe += (double)b->m / sqrt(dd);
+        }
+    }
+}
+
+void orc_compute_potential_bh64(struct potential *po, int64_t n,
+                                const orc_params *prm) {
+    for (int64_t i = 0; i < n; i++) po[i].pe = 0;
+    if (n <= 0) return;
+    orc_tree t = {0};
+    struct potential *scratch = orc_grow(NULL, sizeof(struct potential) * (size_t)n);
+    orc_build(&t, po, n, scratch);
+    free(scratch);
+    orc_multipoles(&t, po, prm->particle_mass);
+    orc_ctx x = {&t, po, prm->force_res, prm->particle_mass, NULL, NULL, 0, 0, 0};
+    if (t.nfinite > 0)
+        for (int64_t c = 0; c < t.nnodes; c++)
+            if (t.nodes[c].left < 0) orc_visit64(&x, t.nodes + c, t.nodes);
+    free(t.nodes);
+}
+
+/* -------------------------------------------------- exported: FP64 direct */
+
+/* pe_i = sum_{j != i, both finite} m / max(|x_i - x_j|, FORCE_RES), every
+   operation in f64 (the differences of float inputs are exact in f64).
+   Positions are read from stride-spaced float triples. */
+void orc_direct64(const float *pos, int64_t stride, int64_t n,
+                  double force_res, double particle_mass, double *pe) {
+#pragma omp parallel for schedule(dynamic, 64)
+    for (int64_t i = 0; i < n; i++) {
+        const float *a = pos + i * stride;
+        double acc = 0.0;
+        if (orc_finite3(a)) {
+            double ax = a[0], ay = a[1], az = a[2];
+            for (int64_t j = 0; j < n; j++) {
+                if (j == i) continue;
+                const float *b = pos + j * stride;
+                if (!orc_finite3(b)) continue;
+                double dx = ax - b[0], dy = ay - b[1], dz = az - b[2];
+                double r = sqrt(dx * dx + dy * dy + dz * dz);
+                if (r < force_res) r = force_res;
+                acc += 1.0 / r;
+            }
+        }
+        pe[i] = acc * particle_mass;
+    }
+}
+
+/* ------------------------------------------------------- kinetic energy */
+
+/* src/hubble.c:8-19 */
+static double orc_weff(const orc_params *p, double a) {
+    if (a != 1.0) return p->W0 + p->WA - p->WA * (a - 1.0) / log(a);
+    return p->W0;
+}
+
+double orc_hubble_scaling(const orc_params *p, double z) {
+    double z1 = 1.0 + z;
+    double a = 1.0 / z1;
+    return sqrt(p->Om * (z1 * z1 * z1) + p->Ol * pow(a, -3.0 * (1.0 + orc_weff(p, a))));
+}
+
+/* src/potential.c:194-211: records with ke < 0 are potential-only and are
+   left alone; the squared velocity is accumulated into the float ke. */
+void orc_kinetic(struct potential *po, int64_t n, const float *vel_cen,
+                 const float *pos_cen, float scale_now, const orc_params *p) {
+    double conv = 0.5 * scale_now / ORC_GC;
+    double z1 = 1.0 / scale_now;
+    double hub = 100.0 * orc_hubble_scaling(p, z1 - 1.0);
+    for (int64_t i = 0; i < n; i++) {
+        if (po[i].ke < 0) continue;
+        po[i].ke = 0;
+        for (int k = 0; k < 3; k++) {
+            float dv_pec = po[i].pos[k + 3] - vel_cen[k];
+            float dx = po[i].pos[k] - pos_cen[k];
+            double dv = (double)dv_pec + (hub * scale_now) * (double)dx;
+            po[i].ke = (float)((double)po[i].ke + dv * dv);
+        }
+        po[i].ke = (float)((double)po[i].ke * conv);
+    }
+}
+
+/* ---------------------------------------------------------- core velocity */
+
+/* src/properties.c:165-214.  po is sorted by r2.  The running mean and
+   variance are Welford's; the selection keeps the prefix whose variance per
+   (i - 3) * i is lowest, the core restricted to the inner 10% in radius. */
+void orc_corevel(orc_halo *h, const struct potential *po, int64_t n,
+                 const orc_params *p) {
+    const double four_thirds_pi = 4.0 * M_PI / 3.0;
+    double thr = p->particle_rvir_dens * four_thirds_pi;
+    int64_t j;
+    for (j = n - 1; j >= 0; j--) {
+        double r6 = po[j].r2 * po[j].r2 * po[j].r2; /* float product, promoted */
+        if ((double)(j * j) > r6 * (thr * thr)) break;
+    }
+    int64_t rvir_max = j;
+    if (rvir_max < 1) return;
+    for (j = n - 1; j >= 0; j--)
+        if (po[j].r2 * 100.0 < po[rvir_max].r2) break;
+    int64_t core_max = j < 100 ? 100 : j;
+
+    double mean[3] = {0, 0, 0}, m2[3] = {0, 0, 0}, best = 0;
+    for (int64_t i = 0; i < rvir_max; i++) {
+        for (int k = 0; k < 3; k++) {
+            double v = po[i].pos[k + 3];
+            double d = v - mean[k];
+            mean[k] += d / ((double)(i + 1));
+            m2[k] += d * (v - mean[k]);
+        }
+        double var = m2[0] + m2[1] + m2[2];
+        if (i < 10 || var < best * (i - 3) * i) {
+            best = (i > 3) ? var / (double)((i - 3) * i) : 0.0;
+            if (i < core_max) {
+                h->n_core = i;
+                h->min_vel_err = (float)best;
+                for (int k = 0; k < 3; k++) h->corevel[k] = (float)mean[k];
+            }
+            for (int k = 0; k < 3; k++) h->bulkvel[k] = (float)mean[k];
+            h->min_bulkvel_err = (float)best;
+        }
+    }
+    for (int k = 0; k < 3; k++) h->pos[k + 3] = h->corevel[k];
+}
+
+/* ------------------------------------------------------- bound-pass driver */
+
+static void orc_merge_sort_r2(struct potential *a, struct potential *tmp,
+                              int64_t n) {
+    /* bottom-up stable merge sort by r2 (the reference's qsort is glibc's
+       merge sort in practice, stable for equal keys) */
+    for (int64_t w = 1; w < n; w *= 2) {
+        for (int64_t lo = 0; lo < n; lo += 2 * w) {
+            int64_t mid = lo + w < n ? lo + w : n;
+            int64_t hi = lo + 2 * w < n ? lo + 2 * w : n;
+            int64_t i = lo, j = mid, k = lo;
+            while (i < mid && j < hi) tmp[k++] = (a[j].r2 < a[i].r2) ? a[j++] : a[i++];
+            while (i < mid) tmp[k++] = a[i++];
+            while (j < hi) tmp[k++] = a[j++];
+        }
+        memcpy(a, tmp, sizeof(struct potential) * (size_t)n);
+    }
+}
+
+/* src/properties.c:537-550: potential, drop potential-only records, sort by
+   radius, core velocity, kinetic energy about the core (sub-halos) or bulk
+   (hosts) velocity.  Returns the kept record count; bound[i] = !(pe < ke)
+   (src/properties.c:403) for the kept, sorted records. */
+int64_t orc_bound_pass(struct potential *po, int64_t n, orc_halo *h,
+                       const orc_params *p, uint8_t *bound) {
+    if (n > 1) orc_compute_potential(po, n, p, NULL, NULL, NULL, NULL, 0);
+    int64_t kept = 0;
+    for (int64_t i = 0; i < n; i++)
+        if (!(po[i].ke < 0)) po[kept++] = po[i];
+    if (kept > 1) {
+        struct potential *tmp = orc_grow(NULL, sizeof(struct potential) * (size_t)kept);
+        orc_merge_sort_r2(po, tmp, kept);
+        free(tmp);
+    }
+    orc_corevel(h, po, kept, p);
+    orc_kinetic(po, kept, h->is_sub ? h->corevel : h->bulkvel, h->pos,
+                (float)p->scale_now, p);
+    int64_t nb = 0;
+    for (int64_t i = 0; i < kept; i++) {
+        int b = !(po[i].pe < po[i].ke);
+        nb += b;
+        if (bound) bound[i] = (uint8_t)b;
+    }
+    h->kept = kept;
+    h->n_bound = nb;
+    return kept;
+}
+
+/* The reference's concurrency context (src/mpi_main.cpp:1377): dynamic
+   schedule over halos, one halo per thread.  first[h] / count[h] index the
+   concatenated record array; kept[h] receives each halo's kept count. */
+void orc_bound_pass_batch(struct potential *po, const int64_t *first,
+                          const int64_t *count, orc_halo *halos,
+                          int64_t nhalos, const orc_params *p, uint8_t *bound,
+                          int64_t *kept, int nthreads) {
+#ifdef _OPENMP
+    if (nthreads > 0) omp_set_num_threads(nthreads);
+#else
+    (void)nthreads;
+#endif
+#pragma omp parallel for schedule(dynamic)
+    for (int64_t h = 0; h < nhalos; h++)
+        kept[h] = orc_bound_pass(po + first[h], count[h], halos + h, p,
+                                 bound ? bound + first[h] : NULL);
+}
+
+/* src/groupies.c:97-100 */
+double orc_vir_density(const orc_params *p, double a) {
+    double e = orc_hubble_scaling(p, 1.0 / a - 1.0);
+    double x = (p->Om / pow(a, 3)) / pow(e, 2.0) - 1.0;
+    return ((18 * M_PI * M_PI + 82.0 * x - 39 * x * x) / (1.0 + x));
+}
+
+int orc_max_threads(void) {
+#ifdef _OPENMP
+    return omp_get_max_threads();
+#else
+    return 1;
+#endif
+}
+
+int64_t orc_sizeof_node(void) { return (int64_t)sizeof(orc_node); }
+int64_t orc_sizeof_halo(void) { return (int64_t)sizeof(orc_halo); }
+int64_t orc_sizeof_potential(void) { return (int64_t)sizeof(struct potential); }

@@ -1,0 +1,164 @@
+"""GPU parity of compute_potential (libhalopot through its C ABI) against the
+oracle: the restatement of src/potential.c, the compiled reference where it
+was built, and the FP64 evaluations.
+
+Tolerances:
+  * tree: identical (node boxes, counts, centres bit for bit; record order in
+    tree order identical; dmin within one float ulp, its f64 sum being
+    reduced in another order);
+  * interaction counts (P2P pairs, P2M terms): identical to the restatement's,
+    i.e. every acceptance decision is the reference's;
+  * pe vs the restatement / reference Barnes-Hut: max relative 1e-5
+    (FP32 pair terms, compensated sums);
+  * pe vs the FP64 evaluation of the same tree: median 1e-6, max 1e-5.
+"""
+import numpy as np
+import pytest
+
+import halopot
+from halopot import catalog
+from conftest import by_id, with_ids
+from gpu_util import rel, run_plan
+
+pytestmark = pytest.mark.gpu
+
+
+def _compare_trees(gnodes, perm, onodes, opo):
+    """Walks both trees from the root in parallel."""
+    stack = [(0, 0)]
+    seen = 0
+    while stack:
+        g, o = stack.pop()
+        G, O = gnodes[g], onodes[o]
+        seen += 1
+        assert G["count"] == O["n"], (g, o)
+        assert G["first"] == O["first"], (g, o)
+        np.testing.assert_array_equal(G["lo"], O["lo"])
+        np.testing.assert_array_equal(G["hi"], O["hi"])
+        np.testing.assert_array_equal(G["com"], O["com"])
+        assert G["mass"] == O["m"]
+        d = np.float32(O["dmin"])
+        assert abs(int(np.float32(G["dmin"]).view(np.int32)) - int(d.view(np.int32))) <= 1
+        assert (G["left"] < 0) == (O["left"] < 0), (g, o)
+        if G["left"] >= 0:
+            stack.append((G["left"], O["left"]))
+            stack.append((G["left"] + 1, O["left"] + 1))
+    assert seen == len(onodes) == len(gnodes)
+    np.testing.assert_array_equal(perm, opo["flags"])
+
+
+@pytest.mark.parametrize("which", ["c1", "small"])
+def test_tree_identical(restatement, which):
+    cat = catalog.config1() if which == "c1" else catalog.small_batch(n_halos=1, lo=2000, hi=2001)
+    prm = cat.params
+    halopot.set_config(prm)
+    po = with_ids(cat.segment(0))
+    gnodes, perm = halopot.debug_tree(po)
+    opo = po.copy()
+    onodes = restatement.build_tree(opo, prm)
+    _compare_trees(gnodes, perm, onodes, opo)
+
+
+def test_c1_potential_parity(restatement, request):
+    cat = catalog.config1()
+    prm = cat.params
+    halopot.set_config(prm)
+    po = with_ids(cat.po)
+    g = po.copy()
+    halopot.compute_potential(g)
+    np.testing.assert_array_equal(g["flags"], po["flags"])  # records keep their places
+    o = po.copy()
+    restatement.compute_potential(o, prm)
+    po_ref = by_id(o)
+    r = rel(g["pe"], po_ref)
+    assert np.max(r) < 1e-5, np.max(r)
+    b64 = po.copy()
+    restatement.compute_potential_bh64(b64, prm)
+    r64 = rel(g["pe"], by_id(b64))
+    assert np.median(r64) < 1e-6 and np.max(r64) < 1e-5, (np.median(r64), np.max(r64))
+    try:
+        ref = request.getfixturevalue("reference")
+    except pytest.skip.Exception:
+        return
+    a = po.copy()
+    ref.compute_potential(a, prm)
+    rr = rel(g["pe"], by_id(a))
+    assert np.max(rr) < 1e-5, np.max(rr)
+
+
+@pytest.mark.parametrize("acc", [halopot.ACC_COMPENSATED, halopot.ACC_NAIVE, halopot.ACC_F64])
+def test_batch_counts_and_accumulation(restatement, acc):
+    cat = catalog.small_batch(seed=7, n_halos=60, lo=2, hi=5000)
+    prm = cat.params
+    po_out, _, _, st = run_plan(cat, halopot.options(accumulate=acc, count_work=1),
+                                potential_only=True)
+    want_p2p = want_p2m = 0
+    for h in range(cat.n_halos):
+        seg = with_ids(cat.segment(h))
+        o = seg.copy()
+        s = restatement.compute_potential(o, prm)
+        want_p2p += s["p2p"]
+        want_p2m += s["p2m"]
+        f = int(cat.halos["first"][h])
+        got = po_out["pe"][f:f + seg.shape[0]]
+        if acc == halopot.ACC_F64:
+            # every term in FP64 on the same tree: the FP64 evaluation of the
+            # reference's decisions (its own P2M quotient is a float)
+            b = seg.copy()
+            restatement.compute_potential_bh64(b, prm)
+            r = rel(got, by_id(b))
+        else:
+            r = rel(got, by_id(o))
+        tol = {halopot.ACC_COMPENSATED: 1e-5, halopot.ACC_NAIVE: 1e-4, halopot.ACC_F64: 1e-12}[acc]
+        assert np.max(r, initial=0) < tol, (h, np.max(r))
+    assert st["p2p"] == want_p2p
+    assert st["p2m"] == want_p2m
+
+
+def test_edge_cases(restatement):
+    prm = catalog.Params(particle_mass=1e9)
+    halopot.set_config(prm)
+    rng = np.random.default_rng(5)
+    cases = []
+    # two particles: pe = m / max(d, FORCE_RES) each
+    two = np.zeros(2, catalog.POTENTIAL_DTYPE)
+    two["pos"][1, 0] = 0.5
+    cases.append(two)
+    # closer than the softening
+    near = two.copy()
+    near["pos"][1, 0] = 0.001
+    cases.append(near)
+    # 15 coincident points (<= 20: every pair at FORCE_RES) and 30 (> 20:
+    # a degenerate leaf, no self term)
+    for n in (15, 30):
+        c = np.zeros(n, catalog.POTENTIAL_DTYPE)
+        c["pos"][:, :3] = 3.25
+        cases.append(c)
+    # non-finite points are left out and get pe = 0
+    nf = np.zeros(40, catalog.POTENTIAL_DTYPE)
+    nf["pos"][:, :3] = rng.uniform(0, 1, (40, 3))
+    nf["pos"][3, 1] = np.nan
+    nf["pos"][17, 0] = np.inf
+    cases.append(nf)
+    # single particle, all-NaN
+    one = np.zeros(1, catalog.POTENTIAL_DTYPE)
+    cases.append(one)
+    for c in cases:
+        g = with_ids(c)
+        halopot.compute_potential(g)
+        o = with_ids(c)
+        restatement.compute_potential(o, prm)
+        np.testing.assert_allclose(g["pe"], by_id(o), rtol=1e-6, atol=0)
+    assert cases[0]["pe"].sum() == 0  # inputs untouched
+    g = with_ids(cases[0])
+    halopot.compute_potential(g)
+    np.testing.assert_allclose(g["pe"], [1e9 / 0.5] * 2, rtol=1e-6)
+    g = with_ids(cases[2])
+    halopot.compute_potential(g)
+    np.testing.assert_allclose(g["pe"], 14 * 1e9 / 0.003, rtol=1e-6)
+    g = with_ids(cases[3])
+    halopot.compute_potential(g)
+    assert np.all(g["pe"] == 0)
+    g = with_ids(cases[4])
+    halopot.compute_potential(g)
+    assert g["pe"][3] == 0 and g["pe"][17] == 0 and np.all(g["pe"][[0, 1, 2, 4]] > 0)
