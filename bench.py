@@ -1,0 +1,248 @@
+#!/usr/bin/env python3
+"""Throughput of the batched potential + bound pass on one MI355X.
+
+A step is one hp_plan_run over the whole synthetic catalogue: the
+Barnes-Hut potential of every call (src/potential.c:183-192) and the bound
+pass after it (src/properties.c:539-550: drop potential-only records, sort by
+r2, core / bulk velocity, kinetic energy, bound flag).  Records and halo
+descriptors are resident in HBM before the timed region; each step first
+restores the pristine input with a device-to-device copy (the pass reorders
+records in place), which is included in the timing.
+
+Default workload: the batching configuration, 2,000 NFW halos with
+n ~ n^-1.9 on [20, 200000] plus 5% contaminants (seeded), one batched plan.
+Beside it, the same catalogue through the plain per-halo GPU path (every
+call with its own launches) and the reference's CPU path (its own
+compute_potential / compute_kinetic_energy from oracle/_ref, OpenMP over
+halos) on a bounded sample of host time.
+
+Prints ONE JSON line.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import pathlib
+import sys
+import time
+
+ROOT = pathlib.Path(__file__).resolve().parent
+sys.path.insert(0, str(ROOT / "mpi-rockstar_amd"))
+
+METRIC = "halo-particle potentials/s, batched vs per-halo GPU, ≤1e-4 rel err vs FP64"
+UNIT = "halo-particle potentials/s"
+FP32_PEAK_TFLOPS = 157.3   # MI355X FP32 (vector, dual-issue / packed), published
+HBM_PEAK_GBS = 8000.0
+
+WORKLOADS = {
+    "c2": "batching: 2000 NFW halos, n ~ n^-1.9 on [20, 200000], c ~ lognormal(8, 0.3), "
+          "+5% unbound contaminants, seed 0x5EED0002",
+    "c5": "one-GPU catalogue: 16,777,216 particles / 250 Mpc/h, 40% in 50,000 NFW halos "
+          "(n^-1.9 on [20, 500000] scaled to 6,710,886), seed 0x5EED0005",
+    "c3": "deep tree: 2,000,000-particle NFW host (c=6, r_vir=2) + 30 sub-halos, 31 calls",
+    "c1": "one NFW halo of 10,000 + 1,000 background particles",
+}
+
+
+def log(msg: str) -> None:
+    print(f"[bench] {msg}", file=sys.stderr, flush=True)
+
+
+def cpu_threads() -> int:
+    env = os.environ.get("OMP_NUM_THREADS")
+    if env and env.isdigit():
+        return int(env)
+    try:
+        return min(16, len(os.sched_getaffinity(0)))
+    except AttributeError:
+        return min(16, os.cpu_count() or 1)
+
+
+def run_cpu_baseline(cat, seconds: float):
+    """The reference's CPU path on the host cores: oracle/_ref's compiled
+    compute_potential + compute_kinetic_energy inside the OpenMP-over-halos
+    bound pass, repeated over the whole catalogue until `seconds` elapse."""
+    sys.path.insert(0, str(ROOT / "oracle"))
+    import oracle
+
+    O = oracle.Restatement()
+    nt = cpu_threads()
+    if oracle.Reference.available():
+        R = oracle.Reference()
+        kind = "reference"
+
+        def one():
+            po = cat.po.copy()
+            h = cat.halos.copy()
+            O.bound_pass_batch_with(po, h, cat.params, R, nthreads=nt)
+    else:
+        kind = "port"
+
+        def one():
+            po = cat.po.copy()
+            h = cat.halos.copy()
+            O.bound_pass_batch(po, h, cat.params, nthreads=nt)
+    passes = 0
+    busy = 0.0
+    while busy < seconds or passes == 0:
+        t0 = time.perf_counter()
+        one()
+        busy += time.perf_counter() - t0
+        passes += 1
+    return {
+        "value": passes * cat.n_records / busy,
+        "unit": UNIT,
+        "cores": nt,
+        "kind": kind,
+        "sample": f"whole catalogue x {passes} pass(es), {busy:.1f} s wall on {nt} threads "
+                  f"(OpenMP dynamic over halos, as src/mpi_main.cpp:1377)",
+    }
+
+
+def main() -> int:
+    ap = argparse.ArgumentParser(description=__doc__.split("\n\n")[0])
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--workload", default="c2", choices=sorted(WORKLOADS))
+    ap.add_argument("--accumulate", default="compensated", choices=["compensated", "naive", "f64"])
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
+    ap.add_argument("--no-per-halo", action="store_true", help="skip the per-halo GPU path")
+    ap.add_argument("--cpu-seconds", type=float, default=8.0)
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py measures one GPU: this build has no multi-GPU path (asked for "
+              f"--gpus {args.gpus}); halos would shard across GPUs by call, see DESIGN.md",
+              file=sys.stderr)
+        return 2
+
+    import numpy as np
+
+    import halopot
+    from halopot import catalog
+    from halopot.hipmem import DeviceArray, synchronize
+
+    t = time.perf_counter()
+    cat = catalog.make(args.workload)
+    log(f"workload {args.workload}: {cat.n_records} records in {cat.n_halos} calls "
+        f"({time.perf_counter() - t:.1f} s to generate)")
+    halopot.set_config(cat.params)
+    acc = {"compensated": halopot.ACC_COMPENSATED, "naive": halopot.ACC_NAIVE,
+           "f64": halopot.ACC_F64}[args.accumulate]
+
+    d_src = DeviceArray.from_numpy(cat.po)
+    d_hsrc = DeviceArray.from_numpy(cat.halos)
+    d_po = DeviceArray(cat.po.nbytes)
+    d_h = DeviceArray(cat.halos.nbytes)
+    d_b = DeviceArray.zeros(cat.n_records)
+    hip = halopot.hipmem.hip()
+
+    def restore():
+        hip.hipMemcpy(d_po.ptr, d_src.ptr, cat.po.nbytes, halopot.hipmem.D2D)
+        hip.hipMemcpy(d_h.ptr, d_hsrc.ptr, cat.halos.nbytes, halopot.hipmem.D2D)
+
+    # work counters of the reference's walk decisions (untimed)
+    plan_c = halopot.Plan(cat.halos, halopot.options(accumulate=acc, count_work=1))
+    restore()
+    plan_c.potential(d_po.address)
+    wst = plan_c.stats()
+    plan_c.close()
+    flops = 11 * wst["p2p"] + 10 * wst["p2m"]
+    log(f"walk work: {wst['p2p']} P2P pairs, {wst['p2m']} P2M terms, {wst['visits']} "
+        f"wavefront node visits, {wst['nodes']} nodes, depth {wst['max_depth']}")
+
+    plan = halopot.Plan(cat.halos, halopot.options(accumulate=acc, timing=1))
+    for i in range(args.warmup):
+        restore()
+        plan.run(d_po.address, d_h.address, d_b.address)
+    synchronize()
+    log(f"warmup done ({args.warmup} steps)")
+    walk_ms, phase = [], {"build": 0.0, "multipole": 0.0, "walk": 0.0, "unbind": 0.0}
+    synchronize()
+    t0 = time.perf_counter()
+    for i in range(args.steps):
+        restore()
+        plan.run(d_po.address, d_h.address, d_b.address)
+        st = plan.stats()
+        walk_ms.append(st["ms_walk"])
+        for k in phase:
+            phase[k] += st["ms_" + k]
+    synchronize()
+    elapsed = time.perf_counter() - t0
+    plan.close()
+    ms_step = 1e3 * elapsed / args.steps
+    value = cat.n_records * args.steps / elapsed
+    log(f"batched: {ms_step:.3f} ms/step, {value:.4g} records/s; phases per step (ms): " +
+        ", ".join(f"{k} {v / args.steps:.3f}" for k, v in phase.items()))
+
+    walk_avg = sum(walk_ms) / len(walk_ms)
+    achieved = flops / (walk_avg * 1e-3) / 1e12
+    roofline = {
+        "bound": "mfma",
+        "achieved": achieved,
+        "peak": FP32_PEAK_TFLOPS,
+        "unit": "TFLOP/s",
+        "frac": achieved / FP32_PEAK_TFLOPS,
+        "traffic": None,
+        "kernel": "k_walk (Barnes-Hut group walk)",
+        "bound_detail": "FP32 VALU + v_rsq issue (no dense contraction: MFMA unused); peak is "
+                        "the published FP32 vector figure",
+        "flops_per_launch": flops,
+        "flop_convention": "11 per P2P pair, 10 per P2M term, counts of the reference's walk",
+        "walk_ms_avg": walk_avg,
+    }
+
+    out = {
+        "metric": METRIC,
+        "value": value,
+        "unit": UNIT,
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": ms_step,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic",
+        "config": {"workload": args.workload + ": " + WORKLOADS[args.workload],
+                   "records": cat.n_records, "calls": cat.n_halos,
+                   "accumulate": args.accumulate, "parallelism": "one GPU"},
+        "roofline": roofline,
+    }
+
+    if not args.no_per_halo:
+        plan_p = halopot.Plan(cat.halos, halopot.options(accumulate=acc, per_call=1))
+        restore()
+        plan_p.run(d_po.address, d_h.address, d_b.address)  # warm
+        synchronize()
+        reps = 0
+        tp = time.perf_counter()
+        while reps == 0 or (time.perf_counter() - tp < 2.0 and reps < args.steps):
+            restore()
+            plan_p.run(d_po.address, d_h.address, d_b.address)
+            reps += 1
+        synchronize()
+        ep = time.perf_counter() - tp
+        plan_p.close()
+        vp = cat.n_records * reps / ep
+        out["per_halo_gpu"] = {"value": vp, "unit": UNIT, "ms_per_pass": 1e3 * ep / reps,
+                               "passes": reps}
+        out["batched_vs_per_halo"] = value / vp
+        log(f"per-halo GPU path: {1e3 * ep / reps:.1f} ms/pass, {vp:.4g} records/s; "
+            f"batched/per-halo = {value / vp:.1f}x")
+
+    if not args.no_cpu:
+        cb = run_cpu_baseline(cat, args.cpu_seconds)
+        out["cpu_baseline"] = cb
+        log(f"cpu baseline ({cb['kind']}, {cb['cores']} threads): {cb['value']:.4g} records/s")
+
+    for d in (d_src, d_hsrc, d_po, d_h, d_b):
+        d.free()
+    print(json.dumps(out), flush=True)
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

@@ -108,7 +108,9 @@ typedef struct hp_options {
     int32_t use_graph;      /* replay the launch sequence from a hipGraph */
     int32_t timing;         /* record HIP events around each phase */
     int32_t count_work;     /* count node visits / interactions (slower) */
-    int32_t reserved[3];
+    int32_t per_call;       /* 1: run the calls one after another, each with
+                               its own launches (the plain per-halo path) */
+    int32_t reserved[2];
 } hp_options;
 
 typedef struct hp_plan hp_plan;

@@ -130,14 +130,30 @@ struct BuildArgs {
     Task *small_next;
     Task *big;              // append-only
     int2 *tiles;            // append-only (task index, offset)
+    int32_t *fat;           // leaves of more than kFatLeaf points
     int32_t *counters;
     int32_t big_threshold;
-    int32_t cap_small, cap_big, cap_tiles, cap_nodes;
+    int32_t cap_small, cap_big, cap_tiles, cap_nodes, cap_fat;
 };
 
-// Queues node `id` (count > 10 points, tight box known) for splitting.
-__device__ void push_task(const BuildArgs &a, int32_t id, const float *lo, const float *hi,
-                          int32_t count, int32_t depth) {
+__device__ void note_fat_leaf(const BuildArgs &a, int32_t id, int32_t count) {
+    if (count <= kFatLeaf) return;
+    int32_t k = atomicAdd(&a.counters[kFatCount], 1);
+    if (k < a.cap_fat) a.fat[k] = id;
+    else atomicOr(&a.counters[kErrFlags], kErrCapacity);
+}
+
+// Node ids and small-task slots are allocated together: counters[kNodeCount]
+// and counters[kSmallNext] are adjacent 32-bit halves of one 64-bit counter,
+// so a split costs one atomic round trip.
+__device__ __forceinline__ uint64_t alloc_nodes_and_small(const BuildArgs &a, int32_t nodes,
+                                                          int32_t small) {
+    static_assert(kNodeCount == 0 && kSmallNext == 1, "packed counter layout");
+    return atomicAdd(reinterpret_cast<unsigned long long *>(a.counters),
+                     (unsigned long long)nodes | ((unsigned long long)small << 32));
+}
+
+__device__ Task make_task(int32_t id, const float *lo, const float *hi, int32_t depth) {
     Task t;
     t.node = id;
     t.dim = split_dim(lo, hi);
@@ -146,26 +162,35 @@ __device__ void push_task(const BuildArgs &a, int32_t id, const float *lo, const
     t.tile0 = 0;
     t.ntiles = 0;
     t.pad[0] = t.pad[1] = 0;
-    if (count > a.big_threshold) {
-        int32_t nt = (count + kTile - 1) / kTile;
-        int32_t bi = atomicAdd(&a.counters[kBigNext], 1);
-        int32_t t0 = atomicAdd(&a.counters[kTileNext], nt);
-        if (bi >= a.cap_big || t0 + nt > a.cap_tiles) {
-            atomicOr(&a.counters[kErrFlags], kErrCapacity);
-            return;
-        }
-        t.tile0 = t0;
-        t.ntiles = nt;
-        a.big[bi] = t;
-        for (int32_t j = 0; j < nt; j++) a.tiles[t0 + j] = make_int2(bi, j * kTile);
-    } else {
-        int32_t si = atomicAdd(&a.counters[kSmallNext], 1);
-        if (si >= a.cap_small) {
-            atomicOr(&a.counters[kErrFlags], kErrCapacity);
-            return;
-        }
-        a.small_next[si] = t;
+    return t;
+}
+
+// Queues node `id` (more than 10 points, tight box known) at a small-task
+// slot already allocated.
+__device__ void push_small_at(const BuildArgs &a, int32_t slot, int32_t id, const float *lo,
+                              const float *hi, int32_t depth) {
+    if (slot >= a.cap_small) {
+        atomicOr(&a.counters[kErrFlags], kErrCapacity);
+        return;
     }
+    a.small_next[slot] = make_task(id, lo, hi, depth);
+}
+
+// Queues a node of more than big_threshold points with its tiles.
+__device__ void push_big(const BuildArgs &a, int32_t id, const float *lo, const float *hi,
+                         int32_t count, int32_t depth) {
+    Task t = make_task(id, lo, hi, depth);
+    const int32_t nt = (count + kTile - 1) / kTile;
+    const int32_t bi = atomicAdd(&a.counters[kBigNext], 1);
+    const int32_t t0 = atomicAdd(&a.counters[kTileNext], nt);
+    if (bi >= a.cap_big || t0 + nt > a.cap_tiles) {
+        atomicOr(&a.counters[kErrFlags], kErrCapacity);
+        return;
+    }
+    t.tile0 = t0;
+    t.ntiles = nt;
+    a.big[bi] = t;
+    for (int32_t j = 0; j < nt; j++) a.tiles[t0 + j] = make_int2(bi, j * kTile);
 }
 
 __device__ void write_node(Node *nd, const float *lo, const float *hi, int32_t first,
@@ -252,94 +277,184 @@ __global__ __launch_bounds__(256) void k_roots(const int64_t *__restrict__ hfirs
     if (threadIdx.x == 0) {
         write_node(a.nodes + h, lo, hi, first, nfin, -1);
         a.parent[h] = -1;
-        if (nfin > kLeafPoints) push_task(a, h, lo, hi, nfin, 0);
-        else if (nfin > 0) atomicAdd(&a.counters[kLeafCount], 1);
+        if (nfin > a.big_threshold) {
+            push_big(a, h, lo, hi, nfin, 0);
+        } else if (nfin > kLeafPoints) {
+            const uint64_t old = alloc_nodes_and_small(a, 0, 1);
+            push_small_at(a, (int32_t)(old >> 32), h, lo, hi, 0);
+        }
     }
 }
 
 // ------------------------------------------------------------ small tasks
 
-// Creates the two children of a split node and queues or finishes them.
+// Writes the two children of a split node and queues those that need a
+// split: small ones at slots P, P + 1 (allocated with the node ids), big ones
+// with their tiles.
 __device__ void make_children(const BuildArgs &a, const Task &t, const Node &nd, int32_t L,
-                              int32_t nl, const float *loL, const float *hiL, const float *loR,
-                              const float *hiR) {
+                              int32_t P, int32_t nl, const float *loL, const float *hiL,
+                              const float *loR, const float *hiR) {
     const int32_t nr = nd.count - nl;
     write_node(a.nodes + L, loL, hiL, nd.first, nl, L + 1);
     write_node(a.nodes + L + 1, loR, hiR, nd.first + nl, nr, nd.skip);
     a.parent[L] = a.parent[L + 1] = t.node;
     a.nodes[t.node].left = L;
-    atomicMax(&a.counters[kMaxDepth], t.depth + 1);
-    if (nl > kLeafPoints) push_task(a, L, loL, hiL, nl, t.depth + 1);
-    else atomicAdd(&a.counters[kLeafCount], 1);
-    if (nr > kLeafPoints) push_task(a, L + 1, loR, hiR, nr, t.depth + 1);
-    else atomicAdd(&a.counters[kLeafCount], 1);
+    if (nl > a.big_threshold) push_big(a, L, loL, hiL, nl, t.depth + 1);
+    else if (nl > kLeafPoints) push_small_at(a, P++, L, loL, hiL, t.depth + 1);
+    if (nr > a.big_threshold) push_big(a, L + 1, loR, hiR, nr, t.depth + 1);
+    else if (nr > kLeafPoints) push_small_at(a, P, L + 1, loR, hiR, t.depth + 1);
 }
 
-// One wavefront per task.
-__global__ __launch_bounds__(256) void k_split_small(const float4 *__restrict__ src,
-                                                     float4 *__restrict__ dst,
-                                                     const Task *__restrict__ tasks,
-                                                     int32_t ntasks, BuildArgs a) {
-    const int32_t wid = (int32_t)((blockIdx.x * blockDim.x + threadIdx.x) / kWave);
-    if (wid >= ntasks) return;
+__device__ __forceinline__ int32_t small_pushes(int32_t nl, int32_t nr, int32_t thr) {
+    return (nl > kLeafPoints && nl <= thr ? 1 : 0) + (nr > kLeafPoints && nr <= thr ? 1 : 0);
+}
+
+__device__ __forceinline__ void split_small_task(const float4 *__restrict__ src,
+                                                 float4 *__restrict__ dst,
+                                                 const Task *__restrict__ tasks, int32_t wid,
+                                                 int32_t ntasks, const BuildArgs &a);
+
+// One wavefront per task of at most kSmallMax points, held in registers:
+// one read per point, a count pass and a stable scatter pass over ballots.
+constexpr int kSplitThreads = 1024;  // 16 wavefronts share one allocation
+
+__global__ __launch_bounds__(kSplitThreads) void k_split_small(const float4 *__restrict__ src,
+                                                               float4 *__restrict__ dst,
+                                                               const Task *__restrict__ tasks,
+                                                               BuildArgs a) {
+    const int32_t ntasks = a.counters[kCurSmall];
+    constexpr int kWaves = kSplitThreads / kWave;
+    // the workgroup walks the task list kWaves tasks at a time; every
+    // wavefront takes part in every round (the allocation is per round)
+    for (int32_t base = blockIdx.x * kWaves; base < ntasks; base += gridDim.x * kWaves)
+        split_small_task(src, dst, tasks, base + (int32_t)(threadIdx.x / kWave), ntasks, a);
+}
+
+// One round of k_split_small: wavefront w of the workgroup splits task wid
+// (if wid < ntasks).  Node ids and small-task slots are allocated once per
+// workgroup: the wavefronts' demands are summed in LDS and thread 0 makes a
+// single atomic request.
+__device__ __forceinline__ void split_small_task(const float4 *__restrict__ src,
+                                                 float4 *__restrict__ dst,
+                                                 const Task *__restrict__ tasks, int32_t wid,
+                                                 int32_t ntasks, const BuildArgs &a) {
+    constexpr int kWaves = kSplitThreads / kWave;
+    __shared__ uint32_t s_nodes[kWaves], s_push[kWaves];
+    __shared__ unsigned long long s_base;
     const int lane = lane_id();
-    const Task t = tasks[wid];
-    const Node nd = a.nodes[t.node];
+    const int w = (int)(threadIdx.x / kWave);
+    const bool has = wid < ntasks;
+    Task t;
+    Node nd;
+    if (has) {
+        t = tasks[wid];
+        nd = a.nodes[t.node];
+    } else {
+        t.node = -1;
+        t.dim = 2;
+        t.lim = 0.0f;
+        t.depth = 0;
+        nd.first = 0;
+        nd.count = 0;
+        nd.skip = -1;
+    }
     const int32_t first = nd.first, count = nd.count;
     const int dim = t.dim;
     const float lim = t.lim;
+    const int nch = (count + kWave - 1) / kWave;  // <= kSmallChunks
 
-    int32_t nl = 0;
-    for (int32_t base = 0; base < count; base += kWave) {
-        int32_t i = base + lane;
-        bool f = false;
-        if (i < count) f = comp(src[first + i], dim) <= lim;
-        nl += __popcll(ballot(f));
+    float4 pts[kSmallChunks];
+#pragma unroll
+    for (int c = 0; c < kSmallChunks; c++) {
+        const int32_t i = c * kWave + lane;
+        pts[c] = (c < nch && i < count) ? src[first + i] : make_float4(0.0f, 0.0f, 0.0f, 0.0f);
     }
-    if (nl == 0 || nl == count) {  // src/fast3tree.c:805-809: a leaf
-        for (int32_t i = lane; i < count; i += kWave) {
-            a.fin[first + i] = src[first + i];
-            a.leaf_of[first + i] = t.node;
-        }
-        if (lane == 0) atomicAdd(&a.counters[kLeafCount], 1);
-        return;
-    }
-    int32_t L = 0;
-    if (lane == 0) L = atomicAdd(&a.counters[kNodeCount], 2);
-    L = __shfl(L, 0, 64);
-    if (L + 2 > a.cap_nodes) {
-        if (lane == 0) atomicOr(&a.counters[kErrFlags], kErrCapacity);
-        return;
-    }
-    const int32_t nr = count - nl;
-    const bool leafL = nl <= kLeafPoints, leafR = nr <= kLeafPoints;
+    // pass 1: left count and both children's boxes
     float lo[2][3], hi[2][3];
     for (int s = 0; s < 2; s++)
         for (int k = 0; k < 3; k++) {
             lo[s][k] = __int_as_float(0x7f800000);
             hi[s][k] = __int_as_float((int)0xff800000);
         }
+    int32_t nl = 0;
+#pragma unroll
+    for (int c = 0; c < kSmallChunks; c++) {
+        if (c < nch) {
+            const int32_t i = c * kWave + lane;
+            const float4 p = pts[c];
+            const bool v = i < count;
+            const bool f = v && comp(p, dim) <= lim;
+            nl += __popcll(ballot(f));
+            if (v) {
+                const int s = f ? 0 : 1;
+                lo[s][0] = fminf(lo[s][0], p.x); hi[s][0] = fmaxf(hi[s][0], p.x);
+                lo[s][1] = fminf(lo[s][1], p.y); hi[s][1] = fmaxf(hi[s][1], p.y);
+                lo[s][2] = fminf(lo[s][2], p.z); hi[s][2] = fmaxf(hi[s][2], p.z);
+            }
+        }
+    }
+    const bool failed = has && (nl == 0 || nl == count);  // src/fast3tree.c:805-809
+    const bool splits = has && !failed;
+    const int32_t nr = count - nl;
+    // workgroup allocation: exclusive offsets of every wavefront's demand
+    if (lane == 0) {
+        s_nodes[w] = splits ? 2u : 0u;
+        s_push[w] = splits ? (uint32_t)small_pushes(nl, nr, a.big_threshold) : 0u;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        uint32_t tn = 0, tp = 0;
+        for (int q = 0; q < kWaves; q++) {
+            tn += s_nodes[q];
+            tp += s_push[q];
+        }
+        s_base = (tn | tp) ? alloc_nodes_and_small(a, (int32_t)tn, (int32_t)tp) : 0ull;
+    }
+    __syncthreads();
+    uint32_t on = (uint32_t)s_base, op = (uint32_t)(s_base >> 32);
+    for (int q = 0; q < w; q++) {
+        on += s_nodes[q];
+        op += s_push[q];
+    }
+    __syncthreads();  // s_* are reused by the next round
+    if (failed) {  // a leaf
+#pragma unroll
+        for (int c = 0; c < kSmallChunks; c++) {
+            const int32_t i = c * kWave + lane;
+            if (c < nch && i < count) {
+                a.fin[first + i] = pts[c];
+                a.leaf_of[first + i] = t.node;
+            }
+        }
+        if (lane == 0) note_fat_leaf(a, t.node, count);
+        return;
+    }
+    if (!splits) return;
+    const int32_t L = (int32_t)on, P = (int32_t)op;
+    if (L + 2 > a.cap_nodes) {
+        if (lane == 0) atomicOr(&a.counters[kErrFlags], kErrCapacity);
+        return;
+    }
+    const bool leafL = nl <= kLeafPoints, leafR = nr <= kLeafPoints;
+    // pass 2: stable scatter
     int32_t bl = 0, br = 0;
-    for (int32_t base = 0; base < count; base += kWave) {
-        int32_t i = base + lane;
-        bool v = i < count;
-        float4 p = v ? src[first + i] : make_float4(0, 0, 0, 0);
-        bool f = v && comp(p, dim) <= lim;
-        bool g = v && !f;
-        uint64_t mL = ballot(f), mR = ballot(g);
+#pragma unroll
+    for (int c = 0; c < kSmallChunks; c++) {
+        if (c >= nch) continue;
+        const int32_t i = c * kWave + lane;
+        const bool v = i < count;
+        const float4 p = pts[c];
+        const bool f = v && comp(p, dim) <= lim;
+        const bool g = v && !f;
+        const uint64_t mL = ballot(f), mR = ballot(g);
         if (v) {
-            int s = f ? 0 : 1;
-            int32_t d = f ? first + bl + rank_below(mL) : first + nl + br + rank_below(mR);
-            bool leaf = f ? leafL : leafR;
-            if (leaf) {
+            const int32_t d = f ? first + bl + rank_below(mL) : first + nl + br + rank_below(mR);
+            if (f ? leafL : leafR) {
                 a.fin[d] = p;
-                a.leaf_of[d] = L + s;
+                a.leaf_of[d] = L + (f ? 0 : 1);
             } else {
                 dst[d] = p;
             }
-            lo[s][0] = fminf(lo[s][0], p.x); hi[s][0] = fmaxf(hi[s][0], p.x);
-            lo[s][1] = fminf(lo[s][1], p.y); hi[s][1] = fmaxf(hi[s][1], p.y);
-            lo[s][2] = fminf(lo[s][2], p.z); hi[s][2] = fmaxf(hi[s][2], p.z);
         }
         bl += __popcll(mL);
         br += __popcll(mR);
@@ -349,7 +464,7 @@ __global__ __launch_bounds__(256) void k_split_small(const float4 *__restrict__ 
             lo[s][k] = wave_min(lo[s][k]);
             hi[s][k] = wave_max(hi[s][k]);
         }
-    if (lane == 0) make_children(a, t, nd, L, nl, lo[0], hi[0], lo[1], hi[1]);
+    if (lane == 0) make_children(a, t, nd, L, P, nl, lo[0], hi[0], lo[1], hi[1]);
 }
 
 // -------------------------------------------------------------- big tasks
@@ -386,16 +501,28 @@ __device__ void block_box_reduce(float (*lo)[3], float (*hi)[3], int &nl, TilePa
         for (int j = 0; j < kTileThreads / kWave; j++) tp.nleft += s_n[j];
         *out = tp;
     }
+    __syncthreads();
 }
+
+__device__ void big_count_tile(const float4 *__restrict__ src, const int2 *__restrict__ tiles,
+                               const Task *__restrict__ big, const Node *__restrict__ nodes,
+                               TilePart *__restrict__ parts, int32_t ti);
 
 // Count pass over one tile of a big task.
 __global__ __launch_bounds__(kTileThreads) void k_big_count(const float4 *__restrict__ src,
                                                             const int2 *__restrict__ tiles,
-                                                            int32_t tile_base,
                                                             const Task *__restrict__ big,
                                                             const Node *__restrict__ nodes,
-                                                            TilePart *__restrict__ parts) {
-    const int32_t ti = tile_base + blockIdx.x;
+                                                            TilePart *__restrict__ parts,
+                                                            const int32_t *__restrict__ counters) {
+    const int32_t t1 = counters[kTileEnd];
+    for (int32_t ti = counters[kTileBegin] + blockIdx.x; ti < t1; ti += gridDim.x)
+        big_count_tile(src, tiles, big, nodes, parts, ti);
+}
+
+__device__ void big_count_tile(const float4 *__restrict__ src, const int2 *__restrict__ tiles,
+                               const Task *__restrict__ big, const Node *__restrict__ nodes,
+                               TilePart *__restrict__ parts, int32_t ti) {
     const int2 tl = tiles[ti];
     const Task t = big[tl.x];
     const Node nd = nodes[t.node];
@@ -420,12 +547,19 @@ __global__ __launch_bounds__(kTileThreads) void k_big_count(const float4 *__rest
 }
 
 // One thread per big task of the level: scan of the tile counts, children.
-__global__ void k_big_split(int32_t task0, int32_t ntasks, const Task *__restrict__ big,
-                            TilePart *__restrict__ parts, BigSplit *__restrict__ res,
-                            BuildArgs a) {
-    const int32_t k = blockIdx.x * blockDim.x + threadIdx.x;
-    if (k >= ntasks) return;
-    const int32_t bi = task0 + k;
+__device__ void big_split_task(const Task *__restrict__ big, TilePart *__restrict__ parts,
+                               BigSplit *__restrict__ res, const BuildArgs &a, int32_t bi);
+
+__global__ void k_big_split(const Task *__restrict__ big, TilePart *__restrict__ parts,
+                            BigSplit *__restrict__ res, BuildArgs a) {
+    const int32_t b1 = a.counters[kBigEnd];
+    for (int32_t bi = a.counters[kBigBegin] + blockIdx.x * blockDim.x + threadIdx.x; bi < b1;
+         bi += gridDim.x * blockDim.x)
+        big_split_task(big, parts, res, a, bi);
+}
+
+__device__ void big_split_task(const Task *__restrict__ big, TilePart *__restrict__ parts,
+                               BigSplit *__restrict__ res, const BuildArgs &a, int32_t bi) {
     const Task t = big[bi];
     const Node nd = a.nodes[t.node];
     int32_t nl = 0;
@@ -451,28 +585,43 @@ __global__ void k_big_split(int32_t task0, int32_t ntasks, const Task *__restric
     r.left = -1;
     r.leafL = r.leafR = 0;
     if (nl == 0 || nl == nd.count) {
-        atomicAdd(&a.counters[kLeafCount], 1);
+        note_fat_leaf(a, t.node, nd.count);
     } else {
-        int32_t L = atomicAdd(&a.counters[kNodeCount], 2);
+        const int32_t nr = nd.count - nl;
+        const uint64_t al = alloc_nodes_and_small(a, 2, small_pushes(nl, nr, a.big_threshold));
+        const int32_t L = (int32_t)al, P = (int32_t)(al >> 32);
         if (L + 2 > a.cap_nodes) {
             atomicOr(&a.counters[kErrFlags], kErrCapacity);
         } else {
             r.left = L;
             r.leafL = nl <= kLeafPoints;
-            r.leafR = (nd.count - nl) <= kLeafPoints;
-            make_children(a, t, nd, L, nl, lo[0], hi[0], lo[1], hi[1]);
+            r.leafR = nr <= kLeafPoints;
+            make_children(a, t, nd, L, P, nl, lo[0], hi[0], lo[1], hi[1]);
         }
     }
     res[bi] = r;
 }
 
 // Scatter pass over one tile of a big task: stable ranks in rounds of 256.
+__device__ void big_scatter_tile(const float4 *__restrict__ src, float4 *__restrict__ dst,
+                                 const int2 *__restrict__ tiles, const Task *__restrict__ big,
+                                 const TilePart *__restrict__ parts,
+                                 const BigSplit *__restrict__ res, const BuildArgs &a, int32_t ti);
+
 __global__ __launch_bounds__(kTileThreads) void k_big_scatter(
     const float4 *__restrict__ src, float4 *__restrict__ dst, const int2 *__restrict__ tiles,
-    int32_t tile_base, const Task *__restrict__ big, const TilePart *__restrict__ parts,
+    const Task *__restrict__ big, const TilePart *__restrict__ parts,
     const BigSplit *__restrict__ res, BuildArgs a) {
+    const int32_t t1 = a.counters[kTileEnd];
+    for (int32_t ti = a.counters[kTileBegin] + blockIdx.x; ti < t1; ti += gridDim.x)
+        big_scatter_tile(src, dst, tiles, big, parts, res, a, ti);
+}
+
+__device__ void big_scatter_tile(const float4 *__restrict__ src, float4 *__restrict__ dst,
+                                 const int2 *__restrict__ tiles, const Task *__restrict__ big,
+                                 const TilePart *__restrict__ parts,
+                                 const BigSplit *__restrict__ res, const BuildArgs &a, int32_t ti) {
     __shared__ int s_cnt[2][kTileThreads / kWave];
-    const int32_t ti = tile_base + blockIdx.x;
     const int2 tl = tiles[ti];
     const Task t = big[tl.x];
     const Node nd = a.nodes[t.node];
@@ -537,63 +686,97 @@ __global__ __launch_bounds__(kTileThreads) void k_big_scatter(
     }
 }
 
+// Moves the level state forward on the device: the pushes of the level just
+// run become the next level's work; depth_end[level] records the node count
+// (node ids grow with depth).
+__global__ void k_level_advance(int32_t *__restrict__ c, int32_t *__restrict__ depth_end) {
+    if (threadIdx.x != 0 || blockIdx.x != 0) return;
+    const int32_t ns = c[kSmallNext];
+    c[kCurSmall] = ns;
+    c[kSmallNext] = 0;
+    c[kBigBegin] = c[kBigEnd];
+    c[kBigEnd] = c[kBigNext];
+    c[kTileBegin] = c[kTileEnd];
+    c[kTileEnd] = c[kTileNext];
+    const int32_t lvl = c[kLevel];
+    if (lvl < kMaxLevels) depth_end[lvl] = c[kNodeCount];
+    else c[kErrFlags] |= kErrCapacity;
+    c[kLevel] = lvl + 1;
+    c[kActive] = (ns > 0 || c[kBigEnd] > c[kBigBegin]) ? 1 : 0;
+}
+
 // --------------------------------------------------------------- multipoles
 
-// One wavefront per node: leaves compute mass and centre (f64 sums,
-// src/potential.c:100-107) and climb; the second child to arrive at a parent
-// computes the parent's float centre (src/potential.c:115-122).
-__global__ __launch_bounds__(256) void k_com(Node *__restrict__ nodes,
-                                             const int32_t *__restrict__ parent,
-                                             int32_t *__restrict__ arrive,
-                                             const float4 *__restrict__ fin, int32_t nnodes,
-                                             double particle_mass) {
-    const int32_t id = (int32_t)((blockIdx.x * blockDim.x + threadIdx.x) / kWave);
+// Leaf mass and centre (src/potential.c:100-107): f64 sums of the float
+// positions, one thread per leaf of at most kFatLeaf points.
+__global__ void k_leaf_com(Node *__restrict__ nodes, const float4 *__restrict__ fin,
+                           int32_t nnodes, double particle_mass) {
+    const int32_t id = blockIdx.x * blockDim.x + threadIdx.x;
     if (id >= nnodes) return;
-    const int lane = lane_id();
     const Node nd = nodes[id];
-    if (nd.left >= 0 || nd.count <= 0) return;
+    if (nd.left >= 0 || nd.count <= 0 || nd.count > kFatLeaf) return;
+    double s0 = 0.0, s1 = 0.0, s2 = 0.0;
+    for (int32_t i = 0; i < nd.count; i++) {
+        const float4 p = fin[nd.first + i];
+        s0 += (double)p.x;
+        s1 += (double)p.y;
+        s2 += (double)p.z;
+    }
+    const double n = (double)nd.count;
+    Node *me = nodes + id;
+    me->com[0] = s0 != 0.0 ? (float)(s0 / n) : 0.0f;
+    me->com[1] = s1 != 0.0 ? (float)(s1 / n) : 0.0f;
+    me->com[2] = s2 != 0.0 ? (float)(s2 / n) : 0.0f;
+    me->mass = (float)(particle_mass * n);
+}
+
+// The same for leaves of more than kFatLeaf points (failed splits), one
+// wavefront each.  Sums of floats in f64 are exact here, so the order of the
+// wavefront reduction does not matter.
+__global__ __launch_bounds__(256) void k_fat_leaf_com(Node *__restrict__ nodes,
+                                                      const int32_t *__restrict__ list,
+                                                      int32_t nlist,
+                                                      const float4 *__restrict__ fin,
+                                                      double particle_mass) {
+    const int32_t k = (int32_t)((blockIdx.x * blockDim.x + threadIdx.x) / kWave);
+    if (k >= nlist) return;
+    const int lane = lane_id();
+    const int32_t id = list[k];
+    const Node nd = nodes[id];
     double s[3] = {0.0, 0.0, 0.0};
     for (int32_t i = lane; i < nd.count; i += kWave) {
-        float4 p = fin[nd.first + i];
+        const float4 p = fin[nd.first + i];
         s[0] += (double)p.x;
         s[1] += (double)p.y;
         s[2] += (double)p.z;
     }
-    for (int k = 0; k < 3; k++) s[k] = wave_sum(s[k]);
-    if (lane != 0) return;
-    float com[3];
-    for (int k = 0; k < 3; k++) com[k] = s[k] != 0.0 ? (float)(s[k] / (double)nd.count) : 0.0f;
-    float m = (float)(particle_mass * (double)nd.count);
-    Node *me = nodes + id;
-    me->com[0] = com[0];
-    me->com[1] = com[1];
-    me->com[2] = com[2];
-    me->mass = m;
-    int32_t c = parent[id];
-    int guard = 0;
-    while (c >= 0 && guard++ < (1 << 24)) {
-        int old = __hip_atomic_fetch_add(arrive + c, 1, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
-        if (old == 0) break;
-        Node *pn = nodes + c;
-        const int32_t L = pn->left;
-        const Node *l = nodes + L, *r = nodes + L + 1;
-        float ml = __hip_atomic_load(&l->mass, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        float mr = __hip_atomic_load(&r->mass, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        float mm = ml + mr;
-        float cc[3];
-        for (int k = 0; k < 3; k++) {
-            float cl = __hip_atomic_load(&l->com[k], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-            float cr = __hip_atomic_load(&r->com[k], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-            float rr = cr * mr;
-            float ll = cl * ml;
-            cc[k] = mm != 0.0f ? (rr + ll) / mm : 0.0f;
-        }
-        __hip_atomic_store(&pn->com[0], cc[0], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        __hip_atomic_store(&pn->com[1], cc[1], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        __hip_atomic_store(&pn->com[2], cc[2], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        __hip_atomic_store(&pn->mass, mm, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        c = parent[c];
+    for (int k2 = 0; k2 < 3; k2++) s[k2] = wave_sum(s[k2]);
+    if (lane == 0) {
+        const double n = (double)nd.count;
+        Node *me = nodes + id;
+        for (int k2 = 0; k2 < 3; k2++) me->com[k2] = s[k2] != 0.0 ? (float)(s[k2] / n) : 0.0f;
+        me->mass = (float)(particle_mass * n);
     }
+}
+
+// Internal nodes of one depth, ids [begin, end): float mass and centre from
+// the two children, right term first (src/potential.c:115-122).  Node ids
+// grow with depth, so the host runs the depths deepest first.
+__global__ void k_internal_com(Node *__restrict__ nodes, int32_t begin, int32_t end) {
+    const int32_t id = begin + blockIdx.x * blockDim.x + threadIdx.x;
+    if (id >= end) return;
+    Node *me = nodes + id;
+    const int32_t L = me->left;
+    if (L < 0) return;
+    const Node *l = nodes + L, *r = nodes + L + 1;
+    const float ml = l->mass, mr = r->mass;
+    const float m = ml + mr;
+    for (int k = 0; k < 3; k++) {
+        const float rr = r->com[k] * mr;
+        const float ll = l->com[k] * ml;
+        me->com[k] = m != 0.0f ? (rr + ll) / m : 0.0f;
+    }
+    me->mass = m;
 }
 
 __device__ __forceinline__ double d2_ref(const float *c, const float4 &p) {

@@ -18,8 +18,12 @@ constexpr int kLeafPoints = 10;   // POINTS_PER_LEAF, src/potential.c:57
 constexpr int kDegenerate = 20;   // 2 * POINTS_PER_LEAF, src/potential.c:145
 constexpr int kWave = 64;
 constexpr int kGroup = 64;        // sink particles per walk wavefront
-constexpr int kTile = 4096;       // points per workgroup tile of a big node
+constexpr int kTile = 1024;       // points per workgroup tile of a big node
+constexpr int kSmallMax = 1024;   // largest node split by one wavefront
+constexpr int kSmallChunks = kSmallMax / 64;
+constexpr int kFatLeaf = 64;      // leaves above this size get a wavefront
 constexpr int kTileThreads = 256;
+constexpr int kMaxLevels = 4096;  // depth_end entries kept per build
 
 // Tree node, 64 bytes, read by the walk with one scalar load.
 //   [first, first + count) is the node's range in tree order (batch-global);
@@ -86,6 +90,15 @@ enum Counter {
     kLeafCount = 5,
     kErrFlags = 6,
     kBigDmin = 7,     // nodes whose dmin is reduced by tiles
+    kFatCount = 8,    // leaves of more than kFatLeaf points (failed splits)
+    // level state, advanced on the device between levels
+    kCurSmall = 9,    // small tasks of the current level
+    kBigBegin = 10,   // big tasks of the current level: [kBigBegin, kBigEnd)
+    kBigEnd = 11,
+    kTileBegin = 12,  // their tiles: [kTileBegin, kTileEnd)
+    kTileEnd = 13,
+    kLevel = 14,      // levels advanced so far
+    kActive = 15,     // the current level has work
     kNumCounters = 16
 };
 

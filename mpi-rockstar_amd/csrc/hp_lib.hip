@@ -221,17 +221,18 @@ struct Engine {
     int32_t *h_counters = nullptr;  // pinned mirror
     unsigned long long *h_work = nullptr;
     hipEvent_t ev[8] = {};
-    int32_t big_threshold = 4096;
+    int32_t big_threshold = kSmallMax;
 
     // batch tables on the device
     DevBuf hfirst, hcount, root_tiles, groups, direct_blocks, rec_call, scan_tiles, call_tile0,
         call_ntiles;
     // build
-    DevBuf box, bad, A, B, F, leaf_of, nodes, parent, arrive, small0, small1, big, tiles, parts,
-        splitres, dparts, counters, work;
+    DevBuf box, bad, A, B, F, leaf_of, nodes, parent, fat, small0, small1, big, tiles, parts,
+        splitres, dparts, counters, work, depth_buf;
     // unbind
-    DevBuf keys0, keys1, vals0, vals1, hist, srt, callinfo, tsum, tcand, nbound;
-    int64_t cap_nodes = 0, cap_small = 0, cap_big = 0, cap_tiles = 0;
+    DevBuf keys0, keys1, vals0, vals1, hist, hsum, srt, callinfo, tsum, tcand, tile_i32;
+    int64_t cap_nodes = 0, cap_small = 0, cap_big = 0, cap_tiles = 0, cap_fat = 0;
+    std::vector<int32_t> depth_end;  // node count after each depth's level
 
     // last run
     hp_stats st{};
@@ -254,15 +255,17 @@ struct Engine {
         HP_TRY(hipHostMalloc((void **)&h_work, sizeof(unsigned long long) * 4));
         for (auto &e : ev) HP_TRY(hipEventCreate(&e));
         if (const char *s = getenv("HALOPOT_BIG_THRESHOLD")) big_threshold = atoi(s);
+        if (big_threshold > kSmallMax) big_threshold = kSmallMax;
+        if (big_threshold < kLeafPoints) big_threshold = kLeafPoints;
         return HP_OK;
     }
 
     ~Engine() {
         DevBuf *all[] = {&hfirst, &hcount, &root_tiles, &groups, &direct_blocks, &rec_call,
                          &scan_tiles, &call_tile0, &call_ntiles, &box, &bad, &A, &B, &F,
-                         &leaf_of, &nodes, &parent, &arrive, &small0, &small1, &big, &tiles,
-                         &parts, &splitres, &dparts, &counters, &work, &keys0, &keys1, &vals0,
-                         &vals1, &hist, &srt, &callinfo, &tsum, &tcand, &nbound};
+                         &leaf_of, &nodes, &parent, &fat, &small0, &small1, &big, &tiles,
+                         &parts, &splitres, &dparts, &counters, &work, &depth_buf, &keys0, &keys1, &vals0,
+                         &vals1, &hist, &hsum, &srt, &callinfo, &tsum, &tcand, &tile_i32};
         if (stream) {
             (void)hipStreamSynchronize(stream);
             for (DevBuf *b : all) b->release();
@@ -294,7 +297,8 @@ struct Engine {
         cap_nodes = 2 * N + H + 2;
         HP_ENSURE(nodes, sizeof(Node) * cap_nodes);
         HP_ENSURE(parent, sizeof(int32_t) * cap_nodes);
-        HP_ENSURE(arrive, sizeof(int32_t) * cap_nodes);
+        cap_fat = N / (kFatLeaf + 1) + 16;
+        HP_ENSURE(fat, sizeof(int32_t) * cap_fat);
         cap_small = N / (kLeafPoints + 1) + H + 64;
         HP_ENSURE(small0, sizeof(Task) * cap_small);
         HP_ENSURE(small1, sizeof(Task) * cap_small);
@@ -307,6 +311,7 @@ struct Engine {
         HP_ENSURE(parts, sizeof(TilePart) * cap_tiles);
         HP_ENSURE(dparts, sizeof(double2) * cap_tiles);
         HP_ENSURE(counters, sizeof(int32_t) * kNumCounters);
+        HP_ENSURE(depth_buf, sizeof(int32_t) * kMaxLevels);
         HP_ENSURE(work, sizeof(unsigned long long) * 4);
         if (unbind) {
             HP_ENSURE(keys0, sizeof(uint64_t) * (N + 1));
@@ -315,11 +320,12 @@ struct Engine {
             HP_ENSURE(vals1, sizeof(int32_t) * (N + 1));
             const int64_t nt = (N + kSortTile - 1) / kSortTile + 1;
             HP_ENSURE(hist, sizeof(int32_t) * 256 * nt);
+            HP_ENSURE(hsum, sizeof(int32_t) * (256 * nt / kScanBlock + 2));
             HP_ENSURE(srt, sizeof(struct potential) * (N + 1));
             HP_ENSURE(callinfo, sizeof(CallInfo) * (H + 1));
             HP_ENSURE(tsum, sizeof(Moments) * (T.scan_tiles.size() + 1));
             HP_ENSURE(tcand, sizeof(Cand) * 2 * (T.scan_tiles.size() + 1));
-            HP_ENSURE(nbound, sizeof(int32_t) * (H + 1));
+            HP_ENSURE(tile_i32, sizeof(int32_t) * (T.scan_tiles.size() + 1));
         }
         return HP_OK;
     }
@@ -354,12 +360,14 @@ struct Engine {
         a.small_next = small_next;
         a.big = big.as<Task>();
         a.tiles = tiles.as<int2>();
+        a.fat = fat.as<int32_t>();
         a.counters = counters.as<int32_t>();
         a.big_threshold = big_threshold;
         a.cap_small = (int32_t)cap_small;
         a.cap_big = (int32_t)cap_big;
         a.cap_tiles = (int32_t)cap_tiles;
         a.cap_nodes = (int32_t)cap_nodes;
+        a.cap_fat = (int32_t)cap_fat;
         return a;
     }
 
@@ -393,67 +401,75 @@ struct Engine {
             HP_LAUNCHED();
         }
         int r;
-        if ((r = read_counters())) return r;
-        int32_t ns = h_counters[kSmallNext], nb = h_counters[kBigNext],
-                nt = h_counters[kTileNext];
-        int32_t b0 = 0, t0 = 0;
-        int cur = 0, level = 1;
-        while ((ns > 0 || nb > b0) && !(h_counters[kErrFlags])) {
-            if (level > 100000) return set_error(HP_EINVAL, "tree deeper than 100000 levels");
-            float4 *src = (level & 1) ? A.as<float4>() : B.as<float4>();
-            float4 *dst = (level & 1) ? B.as<float4>() : A.as<float4>();
-            BuildArgs ba = build_args(lists[cur ^ 1]);
-            k_fill_i32<<<1, 1, 0, s>>>(counters.as<int32_t>() + kSmallNext, 0, 1);
-            HP_LAUNCHED();
-            if (nt > t0) {
-                k_big_count<<<nt - t0, kTileThreads, 0, s>>>(src, tiles.as<int2>(), t0,
-                                                              big.as<Task>(), nodes.as<Node>(),
-                                                              parts.as<TilePart>());
+        k_level_advance<<<1, 1, 0, s>>>(counters.as<int32_t>(), depth_buf.as<int32_t>());
+        HP_LAUNCHED();
+        // levels run on device state; the host checks every kLevelBatch levels
+        constexpr int kLevelBatch = 8;
+        int level = 1;
+        for (;;) {
+            for (int k = 0; k < kLevelBatch; k++, level++) {
+                float4 *src = (level & 1) ? A.as<float4>() : B.as<float4>();
+                float4 *dst = (level & 1) ? B.as<float4>() : A.as<float4>();
+                const BuildArgs ba = build_args(lists[level & 1]);
+                k_big_count<<<1024, kTileThreads, 0, s>>>(src, tiles.as<int2>(), big.as<Task>(),
+                                                          nodes.as<Node>(), parts.as<TilePart>(),
+                                                          counters.as<int32_t>());
+                HP_LAUNCHED();
+                k_big_split<<<64, 64, 0, s>>>(big.as<Task>(), parts.as<TilePart>(),
+                                              splitres.as<BigSplit>(), ba);
+                HP_LAUNCHED();
+                k_big_scatter<<<1024, kTileThreads, 0, s>>>(src, dst, tiles.as<int2>(),
+                                                            big.as<Task>(), parts.as<TilePart>(),
+                                                            splitres.as<BigSplit>(), ba);
+                HP_LAUNCHED();
+                k_split_small<<<1024, kSplitThreads, 0, s>>>(src, dst, lists[(level - 1) & 1], ba);
+                HP_LAUNCHED();
+                k_level_advance<<<1, 1, 0, s>>>(counters.as<int32_t>(), depth_buf.as<int32_t>());
                 HP_LAUNCHED();
             }
-            if (nb > b0) {
-                k_big_split<<<blocks(nb - b0, 64), 64, 0, s>>>(b0, nb - b0, big.as<Task>(),
-                                                               parts.as<TilePart>(),
-                                                               splitres.as<BigSplit>(), ba);
-                HP_LAUNCHED();
-            }
-            if (nt > t0) {
-                k_big_scatter<<<nt - t0, kTileThreads, 0, s>>>(
-                    src, dst, tiles.as<int2>(), t0, big.as<Task>(), parts.as<TilePart>(),
-                    splitres.as<BigSplit>(), ba);
-                HP_LAUNCHED();
-            }
-            if (ns > 0) {
-                k_split_small<<<blocks((int64_t)ns * kWave, 256), 256, 0, s>>>(src, dst, lists[cur],
-                                                                           ns, ba);
-                HP_LAUNCHED();
-            }
-            b0 = nb;
-            t0 = nt;
-            cur ^= 1;
-            level++;
             if ((r = read_counters())) return r;
-            ns = h_counters[kSmallNext];
-            nb = h_counters[kBigNext];
-            nt = h_counters[kTileNext];
+            if (!h_counters[kActive] || h_counters[kErrFlags]) break;
+            if (level > kMaxLevels - kLevelBatch)
+                return set_error(HP_EINVAL, "tree deeper than %d levels", kMaxLevels);
         }
         if (h_counters[kErrFlags])
             return set_error(HP_EINVAL, "tree build exceeded its capacity (flags %d)",
                              h_counters[kErrFlags]);
+        const int32_t nlev = h_counters[kLevel];
+        depth_end.resize(nlev);
+        HP_TRY(hipMemcpyAsync(depth_end.data(), depth_buf.p, sizeof(int32_t) * nlev,
+                              hipMemcpyDeviceToHost, s));
+        HP_TRY(hipStreamSynchronize(s));
+        const int32_t nb = h_counters[kBigNext], nt = h_counters[kTileNext];
         const int32_t nn = h_counters[kNodeCount];
         st.nodes = nn;
-        st.leaves = h_counters[kLeafCount];
-        st.levels = level;
-        st.max_depth = h_counters[kMaxDepth];
+        // a full binary tree of n nodes has (n + 1) / 2 leaves
+        st.leaves = (nn + H) / 2;
+        st.levels = nlev;
+        st.max_depth = 0;
+        for (int d = 1; d < nlev; d++)
+            if (depth_end[d] > depth_end[d - 1]) st.max_depth = d;
         if (timing) HP_TRY(hipEventRecord(ev[1], s));
         // multipoles
         if (nn > 0) {
-            k_fill_i32<<<blocks(nn, 256), 256, 0, s>>>(arrive.as<int32_t>(), 0, nn);
+            k_leaf_com<<<blocks(nn, 256), 256, 0, s>>>(nodes.as<Node>(), F.as<float4>(), nn,
+                                                       PARTICLE_MASS);
             HP_LAUNCHED();
-            k_com<<<blocks((int64_t)nn * kWave, 256), 256, 0, s>>>(
-                nodes.as<Node>(), parent.as<int32_t>(), arrive.as<int32_t>(), F.as<float4>(), nn,
-                PARTICLE_MASS);
-            HP_LAUNCHED();
+            const int32_t nfat = h_counters[kFatCount];
+            if (nfat > 0) {
+                k_fat_leaf_com<<<blocks((int64_t)nfat * kWave, 256), 256, 0, s>>>(
+                    nodes.as<Node>(), fat.as<int32_t>(), nfat, F.as<float4>(), PARTICLE_MASS);
+                HP_LAUNCHED();
+            }
+            // internal nodes, deepest depth first; depth d holds ids
+            // [depth_end[d - 1], depth_end[d]), depth 0 the roots [0, H)
+            for (int d = (int)depth_end.size() - 1; d >= 0; d--) {
+                const int32_t b = d ? depth_end[d - 1] : 0, e = depth_end[d];
+                if (e > b) {
+                    k_internal_com<<<blocks(e - b, 256), 256, 0, s>>>(nodes.as<Node>(), b, e);
+                    HP_LAUNCHED();
+                }
+            }
             k_dmin_small<<<blocks((int64_t)nn * kWave, 256), 256, 0, s>>>(
                 nodes.as<Node>(), F.as<float4>(), nn, big_threshold);
             HP_LAUNCHED();
@@ -619,7 +635,13 @@ struct Engine {
         for (int shift = 0; shift < bits; shift += 8) {
             k_radix_hist<<<nt, kSortThreads, 0, s>>>(k0, (int32_t)n, shift, nt, hist.as<int32_t>());
             HP_LAUNCHED();
-            k_scan_i32<<<1, 1024, 0, s>>>(hist.as<int32_t>(), 256 * nt);
+            const int32_t m = 256 * nt;
+            const int32_t nbk = (m + kScanBlock - 1) / kScanBlock;
+            k_scan_blocks<<<nbk, 1024, 0, s>>>(hist.as<int32_t>(), m, hsum.as<int32_t>());
+            HP_LAUNCHED();
+            k_scan_top<<<1, 1024, 0, s>>>(hsum.as<int32_t>(), nbk);
+            HP_LAUNCHED();
+            k_scan_add<<<nbk, 1024, 0, s>>>(hist.as<int32_t>(), m, hsum.as<int32_t>());
             HP_LAUNCHED();
             k_radix_scatter<<<nt, kSortThreads, 0, s>>>(k0, v0, k1, v1, (int32_t)n, shift, nt,
                                                         hist.as<int32_t>());
@@ -628,9 +650,11 @@ struct Engine {
             std::swap(v0, v1);
         }
         sorted_vals = v0;
+        sorted_keys = k0;
         return HP_OK;
     }
     int32_t *sorted_vals = nullptr;
+    uint64_t *sorted_keys = nullptr;
 
     int unbind(const Tables &T, struct potential *d_po, hp_halo *d_halos, uint8_t *d_bound,
                const hp_options &o) {
@@ -649,15 +673,19 @@ struct Engine {
         k_gather_sorted<<<blocks(N, 256), 256, 0, s>>>(d_po, sorted_vals, (int32_t)N, S);
         HP_LAUNCHED();
         CallInfo *ci = callinfo.as<CallInfo>();
-        k_call_init<<<blocks(H, 256), 256, 0, s>>>(d_halos, H, ci);
+        const int32_t nst = (int32_t)T.scan_tiles.size();
+        int32_t *ti32 = tile_i32.as<int32_t>();
+        k_call_kept<<<blocks(H, 256), 256, 0, s>>>(sorted_keys, hfirst.as<int64_t>(),
+                                                   hcount.as<int64_t>(), H, ci);
         HP_LAUNCHED();
         const double rvir_thresh = particle_rvir_dens() * (4.0 * M_PI / 3.0);
-        k_kept_rvir<<<blocks(N, 256), 256, 0, s>>>(S, rec_call.as<int32_t>(), (int32_t)N,
-                                                  rvir_thresh, ci);
+        if (nst > 0) {
+            k_rvir_tiles<<<nst, 256, 0, s>>>(S, scan_tiles.as<ScanTile>(), nst, ci, rvir_thresh, ti32);
+            HP_LAUNCHED();
+        }
+        k_rvir_calls<<<blocks(H, 256), 256, 0, s>>>(S, call_tile0.as<int32_t>(),
+                                                    call_ntiles.as<int32_t>(), ti32, H, ci);
         HP_LAUNCHED();
-        k_core_max<<<blocks(H, 256), 256, 0, s>>>(S, H, ci);
-        HP_LAUNCHED();
-        const int32_t nst = (int32_t)T.scan_tiles.size();
         if (nst > 0) {
             k_vel_sums<<<nst, 256, 0, s>>>(S, scan_tiles.as<ScanTile>(), nst, ci,
                                            tsum.as<Moments>());
@@ -672,7 +700,7 @@ struct Engine {
         }
         k_vel_finish<<<blocks(H, 256), 256, 0, s>>>(S, call_tile0.as<int32_t>(),
                                                     call_ntiles.as<int32_t>(), H, ci,
-                                                    tsum.as<Moments>(), tcand.as<Cand>(), d_halos);
+                                                    tcand.as<Cand>(), d_halos);
         HP_LAUNCHED();
         // kinetic energy (src/potential.c:194-211 with scale_now = SCALE_NOW)
         const float a = (float)SCALE_NOW;
@@ -680,13 +708,17 @@ struct Engine {
         const double z1 = 1.0 / a;
         const double hub = 100.0 * hubble_scaling(z1 - 1.0);
         const double hub_a = hub * a;
-        k_fill_i32<<<blocks(H, 256), 256, 0, s>>>(nbound.as<int32_t>(), 0, H);
-        HP_LAUNCHED();
-        k_kinetic_bound<<<blocks(N, 256), 256, 0, s>>>(S, rec_call.as<int32_t>(), (int32_t)N,
-                                                       d_halos, hub_a, conv, d_bound,
-                                                       nbound.as<int32_t>());
-        HP_LAUNCHED();
-        k_write_nbound<<<blocks(H, 256), 256, 0, s>>>(nbound.as<int32_t>(), H, d_halos);
+        if (nst > 0) {
+            k_kinetic_tiles<<<nst, 256, 0, s>>>(S, scan_tiles.as<ScanTile>(), nst, ci, d_halos,
+                                                hub_a, conv, d_bound, ti32);
+            HP_LAUNCHED();
+        }
+        if (d_bound) {
+            k_bound_tail<<<H, 256, 0, s>>>(hfirst.as<int64_t>(), hcount.as<int64_t>(), ci, H, d_bound);
+            HP_LAUNCHED();
+        }
+        k_nbound_calls<<<blocks(H, 256), 256, 0, s>>>(call_tile0.as<int32_t>(),
+                                                      call_ntiles.as<int32_t>(), ti32, H, d_halos);
         HP_LAUNCHED();
         HP_TRY(hipMemcpyAsync(d_po, S, sizeof(struct potential) * N, hipMemcpyDeviceToDevice, s));
         return HP_OK;
@@ -725,6 +757,7 @@ static hp_options default_options() {
 struct hp_plan {
     hp::Engine eng;
     hp::Tables T;
+    std::vector<hp::Tables> per;  // per_call: one single-call table per call
     hp_options opt;
 };
 
@@ -815,8 +848,21 @@ __attribute__((visibility("default"))) int hp_plan_create(hp_plan **out, const h
             c[h] = halos[h].count;
         }
         r = make_tables(p->T, f.data(), c.data(), nhalos, p->opt.method, true);
+        if (!r && p->opt.per_call) {
+            p->per.resize(nhalos);
+            int64_t big = 0, bigh = 0;
+            for (int64_t h = 0; h < nhalos && !r; h++) {
+                int64_t f0 = 0, c0 = c[h];
+                r = make_tables(p->per[h], &f0, &c0, 1, p->opt.method, true);
+                if (c0 > big) {
+                    big = c0;
+                    bigh = h;
+                }
+            }
+            if (!r && nhalos) r = p->eng.reserve(p->per[bigh], true);
+        }
     }
-    if (!r) r = p->eng.reserve(p->T, true);
+    if (!r && !p->opt.per_call) r = p->eng.reserve(p->T, true);
     if (!r) r = p->eng.load_tables(p->T);
     if (!r && hipStreamSynchronize(p->eng.stream) != hipSuccess)
         r = set_error(HP_EHIP, "plan upload failed");
@@ -828,10 +874,31 @@ __attribute__((visibility("default"))) int hp_plan_create(hp_plan **out, const h
     return HP_OK;
 }
 
+// The plain per-halo path: every call with its own launches, one after
+// another on the plan's stream.
+static int run_per_call(hp_plan *p, struct potential *d_po, hp_halo *d_halos, uint8_t *d_bound,
+                        bool unbind) {
+    int r;
+    for (size_t h = 0; h < p->per.size(); h++) {
+        const Tables &t = p->per[h];
+        if (t.nrec == 0) continue;
+        struct potential *po_h = d_po + p->T.first[h];
+        if ((r = p->eng.load_tables(t))) return r;
+        if ((r = p->eng.potential(t, po_h, p->opt))) return r;
+        if (unbind &&
+            (r = p->eng.unbind(t, po_h, d_halos + h, d_bound ? d_bound + p->T.first[h] : nullptr,
+                               p->opt)))
+            return r;
+    }
+    HP_TRY(hipStreamSynchronize(p->eng.stream));
+    return HP_OK;
+}
+
 __attribute__((visibility("default"))) int hp_plan_potential(hp_plan *p, struct potential *d_po,
                                                              void *stream) {
     if (!p || (!d_po && p->T.nrec)) return set_error(HP_EINVAL, "hp_plan_potential: bad arguments");
     (void)stream;
+    if (p->opt.per_call) return run_per_call(p, d_po, nullptr, nullptr, false);
     return p->eng.potential(p->T, d_po, p->opt);
 }
 
@@ -841,6 +908,7 @@ __attribute__((visibility("default"))) int hp_plan_run(hp_plan *p, struct potent
     if (!p || ((!d_po || !d_halos) && p->T.nrec))
         return set_error(HP_EINVAL, "hp_plan_run: bad arguments");
     (void)stream;
+    if (p->opt.per_call) return run_per_call(p, d_po, d_halos, d_bound, true);
     int r = p->eng.potential(p->T, d_po, p->opt);
     if (r) return r;
     hipEvent_t e0 = p->eng.ev[5], e1 = p->eng.ev[6];

@@ -65,31 +65,69 @@ __global__ __launch_bounds__(kSortThreads) void k_radix_hist(const uint64_t *__r
     hist[threadIdx.x * ntiles + blockIdx.x] = s_h[threadIdx.x];
 }
 
-// exclusive scan of hist (length m) in place; one workgroup, chunked
-__global__ __launch_bounds__(1024) void k_scan_i32(int32_t *__restrict__ v, int32_t m) {
+// Exclusive scan of hist (length m) in place, in three launches: per block
+// of 4096 (1024 threads x 4), the block totals, then the offsets.
+constexpr int kScanBlock = 4096;
+
+__device__ __forceinline__ int32_t block_excl_scan_1024(int32_t x, int32_t *total) {
     __shared__ int32_t s_w[16];
+    int32_t y = x;
+    for (int o = 1; o < 64; o <<= 1) {
+        const int32_t t = __shfl_up(y, o, 64);
+        if (lane_id() >= o) y += t;
+    }
+    const int w = threadIdx.x / kWave;
+    if (lane_id() == 63) s_w[w] = y;
+    __syncthreads();
+    int32_t off = 0, tot = 0;
+    for (int j = 0; j < 16; j++) {
+        if (j < w) off += s_w[j];
+        tot += s_w[j];
+    }
+    __syncthreads();
+    *total = tot;
+    return off + y - x;
+}
+
+__global__ __launch_bounds__(1024) void k_scan_blocks(int32_t *__restrict__ v, int32_t m,
+                                                      int32_t *__restrict__ bsum) {
+    const int32_t base = blockIdx.x * kScanBlock + threadIdx.x * 4;
+    int32_t x[4], s = 0;
+    for (int k = 0; k < 4; k++) {
+        x[k] = base + k < m ? v[base + k] : 0;
+        s += x[k];
+    }
+    int32_t tot;
+    int32_t off = block_excl_scan_1024(s, &tot);
+    for (int k = 0; k < 4; k++) {
+        if (base + k < m) v[base + k] = off;
+        off += x[k];
+    }
+    if (threadIdx.x == 0) bsum[blockIdx.x] = tot;
+}
+
+__global__ __launch_bounds__(1024) void k_scan_top(int32_t *__restrict__ bsum, int32_t nb) {
     __shared__ int32_t s_carry;
     if (threadIdx.x == 0) s_carry = 0;
     __syncthreads();
-    for (int32_t base = 0; base < m; base += 1024) {
-        int32_t i = base + threadIdx.x;
-        int32_t x = i < m ? v[i] : 0;
-        // inclusive wave scan
-        int32_t y = x;
-        for (int o = 1; o < 64; o <<= 1) {
-            int32_t t = __shfl_up(y, o, 64);
-            if (lane_id() >= o) y += t;
-        }
-        const int w = threadIdx.x / kWave;
-        if (lane_id() == 63) s_w[w] = y;
+    for (int32_t base = 0; base < nb; base += 1024) {
+        const int32_t i = base + threadIdx.x;
+        const int32_t x = i < nb ? bsum[i] : 0;
+        int32_t tot;
+        const int32_t ex = block_excl_scan_1024(x, &tot);
+        if (i < nb) bsum[i] = s_carry + ex;
         __syncthreads();
-        int32_t off = s_carry;
-        for (int j = 0; j < w; j++) off += s_w[j];
-        if (i < m) v[i] = off + y - x;
-        __syncthreads();
-        if (threadIdx.x == 1023) s_carry = off + y;
+        if (threadIdx.x == 0) s_carry += tot;
         __syncthreads();
     }
+}
+
+__global__ __launch_bounds__(1024) void k_scan_add(int32_t *__restrict__ v, int32_t m,
+                                                   const int32_t *__restrict__ bsum) {
+    const int32_t add = bsum[blockIdx.x];
+    const int32_t base = blockIdx.x * kScanBlock + threadIdx.x * 4;
+    for (int k = 0; k < 4; k++)
+        if (base + k < m) v[base + k] += add;
 }
 
 // stable scatter of one tile: rounds of 256 consecutive keys, wave-level
@@ -155,68 +193,120 @@ struct CallInfo {
     int32_t core_max;
 };
 
-// kept counts: number of records with ke >= 0 per call (sorted order puts
-// them first); rvir_max by a per-call max reduction
-__global__ void k_call_init(const hp_halo *__restrict__ halos, int32_t nh,
+// kept = records with ke >= 0: they sort before the potential-only ones, so
+// it is the first index whose key has the potential-only bit (binary search)
+__global__ void k_call_kept(const uint64_t *__restrict__ skeys, const int64_t *__restrict__ hfirst,
+                            const int64_t *__restrict__ hcount, int32_t nh,
                             CallInfo *__restrict__ ci) {
     const int32_t h = blockIdx.x * blockDim.x + threadIdx.x;
     if (h >= nh) return;
+    const int32_t first = (int32_t)hfirst[h];
+    int32_t lo = 0, hi = (int32_t)hcount[h];
+    while (lo < hi) {
+        const int32_t mid = (lo + hi) >> 1;
+        if ((skeys[first + mid] >> 32) & 1ull) hi = mid;
+        else lo = mid + 1;
+    }
     CallInfo c;
-    c.first = (int32_t)halos[h].first;
-    c.kept = 0;
+    c.first = first;
+    c.kept = lo;
     c.rvir_max = -1;
     c.core_max = 100;
     ci[h] = c;
 }
 
-__global__ void k_kept_rvir(const struct potential *__restrict__ srt,
-                            const int32_t *__restrict__ rec_call, int32_t nrec,
-                            double rvir_thresh, CallInfo *__restrict__ ci) {
-    const int32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= nrec) return;
-    const struct potential &r = srt[i];
-    if (r.ke < 0.0f) return;
-    const int32_t h = rec_call[i];
-    const int32_t j = i - ci[h].first;
-    atomicAdd(&ci[h].kept, 1);
-    // src/properties.c:173-176: float r2^3, then double products
-    const float r6 = (r.r2 * r.r2) * r.r2;
-    if ((double)((int64_t)j * (int64_t)j) > (double)r6 * (rvir_thresh * rvir_thresh))
-        atomicMax(&ci[h].rvir_max, j);
-}
-
-// core_max: records with 100 r2 < r2[rvir_max] form a prefix of the sorted
-// call; binary search (src/properties.c:179-184)
-__global__ void k_core_max(const struct potential *__restrict__ srt, int32_t nh,
-                           CallInfo *__restrict__ ci) {
-    const int32_t h = blockIdx.x * blockDim.x + threadIdx.x;
-    if (h >= nh) return;
-    CallInfo c = ci[h];
-    if (c.rvir_max < 1) return;
-    const float ref = srt[c.first + c.rvir_max].r2;
-    int32_t lo = 0, hi = c.kept;  // first index with !(r2 * 100 < ref)
-    while (lo < hi) {
-        int32_t mid = (lo + hi) >> 1;
-        if ((double)srt[c.first + mid].r2 * 100.0 < (double)ref) lo = mid + 1;
-        else hi = mid;
-    }
-    int32_t cm = lo - 1;
-    ci[h].core_max = cm < 100 ? 100 : cm;
-}
-
-// Scan tiles: tile t covers sorted records [first + off, first + min(off +
-// 2048, limit)) of call h, limit = rvir_max.
+// Scan tiles: tile t covers sorted records [first + off, first + off + 2048)
+// of one call; every kernel below clips it to the call's live prefix.
 struct ScanTile {
     int32_t call;
     int32_t off;
 };
+
+__device__ __forceinline__ int32_t block_max_i32(int32_t v) {
+    __shared__ int32_t s_m[4];
+    v = wave_max(v);
+    const int w = threadIdx.x / kWave;
+    if (lane_id() == 0) s_m[w] = v;
+    __syncthreads();
+    int32_t m = s_m[0];
+    for (int q = 1; q < 4; q++) m = s_m[q] > m ? s_m[q] : m;
+    __syncthreads();
+    return m;
+}
+
+__device__ __forceinline__ int32_t block_sum_i32(int32_t v) {
+    __shared__ int32_t s_s[4];
+    v = wave_sum(v);
+    const int w = threadIdx.x / kWave;
+    if (lane_id() == 0) s_s[w] = v;
+    __syncthreads();
+    int32_t m = s_s[0] + s_s[1] + s_s[2] + s_s[3];
+    __syncthreads();
+    return m;
+}
+
+// rvir_max (src/properties.c:173-177): the largest j < kept with
+// j^2 > r2_j^3 (4pi/3 rho_vir)^2, r2^3 in float, the rest in double.
+// Per tile, then per call.
+__global__ __launch_bounds__(256) void k_rvir_tiles(const struct potential *__restrict__ srt,
+                                                    const ScanTile *__restrict__ tiles,
+                                                    int32_t ntiles,
+                                                    const CallInfo *__restrict__ ci,
+                                                    double rvir_thresh,
+                                                    int32_t *__restrict__ tile_rvir) {
+    const int32_t t = blockIdx.x;
+    if (t >= ntiles) return;
+    const ScanTile st = tiles[t];
+    const CallInfo c = ci[st.call];
+    int32_t best = -1;
+    for (int r = 0; r < kScanTile / 256; r++) {
+        const int32_t j = st.off + r * 256 + threadIdx.x;
+        if (j < c.kept) {
+            const float r2 = srt[c.first + j].r2;
+            const float r6 = (r2 * r2) * r2;
+            if ((double)((int64_t)j * (int64_t)j) > (double)r6 * (rvir_thresh * rvir_thresh))
+                best = j > best ? j : best;
+        }
+    }
+    best = block_max_i32(best);
+    if (threadIdx.x == 0) tile_rvir[t] = best;
+}
+
+// per call: rvir_max over the tiles, then core_max, the last j with
+// 100 r2_j < r2[rvir_max], by binary search over the sorted call
+// (src/properties.c:179-184), at least 100
+__global__ void k_rvir_calls(const struct potential *__restrict__ srt,
+                             const int32_t *__restrict__ call_tile0,
+                             const int32_t *__restrict__ call_ntiles,
+                             const int32_t *__restrict__ tile_rvir, int32_t nh,
+                             CallInfo *__restrict__ ci) {
+    const int32_t h = blockIdx.x * blockDim.x + threadIdx.x;
+    if (h >= nh) return;
+    CallInfo c = ci[h];
+    int32_t rv = -1;
+    for (int32_t t = call_tile0[h]; t < call_tile0[h] + call_ntiles[h]; t++)
+        rv = tile_rvir[t] > rv ? tile_rvir[t] : rv;
+    c.rvir_max = rv;
+    if (rv >= 1) {
+        const float ref = srt[c.first + rv].r2;
+        int32_t lo = 0, hi = c.kept;
+        while (lo < hi) {
+            const int32_t mid = (lo + hi) >> 1;
+            if ((double)srt[c.first + mid].r2 * 100.0 < (double)ref) lo = mid + 1;
+            else hi = mid;
+        }
+        const int32_t cm = lo - 1;
+        c.core_max = cm < 100 ? 100 : cm;
+    }
+    ci[h] = c;
+}
 
 struct Moments {
     double s1[3];
     double s2[3];
 };
 
-// per-tile sums of v and v^2
+// per-tile sums of v and v^2 over the tile's part of [0, rvir_max)
 __global__ __launch_bounds__(256) void k_vel_sums(const struct potential *__restrict__ srt,
                                                   const ScanTile *__restrict__ tiles,
                                                   int32_t ntiles, const CallInfo *__restrict__ ci,
@@ -229,18 +319,18 @@ __global__ __launch_bounds__(256) void k_vel_sums(const struct potential *__rest
     const int32_t lim = c.rvir_max;
     double m[6] = {0, 0, 0, 0, 0, 0};
     for (int r = 0; r < kScanTile / 256; r++) {
-        int32_t j = st.off + r * 256 + threadIdx.x;
+        const int32_t j = st.off + r * 256 + threadIdx.x;
         if (j >= lim) break;
         const float *v = srt[c.first + j].pos + 3;
         for (int k = 0; k < 3; k++) {
-            double x = (double)v[k];
+            const double x = (double)v[k];
             m[k] += x;
             m[3 + k] += x * x;
         }
     }
     const int w = threadIdx.x / kWave;
     for (int k = 0; k < 6; k++) {
-        double s = wave_sum(m[k]);
+        const double s = wave_sum(m[k]);
         if (lane_id() == 0) s_m[k][w] = s;
     }
     __syncthreads();
@@ -262,8 +352,8 @@ __global__ void k_vel_prefix(const int32_t *__restrict__ call_tile0,
     double a[6] = {0, 0, 0, 0, 0, 0};
     const int32_t t0 = call_tile0[h], nt = call_ntiles[h];
     for (int32_t t = t0; t < t0 + nt; t++) {
-        Moments m = tsum[t];
-        double b[6] = {m.s1[0], m.s1[1], m.s1[2], m.s2[0], m.s2[1], m.s2[2]};
+        const Moments m = tsum[t];
+        const double b[6] = {m.s1[0], m.s1[1], m.s1[2], m.s2[0], m.s2[1], m.s2[2]};
         for (int k = 0; k < 3; k++) {
             tsum[t].s1[k] = a[k];
             tsum[t].s2[k] = a[3 + k];
@@ -272,18 +362,26 @@ __global__ void k_vel_prefix(const int32_t *__restrict__ call_tile0,
     }
 }
 
-// Candidate of the running-minimum selection: value and index.
+// Candidate prefix of the running-minimum selection: its normalised
+// variance, index, and mean velocity.
 struct Cand {
     double val;
     int32_t idx;
-    int32_t pad;
+    float mean[3];
 };
 
-__device__ __forceinline__ Cand better(Cand a, Cand b) {
+__device__ __forceinline__ Cand no_cand() {
+    Cand c;
+    c.val = __longlong_as_double(0x7ff0000000000000ll);
+    c.idx = 0x7fffffff;
+    c.mean[0] = c.mean[1] = c.mean[2] = 0.0f;
+    return c;
+}
+
+__device__ __forceinline__ Cand better(const Cand &a, const Cand &b) {
     // smaller value wins; ties go to the smaller index (the reference keeps a
     // prefix only when it is strictly better)
-    if (b.val < a.val || (b.val == a.val && b.idx < a.idx)) return b;
-    return a;
+    return (b.val < a.val || (b.val == a.val && b.idx < a.idx)) ? b : a;
 }
 
 __device__ __forceinline__ Cand wave_better(Cand c) {
@@ -291,7 +389,9 @@ __device__ __forceinline__ Cand wave_better(Cand c) {
         Cand d;
         d.val = __shfl_xor(c.val, o, 64);
         d.idx = __shfl_xor(c.idx, o, 64);
-        d.pad = 0;
+        d.mean[0] = __shfl_xor(c.mean[0], o, 64);
+        d.mean[1] = __shfl_xor(c.mean[1], o, 64);
+        d.mean[2] = __shfl_xor(c.mean[2], o, 64);
         c = better(c, d);
     }
     return c;
@@ -316,10 +416,7 @@ __global__ __launch_bounds__(256) void k_vel_select(const struct potential *__re
     const int32_t core_lim = min(c.core_max, lim);
     if (threadIdx.x < 6)
         s_run[threadIdx.x] = threadIdx.x < 3 ? tsum[t].s1[threadIdx.x] : tsum[t].s2[threadIdx.x - 3];
-    Cand bc, bb;
-    bc.val = bb.val = __longlong_as_double(0x7ff0000000000000ll);
-    bc.idx = bb.idx = 0x7fffffff;
-    bc.pad = bb.pad = 0;
+    Cand bc = no_cand(), bb = no_cand();
     const int w = threadIdx.x / kWave;
     __syncthreads();
     for (int r = 0; r < kScanTile / 256; r++) {
@@ -333,12 +430,11 @@ __global__ __launch_bounds__(256) void k_vel_select(const struct potential *__re
                 x[3 + k] = x[k] * x[k];
             }
         }
-        // inclusive block scan of the six moments
         double y[6];
         for (int k = 0; k < 6; k++) {
             double s = x[k];
             for (int o = 1; o < 64; o <<= 1) {
-                double u = __shfl_up(s, o, 64);
+                const double u = __shfl_up(s, o, 64);
                 if (lane_id() >= o) s += u;
             }
             y[k] = s;
@@ -359,7 +455,7 @@ __global__ __launch_bounds__(256) void k_vel_select(const struct potential *__re
             Cand me;
             me.val = j > 3 ? var / (double)((int64_t)(j - 3) * j) : 0.0;
             me.idx = j;
-            me.pad = 0;
+            for (int k = 0; k < 3; k++) me.mean[k] = (float)(y[k] / n);
             if (j < core_lim) bc = better(bc, me);
             bb = better(bb, me);
         }
@@ -385,44 +481,32 @@ __global__ __launch_bounds__(256) void k_vel_select(const struct potential *__re
     }
 }
 
-// prefix mean of velocities at index j (sum over [0, j])
-__device__ void prefix_mean(const struct potential *__restrict__ srt, int32_t first, int32_t j,
-                            const Moments *__restrict__ tsum, int32_t tile0, float *out) {
-    const int32_t t = tile0 + j / kScanTile;
-    double s[3] = {tsum[t].s1[0], tsum[t].s1[1], tsum[t].s1[2]};
-    for (int32_t q = (j / kScanTile) * kScanTile; q <= j; q++) {
-        const float *v = srt[first + q].pos + 3;
-        for (int k = 0; k < 3; k++) s[k] += (double)v[k];
-    }
-    for (int k = 0; k < 3; k++) out[k] = (float)(s[k] / (double)(j + 1));
-}
-
-__device__ double prefix_val(const struct potential *__restrict__ srt, int32_t first, int32_t j,
-                             const Moments *__restrict__ tsum, int32_t tile0) {
-    if (j <= 3) return 0.0;
-    const int32_t t = tile0 + j / kScanTile;
-    double s[6] = {tsum[t].s1[0], tsum[t].s1[1], tsum[t].s1[2],
-                   tsum[t].s2[0], tsum[t].s2[1], tsum[t].s2[2]};
-    for (int32_t q = (j / kScanTile) * kScanTile; q <= j; q++) {
-        const float *v = srt[first + q].pos + 3;
+// mean and normalised variance of the first j + 1 records, j < 10, directly
+__device__ void small_prefix(const struct potential *__restrict__ srt, int32_t first, int32_t j,
+                             float *mean, double *val) {
+    double s[3] = {0, 0, 0}, q[3] = {0, 0, 0};
+    for (int32_t i = 0; i <= j; i++) {
+        const float *v = srt[first + i].pos + 3;
         for (int k = 0; k < 3; k++) {
-            double x = (double)v[k];
-            s[k] += x;
-            s[3 + k] += x * x;
+            s[k] += (double)v[k];
+            q[k] += (double)v[k] * (double)v[k];
         }
     }
     const double n = (double)(j + 1);
     double var = 0.0;
-    for (int k = 0; k < 3; k++) var += s[3 + k] - s[k] * s[k] / n;
-    return var / (double)((int64_t)(j - 3) * j);
+    for (int k = 0; k < 3; k++) {
+        mean[k] = (float)(s[k] / n);
+        var += q[k] - s[k] * s[k] / n;
+    }
+    *val = j > 3 ? var / (double)((int64_t)(j - 3) * j) : 0.0;
 }
 
 // one thread per call: final selection, halo fields
 __global__ void k_vel_finish(const struct potential *__restrict__ srt,
                              const int32_t *__restrict__ call_tile0,
                              const int32_t *__restrict__ call_ntiles, int32_t nh,
-                             const CallInfo *__restrict__ ci, const Moments *__restrict__ tsum,
-                             const Cand *__restrict__ tcand, hp_halo *__restrict__ halos) {
+                             const CallInfo *__restrict__ ci, const Cand *__restrict__ tcand,
+                             hp_halo *__restrict__ halos) {
     const int32_t h = blockIdx.x * blockDim.x + threadIdx.x;
     if (h >= nh) return;
     const CallInfo c = ci[h];
@@ -432,64 +516,94 @@ __global__ void k_vel_finish(const struct potential *__restrict__ srt,
     if (c.rvir_max >= 1) {
         const int32_t lim = c.rvir_max;
         const int32_t core_lim = min(c.core_max, lim);
-        const int32_t t0 = call_tile0[h], nt = call_ntiles[h];
-        Cand a, b;
-        a.val = b.val = __longlong_as_double(0x7ff0000000000000ll);
-        a.idx = b.idx = 0x7fffffff;
-        a.pad = b.pad = 0;
-        for (int32_t t = t0; t < t0 + nt; t++) {
+        Cand a = no_cand(), b = no_cand();
+        for (int32_t t = call_tile0[h]; t < call_tile0[h] + call_ntiles[h]; t++) {
             a = better(a, tcand[2 * t]);
             b = better(b, tcand[2 * t + 1]);
         }
-        // windows shorter than 10: the last forced update wins
-        const int32_t ic = core_lim <= 10 ? core_lim - 1 : a.idx;
-        const int32_t ib = lim <= 10 ? lim - 1 : b.idx;
-        float vc[3], vb[3];
-        prefix_mean(srt, c.first, ic, tsum, t0, vc);
-        prefix_mean(srt, c.first, ib, tsum, t0, vb);
-        hh.n_core = ic;
-        hh.min_vel_err = (float)prefix_val(srt, c.first, ic, tsum, t0);
-        hh.min_bulkvel_err = (float)prefix_val(srt, c.first, ib, tsum, t0);
-        for (int k = 0; k < 3; k++) {
-            hh.corevel[k] = vc[k];
-            hh.bulkvel[k] = vb[k];
+        // windows of at most 10 records: the last forced update wins
+        if (core_lim <= 10) {
+            a.idx = core_lim - 1;
+            small_prefix(srt, c.first, a.idx, a.mean, &a.val);
         }
-        for (int k = 0; k < 3; k++) hh.pos[3 + k] = hh.corevel[k];  // src/properties.c:213
+        if (lim <= 10) {
+            b.idx = lim - 1;
+            small_prefix(srt, c.first, b.idx, b.mean, &b.val);
+        }
+        hh.n_core = a.idx;
+        hh.min_vel_err = (float)a.val;
+        hh.min_bulkvel_err = (float)b.val;
+        for (int k = 0; k < 3; k++) {
+            hh.corevel[k] = a.mean[k];
+            hh.bulkvel[k] = b.mean[k];
+            hh.pos[3 + k] = a.mean[k];  // src/properties.c:213
+        }
     }
     halos[h] = hh;
 }
 
 // ------------------------------------------------------- kinetic energy
 
-// src/potential.c:194-211 on the sorted kept records of each call; bound flag
-// !(pe < ke) (src/properties.c:403).  Potential-only records keep ke < 0.
-__global__ void k_kinetic_bound(struct potential *__restrict__ srt,
-                                const int32_t *__restrict__ rec_call, int32_t nrec,
-                                const hp_halo *__restrict__ halos, double hub_a, double conv,
-                                uint8_t *__restrict__ bound, int32_t *__restrict__ nbound) {
-    const int32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= nrec) return;
-    struct potential r = srt[i];
-    const int32_t h = rec_call[i];
-    if (r.ke < 0.0f) {
-        if (bound) bound[i] = 0;
-        return;
-    }
-    const hp_halo &hh = halos[h];
+// src/potential.c:194-211 on the sorted kept records of each call, about
+// corevel (sub-halos) or bulkvel (hosts); bound flag !(pe < ke)
+// (src/properties.c:403).  One workgroup per scan tile; bound counts per tile.
+__global__ __launch_bounds__(256) void k_kinetic_tiles(struct potential *__restrict__ srt,
+                                                       const ScanTile *__restrict__ tiles,
+                                                       int32_t ntiles,
+                                                       const CallInfo *__restrict__ ci,
+                                                       const hp_halo *__restrict__ halos,
+                                                       double hub_a, double conv,
+                                                       uint8_t *__restrict__ bound,
+                                                       int32_t *__restrict__ tile_nb) {
+    const int32_t t = blockIdx.x;
+    if (t >= ntiles) return;
+    const ScanTile st = tiles[t];
+    const CallInfo c = ci[st.call];
+    const hp_halo &hh = halos[st.call];
     const float *vc = hh.is_sub ? hh.corevel : hh.bulkvel;
-    float ke = 0.0f;
-    for (int k = 0; k < 3; k++) {
-        const float dvp = r.pos[k + 3] - vc[k];
-        const float dxp = r.pos[k] - hh.pos[k];
-        const double dv = (double)dvp + hub_a * (double)dxp;
-        ke = (float)((double)ke + dv * dv);
+    const float v0 = vc[0], v1 = vc[1], v2 = vc[2];
+    const float c0 = hh.pos[0], c1 = hh.pos[1], c2 = hh.pos[2];
+    int32_t nb = 0;
+    for (int r = 0; r < kScanTile / 256; r++) {
+        const int32_t j = st.off + r * 256 + threadIdx.x;
+        if (j >= c.kept) continue;
+        struct potential &rec = srt[c.first + j];
+        const float cen_v[3] = {v0, v1, v2}, cen_x[3] = {c0, c1, c2};
+        float ke = 0.0f;
+        for (int k = 0; k < 3; k++) {
+            const float dvp = rec.pos[k + 3] - cen_v[k];
+            const float dxp = rec.pos[k] - cen_x[k];
+            const double dv = (double)dvp + hub_a * (double)dxp;
+            ke = (float)((double)ke + dv * dv);
+        }
+        ke = (float)((double)ke * conv);
+        rec.ke = ke;
+        const bool b = !(rec.pe < (double)ke);
+        if (bound) bound[c.first + j] = b ? 1 : 0;
+        nb += b ? 1 : 0;
     }
-    ke = (float)((double)ke * conv);
-    r.ke = ke;
-    srt[i].ke = ke;
-    const bool b = !(r.pe < (double)ke);
-    if (bound) bound[i] = b ? 1 : 0;
-    if (b) atomicAdd(&nbound[h], 1);
+    nb = block_sum_i32(nb);
+    if (threadIdx.x == 0) tile_nb[t] = nb;
+}
+
+// potential-only records: bound = 0
+__global__ void k_bound_tail(const int64_t *__restrict__ hfirst, const int64_t *__restrict__ hcount,
+                             const CallInfo *__restrict__ ci, int32_t nh, uint8_t *__restrict__ bound) {
+    const int32_t h = blockIdx.x;
+    if (h >= nh) return;
+    const CallInfo c = ci[h];
+    for (int64_t j = c.kept + threadIdx.x; j < hcount[h]; j += blockDim.x) bound[hfirst[h] + j] = 0;
+}
+
+__global__ void k_nbound_calls(const int32_t *__restrict__ call_tile0,
+                               const int32_t *__restrict__ call_ntiles,
+                               const int32_t *__restrict__ tile_nb, int32_t nh,
+                               hp_halo *__restrict__ halos) {
+    const int32_t h = blockIdx.x * blockDim.x + threadIdx.x;
+    if (h >= nh) return;
+    int64_t n = 0;
+    for (int32_t t = call_tile0[h]; t < call_tile0[h] + call_ntiles[h]; t++) n += tile_nb[t];
+    halos[h].n_bound = n;
 }
 
 // the plain reference entry point: every record with ke >= 0, one centre
@@ -508,12 +622,6 @@ __global__ void k_kinetic_plain(struct potential *__restrict__ po, int64_t n, fl
         ke = (float)((double)ke + dv * dv);
     }
     po[i].ke = (float)((double)ke * conv);
-}
-
-__global__ void k_write_nbound(const int32_t *__restrict__ nbound, int32_t nh,
-                               hp_halo *__restrict__ halos) {
-    const int32_t h = blockIdx.x * blockDim.x + threadIdx.x;
-    if (h < nh) halos[h].n_bound = nbound[h];
 }
 
 }  // namespace hp

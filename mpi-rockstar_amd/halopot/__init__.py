@@ -47,7 +47,8 @@ class HalopotError(RuntimeError):
 
 class Options(C.Structure):
     _fields_ = [("accumulate", C.c_int32), ("method", C.c_int32), ("use_graph", C.c_int32),
-                ("timing", C.c_int32), ("count_work", C.c_int32), ("reserved", C.c_int32 * 3)]
+                ("timing", C.c_int32), ("count_work", C.c_int32), ("per_call", C.c_int32),
+                ("reserved", C.c_int32 * 2)]
 
 
 class Stats(C.Structure):
@@ -141,10 +142,10 @@ def compute_kinetic_energy(po: np.ndarray, vel_cen, pos_cen, scale_now: float) -
 
 
 def options(accumulate=ACC_COMPENSATED, method=METHOD_TREE, use_graph=0, timing=0,
-            count_work=0) -> Options:
+            count_work=0, per_call=0) -> Options:
     o = Options()
-    o.accumulate, o.method, o.use_graph, o.timing, o.count_work = (
-        accumulate, method, use_graph, timing, count_work)
+    o.accumulate, o.method, o.use_graph, o.timing, o.count_work, o.per_call = (
+        accumulate, method, use_graph, timing, count_work, per_call)
     return o
 
 

@@ -96,6 +96,9 @@ class Restatement:
         L.orc_bound_pass_batch.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p,
                                            C.c_int64, C.POINTER(_Params), C.c_void_p,
                                            C.c_void_p, C.c_int]
+        L.orc_bound_pass_batch_fn.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p,
+                                              C.c_int64, C.POINTER(_Params), C.c_void_p, C.c_int,
+                                              C.c_void_p, C.c_void_p]
         L.orc_hubble_scaling.restype = C.c_double
         L.orc_hubble_scaling.argtypes = [C.POINTER(_Params), C.c_double]
         L.orc_max_threads.restype = C.c_int
@@ -168,6 +171,21 @@ class Restatement:
         self.lib.orc_bound_pass_batch(_p(po), _p(first), _p(count), _p(halos), halos.shape[0],
                                       C.byref(_params(prm)), _p(bound), _p(kept), nthreads)
         return kept, bound
+
+    def bound_pass_batch_with(self, po, halos, prm, reference, nthreads=0):
+        """The driver of bound_pass_batch with the compiled reference's own
+        compute_potential and compute_kinetic_energy (the reference's CPU
+        path); reference is a Reference whose globals are set from prm."""
+        _check_records(po)
+        reference.set_globals(prm)
+        first = np.ascontiguousarray(halos["first"])
+        count = np.ascontiguousarray(halos["count"])
+        kept = np.zeros(halos.shape[0], np.int64)
+        cp = C.cast(reference.lib.compute_potential, C.c_void_p)
+        ke = C.cast(reference.lib.compute_kinetic_energy, C.c_void_p)
+        self.lib.orc_bound_pass_batch_fn(_p(po), _p(first), _p(count), _p(halos), halos.shape[0],
+                                         C.byref(_params(prm)), _p(kept), nthreads, cp, ke)
+        return kept
 
     def hubble_scaling(self, prm, z):
         return self.lib.orc_hubble_scaling(C.byref(_params(prm)), z)

@@ -622,6 +622,48 @@ void orc_bound_pass_batch(struct potential *po, const int64_t *first,
                                  bound ? bound + first[h] : NULL);
 }
 
+/* The same driver with caller-supplied potential and kinetic-energy
+   functions: with the compiled reference's compute_potential and
+   compute_kinetic_energy this is the reference's own CPU path for the bound
+   pass (src/properties.c:537-550, src/mpi_main.cpp:1377), only the short
+   strip / sort / calculate_corevel steps being restated. */
+typedef void (*orc_cp_fn)(struct potential *, int64_t);
+typedef void (*orc_ke_fn)(struct potential *, int64_t, float *, float *, float);
+
+static int64_t orc_bound_pass_fn(struct potential *po, int64_t n, orc_halo *h,
+                                 const orc_params *p, orc_cp_fn cp, orc_ke_fn ke) {
+    if (n > 1) cp(po, n);
+    int64_t kept = 0;
+    for (int64_t i = 0; i < n; i++)
+        if (!(po[i].ke < 0)) po[kept++] = po[i];
+    if (kept > 1) {
+        struct potential *tmp = orc_grow(NULL, sizeof(struct potential) * (size_t)kept);
+        orc_merge_sort_r2(po, tmp, kept);
+        free(tmp);
+    }
+    orc_corevel(h, po, kept, p);
+    ke(po, kept, h->is_sub ? h->corevel : h->bulkvel, h->pos, (float)p->scale_now);
+    int64_t nb = 0;
+    for (int64_t i = 0; i < kept; i++) nb += !(po[i].pe < po[i].ke);
+    h->kept = kept;
+    h->n_bound = nb;
+    return kept;
+}
+
+void orc_bound_pass_batch_fn(struct potential *po, const int64_t *first,
+                             const int64_t *count, orc_halo *halos, int64_t nhalos,
+                             const orc_params *p, int64_t *kept, int nthreads,
+                             orc_cp_fn cp, orc_ke_fn ke) {
+#ifdef _OPENMP
+    if (nthreads > 0) omp_set_num_threads(nthreads);
+#else
+    (void)nthreads;
+#endif
+#pragma omp parallel for schedule(dynamic)
+    for (int64_t h = 0; h < nhalos; h++)
+        kept[h] = orc_bound_pass_fn(po + first[h], count[h], halos + h, p, cp, ke);
+}
+
 /* src/groupies.c:97-100 */
 double orc_vir_density(const orc_params *p, double a) {
     double e = orc_hubble_scaling(p, 1.0 / a - 1.0);
