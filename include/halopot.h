@@ -56,11 +56,13 @@ void compute_potential(struct potential *po, int64_t num_po);
 void compute_kinetic_energy(struct potential *po, int64_t num_po,
                             float *vel_cen, float *pos_cen, float scale_now);
 
-/* Configuration globals read at call time (src/config.template.h:7,16,22-25
- * through src/config_vars.h).  libhalopot carries weak definitions with the
- * reference's defaults, so the program that defines the real ones (the
- * halo finder) overrides them and stand-alone callers can set them here. */
-extern double FORCE_RES, PARTICLE_MASS, Om, Ol, W0, WA;
+/* Configuration globals read at call time (src/config.template.h:7,16,20,
+ * 22-25 through src/config_vars.h).  libhalopot carries weak definitions with
+ * the reference's defaults, so the program that defines the real ones (the
+ * halo finder) overrides them and stand-alone callers can set them here.
+ * SCALE_NOW is read only by the batched bound pass (the reference entry
+ * point takes scale_now as an argument). */
+extern double FORCE_RES, PARTICLE_MASS, Om, Ol, W0, WA, SCALE_NOW;
 
 /* ----------------------------------------------------------- batched path */
 

@@ -99,6 +99,8 @@ class Restatement:
         L.orc_bound_pass_batch_fn.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p,
                                               C.c_int64, C.POINTER(_Params), C.c_void_p, C.c_int,
                                               C.c_void_p, C.c_void_p]
+        L.orc_set_never_accept.argtypes = [C.c_int]
+        L.orc_set_nmass_out.argtypes = [C.c_void_p]
         L.orc_hubble_scaling.restype = C.c_double
         L.orc_hubble_scaling.argtypes = [C.POINTER(_Params), C.c_double]
         L.orc_max_threads.restype = C.c_int
@@ -128,6 +130,21 @@ class Restatement:
         if tree:
             out["tree"] = nodes[:nn]
         return out
+
+    def compute_potential_mass(self, po: np.ndarray, prm):
+        """compute_potential that also returns, per record in tree order, the
+        P2P partner count and the number of points behind its monopoles."""
+        nmass = np.zeros(po.shape[0], np.int64)
+        self.lib.orc_set_nmass_out(_p(nmass))
+        try:
+            out = self.compute_potential(po, prm, counts=True, tree=True)
+        finally:
+            self.lib.orc_set_nmass_out(None)
+        out["nmass"] = nmass
+        return out
+
+    def never_accept(self, on: bool) -> None:
+        self.lib.orc_set_never_accept(1 if on else 0)
 
     def build_tree(self, po: np.ndarray, prm) -> np.ndarray:
         _check_records(po)

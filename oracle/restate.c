@@ -270,7 +270,13 @@ typedef struct orc_ctx {
     double force_res, particle_mass;
     int32_t *np2p, *np2m; /* optional per-particle interaction counters */
     int64_t p2p, p2m, visits;
+    int64_t *nmass;       /* optional: points behind the accepted monopoles */
 } orc_ctx;
+
+/* Test hook: 1 makes every monopole unacceptable, so the walk reduces to the
+   direct sum over leaf pairs. */
+static int orc_never_accept = 0;
+void orc_set_never_accept(int on) { orc_never_accept = on; }
 
 static double orc_pair(const orc_ctx *x, const float *a, const float *b) {
     double r = sqrt(orc_d2(a, b));
@@ -311,11 +317,14 @@ static void orc_monopole(orc_ctx *x, const orc_node *s, const orc_node *b) {
     x->p2m += s->n;
     if (x->np2m)
         for (int64_t i = 0; i < s->n; i++) x->np2m[s->first + i] += 1;
+    if (x->nmass)
+        for (int64_t i = 0; i < s->n; i++) x->nmass[s->first + i] += b->n;
 }
 
 /* src/potential.c:126-131: float distance of the two centres against the
    float square of the source's dmin */
 static int orc_accept(const orc_node *s, const orc_node *b) {
+    if (orc_never_accept) return 0;
     float r2 = (float)orc_d2(s->com, b->com);
     return r2 > b->dmin * b->dmin;
 }
@@ -344,6 +353,11 @@ static void orc_visit(orc_ctx *x, const orc_node *s, const orc_node *b) {
 
 /* --------------------------------------------------------- exported: tree */
 
+/* optional per-record output of the next orc_compute_potential call: the
+   number of points behind the monopoles each record received */
+static int64_t *orc_nmass_out = NULL;
+void orc_set_nmass_out(int64_t *nmass) { orc_nmass_out = nmass; }
+
 typedef struct orc_walk_stats {
     int64_t p2p, p2m, visits, nodes, leaves, max_depth, nfinite;
 } orc_walk_stats;
@@ -359,13 +373,14 @@ int64_t orc_compute_potential(struct potential *po, int64_t n,
     for (int64_t i = 0; i < n; i++) po[i].pe = 0;
     if (np2p) memset(np2p, 0, sizeof(int32_t) * (size_t)(n > 0 ? n : 0));
     if (np2m) memset(np2m, 0, sizeof(int32_t) * (size_t)(n > 0 ? n : 0));
+    if (orc_nmass_out) memset(orc_nmass_out, 0, sizeof(int64_t) * (size_t)(n > 0 ? n : 0));
     if (n <= 0) return 0;
     orc_tree t = {0};
     struct potential *scratch = orc_grow(NULL, sizeof(struct potential) * (size_t)n);
     orc_build(&t, po, n, scratch);
     free(scratch);
     orc_multipoles(&t, po, prm->particle_mass);
-    orc_ctx x = {&t, po, prm->force_res, prm->particle_mass, np2p, np2m, 0, 0, 0};
+    orc_ctx x = {&t, po, prm->force_res, prm->particle_mass, np2p, np2m, 0, 0, 0, orc_nmass_out};
     int64_t leaves = 0;
     if (t.nfinite > 0)
         for (int64_t c = 0; c < t.nnodes; c++)
@@ -445,7 +460,7 @@ void orc_compute_potential_bh64(struct potential *po, int64_t n,
     orc_build(&t, po, n, scratch);
     free(scratch);
     orc_multipoles(&t, po, prm->particle_mass);
-    orc_ctx x = {&t, po, prm->force_res, prm->particle_mass, NULL, NULL, 0, 0, 0};
+    orc_ctx x = {&t, po, prm->force_res, prm->particle_mass, NULL, NULL, 0, 0, 0, NULL};
     if (t.nfinite > 0)
         for (int64_t c = 0; c < t.nnodes; c++)
             if (t.nodes[c].left < 0) orc_visit64(&x, t.nodes + c, t.nodes);
