@@ -107,6 +107,7 @@ def main() -> int:
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--workload", default="c2", choices=sorted(WORKLOADS))
     ap.add_argument("--accumulate", default="compensated", choices=["compensated", "naive", "f64"])
+    ap.add_argument("--no-graph", action="store_true", help="eager launches instead of hipGraph replay")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--no-per-halo", action="store_true", help="skip the per-halo GPU path")
     ap.add_argument("--cpu-seconds", type=float, default=8.0)
@@ -152,7 +153,8 @@ def main() -> int:
     log(f"walk work: {wst['p2p']} P2P pairs, {wst['p2m']} P2M terms, {wst['visits']} "
         f"wavefront node visits, {wst['nodes']} nodes, depth {wst['max_depth']}")
 
-    plan = halopot.Plan(cat.halos, halopot.options(accumulate=acc, timing=1))
+    use_graph = 0 if args.no_graph else 1
+    plan = halopot.Plan(cat.halos, halopot.options(accumulate=acc, timing=1, use_graph=use_graph))
     for i in range(args.warmup):
         restore()
         plan.run(d_po.address, d_h.address, d_b.address)
@@ -208,7 +210,8 @@ def main() -> int:
         "data": "synthetic",
         "config": {"workload": args.workload + ": " + WORKLOADS[args.workload],
                    "records": cat.n_records, "calls": cat.n_halos,
-                   "accumulate": args.accumulate, "parallelism": "one GPU"},
+                   "accumulate": args.accumulate, "graph": bool(use_graph),
+                   "parallelism": "one GPU"},
         "roofline": roofline,
     }
 

@@ -707,27 +707,33 @@ __global__ void k_level_advance(int32_t *__restrict__ c, int32_t *__restrict__ d
 
 // --------------------------------------------------------------- multipoles
 
+// The multipole kernels take their sizes from the device counters (node
+// count, fat-leaf count, per-depth node ranges, big tasks and tiles), so a
+// whole build can be replayed from a graph.
+
 // Leaf mass and centre (src/potential.c:100-107): f64 sums of the float
 // positions, one thread per leaf of at most kFatLeaf points.
 __global__ void k_leaf_com(Node *__restrict__ nodes, const float4 *__restrict__ fin,
-                           int32_t nnodes, double particle_mass) {
-    const int32_t id = blockIdx.x * blockDim.x + threadIdx.x;
-    if (id >= nnodes) return;
-    const Node nd = nodes[id];
-    if (nd.left >= 0 || nd.count <= 0 || nd.count > kFatLeaf) return;
-    double s0 = 0.0, s1 = 0.0, s2 = 0.0;
-    for (int32_t i = 0; i < nd.count; i++) {
-        const float4 p = fin[nd.first + i];
-        s0 += (double)p.x;
-        s1 += (double)p.y;
-        s2 += (double)p.z;
+                           const int32_t *__restrict__ counters, double particle_mass) {
+    const int32_t nnodes = counters[kNodeCount];
+    for (int32_t id = blockIdx.x * blockDim.x + threadIdx.x; id < nnodes;
+         id += gridDim.x * blockDim.x) {
+        const Node nd = nodes[id];
+        if (nd.left >= 0 || nd.count <= 0 || nd.count > kFatLeaf) continue;
+        double s0 = 0.0, s1 = 0.0, s2 = 0.0;
+        for (int32_t i = 0; i < nd.count; i++) {
+            const float4 p = fin[nd.first + i];
+            s0 += (double)p.x;
+            s1 += (double)p.y;
+            s2 += (double)p.z;
+        }
+        const double n = (double)nd.count;
+        Node *me = nodes + id;
+        me->com[0] = s0 != 0.0 ? (float)(s0 / n) : 0.0f;
+        me->com[1] = s1 != 0.0 ? (float)(s1 / n) : 0.0f;
+        me->com[2] = s2 != 0.0 ? (float)(s2 / n) : 0.0f;
+        me->mass = (float)(particle_mass * n);
     }
-    const double n = (double)nd.count;
-    Node *me = nodes + id;
-    me->com[0] = s0 != 0.0 ? (float)(s0 / n) : 0.0f;
-    me->com[1] = s1 != 0.0 ? (float)(s1 / n) : 0.0f;
-    me->com[2] = s2 != 0.0 ? (float)(s2 / n) : 0.0f;
-    me->mass = (float)(particle_mass * n);
 }
 
 // The same for leaves of more than kFatLeaf points (failed splits), one
@@ -735,48 +741,55 @@ __global__ void k_leaf_com(Node *__restrict__ nodes, const float4 *__restrict__ 
 // wavefront reduction does not matter.
 __global__ __launch_bounds__(256) void k_fat_leaf_com(Node *__restrict__ nodes,
                                                       const int32_t *__restrict__ list,
-                                                      int32_t nlist,
-                                                      const float4 *__restrict__ fin,
+                                                      const int32_t *__restrict__ counters,
+                                                      int32_t cap, const float4 *__restrict__ fin,
                                                       double particle_mass) {
-    const int32_t k = (int32_t)((blockIdx.x * blockDim.x + threadIdx.x) / kWave);
-    if (k >= nlist) return;
+    const int32_t nlist = min(counters[kFatCount], cap);
+    const int32_t nw = (int32_t)(gridDim.x * blockDim.x / kWave);
     const int lane = lane_id();
-    const int32_t id = list[k];
-    const Node nd = nodes[id];
-    double s[3] = {0.0, 0.0, 0.0};
-    for (int32_t i = lane; i < nd.count; i += kWave) {
-        const float4 p = fin[nd.first + i];
-        s[0] += (double)p.x;
-        s[1] += (double)p.y;
-        s[2] += (double)p.z;
-    }
-    for (int k2 = 0; k2 < 3; k2++) s[k2] = wave_sum(s[k2]);
-    if (lane == 0) {
-        const double n = (double)nd.count;
-        Node *me = nodes + id;
-        for (int k2 = 0; k2 < 3; k2++) me->com[k2] = s[k2] != 0.0 ? (float)(s[k2] / n) : 0.0f;
-        me->mass = (float)(particle_mass * n);
+    for (int32_t k = (int32_t)((blockIdx.x * blockDim.x + threadIdx.x) / kWave); k < nlist;
+         k += nw) {
+        const int32_t id = list[k];
+        const Node nd = nodes[id];
+        double s[3] = {0.0, 0.0, 0.0};
+        for (int32_t i = lane; i < nd.count; i += kWave) {
+            const float4 p = fin[nd.first + i];
+            s[0] += (double)p.x;
+            s[1] += (double)p.y;
+            s[2] += (double)p.z;
+        }
+        for (int k2 = 0; k2 < 3; k2++) s[k2] = wave_sum(s[k2]);
+        if (lane == 0) {
+            const double n = (double)nd.count;
+            Node *me = nodes + id;
+            for (int k2 = 0; k2 < 3; k2++) me->com[k2] = s[k2] != 0.0 ? (float)(s[k2] / n) : 0.0f;
+            me->mass = (float)(particle_mass * n);
+        }
     }
 }
 
-// Internal nodes of one depth, ids [begin, end): float mass and centre from
-// the two children, right term first (src/potential.c:115-122).  Node ids
-// grow with depth, so the host runs the depths deepest first.
-__global__ void k_internal_com(Node *__restrict__ nodes, int32_t begin, int32_t end) {
-    const int32_t id = begin + blockIdx.x * blockDim.x + threadIdx.x;
-    if (id >= end) return;
-    Node *me = nodes + id;
-    const int32_t L = me->left;
-    if (L < 0) return;
-    const Node *l = nodes + L, *r = nodes + L + 1;
-    const float ml = l->mass, mr = r->mass;
-    const float m = ml + mr;
-    for (int k = 0; k < 3; k++) {
-        const float rr = r->com[k] * mr;
-        const float ll = l->com[k] * ml;
-        me->com[k] = m != 0.0f ? (rr + ll) / m : 0.0f;
+// Internal nodes of depth d: float mass and centre from the two children,
+// right term first (src/potential.c:115-122).  Node ids grow with depth:
+// depth d holds [depth_end[d - 1], depth_end[d]), depth 0 the roots; the
+// host runs the depths deepest first.
+__global__ void k_internal_com(Node *__restrict__ nodes, const int32_t *__restrict__ depth_end,
+                               int32_t d) {
+    const int32_t begin = d ? depth_end[d - 1] : 0, end = depth_end[d];
+    for (int32_t id = begin + blockIdx.x * blockDim.x + threadIdx.x; id < end;
+         id += gridDim.x * blockDim.x) {
+        Node *me = nodes + id;
+        const int32_t L = me->left;
+        if (L < 0) continue;
+        const Node *l = nodes + L, *r = nodes + L + 1;
+        const float ml = l->mass, mr = r->mass;
+        const float m = ml + mr;
+        for (int k = 0; k < 3; k++) {
+            const float rr = r->com[k] * mr;
+            const float ll = l->com[k] * ml;
+            me->com[k] = m != 0.0f ? (rr + ll) / m : 0.0f;
+        }
+        me->mass = m;
     }
-    me->mass = m;
 }
 
 __device__ __forceinline__ double d2_ref(const float *c, const float4 &p) {
@@ -801,70 +814,81 @@ __device__ __forceinline__ void finish_dmin(Node *nd, double big, double sum, in
 // One wavefront per node of at most big_threshold points.
 __global__ __launch_bounds__(256) void k_dmin_small(Node *__restrict__ nodes,
                                                     const float4 *__restrict__ fin,
-                                                    int32_t nnodes, int32_t big_threshold) {
-    const int32_t id = (int32_t)((blockIdx.x * blockDim.x + threadIdx.x) / kWave);
-    if (id >= nnodes) return;
+                                                    const int32_t *__restrict__ counters,
+                                                    int32_t big_threshold) {
+    const int32_t nnodes = counters[kNodeCount];
+    const int32_t nw = (int32_t)(gridDim.x * blockDim.x / kWave);
     const int lane = lane_id();
-    const Node nd = nodes[id];
-    if (nd.count <= 0 || nd.count > big_threshold) return;
-    double big = 0.0, sum = 0.0;
-    for (int32_t i = lane; i < nd.count; i += kWave) {
-        double d = d2_ref(nd.com, fin[nd.first + i]);
-        big = d > big ? d : big;
-        sum += d;
+    for (int32_t id = (int32_t)((blockIdx.x * blockDim.x + threadIdx.x) / kWave); id < nnodes;
+         id += nw) {
+        const Node nd = nodes[id];
+        if (nd.count <= 0 || nd.count > big_threshold) continue;
+        double big = 0.0, sum = 0.0;
+        for (int32_t i = lane; i < nd.count; i += kWave) {
+            const double d = d2_ref(nd.com, fin[nd.first + i]);
+            big = d > big ? d : big;
+            sum += d;
+        }
+        big = wave_max(big);
+        sum = wave_sum(sum);
+        if (lane == 0) finish_dmin(nodes + id, big, sum, nd.count);
     }
-    big = wave_max(big);
-    sum = wave_sum(sum);
-    if (lane == 0) finish_dmin(nodes + id, big, sum, nd.count);
 }
 
-// Tile partials of dmin for big nodes, then one thread per big node.
+// Tile partials of dmin for big nodes (every big task of every level), then
+// one thread per big node sums its tiles in order.
 __global__ __launch_bounds__(kTileThreads) void k_dmin_tiles(const Node *__restrict__ nodes,
                                                              const float4 *__restrict__ fin,
                                                              const int2 *__restrict__ tiles,
                                                              const Task *__restrict__ big,
+                                                             const int32_t *__restrict__ counters,
                                                              double2 *__restrict__ part) {
     __shared__ double s_b[kTileThreads / kWave], s_s[kTileThreads / kWave];
-    const int2 tl = tiles[blockIdx.x];
-    const Node nd = nodes[big[tl.x].node];
-    double bmax = 0.0, sum = 0.0;
-    for (int r = 0; r < kTile / kTileThreads; r++) {
-        int32_t i = tl.y + r * kTileThreads + threadIdx.x;
-        if (i >= nd.count) break;
-        double d = d2_ref(nd.com, fin[nd.first + i]);
-        bmax = d > bmax ? d : bmax;
-        sum += d;
-    }
-    bmax = wave_max(bmax);
-    sum = wave_sum(sum);
-    const int w = threadIdx.x / kWave;
-    if (lane_id() == 0) {
-        s_b[w] = bmax;
-        s_s[w] = sum;
-    }
-    __syncthreads();
-    if (threadIdx.x == 0) {
-        double b = s_b[0], s = s_s[0];
-        for (int j = 1; j < kTileThreads / kWave; j++) {
-            b = s_b[j] > b ? s_b[j] : b;
-            s += s_s[j];
+    const int32_t ntiles = counters[kTileNext];
+    for (int32_t ti = blockIdx.x; ti < ntiles; ti += gridDim.x) {
+        const int2 tl = tiles[ti];
+        const Node nd = nodes[big[tl.x].node];
+        double bmax = 0.0, sum = 0.0;
+        for (int r = 0; r < kTile / kTileThreads; r++) {
+            const int32_t i = tl.y + r * kTileThreads + threadIdx.x;
+            if (i >= nd.count) break;
+            const double d = d2_ref(nd.com, fin[nd.first + i]);
+            bmax = d > bmax ? d : bmax;
+            sum += d;
         }
-        part[blockIdx.x] = make_double2(b, s);
+        bmax = wave_max(bmax);
+        sum = wave_sum(sum);
+        const int w = threadIdx.x / kWave;
+        if (lane_id() == 0) {
+            s_b[w] = bmax;
+            s_s[w] = sum;
+        }
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            double b = s_b[0], s = s_s[0];
+            for (int j = 1; j < kTileThreads / kWave; j++) {
+                b = s_b[j] > b ? s_b[j] : b;
+                s += s_s[j];
+            }
+            part[ti] = make_double2(b, s);
+        }
+        __syncthreads();
     }
 }
 
 __global__ void k_dmin_big(Node *__restrict__ nodes, const Task *__restrict__ big,
-                           int32_t nbig, const double2 *__restrict__ part) {
-    const int32_t k = blockIdx.x * blockDim.x + threadIdx.x;
-    if (k >= nbig) return;
-    const Task t = big[k];
-    double b = 0.0, s = 0.0;
-    for (int32_t j = 0; j < t.ntiles; j++) {
-        double2 p = part[t.tile0 + j];
-        b = p.x > b ? p.x : b;
-        s += p.y;
+                           const int32_t *__restrict__ counters, const double2 *__restrict__ part) {
+    const int32_t nbig = counters[kBigNext];
+    for (int32_t k = blockIdx.x * blockDim.x + threadIdx.x; k < nbig; k += gridDim.x * blockDim.x) {
+        const Task t = big[k];
+        double b = 0.0, s = 0.0;
+        for (int32_t j = 0; j < t.ntiles; j++) {
+            const double2 p = part[t.tile0 + j];
+            b = p.x > b ? p.x : b;
+            s += p.y;
+        }
+        finish_dmin(nodes + t.node, b, s, nodes[t.node].count);
     }
-    finish_dmin(nodes + t.node, b, s, nodes[t.node].count);
 }
 
 }  // namespace hp

@@ -268,6 +268,7 @@ struct Engine {
                          &vals1, &hist, &hsum, &srt, &callinfo, &tsum, &tcand, &tile_i32};
         if (stream) {
             (void)hipStreamSynchronize(stream);
+            drop_graphs();
             for (DevBuf *b : all) b->release();
             for (auto &e : ev)
                 if (e) (void)hipEventDestroy(e);
@@ -373,12 +374,15 @@ struct Engine {
 
     static int blocks(int64_t n, int per) { return (int)((n + per - 1) / per); }
 
-    // Tree build + multipoles for every call with a root queued; leaves the
-    // tree in nodes / F / leaf_of.
-    int build(const Tables &T, const struct potential *d_po, bool timing) {
+    // ------------------------------------------------------ enqueue stages
+    // Nothing below reads device memory back: every stage can be recorded
+    // into a graph.  The eager driver (potential) checks the level state
+    // between batches of levels; the graph driver replays a planned number
+    // of levels and checks once.
+
+    int enqueue_roots(const Tables &T, const struct potential *d_po) {
         const int32_t H = (int32_t)T.nhalos;
         hipStream_t s = stream;
-        if (timing) HP_TRY(hipEventRecord(ev[0], s));
         if (H > 0) {
             k_root_init<<<blocks(H, 256), 256, 0, s>>>(box.as<int32_t>(), bad.as<int32_t>(), H);
             HP_LAUNCHED();
@@ -393,104 +397,71 @@ struct Engine {
         HP_LAUNCHED();
         k_fill_i32<<<1, 1, 0, s>>>(counters.as<int32_t>() + kNodeCount, H, 1);
         HP_LAUNCHED();
-        Task *lists[2] = {small0.as<Task>(), small1.as<Task>()};
         if (H > 0) {
             k_roots<<<H, 256, 0, s>>>(hfirst.as<int64_t>(), hcount.as<int64_t>(),
                                       box.as<int32_t>(), bad.as<int32_t>(), A.as<float4>(),
-                                      build_args(lists[0]));
+                                      build_args(small0.as<Task>()));
             HP_LAUNCHED();
         }
-        int r;
         k_level_advance<<<1, 1, 0, s>>>(counters.as<int32_t>(), depth_buf.as<int32_t>());
         HP_LAUNCHED();
-        // levels run on device state; the host checks every kLevelBatch levels
-        constexpr int kLevelBatch = 8;
-        int level = 1;
-        for (;;) {
-            for (int k = 0; k < kLevelBatch; k++, level++) {
-                float4 *src = (level & 1) ? A.as<float4>() : B.as<float4>();
-                float4 *dst = (level & 1) ? B.as<float4>() : A.as<float4>();
-                const BuildArgs ba = build_args(lists[level & 1]);
-                k_big_count<<<1024, kTileThreads, 0, s>>>(src, tiles.as<int2>(), big.as<Task>(),
-                                                          nodes.as<Node>(), parts.as<TilePart>(),
-                                                          counters.as<int32_t>());
-                HP_LAUNCHED();
-                k_big_split<<<64, 64, 0, s>>>(big.as<Task>(), parts.as<TilePart>(),
-                                              splitres.as<BigSplit>(), ba);
-                HP_LAUNCHED();
-                k_big_scatter<<<1024, kTileThreads, 0, s>>>(src, dst, tiles.as<int2>(),
-                                                            big.as<Task>(), parts.as<TilePart>(),
-                                                            splitres.as<BigSplit>(), ba);
-                HP_LAUNCHED();
-                k_split_small<<<1024, kSplitThreads, 0, s>>>(src, dst, lists[(level - 1) & 1], ba);
-                HP_LAUNCHED();
-                k_level_advance<<<1, 1, 0, s>>>(counters.as<int32_t>(), depth_buf.as<int32_t>());
-                HP_LAUNCHED();
-            }
-            if ((r = read_counters())) return r;
-            if (!h_counters[kActive] || h_counters[kErrFlags]) break;
-            if (level > kMaxLevels - kLevelBatch)
-                return set_error(HP_EINVAL, "tree deeper than %d levels", kMaxLevels);
-        }
-        if (h_counters[kErrFlags])
-            return set_error(HP_EINVAL, "tree build exceeded its capacity (flags %d)",
-                             h_counters[kErrFlags]);
-        const int32_t nlev = h_counters[kLevel];
-        depth_end.resize(nlev);
-        HP_TRY(hipMemcpyAsync(depth_end.data(), depth_buf.p, sizeof(int32_t) * nlev,
-                              hipMemcpyDeviceToHost, s));
-        HP_TRY(hipStreamSynchronize(s));
-        const int32_t nb = h_counters[kBigNext], nt = h_counters[kTileNext];
-        const int32_t nn = h_counters[kNodeCount];
-        st.nodes = nn;
-        // a full binary tree of n nodes has (n + 1) / 2 leaves
-        st.leaves = (nn + H) / 2;
-        st.levels = nlev;
-        st.max_depth = 0;
-        for (int d = 1; d < nlev; d++)
-            if (depth_end[d] > depth_end[d - 1]) st.max_depth = d;
-        if (timing) HP_TRY(hipEventRecord(ev[1], s));
-        // multipoles
-        if (nn > 0) {
-            k_leaf_com<<<blocks(nn, 256), 256, 0, s>>>(nodes.as<Node>(), F.as<float4>(), nn,
-                                                       PARTICLE_MASS);
-            HP_LAUNCHED();
-            const int32_t nfat = h_counters[kFatCount];
-            if (nfat > 0) {
-                k_fat_leaf_com<<<blocks((int64_t)nfat * kWave, 256), 256, 0, s>>>(
-                    nodes.as<Node>(), fat.as<int32_t>(), nfat, F.as<float4>(), PARTICLE_MASS);
-                HP_LAUNCHED();
-            }
-            // internal nodes, deepest depth first; depth d holds ids
-            // [depth_end[d - 1], depth_end[d]), depth 0 the roots [0, H)
-            for (int d = (int)depth_end.size() - 1; d >= 0; d--) {
-                const int32_t b = d ? depth_end[d - 1] : 0, e = depth_end[d];
-                if (e > b) {
-                    k_internal_com<<<blocks(e - b, 256), 256, 0, s>>>(nodes.as<Node>(), b, e);
-                    HP_LAUNCHED();
-                }
-            }
-            k_dmin_small<<<blocks((int64_t)nn * kWave, 256), 256, 0, s>>>(
-                nodes.as<Node>(), F.as<float4>(), nn, big_threshold);
-            HP_LAUNCHED();
-        }
-        if (nt > 0) {
-            k_dmin_tiles<<<nt, kTileThreads, 0, s>>>(nodes.as<Node>(), F.as<float4>(),
-                                                     tiles.as<int2>(), big.as<Task>(),
-                                                     dparts.as<double2>());
-            HP_LAUNCHED();
-            k_dmin_big<<<blocks(nb, 64), 64, 0, s>>>(nodes.as<Node>(), big.as<Task>(), nb,
-                                                     dparts.as<double2>());
-            HP_LAUNCHED();
-        }
-        if (timing) HP_TRY(hipEventRecord(ev[2], s));
         return HP_OK;
     }
 
-    int walk(const Tables &T, struct potential *d_po, const hp_options &o) {
+    // split levels [first, first + n); level L reads its small tasks from
+    // list (L - 1) & 1 and the points from buffer A (odd L) or B (even L)
+    int enqueue_levels(int first, int n) {
         hipStream_t s = stream;
-        const bool timing = o.timing != 0;
-        if (o.count_work) HP_TRY(hipMemsetAsync(work.p, 0, sizeof(unsigned long long) * 4, s));
+        Task *lists[2] = {small0.as<Task>(), small1.as<Task>()};
+        for (int level = first; level < first + n; level++) {
+            float4 *src = (level & 1) ? A.as<float4>() : B.as<float4>();
+            float4 *dst = (level & 1) ? B.as<float4>() : A.as<float4>();
+            const BuildArgs ba = build_args(lists[level & 1]);
+            k_big_count<<<1024, kTileThreads, 0, s>>>(src, tiles.as<int2>(), big.as<Task>(),
+                                                      nodes.as<Node>(), parts.as<TilePart>(),
+                                                      counters.as<int32_t>());
+            HP_LAUNCHED();
+            k_big_split<<<64, 64, 0, s>>>(big.as<Task>(), parts.as<TilePart>(),
+                                          splitres.as<BigSplit>(), ba);
+            HP_LAUNCHED();
+            k_big_scatter<<<1024, kTileThreads, 0, s>>>(src, dst, tiles.as<int2>(), big.as<Task>(),
+                                                        parts.as<TilePart>(),
+                                                        splitres.as<BigSplit>(), ba);
+            HP_LAUNCHED();
+            k_split_small<<<1024, kSplitThreads, 0, s>>>(src, dst, lists[(level - 1) & 1], ba);
+            HP_LAUNCHED();
+            k_level_advance<<<1, 1, 0, s>>>(counters.as<int32_t>(), depth_buf.as<int32_t>());
+            HP_LAUNCHED();
+        }
+        return HP_OK;
+    }
+
+    // multipoles of a tree built in `levels` split levels
+    int enqueue_multipole(int levels) {
+        hipStream_t s = stream;
+        const int32_t *cnt = counters.as<int32_t>();
+        k_leaf_com<<<2048, 256, 0, s>>>(nodes.as<Node>(), F.as<float4>(), cnt, PARTICLE_MASS);
+        HP_LAUNCHED();
+        k_fat_leaf_com<<<256, 256, 0, s>>>(nodes.as<Node>(), fat.as<int32_t>(), cnt,
+                                           (int32_t)cap_fat, F.as<float4>(), PARTICLE_MASS);
+        HP_LAUNCHED();
+        for (int d = levels; d >= 0; d--) {
+            k_internal_com<<<512, 256, 0, s>>>(nodes.as<Node>(), depth_buf.as<int32_t>(), d);
+            HP_LAUNCHED();
+        }
+        k_dmin_small<<<4096, 256, 0, s>>>(nodes.as<Node>(), F.as<float4>(), cnt, big_threshold);
+        HP_LAUNCHED();
+        k_dmin_tiles<<<1024, kTileThreads, 0, s>>>(nodes.as<Node>(), F.as<float4>(),
+                                                   tiles.as<int2>(), big.as<Task>(), cnt,
+                                                   dparts.as<double2>());
+        HP_LAUNCHED();
+        k_dmin_big<<<64, 256, 0, s>>>(nodes.as<Node>(), big.as<Task>(), cnt, dparts.as<double2>());
+        HP_LAUNCHED();
+        return HP_OK;
+    }
+
+    int enqueue_walk(const Tables &T, struct potential *d_po, const hp_options &o) {
+        hipStream_t s = stream;
         if (!T.groups.empty()) {
             WalkArgs a;
             a.nodes = nodes.as<Node>();
@@ -498,7 +469,8 @@ struct Engine {
             a.leaf_of = leaf_of.as<int32_t>();
             a.groups = groups.as<Group>();
             a.ngroups = (int32_t)T.groups.size();
-            a.nnodes = (int32_t)st.nodes;
+            a.counters = counters.as<int32_t>();
+            a.nrec = (int32_t)T.nrec;
             a.pm = (float)PARTICLE_MASS;
             a.inv_fr = (float)(1.0 / FORCE_RES);
             a.pm_d = PARTICLE_MASS;
@@ -507,20 +479,18 @@ struct Engine {
             a.err = counters.as<int32_t>() + kErrFlags;
             a.work = work.as<unsigned long long>();
             const int nb = blocks((int64_t)T.groups.size() * kWave, 256);
-            const int acc = o.accumulate;
             const bool cnt = o.count_work != 0;
 #define HP_WALK(ACC)                                                                   \
     do {                                                                               \
         if (cnt) k_walk<ACC, true><<<nb, 256, 0, s>>>(a);                              \
         else k_walk<ACC, false><<<nb, 256, 0, s>>>(a);                                 \
     } while (0)
-            if (acc == HP_ACC_NAIVE) HP_WALK(HP_ACC_NAIVE);
-            else if (acc == HP_ACC_F64) HP_WALK(HP_ACC_F64);
+            if (o.accumulate == HP_ACC_NAIVE) HP_WALK(HP_ACC_NAIVE);
+            else if (o.accumulate == HP_ACC_F64) HP_WALK(HP_ACC_F64);
             else HP_WALK(HP_ACC_COMPENSATED);
 #undef HP_WALK
             HP_LAUNCHED();
         }
-        if (timing) HP_TRY(hipEventRecord(ev[3], s));
         if (!T.direct_blocks.empty()) {
             DirectArgs d;
             d.nodes = nodes.as<Node>();
@@ -534,100 +504,223 @@ struct Engine {
             d.po = d_po;
             if (T.method == HP_METHOD_DIRECT) {
                 k_zero_pe<<<blocks(T.nrec, 256), 256, 0, s>>>(d_po, T.nrec);
-                HP_LAUNCHED();
             } else {
                 k_zero_pe_calls<<<(int)T.direct_blocks.size(), 256, 0, s>>>(
                     d_po, hfirst.as<int64_t>(), hcount.as<int64_t>(), direct_blocks.as<int2>());
-                HP_LAUNCHED();
             }
+            HP_LAUNCHED();
             const int nbk = (int)T.direct_blocks.size();
             if (o.accumulate == HP_ACC_NAIVE) k_direct<HP_ACC_NAIVE><<<nbk, 256, 0, s>>>(d);
             else if (o.accumulate == HP_ACC_F64) k_direct<HP_ACC_F64><<<nbk, 256, 0, s>>>(d);
             else k_direct<HP_ACC_COMPENSATED><<<nbk, 256, 0, s>>>(d);
             HP_LAUNCHED();
         }
-        if (timing) HP_TRY(hipEventRecord(ev[4], s));
         return HP_OK;
     }
 
-    int potential(const Tables &T, struct potential *d_po, const hp_options &o) {
-        st = hp_stats();
-        st.records = T.nrec;
-        st.halos = T.nhalos;
+    int read_work() {
+        HP_TRY(hipMemcpyAsync(h_work, work.p, sizeof(unsigned long long) * 4,
+                              hipMemcpyDeviceToHost, stream));
+        HP_TRY(hipStreamSynchronize(stream));
+        st.visits = (int64_t)h_work[0];
+        st.p2p = (int64_t)h_work[1];
+        st.p2m = (int64_t)h_work[2];
+        return HP_OK;
+    }
+
+    // Runs split levels from `level` until the level state is inactive,
+    // checking every kLevelBatch levels.  Returns the next level index.
+    int finish_levels(int level, int *next) {
+        constexpr int kLevelBatch = 8;
         int r;
-        if (T.method == HP_METHOD_DIRECT) {
-            // the direct kernel needs only the gathered, compacted roots
-            if ((r = build_roots_only(T, d_po, o.timing != 0))) return r;
-        } else if ((r = build(T, d_po, o.timing != 0))) {
-            return r;
+        for (;;) {
+            if ((r = enqueue_levels(level, kLevelBatch))) return r;
+            level += kLevelBatch;
+            if ((r = read_counters())) return r;
+            if (!h_counters[kActive] || h_counters[kErrFlags]) break;
+            if (level > kMaxLevels - kLevelBatch)
+                return set_error(HP_EINVAL, "tree deeper than %d levels", kMaxLevels);
         }
-        if ((r = walk(T, d_po, o))) return r;
+        *next = level;
+        return HP_OK;
+    }
+
+    int check_build() {
+        if (h_counters[kErrFlags])
+            return set_error(HP_EINVAL, "tree build failed (flags %d)", h_counters[kErrFlags]);
+        const int32_t nn = h_counters[kNodeCount];
+        st.nodes = nn;
+        st.leaves = (nn + st.halos) / 2;  // a full binary tree of n nodes has (n + 1) / 2 leaves
+        st.levels = h_counters[kLevel] - 1;
+        return HP_OK;
+    }
+
+    int check_walk() {
+        int r;
         if ((r = read_counters())) return r;
         if (h_counters[kErrFlags])
             return set_error(HP_EINVAL, "walk stopped early (flags %d): corrupt tree",
                              h_counters[kErrFlags]);
+        return HP_OK;
+    }
+
+    // The eager driver: learns the depth of the tree as it builds it.
+    int potential(const Tables &T, struct potential *d_po, const hp_options &o) {
+        st = hp_stats();
+        st.records = T.nrec;
+        st.halos = T.nhalos;
+        hipStream_t s = stream;
+        const bool timing = o.timing != 0;
+        int r;
+        if (timing) HP_TRY(hipEventRecord(ev[0], s));
+        if ((r = enqueue_roots(T, d_po))) return r;
+        if (T.method != HP_METHOD_DIRECT) {
+            int next = 1;
+            if ((r = finish_levels(1, &next))) return r;
+            if ((r = check_build())) return r;
+            if (timing) HP_TRY(hipEventRecord(ev[1], s));
+            if ((r = enqueue_multipole((int)st.levels))) return r;
+        } else if (timing) {
+            HP_TRY(hipEventRecord(ev[1], s));
+        }
+        if (timing) HP_TRY(hipEventRecord(ev[2], s));
+        if (o.count_work) HP_TRY(hipMemsetAsync(work.p, 0, sizeof(unsigned long long) * 4, s));
+        if ((r = enqueue_walk(T, d_po, o))) return r;
+        if (timing) HP_TRY(hipEventRecord(ev[3], s));
+        if ((r = check_walk())) return r;
         st.walk_groups = (int64_t)T.groups.size();
         st.walk_launches = T.groups.empty() ? 0 : 1;
-        if (o.count_work) {
-            HP_TRY(hipMemcpyAsync(h_work, work.p, sizeof(unsigned long long) * 4,
-                                  hipMemcpyDeviceToHost, stream));
-            HP_TRY(hipStreamSynchronize(stream));
-            st.visits = (int64_t)h_work[0];
-            st.p2p = (int64_t)h_work[1];
-            st.p2m = (int64_t)h_work[2];
-        }
-        if (o.timing) {
-            HP_TRY(hipStreamSynchronize(stream));
+        if (o.count_work && (r = read_work())) return r;
+        if (timing) {
             float t;
-            if (T.method != HP_METHOD_DIRECT) {
-                HP_TRY(hipEventElapsedTime(&t, ev[0], ev[1]));
-                st.ms_build = t;
-                HP_TRY(hipEventElapsedTime(&t, ev[1], ev[2]));
-                st.ms_multipole = t;
-            }
+            HP_TRY(hipEventElapsedTime(&t, ev[0], ev[1]));
+            st.ms_build = t;
+            HP_TRY(hipEventElapsedTime(&t, ev[1], ev[2]));
+            st.ms_multipole = t;
             HP_TRY(hipEventElapsedTime(&t, ev[2], ev[3]));
             st.ms_walk = t;
-            HP_TRY(hipEventElapsedTime(&t, ev[3], ev[4]));
-            st.ms_direct = t;
-            HP_TRY(hipEventElapsedTime(&t, ev[0], ev[4]));
+            HP_TRY(hipEventElapsedTime(&t, ev[0], ev[3]));
             st.ms_total = t;
         }
         return HP_OK;
     }
 
-    int build_roots_only(const Tables &T, const struct potential *d_po, bool timing) {
-        const int32_t H = (int32_t)T.nhalos;
+    // ------------------------------------------------------------ graphs
+
+    struct Graphs {
+        hipGraphExec_t build = nullptr, multipole = nullptr, walk = nullptr, unbind = nullptr;
+        int levels = 0;  // split levels recorded in `build`
+        const void *po = nullptr, *halos = nullptr, *bound = nullptr;
+        bool ok = false;
+    } g;
+
+    void drop_graphs() {
+        for (hipGraphExec_t *e : {&g.build, &g.multipole, &g.walk, &g.unbind})
+            if (*e) {
+                (void)hipGraphExecDestroy(*e);
+                *e = nullptr;
+            }
+        g.ok = false;
+    }
+
+    template <class F>
+    int capture(hipGraphExec_t *out, F &&enqueue) {
+        hipGraph_t graph = nullptr;
+        HP_TRY(hipStreamBeginCapture(stream, hipStreamCaptureModeThreadLocal));
+        const int r = enqueue();
+        const hipError_t e = hipStreamEndCapture(stream, &graph);
+        if (r) {
+            if (graph) (void)hipGraphDestroy(graph);
+            return r;
+        }
+        if (e != hipSuccess)
+            return set_error(HP_EHIP, "graph capture failed: %s", hipGetErrorString(e));
+        const hipError_t ie = hipGraphInstantiate(out, graph, nullptr, nullptr, 0);
+        (void)hipGraphDestroy(graph);
+        if (ie != hipSuccess)
+            return set_error(HP_EHIP, "graph instantiation failed: %s", hipGetErrorString(ie));
+        return HP_OK;
+    }
+
+    // Records the four stages for this batch and these buffers, with the
+    // number of split levels the last eager build needed.
+    int capture_all(const Tables &T, struct potential *d_po, hp_halo *d_halos, uint8_t *d_bound,
+                    const hp_options &o, int levels) {
+        drop_graphs();
+        int r;
+        if ((r = capture(&g.build, [&] {
+                 int q = enqueue_roots(T, d_po);
+                 return q ? q : enqueue_levels(1, levels);
+             })))
+            return r;
+        if ((r = capture(&g.multipole, [&] { return enqueue_multipole(levels); }))) return r;
+        if ((r = capture(&g.walk, [&] { return enqueue_walk(T, d_po, o); }))) return r;
+        if ((r = capture(&g.unbind, [&] { return enqueue_unbind(T, d_po, d_halos, d_bound); })))
+            return r;
+        g.levels = levels;
+        g.po = d_po;
+        g.halos = d_halos;
+        g.bound = d_bound;
+        g.ok = true;
+        return HP_OK;
+    }
+
+    // Graph driver of hp_plan_run.  A tree deeper than the recorded levels
+    // is finished eagerly from the device state and the graphs re-recorded
+    // on the next call.
+    int run_graphs(const Tables &T, struct potential *d_po, hp_halo *d_halos, uint8_t *d_bound,
+                   const hp_options &o) {
         hipStream_t s = stream;
-        if (timing) HP_TRY(hipEventRecord(ev[0], s));
-        if (H > 0) {
-            k_root_init<<<blocks(H, 256), 256, 0, s>>>(box.as<int32_t>(), bad.as<int32_t>(), H);
-            HP_LAUNCHED();
+        int r;
+        if (!g.ok || g.po != d_po || g.halos != d_halos || g.bound != d_bound) {
+            // learn the depth eagerly, then record
+            if ((r = potential(T, d_po, o))) return r;
+            if ((r = enqueue_unbind(T, d_po, d_halos, d_bound))) return r;
+            HP_TRY(hipStreamSynchronize(s));
+            return capture_all(T, d_po, d_halos, d_bound, o, (int)st.levels + 1);
         }
-        if (!T.root_tiles.empty()) {
-            k_gather<<<(int)T.root_tiles.size(), kTileThreads, 0, s>>>(
-                d_po, hfirst.as<int64_t>(), hcount.as<int64_t>(), root_tiles.as<RootTile>(),
-                A.as<float4>(), box.as<int32_t>(), bad.as<int32_t>());
-            HP_LAUNCHED();
+        st = hp_stats();
+        st.records = T.nrec;
+        st.halos = T.nhalos;
+        HP_TRY(hipEventRecord(ev[0], s));
+        HP_TRY(hipGraphLaunch(g.build, s));
+        if ((r = read_counters())) return r;
+        bool fresh = true;
+        if (h_counters[kActive] && !h_counters[kErrFlags]) {
+            int next = 0;
+            if ((r = finish_levels(g.levels + 1, &next))) return r;
+            fresh = false;
         }
-        k_fill_i32<<<1, 64, 0, s>>>(counters.as<int32_t>(), 0, kNumCounters);
-        k_fill_i32<<<1, 1, 0, s>>>(counters.as<int32_t>() + kNodeCount, H, 1);
-        HP_LAUNCHED();
-        if (H > 0) {
-            k_roots<<<H, 256, 0, s>>>(hfirst.as<int64_t>(), hcount.as<int64_t>(),
-                                      box.as<int32_t>(), bad.as<int32_t>(), A.as<float4>(),
-                                      build_args(small0.as<Task>()));
-            HP_LAUNCHED();
-        }
-        if (timing) {
-            HP_TRY(hipEventRecord(ev[1], s));
-            HP_TRY(hipEventRecord(ev[2], s));
-        }
+        if ((r = check_build())) return r;
+        HP_TRY(hipEventRecord(ev[1], s));
+        if (fresh) HP_TRY(hipGraphLaunch(g.multipole, s));
+        else if ((r = enqueue_multipole((int)st.levels))) return r;
+        HP_TRY(hipEventRecord(ev[2], s));
+        HP_TRY(hipGraphLaunch(g.walk, s));
+        HP_TRY(hipEventRecord(ev[3], s));
+        HP_TRY(hipGraphLaunch(g.unbind, s));
+        HP_TRY(hipEventRecord(ev[4], s));
+        if ((r = check_walk())) return r;
+        if (!fresh) g.ok = false;  // record again with more levels next time
+        st.walk_groups = (int64_t)T.groups.size();
+        st.walk_launches = T.groups.empty() ? 0 : 1;
+        float t;
+        HP_TRY(hipEventElapsedTime(&t, ev[0], ev[1]));
+        st.ms_build = t;
+        HP_TRY(hipEventElapsedTime(&t, ev[1], ev[2]));
+        st.ms_multipole = t;
+        HP_TRY(hipEventElapsedTime(&t, ev[2], ev[3]));
+        st.ms_walk = t;
+        HP_TRY(hipEventElapsedTime(&t, ev[3], ev[4]));
+        st.ms_unbind = t;
+        HP_TRY(hipEventElapsedTime(&t, ev[0], ev[4]));
+        st.ms_total = t;
         return HP_OK;
     }
 
     // ------------------------------------------------------------ bound pass
 
-    int radix_sort(int64_t n, int bits) {
+    int enqueue_sort(int64_t n, int bits) {
         hipStream_t s = stream;
         const int32_t nt = (int32_t)((n + kSortTile - 1) / kSortTile);
         uint64_t *k0 = keys0.as<uint64_t>(), *k1 = keys1.as<uint64_t>();
@@ -656,8 +749,8 @@ struct Engine {
     int32_t *sorted_vals = nullptr;
     uint64_t *sorted_keys = nullptr;
 
-    int unbind(const Tables &T, struct potential *d_po, hp_halo *d_halos, uint8_t *d_bound,
-               const hp_options &o) {
+    int enqueue_unbind(const Tables &T, struct potential *d_po, hp_halo *d_halos,
+                       uint8_t *d_bound) {
         hipStream_t s = stream;
         const int64_t N = T.nrec;
         const int32_t H = (int32_t)T.nhalos;
@@ -668,7 +761,7 @@ struct Engine {
         int hbits = 1;
         while ((1ll << hbits) < H) hbits++;
         int r;
-        if ((r = radix_sort(N, 33 + hbits))) return r;
+        if ((r = enqueue_sort(N, 33 + hbits))) return r;
         struct potential *S = srt.as<struct potential>();
         k_gather_sorted<<<blocks(N, 256), 256, 0, s>>>(d_po, sorted_vals, (int32_t)N, S);
         HP_LAUNCHED();
@@ -885,9 +978,8 @@ static int run_per_call(hp_plan *p, struct potential *d_po, hp_halo *d_halos, ui
         struct potential *po_h = d_po + p->T.first[h];
         if ((r = p->eng.load_tables(t))) return r;
         if ((r = p->eng.potential(t, po_h, p->opt))) return r;
-        if (unbind &&
-            (r = p->eng.unbind(t, po_h, d_halos + h, d_bound ? d_bound + p->T.first[h] : nullptr,
-                               p->opt)))
+        if (unbind && (r = p->eng.enqueue_unbind(t, po_h, d_halos + h,
+                                                 d_bound ? d_bound + p->T.first[h] : nullptr)))
             return r;
     }
     HP_TRY(hipStreamSynchronize(p->eng.stream));
@@ -909,11 +1001,16 @@ __attribute__((visibility("default"))) int hp_plan_run(hp_plan *p, struct potent
         return set_error(HP_EINVAL, "hp_plan_run: bad arguments");
     (void)stream;
     if (p->opt.per_call) return run_per_call(p, d_po, d_halos, d_bound, true);
-    int r = p->eng.potential(p->T, d_po, p->opt);
-    if (r) return r;
+    int r;
+    if (p->opt.use_graph && p->opt.method != HP_METHOD_DIRECT && !p->opt.count_work) {
+        if ((r = p->eng.run_graphs(p->T, d_po, d_halos, d_bound, p->opt))) return r;
+        HP_TRY(hipStreamSynchronize(p->eng.stream));
+        return HP_OK;
+    }
+    if ((r = p->eng.potential(p->T, d_po, p->opt))) return r;
     hipEvent_t e0 = p->eng.ev[5], e1 = p->eng.ev[6];
     if (p->opt.timing) HP_TRY(hipEventRecord(e0, p->eng.stream));
-    if ((r = p->eng.unbind(p->T, d_po, d_halos, d_bound, p->opt))) return r;
+    if ((r = p->eng.enqueue_unbind(p->T, d_po, d_halos, d_bound))) return r;
     if (p->opt.timing) {
         HP_TRY(hipEventRecord(e1, p->eng.stream));
         HP_TRY(hipEventSynchronize(e1));
@@ -992,7 +1089,12 @@ __attribute__((visibility("default"))) int64_t hp_debug_tree(const struct potent
     DevBuf &dpo = e->srt;
     if ((r = dpo.ensure(sizeof(struct potential) * (n + 1)))) return r;
     HP_TRY(hipMemcpyAsync(dpo.p, po, sizeof(struct potential) * n, hipMemcpyHostToDevice, e->stream));
-    if ((r = e->build(T, dpo.as<struct potential>(), false))) return r;
+    e->st = hp_stats();
+    e->st.halos = 1;
+    int next = 0;
+    if ((r = e->enqueue_roots(T, dpo.as<struct potential>())) || (r = e->finish_levels(1, &next)) ||
+        (r = e->check_build()) || (r = e->enqueue_multipole((int)e->st.levels)))
+        return r;
     const int64_t nn = e->st.nodes;
     std::vector<Node> nodes(nn);
     std::vector<int32_t> par(nn);

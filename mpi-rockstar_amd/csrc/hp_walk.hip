@@ -107,7 +107,8 @@ struct WalkArgs {
     const int32_t *leaf_of;
     const Group *groups;
     int32_t ngroups;
-    int32_t nnodes;   // valid node ids are [0, nnodes)
+    const int32_t *counters;  // node count (valid ids are [0, counters[0]))
+    int32_t nrec;     // records of the batch
     float pm;         // PARTICLE_MASS (float)
     float inv_fr;     // 1 / FORCE_RES (float)
     double pm_d;
@@ -123,6 +124,7 @@ __global__ __launch_bounds__(256) void k_walk(WalkArgs a) {
     if (gid >= a.ngroups) return;
     const int lane = lane_id();
     const Group g = a.groups[gid];
+    const int32_t nnodes = uniform(a.counters[kNodeCount]);
     const int32_t root = uniform(g.root);
     const Node rn = a.nodes[root];
     const int32_t p = g.p0 + lane;
@@ -134,7 +136,7 @@ __global__ __launch_bounds__(256) void k_walk(WalkArgs a) {
     bool bad_leaf = false;
     if (live) {
         const int32_t lf = a.leaf_of[p];
-        bad_leaf = (uint32_t)lf >= (uint32_t)a.nnodes;
+        bad_leaf = (uint32_t)lf >= (uint32_t)nnodes;
         const Node *ln = a.nodes + (bad_leaf ? 0 : lf);
         lc0 = ln->com[0];
         lc1 = ln->com[1];
@@ -158,7 +160,7 @@ __global__ __launch_bounds__(256) void k_walk(WalkArgs a) {
             break;
         }
         c = uniform(c);
-        if ((uint32_t)c >= (uint32_t)a.nnodes) {
+        if ((uint32_t)c >= (uint32_t)nnodes) {
             if (lane == 0) atomicOr(a.err, kErrBadTree);
             break;
         }
@@ -241,7 +243,8 @@ __global__ __launch_bounds__(256) void k_walk(WalkArgs a) {
     if constexpr (ACC == HP_ACC_COMPENSATED) acc.fold();
     if (inseg) {
         const int32_t idx = __float_as_int(me.w);
-        a.po[idx].pe = live ? acc.value() : 0.0;
+        if ((uint32_t)idx < (uint32_t)a.nrec) a.po[idx].pe = live ? acc.value() : 0.0;
+        else atomicOr(a.err, kErrBadTree);
     }
     if constexpr (COUNT) {
         if (lane == 0) {
