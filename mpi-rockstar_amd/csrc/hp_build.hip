@@ -36,6 +36,25 @@
 
 namespace hp {
 
+__device__ __forceinline__ double d2_ref(const float *c, const float4 &p) {
+    // src/potential.c:22-29: float differences, double squares, summed in order
+    double dx = (double)(c[0] - p.x);
+    double dy = (double)(c[1] - p.y);
+    double dz = (double)(c[2] - p.z);
+    double s = dx * dx;
+    s = s + dy * dy;
+    s = s + dz * dz;
+    return s;
+}
+
+__device__ __forceinline__ void finish_dmin(Node *nd, double big, double sum, int32_t count) {
+    // src/potential.c:79-81
+    double b = sqrt(big) / 2.0;
+    float dmin = (float)(b + sqrt(b * b + sum / ((double)count * 1.0)));
+    nd->dmin = dmin;
+    nd->dmin2 = dmin * dmin;
+}
+
 // ------------------------------------------------------------------ level 0
 
 struct RootTile {
@@ -193,7 +212,7 @@ __device__ void push_big(const BuildArgs &a, int32_t id, const float *lo, const 
     for (int32_t j = 0; j < nt; j++) a.tiles[t0 + j] = make_int2(bi, j * kTile);
 }
 
-__device__ void write_node(Node *nd, const float *lo, const float *hi, int32_t first,
+__device__ __forceinline__ void write_node(Node *nd, const float *lo, const float *hi, int32_t first,
                            int32_t count, int32_t skip) {
     Node n;
     for (int k = 0; k < 3; k++) {
@@ -309,162 +328,299 @@ __device__ __forceinline__ int32_t small_pushes(int32_t nl, int32_t nr, int32_t 
     return (nl > kLeafPoints && nl <= thr ? 1 : 0) + (nr > kLeafPoints && nr <= thr ? 1 : 0);
 }
 
-__device__ __forceinline__ void split_small_task(const float4 *__restrict__ src,
-                                                 float4 *__restrict__ dst,
-                                                 const Task *__restrict__ tasks, int32_t wid,
-                                                 int32_t ntasks, const BuildArgs &a);
+// ------------------------------------------------------------- subtrees
+//
+// Every node of at most kSmallMax points is finished by one workgroup: its
+// points are staged in LDS (two buffers, ping-pong per level), the subtree
+// is split level by level (the wavefronts share the nodes of a level),
+// leaves go straight to tree order, and the centres of mass (bottom-up by
+// level) and dmin (over the final order) of every subtree node are computed
+// before the workgroup moves on.  Node ids come from a reservation made
+// once per subtree (refilled if a subtree needs more); unused reserved ids
+// are written as empty nodes.
 
-// One wavefront per task of at most kSmallMax points, held in registers:
-// one read per point, a count pass and a stable scatter pass over ballots.
-constexpr int kSplitThreads = 1024;  // 16 wavefronts share one allocation
+constexpr int kSubThreads = 256;
+constexpr int kSubWaves = kSubThreads / kWave;
+constexpr int kSubList = 2 * (kSmallMax / (kLeafPoints + 1)) + 2;  // entries of one level
 
-__global__ __launch_bounds__(kSplitThreads) void k_split_small(const float4 *__restrict__ src,
-                                                               float4 *__restrict__ dst,
-                                                               const Task *__restrict__ tasks,
-                                                               BuildArgs a) {
-    const int32_t ntasks = a.counters[kCurSmall];
-    constexpr int kWaves = kSplitThreads / kWave;
-    // the workgroup walks the task list kWaves tasks at a time; every
-    // wavefront takes part in every round (the allocation is per round)
-    for (int32_t base = blockIdx.x * kWaves; base < ntasks; base += gridDim.x * kWaves)
-        split_small_task(src, dst, tasks, base + (int32_t)(threadIdx.x / kWave), ntasks, a);
+struct SubNode {
+    int32_t id;
+    uint16_t lf;     // first point, relative to the subtree
+    uint16_t n;
+    uint16_t nl;
+    uint8_t kind;    // 0 leaf, 1 split
+    uint8_t dim;     // split plane of the node (from its tight box)
+    float lim;
+    int32_t child;   // first child id (splits)
+    int32_t skip;    // the node's skip pointer
+};
+
+__device__ __forceinline__ void write_empty_node(const BuildArgs &a, int32_t id) {
+    const float z[3] = {0.0f, 0.0f, 0.0f};
+    write_node(a.nodes + id, z, z, 0, 0, -1);
+    a.parent[id] = -1;
 }
 
-// One round of k_split_small: wavefront w of the workgroup splits task wid
-// (if wid < ntasks).  Node ids and small-task slots are allocated once per
-// workgroup: the wavefronts' demands are summed in LDS and thread 0 makes a
-// single atomic request.
-__device__ __forceinline__ void split_small_task(const float4 *__restrict__ src,
-                                                 float4 *__restrict__ dst,
-                                                 const Task *__restrict__ tasks, int32_t wid,
-                                                 int32_t ntasks, const BuildArgs &a) {
-    constexpr int kWaves = kSplitThreads / kWave;
-    __shared__ uint32_t s_nodes[kWaves], s_push[kWaves];
-    __shared__ unsigned long long s_base;
-    const int lane = lane_id();
-    const int w = (int)(threadIdx.x / kWave);
-    const bool has = wid < ntasks;
-    Task t;
-    Node nd;
-    if (has) {
-        t = tasks[wid];
-        nd = a.nodes[t.node];
-    } else {
-        t.node = -1;
-        t.dim = 2;
-        t.lim = 0.0f;
-        t.depth = 0;
-        nd.first = 0;
-        nd.count = 0;
-        nd.skip = -1;
-    }
-    const int32_t first = nd.first, count = nd.count;
-    const int dim = t.dim;
-    const float lim = t.lim;
-    const int nch = (count + kWave - 1) / kWave;  // <= kSmallChunks
+__device__ __forceinline__ int32_t even_up(int32_t x) { return (x + 1) & ~1; }
 
-    float4 pts[kSmallChunks];
-#pragma unroll
-    for (int c = 0; c < kSmallChunks; c++) {
-        const int32_t i = c * kWave + lane;
-        pts[c] = (c < nch && i < count) ? src[first + i] : make_float4(0.0f, 0.0f, 0.0f, 0.0f);
-    }
-    // pass 1: left count and both children's boxes
-    float lo[2][3], hi[2][3];
-    for (int s = 0; s < 2; s++)
-        for (int k = 0; k < 3; k++) {
-            lo[s][k] = __int_as_float(0x7f800000);
-            hi[s][k] = __int_as_float((int)0xff800000);
-        }
-    int32_t nl = 0;
-#pragma unroll
-    for (int c = 0; c < kSmallChunks; c++) {
-        if (c < nch) {
-            const int32_t i = c * kWave + lane;
-            const float4 p = pts[c];
-            const bool v = i < count;
-            const bool f = v && comp(p, dim) <= lim;
-            nl += __popcll(ballot(f));
-            if (v) {
-                const int s = f ? 0 : 1;
-                lo[s][0] = fminf(lo[s][0], p.x); hi[s][0] = fmaxf(hi[s][0], p.x);
-                lo[s][1] = fminf(lo[s][1], p.y); hi[s][1] = fmaxf(hi[s][1], p.y);
-                lo[s][2] = fminf(lo[s][2], p.z); hi[s][2] = fmaxf(hi[s][2], p.z);
+__device__ __forceinline__ SubNode sub_entry(int32_t id, int lf, int n, const float *lo,
+                                             const float *hi, int32_t skip) {
+    SubNode e;
+    e.id = id;
+    e.lf = (uint16_t)lf;
+    e.n = (uint16_t)n;
+    e.nl = 0;
+    e.kind = 0;
+    const int d = split_dim(lo, hi);
+    e.dim = (uint8_t)d;
+    e.lim = split_lim(lo, hi, d);
+    e.child = -1;
+    e.skip = skip;
+    return e;
+}
+
+__global__ __launch_bounds__(kSubThreads) void k_subtree(const float4 *__restrict__ A,
+                                                         const float4 *__restrict__ B,
+                                                         const Task *__restrict__ tasks,
+                                                         BuildArgs a, double particle_mass) {
+    __shared__ float4 P[2][kSmallMax];
+    __shared__ SubNode E[2][kSubList];
+    __shared__ int32_t made[2 * kSmallMax];
+    __shared__ uint16_t lvl_end[kSmallMax + 1];
+    __shared__ int32_t s_n[2], s_made, s_levels, s_pool, s_pool_end, s_base;
+    const int tid = threadIdx.x, lane = lane_id(), w = tid / kWave;
+    const int32_t ntasks = a.counters[kSmallNext];
+    // from the end of the list: the largest subtrees (children of the big
+    // nodes of the last split levels) are queued last and should start first
+    for (int32_t k = blockIdx.x; k < ntasks; k += gridDim.x) {
+        const Task t = tasks[ntasks - 1 - k];
+        const Node root = a.nodes[t.node];
+        const float4 *src = (t.depth & 1) ? B : A;
+        const int32_t first = root.first, count = root.count;
+        for (int32_t i = tid; i < count; i += kSubThreads) P[0][i] = src[first + i];
+        if (tid == 0) {
+            E[0][0] = sub_entry(t.node, 0, count, root.lo, root.hi, root.skip);
+            s_n[0] = 1;
+            made[0] = t.node;
+            s_made = 1;
+            lvl_end[0] = 1;
+            s_levels = 1;
+            const int32_t R = even_up(count * 2 / 5 + 8);
+            const int32_t base = (int32_t)alloc_nodes_and_small(a, R, 0);
+            s_pool = base;
+            s_pool_end = base + R;
+            if (base + R > a.cap_nodes) {
+                atomicOr(&a.counters[kErrFlags], kErrCapacity);
+                s_n[0] = 0;  // leave this subtree unsplit
+                s_pool = s_pool_end = 0;
             }
         }
-    }
-    const bool failed = has && (nl == 0 || nl == count);  // src/fast3tree.c:805-809
-    const bool splits = has && !failed;
-    const int32_t nr = count - nl;
-    // workgroup allocation: exclusive offsets of every wavefront's demand
-    if (lane == 0) {
-        s_nodes[w] = splits ? 2u : 0u;
-        s_push[w] = splits ? (uint32_t)small_pushes(nl, nr, a.big_threshold) : 0u;
-    }
-    __syncthreads();
-    if (threadIdx.x == 0) {
-        uint32_t tn = 0, tp = 0;
-        for (int q = 0; q < kWaves; q++) {
-            tn += s_nodes[q];
-            tp += s_push[q];
+        __syncthreads();
+        int cur = 0, buf = 0;
+        while (s_n[cur] > 0) {
+            const int nn = s_n[cur];
+            // pass A: left count of every node of the level (src/fast3tree.c:751-774)
+            for (int k = w; k < nn; k += kSubWaves) {
+                const SubNode e = E[cur][k];
+                int kind = 0, nl = 0;
+                if (e.n > kLeafPoints) {
+                    for (int c = 0; c < e.n; c += kWave) {
+                        const int i = c + lane;
+                        const bool f = i < e.n && comp(P[buf][e.lf + i], e.dim) <= e.lim;
+                        nl += __popcll(ballot(f));
+                    }
+                    kind = (nl == 0 || nl == e.n) ? 0 : 1;  // src/fast3tree.c:805-809
+                }
+                if (lane == 0) {
+                    E[cur][k].kind = (uint8_t)kind;
+                    E[cur][k].nl = (uint16_t)nl;
+                }
+            }
+            __syncthreads();
+            // wavefront 0: node ids for the splits, in level order
+            if (w == 0) {
+                int32_t need = 0;
+                for (int c = 0; c < nn; c += kWave)
+                    need += 2 * __popcll(ballot(c + lane < nn && E[cur][c + lane].kind));
+                int32_t base = 0;
+                if (lane == 0) {
+                    if (s_pool + need > s_pool_end) {
+                        for (int32_t j = s_pool; j < s_pool_end; j++) write_empty_node(a, j);
+                        const int32_t R = even_up(max(need, count / 4 + 8));
+                        const int32_t nb = (int32_t)alloc_nodes_and_small(a, R, 0);
+                        s_pool = nb;
+                        s_pool_end = nb + R;
+                        if (nb + R > a.cap_nodes) {
+                            atomicOr(&a.counters[kErrFlags], kErrCapacity);
+                            s_pool = s_pool_end = 0;
+                            need = -1;
+                        }
+                    }
+                    base = s_pool;
+                }
+                base = __shfl(base, 0, 64);
+                need = __shfl(need, 0, 64);
+                int32_t done = 0;
+                for (int c = 0; c < nn; c += kWave) {
+                    const int k = c + lane;
+                    bool split = k < nn && E[cur][k].kind;
+                    if (need < 0 && k < nn) {  // out of node ids: finish as leaves
+                        E[cur][k].kind = 0;
+                        split = false;
+                    }
+                    const uint64_t m = ballot(split);
+                    if (split) E[cur][k].child = base + 2 * (done + rank_below(m));
+                    done += __popcll(m);
+                }
+                if (lane == 0) {
+                    s_base = base;
+                    if (need >= 0) s_pool = base + 2 * done;
+                    s_n[cur ^ 1] = 2 * done;
+                }
+            }
+            __syncthreads();
+            const int32_t level_base = s_base;
+            // pass B: leaves to tree order, splits to the other buffer
+            for (int k = w; k < nn; k += kSubWaves) {
+                const SubNode e = E[cur][k];
+                if (!e.kind) {
+                    double sx = 0.0, sy = 0.0, sz = 0.0;
+                    for (int i = lane; i < e.n; i += kWave) {
+                        const float4 p = P[buf][e.lf + i];
+                        a.fin[first + e.lf + i] = p;
+                        a.leaf_of[first + e.lf + i] = e.id;
+                        sx += (double)p.x;
+                        sy += (double)p.y;
+                        sz += (double)p.z;
+                    }
+                    sx = wave_sum(sx);
+                    sy = wave_sum(sy);
+                    sz = wave_sum(sz);
+                    if (lane == 0) {
+                        // src/potential.c:100-107
+                        const double n = (double)e.n;
+                        Node *me = a.nodes + e.id;
+                        me->com[0] = sx != 0.0 ? (float)(sx / n) : 0.0f;
+                        me->com[1] = sy != 0.0 ? (float)(sy / n) : 0.0f;
+                        me->com[2] = sz != 0.0 ? (float)(sz / n) : 0.0f;
+                        me->mass = (float)(particle_mass * n);
+                    }
+                    continue;
+                }
+                const int slot = e.child - level_base;
+                const float inf = __int_as_float(0x7f800000), ninf = __int_as_float((int)0xff800000);
+                float lx0 = inf, ly0 = inf, lz0 = inf, hx0 = ninf, hy0 = ninf, hz0 = ninf;
+                float lx1 = inf, ly1 = inf, lz1 = inf, hx1 = ninf, hy1 = ninf, hz1 = ninf;
+                int32_t bl = 0, br = 0;
+                for (int c = 0; c < e.n; c += kWave) {
+                    const int i = c + lane;
+                    const bool v = i < e.n;
+                    const float4 p = v ? P[buf][e.lf + i] : make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+                    const bool f = v && comp(p, e.dim) <= e.lim;
+                    const bool g = v && !f;
+                    const uint64_t mL = ballot(f), mR = ballot(g);
+                    if (v) {
+                        const int d = f ? e.lf + bl + rank_below(mL) : e.lf + e.nl + br + rank_below(mR);
+                        P[buf ^ 1][d] = p;
+                    }
+                    if (f) {
+                        lx0 = fminf(lx0, p.x); hx0 = fmaxf(hx0, p.x);
+                        ly0 = fminf(ly0, p.y); hy0 = fmaxf(hy0, p.y);
+                        lz0 = fminf(lz0, p.z); hz0 = fmaxf(hz0, p.z);
+                    }
+                    if (g) {
+                        lx1 = fminf(lx1, p.x); hx1 = fmaxf(hx1, p.x);
+                        ly1 = fminf(ly1, p.y); hy1 = fmaxf(hy1, p.y);
+                        lz1 = fminf(lz1, p.z); hz1 = fmaxf(hz1, p.z);
+                    }
+                    bl += __popcll(mL);
+                    br += __popcll(mR);
+                }
+                const float lo0[3] = {wave_min(lx0), wave_min(ly0), wave_min(lz0)};
+                const float hi0[3] = {wave_max(hx0), wave_max(hy0), wave_max(hz0)};
+                const float lo1[3] = {wave_min(lx1), wave_min(ly1), wave_min(lz1)};
+                const float hi1[3] = {wave_max(hx1), wave_max(hy1), wave_max(hz1)};
+                if (lane == 0) {
+                    const int32_t C = e.child;
+                    write_node(a.nodes + C, lo0, hi0, first + e.lf, e.nl, C + 1);
+                    write_node(a.nodes + C + 1, lo1, hi1, first + e.lf + e.nl, e.n - e.nl, e.skip);
+                    a.parent[C] = a.parent[C + 1] = e.id;
+                    a.nodes[e.id].left = C;
+                    E[cur ^ 1][slot] = sub_entry(C, e.lf, e.nl, lo0, hi0, C + 1);
+                    E[cur ^ 1][slot + 1] = sub_entry(C + 1, e.lf + e.nl, e.n - e.nl, lo1, hi1, e.skip);
+                    made[s_made + slot] = C;
+                    made[s_made + slot + 1] = C + 1;
+                }
+            }
+            __syncthreads();
+            if (tid == 0) {
+                s_made += s_n[cur ^ 1];
+                lvl_end[s_levels++] = (uint16_t)s_made;
+            }
+            cur ^= 1;
+            buf ^= 1;
+            __syncthreads();
         }
-        s_base = (tn | tp) ? alloc_nodes_and_small(a, (int32_t)tn, (int32_t)tp) : 0ull;
-    }
-    __syncthreads();
-    uint32_t on = (uint32_t)s_base, op = (uint32_t)(s_base >> 32);
-    for (int q = 0; q < w; q++) {
-        on += s_nodes[q];
-        op += s_push[q];
-    }
-    __syncthreads();  // s_* are reused by the next round
-    if (failed) {  // a leaf
-#pragma unroll
-        for (int c = 0; c < kSmallChunks; c++) {
-            const int32_t i = c * kWave + lane;
-            if (c < nch && i < count) {
-                a.fin[first + i] = pts[c];
-                a.leaf_of[first + i] = t.node;
+        // unused reserved ids become empty nodes
+        for (int32_t j = s_pool + tid; j < s_pool_end; j += kSubThreads) write_empty_node(a, j);
+        __threadfence_block();
+        __syncthreads();
+        // centres of mass of the internal nodes, deepest level first
+        // (src/potential.c:115-122: float, right term first)
+        for (int lv = s_levels - 1; lv >= 0; lv--) {
+            const int32_t b0 = lv ? lvl_end[lv - 1] : 0, b1 = lvl_end[lv];
+            for (int32_t j = b0 + tid; j < b1; j += kSubThreads) {
+                Node *me = a.nodes + made[j];
+                const int32_t L = me->left;
+                if (L < 0) continue;
+                const Node *l = a.nodes + L, *r = a.nodes + L + 1;
+                const float ml = l->mass, mr = r->mass;
+                const float m = ml + mr;
+                for (int k = 0; k < 3; k++) {
+                    const float rr = r->com[k] * mr;
+                    const float ll = l->com[k] * ml;
+                    me->com[k] = m != 0.0f ? (rr + ll) / m : 0.0f;
+                }
+                me->mass = m;
+            }
+            __threadfence_block();
+            __syncthreads();
+        }
+        // dmin over the final order (src/potential.c:71-82)
+        for (int32_t i = tid; i < count; i += kSubThreads) P[0][i] = a.fin[first + i];
+        __syncthreads();
+        // each wavefront takes 64 nodes at a time: their records are loaded
+        // lane-parallel, then walked one by one from registers
+        for (int32_t j0 = w * kWave; j0 < s_made; j0 += kSubWaves * kWave) {
+            const int32_t jj = j0 + lane;
+            int32_t my_rel = 0, my_cnt = 0;
+            float my_c0 = 0.0f, my_c1 = 0.0f, my_c2 = 0.0f;
+            if (jj < s_made) {
+                const Node *np = a.nodes + made[jj];
+                my_rel = np->first - first;
+                my_cnt = np->count;
+                my_c0 = np->com[0];
+                my_c1 = np->com[1];
+                my_c2 = np->com[2];
+            }
+            const int nb = min(kWave, s_made - j0);
+            for (int q = 0; q < nb; q++) {
+                const int32_t rel = __shfl(my_rel, q, 64), cnt = __shfl(my_cnt, q, 64);
+                const float com[3] = {__shfl(my_c0, q, 64), __shfl(my_c1, q, 64),
+                                      __shfl(my_c2, q, 64)};
+                double big = 0.0, sum = 0.0;
+                for (int32_t i = lane; i < cnt; i += kWave) {
+                    const double d = d2_ref(com, P[0][rel + i]);
+                    big = d > big ? d : big;
+                    sum += d;
+                }
+                big = wave_max(big);
+                sum = wave_sum(sum);
+                if (lane == 0) finish_dmin(a.nodes + made[j0 + q], big, sum, cnt);
             }
         }
-        if (lane == 0) note_fat_leaf(a, t.node, count);
-        return;
+        __syncthreads();
     }
-    if (!splits) return;
-    const int32_t L = (int32_t)on, P = (int32_t)op;
-    if (L + 2 > a.cap_nodes) {
-        if (lane == 0) atomicOr(&a.counters[kErrFlags], kErrCapacity);
-        return;
-    }
-    const bool leafL = nl <= kLeafPoints, leafR = nr <= kLeafPoints;
-    // pass 2: stable scatter
-    int32_t bl = 0, br = 0;
-#pragma unroll
-    for (int c = 0; c < kSmallChunks; c++) {
-        if (c >= nch) continue;
-        const int32_t i = c * kWave + lane;
-        const bool v = i < count;
-        const float4 p = pts[c];
-        const bool f = v && comp(p, dim) <= lim;
-        const bool g = v && !f;
-        const uint64_t mL = ballot(f), mR = ballot(g);
-        if (v) {
-            const int32_t d = f ? first + bl + rank_below(mL) : first + nl + br + rank_below(mR);
-            if (f ? leafL : leafR) {
-                a.fin[d] = p;
-                a.leaf_of[d] = L + (f ? 0 : 1);
-            } else {
-                dst[d] = p;
-            }
-        }
-        bl += __popcll(mL);
-        br += __popcll(mR);
-    }
-    for (int s = 0; s < 2; s++)
-        for (int k = 0; k < 3; k++) {
-            lo[s][k] = wave_min(lo[s][k]);
-            hi[s][k] = wave_max(hi[s][k]);
-        }
-    if (lane == 0) make_children(a, t, nd, L, P, nl, lo[0], hi[0], lo[1], hi[1]);
 }
 
 // -------------------------------------------------------------- big tasks
@@ -547,59 +703,69 @@ __device__ void big_count_tile(const float4 *__restrict__ src, const int2 *__res
 }
 
 // One thread per big task of the level: scan of the tile counts, children.
-__device__ void big_split_task(const Task *__restrict__ big, TilePart *__restrict__ parts,
-                               BigSplit *__restrict__ res, const BuildArgs &a, int32_t bi);
-
-__global__ void k_big_split(const Task *__restrict__ big, TilePart *__restrict__ parts,
-                            BigSplit *__restrict__ res, BuildArgs a) {
+// One wavefront per big task of the level: exclusive scan of the tiles'
+// left counts (lanes over tiles), the children's boxes, then the split.
+__global__ __launch_bounds__(256) void k_big_split(const Task *__restrict__ big,
+                                                   TilePart *__restrict__ parts,
+                                                   BigSplit *__restrict__ res, BuildArgs a) {
     const int32_t b1 = a.counters[kBigEnd];
-    for (int32_t bi = a.counters[kBigBegin] + blockIdx.x * blockDim.x + threadIdx.x; bi < b1;
-         bi += gridDim.x * blockDim.x)
-        big_split_task(big, parts, res, a, bi);
-}
-
-__device__ void big_split_task(const Task *__restrict__ big, TilePart *__restrict__ parts,
-                               BigSplit *__restrict__ res, const BuildArgs &a, int32_t bi) {
-    const Task t = big[bi];
-    const Node nd = a.nodes[t.node];
-    int32_t nl = 0;
-    float lo[2][3], hi[2][3];
-    for (int s = 0; s < 2; s++)
-        for (int d = 0; d < 3; d++) {
-            lo[s][d] = __int_as_float(0x7f800000);
-            hi[s][d] = __int_as_float((int)0xff800000);
+    const int32_t nw = (int32_t)(gridDim.x * blockDim.x / kWave);
+    const int lane = lane_id();
+    for (int32_t bi = a.counters[kBigBegin] + (int32_t)((blockIdx.x * blockDim.x + threadIdx.x) / kWave);
+         bi < b1; bi += nw) {
+        const Task t = big[bi];
+        const Node nd = a.nodes[t.node];
+        const float inf = __int_as_float(0x7f800000), ninf = __int_as_float((int)0xff800000);
+        float lo[2][3] = {{inf, inf, inf}, {inf, inf, inf}};
+        float hi[2][3] = {{ninf, ninf, ninf}, {ninf, ninf, ninf}};
+        int32_t nl = 0;
+        for (int32_t j0 = 0; j0 < t.ntiles; j0 += kWave) {
+            const int32_t jj = j0 + lane;
+            const bool v = jj < t.ntiles;
+            TilePart *p = parts + t.tile0 + (v ? jj : 0);
+            const int32_t c = v ? p->nleft : 0;
+            int32_t incl = c;
+            for (int o = 1; o < kWave; o <<= 1) {
+                const int32_t u = __shfl_up(incl, o, 64);
+                if (lane >= o) incl += u;
+            }
+            if (v) {
+                p->pad[0] = nl + incl - c;  // left points in the task's earlier tiles
+                for (int s = 0; s < 2; s++)
+                    for (int d = 0; d < 3; d++) {
+                        lo[s][d] = fminf(lo[s][d], p->lo[s][d]);
+                        hi[s][d] = fmaxf(hi[s][d], p->hi[s][d]);
+                    }
+            }
+            nl += __shfl(incl, kWave - 1, 64);
         }
-    for (int32_t j = 0; j < t.ntiles; j++) {
-        TilePart &p = parts[t.tile0 + j];
-        int32_t c = p.nleft;
-        p.pad[0] = nl;  // left points in the task's earlier tiles
-        nl += c;
         for (int s = 0; s < 2; s++)
             for (int d = 0; d < 3; d++) {
-                lo[s][d] = fminf(lo[s][d], p.lo[s][d]);
-                hi[s][d] = fmaxf(hi[s][d], p.hi[s][d]);
+                lo[s][d] = wave_min(lo[s][d]);
+                hi[s][d] = wave_max(hi[s][d]);
             }
-    }
-    BigSplit r;
-    r.nleft = nl;
-    r.left = -1;
-    r.leafL = r.leafR = 0;
-    if (nl == 0 || nl == nd.count) {
-        note_fat_leaf(a, t.node, nd.count);
-    } else {
-        const int32_t nr = nd.count - nl;
-        const uint64_t al = alloc_nodes_and_small(a, 2, small_pushes(nl, nr, a.big_threshold));
-        const int32_t L = (int32_t)al, P = (int32_t)(al >> 32);
-        if (L + 2 > a.cap_nodes) {
-            atomicOr(&a.counters[kErrFlags], kErrCapacity);
+        if (lane != 0) continue;
+        BigSplit r;
+        r.nleft = nl;
+        r.left = -1;
+        r.leafL = r.leafR = 0;
+        if (nl == 0 || nl == nd.count) {
+            note_fat_leaf(a, t.node, nd.count);
         } else {
-            r.left = L;
-            r.leafL = nl <= kLeafPoints;
-            r.leafR = nr <= kLeafPoints;
-            make_children(a, t, nd, L, P, nl, lo[0], hi[0], lo[1], hi[1]);
+            const int32_t nr = nd.count - nl;
+            const uint64_t al = alloc_nodes_and_small(a, 2, small_pushes(nl, nr, a.big_threshold));
+            const int32_t L = (int32_t)al, P = (int32_t)(al >> 32);
+            if (L + 2 > a.cap_nodes) {
+                atomicOr(&a.counters[kErrFlags], kErrCapacity);
+            } else {
+                r.left = L;
+                r.leafL = nl <= kLeafPoints;
+                r.leafR = nr <= kLeafPoints;
+                make_children(a, t, nd, L, P, nl, lo[0], hi[0], lo[1], hi[1]);
+            }
         }
+        res[bi] = r;
     }
-    res[bi] = r;
 }
 
 // Scatter pass over one tile of a big task: stable ranks in rounds of 256.
@@ -691,9 +857,7 @@ __device__ void big_scatter_tile(const float4 *__restrict__ src, float4 *__restr
 // (node ids grow with depth).
 __global__ void k_level_advance(int32_t *__restrict__ c, int32_t *__restrict__ depth_end) {
     if (threadIdx.x != 0 || blockIdx.x != 0) return;
-    const int32_t ns = c[kSmallNext];
-    c[kCurSmall] = ns;
-    c[kSmallNext] = 0;
+    c[kGlobalNodes] = c[kNodeCount];
     c[kBigBegin] = c[kBigEnd];
     c[kBigEnd] = c[kBigNext];
     c[kTileBegin] = c[kTileEnd];
@@ -702,7 +866,7 @@ __global__ void k_level_advance(int32_t *__restrict__ c, int32_t *__restrict__ d
     if (lvl < kMaxLevels) depth_end[lvl] = c[kNodeCount];
     else c[kErrFlags] |= kErrCapacity;
     c[kLevel] = lvl + 1;
-    c[kActive] = (ns > 0 || c[kBigEnd] > c[kBigBegin]) ? 1 : 0;
+    c[kActive] = c[kBigEnd] > c[kBigBegin] ? 1 : 0;
 }
 
 // --------------------------------------------------------------- multipoles
@@ -711,15 +875,17 @@ __global__ void k_level_advance(int32_t *__restrict__ c, int32_t *__restrict__ d
 // count, fat-leaf count, per-depth node ranges, big tasks and tiles), so a
 // whole build can be replayed from a graph.
 
-// Leaf mass and centre (src/potential.c:100-107): f64 sums of the float
-// positions, one thread per leaf of at most kFatLeaf points.
+// Leaves made by the split levels (at most 10 points): mass, centre
+// (src/potential.c:100-107, f64 sums) and dmin (src/potential.c:71-82), one
+// thread each.  Subtree nodes have ids at or above counters[kGlobalNodes]
+// and are finished by their workgroup.
 __global__ void k_leaf_com(Node *__restrict__ nodes, const float4 *__restrict__ fin,
                            const int32_t *__restrict__ counters, double particle_mass) {
-    const int32_t nnodes = counters[kNodeCount];
+    const int32_t nnodes = counters[kGlobalNodes];
     for (int32_t id = blockIdx.x * blockDim.x + threadIdx.x; id < nnodes;
          id += gridDim.x * blockDim.x) {
         const Node nd = nodes[id];
-        if (nd.left >= 0 || nd.count <= 0 || nd.count > kFatLeaf) continue;
+        if (nd.left >= 0 || nd.count <= 0 || nd.count > kLeafPoints) continue;
         double s0 = 0.0, s1 = 0.0, s2 = 0.0;
         for (int32_t i = 0; i < nd.count; i++) {
             const float4 p = fin[nd.first + i];
@@ -728,11 +894,22 @@ __global__ void k_leaf_com(Node *__restrict__ nodes, const float4 *__restrict__ 
             s2 += (double)p.z;
         }
         const double n = (double)nd.count;
+        float com[3];
+        com[0] = s0 != 0.0 ? (float)(s0 / n) : 0.0f;
+        com[1] = s1 != 0.0 ? (float)(s1 / n) : 0.0f;
+        com[2] = s2 != 0.0 ? (float)(s2 / n) : 0.0f;
+        double big = 0.0, sum = 0.0;
+        for (int32_t i = 0; i < nd.count; i++) {
+            const double d = d2_ref(com, fin[nd.first + i]);
+            big = d > big ? d : big;
+            sum += d;
+        }
         Node *me = nodes + id;
-        me->com[0] = s0 != 0.0 ? (float)(s0 / n) : 0.0f;
-        me->com[1] = s1 != 0.0 ? (float)(s1 / n) : 0.0f;
-        me->com[2] = s2 != 0.0 ? (float)(s2 / n) : 0.0f;
+        me->com[0] = com[0];
+        me->com[1] = com[1];
+        me->com[2] = com[2];
         me->mass = (float)(particle_mass * n);
+        finish_dmin(me, big, sum, nd.count);
     }
 }
 
@@ -789,49 +966,6 @@ __global__ void k_internal_com(Node *__restrict__ nodes, const int32_t *__restri
             me->com[k] = m != 0.0f ? (rr + ll) / m : 0.0f;
         }
         me->mass = m;
-    }
-}
-
-__device__ __forceinline__ double d2_ref(const float *c, const float4 &p) {
-    // src/potential.c:22-29: float differences, double squares, summed in order
-    double dx = (double)(c[0] - p.x);
-    double dy = (double)(c[1] - p.y);
-    double dz = (double)(c[2] - p.z);
-    double s = dx * dx;
-    s = s + dy * dy;
-    s = s + dz * dz;
-    return s;
-}
-
-__device__ __forceinline__ void finish_dmin(Node *nd, double big, double sum, int32_t count) {
-    // src/potential.c:79-81
-    double b = sqrt(big) / 2.0;
-    float dmin = (float)(b + sqrt(b * b + sum / ((double)count * 1.0)));
-    nd->dmin = dmin;
-    nd->dmin2 = dmin * dmin;
-}
-
-// One wavefront per node of at most big_threshold points.
-__global__ __launch_bounds__(256) void k_dmin_small(Node *__restrict__ nodes,
-                                                    const float4 *__restrict__ fin,
-                                                    const int32_t *__restrict__ counters,
-                                                    int32_t big_threshold) {
-    const int32_t nnodes = counters[kNodeCount];
-    const int32_t nw = (int32_t)(gridDim.x * blockDim.x / kWave);
-    const int lane = lane_id();
-    for (int32_t id = (int32_t)((blockIdx.x * blockDim.x + threadIdx.x) / kWave); id < nnodes;
-         id += nw) {
-        const Node nd = nodes[id];
-        if (nd.count <= 0 || nd.count > big_threshold) continue;
-        double big = 0.0, sum = 0.0;
-        for (int32_t i = lane; i < nd.count; i += kWave) {
-            const double d = d2_ref(nd.com, fin[nd.first + i]);
-            big = d > big ? d : big;
-            sum += d;
-        }
-        big = wave_max(big);
-        sum = wave_sum(sum);
-        if (lane == 0) finish_dmin(nodes + id, big, sum, nd.count);
     }
 }
 

@@ -19,7 +19,7 @@ constexpr int kDegenerate = 20;   // 2 * POINTS_PER_LEAF, src/potential.c:145
 constexpr int kWave = 64;
 constexpr int kGroup = 64;        // sink particles per walk wavefront
 constexpr int kTile = 1024;       // points per workgroup tile of a big node
-constexpr int kSmallMax = 1024;   // largest node split by one wavefront
+constexpr int kSmallMax = 1024;   // largest subtree finished by one workgroup
 constexpr int kSmallChunks = kSmallMax / 64;
 constexpr int kFatLeaf = 64;      // leaves above this size get a wavefront
 constexpr int kTileThreads = 256;
@@ -83,7 +83,7 @@ struct Group {
 // Device-side counters of one build.
 enum Counter {
     kNodeCount = 0,
-    kSmallNext = 1,   // small tasks pushed for the next level
+    kSmallNext = 1,   // subtree tasks (<= kSmallMax points), append-only
     kBigNext = 2,     // big tasks pushed for the next level (append-only)
     kTileNext = 3,    // tiles of big tasks (append-only)
     kMaxDepth = 4,
@@ -92,13 +92,13 @@ enum Counter {
     kBigDmin = 7,     // nodes whose dmin is reduced by tiles
     kFatCount = 8,    // leaves of more than kFatLeaf points (failed splits)
     // level state, advanced on the device between levels
-    kCurSmall = 9,    // small tasks of the current level
+    kGlobalNodes = 9, // nodes made by the split levels (subtree nodes follow)
     kBigBegin = 10,   // big tasks of the current level: [kBigBegin, kBigEnd)
     kBigEnd = 11,
     kTileBegin = 12,  // their tiles: [kTileBegin, kTileEnd)
     kTileEnd = 13,
     kLevel = 14,      // levels advanced so far
-    kActive = 15,     // the current level has work
+    kActive = 15,     // the current level has big tasks
     kNumCounters = 16
 };
 
@@ -155,7 +155,7 @@ __device__ __forceinline__ bool finite3(float x, float y, float z) {
 
 // Split dimension of a tight box: z first, x or y only when strictly wider
 // (src/fast3tree.c:736-747).
-__host__ __device__ inline int split_dim(const float *lo, const float *hi) {
+__host__ __device__ __forceinline__ int split_dim(const float *lo, const float *hi) {
     int dim = 2;
     float w = hi[2] - lo[2];
     for (int k = 0; k < 2; k++) {
@@ -169,8 +169,11 @@ __host__ __device__ inline int split_dim(const float *lo, const float *hi) {
 }
 
 // Midpoint in float, halved exactly (src/fast3tree.c:757).
-__host__ __device__ inline float split_lim(const float *lo, const float *hi, int dim) {
-    return 0.5f * (hi[dim] + lo[dim]);
+__host__ __device__ __forceinline__ float split_lim(const float *lo, const float *hi, int dim) {
+    // selects instead of hi[dim]: a runtime index would send the box to scratch
+    const float h = dim == 0 ? hi[0] : (dim == 1 ? hi[1] : hi[2]);
+    const float l = dim == 0 ? lo[0] : (dim == 1 ? lo[1] : lo[2]);
+    return 0.5f * (h + l);
 }
 
 // Order-preserving map of a float onto an int, for atomic min/max.

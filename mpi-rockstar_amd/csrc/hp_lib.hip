@@ -227,7 +227,7 @@ struct Engine {
     DevBuf hfirst, hcount, root_tiles, groups, direct_blocks, rec_call, scan_tiles, call_tile0,
         call_ntiles;
     // build
-    DevBuf box, bad, A, B, F, leaf_of, nodes, parent, fat, small0, small1, big, tiles, parts,
+    DevBuf box, bad, A, B, F, leaf_of, nodes, parent, fat, small0, big, tiles, parts,
         splitres, dparts, counters, work, depth_buf;
     // unbind
     DevBuf keys0, keys1, vals0, vals1, hist, hsum, srt, callinfo, tsum, tcand, tile_i32;
@@ -263,7 +263,7 @@ struct Engine {
     ~Engine() {
         DevBuf *all[] = {&hfirst, &hcount, &root_tiles, &groups, &direct_blocks, &rec_call,
                          &scan_tiles, &call_tile0, &call_ntiles, &box, &bad, &A, &B, &F,
-                         &leaf_of, &nodes, &parent, &fat, &small0, &small1, &big, &tiles,
+                         &leaf_of, &nodes, &parent, &fat, &small0, &big, &tiles,
                          &parts, &splitres, &dparts, &counters, &work, &depth_buf, &keys0, &keys1, &vals0,
                          &vals1, &hist, &hsum, &srt, &callinfo, &tsum, &tcand, &tile_i32};
         if (stream) {
@@ -295,14 +295,15 @@ struct Engine {
         HP_ENSURE(B, sizeof(float4) * (N + 1));
         HP_ENSURE(F, sizeof(float4) * (N + 1));
         HP_ENSURE(leaf_of, sizeof(int32_t) * (N + 1));
-        cap_nodes = 2 * N + H + 2;
+        // split levels use at most 2N + H ids; subtrees reserve ids in blocks
+        // (unused ones become empty nodes), at most ~0.7N + 16 per task more
+        cap_nodes = 5 * N + H + 1024;
         HP_ENSURE(nodes, sizeof(Node) * cap_nodes);
         HP_ENSURE(parent, sizeof(int32_t) * cap_nodes);
         cap_fat = N / (kFatLeaf + 1) + 16;
         HP_ENSURE(fat, sizeof(int32_t) * cap_fat);
         cap_small = N / (kLeafPoints + 1) + H + 64;
         HP_ENSURE(small0, sizeof(Task) * cap_small);
-        HP_ENSURE(small1, sizeof(Task) * cap_small);
         const int64_t bt = big_threshold > 0 ? big_threshold : 1;
         cap_big = 128 * (N / bt + 1) + H + 64;
         cap_tiles = 128 * (N / kTile + 2) + cap_big;
@@ -408,27 +409,25 @@ struct Engine {
         return HP_OK;
     }
 
-    // split levels [first, first + n); level L reads its small tasks from
-    // list (L - 1) & 1 and the points from buffer A (odd L) or B (even L)
+    // split levels [first, first + n) of the nodes of more than big_threshold
+    // points; level L reads the points from buffer A (odd L) or B (even L)
+    // and queues children of at most big_threshold points as subtree tasks
     int enqueue_levels(int first, int n) {
         hipStream_t s = stream;
-        Task *lists[2] = {small0.as<Task>(), small1.as<Task>()};
         for (int level = first; level < first + n; level++) {
             float4 *src = (level & 1) ? A.as<float4>() : B.as<float4>();
             float4 *dst = (level & 1) ? B.as<float4>() : A.as<float4>();
-            const BuildArgs ba = build_args(lists[level & 1]);
+            const BuildArgs ba = build_args(small0.as<Task>());
             k_big_count<<<1024, kTileThreads, 0, s>>>(src, tiles.as<int2>(), big.as<Task>(),
                                                       nodes.as<Node>(), parts.as<TilePart>(),
                                                       counters.as<int32_t>());
             HP_LAUNCHED();
-            k_big_split<<<64, 64, 0, s>>>(big.as<Task>(), parts.as<TilePart>(),
+            k_big_split<<<64, 256, 0, s>>>(big.as<Task>(), parts.as<TilePart>(),
                                           splitres.as<BigSplit>(), ba);
             HP_LAUNCHED();
             k_big_scatter<<<1024, kTileThreads, 0, s>>>(src, dst, tiles.as<int2>(), big.as<Task>(),
                                                         parts.as<TilePart>(),
                                                         splitres.as<BigSplit>(), ba);
-            HP_LAUNCHED();
-            k_split_small<<<1024, kSplitThreads, 0, s>>>(src, dst, lists[(level - 1) & 1], ba);
             HP_LAUNCHED();
             k_level_advance<<<1, 1, 0, s>>>(counters.as<int32_t>(), depth_buf.as<int32_t>());
             HP_LAUNCHED();
@@ -436,10 +435,13 @@ struct Engine {
         return HP_OK;
     }
 
-    // multipoles of a tree built in `levels` split levels
+    // subtrees, then the multipoles of the nodes of `levels` split levels
     int enqueue_multipole(int levels) {
         hipStream_t s = stream;
         const int32_t *cnt = counters.as<int32_t>();
+        k_subtree<<<2048, kSubThreads, 0, s>>>(A.as<float4>(), B.as<float4>(), small0.as<Task>(),
+                                               build_args(small0.as<Task>()), PARTICLE_MASS);
+        HP_LAUNCHED();
         k_leaf_com<<<2048, 256, 0, s>>>(nodes.as<Node>(), F.as<float4>(), cnt, PARTICLE_MASS);
         HP_LAUNCHED();
         k_fat_leaf_com<<<256, 256, 0, s>>>(nodes.as<Node>(), fat.as<int32_t>(), cnt,
@@ -449,8 +451,6 @@ struct Engine {
             k_internal_com<<<512, 256, 0, s>>>(nodes.as<Node>(), depth_buf.as<int32_t>(), d);
             HP_LAUNCHED();
         }
-        k_dmin_small<<<4096, 256, 0, s>>>(nodes.as<Node>(), F.as<float4>(), cnt, big_threshold);
-        HP_LAUNCHED();
         k_dmin_tiles<<<1024, kTileThreads, 0, s>>>(nodes.as<Node>(), F.as<float4>(),
                                                    tiles.as<int2>(), big.as<Task>(), cnt,
                                                    dparts.as<double2>());
@@ -471,6 +471,7 @@ struct Engine {
             a.ngroups = (int32_t)T.groups.size();
             a.counters = counters.as<int32_t>();
             a.nrec = (int32_t)T.nrec;
+            a.cap_nodes = (int32_t)cap_nodes;
             a.pm = (float)PARTICLE_MASS;
             a.inv_fr = (float)(1.0 / FORCE_RES);
             a.pm_d = PARTICLE_MASS;
@@ -561,6 +562,7 @@ struct Engine {
         if (h_counters[kErrFlags])
             return set_error(HP_EINVAL, "walk stopped early (flags %d): corrupt tree",
                              h_counters[kErrFlags]);
+        st.nodes = h_counters[kNodeCount];  // subtree ids included (some unused)
         return HP_OK;
     }
 
@@ -1093,9 +1095,12 @@ __attribute__((visibility("default"))) int64_t hp_debug_tree(const struct potent
     e->st.halos = 1;
     int next = 0;
     if ((r = e->enqueue_roots(T, dpo.as<struct potential>())) || (r = e->finish_levels(1, &next)) ||
-        (r = e->check_build()) || (r = e->enqueue_multipole((int)e->st.levels)))
+        (r = e->check_build()) || (r = e->enqueue_multipole((int)e->st.levels)) ||
+        (r = e->read_counters()))
         return r;
-    const int64_t nn = e->st.nodes;
+    if (e->h_counters[kErrFlags])
+        return set_error(HP_EINVAL, "tree build failed (flags %d)", e->h_counters[kErrFlags]);
+    const int64_t nn = e->h_counters[kNodeCount];
     std::vector<Node> nodes(nn);
     std::vector<int32_t> par(nn);
     std::vector<float4> fin(n);

@@ -109,6 +109,7 @@ struct WalkArgs {
     int32_t ngroups;
     const int32_t *counters;  // node count (valid ids are [0, counters[0]))
     int32_t nrec;     // records of the batch
+    int32_t cap_nodes;
     float pm;         // PARTICLE_MASS (float)
     float inv_fr;     // 1 / FORCE_RES (float)
     double pm_d;
@@ -124,7 +125,7 @@ __global__ __launch_bounds__(256) void k_walk(WalkArgs a) {
     if (gid >= a.ngroups) return;
     const int lane = lane_id();
     const Group g = a.groups[gid];
-    const int32_t nnodes = uniform(a.counters[kNodeCount]);
+    const int32_t nnodes = uniform(min(a.counters[kNodeCount], a.cap_nodes));
     const int32_t root = uniform(g.root);
     const Node rn = a.nodes[root];
     const int32_t p = g.p0 + lane;

@@ -43,7 +43,17 @@ def _compare_trees(gnodes, perm, onodes, opo):
         if G["left"] >= 0:
             stack.append((G["left"], O["left"]))
             stack.append((G["left"] + 1, O["left"] + 1))
-    assert seen == len(onodes) == len(gnodes)
+    # every oracle node is reached; the GPU array may also hold unused
+    # (empty) ids reserved by subtree workgroups
+    assert seen == len(onodes)
+    reach = np.zeros(len(gnodes), bool)
+    stack = [0]
+    while stack:
+        g = stack.pop()
+        reach[g] = True
+        if gnodes[g]["left"] >= 0:
+            stack += [gnodes[g]["left"], gnodes[g]["left"] + 1]
+    assert np.all(gnodes["count"][~reach] == 0)
     np.testing.assert_array_equal(perm, opo["flags"])
 
 
