@@ -339,6 +339,7 @@ __device__ __forceinline__ int32_t small_pushes(int32_t nl, int32_t nr, int32_t 
 // once per subtree (refilled if a subtree needs more); unused reserved ids
 // are written as empty nodes.
 
+constexpr int kWaveTask = 64;   // subtrees up to this size: one wavefront
 constexpr int kSubThreads = 256;
 constexpr int kSubWaves = kSubThreads / kWave;
 constexpr int kSubList = 2 * (kSmallMax / (kLeafPoints + 1)) + 2;  // entries of one level
@@ -389,7 +390,7 @@ __global__ __launch_bounds__(kSubThreads) void k_subtree(const float4 *__restric
     __shared__ uint16_t lvl_end[kSmallMax + 1];
     __shared__ int32_t s_n[2], s_made, s_levels, s_pool, s_pool_end, s_base;
     const int tid = threadIdx.x, lane = lane_id(), w = tid / kWave;
-    const int32_t ntasks = a.counters[kSmallNext];
+    const int32_t ntasks = min(a.counters[kSmallNext], a.cap_small);
     // from the end of the list: the largest subtrees (children of the big
     // nodes of the last split levels) are queued last and should start first
     for (int32_t k = blockIdx.x; k < ntasks; k += gridDim.x) {
@@ -397,6 +398,7 @@ __global__ __launch_bounds__(kSubThreads) void k_subtree(const float4 *__restric
         const Node root = a.nodes[t.node];
         const float4 *src = (t.depth & 1) ? B : A;
         const int32_t first = root.first, count = root.count;
+        if (count <= kWaveTask) continue;  // a wavefront subtree
         for (int32_t i = tid; i < count; i += kSubThreads) P[0][i] = src[first + i];
         if (tid == 0) {
             E[0][0] = sub_entry(t.node, 0, count, root.lo, root.hi, root.skip);
@@ -618,6 +620,237 @@ __global__ __launch_bounds__(kSubThreads) void k_subtree(const float4 *__restric
                 sum = wave_sum(sum);
                 if (lane == 0) finish_dmin(a.nodes + made[j0 + q], big, sum, cnt);
             }
+        }
+        __syncthreads();
+    }
+}
+
+// ---------------------------------------------------- wavefront subtrees
+//
+// A subtree of at most kWaveTask points is built by one wavefront, one point
+// per lane.  Each level, every lane tests its point against its node's split
+// plane; ballots give the left counts and the stable scatter positions (a
+// node's points occupy a contiguous run of lanes), and the children's tight
+// boxes come from segmented scans across the lanes.  The node table stays in
+// LDS with local ids (children in pairs, levels in order) until the subtree
+// is complete; then one atomic reserves exactly its node ids.  Centres of
+// mass and dmin are computed with a lane per node, summing the node's points
+// in tree order, the order of the reference's loops.
+
+constexpr int kWaveNodes = 2 * kWaveTask;  // a tree over n points has at most 2n - 1 nodes
+constexpr int kRootSkip = -2;              // local skip pointer: the subtree root's own skip
+
+__device__ __forceinline__ uint64_t lanes_below(int i) {
+    return i >= 64 ? ~0ull : ((1ull << i) - 1ull);
+}
+
+// bits [lf, lf + n) of a wavefront mask (lf + n <= 64)
+__device__ __forceinline__ uint64_t lane_run(int lf, int n) {
+    return (n >= 64 ? ~0ull : ((1ull << n) - 1ull)) << lf;
+}
+
+__global__ __launch_bounds__(kWave) void k_subtree_wave(const float4 *__restrict__ A,
+                                                        const float4 *__restrict__ B,
+                                                        const Task *__restrict__ tasks,
+                                                        BuildArgs a, double particle_mass) {
+    __shared__ float4 P[kWaveTask];
+    __shared__ int32_t Q[kWaveTask];
+    __shared__ float s_lo[3][kWaveNodes], s_hi[3][kWaveNodes];
+    __shared__ float s_com[3][kWaveNodes], s_mass[kWaveNodes];
+    __shared__ int16_t s_lf[kWaveNodes], s_cnt[kWaveNodes], s_left[kWaveNodes];
+    __shared__ int16_t s_par[kWaveNodes], s_skip[kWaveNodes];
+    __shared__ int16_t s_lvl[kWaveTask + 2];  // first local id of each level
+    const int lane = lane_id();
+    const int32_t ntasks = min(a.counters[kSmallNext], a.cap_small);
+    for (int32_t k = blockIdx.x; k < ntasks; k += gridDim.x) {
+        const Task t = tasks[k];
+        const Node root = a.nodes[t.node];
+        const int32_t n = root.count;
+        if (n > kWaveTask) continue;  // a workgroup subtree
+        const float4 *src = (t.depth & 1) ? B : A;
+        const int32_t first = root.first;
+        const bool v = lane < n;
+        float4 p = v ? src[first + lane] : make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+        if (lane == 0) {
+            for (int d = 0; d < 3; d++) {
+                s_lo[d][0] = root.lo[d];
+                s_hi[d][0] = root.hi[d];
+            }
+            s_lf[0] = 0;
+            s_cnt[0] = (int16_t)n;
+            s_left[0] = -1;
+            s_par[0] = -1;
+            s_skip[0] = kRootSkip;
+            s_lvl[0] = 0;
+            s_lvl[1] = 1;
+        }
+        __syncthreads();
+        int nid = 0, M = 1, nlev = 0;
+        bool pend = v && n > kLeafPoints;
+        while (ballot(pend)) {
+            int lf = 0, c = 0, dim = 0;
+            float lim = 0.0f;
+            if (pend) {
+                lf = s_lf[nid];
+                c = s_cnt[nid];
+                const float lo3[3] = {s_lo[0][nid], s_lo[1][nid], s_lo[2][nid]};
+                const float hi3[3] = {s_hi[0][nid], s_hi[1][nid], s_hi[2][nid]};
+                dim = split_dim(lo3, hi3);
+                lim = split_lim(lo3, hi3, dim);
+            }
+            // src/fast3tree.c:751-774 (partition), 805-809 (failed split)
+            const bool gl = pend && comp(p, dim) <= lim;
+            const uint64_t mL = ballot(gl);
+            const uint64_t mN = pend ? lane_run(lf, c) : 0ull;
+            const int nl = __popcll(mL & mN);
+            const bool split = pend && nl > 0 && nl < c;
+            const uint64_t mH = ballot(split && lane == lf);
+            const int C = M + 2 * __popcll(mH & lanes_below(lf));
+            int dst = lane, nn = nid;
+            if (split) {
+                const uint64_t below = mN & lanes_below(lane);
+                dst = gl ? lf + __popcll(mL & below) : lf + nl + __popcll(~mL & below);
+                nn = gl ? C : C + 1;
+                if (lane == lf) {
+                    s_lf[C] = (int16_t)lf;
+                    s_cnt[C] = (int16_t)nl;
+                    s_lf[C + 1] = (int16_t)(lf + nl);
+                    s_cnt[C + 1] = (int16_t)(c - nl);
+                    s_left[C] = s_left[C + 1] = -1;
+                    s_par[C] = s_par[C + 1] = (int16_t)nid;
+                    s_skip[C] = (int16_t)(C + 1);
+                    s_skip[C + 1] = s_skip[nid];
+                    s_left[nid] = (int16_t)C;
+                }
+            }
+            if (v) {
+                P[dst] = p;
+                Q[dst] = nn;
+            }
+            __syncthreads();
+            if (v) {
+                p = P[lane];
+                nid = Q[lane];
+            }
+            const int M0 = M;
+            M += 2 * __popcll(mH);
+            // tight boxes of the new children: segmented scans over their runs
+            const bool fresh = v && nid >= M0;
+            const int seg = fresh ? s_lf[nid] : lane;
+            float l0 = p.x, l1 = p.y, l2 = p.z, h0 = p.x, h1 = p.y, h2 = p.z;
+            for (int o = 1; o < kWave; o <<= 1) {
+                const float u0 = __shfl_up(l0, o, 64), u1 = __shfl_up(l1, o, 64),
+                            u2 = __shfl_up(l2, o, 64), w0 = __shfl_up(h0, o, 64),
+                            w1 = __shfl_up(h1, o, 64), w2 = __shfl_up(h2, o, 64);
+                if (lane - o >= seg) {
+                    l0 = fminf(l0, u0);
+                    l1 = fminf(l1, u1);
+                    l2 = fminf(l2, u2);
+                    h0 = fmaxf(h0, w0);
+                    h1 = fmaxf(h1, w1);
+                    h2 = fmaxf(h2, w2);
+                }
+            }
+            const int cnt_new = fresh ? s_cnt[nid] : 0;
+            if (fresh && lane == seg + cnt_new - 1) {
+                s_lo[0][nid] = l0;
+                s_lo[1][nid] = l1;
+                s_lo[2][nid] = l2;
+                s_hi[0][nid] = h0;
+                s_hi[1][nid] = h1;
+                s_hi[2][nid] = h2;
+            }
+            if (M > M0) {
+                nlev++;
+                if (lane == 0) s_lvl[nlev + 1] = (int16_t)M;
+            }
+            __syncthreads();
+            pend = fresh && cnt_new > kLeafPoints;
+        }
+        // every lane's node is a leaf now; reserve the subtree's node ids
+        int32_t base = 0;
+        if (lane == 0 && M > 1) {
+            base = (int32_t)alloc_nodes_and_small(a, M - 1, 0);
+            if (base + M - 1 > a.cap_nodes) {
+                atomicOr(&a.counters[kErrFlags], kErrCapacity);
+                base = -1;
+            }
+        }
+        base = __shfl(base, 0, 64);
+        if (base < 0) {  // out of node ids: the root stays a leaf
+            M = 1;
+            nlev = 0;
+            nid = 0;
+            if (lane == 0) s_left[0] = -1;
+        }
+        if (v) {
+            P[lane] = p;
+            a.fin[first + lane] = p;
+            a.leaf_of[first + lane] = nid ? base + nid - 1 : t.node;
+        }
+        __syncthreads();
+        // leaf centres: f64 sums in tree order (src/potential.c:100-107)
+        for (int j = lane; j < M; j += kWave) {
+            if (s_left[j] >= 0) continue;
+            const int lf = s_lf[j], c = s_cnt[j];
+            double sx = 0.0, sy = 0.0, sz = 0.0;
+            for (int i = lf; i < lf + c; i++) {
+                const float4 q = P[i];
+                sx += (double)q.x;
+                sy += (double)q.y;
+                sz += (double)q.z;
+            }
+            const double dn = (double)c;
+            s_com[0][j] = sx != 0.0 ? (float)(sx / dn) : 0.0f;
+            s_com[1][j] = sy != 0.0 ? (float)(sy / dn) : 0.0f;
+            s_com[2][j] = sz != 0.0 ? (float)(sz / dn) : 0.0f;
+            s_mass[j] = (float)(particle_mass * dn);
+        }
+        __syncthreads();
+        // internal centres, deepest level first (src/potential.c:115-122:
+        // float, right term first)
+        for (int L = nlev; L >= 0; L--) {
+            for (int j = s_lvl[L] + lane; j < s_lvl[L + 1]; j += kWave) {
+                const int cl = s_left[j];
+                if (cl < 0) continue;
+                const float ml = s_mass[cl], mr = s_mass[cl + 1];
+                const float m = ml + mr;
+                for (int d = 0; d < 3; d++) {
+                    const float rr = s_com[d][cl + 1] * mr;
+                    const float ll = s_com[d][cl] * ml;
+                    s_com[d][j] = m != 0.0f ? (rr + ll) / m : 0.0f;
+                }
+                s_mass[j] = m;
+            }
+            __syncthreads();
+        }
+        // dmin over each node's points in tree order (src/potential.c:71-82),
+        // then the node records
+        for (int j = lane; j < M; j += kWave) {
+            const int lf = s_lf[j], c = s_cnt[j];
+            const float com[3] = {s_com[0][j], s_com[1][j], s_com[2][j]};
+            double big = 0.0, sum = 0.0;
+            for (int i = lf; i < lf + c; i++) {
+                const double d = d2_ref(com, P[i]);
+                big = d > big ? d : big;
+                sum += d;
+            }
+            Node nd;
+            for (int d = 0; d < 3; d++) {
+                nd.lo[d] = s_lo[d][j];
+                nd.hi[d] = s_hi[d][j];
+                nd.com[d] = com[d];
+            }
+            nd.mass = s_mass[j];
+            nd.first = first + lf;
+            nd.count = c;
+            const int cl = s_left[j], sk = s_skip[j];
+            nd.left = cl >= 0 ? base + cl - 1 : -1;
+            nd.skip = sk == kRootSkip ? root.skip : base + sk - 1;
+            finish_dmin(&nd, big, sum, c);
+            const int32_t id = j ? base + j - 1 : t.node;
+            a.nodes[id] = nd;
+            if (j) a.parent[id] = s_par[j] ? base + s_par[j] - 1 : t.node;
         }
         __syncthreads();
     }

@@ -442,6 +442,9 @@ struct Engine {
         k_subtree<<<2048, kSubThreads, 0, s>>>(A.as<float4>(), B.as<float4>(), small0.as<Task>(),
                                                build_args(small0.as<Task>()), PARTICLE_MASS);
         HP_LAUNCHED();
+        k_subtree_wave<<<8192, kWave, 0, s>>>(A.as<float4>(), B.as<float4>(), small0.as<Task>(),
+                                              build_args(small0.as<Task>()), PARTICLE_MASS);
+        HP_LAUNCHED();
         k_leaf_com<<<2048, 256, 0, s>>>(nodes.as<Node>(), F.as<float4>(), cnt, PARTICLE_MASS);
         HP_LAUNCHED();
         k_fat_leaf_com<<<256, 256, 0, s>>>(nodes.as<Node>(), fat.as<int32_t>(), cnt,

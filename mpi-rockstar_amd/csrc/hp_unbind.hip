@@ -419,7 +419,9 @@ __global__ __launch_bounds__(256) void k_vel_select(const struct potential *__re
     Cand bc = no_cand(), bb = no_cand();
     const int w = threadIdx.x / kWave;
     __syncthreads();
-    for (int r = 0; r < kScanTile / 256; r++) {
+    // only the rounds that hold records below rvir_max (block-uniform)
+    const int rounds = min(kScanTile / 256, max(0, (lim - st.off + 255) / 256));
+    for (int r = 0; r < rounds; r++) {
         const int32_t j = st.off + r * 256 + threadIdx.x;
         const bool v = j < lim;
         double x[6] = {0, 0, 0, 0, 0, 0};

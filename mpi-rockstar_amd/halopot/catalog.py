@@ -383,9 +383,8 @@ def config5(seed=SEEDS["c5"], total=16_777_216, box=250.0, n_halos=50_000, frac_
     rng = np.random.Generator(np.random.Philox(seed))
     prm = Params(particle_mass=CRITICAL_DENSITY * box ** 3 * 0.27 / total)
     n_in = int(round(total * frac_halo))
-    n = power_law_counts(rng, n_halos, lo, hi).astype(np.float64)
-    n = np.maximum(lo, np.floor(n * (n_in / n.sum()))).astype(np.int64)
-    n[np.argmax(n)] += n_in - int(n.sum())
+    raw = power_law_counts(rng, n_halos, lo, hi).astype(np.float64)
+    n = _scale_counts(raw, n_in, lo)
     mvir = n * prm.particle_mass
     conc = 9.0 * (mvir / 1e13) ** -0.13 * np.exp(rng.normal(0.0, 0.3, n_halos))
     rvir = virial_radius(prm, mvir)
@@ -397,6 +396,33 @@ def config5(seed=SEEDS["c5"], total=16_777_216, box=250.0, n_halos=50_000, frac_
     po = _records(pos, vel, C)
     halos = _halo_rows(firsts, n, cen, bulk, np.zeros(n_halos, dtype=np.int32))
     return Catalogue("c5", prm, po, halos)
+
+
+def _scale_counts(raw, total, lo):
+    """Integer counts max(lo, floor(s raw)) summing to `total`: s by bisection
+    so that the clamped sum stays at or below the total, then the remainder
+    one particle each to the largest halos (ties by index)."""
+    if lo * raw.shape[0] > total:
+        raise ValueError("total too small for the clamp")
+
+    def clamped(s):
+        return np.maximum(lo, np.floor(raw * s)).astype(np.int64)
+
+    a, b = 0.0, 1.0
+    while clamped(b).sum() <= total:
+        b *= 2.0
+    for _ in range(200):
+        m = 0.5 * (a + b)
+        if clamped(m).sum() <= total:
+            a = m
+        else:
+            b = m
+    n = clamped(a)
+    rem = total - int(n.sum())
+    order = np.argsort(-n, kind="stable")
+    n[order[:rem]] += 1
+    assert int(n.sum()) == total and n.min() >= lo
+    return n
 
 
 def make(name: str, **kw) -> Catalogue:

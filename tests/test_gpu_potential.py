@@ -69,6 +69,41 @@ def test_tree_identical(restatement, which):
     _compare_trees(gnodes, perm, onodes, opo)
 
 
+@pytest.mark.parametrize("n", [11, 37, 64, 65, 300])
+def test_tree_identical_small_calls(restatement, n):
+    """Calls around the size where subtrees move from one wavefront (<= 64
+    points) to a workgroup."""
+    cat = catalog.config1()
+    prm = cat.params
+    halopot.set_config(prm)
+    po = with_ids(cat.segment(0)[:n])
+    gnodes, perm = halopot.debug_tree(po)
+    opo = po.copy()
+    onodes = restatement.build_tree(opo, prm)
+    _compare_trees(gnodes, perm, onodes, opo)
+
+
+def test_tree_identical_coincident_points(restatement):
+    """31 of 40 points coincide: the split that cannot separate them leaves a
+    31-point leaf (src/fast3tree.c:805-809), whose self term the walk skips
+    (more than 20 points, src/potential.c:144-146)."""
+    cat = catalog.config1()
+    prm = cat.params
+    halopot.set_config(prm)
+    po = with_ids(cat.segment(0)[:40])
+    po["pos"][10:, :3] = po["pos"][0, :3]
+    gnodes, perm = halopot.debug_tree(po)
+    opo = po.copy()
+    onodes = restatement.build_tree(opo, prm)
+    _compare_trees(gnodes, perm, onodes, opo)
+    assert np.any((gnodes["left"] < 0) & (gnodes["count"] == 31))
+    g = po.copy()
+    halopot.compute_potential(g)
+    o = po.copy()
+    restatement.compute_potential(o, prm)  # permutes o
+    np.testing.assert_allclose(g["pe"], by_id(o), rtol=1e-5)
+
+
 def test_c1_potential_parity(restatement, request):
     cat = catalog.config1()
     prm = cat.params
