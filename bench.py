@@ -9,12 +9,15 @@ descriptors are resident in HBM before the timed region; each step first
 restores the pristine input with a device-to-device copy (the pass reorders
 records in place), which is included in the timing.
 
-Default workload: the batching configuration, 2,000 NFW halos with
-n ~ n^-1.9 on [20, 200000] plus 5% contaminants (seeded), one batched plan.
-Beside it, the same catalogue through the plain per-halo GPU path (every
-call with its own launches) and the reference's CPU path (its own
-compute_potential / compute_kinetic_energy from oracle/_ref, OpenMP over
-halos) on a bounded sample of host time.
+Default workload: the one-GPU catalogue, 16,777,216 particles in a
+250 Mpc/h box of which 40% (6,710,886) sit in 50,000 NFW halos with counts
+from n^-1.9 on [20, 500000] (seeded); every halo is one compute_potential
+call, all of them in one batched plan replayed from hipGraphs.  Beside it,
+the same catalogue through the plain per-halo GPU path (every call with its
+own launches), through the host-buffer entry point (PCIe copies included),
+and through the reference's CPU path (its own compute_potential /
+compute_kinetic_energy from oracle/_ref, OpenMP over halos) on a bounded
+sample of host time.
 
 Prints ONE JSON line.
 """
@@ -57,6 +60,19 @@ def cpu_threads() -> int:
         return min(16, len(os.sched_getaffinity(0)))
     except AttributeError:
         return min(16, os.cpu_count() or 1)
+
+
+def walk_traffic(workload: str):
+    """HBM bytes per k_walk launch for this workload, from the committed
+    FETCH_SIZE / WRITE_SIZE counter runs (scaled by a copy of known size),
+    or None when no such run exists."""
+    f = ROOT / "profiles" / f"hbm_traffic.{workload}_catalogue.md"
+    if not f.exists():
+        return None
+    for line in f.read_text().splitlines():
+        if line.startswith(f"traffic k_walk {workload}:"):
+            return float(line.split(":")[1])
+    return None
 
 
 def run_cpu_baseline(cat, seconds: float):
@@ -105,11 +121,12 @@ def main() -> int:
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
-    ap.add_argument("--workload", default="c2", choices=sorted(WORKLOADS))
+    ap.add_argument("--workload", default="c5", choices=sorted(WORKLOADS))
     ap.add_argument("--accumulate", default="compensated", choices=["compensated", "naive", "f64"])
     ap.add_argument("--no-graph", action="store_true", help="eager launches instead of hipGraph replay")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--no-per-halo", action="store_true", help="skip the per-halo GPU path")
+    ap.add_argument("--no-host", action="store_true", help="skip the host-buffer (PCIe) leg")
     ap.add_argument("--cpu-seconds", type=float, default=8.0)
     args = ap.parse_args()
     if args.gpus != 1:
@@ -186,7 +203,7 @@ def main() -> int:
         "peak": FP32_PEAK_TFLOPS,
         "unit": "TFLOP/s",
         "frac": achieved / FP32_PEAK_TFLOPS,
-        "traffic": None,
+        "traffic": walk_traffic(args.workload),
         "kernel": "k_walk (Barnes-Hut group walk)",
         "bound_detail": "FP32 VALU + v_rsq issue (no dense contraction: MFMA unused); peak is "
                         "the published FP32 vector figure",
@@ -235,6 +252,17 @@ def main() -> int:
         out["batched_vs_per_halo"] = value / vp
         log(f"per-halo GPU path: {1e3 * ep / reps:.1f} ms/pass, {vp:.4g} records/s; "
             f"batched/per-halo = {value / vp:.1f}x")
+
+    if not args.no_host:
+        # the same pass from pageable host buffers: H2D, plan, run, D2H
+        po_h = cat.po.copy()
+        h_h = cat.halos.copy()
+        th = time.perf_counter()
+        halopot.bound_pass_host(po_h, h_h, halopot.options(accumulate=acc))
+        eh = time.perf_counter() - th
+        out["host_buffers"] = {"value": cat.n_records / eh, "unit": UNIT, "ms_per_pass": 1e3 * eh,
+                               "includes": "plan creation, device scratch, H2D + D2H over PCIe"}
+        log(f"host-buffer pass (PCIe included): {1e3 * eh:.1f} ms, {cat.n_records / eh:.4g} records/s")
 
     if not args.no_cpu:
         cb = run_cpu_baseline(cat, args.cpu_seconds)

@@ -180,8 +180,8 @@ def debug_tree(po: np.ndarray):
 class Plan:
     """A planned batch (hp_plan_create) run on device-resident data.
 
-    Device memory comes from torch (plumbing only); records and halos are
-    passed as raw device pointers to the C ABI."""
+    Records and halos are passed as raw device pointers to the C ABI (for
+    example halopot.hipmem.DeviceArray addresses)."""
 
     def __init__(self, halos: np.ndarray, opts: Options | None = None):
         if halos.dtype != HALO_DTYPE or not halos.flags.c_contiguous:
