@@ -1,0 +1,106 @@
+"""Precision sweep on the one-halo catalogue (10,000 NFW particles + 1,000
+background, seeded): the GPU potential with FP32 naive sums, FP32
+compensated (two-sum) sums and FP64 terms, each against
+
+  * the FP64 direct sum (the physical potential; the reference's own
+    Barnes-Hut, ERR_TOL 1 with monopoles, is ~2e-3 from it),
+  * the FP64 evaluation of the reference's tree and decisions (what the
+    FP32 arithmetic is measured against),
+  * the reference's Barnes-Hut output itself (the compiled src/potential.c
+    when present, else its restatement),
+
+and the bound flags of the full bound pass against the oracle's, with every
+mismatch binned by |KE + PE| / |PE|.  The table is printed (pytest -s).
+
+Bars (median / max relative error of pe):
+  compensated: vs FP64 same tree <= 1e-6 / 1e-5, vs reference BH max 1e-5;
+  naive:       vs FP64 same tree <= 1e-4 / 1e-3;
+  f64:         vs FP64 same tree <= 1e-12 / 1e-10;
+every mode: flag mismatches only where |KE + PE| / |PE| < 1e-4."""
+import numpy as np
+import pytest
+
+import halopot
+from halopot import catalog
+from conftest import by_id, with_ids
+from gpu_util import rel, run_plan
+
+pytestmark = pytest.mark.gpu
+
+MODES = [("naive", halopot.ACC_NAIVE), ("compensated", halopot.ACC_COMPENSATED),
+         ("f64", halopot.ACC_F64)]
+BARS = {"naive": (1e-4, 1e-3), "compensated": (1e-6, 1e-5), "f64": (1e-12, 1e-10)}
+BINS = [0.0, 1e-8, 1e-7, 1e-6, 1e-5, 1e-4, 1e-3, 1e-2, 1e-1, 1.0, np.inf]
+
+
+@pytest.fixture(scope="module")
+def c1_refs(restatement, request):
+    cat = catalog.config1()
+    cat.po["flags"] = np.arange(cat.n_records, dtype=np.int32)
+    prm = cat.params
+    po = with_ids(cat.po)
+    direct = restatement.direct64(po, prm)
+    b64 = po.copy()
+    restatement.compute_potential_bh64(b64, prm)
+    try:
+        ref = request.getfixturevalue("reference")
+        a = po.copy()
+        ref.compute_potential(a, prm)
+        ref_kind = "compiled reference"
+    except pytest.skip.Exception:
+        a = po.copy()
+        restatement.compute_potential(a, prm)
+        ref_kind = "restatement"
+    po_o = cat.po.copy()
+    h_o = cat.halos.copy()
+    kept_o, bound_o = restatement.bound_pass_batch(po_o, h_o, prm)
+    return cat, direct, by_id(b64), by_id(a), ref_kind, po_o, int(kept_o[0]), bound_o
+
+
+def test_precision_sweep(c1_refs):
+    cat, direct, bh64, refbh, ref_kind, po_o, kept, bound_o = c1_refs
+    lines = [f"precision sweep, C1 ({cat.n_records} records); reference BH = {ref_kind}",
+             "| mode | vs FP64 direct (median / p99) | vs FP64 same tree (median / p99 / max) "
+             "| vs reference BH (median / max) | flag mismatches (by |KE+PE|/|PE| bin) |",
+             "|---|---|---|---|---|"]
+    rb = rel(refbh, direct)
+    lines.append(f"| reference BH | {np.median(rb):.2e} / {np.percentile(rb, 99):.2e} | "
+                 f"{np.median(rel(refbh, bh64)):.2e} / {np.percentile(rel(refbh, bh64), 99):.2e} / "
+                 f"{np.max(rel(refbh, bh64)):.2e} | 0 / 0 | — |")
+    failures = []
+    for name, acc in MODES:
+        po_g, _, _, _ = run_plan(cat, halopot.options(accumulate=acc), potential_only=True)
+        pe = po_g["pe"]  # records keep their places: index = record id
+        rd = rel(pe, direct)
+        rs = rel(pe, bh64)
+        rr = rel(pe, refbh)
+        # the full bound pass in this mode
+        po_b, h_b, bound_b, _ = run_plan(cat, halopot.options(accumulate=acc))
+        assert int(h_b["kept"][0]) == kept
+        g = po_b[:kept]
+        o = po_o[:kept]
+        gi = np.lexsort((g["flags"], g["r2"]))
+        oi = np.lexsort((o["flags"], o["r2"]))
+        bg = bound_b[:kept][gi].astype(bool)
+        bo = bound_o[:kept][oi].astype(bool)
+        pe_o, ke_o = o["pe"][oi], o["ke"][oi].astype(np.float64)
+        # physical energies: pe is stored positive (binding), bound <=> pe > ke
+        margin = np.abs(ke_o - pe_o) / np.maximum(np.abs(pe_o), 1e-300)
+        mism = bg != bo
+        hist, _ = np.histogram(margin[mism], bins=BINS)
+        hist_s = ", ".join(f"<{BINS[i + 1]:.0e}: {hist[i]}" for i in range(len(hist)) if hist[i])
+        lines.append(f"| {name} | {np.median(rd):.2e} / {np.percentile(rd, 99):.2e} | "
+                     f"{np.median(rs):.2e} / {np.percentile(rs, 99):.2e} / {np.max(rs):.2e} | "
+                     f"{np.median(rr):.2e} / {np.max(rr):.2e} | "
+                     f"{int(mism.sum())} of {kept}{' (' + hist_s + ')' if hist_s else ''} |")
+        med_bar, max_bar = BARS[name]
+        if not (np.median(rs) <= med_bar and np.max(rs) <= max_bar):
+            failures.append(f"{name}: vs FP64 same tree median {np.median(rs):.2e} max {np.max(rs):.2e}")
+        if name == "compensated" and np.max(rr) > 1e-5:
+            failures.append(f"{name}: vs reference BH max {np.max(rr):.2e}")
+        if np.any(margin[mism] >= 1e-4):
+            failures.append(f"{name}: a bound-flag mismatch outside the 1e-4 margin")
+        # the physical error is the reference algorithm's, not the arithmetic's
+        assert abs(np.median(rd) - np.median(rb)) < 0.05 * np.median(rb)
+    print("\n" + "\n".join(lines))
+    assert not failures, failures
