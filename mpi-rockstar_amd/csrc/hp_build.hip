@@ -486,20 +486,19 @@ __global__ __launch_bounds__(kSubThreads) void k_subtree(const float4 *__restric
             for (int k = w; k < nn; k += kSubWaves) {
                 const SubNode e = E[cur][k];
                 if (!e.kind) {
-                    double sx = 0.0, sy = 0.0, sz = 0.0;
                     for (int i = lane; i < e.n; i += kWave) {
-                        const float4 p = P[buf][e.lf + i];
-                        a.fin[first + e.lf + i] = p;
+                        a.fin[first + e.lf + i] = P[buf][e.lf + i];
                         a.leaf_of[first + e.lf + i] = e.id;
-                        sx += (double)p.x;
-                        sy += (double)p.y;
-                        sz += (double)p.z;
                     }
-                    sx = wave_sum(sx);
-                    sy = wave_sum(sy);
-                    sz = wave_sum(sz);
                     if (lane == 0) {
-                        // src/potential.c:100-107
+                        // src/potential.c:100-107: f64 sums in tree order
+                        double sx = 0.0, sy = 0.0, sz = 0.0;
+                        for (int i = 0; i < e.n; i++) {
+                            const float4 p = P[buf][e.lf + i];
+                            sx += (double)p.x;
+                            sy += (double)p.y;
+                            sz += (double)p.z;
+                        }
                         const double n = (double)e.n;
                         Node *me = a.nodes + e.id;
                         me->com[0] = sx != 0.0 ? (float)(sx / n) : 0.0f;
@@ -591,8 +590,23 @@ __global__ __launch_bounds__(kSubThreads) void k_subtree(const float4 *__restric
         // dmin over the final order (src/potential.c:71-82)
         for (int32_t i = tid; i < count; i += kSubThreads) P[0][i] = a.fin[first + i];
         __syncthreads();
-        // each wavefront takes 64 nodes at a time: their records are loaded
-        // lane-parallel, then walked one by one from registers
+        // nodes of at most 64 points: a thread each, summing in tree order
+        for (int32_t j = tid; j < s_made; j += kSubThreads) {
+            Node *np = a.nodes + made[j];
+            const int32_t cnt = np->count;
+            if (cnt > kWave) continue;
+            const int32_t rel = np->first - first;
+            const float com[3] = {np->com[0], np->com[1], np->com[2]};
+            double big = 0.0, sum = 0.0;
+            for (int32_t i = rel; i < rel + cnt; i++) {
+                const double d = d2_ref(com, P[0][i]);
+                big = d > big ? d : big;
+                sum += d;
+            }
+            finish_dmin(np, big, sum, cnt);
+        }
+        // larger nodes: a wavefront each; each wavefront takes 64 nodes at a
+        // time, their records loaded lane-parallel, then walked one by one
         for (int32_t j0 = w * kWave; j0 < s_made; j0 += kSubWaves * kWave) {
             const int32_t jj = j0 + lane;
             int32_t my_rel = 0, my_cnt = 0;
@@ -608,6 +622,7 @@ __global__ __launch_bounds__(kSubThreads) void k_subtree(const float4 *__restric
             const int nb = min(kWave, s_made - j0);
             for (int q = 0; q < nb; q++) {
                 const int32_t rel = __shfl(my_rel, q, 64), cnt = __shfl(my_cnt, q, 64);
+                if (cnt <= kWave) continue;  // done above
                 const float com[3] = {__shfl(my_c0, q, 64), __shfl(my_c1, q, 64),
                                       __shfl(my_c2, q, 64)};
                 double big = 0.0, sum = 0.0;

@@ -118,31 +118,63 @@ __device__ __forceinline__ int rank_below(uint64_t m) {
 
 __device__ __forceinline__ int uniform(int v) { return __builtin_amdgcn_readfirstlane(v); }
 
+// Wavefront reductions on DPP lane moves (no LDS round trips): swap within
+// pairs and quads, mirror within half rows and rows (every lane then holds
+// its row's result), then combine the four row results read from lanes 0,
+// 16, 32 and 48.  The result is wavefront-uniform.  Call with all 64 lanes
+// active, in wavefront-uniform control flow.
+template <int CTRL>
+__device__ __forceinline__ int dpp_i32(int v) {
+    return __builtin_amdgcn_update_dpp(0, v, CTRL, 0xf, 0xf, false);
+}
+
+template <int CTRL, typename T>
+__device__ __forceinline__ T dpp_move(T v) {
+    static_assert(sizeof(T) == 4 || sizeof(T) == 8, "32- or 64-bit lanes");
+    if constexpr (sizeof(T) == 4) {
+        return __builtin_bit_cast(T, dpp_i32<CTRL>(__builtin_bit_cast(int, v)));
+    } else {
+        const uint64_t u = __builtin_bit_cast(uint64_t, v);
+        const uint32_t lo = (uint32_t)dpp_i32<CTRL>((int)(uint32_t)u);
+        const uint32_t hi = (uint32_t)dpp_i32<CTRL>((int)(uint32_t)(u >> 32));
+        return __builtin_bit_cast(T, ((uint64_t)hi << 32) | lo);
+    }
+}
+
+template <typename T>
+__device__ __forceinline__ T read_lane(T v, int l) {
+    if constexpr (sizeof(T) == 4) {
+        return __builtin_bit_cast(T, __builtin_amdgcn_readlane(__builtin_bit_cast(int, v), l));
+    } else {
+        const uint64_t u = __builtin_bit_cast(uint64_t, v);
+        const uint32_t lo = (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)u, l);
+        const uint32_t hi = (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)(u >> 32), l);
+        return __builtin_bit_cast(T, ((uint64_t)hi << 32) | lo);
+    }
+}
+
+template <typename T, typename Op>
+__device__ __forceinline__ T wave_reduce(T v, Op op) {
+    v = op(v, dpp_move<0xB1>(v));   // quad_perm [1, 0, 3, 2]
+    v = op(v, dpp_move<0x4E>(v));   // quad_perm [2, 3, 0, 1]
+    v = op(v, dpp_move<0x141>(v));  // row_half_mirror
+    v = op(v, dpp_move<0x140>(v));  // row_mirror
+    return op(op(read_lane(v, 0), read_lane(v, 16)), op(read_lane(v, 32), read_lane(v, 48)));
+}
+
 template <typename T>
 __device__ __forceinline__ T wave_min(T v) {
-#pragma unroll
-    for (int o = 32; o > 0; o >>= 1) {
-        T w = __shfl_xor(v, o, 64);
-        v = w < v ? w : v;
-    }
-    return v;
+    return wave_reduce(v, [](T a, T b) { return b < a ? b : a; });
 }
 
 template <typename T>
 __device__ __forceinline__ T wave_max(T v) {
-#pragma unroll
-    for (int o = 32; o > 0; o >>= 1) {
-        T w = __shfl_xor(v, o, 64);
-        v = w > v ? w : v;
-    }
-    return v;
+    return wave_reduce(v, [](T a, T b) { return b > a ? b : a; });
 }
 
 template <typename T>
 __device__ __forceinline__ T wave_sum(T v) {
-#pragma unroll
-    for (int o = 32; o > 0; o >>= 1) v = v + __shfl_xor(v, o, 64);
-    return v;
+    return wave_reduce(v, [](T a, T b) { return a + b; });
 }
 
 __device__ __forceinline__ float comp(const float4 &p, int d) {
