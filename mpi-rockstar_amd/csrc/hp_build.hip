@@ -330,310 +330,422 @@ __device__ __forceinline__ int32_t small_pushes(int32_t nl, int32_t nr, int32_t 
 
 // ------------------------------------------------------------- subtrees
 //
-// Every node of at most kSmallMax points is finished by one workgroup: its
-// points are staged in LDS (two buffers, ping-pong per level), the subtree
-// is split level by level (the wavefronts share the nodes of a level),
-// leaves go straight to tree order, and the centres of mass (bottom-up by
-// level) and dmin (over the final order) of every subtree node are computed
-// before the workgroup moves on.  Node ids come from a reservation made
-// once per subtree (refilled if a subtree needs more); unused reserved ids
-// are written as empty nodes.
+// A node of 65..kSmallMax points is finished by one workgroup, a node of at
+// most 64 points by one wavefront (below).  Both keep the points in LDS and
+// number the subtree's nodes locally in creation order: levels in order,
+// the two children of a split as a pair.
+//
+// Workgroup subtrees are point-parallel per level over the whole subtree:
+// thread t takes positions t, t + 256, ...; a block-wide prefix of the
+// left flags (one 64-bit ballot per chunk of 64 positions) gives every
+// node's left count and every point's stable scatter position at once; the
+// children's tight boxes come from segmented scans over runs of lanes, one
+// LDS atomic per run and coordinate.  Per-node bookkeeping runs a thread per
+// node.  Local ids map onto a block of 2n - 2 global ids reserved with one
+// atomic when the subtree starts; ids it does not use are never referenced.
+// Centres of mass are combined bottom-up in LDS and dmin is computed over
+// the final order in LDS.
 
 constexpr int kWaveTask = 64;   // subtrees up to this size: one wavefront
+constexpr int kRootSkip = -2;   // local skip pointer: the subtree root's own skip
 constexpr int kSubThreads = 256;
 constexpr int kSubWaves = kSubThreads / kWave;
-constexpr int kSubList = 2 * (kSmallMax / (kLeafPoints + 1)) + 2;  // entries of one level
+constexpr int kSubRounds = kSmallMax / kSubThreads;              // positions per thread
+constexpr int kSubChunks = kSmallMax / kWave;                    // ballot chunks
+constexpr int kSubList = 2 * (kSmallMax / (kLeafPoints + 1)) + 2;  // nodes of one level
+constexpr int kSubNodes = 2 * kSmallMax;                         // local ids (2n - 1 at most)
+constexpr int kSubPerThread = kSubNodes / kSubThreads;
+constexpr uint8_t kNoEntry = 0xff;
+static_assert(kSubList < kNoEntry, "level entries fit a byte");
 
-struct SubNode {
-    int32_t id;
-    uint16_t lf;     // first point, relative to the subtree
-    uint16_t n;
-    uint16_t nl;
-    uint8_t kind;    // 0 leaf, 1 split
-    uint8_t dim;     // split plane of the node (from its tight box)
-    float lim;
-    int32_t child;   // first child id (splits)
-    int32_t skip;    // the node's skip pointer
+__device__ __forceinline__ uint64_t lanes_below(int i) {
+    return i >= 64 ? ~0ull : ((1ull << i) - 1ull);
+}
+
+// bits [lf, lf + n) of a wavefront mask (lf + n <= 64)
+__device__ __forceinline__ uint64_t lane_run(int lf, int n) {
+    return (n >= 64 ? ~0ull : ((1ull << n) - 1ull)) << lf;
+}
+
+// Inclusive scan of v over this lane's run [seg, lane] of the wavefront
+// (seg <= lane; runs are contiguous), on DPP moves: row_shr 1, 2, 4, 8
+// within rows of 16, then row_bcast 15 and 31 across rows.
+template <typename Op>
+__device__ __forceinline__ float seg_scan(float v, int seg, Op op) {
+    const int lane = lane_id(), r = lane & 15;
+    float u;
+    u = dpp_move<0x111>(v);
+    if (r >= 1 && lane - 1 >= seg) v = op(v, u);
+    u = dpp_move<0x112>(v);
+    if (r >= 2 && lane - 2 >= seg) v = op(v, u);
+    u = dpp_move<0x114>(v);
+    if (r >= 4 && lane - 4 >= seg) v = op(v, u);
+    u = dpp_move<0x118>(v);
+    if (r >= 8 && lane - 8 >= seg) v = op(v, u);
+    u = dpp_move<0x142>(v);  // row_bcast:15, used by rows 1 and 3
+    if (((lane >> 4) & 1) && (lane & ~15) - 1 >= seg) v = op(v, u);
+    u = dpp_move<0x143>(v);  // row_bcast:31, used by rows 2 and 3
+    if (lane >= 32 && 31 >= seg) v = op(v, u);
+    return v;
+}
+
+struct SubEntry {       // a node of the level being split, 20 bytes
+    float lim;          // split plane, from the node's tight box
+    int16_t loc;        // local node id
+    int16_t lf, n, nl;  // first point (subtree-relative), points, left count
+    int16_t child;      // local id of the left child (splits), else -1
+    int16_t skip;       // local skip pointer
+    uint8_t kind, dim;  // 1: split; split dimension
 };
-
-__device__ __forceinline__ void write_empty_node(const BuildArgs &a, int32_t id) {
-    const float z[3] = {0.0f, 0.0f, 0.0f};
-    write_node(a.nodes + id, z, z, 0, 0, -1);
-    a.parent[id] = -1;
-}
-
-__device__ __forceinline__ int32_t even_up(int32_t x) { return (x + 1) & ~1; }
-
-__device__ __forceinline__ SubNode sub_entry(int32_t id, int lf, int n, const float *lo,
-                                             const float *hi, int32_t skip) {
-    SubNode e;
-    e.id = id;
-    e.lf = (uint16_t)lf;
-    e.n = (uint16_t)n;
-    e.nl = 0;
-    e.kind = 0;
-    const int d = split_dim(lo, hi);
-    e.dim = (uint8_t)d;
-    e.lim = split_lim(lo, hi, d);
-    e.child = -1;
-    e.skip = skip;
-    return e;
-}
 
 __global__ __launch_bounds__(kSubThreads) void k_subtree(const float4 *__restrict__ A,
                                                          const float4 *__restrict__ B,
                                                          const Task *__restrict__ tasks,
                                                          BuildArgs a, double particle_mass) {
     __shared__ float4 P[2][kSmallMax];
-    __shared__ SubNode E[2][kSubList];
-    __shared__ int32_t made[2 * kSmallMax];
-    __shared__ uint16_t lvl_end[kSmallMax + 1];
-    __shared__ int32_t s_n[2], s_made, s_levels, s_pool, s_pool_end, s_base;
+    __shared__ uint8_t Q[2][kSmallMax];       // level entry of each position
+    __shared__ SubEntry E[2][kSubList];
+    __shared__ int32_t BOX[kSubList][6];      // next level's tight boxes, float keys
+    __shared__ int16_t s_left[kSubNodes];     // left child of each local node, -1: leaf
+    __shared__ uint16_t s_lvl[kSmallMax + 2]; // first local id of each level
+    __shared__ uint64_t s_mask[kSubChunks + 1];
+    __shared__ int32_t s_pre[kSubChunks + 1];
+    __shared__ int32_t s_nn[2], s_M, s_nlev, s_base;
     const int tid = threadIdx.x, lane = lane_id(), w = tid / kWave;
     const int32_t ntasks = min(a.counters[kSmallNext], a.cap_small);
+    const float inf = __int_as_float(0x7f800000), ninf = __int_as_float((int)0xff800000);
     // from the end of the list: the largest subtrees (children of the big
     // nodes of the last split levels) are queued last and should start first
     for (int32_t k = blockIdx.x; k < ntasks; k += gridDim.x) {
         const Task t = tasks[ntasks - 1 - k];
         const Node root = a.nodes[t.node];
+        const int32_t first = root.first, n = root.count;
+        if (n <= kWaveTask) continue;  // a wavefront subtree
         const float4 *src = (t.depth & 1) ? B : A;
-        const int32_t first = root.first, count = root.count;
-        if (count <= kWaveTask) continue;  // a wavefront subtree
-        for (int32_t i = tid; i < count; i += kSubThreads) P[0][i] = src[first + i];
+        for (int i = tid; i < n; i += kSubThreads) {
+            P[0][i] = src[first + i];
+            Q[0][i] = 0;
+        }
         if (tid == 0) {
-            E[0][0] = sub_entry(t.node, 0, count, root.lo, root.hi, root.skip);
-            s_n[0] = 1;
-            made[0] = t.node;
-            s_made = 1;
-            lvl_end[0] = 1;
-            s_levels = 1;
-            const int32_t R = even_up(count * 2 / 5 + 8);
-            const int32_t base = (int32_t)alloc_nodes_and_small(a, R, 0);
-            s_pool = base;
-            s_pool_end = base + R;
-            if (base + R > a.cap_nodes) {
+            SubEntry e;
+            e.loc = 0;
+            e.lf = 0;
+            e.n = (int16_t)n;
+            e.nl = 0;
+            e.child = -1;
+            e.skip = kRootSkip;
+            e.kind = 0;
+            e.dim = (uint8_t)split_dim(root.lo, root.hi);
+            e.lim = split_lim(root.lo, root.hi, e.dim);
+            E[0][0] = e;
+            s_nn[0] = 1;
+            s_M = 1;
+            s_nlev = 0;
+            s_lvl[0] = 0;
+            s_lvl[1] = 1;
+            s_left[0] = -1;
+            s_mask[kSubChunks] = 0;
+            const int32_t R = 2 * n - 2;
+            int32_t base = (int32_t)alloc_nodes_and_small(a, R, 0);
+            if (base + R > a.cap_nodes) {  // out of node ids: the root stays a leaf
                 atomicOr(&a.counters[kErrFlags], kErrCapacity);
-                s_n[0] = 0;  // leave this subtree unsplit
-                s_pool = s_pool_end = 0;
+                base = -1;
+                s_nn[0] = 0;
             }
+            s_base = base;
         }
         __syncthreads();
+        const int32_t base = s_base;
+        auto gid = [&](int j) { return j == 0 ? t.node : base + j - 1; };
+        auto gskip = [&](int s) { return s == kRootSkip ? root.skip : gid(s); };
         int cur = 0, buf = 0;
-        while (s_n[cur] > 0) {
-            const int nn = s_n[cur];
-            // pass A: left count of every node of the level (src/fast3tree.c:751-774)
-            for (int k = w; k < nn; k += kSubWaves) {
-                const SubNode e = E[cur][k];
-                int kind = 0, nl = 0;
-                if (e.n > kLeafPoints) {
-                    for (int c = 0; c < e.n; c += kWave) {
-                        const int i = c + lane;
-                        const bool f = i < e.n && comp(P[buf][e.lf + i], e.dim) <= e.lim;
-                        nl += __popcll(ballot(f));
-                    }
-                    kind = (nl == 0 || nl == e.n) ? 0 : 1;  // src/fast3tree.c:805-809
+        while (true) {
+            const int nn = s_nn[cur];
+            if (nn == 0) break;
+            const int nxt = cur ^ 1;
+            const int c0 = s_M;  // first local id of the next level
+            // 1. left flags (src/fast3tree.c:751-774), one ballot per chunk
+            for (int r = 0; r < kSubRounds; r++) {
+                const int i = r * kSubThreads + tid;
+                bool gl = false;
+                if (i < n) {
+                    const uint8_t e = Q[cur][i];
+                    if (e != kNoEntry && E[cur][e].n > kLeafPoints)
+                        gl = comp(P[buf][i], E[cur][e].dim) <= E[cur][e].lim;
                 }
-                if (lane == 0) {
-                    E[cur][k].kind = (uint8_t)kind;
-                    E[cur][k].nl = (uint16_t)nl;
-                }
+                const uint64_t m = ballot(gl);
+                if (lane == 0) s_mask[r * kSubWaves + w] = m;
             }
             __syncthreads();
-            // wavefront 0: node ids for the splits, in level order
+            // 2. wavefront 0: chunk prefix, left counts, splits, child ids
+            //    and the next level's entries
             if (w == 0) {
-                int32_t need = 0;
-                for (int c = 0; c < nn; c += kWave)
-                    need += 2 * __popcll(ballot(c + lane < nn && E[cur][c + lane].kind));
-                int32_t base = 0;
-                if (lane == 0) {
-                    if (s_pool + need > s_pool_end) {
-                        for (int32_t j = s_pool; j < s_pool_end; j++) write_empty_node(a, j);
-                        const int32_t R = even_up(max(need, count / 4 + 8));
-                        const int32_t nb = (int32_t)alloc_nodes_and_small(a, R, 0);
-                        s_pool = nb;
-                        s_pool_end = nb + R;
-                        if (nb + R > a.cap_nodes) {
-                            atomicOr(&a.counters[kErrFlags], kErrCapacity);
-                            s_pool = s_pool_end = 0;
-                            need = -1;
-                        }
-                    }
-                    base = s_pool;
+                const uint64_t mk = lane < kSubChunks ? s_mask[lane] : 0ull;
+                const int cnt = __popcll(mk);
+                int incl = cnt;
+                for (int o = 1; o < kSubChunks; o <<= 1) {
+                    const int u = __shfl_up(incl, o, 64);
+                    if (lane >= o) incl += u;
                 }
-                base = __shfl(base, 0, 64);
-                need = __shfl(need, 0, 64);
-                int32_t done = 0;
-                for (int c = 0; c < nn; c += kWave) {
-                    const int k = c + lane;
-                    bool split = k < nn && E[cur][k].kind;
-                    if (need < 0 && k < nn) {  // out of node ids: finish as leaves
-                        E[cur][k].kind = 0;
-                        split = false;
-                    }
-                    const uint64_t m = ballot(split);
-                    if (split) E[cur][k].child = base + 2 * (done + rank_below(m));
-                    done += __popcll(m);
-                }
-                if (lane == 0) {
-                    s_base = base;
-                    if (need >= 0) s_pool = base + 2 * done;
-                    s_n[cur ^ 1] = 2 * done;
-                }
-            }
-            __syncthreads();
-            const int32_t level_base = s_base;
-            // pass B: leaves to tree order, splits to the other buffer
-            for (int k = w; k < nn; k += kSubWaves) {
-                const SubNode e = E[cur][k];
-                if (!e.kind) {
-                    for (int i = lane; i < e.n; i += kWave) {
-                        a.fin[first + e.lf + i] = P[buf][e.lf + i];
-                        a.leaf_of[first + e.lf + i] = e.id;
-                    }
-                    if (lane == 0) {
-                        // src/potential.c:100-107: f64 sums in tree order
-                        double sx = 0.0, sy = 0.0, sz = 0.0;
-                        for (int i = 0; i < e.n; i++) {
-                            const float4 p = P[buf][e.lf + i];
-                            sx += (double)p.x;
-                            sy += (double)p.y;
-                            sz += (double)p.z;
-                        }
-                        const double n = (double)e.n;
-                        Node *me = a.nodes + e.id;
-                        me->com[0] = sx != 0.0 ? (float)(sx / n) : 0.0f;
-                        me->com[1] = sy != 0.0 ? (float)(sy / n) : 0.0f;
-                        me->com[2] = sz != 0.0 ? (float)(sz / n) : 0.0f;
-                        me->mass = (float)(particle_mass * n);
-                    }
-                    continue;
-                }
-                const int slot = e.child - level_base;
-                const float inf = __int_as_float(0x7f800000), ninf = __int_as_float((int)0xff800000);
-                float lx0 = inf, ly0 = inf, lz0 = inf, hx0 = ninf, hy0 = ninf, hz0 = ninf;
-                float lx1 = inf, ly1 = inf, lz1 = inf, hx1 = ninf, hy1 = ninf, hz1 = ninf;
-                int32_t bl = 0, br = 0;
-                for (int c = 0; c < e.n; c += kWave) {
-                    const int i = c + lane;
-                    const bool v = i < e.n;
-                    const float4 p = v ? P[buf][e.lf + i] : make_float4(0.0f, 0.0f, 0.0f, 0.0f);
-                    const bool f = v && comp(p, e.dim) <= e.lim;
-                    const bool g = v && !f;
-                    const uint64_t mL = ballot(f), mR = ballot(g);
+                const int excl = incl - cnt;
+                const int total = __shfl(incl, kSubChunks - 1, 64);
+                if (lane < kSubChunks) s_pre[lane] = excl;
+                if (lane == kSubChunks) s_pre[kSubChunks] = total;
+                // left flags in [0, x)
+                auto S = [&](int x) {
+                    const int c = x >> 6;
+                    const int b = __shfl(excl, c < kSubChunks ? c : 0, 64);
+                    return (c < kSubChunks ? b : total) +
+                           __popcll(s_mask[c] & lanes_below(x & 63));
+                };
+                int M = c0;
+                for (int k0 = 0; k0 < nn; k0 += kWave) {
+                    const int kk = k0 + lane;
+                    const bool v = kk < nn;
+                    SubEntry en;
+                    en.lf = en.n = 0;
+                    if (v) en = E[cur][kk];
+                    const int s0 = S(en.lf), s1 = S(en.lf + en.n);
+                    const int nl = s1 - s0;
+                    // src/fast3tree.c:805-809: a split with an empty side is a leaf
+                    const bool split = v && en.n > kLeafPoints && nl > 0 && nl < en.n;
+                    const uint64_t ms = ballot(split);
+                    const int C = M + 2 * rank_below(ms);
                     if (v) {
-                        const int d = f ? e.lf + bl + rank_below(mL) : e.lf + e.nl + br + rank_below(mR);
-                        P[buf ^ 1][d] = p;
+                        E[cur][kk].nl = (int16_t)nl;
+                        E[cur][kk].kind = split ? 1 : 0;
+                        E[cur][kk].child = split ? (int16_t)C : (int16_t)-1;
+                        s_left[en.loc] = split ? (int16_t)C : (int16_t)-1;
                     }
-                    if (f) {
-                        lx0 = fminf(lx0, p.x); hx0 = fmaxf(hx0, p.x);
-                        ly0 = fminf(ly0, p.y); hy0 = fmaxf(hy0, p.y);
-                        lz0 = fminf(lz0, p.z); hz0 = fmaxf(hz0, p.z);
+                    if (split) {
+                        const int slot = C - c0;
+                        SubEntry L, R;
+                        L.loc = (int16_t)C;
+                        L.lf = en.lf;
+                        L.n = (int16_t)nl;
+                        L.skip = (int16_t)(C + 1);
+                        R.loc = (int16_t)(C + 1);
+                        R.lf = (int16_t)(en.lf + nl);
+                        R.n = (int16_t)(en.n - nl);
+                        R.skip = en.skip;
+                        L.nl = R.nl = 0;
+                        L.child = R.child = -1;
+                        L.kind = R.kind = 0;
+                        L.dim = R.dim = 0;
+                        L.lim = R.lim = 0.0f;
+                        E[nxt][slot] = L;
+                        E[nxt][slot + 1] = R;
+                        for (int d = 0; d < 3; d++) {
+                            BOX[slot][d] = BOX[slot + 1][d] = float_key(inf);
+                            BOX[slot][3 + d] = BOX[slot + 1][3 + d] = float_key(ninf);
+                        }
+                        a.nodes[gid(en.loc)].left = gid(C);
+                        a.parent[gid(C)] = a.parent[gid(C + 1)] = gid(en.loc);
                     }
-                    if (g) {
-                        lx1 = fminf(lx1, p.x); hx1 = fmaxf(hx1, p.x);
-                        ly1 = fminf(ly1, p.y); hy1 = fmaxf(hy1, p.y);
-                        lz1 = fminf(lz1, p.z); hz1 = fmaxf(hz1, p.z);
-                    }
-                    bl += __popcll(mL);
-                    br += __popcll(mR);
+                    M += 2 * __popcll(ms);
                 }
-                const float lo0[3] = {wave_min(lx0), wave_min(ly0), wave_min(lz0)};
-                const float hi0[3] = {wave_max(hx0), wave_max(hy0), wave_max(hz0)};
-                const float lo1[3] = {wave_min(lx1), wave_min(ly1), wave_min(lz1)};
-                const float hi1[3] = {wave_max(hx1), wave_max(hy1), wave_max(hz1)};
                 if (lane == 0) {
-                    const int32_t C = e.child;
-                    write_node(a.nodes + C, lo0, hi0, first + e.lf, e.nl, C + 1);
-                    write_node(a.nodes + C + 1, lo1, hi1, first + e.lf + e.nl, e.n - e.nl, e.skip);
-                    a.parent[C] = a.parent[C + 1] = e.id;
-                    a.nodes[e.id].left = C;
-                    E[cur ^ 1][slot] = sub_entry(C, e.lf, e.nl, lo0, hi0, C + 1);
-                    E[cur ^ 1][slot + 1] = sub_entry(C + 1, e.lf + e.nl, e.n - e.nl, lo1, hi1, e.skip);
-                    made[s_made + slot] = C;
-                    made[s_made + slot + 1] = C + 1;
+                    s_M = M;
+                    s_nn[nxt] = M - c0;
+                    if (M > c0) {
+                        s_nlev = s_nlev + 1;
+                        s_lvl[s_nlev + 1] = (uint16_t)M;
+                    }
                 }
             }
             __syncthreads();
-            if (tid == 0) {
-                s_made += s_n[cur ^ 1];
-                lvl_end[s_levels++] = (uint16_t)s_made;
+            // 3. scatter: split points to their child's place in the next
+            //    buffer; leaf points are in tree order already
+            auto Sx = [&](int x) {
+                const int c = x >> 6;
+                return s_pre[c] + __popcll(s_mask[c] & lanes_below(x & 63));
+            };
+            for (int r = 0; r < kSubRounds; r++) {
+                const int i = r * kSubThreads + tid;
+                if (i >= n) break;
+                const float4 p = P[buf][i];
+                const uint8_t e = Q[cur][i];
+                int dst = i;
+                uint8_t q = kNoEntry;
+                if (e != kNoEntry) {
+                    const SubEntry en = E[cur][e];
+                    if (en.kind) {
+                        const bool gl = comp(p, en.dim) <= en.lim;
+                        const int sl = Sx(i) - Sx(en.lf);
+                        dst = gl ? en.lf + sl : en.lf + en.nl + (i - en.lf - sl);
+                        q = (uint8_t)(en.child - c0 + (gl ? 0 : 1));
+                    } else {
+                        a.fin[first + i] = p;
+                        a.leaf_of[first + i] = gid(en.loc);
+                    }
+                }
+                P[buf ^ 1][dst] = p;
+                Q[nxt][dst] = q;
             }
-            cur ^= 1;
-            buf ^= 1;
             __syncthreads();
+            // 4. tight boxes of the next level's nodes
+            const int nn_next = s_nn[nxt];
+            if (nn_next > 0) {
+                for (int r = 0; r < kSubRounds; r++) {
+                    const int i = r * kSubThreads + tid;
+                    const uint8_t q = i < n ? Q[nxt][i] : kNoEntry;
+                    const bool fresh = q != kNoEntry;
+                    if (!ballot(fresh)) continue;
+                    const float4 p = fresh ? P[buf ^ 1][i] : make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+                    const int lf = fresh ? E[nxt][q].lf : i;
+                    const int chunk0 = i - lane;
+                    const int seg = lf > chunk0 ? lf - chunk0 : 0;
+                    const int sg = fresh ? seg : lane;
+                    auto mn = [](float x, float y) { return fminf(x, y); };
+                    auto mx = [](float x, float y) { return fmaxf(x, y); };
+                    const float l0 = seg_scan(p.x, sg, mn), l1 = seg_scan(p.y, sg, mn),
+                                l2 = seg_scan(p.z, sg, mn);
+                    const float h0 = seg_scan(p.x, sg, mx), h1 = seg_scan(p.y, sg, mx),
+                                h2 = seg_scan(p.z, sg, mx);
+                    const int qn = __shfl_down((int)q, 1, 64);
+                    const bool last = fresh && (lane == kWave - 1 || i + 1 >= n || qn != (int)q);
+                    if (last) {
+                        atomicMin(&BOX[q][0], float_key(l0));
+                        atomicMin(&BOX[q][1], float_key(l1));
+                        atomicMin(&BOX[q][2], float_key(l2));
+                        atomicMax(&BOX[q][3], float_key(h0));
+                        atomicMax(&BOX[q][4], float_key(h1));
+                        atomicMax(&BOX[q][5], float_key(h2));
+                    }
+                }
+            }
+            __syncthreads();
+            // 5. the next level's node records and split planes, a thread each
+            for (int c = tid; c < nn_next; c += kSubThreads) {
+                SubEntry &en = E[nxt][c];
+                float lo[3], hi[3];
+                for (int d = 0; d < 3; d++) {
+                    lo[d] = key_float(BOX[c][d]);
+                    hi[d] = key_float(BOX[c][3 + d]);
+                }
+                const int dim = split_dim(lo, hi);
+                en.dim = (uint8_t)dim;
+                en.lim = split_lim(lo, hi, dim);
+                write_node(a.nodes + gid(en.loc), lo, hi, first + en.lf, en.n, gskip(en.skip));
+            }
+            __syncthreads();
+            cur = nxt;
+            buf ^= 1;
         }
-        // unused reserved ids become empty nodes
-        for (int32_t j = s_pool + tid; j < s_pool_end; j += kSubThreads) write_empty_node(a, j);
+        if (base < 0) {  // the root stays a leaf over the points in input order
+            for (int i = tid; i < n; i += kSubThreads) {
+                a.fin[first + i] = P[0][i];
+                a.leaf_of[first + i] = t.node;
+            }
+        }
+        // centres of mass and dmin (src/potential.c:71-124); P[buf] holds the
+        // final order, P[buf ^ 1] is free for com (x, y, z) and mass
+        const int M = s_M, nlev = s_nlev;
+        const float4 *F = P[buf];
+        float4 *CM = P[buf ^ 1];
+        auto cm_get = [&](int j) {
+            if (j < kSmallMax) return CM[j];
+            const Node *np = a.nodes + gid(j);
+            return make_float4(np->com[0], np->com[1], np->com[2], np->mass);
+        };
+        auto cm_put = [&](int j, float4 v) {
+            if (j < kSmallMax) CM[j] = v;
+            Node *np = a.nodes + gid(j);
+            np->com[0] = v.x;
+            np->com[1] = v.y;
+            np->com[2] = v.z;
+            np->mass = v.w;
+        };
+        int32_t my_lf[kSubPerThread], my_n[kSubPerThread];
+#pragma unroll
+        for (int q = 0; q < kSubPerThread; q++) {
+            const int j = tid + q * kSubThreads;
+            my_lf[q] = my_n[q] = 0;
+            if (j < M) {
+                const Node *np = a.nodes + gid(j);
+                my_lf[q] = np->first - first;
+                my_n[q] = np->count;
+            }
+        }
+        // leaves: f64 sums in tree order (src/potential.c:100-107)
+#pragma unroll
+        for (int q = 0; q < kSubPerThread; q++) {
+            const int j = tid + q * kSubThreads;
+            if (j >= M || s_left[j] >= 0) continue;
+            double sx = 0.0, sy = 0.0, sz = 0.0;
+            for (int i = my_lf[q]; i < my_lf[q] + my_n[q]; i++) {
+                const float4 p = F[i];
+                sx += (double)p.x;
+                sy += (double)p.y;
+                sz += (double)p.z;
+            }
+            const double dn = (double)my_n[q];
+            cm_put(j, make_float4(sx != 0.0 ? (float)(sx / dn) : 0.0f,
+                                  sy != 0.0 ? (float)(sy / dn) : 0.0f,
+                                  sz != 0.0 ? (float)(sz / dn) : 0.0f, (float)(particle_mass * dn)));
+        }
         __threadfence_block();
         __syncthreads();
-        // centres of mass of the internal nodes, deepest level first
-        // (src/potential.c:115-122: float, right term first)
-        for (int lv = s_levels - 1; lv >= 0; lv--) {
-            const int32_t b0 = lv ? lvl_end[lv - 1] : 0, b1 = lvl_end[lv];
-            for (int32_t j = b0 + tid; j < b1; j += kSubThreads) {
-                Node *me = a.nodes + made[j];
-                const int32_t L = me->left;
-                if (L < 0) continue;
-                const Node *l = a.nodes + L, *r = a.nodes + L + 1;
-                const float ml = l->mass, mr = r->mass;
-                const float m = ml + mr;
-                for (int k = 0; k < 3; k++) {
-                    const float rr = r->com[k] * mr;
-                    const float ll = l->com[k] * ml;
-                    me->com[k] = m != 0.0f ? (rr + ll) / m : 0.0f;
+        // internal nodes, deepest level first (src/potential.c:115-122:
+        // float, right term first)
+        for (int L = nlev; L >= 0; L--) {
+            for (int j = s_lvl[L] + tid; j < s_lvl[L + 1]; j += kSubThreads) {
+                const int cl = s_left[j];
+                if (cl < 0) continue;
+                const float4 l = cm_get(cl), r = cm_get(cl + 1);
+                const float m = l.w + r.w;
+                float c[3];
+                const float lc[3] = {l.x, l.y, l.z}, rc[3] = {r.x, r.y, r.z};
+                for (int d = 0; d < 3; d++) {
+                    const float rr = rc[d] * r.w;
+                    const float ll = lc[d] * l.w;
+                    c[d] = m != 0.0f ? (rr + ll) / m : 0.0f;
                 }
-                me->mass = m;
+                cm_put(j, make_float4(c[0], c[1], c[2], m));
             }
             __threadfence_block();
             __syncthreads();
         }
-        // dmin over the final order (src/potential.c:71-82)
-        for (int32_t i = tid; i < count; i += kSubThreads) P[0][i] = a.fin[first + i];
-        __syncthreads();
-        // nodes of at most 64 points: a thread each, summing in tree order
-        for (int32_t j = tid; j < s_made; j += kSubThreads) {
-            Node *np = a.nodes + made[j];
-            const int32_t cnt = np->count;
-            if (cnt > kWave) continue;
-            const int32_t rel = np->first - first;
-            const float com[3] = {np->com[0], np->com[1], np->com[2]};
+        // dmin: nodes of at most 64 points a thread each, in tree order
+#pragma unroll
+        for (int q = 0; q < kSubPerThread; q++) {
+            const int j = tid + q * kSubThreads;
+            if (j >= M || my_n[q] > kWave) continue;
+            const float4 c4 = cm_get(j);
+            const float com[3] = {c4.x, c4.y, c4.z};
             double big = 0.0, sum = 0.0;
-            for (int32_t i = rel; i < rel + cnt; i++) {
-                const double d = d2_ref(com, P[0][i]);
+            for (int i = my_lf[q]; i < my_lf[q] + my_n[q]; i++) {
+                const double d = d2_ref(com, F[i]);
                 big = d > big ? d : big;
                 sum += d;
             }
-            finish_dmin(np, big, sum, cnt);
+            finish_dmin(a.nodes + gid(j), big, sum, my_n[q]);
         }
-        // larger nodes: a wavefront each; each wavefront takes 64 nodes at a
-        // time, their records loaded lane-parallel, then walked one by one
-        for (int32_t j0 = w * kWave; j0 < s_made; j0 += kSubWaves * kWave) {
-            const int32_t jj = j0 + lane;
-            int32_t my_rel = 0, my_cnt = 0;
-            float my_c0 = 0.0f, my_c1 = 0.0f, my_c2 = 0.0f;
-            if (jj < s_made) {
-                const Node *np = a.nodes + made[jj];
-                my_rel = np->first - first;
-                my_cnt = np->count;
-                my_c0 = np->com[0];
-                my_c1 = np->com[1];
-                my_c2 = np->com[2];
+        // larger nodes a wavefront each; their ranges are read lane-parallel
+        for (int j0 = w * kWave; j0 < M; j0 += kSubThreads) {
+            const int jj = j0 + lane;
+            int32_t rl = 0, rn = 0;
+            if (jj < M) {
+                const Node *np = a.nodes + gid(jj);
+                rl = np->first - first;
+                rn = np->count;
             }
-            const int nb = min(kWave, s_made - j0);
+            const int nb = min(kWave, M - j0);
             for (int q = 0; q < nb; q++) {
-                const int32_t rel = __shfl(my_rel, q, 64), cnt = __shfl(my_cnt, q, 64);
-                if (cnt <= kWave) continue;  // done above
-                const float com[3] = {__shfl(my_c0, q, 64), __shfl(my_c1, q, 64),
-                                      __shfl(my_c2, q, 64)};
+                const int32_t qn = __shfl(rn, q, 64);
+                if (qn <= kWave) continue;
+                const int32_t ql = __shfl(rl, q, 64);
+                const float4 c4 = cm_get(j0 + q);
+                const float com[3] = {c4.x, c4.y, c4.z};
                 double big = 0.0, sum = 0.0;
-                for (int32_t i = lane; i < cnt; i += kWave) {
-                    const double d = d2_ref(com, P[0][rel + i]);
+                for (int32_t i = lane; i < qn; i += kWave) {
+                    const double d = d2_ref(com, F[ql + i]);
                     big = d > big ? d : big;
                     sum += d;
                 }
                 big = wave_max(big);
                 sum = wave_sum(sum);
-                if (lane == 0) finish_dmin(a.nodes + made[j0 + q], big, sum, cnt);
+                if (lane == 0) finish_dmin(a.nodes + gid(j0 + q), big, sum, qn);
             }
         }
         __syncthreads();
@@ -653,16 +765,6 @@ __global__ __launch_bounds__(kSubThreads) void k_subtree(const float4 *__restric
 // in tree order, the order of the reference's loops.
 
 constexpr int kWaveNodes = 2 * kWaveTask;  // a tree over n points has at most 2n - 1 nodes
-constexpr int kRootSkip = -2;              // local skip pointer: the subtree root's own skip
-
-__device__ __forceinline__ uint64_t lanes_below(int i) {
-    return i >= 64 ? ~0ull : ((1ull << i) - 1ull);
-}
-
-// bits [lf, lf + n) of a wavefront mask (lf + n <= 64)
-__device__ __forceinline__ uint64_t lane_run(int lf, int n) {
-    return (n >= 64 ? ~0ull : ((1ull << n) - 1ull)) << lf;
-}
 
 __global__ __launch_bounds__(kWave) void k_subtree_wave(const float4 *__restrict__ A,
                                                         const float4 *__restrict__ B,

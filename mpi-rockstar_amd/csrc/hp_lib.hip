@@ -1094,6 +1094,9 @@ __attribute__((visibility("default"))) int64_t hp_debug_tree(const struct potent
     DevBuf &dpo = e->srt;
     if ((r = dpo.ensure(sizeof(struct potential) * (n + 1)))) return r;
     HP_TRY(hipMemcpyAsync(dpo.p, po, sizeof(struct potential) * n, hipMemcpyHostToDevice, e->stream));
+    // node ids a subtree reserves but does not use are never written: clear
+    // the array so that the export shows them as empty nodes
+    HP_TRY(hipMemsetAsync(e->nodes.p, 0, sizeof(Node) * e->cap_nodes, e->stream));
     e->st = hp_stats();
     e->st.halos = 1;
     int next = 0;
