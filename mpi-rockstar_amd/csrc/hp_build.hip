@@ -426,6 +426,8 @@ __global__ __launch_bounds__(kSubThreads) void k_subtree(const float4 *__restric
             P[0][i] = src[first + i];
             Q[0][i] = 0;
         }
+        if (tid <= kSubChunks) s_mask[tid] = 0;  // chunks past n stay empty
+        const int rounds = (n + kSubThreads - 1) / kSubThreads;
         if (tid == 0) {
             SubEntry e;
             e.loc = 0;
@@ -444,7 +446,6 @@ __global__ __launch_bounds__(kSubThreads) void k_subtree(const float4 *__restric
             s_lvl[0] = 0;
             s_lvl[1] = 1;
             s_left[0] = -1;
-            s_mask[kSubChunks] = 0;
             const int32_t R = 2 * n - 2;
             int32_t base = (int32_t)alloc_nodes_and_small(a, R, 0);
             if (base + R > a.cap_nodes) {  // out of node ids: the root stays a leaf
@@ -465,7 +466,7 @@ __global__ __launch_bounds__(kSubThreads) void k_subtree(const float4 *__restric
             const int nxt = cur ^ 1;
             const int c0 = s_M;  // first local id of the next level
             // 1. left flags (src/fast3tree.c:751-774), one ballot per chunk
-            for (int r = 0; r < kSubRounds; r++) {
+            for (int r = 0; r < rounds; r++) {
                 const int i = r * kSubThreads + tid;
                 bool gl = false;
                 if (i < n) {
@@ -560,7 +561,7 @@ __global__ __launch_bounds__(kSubThreads) void k_subtree(const float4 *__restric
                 const int c = x >> 6;
                 return s_pre[c] + __popcll(s_mask[c] & lanes_below(x & 63));
             };
-            for (int r = 0; r < kSubRounds; r++) {
+            for (int r = 0; r < rounds; r++) {
                 const int i = r * kSubThreads + tid;
                 if (i >= n) break;
                 const float4 p = P[buf][i];
@@ -586,7 +587,7 @@ __global__ __launch_bounds__(kSubThreads) void k_subtree(const float4 *__restric
             // 4. tight boxes of the next level's nodes
             const int nn_next = s_nn[nxt];
             if (nn_next > 0) {
-                for (int r = 0; r < kSubRounds; r++) {
+                for (int r = 0; r < rounds; r++) {
                     const int i = r * kSubThreads + tid;
                     const uint8_t q = i < n ? Q[nxt][i] : kNoEntry;
                     const bool fresh = q != kNoEntry;

@@ -108,7 +108,7 @@ enum ErrFlag { kErrWalkCap = 1, kErrBadTree = 2, kErrCapacity = 4 };
 
 __device__ __forceinline__ int lane_id() { return (int)__lane_id(); }
 
-__device__ __forceinline__ uint64_t ballot(bool p) { return (uint64_t)__ballot(p); }
+__device__ __forceinline__ uint64_t ballot(bool p) { return __builtin_amdgcn_ballot_w64(p); }
 
 // number of set bits of m below this lane
 __device__ __forceinline__ int rank_below(uint64_t m) {
@@ -175,6 +175,26 @@ __device__ __forceinline__ T wave_max(T v) {
 template <typename T>
 __device__ __forceinline__ T wave_sum(T v) {
     return wave_reduce(v, [](T a, T b) { return a + b; });
+}
+
+// A load through the constant address space: at a wavefront-uniform
+// address it becomes a scalar load.  Only for data that nothing in the same
+// launch writes (the scalar cache is invalidated between launches).
+template <typename T>
+__device__ __forceinline__ T load_const(const T *p) {
+    static_assert(sizeof(T) % 4 == 0, "whole dwords");
+    typedef const __attribute__((address_space(4))) int32_t CInt;
+    const CInt *q = (const CInt *)p;
+    struct Words {
+        int32_t w[sizeof(T) / 4];
+    } v;
+#pragma unroll
+    for (int i = 0; i < (int)(sizeof(T) / 4); i++) v.w[i] = q[i];
+    return __builtin_bit_cast(T, v);
+}
+
+__device__ __forceinline__ float lane_float(float v, int l) {
+    return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), l));
 }
 
 __device__ __forceinline__ float comp(const float4 &p, int d) {

@@ -124,10 +124,10 @@ __global__ __launch_bounds__(256) void k_walk(WalkArgs a) {
     const int32_t gid = (int32_t)((blockIdx.x * blockDim.x + threadIdx.x) / kWave);
     if (gid >= a.ngroups) return;
     const int lane = lane_id();
-    const Group g = a.groups[gid];
-    const int32_t nnodes = uniform(min(a.counters[kNodeCount], a.cap_nodes));
+    const Group g = load_const(a.groups + gid);
+    const int32_t nnodes = min(load_const(a.counters + kNodeCount), a.cap_nodes);
     const int32_t root = uniform(g.root);
-    const Node rn = a.nodes[root];
+    const Node rn = load_const(a.nodes + root);
     const int32_t p = g.p0 + lane;
     const bool inseg = p < g.pend;
     const bool live = (uint32_t)(p - rn.first) < (uint32_t)rn.count;
@@ -155,6 +155,13 @@ __global__ __launch_bounds__(256) void k_walk(WalkArgs a) {
     int since = 0;
     unsigned long long n_vis = 0, n_p2p = 0, n_p2m = 0;
 
+    // the arguments used inside the loop are made resident here, so that the
+    // loop's waits are for its own node loads only
+    const float pm = a.pm, inv_fr = a.inv_fr;
+    const double pm_d = a.pm_d, fr_d = a.fr_d;
+    const float4 *__restrict__ fin = a.fin;
+    const Node *__restrict__ nodes = a.nodes;
+    asm volatile("" ::"s"(pm), "s"(inv_fr), "s"(pm_d), "s"(fr_d), "s"(fin), "s"(nodes));
     while (c >= 0) {
         if (++guard > cap) {
             if (lane == 0) atomicOr(a.err, kErrWalkCap);
@@ -165,22 +172,20 @@ __global__ __launch_bounds__(256) void k_walk(WalkArgs a) {
             if (lane == 0) atomicOr(a.err, kErrBadTree);
             break;
         }
-        const Node nd = a.nodes[c];
+        const Node nd = load_const(nodes + c);
         const bool act = resume == c;
         const bool inr = (uint32_t)(p - nd.first) < (uint32_t)nd.count;
-        const bool leaf = nd.left < 0;
-        const bool cand = act && !inr;
+        const bool cand = act & !inr;
         // acceptance, src/potential.c:126-131
         const float dx = lc0 - nd.com[0], dy = lc1 - nd.com[1], dz = lc2 - nd.com[2];
         const float f = fmaf(dz, dz, fmaf(dy, dy, dx * dx));
         bool acc_ok = f > nd.dmin2;
-        const bool amb = cand && fabsf(f - nd.dmin2) <= nd.dmin2 * 0x1p-20f;
-        if (ballot(amb)) {
-            if (amb) acc_ok = accept_exact(lc0, lc1, lc2, nd.com, nd.dmin2);
+        const bool amb = cand & (fabsf(f - nd.dmin2) <= nd.dmin2 * 0x1p-20f);
+        if (__builtin_expect(ballot(amb) != 0, 0)) {
+            const bool ex = accept_exact(lc0, lc1, lc2, nd.com, nd.dmin2);
+            acc_ok = amb ? ex : acc_ok;
         }
-        const bool p2m = cand && acc_ok;
-        const bool p2p = act && leaf && (inr ? nd.count <= kDegenerate : !acc_ok);
-        const bool open = act && !leaf && (inr || !acc_ok);
+        const bool p2m = cand & acc_ok;
 
         if constexpr (COUNT) {
             n_vis += 1;
@@ -207,39 +212,58 @@ __global__ __launch_bounds__(256) void k_walk(WalkArgs a) {
                 }
             }
         }
-        // particle pairs, softened (src/potential.c:38-55)
+        if (nd.left >= 0) {
+            // an internal node: opened by a lane whose leaf lies inside it
+            // or that does not accept it
+            const bool open = act & (inr | !acc_ok);
+            if (act) resume = open ? nd.left : nd.skip;
+            c = ballot(open) != 0 ? nd.left : nd.skip;
+            continue;
+        }
+        // a leaf: particle pairs, softened (src/potential.c:38-55); the self
+        // leaf only up to 20 points (src/potential.c:144-146).  The leaf's
+        // sources are loaded once, one per lane, and broadcast lane by lane.
+        const bool small = nd.count <= kDegenerate;
+        const bool p2p = act & ((inr & small) | (!inr & !acc_ok));
         if (ballot(p2p)) {
             if constexpr (COUNT) n_p2p += (unsigned long long)wave_sum(p2p ? (int)nd.count - (inr ? 1 : 0) : 0);
-            for (int32_t j0 = 0; j0 < nd.count; j0 += 16) {
-                const int32_t j1 = min(j0 + 16, nd.count);
-                if constexpr (ACC == HP_ACC_F64) {
-                    double s = 0.0;
-                    for (int32_t j = j0; j < j1; j++) {
-                        const float4 q = a.fin[nd.first + j];
-                        double ex = (double)(me.x - q.x), ey = (double)(me.y - q.y),
-                               ez = (double)(me.z - q.z);
-                        double r = sqrt((ex * ex + ey * ey) + ez * ez);
-                        r = r < a.fr_d ? a.fr_d : r;
-                        double t = a.pm_d * (1.0 / r);
-                        s += (p2p && nd.first + j != p) ? t : 0.0;
+            for (int32_t j0 = 0; j0 < nd.count; j0 += kWave) {
+                const int32_t cnt = min(kWave, nd.count - j0);
+                const float4 src = lane < cnt ? fin[nd.first + j0 + lane]
+                                              : make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+                const int32_t self = p - (nd.first + j0);  // this lane's own slot, if any
+                for (int32_t jb = 0; jb < cnt; jb += 16) {
+                    const int32_t je = min(jb + 16, cnt);
+                    if constexpr (ACC == HP_ACC_F64) {
+                        double s = 0.0;
+                        for (int32_t j = jb; j < je; j++) {
+                            const float qx = lane_float(src.x, j), qy = lane_float(src.y, j),
+                                        qz = lane_float(src.z, j);
+                            double ex = (double)(me.x - qx), ey = (double)(me.y - qy),
+                                   ez = (double)(me.z - qz);
+                            double r = sqrt((ex * ex + ey * ey) + ez * ez);
+                            r = r < fr_d ? fr_d : r;
+                            double t = pm_d * (1.0 / r);
+                            s += (p2p && j != self) ? t : 0.0;
+                        }
+                        acc.add_block(s);
+                    } else {
+                        float s = 0.0f;
+                        for (int32_t j = jb; j < je; j++) {
+                            const float qx = lane_float(src.x, j), qy = lane_float(src.y, j),
+                                        qz = lane_float(src.z, j);
+                            const float ex = me.x - qx, ey = me.y - qy, ez = me.z - qz;
+                            const float r2 = fmaf(ez, ez, fmaf(ey, ey, ex * ex));
+                            const float t = fminf(rsq(r2), inv_fr);
+                            s += (p2p && j != self) ? t : 0.0f;
+                        }
+                        acc.add_block(pm * s);
                     }
-                    acc.add_block(s);
-                } else {
-                    float s = 0.0f;
-                    for (int32_t j = j0; j < j1; j++) {
-                        const float4 q = a.fin[nd.first + j];
-                        const float ex = me.x - q.x, ey = me.y - q.y, ez = me.z - q.z;
-                        const float r2 = fmaf(ez, ez, fmaf(ey, ey, ex * ex));
-                        const float t = fminf(rsq(r2), a.inv_fr);
-                        s += (p2p && nd.first + j != p) ? t : 0.0f;
-                    }
-                    acc.add_block(a.pm * s);
                 }
             }
         }
-        const bool any_open = ballot(open) != 0;
-        if (act) resume = open ? nd.left : nd.skip;
-        c = any_open ? nd.left : nd.skip;
+        if (act) resume = nd.skip;
+        c = nd.skip;
     }
     if constexpr (ACC == HP_ACC_COMPENSATED) acc.fold();
     if (inseg) {
