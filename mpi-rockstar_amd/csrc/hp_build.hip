@@ -185,14 +185,18 @@ __device__ Task make_task(int32_t id, const float *lo, const float *hi, int32_t 
 }
 
 // Queues node `id` (more than 10 points, tight box known) at a small-task
-// slot already allocated.
+// slot already allocated; the task carries the node's range, so that the
+// subtree kernels can pick their tasks without reading the node.
 __device__ void push_small_at(const BuildArgs &a, int32_t slot, int32_t id, const float *lo,
-                              const float *hi, int32_t depth) {
+                              const float *hi, int32_t depth, int32_t first, int32_t count) {
     if (slot >= a.cap_small) {
         atomicOr(&a.counters[kErrFlags], kErrCapacity);
         return;
     }
-    a.small_next[slot] = make_task(id, lo, hi, depth);
+    Task t = make_task(id, lo, hi, depth);
+    t.pad[0] = first;
+    t.pad[1] = count;
+    a.small_next[slot] = t;
 }
 
 // Queues a node of more than big_threshold points with its tiles.
@@ -230,36 +234,90 @@ __device__ __forceinline__ void write_node(Node *nd, const float *lo, const floa
     *nd = n;
 }
 
-// One workgroup per halo: root node, stable removal of non-finite records to
-// the tail (src/fast3tree.c:875-887), root queued or finished as a leaf.
+// Root nodes, one thread per call.  A call with non-finite points is listed
+// for k_roots_bad; every other call gets its root node here, its points in
+// tree order when it is a single leaf, and its root queued as a task.  The
+// subtree-task slots of a wavefront's calls are taken with one atomic.
 __global__ __launch_bounds__(256) void k_roots(const int64_t *__restrict__ hfirst,
                                                const int64_t *__restrict__ hcount,
                                                const int32_t *__restrict__ box,
                                                const int32_t *__restrict__ bad,
-                                               float4 *__restrict__ A, BuildArgs a) {
-    __shared__ int s_cnt[2][4];
-    const int h = blockIdx.x;
-    const int32_t first = (int32_t)hfirst[h];
-    const int32_t count = (int32_t)hcount[h];
-    const int32_t nbad = bad[h];
-    const int32_t nfin = count - nbad;
+                                               const float4 *__restrict__ A,
+                                               int32_t *__restrict__ badlist, int nhalos,
+                                               BuildArgs a) {
+    const int h = blockIdx.x * blockDim.x + threadIdx.x;
+    const bool v = h < nhalos;
+    int32_t first = 0, nfin = 0;
+    bool isbad = false;
+    if (v) {
+        first = (int32_t)hfirst[h];
+        const int32_t count = (int32_t)hcount[h];
+        const int32_t nbad = bad[h];
+        nfin = count - nbad;
+        isbad = nbad > 0;
+    }
+    const bool ok = v && !isbad;
+    const bool small = ok && nfin > kLeafPoints && nfin <= a.big_threshold;
+    const uint64_t ms = ballot(small);
+    int32_t slot0 = 0;
+    if (ms && lane_id() == 0) slot0 = (int32_t)(alloc_nodes_and_small(a, 0, __popcll(ms)) >> 32);
+    slot0 = __shfl(slot0, 0, 64);
+    if (isbad) {
+        const int32_t k = atomicAdd(&a.counters[kBadRoots], 1);
+        badlist[k] = h;
+    }
+    if (!ok) return;
     float lo[3], hi[3];
     for (int k = 0; k < 3; k++) {
         lo[k] = nfin ? key_float(box[6 * h + k]) : 0.0f;
         hi[k] = nfin ? key_float(box[6 * h + 3 + k]) : 0.0f;
     }
-    if (nbad) {
-        // rare path: stable partition finite | non-finite through the final
-        // array, then the finite part back into A for splitting
+    if (nfin <= kLeafPoints) {
+        for (int32_t i = 0; i < nfin; i++) {
+            a.fin[first + i] = A[first + i];
+            a.leaf_of[first + i] = h;
+        }
+    }
+    write_node(a.nodes + h, lo, hi, first, nfin, -1);
+    a.parent[h] = -1;
+    if (nfin > a.big_threshold) push_big(a, h, lo, hi, nfin, 0);
+    else if (small) push_small_at(a, slot0 + rank_below(ms), h, lo, hi, 0, first, nfin);
+}
+
+// The listed calls with non-finite points, one workgroup each: stable
+// removal of the non-finite records to the tail (src/fast3tree.c:875-887),
+// root node, root queued or finished as a leaf.
+__global__ __launch_bounds__(256) void k_roots_bad(const int64_t *__restrict__ hfirst,
+                                                   const int64_t *__restrict__ hcount,
+                                                   const int32_t *__restrict__ box,
+                                                   const int32_t *__restrict__ bad,
+                                                   float4 *__restrict__ A,
+                                                   const int32_t *__restrict__ badlist,
+                                                   BuildArgs a) {
+    __shared__ int s_cnt[2][4];
+    const int32_t nlist = a.counters[kBadRoots];
+    for (int32_t li = blockIdx.x; li < nlist; li += gridDim.x) {
+        const int h = badlist[li];
+        const int32_t first = (int32_t)hfirst[h];
+        const int32_t count = (int32_t)hcount[h];
+        const int32_t nbad = bad[h];
+        const int32_t nfin = count - nbad;
+        float lo[3], hi[3];
+        for (int k = 0; k < 3; k++) {
+            lo[k] = nfin ? key_float(box[6 * h + k]) : 0.0f;
+            hi[k] = nfin ? key_float(box[6 * h + 3 + k]) : 0.0f;
+        }
+        // stable partition finite | non-finite through the final array, then
+        // the finite part back into A for splitting
         int32_t runf = 0, runb = 0;
         const int w = threadIdx.x / kWave;
         for (int32_t base = 0; base < count; base += 256) {
             int32_t i = base + threadIdx.x;
-            bool v = i < count;
-            float4 p = v ? A[first + i] : make_float4(0, 0, 0, 0);
-            bool f = v && finite3(p.x, p.y, p.z);
-            bool b = v && !f;
-            uint64_t mf = ballot(f), mb = ballot(b);
+            bool vv = i < count;
+            float4 p = vv ? A[first + i] : make_float4(0, 0, 0, 0);
+            bool f = vv && finite3(p.x, p.y, p.z);
+            bool bb = vv && !f;
+            uint64_t mf = ballot(f), mb = ballot(bb);
             if (lane_id() == 0) {
                 s_cnt[0][w] = __popcll(mf);
                 s_cnt[1][w] = __popcll(mb);
@@ -275,7 +333,7 @@ __global__ __launch_bounds__(256) void k_roots(const int64_t *__restrict__ hfirs
                 tb += s_cnt[1][j];
             }
             if (f) a.fin[first + runf + of + rank_below(mf)] = p;
-            if (b) {
+            if (bb) {
                 int32_t d = first + nfin + runb + ob + rank_below(mb);
                 a.fin[d] = p;
                 a.leaf_of[d] = -1;
@@ -288,20 +346,19 @@ __global__ __launch_bounds__(256) void k_roots(const int64_t *__restrict__ hfirs
         // A keeps the compact finite points of every call (the direct kernel
         // reads them there; the split levels read them when nfin > 10)
         for (int32_t i = threadIdx.x; i < nfin; i += 256) A[first + i] = a.fin[first + i];
-    } else if (nfin <= kLeafPoints) {
-        for (int32_t i = threadIdx.x; i < nfin; i += 256) a.fin[first + i] = A[first + i];
-    }
-    if (nfin <= kLeafPoints)
-        for (int32_t i = threadIdx.x; i < nfin; i += 256) a.leaf_of[first + i] = h;
-    if (threadIdx.x == 0) {
-        write_node(a.nodes + h, lo, hi, first, nfin, -1);
-        a.parent[h] = -1;
-        if (nfin > a.big_threshold) {
-            push_big(a, h, lo, hi, nfin, 0);
-        } else if (nfin > kLeafPoints) {
-            const uint64_t old = alloc_nodes_and_small(a, 0, 1);
-            push_small_at(a, (int32_t)(old >> 32), h, lo, hi, 0);
+        if (nfin <= kLeafPoints)
+            for (int32_t i = threadIdx.x; i < nfin; i += 256) a.leaf_of[first + i] = h;
+        if (threadIdx.x == 0) {
+            write_node(a.nodes + h, lo, hi, first, nfin, -1);
+            a.parent[h] = -1;
+            if (nfin > a.big_threshold) {
+                push_big(a, h, lo, hi, nfin, 0);
+            } else if (nfin > kLeafPoints) {
+                const uint64_t old = alloc_nodes_and_small(a, 0, 1);
+                push_small_at(a, (int32_t)(old >> 32), h, lo, hi, 0, first, nfin);
+            }
         }
+        __syncthreads();
     }
 }
 
@@ -319,9 +376,9 @@ __device__ void make_children(const BuildArgs &a, const Task &t, const Node &nd,
     a.parent[L] = a.parent[L + 1] = t.node;
     a.nodes[t.node].left = L;
     if (nl > a.big_threshold) push_big(a, L, loL, hiL, nl, t.depth + 1);
-    else if (nl > kLeafPoints) push_small_at(a, P++, L, loL, hiL, t.depth + 1);
+    else if (nl > kLeafPoints) push_small_at(a, P++, L, loL, hiL, t.depth + 1, nd.first, nl);
     if (nr > a.big_threshold) push_big(a, L + 1, loR, hiR, nr, t.depth + 1);
-    else if (nr > kLeafPoints) push_small_at(a, P, L + 1, loR, hiR, t.depth + 1);
+    else if (nr > kLeafPoints) push_small_at(a, P, L + 1, loR, hiR, t.depth + 1, nd.first + nl, nr);
 }
 
 __device__ __forceinline__ int32_t small_pushes(int32_t nl, int32_t nr, int32_t thr) {
@@ -418,9 +475,9 @@ __global__ __launch_bounds__(kSubThreads) void k_subtree(const float4 *__restric
     // nodes of the last split levels) are queued last and should start first
     for (int32_t k = blockIdx.x; k < ntasks; k += gridDim.x) {
         const Task t = tasks[ntasks - 1 - k];
-        const Node root = a.nodes[t.node];
-        const int32_t first = root.first, n = root.count;
+        const int32_t first = t.pad[0], n = t.pad[1];
         if (n <= kWaveTask) continue;  // a wavefront subtree
+        const int32_t root_skip = a.nodes[t.node].skip;
         const float4 *src = (t.depth & 1) ? B : A;
         for (int i = tid; i < n; i += kSubThreads) {
             P[0][i] = src[first + i];
@@ -437,8 +494,8 @@ __global__ __launch_bounds__(kSubThreads) void k_subtree(const float4 *__restric
             e.child = -1;
             e.skip = kRootSkip;
             e.kind = 0;
-            e.dim = (uint8_t)split_dim(root.lo, root.hi);
-            e.lim = split_lim(root.lo, root.hi, e.dim);
+            e.dim = (uint8_t)t.dim;
+            e.lim = t.lim;
             E[0][0] = e;
             s_nn[0] = 1;
             s_M = 1;
@@ -458,7 +515,7 @@ __global__ __launch_bounds__(kSubThreads) void k_subtree(const float4 *__restric
         __syncthreads();
         const int32_t base = s_base;
         auto gid = [&](int j) { return j == 0 ? t.node : base + j - 1; };
-        auto gskip = [&](int s) { return s == kRootSkip ? root.skip : gid(s); };
+        auto gskip = [&](int s) { return s == kRootSkip ? root_skip : gid(s); };
         int cur = 0, buf = 0;
         while (true) {
             const int nn = s_nn[cur];
@@ -782,9 +839,9 @@ __global__ __launch_bounds__(kWave) void k_subtree_wave(const float4 *__restrict
     const int32_t ntasks = min(a.counters[kSmallNext], a.cap_small);
     for (int32_t k = blockIdx.x; k < ntasks; k += gridDim.x) {
         const Task t = tasks[k];
-        const Node root = a.nodes[t.node];
-        const int32_t n = root.count;
+        const int32_t n = t.pad[1];
         if (n > kWaveTask) continue;  // a workgroup subtree
+        const Node root = a.nodes[t.node];
         const float4 *src = (t.depth & 1) ? B : A;
         const int32_t first = root.first;
         const bool v = lane < n;

@@ -99,7 +99,8 @@ enum Counter {
     kTileEnd = 13,
     kLevel = 14,      // levels advanced so far
     kActive = 15,     // the current level has big tasks
-    kNumCounters = 16
+    kBadRoots = 16,   // calls with non-finite points, listed for their own pass
+    kNumCounters = 17
 };
 
 enum ErrFlag { kErrWalkCap = 1, kErrBadTree = 2, kErrCapacity = 4 };

@@ -227,7 +227,7 @@ struct Engine {
     DevBuf hfirst, hcount, root_tiles, groups, direct_blocks, rec_call, scan_tiles, call_tile0,
         call_ntiles;
     // build
-    DevBuf box, bad, A, B, F, leaf_of, nodes, parent, fat, small0, big, tiles, parts,
+    DevBuf box, bad, badlist, A, B, F, leaf_of, nodes, parent, fat, small0, big, tiles, parts,
         splitres, dparts, counters, work, depth_buf;
     // unbind
     DevBuf keys0, keys1, vals0, vals1, hist, hsum, srt, callinfo, tsum, tcand, tile_i32;
@@ -262,7 +262,7 @@ struct Engine {
 
     ~Engine() {
         DevBuf *all[] = {&hfirst, &hcount, &root_tiles, &groups, &direct_blocks, &rec_call,
-                         &scan_tiles, &call_tile0, &call_ntiles, &box, &bad, &A, &B, &F,
+                         &scan_tiles, &call_tile0, &call_ntiles, &box, &bad, &badlist, &A, &B, &F,
                          &leaf_of, &nodes, &parent, &fat, &small0, &big, &tiles,
                          &parts, &splitres, &dparts, &counters, &work, &depth_buf, &keys0, &keys1, &vals0,
                          &vals1, &hist, &hsum, &srt, &callinfo, &tsum, &tcand, &tile_i32};
@@ -291,6 +291,7 @@ struct Engine {
         HP_TRY(hipSetDevice(device));
         HP_ENSURE(box, sizeof(int32_t) * 6 * (H + 1));
         HP_ENSURE(bad, sizeof(int32_t) * (H + 1));
+        HP_ENSURE(badlist, sizeof(int32_t) * (H + 1));
         HP_ENSURE(A, sizeof(float4) * (N + 1));
         HP_ENSURE(B, sizeof(float4) * (N + 1));
         HP_ENSURE(F, sizeof(float4) * (N + 1));
@@ -399,9 +400,14 @@ struct Engine {
         k_fill_i32<<<1, 1, 0, s>>>(counters.as<int32_t>() + kNodeCount, H, 1);
         HP_LAUNCHED();
         if (H > 0) {
-            k_roots<<<H, 256, 0, s>>>(hfirst.as<int64_t>(), hcount.as<int64_t>(),
-                                      box.as<int32_t>(), bad.as<int32_t>(), A.as<float4>(),
-                                      build_args(small0.as<Task>()));
+            k_roots<<<blocks(H, 256), 256, 0, s>>>(hfirst.as<int64_t>(), hcount.as<int64_t>(),
+                                                   box.as<int32_t>(), bad.as<int32_t>(),
+                                                   A.as<float4>(), badlist.as<int32_t>(), H,
+                                                   build_args(small0.as<Task>()));
+            HP_LAUNCHED();
+            k_roots_bad<<<64, 256, 0, s>>>(hfirst.as<int64_t>(), hcount.as<int64_t>(),
+                                           box.as<int32_t>(), bad.as<int32_t>(), A.as<float4>(),
+                                           badlist.as<int32_t>(), build_args(small0.as<Task>()));
             HP_LAUNCHED();
         }
         k_level_advance<<<1, 1, 0, s>>>(counters.as<int32_t>(), depth_buf.as<int32_t>());
