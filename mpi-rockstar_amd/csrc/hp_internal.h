@@ -163,6 +163,28 @@ __device__ __forceinline__ T wave_reduce(T v, Op op) {
     return op(op(read_lane(v, 0), read_lane(v, 16)), op(read_lane(v, 32), read_lane(v, 48)));
 }
 
+// Inclusive prefix sum over the lanes of a wavefront, on DPP moves: row_shr
+// 1, 2, 4, 8 within rows of 16, then row_bcast 15 and 31 across rows.  Call
+// with all 64 lanes active.
+template <typename T>
+__device__ __forceinline__ T wave_scan_add(T v) {
+    const int lane = lane_id(), r = lane & 15;
+    T u;
+    u = dpp_move<0x111>(v);
+    if (r >= 1) v = v + u;
+    u = dpp_move<0x112>(v);
+    if (r >= 2) v = v + u;
+    u = dpp_move<0x114>(v);
+    if (r >= 4) v = v + u;
+    u = dpp_move<0x118>(v);
+    if (r >= 8) v = v + u;
+    u = dpp_move<0x142>(v);  // row_bcast:15, rows 1 and 3
+    if ((lane >> 4) & 1) v = v + u;
+    u = dpp_move<0x143>(v);  // row_bcast:31, rows 2 and 3
+    if (lane >= 32) v = v + u;
+    return v;
+}
+
 template <typename T>
 __device__ __forceinline__ T wave_min(T v) {
     return wave_reduce(v, [](T a, T b) { return b < a ? b : a; });

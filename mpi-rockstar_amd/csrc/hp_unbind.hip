@@ -434,11 +434,7 @@ __global__ __launch_bounds__(256) void k_vel_select(const struct potential *__re
         }
         double y[6];
         for (int k = 0; k < 6; k++) {
-            double s = x[k];
-            for (int o = 1; o < 64; o <<= 1) {
-                const double u = __shfl_up(s, o, 64);
-                if (lane_id() >= o) s += u;
-            }
+            const double s = wave_scan_add(x[k]);
             y[k] = s;
             if (lane_id() == 63) s_w[k][w] = s;
         }
