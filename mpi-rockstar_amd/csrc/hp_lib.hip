@@ -784,21 +784,21 @@ struct Engine {
         HP_LAUNCHED();
         const double rvir_thresh = particle_rvir_dens() * (4.0 * M_PI / 3.0);
         if (nst > 0) {
-            k_rvir_tiles<<<nst, 256, 0, s>>>(S, scan_tiles.as<ScanTile>(), nst, ci, rvir_thresh, ti32);
+            k_rvir_tiles<<<nst, kWave, 0, s>>>(S, scan_tiles.as<ScanTile>(), nst, ci, rvir_thresh, ti32);
             HP_LAUNCHED();
         }
         k_rvir_calls<<<blocks(H, 256), 256, 0, s>>>(S, call_tile0.as<int32_t>(),
                                                     call_ntiles.as<int32_t>(), ti32, H, ci);
         HP_LAUNCHED();
         if (nst > 0) {
-            k_vel_sums<<<nst, 256, 0, s>>>(S, scan_tiles.as<ScanTile>(), nst, ci,
+            k_vel_sums<<<nst, kWave, 0, s>>>(S, scan_tiles.as<ScanTile>(), nst, ci,
                                            tsum.as<Moments>());
             HP_LAUNCHED();
             k_vel_prefix<<<blocks(H, 256), 256, 0, s>>>(call_tile0.as<int32_t>(),
                                                         call_ntiles.as<int32_t>(), H,
                                                         tsum.as<Moments>());
             HP_LAUNCHED();
-            k_vel_select<<<nst, 256, 0, s>>>(S, scan_tiles.as<ScanTile>(), nst, ci,
+            k_vel_select<<<nst, kWave, 0, s>>>(S, scan_tiles.as<ScanTile>(), nst, ci,
                                              tsum.as<Moments>(), tcand.as<Cand>());
             HP_LAUNCHED();
         }
@@ -813,8 +813,8 @@ struct Engine {
         const double hub = 100.0 * hubble_scaling(z1 - 1.0);
         const double hub_a = hub * a;
         if (nst > 0) {
-            k_kinetic_tiles<<<nst, 256, 0, s>>>(S, scan_tiles.as<ScanTile>(), nst, ci, d_halos,
-                                                hub_a, conv, d_bound, ti32);
+            k_kinetic_tiles<<<nst, kWave, 0, s>>>(S, d_po, scan_tiles.as<ScanTile>(), nst, ci,
+                                                d_halos, hub_a, conv, d_bound, ti32);
             HP_LAUNCHED();
         }
         if (d_bound) {
@@ -824,7 +824,6 @@ struct Engine {
         k_nbound_calls<<<blocks(H, 256), 256, 0, s>>>(call_tile0.as<int32_t>(),
                                                       call_ntiles.as<int32_t>(), ti32, H, d_halos);
         HP_LAUNCHED();
-        HP_TRY(hipMemcpyAsync(d_po, S, sizeof(struct potential) * N, hipMemcpyDeviceToDevice, s));
         return HP_OK;
     }
 };

@@ -189,6 +189,7 @@ __global__ void k_gather_sorted(const struct potential *__restrict__ po,
 struct CallInfo {
     int32_t first;      // sorted records of the call: [first, first + kept)
     int32_t kept;
+    int32_t count;      // all records of the call (the potential-only ones follow)
     int32_t rvir_max;   // -1: no virial prefix
     int32_t core_max;
 };
@@ -210,6 +211,7 @@ __global__ void k_call_kept(const uint64_t *__restrict__ skeys, const int64_t *_
     CallInfo c;
     c.first = first;
     c.kept = lo;
+    c.count = (int32_t)hcount[h];
     c.rvir_max = -1;
     c.core_max = 100;
     ci[h] = c;
@@ -248,28 +250,26 @@ __device__ __forceinline__ int32_t block_sum_i32(int32_t v) {
 // rvir_max (src/properties.c:173-177): the largest j < kept with
 // j^2 > r2_j^3 (4pi/3 rho_vir)^2, r2^3 in float, the rest in double.
 // Per tile, then per call.
-__global__ __launch_bounds__(256) void k_rvir_tiles(const struct potential *__restrict__ srt,
-                                                    const ScanTile *__restrict__ tiles,
-                                                    int32_t ntiles,
-                                                    const CallInfo *__restrict__ ci,
-                                                    double rvir_thresh,
-                                                    int32_t *__restrict__ tile_rvir) {
+__global__ __launch_bounds__(kWave) void k_rvir_tiles(const struct potential *__restrict__ srt,
+                                                      const ScanTile *__restrict__ tiles,
+                                                      int32_t ntiles,
+                                                      const CallInfo *__restrict__ ci,
+                                                      double rvir_thresh,
+                                                      int32_t *__restrict__ tile_rvir) {
     const int32_t t = blockIdx.x;
     if (t >= ntiles) return;
     const ScanTile st = tiles[t];
     const CallInfo c = ci[st.call];
+    const int32_t end = min(st.off + kScanTile, c.kept);
     int32_t best = -1;
-    for (int r = 0; r < kScanTile / 256; r++) {
-        const int32_t j = st.off + r * 256 + threadIdx.x;
-        if (j < c.kept) {
-            const float r2 = srt[c.first + j].r2;
-            const float r6 = (r2 * r2) * r2;
-            if ((double)((int64_t)j * (int64_t)j) > (double)r6 * (rvir_thresh * rvir_thresh))
-                best = j > best ? j : best;
-        }
+    for (int32_t j = st.off + lane_id(); j < end; j += kWave) {
+        const float r2 = srt[c.first + j].r2;
+        const float r6 = (r2 * r2) * r2;
+        if ((double)((int64_t)j * (int64_t)j) > (double)r6 * (rvir_thresh * rvir_thresh))
+            best = j > best ? j : best;
     }
-    best = block_max_i32(best);
-    if (threadIdx.x == 0) tile_rvir[t] = best;
+    best = wave_max(best);
+    if (lane_id() == 0) tile_rvir[t] = best;
 }
 
 // per call: rvir_max over the tiles, then core_max, the last j with
@@ -307,20 +307,18 @@ struct Moments {
 };
 
 // per-tile sums of v and v^2 over the tile's part of [0, rvir_max)
-__global__ __launch_bounds__(256) void k_vel_sums(const struct potential *__restrict__ srt,
-                                                  const ScanTile *__restrict__ tiles,
-                                                  int32_t ntiles, const CallInfo *__restrict__ ci,
-                                                  Moments *__restrict__ tsum) {
-    __shared__ double s_m[6][4];
+__global__ __launch_bounds__(kWave) void k_vel_sums(const struct potential *__restrict__ srt,
+                                                    const ScanTile *__restrict__ tiles,
+                                                    int32_t ntiles,
+                                                    const CallInfo *__restrict__ ci,
+                                                    Moments *__restrict__ tsum) {
     const int32_t t = blockIdx.x;
     if (t >= ntiles) return;
     const ScanTile st = tiles[t];
     const CallInfo c = ci[st.call];
-    const int32_t lim = c.rvir_max;
+    const int32_t end = min(st.off + kScanTile, c.rvir_max);
     double m[6] = {0, 0, 0, 0, 0, 0};
-    for (int r = 0; r < kScanTile / 256; r++) {
-        const int32_t j = st.off + r * 256 + threadIdx.x;
-        if (j >= lim) break;
+    for (int32_t j = st.off + lane_id(); j < end; j += kWave) {
         const float *v = srt[c.first + j].pos + 3;
         for (int k = 0; k < 3; k++) {
             const double x = (double)v[k];
@@ -328,18 +326,12 @@ __global__ __launch_bounds__(256) void k_vel_sums(const struct potential *__rest
             m[3 + k] += x * x;
         }
     }
-    const int w = threadIdx.x / kWave;
-    for (int k = 0; k < 6; k++) {
-        const double s = wave_sum(m[k]);
-        if (lane_id() == 0) s_m[k][w] = s;
-    }
-    __syncthreads();
-    if (threadIdx.x < 6) {
-        double s = s_m[threadIdx.x][0] + s_m[threadIdx.x][1];
-        s = s + s_m[threadIdx.x][2];
-        s = s + s_m[threadIdx.x][3];
-        if (threadIdx.x < 3) tsum[t].s1[threadIdx.x] = s;
-        else tsum[t].s2[threadIdx.x - 3] = s;
+    for (int k = 0; k < 6; k++) m[k] = wave_sum(m[k]);
+    if (lane_id() == 0) {
+        for (int k = 0; k < 3; k++) {
+            tsum[t].s1[k] = m[k];
+            tsum[t].s2[k] = m[3 + k];
+        }
     }
 }
 
@@ -398,32 +390,31 @@ __device__ __forceinline__ Cand wave_better(Cand c) {
 }
 
 // per tile: inclusive prefix moments per record -> var_i / ((i - 3) i), and
-// the tile's best candidates for the core window and the bulk window
-__global__ __launch_bounds__(256) void k_vel_select(const struct potential *__restrict__ srt,
-                                                    const ScanTile *__restrict__ tiles,
-                                                    int32_t ntiles,
-                                                    const CallInfo *__restrict__ ci,
-                                                    const Moments *__restrict__ tsum,
-                                                    Cand *__restrict__ tcand) {
-    __shared__ double s_run[6];
-    __shared__ double s_w[6][4];
-    __shared__ Cand s_c[2][4];
+// the tile's best candidates for the core window and the bulk window.  One
+// wavefront per tile, rounds of 64 records carried lane 63 to the next.
+__global__ __launch_bounds__(kWave) void k_vel_select(const struct potential *__restrict__ srt,
+                                                      const ScanTile *__restrict__ tiles,
+                                                      int32_t ntiles,
+                                                      const CallInfo *__restrict__ ci,
+                                                      const Moments *__restrict__ tsum,
+                                                      Cand *__restrict__ tcand) {
     const int32_t t = blockIdx.x;
     if (t >= ntiles) return;
     const ScanTile st = tiles[t];
     const CallInfo c = ci[st.call];
     const int32_t lim = c.rvir_max;
     const int32_t core_lim = min(c.core_max, lim);
-    if (threadIdx.x < 6)
-        s_run[threadIdx.x] = threadIdx.x < 3 ? tsum[t].s1[threadIdx.x] : tsum[t].s2[threadIdx.x - 3];
+    const int32_t end = min(st.off + kScanTile, lim);
+    const int lane = lane_id();
+    double run[6];
+    for (int k = 0; k < 3; k++) {
+        run[k] = tsum[t].s1[k];
+        run[3 + k] = tsum[t].s2[k];
+    }
     Cand bc = no_cand(), bb = no_cand();
-    const int w = threadIdx.x / kWave;
-    __syncthreads();
-    // only the rounds that hold records below rvir_max (block-uniform)
-    const int rounds = min(kScanTile / 256, max(0, (lim - st.off + 255) / 256));
-    for (int r = 0; r < rounds; r++) {
-        const int32_t j = st.off + r * 256 + threadIdx.x;
-        const bool v = j < lim;
+    for (int32_t base = st.off; base < end; base += kWave) {
+        const int32_t j = base + lane;
+        const bool v = j < end;
         double x[6] = {0, 0, 0, 0, 0, 0};
         if (v) {
             const float *vel = srt[c.first + j].pos + 3;
@@ -434,17 +425,8 @@ __global__ __launch_bounds__(256) void k_vel_select(const struct potential *__re
         }
         double y[6];
         for (int k = 0; k < 6; k++) {
-            const double s = wave_scan_add(x[k]);
-            y[k] = s;
-            if (lane_id() == 63) s_w[k][w] = s;
-        }
-        __syncthreads();
-        double tot[6];
-        for (int k = 0; k < 6; k++) {
-            double off = s_run[k];
-            for (int q = 0; q < w; q++) off += s_w[k][q];
-            y[k] += off;
-            tot[k] = s_run[k] + s_w[k][0] + s_w[k][1] + s_w[k][2] + s_w[k][3];
+            y[k] = wave_scan_add(x[k]) + run[k];
+            run[k] = read_lane(y[k], kWave - 1);
         }
         if (v && j >= 9) {
             const double n = (double)(j + 1);
@@ -457,25 +439,12 @@ __global__ __launch_bounds__(256) void k_vel_select(const struct potential *__re
             if (j < core_lim) bc = better(bc, me);
             bb = better(bb, me);
         }
-        __syncthreads();
-        if (threadIdx.x < 6) s_run[threadIdx.x] = tot[threadIdx.x];
-        __syncthreads();
     }
     bc = wave_better(bc);
     bb = wave_better(bb);
-    if (lane_id() == 0) {
-        s_c[0][w] = bc;
-        s_c[1][w] = bb;
-    }
-    __syncthreads();
-    if (threadIdx.x == 0) {
-        Cand a = s_c[0][0], b = s_c[1][0];
-        for (int q = 1; q < 4; q++) {
-            a = better(a, s_c[0][q]);
-            b = better(b, s_c[1][q]);
-        }
-        tcand[2 * t] = a;
-        tcand[2 * t + 1] = b;
+    if (lane == 0) {
+        tcand[2 * t] = bc;
+        tcand[2 * t + 1] = bb;
     }
 }
 
@@ -544,44 +513,48 @@ __global__ void k_vel_finish(const struct potential *__restrict__ srt,
 
 // src/potential.c:194-211 on the sorted kept records of each call, about
 // corevel (sub-halos) or bulkvel (hosts); bound flag !(pe < ke)
-// (src/properties.c:403).  One workgroup per scan tile; bound counts per tile.
-__global__ __launch_bounds__(256) void k_kinetic_tiles(struct potential *__restrict__ srt,
-                                                       const ScanTile *__restrict__ tiles,
-                                                       int32_t ntiles,
-                                                       const CallInfo *__restrict__ ci,
-                                                       const hp_halo *__restrict__ halos,
-                                                       double hub_a, double conv,
-                                                       uint8_t *__restrict__ bound,
-                                                       int32_t *__restrict__ tile_nb) {
+// (src/properties.c:403).  One wavefront per scan tile; bound counts per
+// tile.  The records are written to their final place here, the potential-
+// only ones unchanged.
+__global__ __launch_bounds__(kWave) void k_kinetic_tiles(const struct potential *__restrict__ srt,
+                                                         struct potential *__restrict__ out,
+                                                         const ScanTile *__restrict__ tiles,
+                                                         int32_t ntiles,
+                                                         const CallInfo *__restrict__ ci,
+                                                         const hp_halo *__restrict__ halos,
+                                                         double hub_a, double conv,
+                                                         uint8_t *__restrict__ bound,
+                                                         int32_t *__restrict__ tile_nb) {
     const int32_t t = blockIdx.x;
     if (t >= ntiles) return;
     const ScanTile st = tiles[t];
     const CallInfo c = ci[st.call];
     const hp_halo &hh = halos[st.call];
     const float *vc = hh.is_sub ? hh.corevel : hh.bulkvel;
-    const float v0 = vc[0], v1 = vc[1], v2 = vc[2];
-    const float c0 = hh.pos[0], c1 = hh.pos[1], c2 = hh.pos[2];
+    const float cen_v[3] = {vc[0], vc[1], vc[2]};
+    const float cen_x[3] = {hh.pos[0], hh.pos[1], hh.pos[2]};
+    const int32_t end = min(st.off + kScanTile, c.count);
     int32_t nb = 0;
-    for (int r = 0; r < kScanTile / 256; r++) {
-        const int32_t j = st.off + r * 256 + threadIdx.x;
-        if (j >= c.kept) continue;
-        struct potential &rec = srt[c.first + j];
-        const float cen_v[3] = {v0, v1, v2}, cen_x[3] = {c0, c1, c2};
-        float ke = 0.0f;
-        for (int k = 0; k < 3; k++) {
-            const float dvp = rec.pos[k + 3] - cen_v[k];
-            const float dxp = rec.pos[k] - cen_x[k];
-            const double dv = (double)dvp + hub_a * (double)dxp;
-            ke = (float)((double)ke + dv * dv);
+    for (int32_t j = st.off + lane_id(); j < end; j += kWave) {
+        struct potential rec = srt[c.first + j];
+        if (j < c.kept) {
+            float ke = 0.0f;
+            for (int k = 0; k < 3; k++) {
+                const float dvp = rec.pos[k + 3] - cen_v[k];
+                const float dxp = rec.pos[k] - cen_x[k];
+                const double dv = (double)dvp + hub_a * (double)dxp;
+                ke = (float)((double)ke + dv * dv);
+            }
+            ke = (float)((double)ke * conv);
+            rec.ke = ke;
+            const bool b = !(rec.pe < (double)ke);
+            if (bound) bound[c.first + j] = b ? 1 : 0;
+            nb += b ? 1 : 0;
         }
-        ke = (float)((double)ke * conv);
-        rec.ke = ke;
-        const bool b = !(rec.pe < (double)ke);
-        if (bound) bound[c.first + j] = b ? 1 : 0;
-        nb += b ? 1 : 0;
+        out[c.first + j] = rec;
     }
-    nb = block_sum_i32(nb);
-    if (threadIdx.x == 0) tile_nb[t] = nb;
+    nb = wave_sum(nb);
+    if (lane_id() == 0) tile_nb[t] = nb;
 }
 
 // potential-only records: bound = 0
