@@ -322,8 +322,8 @@ struct Engine {
             HP_ENSURE(vals0, sizeof(int32_t) * (N + 1));
             HP_ENSURE(vals1, sizeof(int32_t) * (N + 1));
             const int64_t nt = (N + kSortTile - 1) / kSortTile + 1;
-            HP_ENSURE(hist, sizeof(int32_t) * 256 * nt);
-            HP_ENSURE(hsum, sizeof(int32_t) * (256 * nt / kScanBlock + 2));
+            HP_ENSURE(hist, sizeof(int32_t) * kRadix * nt);
+            HP_ENSURE(hsum, sizeof(int32_t) * (kRadix * nt / kScanBlock + 2));
             HP_ENSURE(srt, sizeof(struct potential) * (N + 1));
             HP_ENSURE(callinfo, sizeof(CallInfo) * (H + 1));
             HP_ENSURE(tsum, sizeof(Moments) * (T.scan_tiles.size() + 1));
@@ -736,10 +736,10 @@ struct Engine {
         const int32_t nt = (int32_t)((n + kSortTile - 1) / kSortTile);
         uint64_t *k0 = keys0.as<uint64_t>(), *k1 = keys1.as<uint64_t>();
         int32_t *v0 = vals0.as<int32_t>(), *v1 = vals1.as<int32_t>();
-        for (int shift = 0; shift < bits; shift += 8) {
+        for (int shift = 0; shift < bits; shift += kRadixBits) {
             k_radix_hist<<<nt, kSortThreads, 0, s>>>(k0, (int32_t)n, shift, nt, hist.as<int32_t>());
             HP_LAUNCHED();
-            const int32_t m = 256 * nt;
+            const int32_t m = kRadix * nt;
             const int32_t nbk = (m + kScanBlock - 1) / kScanBlock;
             k_scan_blocks<<<nbk, 1024, 0, s>>>(hist.as<int32_t>(), m, hsum.as<int32_t>());
             HP_LAUNCHED();

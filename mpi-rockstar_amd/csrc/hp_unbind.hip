@@ -32,6 +32,9 @@ namespace hp {
 constexpr int kSortThreads = 256;
 constexpr int kSortItems = 16;              // per thread per tile
 constexpr int kSortTile = kSortThreads * kSortItems;
+constexpr int kRadixBits = 10;              // digit width: 5 passes cover 50 key bits
+constexpr int kRadix = 1 << kRadixBits;
+constexpr int kRadixPer = kRadix / kSortThreads;  // buckets per thread
 constexpr int kScanTile = 2048;
 
 // ---------------------------------------------------------------- radix sort
@@ -53,16 +56,19 @@ __global__ __launch_bounds__(kSortThreads) void k_radix_hist(const uint64_t *__r
                                                              int32_t n, int shift,
                                                              int32_t ntiles,
                                                              int32_t *__restrict__ hist) {
-    __shared__ int32_t s_h[256];
-    s_h[threadIdx.x] = 0;
+    __shared__ int32_t s_h[kRadix];
+    for (int q = 0; q < kRadixPer; q++) s_h[q * kSortThreads + threadIdx.x] = 0;
     __syncthreads();
     const int32_t base = blockIdx.x * kSortTile;
     for (int r = 0; r < kSortItems; r++) {
         int32_t i = base + r * kSortThreads + threadIdx.x;
-        if (i < n) atomicAdd(&s_h[(keys[i] >> shift) & 255], 1);
+        if (i < n) atomicAdd(&s_h[(keys[i] >> shift) & (kRadix - 1)], 1);
     }
     __syncthreads();
-    hist[threadIdx.x * ntiles + blockIdx.x] = s_h[threadIdx.x];
+    for (int q = 0; q < kRadixPer; q++) {
+        const int d = q * kSortThreads + threadIdx.x;
+        hist[d * ntiles + blockIdx.x] = s_h[d];
+    }
 }
 
 // Exclusive scan of hist (length m) in place, in three launches: per block
@@ -130,47 +136,59 @@ __global__ __launch_bounds__(1024) void k_scan_add(int32_t *__restrict__ v, int3
         if (base + k < m) v[base + k] += add;
 }
 
-// stable scatter of one tile: rounds of 256 consecutive keys, wave-level
-// multisplit by digit, running per-digit counters across rounds
+// stable scatter of one tile: its keys are loaded up front, then scattered
+// in rounds of 256 consecutive keys, wave-level multisplit by digit, running
+// per-digit bases across rounds
 __global__ __launch_bounds__(kSortThreads) void k_radix_scatter(
     const uint64_t *__restrict__ kin, const int32_t *__restrict__ vin,
     uint64_t *__restrict__ kout, int32_t *__restrict__ vout, int32_t n, int shift,
     int32_t ntiles, const int32_t *__restrict__ offs) {
-    __shared__ int32_t s_base[256];
-    __shared__ int32_t s_wc[kSortThreads / kWave][256];
+    __shared__ int32_t s_base[kRadix];
+    __shared__ int32_t s_wc[kSortThreads / kWave][kRadix];
     const int w = threadIdx.x / kWave;
-    s_base[threadIdx.x] = offs[threadIdx.x * ntiles + blockIdx.x];
     const int32_t tile0 = blockIdx.x * kSortTile;
+    uint64_t key[kSortItems];
+    int32_t val[kSortItems];
+#pragma unroll
     for (int r = 0; r < kSortItems; r++) {
-        for (int j = 0; j < kSortThreads / kWave; j++) s_wc[j][threadIdx.x] = 0;
-        __syncthreads();
         const int32_t i = tile0 + r * kSortThreads + threadIdx.x;
+        key[r] = i < n ? kin[i] : 0ull;
+        val[r] = i < n ? vin[i] : 0;
+    }
+    for (int q = 0; q < kRadixPer; q++) {
+        const int d = q * kSortThreads + threadIdx.x;
+        s_base[d] = offs[d * ntiles + blockIdx.x];
+        for (int j = 0; j < kSortThreads / kWave; j++) s_wc[j][d] = 0;
+    }
+    __syncthreads();
+#pragma unroll
+    for (int r = 0; r < kSortItems; r++) {
+        const int32_t i = tile0 + r * kSortThreads + threadIdx.x;
+        if (tile0 + r * kSortThreads >= n) break;  // block-uniform
         const bool v = i < n;
-        uint64_t key = v ? kin[i] : 0;
-        const int32_t val = v ? vin[i] : 0;
-        const int dg = (int)((key >> shift) & 255);
+        const int dg = (int)((key[r] >> shift) & (kRadix - 1));
         // lanes of this wave with the same digit
         uint64_t peers = ballot(v);
-        for (int b = 0; b < 8; b++) {
-            uint64_t m = ballot(((dg >> b) & 1) != 0);
+        for (int b = 0; b < kRadixBits; b++) {
+            const uint64_t m = ballot(((dg >> b) & 1) != 0);
             peers &= ((dg >> b) & 1) ? m : ~m;
         }
         const int rk = rank_below(peers);
-        const int cnt = __popcll(peers);
-        // the lowest peer publishes the wave's count of this digit
-        if (v && rk == 0) s_wc[w][dg] = cnt;
+        const bool lead = v && rk == 0;  // publishes the wave's count of its digit
+        if (lead) s_wc[w][dg] = __popcll(peers);
         __syncthreads();
         if (v) {
             int32_t off = s_base[dg];
             for (int j = 0; j < w; j++) off += s_wc[j][dg];
-            kout[off + rk] = key;
-            vout[off + rk] = val;
+            kout[off + rk] = key[r];
+            vout[off + rk] = val[r];
         }
         __syncthreads();
-        // advance the running base by this round's totals
-        int32_t tot = 0;
-        for (int j = 0; j < kSortThreads / kWave; j++) tot += s_wc[j][threadIdx.x];
-        s_base[threadIdx.x] += tot;
+        // the next round starts after this round's keys of each digit
+        if (lead) {
+            atomicAdd(&s_base[dg], s_wc[w][dg]);
+            s_wc[w][dg] = 0;
+        }
         __syncthreads();
     }
 }
