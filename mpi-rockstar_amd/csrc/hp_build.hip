@@ -11,24 +11,28 @@
 // nodes exactly; only the order inside a node differs (the reference swaps in
 // place).
 //
-// The build is level-synchronous over the whole batch:
-//   level 0   gather: records -> float4 (x, y, z, record index), tight box and
-//             non-finite count per halo (tiles of one halo, block reductions,
-//             integer atomics on order-preserving float keys);
-//             roots: one workgroup per halo writes the root node, moves any
-//             non-finite records to the end of the halo's range (stable), and
-//             queues the root.
-//   level L   small tasks (<= big threshold): one wavefront per node, ballot
-//             ranks, two passes (count, then stable scatter + child boxes);
-//             big tasks: tiles of 4096 points, count pass -> per-task scan ->
-//             scatter pass.
-// Children of at most 10 points are leaves: their points are written straight
-// into the final tree-order array, with their leaf id.
-//
-// Multipoles (src/potential.c:71-124): leaf mass and centre from f64 sums,
-// internal centres combined in float from the two children (right term
-// first) climbing with per-node arrival counters; dmin over the node's whole
-// range in f64, one wavefront per node, or tiles for big nodes.
+// The build runs over the whole batch at once:
+//   roots     gather: records -> float4 (x, y, z, record index), tight box and
+//             non-finite count per call (tiles of one call, block reductions,
+//             integer atomics on order-preserving float keys); roots: one
+//             thread per call writes the root node and queues it (calls with
+//             non-finite points get a workgroup that moves those records to
+//             the end of the call's range, stably).
+//   levels    nodes of more than kSmallMax points: tiles of 1024 points,
+//             count pass -> one wavefront per node (scan of the tile counts,
+//             child boxes) -> scatter pass; host-driven, several levels per
+//             counter read.
+//   subtrees  every node of at most kSmallMax points is finished in one go:
+//             65..kSmallMax points by a workgroup, point-parallel per level
+//             (block prefix of the left flags, segmented DPP scans for the
+//             child boxes); at most 64 points by one wavefront, a point per
+//             lane.  Both keep the points and the node table in LDS, compute
+//             centres of mass bottom-up and dmin over the final order, and
+//             write leaves straight into the final tree-order array.
+// Multipoles of the nodes made by the split levels (src/potential.c:71-124):
+// leaf mass and centre from f64 sums, internal centres combined in float
+// from the two children (right term first) level by level, dmin over the
+// node's whole range in f64 by tiles.
 
 #include "hp_internal.h"
 
@@ -1274,6 +1278,10 @@ __global__ void k_level_advance(int32_t *__restrict__ c, int32_t *__restrict__ d
     if (lvl < kMaxLevels) depth_end[lvl] = c[kNodeCount];
     else c[kErrFlags] |= kErrCapacity;
     c[kLevel] = lvl + 1;
+    if (c[kErrFlags] & kErrCapacity) {  // a push was dropped: no further level
+        c[kBigEnd] = c[kBigBegin];
+        c[kTileEnd] = c[kTileBegin];
+    }
     c[kActive] = c[kBigEnd] > c[kBigBegin] ? 1 : 0;
 }
 
