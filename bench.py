@@ -124,6 +124,7 @@ def main() -> int:
     ap.add_argument("--workload", default="c5", choices=sorted(WORKLOADS))
     ap.add_argument("--accumulate", default="compensated", choices=["compensated", "naive", "f64"])
     ap.add_argument("--no-graph", action="store_true", help="eager launches instead of hipGraph replay")
+    ap.add_argument("--no-tune", action="store_true", help="skip the autotuner (hp_plan_tune)")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--no-per-halo", action="store_true", help="skip the per-halo GPU path")
     ap.add_argument("--no-host", action="store_true", help="skip the host-buffer (PCIe) leg")
@@ -172,6 +173,12 @@ def main() -> int:
 
     use_graph = 0 if args.no_graph else 1
     plan = halopot.Plan(cat.halos, halopot.options(accumulate=acc, timing=1, use_graph=use_graph))
+    tuning = None
+    if not args.no_tune:
+        restore()
+        tuning = plan.tune(d_po.address, reps=2)
+        log(f"autotuner: big threshold {tuning['big_threshold']}, wavefront subtrees up to "
+            f"{tuning['wave_max']} points ({tuning['ms_potential']:.3f} ms per potential pass)")
     for i in range(args.warmup):
         restore()
         plan.run(d_po.address, d_h.address, d_b.address)
@@ -228,6 +235,7 @@ def main() -> int:
         "config": {"workload": args.workload + ": " + WORKLOADS[args.workload],
                    "records": cat.n_records, "calls": cat.n_halos,
                    "accumulate": args.accumulate, "graph": bool(use_graph),
+                   "tuning": tuning,
                    "parallelism": "one GPU"},
         "roofline": roofline,
     }

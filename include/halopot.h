@@ -151,6 +151,25 @@ int hp_plan_potential(hp_plan *plan, struct potential *d_po, void *stream);
 int hp_plan_stats(const hp_plan *plan, hp_stats *out);
 int hp_plan_destroy(hp_plan *plan);
 
+/* Launch settings an autotuned plan uses.  None of them changes a tree, a
+ * walk decision or a sum: they only move work between kernels. */
+typedef struct hp_tuning {
+    int32_t big_threshold;  /* nodes above this many points are split by the
+                               level passes (128, 256 or 512) */
+    int32_t wave_max;       /* subtrees of at most this many points are built
+                               by one wavefront (32 or 64) */
+    int32_t direct_max;     /* HP_METHOD_AUTO: calls of at most this many
+                               points use the direct kernel (0 or 10) */
+    float   ms_potential;   /* the chosen setting's time for the potential */
+} hp_tuning;
+
+/* Autotuner: runs the plan's potential pass (compute_potential of every
+ * call, pe overwritten) reps times per candidate setting on the caller's
+ * device records and keeps the fastest for later runs of the plan.  The
+ * chosen setting is returned in *out when out is not NULL.  Per-call plans
+ * are not tuned (HP_EINVAL). */
+int hp_plan_tune(hp_plan *plan, struct potential *d_po, int reps, hp_tuning *out);
+
 /* Host-buffer convenience: copies in, hp_plan_run, copies out. */
 int hp_bound_pass_host(struct potential *po, int64_t nrecords, hp_halo *halos,
                        int64_t nhalos, uint8_t *bound, const hp_options *opt);

@@ -156,6 +156,7 @@ struct BuildArgs {
     int32_t *fat;           // leaves of more than kFatLeaf points
     int32_t *counters;
     int32_t big_threshold;
+    int32_t wave_max;       // subtrees of at most this many points: one wavefront (<= 64)
     int32_t cap_small, cap_big, cap_tiles, cap_nodes, cap_fat;
 };
 
@@ -480,7 +481,7 @@ __global__ __launch_bounds__(kSubThreads) void k_subtree(const float4 *__restric
     for (int32_t k = blockIdx.x; k < ntasks; k += gridDim.x) {
         const Task t = tasks[ntasks - 1 - k];
         const int32_t first = t.pad[0], n = t.pad[1];
-        if (n <= kWaveTask) continue;  // a wavefront subtree
+        if (n <= a.wave_max) continue;  // a wavefront subtree
         const int32_t root_skip = a.nodes[t.node].skip;
         const float4 *src = (t.depth & 1) ? B : A;
         for (int i = tid; i < n; i += kSubThreads) {
@@ -844,7 +845,7 @@ __global__ __launch_bounds__(kWave) void k_subtree_wave(const float4 *__restrict
     for (int32_t k = blockIdx.x; k < ntasks; k += gridDim.x) {
         const Task t = tasks[k];
         const int32_t n = t.pad[1];
-        if (n > kWaveTask) continue;  // a workgroup subtree
+        if (n > a.wave_max) continue;  // a workgroup subtree
         const Node root = a.nodes[t.node];
         const float4 *src = (t.depth & 1) ? B : A;
         const int32_t first = root.first;

@@ -17,7 +17,7 @@ namespace hp {
 constexpr int kLeafPoints = 10;   // POINTS_PER_LEAF, src/potential.c:57
 constexpr int kDegenerate = 20;   // 2 * POINTS_PER_LEAF, src/potential.c:145
 constexpr int kWave = 64;
-constexpr int kGroup = 64;        // sink particles per walk wavefront
+constexpr int kGroup = 64;        // sink particles per walk wavefront (one per lane)
 constexpr int kTile = 1024;       // points per workgroup tile of a big node
 constexpr int kSmallMax = 512;    // largest subtree finished by one workgroup
 constexpr int kSmallChunks = kSmallMax / 64;
@@ -71,7 +71,7 @@ struct TilePart {
     float hi[2][3];
 };
 
-// One walk wavefront: sink records [p0, min(p0 + 64, pend)) of the halo
+// One walk wavefront: sink records [p0, min(p0 + kGroup, pend)) of the halo
 // whose root node is `root`.
 struct Group {
     int32_t root;

@@ -169,9 +169,10 @@ struct Tables {
 };
 
 static int make_tables(Tables &T, const int64_t *first, const int64_t *count, int64_t nh,
-                       int method, bool want_unbind) {
+                       int method, bool want_unbind, int32_t direct_max = kLeafPoints) {
     T = Tables();
     T.method = method;
+    T.direct_max = direct_max;
     T.nhalos = nh;
     T.first.assign(first, first + nh);
     T.count.assign(count, count + nh);
@@ -222,6 +223,7 @@ struct Engine {
     unsigned long long *h_work = nullptr;
     hipEvent_t ev[8] = {};
     int32_t big_threshold = kSmallMax;
+    int32_t wave_max = kWaveTask;
 
     // batch tables on the device
     DevBuf hfirst, hcount, root_tiles, groups, direct_blocks, rec_call, scan_tiles, call_tile0,
@@ -296,8 +298,8 @@ struct Engine {
         HP_ENSURE(B, sizeof(float4) * (N + 1));
         HP_ENSURE(F, sizeof(float4) * (N + 1));
         HP_ENSURE(leaf_of, sizeof(int32_t) * (N + 1));
-        // split levels use at most 2N + H ids; subtrees reserve ids in blocks
-        // (unused ones become empty nodes), at most ~0.7N + 16 per task more
+        // split levels use at most about 0.1N + H ids; a workgroup subtree of
+        // n points reserves 2n - 2, a wavefront subtree what it uses
         cap_nodes = 5 * N + H + 1024;
         HP_ENSURE(nodes, sizeof(Node) * cap_nodes);
         HP_ENSURE(parent, sizeof(int32_t) * cap_nodes);
@@ -366,6 +368,7 @@ struct Engine {
         a.fat = fat.as<int32_t>();
         a.counters = counters.as<int32_t>();
         a.big_threshold = big_threshold;
+        a.wave_max = wave_max;
         a.cap_small = (int32_t)cap_small;
         a.cap_big = (int32_t)cap_big;
         a.cap_tiles = (int32_t)cap_tiles;
@@ -860,6 +863,7 @@ static hp_options default_options() {
 struct hp_plan {
     hp::Engine eng;
     hp::Tables T;
+    std::vector<int64_t> first, count;  // the calls, for re-planning
     std::vector<hp::Tables> per;  // per_call: one single-call table per call
     hp_options opt;
 };
@@ -951,6 +955,8 @@ __attribute__((visibility("default"))) int hp_plan_create(hp_plan **out, const h
             c[h] = halos[h].count;
         }
         r = make_tables(p->T, f.data(), c.data(), nhalos, p->opt.method, true);
+        p->first = f;
+        p->count = c;
         if (!r && p->opt.per_call) {
             p->per.resize(nhalos);
             int64_t big = 0, bigh = 0;
@@ -974,6 +980,74 @@ __attribute__((visibility("default"))) int hp_plan_create(hp_plan **out, const h
         return r;
     }
     *out = p;
+    return HP_OK;
+}
+
+// Times the plan's potential pass on the caller's records for each setting
+// that changes no tree, decision or sum, and keeps the fastest: the size
+// above which nodes are split by the level passes, the size up to which a
+// subtree is built by one wavefront and, for HP_METHOD_AUTO plans, whether
+// calls of at most 10 points use the direct kernel.
+__attribute__((visibility("default"))) int hp_plan_tune(hp_plan *p, struct potential *d_po,
+                                                        int reps, hp_tuning *out) {
+    if (!p || !d_po || reps < 1) return set_error(HP_EINVAL, "hp_plan_tune: bad arguments");
+    if (p->opt.per_call) return set_error(HP_EINVAL, "hp_plan_tune: per-call plans are not tuned");
+    Engine &e = p->eng;
+    hp_options o = p->opt;
+    o.timing = 0;
+    o.count_work = 0;
+    const int32_t bts[] = {128, 256, 512};
+    const int32_t wms[] = {32, 64};
+    const int32_t dms[] = {kLeafPoints, 0};
+    const int ndm = p->opt.method == HP_METHOD_AUTO ? 2 : 1;
+    hp_tuning best{};
+    float best_ms = -1.0f;
+    int r;
+    hipEvent_t e0 = e.ev[5], e1 = e.ev[6];
+    for (int di = 0; di < ndm; di++) {
+        if (ndm > 1) {
+            if ((r = make_tables(p->T, p->first.data(), p->count.data(), (int64_t)p->first.size(),
+                                 p->opt.method, true, dms[di])) ||
+                (r = e.load_tables(p->T)))
+                return r;
+        }
+        for (int32_t bt : bts) {
+            if (bt > kSmallMax) continue;
+            for (int32_t wm : wms) {
+                e.big_threshold = bt;
+                e.wave_max = wm;
+                e.drop_graphs();
+                if ((r = e.reserve(p->T, true)) || (r = e.potential(p->T, d_po, o))) return r;
+                HP_TRY(hipEventRecord(e0, e.stream));
+                for (int k = 0; k < reps; k++)
+                    if ((r = e.potential(p->T, d_po, o))) return r;
+                HP_TRY(hipEventRecord(e1, e.stream));
+                HP_TRY(hipEventSynchronize(e1));
+                float ms = 0.0f;
+                HP_TRY(hipEventElapsedTime(&ms, e0, e1));
+                ms /= (float)reps;
+                if (best_ms < 0.0f || ms < best_ms) {
+                    best_ms = ms;
+                    best.big_threshold = bt;
+                    best.wave_max = wm;
+                    best.direct_max = ndm > 1 ? dms[di] : p->T.direct_max;
+                    best.ms_potential = ms;
+                }
+            }
+        }
+    }
+    e.big_threshold = best.big_threshold;
+    e.wave_max = best.wave_max;
+    e.drop_graphs();
+    if (ndm > 1 && best.direct_max != p->T.direct_max) {
+        if ((r = make_tables(p->T, p->first.data(), p->count.data(), (int64_t)p->first.size(),
+                             p->opt.method, true, best.direct_max)) ||
+            (r = e.load_tables(p->T)))
+            return r;
+    }
+    if ((r = e.reserve(p->T, true))) return r;
+    HP_TRY(hipStreamSynchronize(e.stream));
+    if (out) *out = best;
     return HP_OK;
 }
 

@@ -63,10 +63,19 @@ class Stats(C.Structure):
         return {f: getattr(self, f) for f, _ in self._fields_}
 
 
+class Tuning(C.Structure):
+    _fields_ = [("big_threshold", C.c_int32), ("wave_max", C.c_int32),
+                ("direct_max", C.c_int32), ("ms_potential", C.c_float)]
+
+    def as_dict(self):
+        return {f: getattr(self, f) for f, _ in self._fields_}
+
+
 _LIB = None
 
 EXPORTS = ("compute_potential", "compute_kinetic_energy", "hp_plan_create", "hp_plan_run",
-           "hp_plan_potential", "hp_plan_stats", "hp_plan_destroy", "hp_bound_pass_host",
+           "hp_plan_potential", "hp_plan_stats", "hp_plan_destroy", "hp_plan_tune",
+           "hp_bound_pass_host",
            "hp_debug_tree", "hp_last_error", "hp_set_error_handler")
 GLOBALS = ("FORCE_RES", "PARTICLE_MASS", "Om", "Ol", "W0", "WA", "SCALE_NOW")
 
@@ -89,6 +98,7 @@ def lib() -> C.CDLL:
         L.hp_plan_potential.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p]
         L.hp_plan_stats.argtypes = [C.c_void_p, C.POINTER(Stats)]
         L.hp_plan_destroy.argtypes = [C.c_void_p]
+        L.hp_plan_tune.argtypes = [C.c_void_p, C.c_void_p, C.c_int, C.POINTER(Tuning)]
         L.hp_bound_pass_host.argtypes = [C.c_void_p, C.c_int64, C.c_void_p, C.c_int64,
                                          C.c_void_p, C.POINTER(Options)]
         L.hp_debug_tree.argtypes = [C.c_void_p, C.c_int64, C.c_void_p, C.c_void_p, C.c_int64]
@@ -190,6 +200,13 @@ class Plan:
         self._h = C.c_void_p()
         _check(lib().hp_plan_create(C.byref(self._h), _ptr(halos), halos.shape[0],
                                     C.byref(self.opts)))
+
+    def tune(self, d_po: int, reps: int = 3) -> dict:
+        """hp_plan_tune: times the potential pass (pe overwritten) for each
+        result-neutral launch setting and keeps the fastest."""
+        t = Tuning()
+        _check(lib().hp_plan_tune(self._h, C.c_void_p(d_po), int(reps), C.byref(t)))
+        return t.as_dict()
 
     def run(self, d_po: int, d_halos: int, d_bound: int | None = None) -> None:
         _check(lib().hp_plan_run(self._h, C.c_void_p(d_po), C.c_void_p(d_halos),
