@@ -20,6 +20,7 @@
 #include "hp_unbind.hip"
 #include "hp_walk.hip"
 
+#include <algorithm>
 #include <atomic>
 #include <cmath>
 #include <cstdarg>
@@ -201,6 +202,11 @@ static int make_tables(Tables &T, const int64_t *first, const int64_t *count, in
                                          (int32_t)(first[h] + count[h]), 0});
         }
     }
+    // the walk's wavefronts of the largest calls first: they walk the
+    // deepest trees, and starting them early shortens the launch's tail
+    std::stable_sort(T.groups.begin(), T.groups.end(), [&](const Group &x, const Group &y) {
+        return count[x.root] > count[y.root];
+    });
     if (want_unbind) {
         T.rec_call.resize(pos);
         T.call_tile0.resize(nh);
@@ -491,12 +497,12 @@ struct Engine {
             a.po = d_po;
             a.err = counters.as<int32_t>() + kErrFlags;
             a.work = work.as<unsigned long long>();
-            const int nb = blocks((int64_t)T.groups.size() * kWave, 256);
+            const int nb = blocks((int64_t)T.groups.size() * kWave, 64);
             const bool cnt = o.count_work != 0;
 #define HP_WALK(ACC)                                                                   \
     do {                                                                               \
-        if (cnt) k_walk<ACC, true><<<nb, 256, 0, s>>>(a);                              \
-        else k_walk<ACC, false><<<nb, 256, 0, s>>>(a);                                 \
+        if (cnt) k_walk<ACC, true><<<nb, 64, 0, s>>>(a);                              \
+        else k_walk<ACC, false><<<nb, 64, 0, s>>>(a);                                 \
     } while (0)
             if (o.accumulate == HP_ACC_NAIVE) HP_WALK(HP_ACC_NAIVE);
             else if (o.accumulate == HP_ACC_F64) HP_WALK(HP_ACC_F64);

@@ -192,7 +192,7 @@ __global__ __launch_bounds__(256) void k_walk(WalkArgs a) {
             n_p2m += __popcll(ballot(p2m));
         }
         // monopole, unsoftened (src/potential.c:162-166)
-        if (ballot(p2m)) {
+        {
             if constexpr (ACC == HP_ACC_F64) {
                 double ex = (double)me.x - (double)nd.com[0];
                 double ey = (double)me.y - (double)nd.com[1];
