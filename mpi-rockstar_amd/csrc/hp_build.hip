@@ -204,13 +204,21 @@ __device__ void push_small_at(const BuildArgs &a, int32_t slot, int32_t id, cons
     a.small_next[slot] = t;
 }
 
+// A big-task slot and its tiles are reserved together: counters[kBigNext]
+// and counters[kTileNext] are the two halves of one 64-bit counter.
+__device__ __forceinline__ uint64_t alloc_big_and_tiles(const BuildArgs &a, int32_t ntiles) {
+    static_assert(kTileNext == kBigNext + 1 && (kBigNext & 1) == 0, "packed counter layout");
+    return atomicAdd(reinterpret_cast<unsigned long long *>(a.counters + kBigNext),
+                     1ull | ((unsigned long long)ntiles << 32));
+}
+
 // Queues a node of more than big_threshold points with its tiles.
 __device__ void push_big(const BuildArgs &a, int32_t id, const float *lo, const float *hi,
                          int32_t count, int32_t depth) {
     Task t = make_task(id, lo, hi, depth);
     const int32_t nt = (count + kTile - 1) / kTile;
-    const int32_t bi = atomicAdd(&a.counters[kBigNext], 1);
-    const int32_t t0 = atomicAdd(&a.counters[kTileNext], nt);
+    const uint64_t al = alloc_big_and_tiles(a, nt);
+    const int32_t bi = (int32_t)al, t0 = (int32_t)(al >> 32);
     if (bi >= a.cap_big || t0 + nt > a.cap_tiles) {
         atomicOr(&a.counters[kErrFlags], kErrCapacity);
         return;
@@ -219,6 +227,29 @@ __device__ void push_big(const BuildArgs &a, int32_t id, const float *lo, const 
     t.ntiles = nt;
     a.big[bi] = t;
     for (int32_t j = 0; j < nt; j++) a.tiles[t0 + j] = make_int2(bi, j * kTile);
+}
+
+// push_big made by a whole wavefront with wavefront-uniform arguments: lane 0
+// reserves the slots, the lanes write the tiles.
+__device__ void push_big_wave(const BuildArgs &a, int32_t id, const float *lo, const float *hi,
+                              int32_t count, int32_t depth) {
+    const int lane = lane_id();
+    const int32_t nt = (count + kTile - 1) / kTile;
+    uint64_t al = 0;
+    if (lane == 0) al = alloc_big_and_tiles(a, nt);
+    al = read_lane(al, 0);
+    const int32_t bi = (int32_t)al, t0 = (int32_t)(al >> 32);
+    if (bi >= a.cap_big || t0 + nt > a.cap_tiles) {
+        if (lane == 0) atomicOr(&a.counters[kErrFlags], kErrCapacity);
+        return;
+    }
+    if (lane == 0) {
+        Task t = make_task(id, lo, hi, depth);
+        t.tile0 = t0;
+        t.ntiles = nt;
+        a.big[bi] = t;
+    }
+    for (int32_t j = lane; j < nt; j += kWave) a.tiles[t0 + j] = make_int2(bi, j * kTile);
 }
 
 __device__ __forceinline__ void write_node(Node *nd, const float *lo, const float *hi, int32_t first,
@@ -372,18 +403,23 @@ __global__ __launch_bounds__(256) void k_roots_bad(const int64_t *__restrict__ h
 // Writes the two children of a split node and queues those that need a
 // split: small ones at slots P, P + 1 (allocated with the node ids), big ones
 // with their tiles.
+// Called by a whole wavefront with wavefront-uniform arguments.
 __device__ void make_children(const BuildArgs &a, const Task &t, const Node &nd, int32_t L,
                               int32_t P, int32_t nl, const float *loL, const float *hiL,
                               const float *loR, const float *hiR) {
     const int32_t nr = nd.count - nl;
-    write_node(a.nodes + L, loL, hiL, nd.first, nl, L + 1);
-    write_node(a.nodes + L + 1, loR, hiR, nd.first + nl, nr, nd.skip);
-    a.parent[L] = a.parent[L + 1] = t.node;
-    a.nodes[t.node].left = L;
-    if (nl > a.big_threshold) push_big(a, L, loL, hiL, nl, t.depth + 1);
-    else if (nl > kLeafPoints) push_small_at(a, P++, L, loL, hiL, t.depth + 1, nd.first, nl);
-    if (nr > a.big_threshold) push_big(a, L + 1, loR, hiR, nr, t.depth + 1);
-    else if (nr > kLeafPoints) push_small_at(a, P, L + 1, loR, hiR, t.depth + 1, nd.first + nl, nr);
+    if (lane_id() == 0) {
+        write_node(a.nodes + L, loL, hiL, nd.first, nl, L + 1);
+        write_node(a.nodes + L + 1, loR, hiR, nd.first + nl, nr, nd.skip);
+        a.parent[L] = a.parent[L + 1] = t.node;
+        a.nodes[t.node].left = L;
+        if (nl > kLeafPoints && nl <= a.big_threshold)
+            push_small_at(a, P++, L, loL, hiL, t.depth + 1, nd.first, nl);
+        if (nr > kLeafPoints && nr <= a.big_threshold)
+            push_small_at(a, P, L + 1, loR, hiR, t.depth + 1, nd.first + nl, nr);
+    }
+    if (nl > a.big_threshold) push_big_wave(a, L, loL, hiL, nl, t.depth + 1);
+    if (nr > a.big_threshold) push_big_wave(a, L + 1, loR, hiR, nr, t.depth + 1);
 }
 
 __device__ __forceinline__ int32_t small_pushes(int32_t nl, int32_t nr, int32_t thr) {
@@ -1157,19 +1193,21 @@ __global__ __launch_bounds__(256) void k_big_split(const Task *__restrict__ big,
                 lo[s][d] = wave_min(lo[s][d]);
                 hi[s][d] = wave_max(hi[s][d]);
             }
-        if (lane != 0) continue;
+        // nl, nd and the reduced boxes are wavefront-uniform from here on
         BigSplit r;
         r.nleft = nl;
         r.left = -1;
         r.leafL = r.leafR = 0;
         if (nl == 0 || nl == nd.count) {
-            note_fat_leaf(a, t.node, nd.count);
+            if (lane == 0) note_fat_leaf(a, t.node, nd.count);
         } else {
             const int32_t nr = nd.count - nl;
-            const uint64_t al = alloc_nodes_and_small(a, 2, small_pushes(nl, nr, a.big_threshold));
+            uint64_t al = 0;
+            if (lane == 0) al = alloc_nodes_and_small(a, 2, small_pushes(nl, nr, a.big_threshold));
+            al = read_lane(al, 0);
             const int32_t L = (int32_t)al, P = (int32_t)(al >> 32);
             if (L + 2 > a.cap_nodes) {
-                atomicOr(&a.counters[kErrFlags], kErrCapacity);
+                if (lane == 0) atomicOr(&a.counters[kErrFlags], kErrCapacity);
             } else {
                 r.left = L;
                 r.leafL = nl <= kLeafPoints;
@@ -1177,7 +1215,7 @@ __global__ __launch_bounds__(256) void k_big_split(const Task *__restrict__ big,
                 make_children(a, t, nd, L, P, nl, lo[0], hi[0], lo[1], hi[1]);
             }
         }
-        res[bi] = r;
+        if (lane == 0) res[bi] = r;
     }
 }
 

@@ -433,14 +433,14 @@ struct Engine {
             float4 *src = (level & 1) ? A.as<float4>() : B.as<float4>();
             float4 *dst = (level & 1) ? B.as<float4>() : A.as<float4>();
             const BuildArgs ba = build_args(small0.as<Task>());
-            k_big_count<<<1024, kTileThreads, 0, s>>>(src, tiles.as<int2>(), big.as<Task>(),
+            k_big_count<<<2048, kTileThreads, 0, s>>>(src, tiles.as<int2>(), big.as<Task>(),
                                                       nodes.as<Node>(), parts.as<TilePart>(),
                                                       counters.as<int32_t>());
             HP_LAUNCHED();
-            k_big_split<<<64, 256, 0, s>>>(big.as<Task>(), parts.as<TilePart>(),
+            k_big_split<<<1024, 256, 0, s>>>(big.as<Task>(), parts.as<TilePart>(),
                                           splitres.as<BigSplit>(), ba);
             HP_LAUNCHED();
-            k_big_scatter<<<1024, kTileThreads, 0, s>>>(src, dst, tiles.as<int2>(), big.as<Task>(),
+            k_big_scatter<<<2048, kTileThreads, 0, s>>>(src, dst, tiles.as<int2>(), big.as<Task>(),
                                                         parts.as<TilePart>(),
                                                         splitres.as<BigSplit>(), ba);
             HP_LAUNCHED();
