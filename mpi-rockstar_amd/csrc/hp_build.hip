@@ -150,14 +150,15 @@ struct BuildArgs {
     int32_t *parent;
     int32_t *leaf_of;
     float4 *fin;            // final tree order
-    Task *small_next;
+    Task *small_next;       // wavefront subtree tasks
+    Task *group_next;       // workgroup subtree tasks
     Task *big;              // append-only
     int2 *tiles;            // append-only (task index, offset)
     int32_t *fat;           // leaves of more than kFatLeaf points
     int32_t *counters;
     int32_t big_threshold;
-    int32_t wave_max;       // subtrees of at most this many points: one wavefront (<= 64)
-    int32_t cap_small, cap_big, cap_tiles, cap_nodes, cap_fat;
+    int32_t wave_max;       // subtrees of at most this many points: one wavefront (32..64)
+    int32_t cap_small, cap_group, cap_big, cap_tiles, cap_nodes, cap_fat;
 };
 
 __device__ void note_fat_leaf(const BuildArgs &a, int32_t id, int32_t count) {
@@ -189,19 +190,40 @@ __device__ Task make_task(int32_t id, const float *lo, const float *hi, int32_t 
     return t;
 }
 
-// Queues node `id` (more than 10 points, tight box known) at a small-task
-// slot already allocated; the task carries the node's range, so that the
-// subtree kernels can pick their tasks without reading the node.
-__device__ void push_small_at(const BuildArgs &a, int32_t slot, int32_t id, const float *lo,
-                              const float *hi, int32_t depth, int32_t first, int32_t count) {
-    if (slot >= a.cap_small) {
+// Subtree tasks: a node of more than 10 and at most wave_max points is built
+// by one wavefront, a larger one of at most big_threshold points by one
+// workgroup.  Each kind has its own list.
+__device__ __forceinline__ bool wave_sized(const BuildArgs &a, int32_t n) {
+    return n > kLeafPoints && n <= a.wave_max && n <= a.big_threshold;
+}
+__device__ __forceinline__ bool group_sized(const BuildArgs &a, int32_t n) {
+    return n > a.wave_max && n <= a.big_threshold;
+}
+
+// Queues node `id` (tight box known) at a slot already allocated in the
+// list of its kind; the task carries the node's range, so that the subtree
+// kernels can pick their tasks without reading the node.
+__device__ void push_task_at(const BuildArgs &a, Task *list, int32_t cap, int32_t slot, int32_t id,
+                             const float *lo, const float *hi, int32_t depth, int32_t first,
+                             int32_t count) {
+    if (slot >= cap) {
         atomicOr(&a.counters[kErrFlags], kErrCapacity);
         return;
     }
     Task t = make_task(id, lo, hi, depth);
     t.pad[0] = first;
     t.pad[1] = count;
-    a.small_next[slot] = t;
+    list[slot] = t;
+}
+__device__ __forceinline__ void push_small_at(const BuildArgs &a, int32_t slot, int32_t id,
+                                              const float *lo, const float *hi, int32_t depth,
+                                              int32_t first, int32_t count) {
+    push_task_at(a, a.small_next, a.cap_small, slot, id, lo, hi, depth, first, count);
+}
+__device__ __forceinline__ void push_group_at(const BuildArgs &a, int32_t slot, int32_t id,
+                                              const float *lo, const float *hi, int32_t depth,
+                                              int32_t first, int32_t count) {
+    push_task_at(a, a.group_next, a.cap_group, slot, id, lo, hi, depth, first, count);
 }
 
 // A big-task slot and its tiles are reserved together: counters[kBigNext]
@@ -293,11 +315,15 @@ __global__ __launch_bounds__(256) void k_roots(const int64_t *__restrict__ hfirs
         isbad = nbad > 0;
     }
     const bool ok = v && !isbad;
-    const bool small = ok && nfin > kLeafPoints && nfin <= a.big_threshold;
-    const uint64_t ms = ballot(small);
-    int32_t slot0 = 0;
-    if (ms && lane_id() == 0) slot0 = (int32_t)(alloc_nodes_and_small(a, 0, __popcll(ms)) >> 32);
+    const bool wsmall = ok && wave_sized(a, nfin), gsmall = ok && group_sized(a, nfin);
+    const uint64_t ms = ballot(wsmall), mg = ballot(gsmall);
+    int32_t slot0 = 0, gslot0 = 0;
+    if (lane_id() == 0) {
+        if (ms) slot0 = (int32_t)(alloc_nodes_and_small(a, 0, __popcll(ms)) >> 32);
+        if (mg) gslot0 = atomicAdd(&a.counters[kGroupNext], __popcll(mg));
+    }
     slot0 = __shfl(slot0, 0, 64);
+    gslot0 = __shfl(gslot0, 0, 64);
     if (isbad) {
         const int32_t k = atomicAdd(&a.counters[kBadRoots], 1);
         badlist[k] = h;
@@ -317,7 +343,8 @@ __global__ __launch_bounds__(256) void k_roots(const int64_t *__restrict__ hfirs
     write_node(a.nodes + h, lo, hi, first, nfin, -1);
     a.parent[h] = -1;
     if (nfin > a.big_threshold) push_big(a, h, lo, hi, nfin, 0);
-    else if (small) push_small_at(a, slot0 + rank_below(ms), h, lo, hi, 0, first, nfin);
+    else if (wsmall) push_small_at(a, slot0 + rank_below(ms), h, lo, hi, 0, first, nfin);
+    else if (gsmall) push_group_at(a, gslot0 + rank_below(mg), h, lo, hi, 0, first, nfin);
 }
 
 // The listed calls with non-finite points, one workgroup each: stable
@@ -389,9 +416,11 @@ __global__ __launch_bounds__(256) void k_roots_bad(const int64_t *__restrict__ h
             a.parent[h] = -1;
             if (nfin > a.big_threshold) {
                 push_big(a, h, lo, hi, nfin, 0);
-            } else if (nfin > kLeafPoints) {
+            } else if (wave_sized(a, nfin)) {
                 const uint64_t old = alloc_nodes_and_small(a, 0, 1);
                 push_small_at(a, (int32_t)(old >> 32), h, lo, hi, 0, first, nfin);
+            } else if (group_sized(a, nfin)) {
+                push_group_at(a, atomicAdd(&a.counters[kGroupNext], 1), h, lo, hi, 0, first, nfin);
             }
         }
         __syncthreads();
@@ -403,9 +432,10 @@ __global__ __launch_bounds__(256) void k_roots_bad(const int64_t *__restrict__ h
 // Writes the two children of a split node and queues those that need a
 // split: small ones at slots P, P + 1 (allocated with the node ids), big ones
 // with their tiles.
-// Called by a whole wavefront with wavefront-uniform arguments.
+// Called by a whole wavefront with wavefront-uniform arguments.  P and G
+// are the slots reserved in the wavefront and workgroup subtree lists.
 __device__ void make_children(const BuildArgs &a, const Task &t, const Node &nd, int32_t L,
-                              int32_t P, int32_t nl, const float *loL, const float *hiL,
+                              int32_t P, int32_t G, int32_t nl, const float *loL, const float *hiL,
                               const float *loR, const float *hiR) {
     const int32_t nr = nd.count - nl;
     if (lane_id() == 0) {
@@ -413,17 +443,20 @@ __device__ void make_children(const BuildArgs &a, const Task &t, const Node &nd,
         write_node(a.nodes + L + 1, loR, hiR, nd.first + nl, nr, nd.skip);
         a.parent[L] = a.parent[L + 1] = t.node;
         a.nodes[t.node].left = L;
-        if (nl > kLeafPoints && nl <= a.big_threshold)
-            push_small_at(a, P++, L, loL, hiL, t.depth + 1, nd.first, nl);
-        if (nr > kLeafPoints && nr <= a.big_threshold)
-            push_small_at(a, P, L + 1, loR, hiR, t.depth + 1, nd.first + nl, nr);
+        if (wave_sized(a, nl)) push_small_at(a, P++, L, loL, hiL, t.depth + 1, nd.first, nl);
+        if (group_sized(a, nl)) push_group_at(a, G++, L, loL, hiL, t.depth + 1, nd.first, nl);
+        if (wave_sized(a, nr)) push_small_at(a, P, L + 1, loR, hiR, t.depth + 1, nd.first + nl, nr);
+        if (group_sized(a, nr)) push_group_at(a, G, L + 1, loR, hiR, t.depth + 1, nd.first + nl, nr);
     }
     if (nl > a.big_threshold) push_big_wave(a, L, loL, hiL, nl, t.depth + 1);
     if (nr > a.big_threshold) push_big_wave(a, L + 1, loR, hiR, nr, t.depth + 1);
 }
 
-__device__ __forceinline__ int32_t small_pushes(int32_t nl, int32_t nr, int32_t thr) {
-    return (nl > kLeafPoints && nl <= thr ? 1 : 0) + (nr > kLeafPoints && nr <= thr ? 1 : 0);
+__device__ __forceinline__ int32_t wave_pushes(const BuildArgs &a, int32_t nl, int32_t nr) {
+    return (wave_sized(a, nl) ? 1 : 0) + (wave_sized(a, nr) ? 1 : 0);
+}
+__device__ __forceinline__ int32_t group_pushes(const BuildArgs &a, int32_t nl, int32_t nr) {
+    return (group_sized(a, nl) ? 1 : 0) + (group_sized(a, nr) ? 1 : 0);
 }
 
 // ------------------------------------------------------------- subtrees
@@ -445,6 +478,7 @@ __device__ __forceinline__ int32_t small_pushes(int32_t nl, int32_t nr, int32_t 
 // the final order in LDS.
 
 constexpr int kWaveTask = 64;   // subtrees up to this size: one wavefront
+constexpr int kWaveMin = 32;    // the smallest wavefront/workgroup boundary (wave_max) used
 constexpr int kRootSkip = -2;   // local skip pointer: the subtree root's own skip
 constexpr int kSubThreads = 256;
 constexpr int kSubWaves = kSubThreads / kWave;
@@ -508,16 +542,21 @@ __global__ __launch_bounds__(kSubThreads) void k_subtree(const float4 *__restric
     __shared__ uint16_t s_lvl[kSmallMax + 2]; // first local id of each level
     __shared__ uint64_t s_mask[kSubChunks + 1];
     __shared__ int32_t s_pre[kSubChunks + 1];
-    __shared__ int32_t s_nn[2], s_M, s_nlev, s_base;
+    __shared__ int32_t s_nn[2], s_M, s_nlev, s_base, s_next;
     const int tid = threadIdx.x, lane = lane_id(), w = tid / kWave;
-    const int32_t ntasks = min(a.counters[kSmallNext], a.cap_small);
+    const int32_t ntasks = min(a.counters[kGroupNext], a.cap_group);
     const float inf = __int_as_float(0x7f800000), ninf = __int_as_float((int)0xff800000);
-    // from the end of the list: the largest subtrees (children of the big
-    // nodes of the last split levels) are queued last and should start first
-    for (int32_t k = blockIdx.x; k < ntasks; k += gridDim.x) {
+    // A dynamic queue over the list: thread 0 takes the workgroup's next
+    // entry when it starts the current one.  Entries are taken from the end
+    // of the list: the largest subtrees (children of the big nodes of the
+    // last split levels) are queued last and should start first.
+    if (tid == 0) s_next = atomicAdd(&a.counters[kGroupCursor], 1);
+    __syncthreads();
+    for (int32_t k = s_next; k < ntasks; k = s_next) {
+        int32_t k_after = 0;
+        if (tid == 0) k_after = atomicAdd(&a.counters[kGroupCursor], 1);
         const Task t = tasks[ntasks - 1 - k];
         const int32_t first = t.pad[0], n = t.pad[1];
-        if (n <= a.wave_max) continue;  // a wavefront subtree
         const int32_t root_skip = a.nodes[t.node].skip;
         const float4 *src = (t.depth & 1) ? B : A;
         for (int i = tid; i < n; i += kSubThreads) {
@@ -786,25 +825,33 @@ __global__ __launch_bounds__(kSubThreads) void k_subtree(const float4 *__restric
         __threadfence_block();
         __syncthreads();
         // internal nodes, deepest level first (src/potential.c:115-122:
-        // float, right term first)
-        for (int L = nlev; L >= 0; L--) {
-            for (int j = s_lvl[L] + tid; j < s_lvl[L + 1]; j += kSubThreads) {
-                const int cl = s_left[j];
-                if (cl < 0) continue;
-                const float4 l = cm_get(cl), r = cm_get(cl + 1);
-                const float m = l.w + r.w;
-                float c[3];
-                const float lc[3] = {l.x, l.y, l.z}, rc[3] = {r.x, r.y, r.z};
-                for (int d = 0; d < 3; d++) {
-                    const float rr = rc[d] * r.w;
-                    const float ll = lc[d] * l.w;
-                    c[d] = m != 0.0f ? (rr + ll) / m : 0.0f;
+        // float, right term first).  One wavefront walks all levels: a level
+        // reads only values this wavefront wrote at deeper levels (or the
+        // leaves, made visible by the barrier above), so the levels need no
+        // workgroup barrier, only program order within the wavefront.
+        if (w == 0) {
+            for (int L = nlev; L >= 0; L--) {
+                const int l1 = s_lvl[L + 1];
+                for (int j = s_lvl[L] + lane; j < l1; j += kWave) {
+                    const int cl = s_left[j];
+                    if (cl < 0) continue;
+                    const float4 l = cm_get(cl), r = cm_get(cl + 1);
+                    const float m = l.w + r.w;
+                    float c[3];
+                    const float lc[3] = {l.x, l.y, l.z}, rc[3] = {r.x, r.y, r.z};
+                    for (int d = 0; d < 3; d++) {
+                        const float rr = rc[d] * r.w;
+                        const float ll = lc[d] * l.w;
+                        c[d] = m != 0.0f ? (rr + ll) / m : 0.0f;
+                    }
+                    cm_put(j, make_float4(c[0], c[1], c[2], m));
                 }
-                cm_put(j, make_float4(c[0], c[1], c[2], m));
+                __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+                __builtin_amdgcn_wave_barrier();
             }
-            __threadfence_block();
-            __syncthreads();
         }
+        __threadfence_block();
+        __syncthreads();
         // dmin: nodes of at most 64 points a thread each, in tree order
 #pragma unroll
         for (int q = 0; q < kSubPerThread; q++) {
@@ -820,20 +867,18 @@ __global__ __launch_bounds__(kSubThreads) void k_subtree(const float4 *__restric
             }
             finish_dmin(a.nodes + gid(j), big, sum, my_n[q]);
         }
-        // larger nodes a wavefront each; their ranges are read lane-parallel
-        for (int j0 = w * kWave; j0 < M; j0 += kSubThreads) {
-            const int jj = j0 + lane;
-            int32_t rl = 0, rn = 0;
-            if (jj < M) {
-                const Node *np = a.nodes + gid(jj);
-                rl = np->first - first;
-                rn = np->count;
-            }
-            const int nb = min(kWave, M - j0);
-            for (int q = 0; q < nb; q++) {
-                const int32_t qn = __shfl(rn, q, 64);
-                if (qn <= kWave) continue;
-                const int32_t ql = __shfl(rl, q, 64);
+        // larger nodes a wavefront each: the wavefront takes the nodes of
+        // its lanes' chunk (j = tid + k * kSubThreads) that have more than
+        // 64 points, one after another, reading each node's range lane-parallel
+#pragma unroll
+        for (int k = 0; k < kSubPerThread; k++) {
+            const int j0 = w * kWave + k * kSubThreads;
+            if (j0 >= M) break;
+            uint64_t wide = ballot(tid + k * kSubThreads < M && my_n[k] > kWave);
+            while (wide) {
+                const int q = __builtin_ctzll(wide);
+                wide &= wide - 1;
+                const int32_t qn = read_lane(my_n[k], q), ql = read_lane(my_lf[k], q);
                 const float4 c4 = cm_get(j0 + q);
                 const float com[3] = {c4.x, c4.y, c4.z};
                 double big = 0.0, sum = 0.0;
@@ -847,6 +892,7 @@ __global__ __launch_bounds__(kSubThreads) void k_subtree(const float4 *__restric
                 if (lane == 0) finish_dmin(a.nodes + gid(j0 + q), big, sum, qn);
             }
         }
+        if (tid == 0) s_next = k_after;
         __syncthreads();
     }
 }
@@ -878,10 +924,21 @@ __global__ __launch_bounds__(kWave) void k_subtree_wave(const float4 *__restrict
     __shared__ int16_t s_lvl[kWaveTask + 2];  // first local id of each level
     const int lane = lane_id();
     const int32_t ntasks = min(a.counters[kSmallNext], a.cap_small);
-    for (int32_t k = blockIdx.x; k < ntasks; k += gridDim.x) {
+    // a dynamic queue over the list: lane 0 takes the wavefront's next
+    // entry when it starts the current one
+    int32_t k = 0;
+    if (lane == 0) k = atomicAdd(&a.counters[kWaveCursor], 1);
+    k = read_lane(k, 0);
+    while (k < ntasks) {
+        int32_t k_after = 0;
+        if (lane == 0) k_after = atomicAdd(&a.counters[kWaveCursor], 1);
         const Task t = tasks[k];
         const int32_t n = t.pad[1];
-        if (n > a.wave_max) continue;  // a workgroup subtree
+        if (n > kWaveTask) {  // never queued here; guards the lane-per-point layout
+            if (lane == 0) atomicOr(&a.counters[kErrFlags], kErrBadTree);
+            k = read_lane(k_after, 0);
+            continue;
+        }
         const Node root = a.nodes[t.node];
         const float4 *src = (t.depth & 1) ? B : A;
         const int32_t first = root.first;
@@ -1069,6 +1126,7 @@ __global__ __launch_bounds__(kWave) void k_subtree_wave(const float4 *__restrict
             if (j) a.parent[id] = s_par[j] ? base + s_par[j] - 1 : t.node;
         }
         __syncthreads();
+        k = read_lane(k_after, 0);
     }
 }
 
@@ -1203,8 +1261,14 @@ __global__ __launch_bounds__(256) void k_big_split(const Task *__restrict__ big,
         } else {
             const int32_t nr = nd.count - nl;
             uint64_t al = 0;
-            if (lane == 0) al = alloc_nodes_and_small(a, 2, small_pushes(nl, nr, a.big_threshold));
+            int32_t G = 0;
+            if (lane == 0) {
+                al = alloc_nodes_and_small(a, 2, wave_pushes(a, nl, nr));
+                const int32_t ng = group_pushes(a, nl, nr);
+                if (ng) G = atomicAdd(&a.counters[kGroupNext], ng);
+            }
             al = read_lane(al, 0);
+            G = read_lane(G, 0);
             const int32_t L = (int32_t)al, P = (int32_t)(al >> 32);
             if (L + 2 > a.cap_nodes) {
                 if (lane == 0) atomicOr(&a.counters[kErrFlags], kErrCapacity);
@@ -1212,7 +1276,7 @@ __global__ __launch_bounds__(256) void k_big_split(const Task *__restrict__ big,
                 r.left = L;
                 r.leafL = nl <= kLeafPoints;
                 r.leafR = nr <= kLeafPoints;
-                make_children(a, t, nd, L, P, nl, lo[0], hi[0], lo[1], hi[1]);
+                make_children(a, t, nd, L, P, G, nl, lo[0], hi[0], lo[1], hi[1]);
             }
         }
         if (lane == 0) res[bi] = r;

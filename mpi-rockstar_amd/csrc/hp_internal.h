@@ -83,7 +83,7 @@ struct Group {
 // Device-side counters of one build.
 enum Counter {
     kNodeCount = 0,
-    kSmallNext = 1,   // subtree tasks (<= kSmallMax points), append-only
+    kSmallNext = 1,   // wavefront subtree tasks (<= wave_max points), append-only
     kBigNext = 2,     // big tasks pushed for the next level (append-only)
     kTileNext = 3,    // tiles of big tasks (append-only)
     kMaxDepth = 4,
@@ -100,7 +100,10 @@ enum Counter {
     kLevel = 14,      // levels advanced so far
     kActive = 15,     // the current level has big tasks
     kBadRoots = 16,   // calls with non-finite points, listed for their own pass
-    kNumCounters = 17
+    kGroupNext = 17,  // workgroup subtree tasks (wave_max < n <= big_threshold), append-only
+    kGroupCursor = 18,  // next workgroup subtree task to start (dynamic queue)
+    kWaveCursor = 19,   // next wavefront subtree task to start (dynamic queue)
+    kNumCounters = 20
 };
 
 enum ErrFlag { kErrWalkCap = 1, kErrBadTree = 2, kErrCapacity = 4 };

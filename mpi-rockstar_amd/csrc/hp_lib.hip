@@ -230,16 +230,17 @@ struct Engine {
     hipEvent_t ev[8] = {};
     int32_t big_threshold = kSmallMax;
     int32_t wave_max = kWaveTask;
+    int sub_grid = 0, wave_grid = 0;  // resident workgroups of the subtree kernels
 
     // batch tables on the device
     DevBuf hfirst, hcount, root_tiles, groups, direct_blocks, rec_call, scan_tiles, call_tile0,
         call_ntiles;
     // build
-    DevBuf box, bad, badlist, A, B, F, leaf_of, nodes, parent, fat, small0, big, tiles, parts,
+    DevBuf box, bad, badlist, A, B, F, leaf_of, nodes, parent, fat, small0, group0, big, tiles, parts,
         splitres, dparts, counters, work, depth_buf;
     // unbind
     DevBuf keys0, keys1, vals0, vals1, hist, hsum, srt, callinfo, tsum, tcand, tile_i32;
-    int64_t cap_nodes = 0, cap_small = 0, cap_big = 0, cap_tiles = 0, cap_fat = 0;
+    int64_t cap_nodes = 0, cap_small = 0, cap_group = 0, cap_big = 0, cap_tiles = 0, cap_fat = 0;
     std::vector<int32_t> depth_end;  // node count after each depth's level
 
     // last run
@@ -262,6 +263,14 @@ struct Engine {
         HP_TRY(hipHostMalloc((void **)&h_counters, sizeof(int32_t) * kNumCounters));
         HP_TRY(hipHostMalloc((void **)&h_work, sizeof(unsigned long long) * 4));
         for (auto &e : ev) HP_TRY(hipEventCreate(&e));
+        // The subtree kernels take tasks from dynamic queues until the lists
+        // are empty; their grids are the workgroups that fit on the device
+        // at once.
+        int per_cu = 0;
+        HP_TRY(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_subtree, kSubThreads, 0));
+        sub_grid = std::max(1, per_cu) * prop.multiProcessorCount;
+        HP_TRY(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_subtree_wave, kWave, 0));
+        wave_grid = std::max(1, per_cu) * prop.multiProcessorCount;
         if (const char *s = getenv("HALOPOT_BIG_THRESHOLD")) big_threshold = atoi(s);
         if (big_threshold > kSmallMax) big_threshold = kSmallMax;
         if (big_threshold < kLeafPoints) big_threshold = kLeafPoints;
@@ -271,7 +280,7 @@ struct Engine {
     ~Engine() {
         DevBuf *all[] = {&hfirst, &hcount, &root_tiles, &groups, &direct_blocks, &rec_call,
                          &scan_tiles, &call_tile0, &call_ntiles, &box, &bad, &badlist, &A, &B, &F,
-                         &leaf_of, &nodes, &parent, &fat, &small0, &big, &tiles,
+                         &leaf_of, &nodes, &parent, &fat, &small0, &group0, &big, &tiles,
                          &parts, &splitres, &dparts, &counters, &work, &depth_buf, &keys0, &keys1, &vals0,
                          &vals1, &hist, &hsum, &srt, &callinfo, &tsum, &tcand, &tile_i32};
         if (stream) {
@@ -313,6 +322,8 @@ struct Engine {
         HP_ENSURE(fat, sizeof(int32_t) * cap_fat);
         cap_small = N / (kLeafPoints + 1) + H + 64;
         HP_ENSURE(small0, sizeof(Task) * cap_small);
+        cap_group = N / (kWaveMin + 1) + H + 64;  // tasks of more than wave_max >= kWaveMin points
+        HP_ENSURE(group0, sizeof(Task) * cap_group);
         const int64_t bt = big_threshold > 0 ? big_threshold : 1;
         cap_big = 128 * (N / bt + 1) + H + 64;
         cap_tiles = 128 * (N / kTile + 2) + cap_big;
@@ -369,6 +380,7 @@ struct Engine {
         a.leaf_of = leaf_of.as<int32_t>();
         a.fin = F.as<float4>();
         a.small_next = small_next;
+        a.group_next = group0.as<Task>();
         a.big = big.as<Task>();
         a.tiles = tiles.as<int2>();
         a.fat = fat.as<int32_t>();
@@ -376,6 +388,7 @@ struct Engine {
         a.big_threshold = big_threshold;
         a.wave_max = wave_max;
         a.cap_small = (int32_t)cap_small;
+        a.cap_group = (int32_t)cap_group;
         a.cap_big = (int32_t)cap_big;
         a.cap_tiles = (int32_t)cap_tiles;
         a.cap_nodes = (int32_t)cap_nodes;
@@ -454,10 +467,10 @@ struct Engine {
     int enqueue_multipole(int levels) {
         hipStream_t s = stream;
         const int32_t *cnt = counters.as<int32_t>();
-        k_subtree<<<2048, kSubThreads, 0, s>>>(A.as<float4>(), B.as<float4>(), small0.as<Task>(),
+        k_subtree<<<sub_grid, kSubThreads, 0, s>>>(A.as<float4>(), B.as<float4>(), group0.as<Task>(),
                                                build_args(small0.as<Task>()), PARTICLE_MASS);
         HP_LAUNCHED();
-        k_subtree_wave<<<8192, kWave, 0, s>>>(A.as<float4>(), B.as<float4>(), small0.as<Task>(),
+        k_subtree_wave<<<wave_grid, kWave, 0, s>>>(A.as<float4>(), B.as<float4>(), small0.as<Task>(),
                                               build_args(small0.as<Task>()), PARTICLE_MASS);
         HP_LAUNCHED();
         k_leaf_com<<<2048, 256, 0, s>>>(nodes.as<Node>(), F.as<float4>(), cnt, PARTICLE_MASS);
@@ -1003,7 +1016,7 @@ __attribute__((visibility("default"))) int hp_plan_tune(hp_plan *p, struct poten
     o.timing = 0;
     o.count_work = 0;
     const int32_t bts[] = {128, 256, 512};
-    const int32_t wms[] = {32, 64};
+    const int32_t wms[] = {kWaveMin, kWaveTask};
     const int32_t dms[] = {kLeafPoints, 0};
     const int ndm = p->opt.method == HP_METHOD_AUTO ? 2 : 1;
     hp_tuning best{};
