@@ -170,6 +170,54 @@ typedef struct hp_tuning {
  * are not tuned (HP_EINVAL). */
 int hp_plan_tune(hp_plan *plan, struct potential *d_po, int reps, hp_tuning *out);
 
+/* ------------------------------------------------------- halo properties */
+
+/* Density thresholds of the mass definitions, in particles per (Mpc/h)^3:
+ * particle_thresh_dens[0..4] and particle_rvir_dens as calc_mass_definition
+ * leaves them (src/groupies.c:102-142; float-rounded values). */
+typedef struct hp_mass_defs {
+    double thresh_dens[5];  /* MASS_DEFINITION, MASS_DEFINITION2..5 */
+    double rvir_dens;       /* "vir" */
+} hp_mass_defs;
+
+/* calc_mass_definition (src/groupies.c:124-142) for the five definition
+ * strings ("vir", "<x>b", "<x>c", optionally prefixed "m"; NULL entries take
+ * the halo finder's defaults vir, 200b, 200c, 500c, 2500c,
+ * src/config.template.h:9-13), with the current SCALE_NOW, PARTICLE_MASS,
+ * Om, Ol, W0 and WA.  Host code only. */
+int hp_mass_definitions(const char *const defs[5], hp_mass_defs *out);
+
+/* One call's results of the halo property pass: src/properties.c:384-510
+ * run for bound = 0 and then bound = 1 (src/properties.c:552-553), with the
+ * defaults BOUND_PROPS = 1 and LIGHTCONE = 0.  The fields named after
+ * struct halo (src/halo.h:8-35) are the values the reference stores there;
+ * calc_shape, calc_scale_radius and _calc_pseudo_evolution_masses (shape,
+ * rs, klypin_rs, m_pe_b, m_pe_d) are not computed.  Radii in kpc/h. */
+typedef struct hp_halo_props {
+    float   m;               /* bound = 0 pass: part_mdelta * PARTICLE_MASS */
+    float   mgrav;           /* bound = 1 pass */
+    float   r;               /* final h->r, from part_mdelta */
+    float   vmax, rvmax, vrms, Xoff, Voff;
+    float   alt_m[4];
+    float   halfmass_radius;
+    float   J[3];
+    float   energy, kin_to_pot, spin, bullock_spin;
+    float   rvir;            /* the pass's virial radius (used by bullock_spin) */
+    int32_t pad;
+    int64_t n_mdelta;        /* part_mdelta: bound records within the main definition */
+    int64_t dens_tot;        /* records [0, dens_tot) enter the averages */
+    int64_t n_vir;           /* bound records within the virial definition */
+    int64_t n_half;          /* index of the record behind halfmass_radius */
+} hp_halo_props;
+
+/* The property pass of every call of a plan, on the device results of the
+ * last hp_plan_run of that plan: d_po and d_halos as hp_plan_run left them
+ * (records sorted by r2 with pe and ke set; first, kept and the centre,
+ * pos[3..5] = corevel).  d_props receives one hp_halo_props per call
+ * (device memory).  Synchronous like hp_plan_run. */
+int hp_plan_props(hp_plan *plan, const struct potential *d_po, const hp_halo *d_halos,
+                  const hp_mass_defs *defs, hp_halo_props *d_props, void *stream);
+
 /* Host-buffer convenience: copies in, hp_plan_run, copies out. */
 int hp_bound_pass_host(struct potential *po, int64_t nrecords, hp_halo *halos,
                        int64_t nhalos, uint8_t *bound, const hp_options *opt);

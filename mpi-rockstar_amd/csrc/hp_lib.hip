@@ -18,6 +18,7 @@
 
 #include "hp_build.hip"
 #include "hp_unbind.hip"
+#include "hp_props.hip"
 #include "hp_walk.hip"
 
 #include <algorithm>
@@ -117,6 +118,27 @@ static double particle_rvir_dens() {
     float scale_now = (float)SCALE_NOW;
     float cons = (float)(Om * 2.77519737e11 / PARTICLE_MASS);
     float thresh = (float)(vir_density(scale_now) * cons);
+    return (double)thresh;
+}
+
+// _calc_mass_definition (src/groupies.c:102-122): a threshold in particles
+// per (Mpc/h)^3 for "vir", "<x>b" (x times the background density) or
+// "<x>c" (x times the critical density), optionally prefixed "m"; any other
+// string is "vir".  Float scale factor, matter fraction, background density
+// and result, as there.
+static double mass_definition(const char *md) {
+    const float scale_now = (float)SCALE_NOW;
+    const size_t length = strlen(md);
+    const char last = length ? md[length - 1] : 0;
+    const float matter_fraction =
+        (float)((Om / pow(scale_now, 3)) / pow(hubble_scaling(1.0 / scale_now - 1.0), 2.0));
+    const float cons = (float)(Om * 2.77519737e11 / PARTICLE_MASS);  // background density
+    const char *mass = md;
+    if (mass[0] == 'm' || mass[0] == 'M') mass++;
+    float thresh;
+    if (last == 'b' || last == 'B') thresh = (float)(atof(mass) * cons);
+    else if (last == 'c' || last == 'C') thresh = (float)(atof(mass) * cons / matter_fraction);
+    else thresh = (float)(vir_density(scale_now) * cons);
     return (double)thresh;
 }
 
@@ -240,6 +262,8 @@ struct Engine {
         splitres, dparts, counters, work, depth_buf;
     // unbind
     DevBuf keys0, keys1, vals0, vals1, hist, hsum, srt, callinfo, tsum, tcand, tile_i32;
+    // halo properties
+    DevBuf pcnt, pscan, pcall, psums;
     int64_t cap_nodes = 0, cap_small = 0, cap_group = 0, cap_big = 0, cap_tiles = 0, cap_fat = 0;
     std::vector<int32_t> depth_end;  // node count after each depth's level
 
@@ -281,7 +305,7 @@ struct Engine {
         DevBuf *all[] = {&hfirst, &hcount, &root_tiles, &groups, &direct_blocks, &rec_call,
                          &scan_tiles, &call_tile0, &call_ntiles, &box, &bad, &badlist, &A, &B, &F,
                          &leaf_of, &nodes, &parent, &fat, &small0, &group0, &big, &tiles,
-                         &parts, &splitres, &dparts, &counters, &work, &depth_buf, &keys0, &keys1, &vals0,
+                         &parts, &splitres, &dparts, &counters, &work, &depth_buf, &pcnt, &pscan, &pcall, &psums, &keys0, &keys1, &vals0,
                          &vals1, &hist, &hsum, &srt, &callinfo, &tsum, &tcand, &tile_i32};
         if (stream) {
             (void)hipStreamSynchronize(stream);
@@ -782,6 +806,57 @@ struct Engine {
     int32_t *sorted_vals = nullptr;
     uint64_t *sorted_keys = nullptr;
 
+    // the halo property pass over the bound pass's results (sorted records,
+    // halos with kept and centre); T's tables must be the loaded ones
+    int props(const Tables &T, const struct potential *d_po, const hp_halo *d_halos,
+              const hp_mass_defs &md, hp_halo_props *d_props) {
+        const int32_t H = (int32_t)T.nhalos, nst = (int32_t)T.scan_tiles.size();
+        if (H == 0) return HP_OK;
+        HP_ENSURE(pcnt, sizeof(PropCount) * (nst + 1));
+        HP_ENSURE(pscan, sizeof(PropScan) * (nst + 1));
+        HP_ENSURE(psums, sizeof(PropSums) * (nst + 1));
+        HP_ENSURE(pcall, sizeof(PropCall) * (H + 1));
+        PropArgs pa;
+        const double four_thirds_pi = 4.0 * M_PI / 3.0;  // src/properties.c:393-398
+        pa.dens_thresh = md.thresh_dens[0] * four_thirds_pi;
+        for (int k = 0; k < 4; k++) pa.d[k] = md.thresh_dens[k + 1] * four_thirds_pi;
+        pa.rvir_thresh = md.rvir_dens * four_thirds_pi;
+        pa.thresh0 = md.thresh_dens[0];
+        pa.rvir_dens = md.rvir_dens;
+        pa.force_res = FORCE_RES;
+        pa.particle_mass = PARTICLE_MASS;
+        pa.scale_now = (float)SCALE_NOW;
+        hipStream_t s = stream;
+        const ScanTile *tl = scan_tiles.as<ScanTile>();
+        const int32_t *t0 = call_tile0.as<int32_t>(), *tn = call_ntiles.as<int32_t>();
+        const int64_t *hf = hfirst.as<int64_t>(), *hc = hcount.as<int64_t>();
+        if (nst > 0) {
+            k_props_counts<<<nst, kWave, 0, s>>>(d_po, tl, nst, d_halos, hf, hc,
+                                                 pcnt.as<PropCount>());
+            HP_LAUNCHED();
+        }
+        k_props_prefix<<<blocks(H, 256), 256, 0, s>>>(t0, tn, H, pcnt.as<PropCount>());
+        HP_LAUNCHED();
+        if (nst > 0) {
+            k_props_scan<<<nst, kWave, 0, s>>>(d_po, tl, nst, d_halos, hf, hc,
+                                               pcnt.as<PropCount>(), pa, pscan.as<PropScan>());
+            HP_LAUNCHED();
+        }
+        k_props_calls<<<blocks(H, 256), 256, 0, s>>>(t0, tn, H, d_halos, hc,
+                                                     pscan.as<PropScan>(), pcall.as<PropCall>());
+        HP_LAUNCHED();
+        if (nst > 0) {
+            k_props_sums<<<nst, kWave, 0, s>>>(d_po, tl, nst, d_halos, hf, pcnt.as<PropCount>(),
+                                               pcall.as<PropCall>(), pa, psums.as<PropSums>());
+            HP_LAUNCHED();
+        }
+        k_props_final<<<blocks(H, 256), 256, 0, s>>>(d_po, t0, tn, H, d_halos, hf,
+                                                     pcall.as<PropCall>(), psums.as<PropSums>(),
+                                                     pa, d_props);
+        HP_LAUNCHED();
+        return HP_OK;
+    }
+
     int enqueue_unbind(const Tables &T, struct potential *d_po, hp_halo *d_halos,
                        uint8_t *d_bound) {
         hipStream_t s = stream;
@@ -1123,6 +1198,33 @@ __attribute__((visibility("default"))) int hp_plan_run(hp_plan *p, struct potent
         HP_TRY(hipEventElapsedTime(&t, p->eng.ev[0], e1));
         p->eng.st.ms_total = t;
     }
+    HP_TRY(hipStreamSynchronize(p->eng.stream));
+    return HP_OK;
+}
+
+__attribute__((visibility("default"))) int hp_mass_definitions(const char *const defs[5],
+                                                               hp_mass_defs *out) {
+    if (!out) return set_error(HP_EINVAL, "hp_mass_definitions: bad arguments");
+    static const char *const dflt[5] = {"vir", "200b", "200c", "500c", "2500c"};
+    if (!(PARTICLE_MASS > 0.0) || !(SCALE_NOW > 0.0))
+        return set_error(HP_EINVAL, "hp_mass_definitions: PARTICLE_MASS and SCALE_NOW must be > 0");
+    for (int k = 0; k < 5; k++)
+        out->thresh_dens[k] = mass_definition(defs && defs[k] ? defs[k] : dflt[k]);
+    out->rvir_dens = mass_definition("vir");
+    return HP_OK;
+}
+
+__attribute__((visibility("default"))) int hp_plan_props(hp_plan *p, const struct potential *d_po,
+                                                         const hp_halo *d_halos,
+                                                         const hp_mass_defs *defs,
+                                                         hp_halo_props *d_props, void *stream) {
+    if (!p || !defs || ((!d_po || !d_halos || !d_props) && p->T.nhalos))
+        return set_error(HP_EINVAL, "hp_plan_props: bad arguments");
+    (void)stream;
+    int r;
+    // a per-call plan's runs load single-call tables; the pass needs the batch's
+    if (p->opt.per_call && (r = p->eng.load_tables(p->T))) return r;
+    if ((r = p->eng.props(p->T, d_po, d_halos, *defs, d_props))) return r;
     HP_TRY(hipStreamSynchronize(p->eng.stream));
     return HP_OK;
 }

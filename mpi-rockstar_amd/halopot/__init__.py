@@ -8,7 +8,8 @@ Mirrors the reference's operator interface for this path:
     src/potential.h:22-23.
   * ``set_config(params)`` -- the configuration globals the path reads
     (FORCE_RES, PARTICLE_MASS, Om, Ol, W0, WA, SCALE_NOW).
-plus the batched path (``Plan``) and ``bound_pass_host``.
+plus the batched path (``Plan``, whose ``props`` runs the halo property pass
+of src/properties.c:384-510), ``mass_definitions`` and ``bound_pass_host``.
 
 There is no CPU fallback: if libhalopot.so is missing or no gfx950 device is
 usable, every call raises.
@@ -21,7 +22,7 @@ import pathlib
 import numpy as np
 
 from . import catalog  # noqa: F401  (re-exported)
-from .catalog import HALO_DTYPE, POTENTIAL_DTYPE, Params
+from .catalog import HALO_DTYPE, POTENTIAL_DTYPE, PROPS_DTYPE, Params  # noqa: F401
 
 PKG = pathlib.Path(__file__).resolve().parent
 LIB_PATH = PKG / "libhalopot.so"
@@ -63,6 +64,13 @@ class Stats(C.Structure):
         return {f: getattr(self, f) for f, _ in self._fields_}
 
 
+class MassDefs(C.Structure):
+    _fields_ = [("thresh_dens", C.c_double * 5), ("rvir_dens", C.c_double)]
+
+    def as_array(self) -> np.ndarray:
+        return np.array(list(self.thresh_dens) + [self.rvir_dens], np.float64)
+
+
 class Tuning(C.Structure):
     _fields_ = [("big_threshold", C.c_int32), ("wave_max", C.c_int32),
                 ("direct_max", C.c_int32), ("ms_potential", C.c_float)]
@@ -75,7 +83,7 @@ _LIB = None
 
 EXPORTS = ("compute_potential", "compute_kinetic_energy", "hp_plan_create", "hp_plan_run",
            "hp_plan_potential", "hp_plan_stats", "hp_plan_destroy", "hp_plan_tune",
-           "hp_bound_pass_host",
+           "hp_mass_definitions", "hp_plan_props", "hp_bound_pass_host",
            "hp_debug_tree", "hp_last_error", "hp_set_error_handler")
 GLOBALS = ("FORCE_RES", "PARTICLE_MASS", "Om", "Ol", "W0", "WA", "SCALE_NOW")
 
@@ -99,6 +107,9 @@ def lib() -> C.CDLL:
         L.hp_plan_stats.argtypes = [C.c_void_p, C.POINTER(Stats)]
         L.hp_plan_destroy.argtypes = [C.c_void_p]
         L.hp_plan_tune.argtypes = [C.c_void_p, C.c_void_p, C.c_int, C.POINTER(Tuning)]
+        L.hp_mass_definitions.argtypes = [C.POINTER(C.c_char_p), C.POINTER(MassDefs)]
+        L.hp_plan_props.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p, C.POINTER(MassDefs),
+                                    C.c_void_p, C.c_void_p]
         L.hp_bound_pass_host.argtypes = [C.c_void_p, C.c_int64, C.c_void_p, C.c_int64,
                                          C.c_void_p, C.POINTER(Options)]
         L.hp_debug_tree.argtypes = [C.c_void_p, C.c_int64, C.c_void_p, C.c_void_p, C.c_int64]
@@ -159,6 +170,19 @@ def options(accumulate=ACC_COMPENSATED, method=METHOD_TREE, use_graph=0, timing=
     return o
 
 
+def mass_definitions(defs=None) -> MassDefs:
+    """calc_mass_definition (src/groupies.c:124-142) with the current
+    configuration globals: particle_thresh_dens[0..4] for the five mass
+    definitions (None: the halo finder's defaults vir, 200b, 200c, 500c,
+    2500c) and particle_rvir_dens."""
+    arr = (C.c_char_p * 5)()
+    for k in range(5):
+        arr[k] = defs[k].encode() if defs is not None and defs[k] is not None else None
+    out = MassDefs()
+    _check(lib().hp_mass_definitions(arr, C.byref(out)))
+    return out
+
+
 def bound_pass_host(po: np.ndarray, halos: np.ndarray, opts: Options | None = None,
                     want_bound: bool = True):
     """Batched bound pass on host arrays (copied in and out).  Returns the
@@ -214,6 +238,14 @@ class Plan:
 
     def potential(self, d_po: int) -> None:
         _check(lib().hp_plan_potential(self._h, C.c_void_p(d_po), None))
+
+    def props(self, d_po: int, d_halos: int, d_props: int, defs: MassDefs | None = None) -> None:
+        """hp_plan_props: the halo property pass (src/properties.c:384-510)
+        on the results of the last run(); d_props receives one PROPS_DTYPE
+        record per call."""
+        md = defs if defs is not None else mass_definitions()
+        _check(lib().hp_plan_props(self._h, C.c_void_p(d_po), C.c_void_p(d_halos), C.byref(md),
+                                   C.c_void_p(d_props), None))
 
     def stats(self) -> dict:
         s = Stats()

@@ -40,6 +40,25 @@ POTENTIAL_DTYPE = np.dtype(
     }
 )
 
+# hp_halo_props (include/halopot.h): the halo property pass's results
+PROPS_FLOAT_FIELDS = ("m", "mgrav", "r", "vmax", "rvmax", "vrms", "Xoff", "Voff", "alt_m",
+                      "halfmass_radius", "J", "energy", "kin_to_pot", "spin", "bullock_spin",
+                      "rvir")
+PROPS_INT_FIELDS = ("n_mdelta", "dens_tot", "n_vir", "n_half")
+PROPS_DTYPE = np.dtype(
+    {
+        "names": list(PROPS_FLOAT_FIELDS) + ["pad"] + list(PROPS_INT_FIELDS),
+        "formats": ["<f4"] * 8 + [("<f4", (4,)), "<f4", ("<f4", (3,))] + ["<f4"] * 5
+                   + ["<i4"] + ["<i8"] * 4,
+        "offsets": [0, 4, 8, 12, 16, 20, 24, 28, 32, 48, 52, 64, 68, 72, 76, 80, 84,
+                    88, 96, 104, 112],
+        "itemsize": 120,
+    }
+)
+
+# mass definitions of the halo finder's defaults (src/config.template.h:9-13)
+DEFAULT_MASS_DEFINITIONS = ("vir", "200b", "200c", "500c", "2500c")
+
 HALO_DTYPE = np.dtype(
     {
         "names": [

@@ -104,7 +104,12 @@ class Restatement:
         L.orc_hubble_scaling.restype = C.c_double
         L.orc_hubble_scaling.argtypes = [C.POINTER(_Params), C.c_double]
         L.orc_max_threads.restype = C.c_int
-        for f in ("orc_sizeof_node", "orc_sizeof_halo", "orc_sizeof_potential"):
+        L.orc_halo_props.argtypes = [C.c_void_p, C.c_int64, C.c_void_p, C.POINTER(_Params),
+                                     C.c_void_p, C.c_void_p]
+        L.orc_halo_props_batch.argtypes = [C.c_void_p, C.c_void_p, C.c_int64, C.POINTER(_Params),
+                                           C.c_void_p, C.c_void_p, C.c_int]
+        for f in ("orc_sizeof_node", "orc_sizeof_halo", "orc_sizeof_potential",
+                  "orc_sizeof_props"):
             getattr(L, f).restype = C.c_int64
         assert L.orc_sizeof_node() == NODE_DTYPE.itemsize
         assert L.orc_sizeof_potential() == 48
@@ -207,6 +212,28 @@ class Restatement:
     def hubble_scaling(self, prm, z):
         return self.lib.orc_hubble_scaling(C.byref(_params(prm)), z)
 
+    def halo_props(self, po: np.ndarray, kept: int, pos, prm, thresh, out: np.ndarray) -> None:
+        """Property pass (src/properties.c:384-510, bound = 0 then 1) of one
+        call's kept records, sorted by r2 with pe and ke; out: one record of
+        the product's PROPS_DTYPE layout (120 bytes)."""
+        _check_records(po)
+        assert out.dtype.itemsize == self.lib.orc_sizeof_props() and out.flags.c_contiguous
+        ps = np.ascontiguousarray(pos, np.float32)
+        th = np.ascontiguousarray(thresh, np.float64)
+        self.lib.orc_halo_props(_p(po), kept, _p(ps), C.byref(_params(prm)), _p(th), _p(out))
+
+    def halo_props_batch(self, po: np.ndarray, halos: np.ndarray, prm, thresh, out: np.ndarray,
+                         nthreads: int = 0) -> None:
+        """halo_props for every call of a batch after its bound pass: halos
+        carry first, kept and the centre (pos[3..5] = corevel); OpenMP
+        dynamic over calls."""
+        _check_records(po)
+        assert out.dtype.itemsize == self.lib.orc_sizeof_props() and out.flags.c_contiguous
+        assert halos.dtype.itemsize == self.lib.orc_sizeof_halo() and halos.flags.c_contiguous
+        th = np.ascontiguousarray(thresh, np.float64)
+        self.lib.orc_halo_props_batch(_p(po), _p(halos), halos.shape[0], C.byref(_params(prm)),
+                                      _p(th), _p(out), nthreads)
+
     def max_threads(self):
         return self.lib.orc_max_threads()
 
@@ -243,3 +270,57 @@ class Reference:
         vc = np.ascontiguousarray(vel_cen, np.float32)
         pc = np.ascontiguousarray(pos_cen, np.float32)
         self.lib.compute_kinetic_energy(_p(po), po.shape[0], _p(vc), _p(pc), scale_now)
+
+
+class ReferenceProps:
+    """The reference's own halo property pass: src/properties.c (inside
+    src/groupies.c) with src/nfw.c, src/jacobi.c, src/hubble.c and
+    src/config_vars.c, compiled by oracle/Makefile and called through the
+    harness oracle/ref_props.c."""
+
+    FIELDS = ("m", "mgrav", "r", "vmax", "rvmax", "vrms", "Xoff", "Voff", "alt_m",
+              "halfmass_radius", "J", "energy", "kin_to_pot", "spin", "bullock_spin",
+              "rs", "klypin_rs", "b_to_a", "c_to_a", "A", "b_to_a2", "c_to_a2", "A2",
+              "m_pe_b", "m_pe_d")
+    DTYPE = np.dtype([(f, "<f4", (4,)) if f == "alt_m" else
+                      (f, "<f4", (3,)) if f in ("J", "A", "A2") else (f, "<f4") for f in FIELDS])
+
+    def __init__(self, path: str | os.PathLike | None = None):
+        path = pathlib.Path(path or HERE / "_ref" / "libref_props.so")
+        if not path.exists():
+            raise FileNotFoundError(f"{path} missing: run make -C oracle where the reference exists")
+        self.lib = L = C.CDLL(str(path))
+        L.ref_halo_props.argtypes = [C.c_void_p, C.c_int64, C.c_void_p, C.c_void_p, C.c_void_p]
+        L.ref_mass_definition.argtypes = [C.c_void_p]
+
+    @staticmethod
+    def available(path=None) -> bool:
+        return pathlib.Path(path or HERE / "_ref" / "libref_props.so").exists()
+
+    def set_globals(self, prm, mass_defs=None) -> None:
+        for name, val in (("FORCE_RES", prm.force_res), ("PARTICLE_MASS", prm.particle_mass),
+                          ("Om", prm.Om), ("Ol", prm.Ol), ("W0", prm.W0), ("WA", prm.WA),
+                          ("SCALE_NOW", prm.scale_now)):
+            C.c_double.in_dll(self.lib, name).value = val
+        if mass_defs is not None:
+            # the char * globals point at buffers this object keeps alive
+            self._defs = [C.create_string_buffer(d.encode()) for d in mass_defs]
+            for name, d in zip(("MASS_DEFINITION", "MASS_DEFINITION2", "MASS_DEFINITION3",
+                                "MASS_DEFINITION4", "MASS_DEFINITION5"), self._defs):
+                C.c_void_p.in_dll(self.lib, name).value = C.addressof(d)
+
+    def mass_definition(self, prm, mass_defs) -> np.ndarray:
+        """particle_thresh_dens[0..4] and particle_rvir_dens."""
+        self.set_globals(prm, mass_defs)
+        out = np.zeros(6, np.float64)
+        self.lib.ref_mass_definition(_p(out))
+        return out
+
+    def halo_props(self, po: np.ndarray, kept: int, pos, prm, thresh) -> np.ndarray:
+        _check_records(po)
+        self.set_globals(prm)
+        ps = np.ascontiguousarray(pos, np.float32)
+        th = np.ascontiguousarray(thresh, np.float64)
+        out = np.zeros(1, self.DTYPE)
+        self.lib.ref_halo_props(_p(po), kept, _p(ps), _p(th), _p(out))
+        return out[0]

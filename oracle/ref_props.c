@@ -1,0 +1,88 @@
+/*
+ * oracle/ref_props.c -- a calling harness around the reference's own halo
+ * property pass, for the checker library _ref/libref_props.so.
+ *
+ * TEST INFRASTRUCTURE ONLY (see restate.c).  Nothing here restates the
+ * reference: the harness fills the reference's globals and `struct halo`,
+ * calls its functions and copies their results out.
+ *
+ *   ref_halo_props      -> _calc_additional_halo_props(h, total_p, 0) and
+ *                          (h, total_p, 1), as src/properties.c:552-553 calls
+ *                          them (src/properties.c:384-510 with calc_shape,
+ *                          calc_scale_radius, estimate_total_energy and
+ *                          _calc_pseudo_evolution_masses inside)
+ *   ref_mass_definition -> calc_mass_definition (src/groupies.c:124-142)
+ *
+ * The records must already be in the state src/properties.c:551 leaves them:
+ * potential-only records dropped, sorted by r2, pe and ke set.
+ */
+#include <stdint.h>
+#include <string.h>
+
+#include "halo.h"
+#include "potential.h"
+
+extern struct potential *po; /* src/groupies.c:33 */
+#pragma omp threadprivate(po)
+extern double particle_thresh_dens[5]; /* src/groupies.c:56 */
+extern double particle_rvir_dens;
+#pragma omp threadprivate(particle_thresh_dens, particle_rvir_dens)
+
+void _calc_additional_halo_props(struct halo *h, int64_t total_p, int64_t bound);
+void calc_mass_definition(void);
+
+typedef struct ref_props {
+    float m, mgrav, r, vmax, rvmax, vrms, Xoff, Voff, alt_m[4], halfmass_radius;
+    float J[3], energy, kin_to_pot, spin, bullock_spin;
+    float rs, klypin_rs, b_to_a, c_to_a, A[3], b_to_a2, c_to_a2, A2[3];
+    float m_pe_b, m_pe_d;
+} ref_props;
+
+void ref_mass_definition(double out[6]) {
+    calc_mass_definition();
+    for (int k = 0; k < 5; k++) out[k] = particle_thresh_dens[k];
+    out[5] = particle_rvir_dens;
+}
+
+/* thresh: particle_thresh_dens[0..4] then particle_rvir_dens, as
+   ref_mass_definition returns them (the halo finder computes them once per
+   snapshot, src/groupies.c:124) */
+void ref_halo_props(struct potential *recs, int64_t total_p, const float pos[6],
+                    const double thresh[6], ref_props *out) {
+    struct halo h;
+    memset(&h, 0, sizeof h);
+    memcpy(h.pos, pos, sizeof h.pos);
+    for (int k = 0; k < 5; k++) particle_thresh_dens[k] = thresh[k];
+    particle_rvir_dens = thresh[5];
+    po = recs;
+    _calc_additional_halo_props(&h, total_p, 0);
+    _calc_additional_halo_props(&h, total_p, 1);
+    po = NULL;
+    ref_props o;
+    o.m = h.m;
+    o.mgrav = h.mgrav;
+    o.r = h.r;
+    o.vmax = h.vmax;
+    o.rvmax = h.rvmax;
+    o.vrms = h.vrms;
+    o.Xoff = h.Xoff;
+    o.Voff = h.Voff;
+    memcpy(o.alt_m, h.alt_m, sizeof o.alt_m);
+    o.halfmass_radius = h.halfmass_radius;
+    memcpy(o.J, h.J, sizeof o.J);
+    o.energy = h.energy;
+    o.kin_to_pot = h.kin_to_pot;
+    o.spin = h.spin;
+    o.bullock_spin = h.bullock_spin;
+    o.rs = h.rs;
+    o.klypin_rs = h.klypin_rs;
+    o.b_to_a = h.b_to_a;
+    o.c_to_a = h.c_to_a;
+    memcpy(o.A, h.A, sizeof o.A);
+    o.b_to_a2 = h.b_to_a2;
+    o.c_to_a2 = h.c_to_a2;
+    memcpy(o.A2, h.A2, sizeof o.A2);
+    o.m_pe_b = h.m_pe_b;
+    o.m_pe_d = h.m_pe_d;
+    *out = o;
+}

@@ -25,6 +25,11 @@
  *   - bound-pass driver .................. src/properties.c:537-550 and the
  *                                           bound test at src/properties.c:403
  *   - vir_density / particle_rvir_dens ... src/groupies.c:97-142
+ *   - halo property pass ................. src/properties.c:127-137 (angular
+ *                                           momentum), 328-349 (energy),
+ *                                           384-510 (without calc_shape,
+ *                                           calc_scale_radius and the
+ *                                           pseudo-evolution masses)
  *
  * Deliberate differences from the reference, none of which changes a node's
  * membership, a walk decision or a value beyond f64 summation order:
@@ -679,6 +684,153 @@ void orc_bound_pass_batch_fn(struct potential *po, const int64_t *first,
         kept[h] = orc_bound_pass_fn(po + first[h], count[h], halos + h, p, cp, ke);
 }
 
+/* ------------------------------------------------------ halo properties */
+
+/* Output of the property pass, in the layout of the product's
+   hp_halo_props (include/halopot.h). */
+typedef struct orc_props {
+    float   m, mgrav, r, vmax, rvmax, vrms, Xoff, Voff;
+    float   alt_m[4];
+    float   halfmass_radius;
+    float   J[3];
+    float   energy, kin_to_pot, spin, bullock_spin;
+    float   rvir;
+    int32_t pad;
+    int64_t n_mdelta, dens_tot, n_vir, n_half;
+} orc_props;
+
+#define ORC_VMAX_CONST 6.55833746e-5 /* src/universal_constants.h:6 */
+
+/* src/properties.c:328-349 over records [0, n), highest index first */
+static float orc_total_energy(const struct potential *po, int64_t n,
+                              const orc_params *p, float *ratio) {
+    double phi = 0, total_phi = 0, ke = 0;
+    for (int64_t i = n - 1; i >= 0; i--) {
+        if (!(po[i].pe > po[i].ke)) continue;
+        ke += po[i].ke;
+        double r = sqrt(po[i].r2);
+        if (r < p->force_res) r = p->force_res;
+        total_phi += p->particle_mass * i / r + phi;
+        phi += p->particle_mass / r;
+    }
+    total_phi /= 2.0;
+    *ratio = 0;
+    if (total_phi) *ratio = (float)(ke / total_phi);
+    float scale_now = (float)p->scale_now;
+    return (float)((ke - total_phi) * p->particle_mass * ORC_GC / scale_now);
+}
+
+/* src/properties.c:384-510 with the defaults BOUND_PROPS = 1 and LIGHTCONE
+   = 0, run for bound = 0 and then bound = 1 as src/properties.c:552-553
+   does; calc_shape, calc_scale_radius and _calc_pseudo_evolution_masses are
+   not restated.  thresh[0..4] = particle_thresh_dens, thresh[5] =
+   particle_rvir_dens.  po: the kept records sorted by r2 with pe and ke. */
+void orc_halo_props(const struct potential *po, int64_t total_p, const float pos[6],
+                    const orc_params *p, const double thresh[6], orc_props *out) {
+    memset(out, 0, sizeof *out);
+    const double four_thirds_pi = 4.0 * M_PI / 3.0;
+    const float scale_now = (float)p->scale_now;
+    for (int bound = 0; bound <= 1; bound++) {
+        double dens_thresh = thresh[0] * four_thirds_pi;
+        double d[4];
+        for (int k = 0; k < 4; k++) d[k] = thresh[k + 1] * four_thirds_pi;
+        double rvir_thresh = thresh[5] * four_thirds_pi;
+        int64_t part_mdelta = 0, num_part = 0, np_alt[4] = {0, 0, 0, 0}, np_vir = 0;
+        int64_t dens_tot = 0, parts_avgd = 0, num_part_half = 0;
+        double vmax = 0, rvmax = 0, L[3] = {0, 0, 0};
+        double xavg[3] = {0, 0, 0}, vavg[3] = {0, 0, 0}, vrms[3] = {0, 0, 0};
+        for (int64_t j = 0; j < total_p; j++) {
+            if (bound && po[j].pe < po[j].ke) continue;
+            num_part++;
+            double r = sqrt(po[j].r2);
+            if (r < p->force_res) r = p->force_res;
+            double cur = (double)num_part / (r * r * r);
+            if (cur > dens_thresh) {
+                part_mdelta = num_part;
+                dens_tot = j;
+            }
+            for (int k = 0; k < 4; k++)
+                if (cur > d[k]) np_alt[k] = num_part;
+            if (cur > rvir_thresh) {
+                double circ = (double)num_part / r;
+                np_vir = num_part;
+                if (part_mdelta && circ > vmax) {
+                    vmax = circ;
+                    rvmax = r;
+                }
+            }
+        }
+        for (int64_t j = 0; j < dens_tot; j++) {
+            if (bound && po[j].pe < po[j].ke) continue;
+            const float *x = po[j].pos;
+            /* src/properties.c:127-137: float differences and products */
+            L[0] += (float)((x[1] - pos[1]) * (x[5] - pos[5]));
+            L[0] -= (float)((x[2] - pos[2]) * (x[4] - pos[4]));
+            L[1] += (float)((x[2] - pos[2]) * (x[3] - pos[3]));
+            L[1] -= (float)((x[0] - pos[0]) * (x[5] - pos[5]));
+            L[2] += (float)((x[0] - pos[0]) * (x[4] - pos[4]));
+            L[2] -= (float)((x[1] - pos[1]) * (x[3] - pos[3]));
+            parts_avgd++;
+            if (parts_avgd * 2 >= part_mdelta && !num_part_half) num_part_half = j;
+            for (int k = 0; k < 3; k++) {
+                xavg[k] += (x[k] - xavg[k]) / ((double)parts_avgd);
+                double md = x[k + 3] - vavg[k];
+                vavg[k] += md / (double)parts_avgd;
+                vrms[k] += md * (x[k + 3] - vavg[k]);
+            }
+        }
+        double m = part_mdelta * p->particle_mass;
+        if (!bound) {
+            out->m = (float)m;
+            continue;
+        }
+        out->mgrav = (float)m;
+        for (int k = 0; k < 3; k++) vrms[k] = parts_avgd > 0 ? vrms[k] / parts_avgd : 0;
+        float xoff = 0, voff = 0; /* float fields, double terms */
+        for (int k = 0; k < 3; k++) {
+            double ds = xavg[k] - pos[k];
+            xoff = (float)(xoff + ds * ds);
+            ds = vavg[k] - pos[k + 3];
+            voff = (float)(voff + ds * ds);
+        }
+        for (int k = 0; k < 4; k++) out->alt_m[k] = (float)(np_alt[k] * p->particle_mass);
+        out->Xoff = (float)(sqrt(xoff) * 1e3);
+        out->Voff = (float)sqrt(voff);
+        out->vrms = (float)sqrt(vrms[0] + vrms[1] + vrms[2]);
+        out->vmax = (float)(ORC_VMAX_CONST * sqrt(vmax * (p->particle_mass / scale_now)));
+        out->rvmax = (float)(rvmax * 1e3);
+        out->halfmass_radius = total_p > 0 ? (float)(sqrt(po[num_part_half].r2) * 1e3) : 0.0f;
+        out->r = (float)(cbrt((3.0 / (4.0 * M_PI)) * part_mdelta / thresh[0]) * 1e3);
+        double rvir = cbrt((3.0 / (4.0 * M_PI)) * np_vir / thresh[5]) * 1e3;
+        double mvir = np_vir * p->particle_mass;
+        out->rvir = (float)rvir;
+        for (int k = 0; k < 3; k++) out->J[k] = (float)(p->particle_mass * scale_now * L[k]);
+        out->energy = orc_total_energy(po, dens_tot, p, &out->kin_to_pot);
+        double Jh = p->particle_mass * scale_now * sqrt(L[0] * L[0] + L[1] * L[1] + L[2] * L[2]);
+        out->spin = (float)((m > 0) ? (Jh * sqrt(fabs(out->energy)) / (ORC_GC * pow(m, 2.5))) : 0);
+        out->bullock_spin =
+            (float)((m > 0) ? (Jh / (mvir * sqrt(2.0 * ORC_GC * mvir * rvir * scale_now / 1e3)))
+                            : 0);
+        out->n_mdelta = part_mdelta;
+        out->dens_tot = dens_tot;
+        out->n_vir = np_vir;
+        out->n_half = num_part_half;
+    }
+}
+
+void orc_halo_props_batch(const struct potential *po, const orc_halo *halos, int64_t nhalos,
+                          const orc_params *p, const double thresh[6], orc_props *out,
+                          int nthreads) {
+#ifdef _OPENMP
+    if (nthreads > 0) omp_set_num_threads(nthreads);
+#else
+    (void)nthreads;
+#endif
+#pragma omp parallel for schedule(dynamic)
+    for (int64_t h = 0; h < nhalos; h++)
+        orc_halo_props(po + halos[h].first, halos[h].kept, halos[h].pos, p, thresh, out + h);
+}
+
 /* src/groupies.c:97-100 */
 double orc_vir_density(const orc_params *p, double a) {
     double e = orc_hubble_scaling(p, 1.0 / a - 1.0);
@@ -697,3 +849,4 @@ int orc_max_threads(void) {
 int64_t orc_sizeof_node(void) { return (int64_t)sizeof(orc_node); }
 int64_t orc_sizeof_halo(void) { return (int64_t)sizeof(orc_halo); }
 int64_t orc_sizeof_potential(void) { return (int64_t)sizeof(struct potential); }
+int64_t orc_sizeof_props(void) { return (int64_t)sizeof(orc_props); }

@@ -1,0 +1,116 @@
+"""GPU parity of the halo property pass (hp_plan_props; src/properties.c:
+384-510 for bound = 0 then bound = 1, src/properties.c:552-553) against the
+restatement, which the CPU tests pin bit for bit to the reference's own
+compiled code.
+
+Both sides read the same input: the GPU bound pass's own output records
+(sorted by r2, pe and ke set) and halos (kept, centre).  Tolerances:
+  * counts and indices (n_mdelta, dens_tot, n_vir, n_half) and every field
+    made from them alone (m, mgrav, alt_m, vmax, rvmax, halfmass_radius):
+    identical;
+  * r and rvir (cbrt on the device vs glibc): 2e-7 relative;
+  * the fields built on f64 sums (J, Xoff, Voff, vrms, energy, kin_to_pot,
+    spin, bullock_spin): 1e-5 relative, the reference's running means and
+    sequential sums being restated as sums of exact differences.
+"""
+import numpy as np
+import pytest
+
+import halopot
+from halopot import catalog
+from halopot.hipmem import DeviceArray, synchronize
+
+pytestmark = pytest.mark.gpu
+
+EXACT = ("m", "mgrav", "alt_m", "vmax", "rvmax", "halfmass_radius")
+CBRT = ("r", "rvir")
+SUMS = ("J", "Xoff", "Voff", "vrms", "energy", "kin_to_pot", "spin", "bullock_spin")
+
+
+def run_props(cat, defs=None, opts=None):
+    halopot.set_config(cat.params)
+    po = np.ascontiguousarray(cat.po.copy())
+    halos = np.ascontiguousarray(cat.halos.copy())
+    md = halopot.mass_definitions(defs)
+    plan = halopot.Plan(halos, opts)
+    d_po = DeviceArray.from_numpy(po)
+    d_h = DeviceArray.from_numpy(halos)
+    d_b = DeviceArray.zeros(max(1, po.shape[0]))
+    d_p = DeviceArray.zeros(max(1, halos.shape[0]) * catalog.PROPS_DTYPE.itemsize)
+    try:
+        plan.run(d_po.address, d_h.address, d_b.address)
+        plan.props(d_po.address, d_h.address, d_p.address, md)
+        synchronize()
+        po_g = d_po.download(np.empty_like(po))
+        h_g = d_h.download(np.empty_like(halos))
+        props = d_p.download(np.empty(halos.shape[0], catalog.PROPS_DTYPE))
+    finally:
+        plan.close()
+        for d in (d_po, d_h, d_b, d_p):
+            d.free()
+    return po_g, h_g, props, md.as_array()
+
+
+def check(restatement, cat, defs=None, opts=None):
+    po_g, h_g, got, th = run_props(cat, defs, opts)
+    want = np.zeros(cat.n_halos, catalog.PROPS_DTYPE)
+    restatement.halo_props_batch(po_g, h_g, cat.params, th, want, nthreads=8)
+    for f in catalog.PROPS_INT_FIELDS:
+        np.testing.assert_array_equal(got[f], want[f], err_msg=f)
+    for f in EXACT:
+        np.testing.assert_array_equal(got[f], want[f], err_msg=f)
+    for f in CBRT:
+        np.testing.assert_allclose(got[f], want[f], rtol=2e-7, atol=0, err_msg=f)
+    for f in SUMS:
+        w = want[f].astype(np.float64)
+        # an absolute floor of 1e-5 of the field's scale over the batch for
+        # values that cancel to near zero (a component of J, a tiny offset)
+        floor = 1e-5 * max(1e-30, float(np.max(np.abs(w))))
+        np.testing.assert_allclose(got[f], w, rtol=1e-5, atol=floor, err_msg=f)
+    return got
+
+
+def test_props_catalogue(restatement):
+    got = check(restatement, catalog.config2())
+    assert (got["n_mdelta"] > 0).sum() > 100 and (got["vmax"] > 0).sum() > 100
+
+
+def test_props_small_calls_and_potential_only(restatement):
+    """Calls of 2 to 3,000 records with 10% potential-only records (dropped
+    before the pass), including calls with no bound record."""
+    cat = catalog.small_batch(seed=71, n_halos=300, lo=2, hi=3000, potential_only=0.1)
+    got = check(restatement, cat)
+    assert (got["n_mdelta"] == 0).any()
+
+
+def test_props_other_epoch_and_definitions(restatement):
+    cat = catalog.small_batch(seed=72, n_halos=60, lo=20, hi=20000)
+    cat.params = catalog.Params(force_res=0.002, particle_mass=cat.params.particle_mass,
+                                Om=0.3, Ol=0.7, scale_now=0.5)
+    check(restatement, cat, defs=("m200c", "vir", "178b", "1000c", "200b"))
+
+
+def test_props_per_call_plan(restatement):
+    cat = catalog.small_batch(seed=73, n_halos=20, lo=5, hi=2000)
+    check(restatement, cat, opts=halopot.options(per_call=1))
+
+
+def test_props_match_the_reference_on_the_catalogue(restatement):
+    """The GPU property pass against the reference's own compiled code,
+    call by call, on the GPU's bound-pass output."""
+    import oracle
+
+    if not oracle.ReferenceProps.available():
+        pytest.skip("compiled reference property pass (oracle/_ref) not present")
+    ref = oracle.ReferenceProps()
+    cat = catalog.small_batch(seed=74, n_halos=120, lo=20, hi=20000)
+    po_g, h_g, got, th = run_props(cat)
+    for h in range(cat.n_halos):
+        f, k = int(h_g["first"][h]), int(h_g["kept"][h])
+        r = ref.halo_props(po_g[f:f + k].copy(), k, h_g["pos"][h], cat.params, th)
+        for name in EXACT:
+            np.testing.assert_array_equal(got[h][name], r[name], err_msg=f"{h} {name}")
+        for name in SUMS + ("r",):
+            np.testing.assert_allclose(got[h][name], r[name], rtol=1e-5,
+                                       atol=1e-5 * float(np.max(np.abs(r[name]))) + 1e-30,
+                                       err_msg=f"{h} {name}")

@@ -1,0 +1,160 @@
+"""The halo property pass on the CPU (src/properties.c:384-510, run for
+bound = 0 and then bound = 1 as src/properties.c:552-553 does).
+
+* The restatement (orc_halo_props, oracle/restate.c) against the reference's
+  own code, compiled from src/groupies.c (which #includes src/properties.c),
+  src/nfw.c, src/jacobi.c, src/hubble.c and src/config_vars.c into
+  oracle/_ref/libref_props.so and called through oracle/ref_props.c: every
+  field bit for bit, on a 2,000-call catalogue and on hand-made edge calls
+  (a single record, no bound record, nothing above the density threshold,
+  radii below FORCE_RES, a tie in n / r, a call whose only records above the
+  threshold are unbound).
+* The product's hp_mass_definitions (host code in libhalopot) against the
+  reference's calc_mass_definition (src/groupies.c:102-142), exactly, and
+  against closed forms at a = 1.
+"""
+import math
+
+import numpy as np
+import pytest
+
+import halopot
+import oracle
+from halopot import catalog
+
+FIELDS = [f for f in catalog.PROPS_FLOAT_FIELDS if f != "rvir"]
+
+
+@pytest.fixture(scope="module")
+def refprops():
+    if not oracle.ReferenceProps.available():
+        pytest.skip("compiled reference property pass (oracle/_ref) not present")
+    return oracle.ReferenceProps()
+
+
+def _thresh(refprops, prm, defs=catalog.DEFAULT_MASS_DEFINITIONS):
+    return refprops.mass_definition(prm, defs)
+
+
+def _compare_call(restatement, refprops, po, kept, pos, prm, th):
+    mine = np.zeros(1, catalog.PROPS_DTYPE)
+    restatement.halo_props(po, kept, pos, prm, th, mine)
+    ref = refprops.halo_props(po.copy(), kept, pos, prm, th)
+    for f in FIELDS:
+        np.testing.assert_array_equal(np.float32(mine[0][f]), np.float32(ref[f]), err_msg=f)
+    return mine[0]
+
+
+def test_restatement_matches_reference_on_a_catalogue(restatement, refprops):
+    cat = catalog.config2()
+    prm = cat.params
+    po = cat.po.copy()
+    halos = cat.halos.copy()
+    kept, _ = restatement.bound_pass_batch(po, halos, prm, nthreads=8)
+    halos["kept"] = kept
+    th = _thresh(refprops, prm)
+    out = np.zeros(cat.n_halos, catalog.PROPS_DTYPE)
+    restatement.halo_props_batch(po, halos, prm, th, out, nthreads=8)
+    for h in range(cat.n_halos):
+        f, k = int(halos["first"][h]), int(kept[h])
+        ref = refprops.halo_props(po[f:f + k].copy(), k, halos["pos"][h], prm, th)
+        for name in FIELDS:
+            np.testing.assert_array_equal(np.float32(out[h][name]), np.float32(ref[name]),
+                                          err_msg=f"call {h} {name}")
+    # the catalogue exercises every branch that matters
+    assert (out["n_mdelta"] > 0).sum() > 100 and (out["vmax"] > 0).sum() > 100
+    assert (out["n_half"] > 0).sum() > 100 and (out["energy"] < 0).sum() > 100
+
+
+def _records(r2, pe, ke, rng, scale=0.1):
+    n = len(r2)
+    po = np.zeros(n, catalog.POTENTIAL_DTYPE)
+    po["r2"] = np.asarray(r2, np.float32)
+    po["pe"] = pe
+    po["ke"] = ke
+    po["pos"][:, :3] = rng.normal(0.0, scale, (n, 3))
+    po["pos"][:, 3:] = rng.normal(0.0, 200.0, (n, 3))
+    return po
+
+
+def test_restatement_matches_reference_on_edge_calls(restatement, refprops):
+    rng = np.random.default_rng(7)
+    prm = catalog.Params()
+    th = _thresh(refprops, prm)
+    pos = np.array([0.01, -0.02, 0.03, 5.0, -7.0, 2.0], np.float32)
+    cases = []
+    # one bound record; one unbound record
+    cases.append(_records([1e-6], [10.0], [1.0], rng))
+    cases.append(_records([1e-6], [1.0], [10.0], rng))
+    # nothing bound
+    r2 = np.sort(rng.uniform(0, 0.01, 50))
+    cases.append(_records(r2, np.ones(50), np.full(50, 2.0), rng))
+    # nothing above the density threshold (points far out)
+    r2 = np.sort(rng.uniform(100.0, 400.0, 40))
+    cases.append(_records(r2, np.full(40, 5.0), np.ones(40), rng))
+    # radii below FORCE_RES (clamped), then a spread
+    r2 = np.sort(np.concatenate([rng.uniform(0, 1e-6, 30), rng.uniform(1e-4, 0.2, 200)]))
+    pe = rng.uniform(0.5, 2.0, 230)
+    cases.append(_records(r2, pe, rng.uniform(0.0, 1.5, 230), rng))
+    # a tie in n / r: n = 1 at r = 2^-7 and n = 2 at r = 2^-6 (exact radii)
+    r2 = np.array([2.0 ** -14, 2.0 ** -12, 0.5, 0.6], np.float32)
+    cases.append(_records(r2, np.full(4, 3.0), np.ones(4), rng))
+    # the inner records unbound: part_mdelta comes late
+    r2 = np.sort(rng.uniform(0, 0.05, 120))
+    ke = np.where(np.arange(120) < 60, 9.0, 0.5)
+    cases.append(_records(r2, np.ones(120), ke, rng))
+    # equal pe and ke: bound for the walks (!(pe < ke)), not for the energy (pe > ke)
+    r2 = np.sort(rng.uniform(0, 0.02, 80))
+    cases.append(_records(r2, np.ones(80), np.ones(80, np.float32), rng))
+    seen_tie = False
+    for po in cases:
+        got = _compare_call(restatement, refprops, po, po.shape[0], pos, prm, th)
+        if po.shape[0] == 4:
+            assert got["rvmax"] == np.float32(2.0 ** -7 * 1e3)  # the first of the tie
+            seen_tie = True
+    assert seen_tie
+
+
+def test_restatement_matches_reference_at_another_epoch(restatement, refprops):
+    cat = catalog.config1()
+    prm = catalog.Params(force_res=0.002, particle_mass=3e8, Om=0.3, Ol=0.7, scale_now=0.5)
+    po = cat.po.copy()
+    h = cat.halos[0:1].copy()
+    kept, _ = restatement.bound_pass(po, h, prm)
+    th = _thresh(refprops, prm, ("m200c", "vir", "178b", "1000c", "200b"))
+    got = _compare_call(restatement, refprops, po[:kept].copy(), kept, h[0]["pos"], prm, th)
+    assert got["n_mdelta"] > 0 and got["vmax"] > 0
+
+
+DEFS = [catalog.DEFAULT_MASS_DEFINITIONS, ("m200c", "Mvir", "178B", "1000C", "xyz")]
+PARAMS = [catalog.Params(), catalog.Params(particle_mass=3e8, Om=0.3, Ol=0.7, scale_now=0.5),
+          catalog.Params(particle_mass=5e10, Om=0.25, Ol=0.75, W0=-0.9, WA=0.1, scale_now=0.8)]
+
+
+@pytest.mark.parametrize("defs", DEFS)
+@pytest.mark.parametrize("pi", range(len(PARAMS)))
+def test_mass_definitions_match_reference(refprops, defs, pi):
+    prm = PARAMS[pi]
+    halopot.set_config(prm)
+    got = halopot.mass_definitions(defs).as_array()
+    want = refprops.mass_definition(prm, defs)
+    np.testing.assert_array_equal(got, want)
+
+
+def test_mass_definitions_closed_forms():
+    """At a = 1 with Om + Ol = 1, E(0) = 1: "<x>b" is x times the background
+    density in particles, "<x>c" x times the critical density, and
+    particle_rvir_dens = Delta_vir * background (Bryan & Norman with
+    x = Om - 1)."""
+    prm = catalog.Params()
+    halopot.set_config(prm)
+    md = halopot.mass_definitions(("200b", "200c", "vir", "500c", "2500c")).as_array()
+    cons = np.float32(prm.Om * catalog.CRITICAL_DENSITY / prm.particle_mass)
+    mf = np.float32(prm.Om)  # matter fraction at a = 1
+    assert md[0] == np.float32(200.0 * float(cons))
+    assert md[1] == np.float32(200.0 * float(cons) / float(mf))
+    assert md[3] == np.float32(500.0 * float(cons) / float(mf))
+    x = prm.Om - 1.0
+    dvir = (18 * math.pi ** 2 + 82.0 * x - 39 * x * x) / (1.0 + x)
+    assert md[2] == np.float32(dvir * float(cons)) == md[5]
+    assert md[5] == prm.particle_rvir_dens
