@@ -17,7 +17,9 @@ the same catalogue through the plain per-halo GPU path (every call with its
 own launches), through the host-buffer entry point (PCIe copies included),
 and through the reference's CPU path (its own compute_potential /
 compute_kinetic_energy from oracle/_ref, OpenMP over halos) on a bounded
-sample of host time.
+sample of host time.  After the timed steps, the halo property pass
+(hp_plan_props) is timed on the last step's results, with the reference's
+own property code beside it (reported under "halo_props", not in "value").
 
 Prints ONE JSON line.
 """
@@ -116,6 +118,70 @@ def run_cpu_baseline(cat, seconds: float):
     }
 
 
+def run_props(plan, cat, d_po, d_h, args):
+    """The halo property pass (hp_plan_props, src/properties.c:384-510) on
+    the last step's results: its own time per pass over the kept records and,
+    beside it, the reference's own property code (oracle/_ref, OpenMP over
+    halos) on the same records for a bounded sample of host time."""
+    import numpy as np
+
+    import halopot
+    from halopot import catalog
+    from halopot.hipmem import DeviceArray, synchronize
+
+    md = halopot.mass_definitions()
+    d_p = DeviceArray.zeros(cat.n_halos * catalog.PROPS_DTYPE.itemsize)
+    plan.props(d_po.address, d_h.address, d_p.address, md)  # warm
+    reps = max(5, args.steps)
+    t = time.perf_counter()
+    for _ in range(reps):
+        plan.props(d_po.address, d_h.address, d_p.address, md)  # synchronous
+    synchronize()
+    e = time.perf_counter() - t
+    h_g = d_h.download(np.empty_like(cat.halos))
+    kept = int(h_g["kept"].sum())
+    res = {"ms_per_pass": 1e3 * e / reps, "value": kept * reps / e,
+           "unit": "kept records/s through the property pass", "kept_records": kept,
+           "calls": cat.n_halos, "passes": reps,
+           "scope": "src/properties.c:384-510 for bound = 0 and 1, without calc_shape, "
+                    "calc_scale_radius and the pseudo-evolution masses"}
+    log(f"halo property pass: {res['ms_per_pass']:.3f} ms/pass, {res['value']:.4g} kept records/s")
+    if not args.no_cpu:
+        sys.path.insert(0, str(ROOT / "oracle"))
+        import oracle
+
+        po_g = d_po.download(np.empty_like(cat.po))
+        nt = cpu_threads()
+        if oracle.ReferenceProps.available():
+            R = oracle.ReferenceProps()
+            kind = "reference"
+
+            def one():
+                R.halo_props_batch(po_g, h_g, cat.params, md.as_array(), nthreads=nt)
+        else:
+            O = oracle.Restatement()
+            kind = "port"
+            buf = np.zeros(cat.n_halos, catalog.PROPS_DTYPE)
+
+            def one():
+                O.halo_props_batch(po_g, h_g, cat.params, md.as_array(), buf, nthreads=nt)
+        passes, busy = 0, 0.0
+        while busy < min(4.0, args.cpu_seconds) or passes == 0:
+            t = time.perf_counter()
+            one()
+            busy += time.perf_counter() - t
+            passes += 1
+        res["cpu_baseline"] = {"value": kept * passes / busy, "unit": res["unit"], "cores": nt,
+                               "kind": kind, "ms_per_pass": 1e3 * busy / passes,
+                               "sample": f"whole catalogue x {passes} pass(es), {busy:.1f} s "
+                                         f"on {nt} threads (OpenMP dynamic over halos)"}
+        res["gpu_vs_cpu"] = res["value"] / res["cpu_baseline"]["value"]
+        log(f"halo property pass on the CPU ({kind}, {nt} threads): "
+            f"{res['cpu_baseline']['ms_per_pass']:.1f} ms/pass")
+    d_p.free()
+    return res
+
+
 def main() -> int:
     ap = argparse.ArgumentParser(description=__doc__.split("\n\n")[0])
     ap.add_argument("--gpus", type=int, default=1)
@@ -128,6 +194,7 @@ def main() -> int:
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--no-per-halo", action="store_true", help="skip the per-halo GPU path")
     ap.add_argument("--no-host", action="store_true", help="skip the host-buffer (PCIe) leg")
+    ap.add_argument("--no-props", action="store_true", help="skip the halo property pass leg")
     ap.add_argument("--cpu-seconds", type=float, default=8.0)
     args = ap.parse_args()
     if args.gpus != 1:
@@ -196,6 +263,7 @@ def main() -> int:
             phase[k] += st["ms_" + k]
     synchronize()
     elapsed = time.perf_counter() - t0
+    props = None if args.no_props else run_props(plan, cat, d_po, d_h, args)
     plan.close()
     ms_step = 1e3 * elapsed / args.steps
     value = cat.n_records * args.steps / elapsed
@@ -239,6 +307,9 @@ def main() -> int:
                    "parallelism": "one GPU"},
         "roofline": roofline,
     }
+
+    if props is not None:
+        out["halo_props"] = props
 
     if not args.no_per_halo:
         plan_p = halopot.Plan(cat.halos, halopot.options(accumulate=acc, per_call=1))

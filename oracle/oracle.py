@@ -292,6 +292,8 @@ class ReferenceProps:
         self.lib = L = C.CDLL(str(path))
         L.ref_halo_props.argtypes = [C.c_void_p, C.c_int64, C.c_void_p, C.c_void_p, C.c_void_p]
         L.ref_mass_definition.argtypes = [C.c_void_p]
+        L.ref_halo_props_batch.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p,
+                                           C.c_int64, C.c_void_p, C.c_void_p, C.c_int]
 
     @staticmethod
     def available(path=None) -> bool:
@@ -324,3 +326,18 @@ class ReferenceProps:
         out = np.zeros(1, self.DTYPE)
         self.lib.ref_halo_props(_p(po), kept, _p(ps), _p(th), _p(out))
         return out[0]
+
+    def halo_props_batch(self, po: np.ndarray, halos: np.ndarray, prm, thresh,
+                         nthreads: int = 0) -> np.ndarray:
+        """ref_halo_props for every call (first, kept, pos of each halo row),
+        OpenMP dynamic over calls."""
+        _check_records(po)
+        self.set_globals(prm)
+        first = np.ascontiguousarray(halos["first"], np.int64)
+        kept = np.ascontiguousarray(halos["kept"], np.int64)
+        pos = np.ascontiguousarray(halos["pos"], np.float32)
+        th = np.ascontiguousarray(thresh, np.float64)
+        out = np.zeros(halos.shape[0], self.DTYPE)
+        self.lib.ref_halo_props_batch(_p(po), _p(first), _p(kept), _p(pos), halos.shape[0],
+                                      _p(th), _p(out), nthreads)
+        return out

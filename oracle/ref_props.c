@@ -12,12 +12,20 @@
  *                          calc_scale_radius, estimate_total_energy and
  *                          _calc_pseudo_evolution_masses inside)
  *   ref_mass_definition -> calc_mass_definition (src/groupies.c:124-142)
+ *   ref_halo_props_batch-> ref_halo_props for every call of a batch, OpenMP
+ *                          dynamic over calls as the halo finder's loop over
+ *                          halos (src/mpi_main.cpp:1377); the reference's
+ *                          per-thread globals (po, particle_thresh_dens) are
+ *                          threadprivate, so calls run concurrently
  *
  * The records must already be in the state src/properties.c:551 leaves them:
  * potential-only records dropped, sorted by r2, pe and ke set.
  */
 #include <stdint.h>
 #include <string.h>
+#ifdef _OPENMP
+#include <omp.h>
+#endif
 
 #include "halo.h"
 #include "potential.h"
@@ -52,6 +60,10 @@ void ref_halo_props(struct potential *recs, int64_t total_p, const float pos[6],
     struct halo h;
     memset(&h, 0, sizeof h);
     memcpy(h.pos, pos, sizeof h.pos);
+    /* every thread's own copy of the mass-definition state, as the halo
+       finder's threads hold it (particle_rvir_dens_z0 feeds the
+       pseudo-evolution masses); then the thresholds given */
+    calc_mass_definition();
     for (int k = 0; k < 5; k++) particle_thresh_dens[k] = thresh[k];
     particle_rvir_dens = thresh[5];
     po = recs;
@@ -85,4 +97,17 @@ void ref_halo_props(struct potential *recs, int64_t total_p, const float pos[6],
     o.m_pe_b = h.m_pe_b;
     o.m_pe_d = h.m_pe_d;
     *out = o;
+}
+
+void ref_halo_props_batch(struct potential *recs, const int64_t *first, const int64_t *kept,
+                          const float *pos6, int64_t nhalos, const double thresh[6],
+                          ref_props *out, int nthreads) {
+#ifdef _OPENMP
+    if (nthreads > 0) omp_set_num_threads(nthreads);
+#else
+    (void)nthreads;
+#endif
+#pragma omp parallel for schedule(dynamic)
+    for (int64_t h = 0; h < nhalos; h++)
+        ref_halo_props(recs + first[h], kept[h], pos6 + 6 * h, thresh, out + h);
 }
