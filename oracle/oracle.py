@@ -273,10 +273,11 @@ class Reference:
 
 
 class ReferenceProps:
-    """The reference's own halo property pass: src/properties.c (inside
-    src/groupies.c) with src/nfw.c, src/jacobi.c, src/hubble.c and
-    src/config_vars.c, compiled by oracle/Makefile and called through the
-    harness oracle/ref_props.c."""
+    """The reference's own halo property pass and bound-pass steps:
+    src/properties.c (inside src/groupies.c) with src/nfw.c, src/jacobi.c,
+    src/hubble.c, src/config_vars.c, src/potential.c and src/error.cpp,
+    compiled by oracle/Makefile and called through the harness
+    oracle/ref_props.c."""
 
     FIELDS = ("m", "mgrav", "r", "vmax", "rvmax", "vrms", "Xoff", "Voff", "alt_m",
               "halfmass_radius", "J", "energy", "kin_to_pot", "spin", "bullock_spin",
@@ -294,6 +295,8 @@ class ReferenceProps:
         L.ref_mass_definition.argtypes = [C.c_void_p]
         L.ref_halo_props_batch.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p,
                                            C.c_int64, C.c_void_p, C.c_void_p, C.c_int]
+        L.ref_bound_pass.restype = C.c_int64
+        L.ref_bound_pass.argtypes = [C.c_void_p, C.c_int64, C.c_void_p]
 
     @staticmethod
     def available(path=None) -> bool:
@@ -310,6 +313,25 @@ class ReferenceProps:
             for name, d in zip(("MASS_DEFINITION", "MASS_DEFINITION2", "MASS_DEFINITION3",
                                 "MASS_DEFINITION4", "MASS_DEFINITION5"), self._defs):
                 C.c_void_p.in_dll(self.lib, name).value = C.addressof(d)
+
+    BOUND_HALO = np.dtype({"names": ["pos", "corevel", "bulkvel", "min_vel_err",
+                                     "min_bulkvel_err", "n_core", "is_sub"],
+                           "formats": [("<f4", (6,)), ("<f4", (3,)), ("<f4", (3,)), "<f4",
+                                       "<f4", "<i8", "<i4"],
+                           "offsets": [0, 24, 36, 48, 52, 56, 64], "itemsize": 72})
+
+    def bound_pass(self, po: np.ndarray, halo_row, prm):
+        """The reference's own bound-pass steps (src/properties.c:537-550:
+        compute_potential, potential-only strip, qsort with dist_compare,
+        calculate_corevel, compute_kinetic_energy) on one call's records, in
+        place.  Returns (kept, halo fields as BOUND_HALO)."""
+        _check_records(po)
+        self.set_globals(prm)
+        bh = np.zeros(1, self.BOUND_HALO)
+        for f in ("pos", "corevel", "bulkvel", "is_sub"):
+            bh[f] = halo_row[f]
+        kept = self.lib.ref_bound_pass(_p(po), po.shape[0], _p(bh))
+        return kept, bh[0]
 
     def mass_definition(self, prm, mass_defs) -> np.ndarray:
         """particle_thresh_dens[0..4] and particle_rvir_dens."""

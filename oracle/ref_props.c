@@ -12,6 +12,12 @@
  *                          calc_scale_radius, estimate_total_energy and
  *                          _calc_pseudo_evolution_masses inside)
  *   ref_mass_definition -> calc_mass_definition (src/groupies.c:124-142)
+ *   ref_bound_pass      -> the bound-pass steps of calc_additional_halo_props
+ *                          (src/properties.c:537-550) on prepared records:
+ *                          compute_potential, the potential-only strip,
+ *                          qsort with dist_compare, calculate_corevel,
+ *                          compute_kinetic_energy -- the reference's own
+ *                          functions; the harness only sequences them
  *   ref_halo_props_batch-> ref_halo_props for every call of a batch, OpenMP
  *                          dynamic over calls as the halo finder's loop over
  *                          halos (src/mpi_main.cpp:1377); the reference's
@@ -22,11 +28,13 @@
  * potential-only records dropped, sorted by r2, pe and ke set.
  */
 #include <stdint.h>
+#include <stdlib.h>
 #include <string.h>
 #ifdef _OPENMP
 #include <omp.h>
 #endif
 
+#include "config_vars.h"
 #include "halo.h"
 #include "potential.h"
 
@@ -38,6 +46,8 @@ extern double particle_rvir_dens;
 
 void _calc_additional_halo_props(struct halo *h, int64_t total_p, int64_t bound);
 void calc_mass_definition(void);
+void calculate_corevel(struct halo *h, struct potential *po, int64_t total_p);
+int dist_compare(const void *a, const void *b);
 
 typedef struct ref_props {
     float m, mgrav, r, vmax, rvmax, vrms, Xoff, Voff, alt_m[4], halfmass_radius;
@@ -110,4 +120,45 @@ void ref_halo_props_batch(struct potential *recs, const int64_t *first, const in
 #pragma omp parallel for schedule(dynamic)
     for (int64_t h = 0; h < nhalos; h++)
         ref_halo_props(recs + first[h], kept[h], pos6 + 6 * h, thresh, out + h);
+}
+
+/* halo fields the bound pass reads and writes (src/halo.h:8-16) */
+typedef struct ref_bound_halo {
+    float   pos[6], corevel[3], bulkvel[3];
+    float   min_vel_err, min_bulkvel_err;
+    int64_t n_core;
+    int32_t is_sub, pad;
+} ref_bound_halo;
+
+/* src/properties.c:537-550 with the halo already resolved: is_sub selects
+   the KE centre as extra_info[].sub_of > -1 does.  Returns total_p after
+   the potential-only records are dropped. */
+int64_t ref_bound_pass(struct potential *recs, int64_t total_p, ref_bound_halo *bh) {
+    struct halo h;
+    memset(&h, 0, sizeof h);
+    memcpy(h.pos, bh->pos, sizeof h.pos);
+    memcpy(h.corevel, bh->corevel, sizeof h.corevel);
+    memcpy(h.bulkvel, bh->bulkvel, sizeof h.bulkvel);
+    float scale_now = SCALE_NOW;
+    calc_mass_definition(); /* particle_rvir_dens for calculate_corevel */
+    if (total_p > 1) compute_potential(recs, total_p);
+    for (int64_t j = 0; j < total_p; j++)
+        if (recs[j].ke < 0) {
+            total_p--;
+            recs[j] = recs[total_p];
+            j--;
+        }
+    qsort(recs, total_p, sizeof(struct potential), dist_compare);
+    calculate_corevel(&h, recs, total_p);
+    if (bh->is_sub)
+        compute_kinetic_energy(recs, total_p, h.corevel, h.pos, scale_now);
+    else
+        compute_kinetic_energy(recs, total_p, h.bulkvel, h.pos, scale_now);
+    memcpy(bh->pos, h.pos, sizeof h.pos);
+    memcpy(bh->corevel, h.corevel, sizeof h.corevel);
+    memcpy(bh->bulkvel, h.bulkvel, sizeof h.bulkvel);
+    bh->min_vel_err = h.min_vel_err;
+    bh->min_bulkvel_err = h.min_bulkvel_err;
+    bh->n_core = h.n_core;
+    return total_p;
 }

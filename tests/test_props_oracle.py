@@ -9,6 +9,11 @@ bound = 0 and then bound = 1 as src/properties.c:552-553 does).
   (a single record, no bound record, nothing above the density threshold,
   radii below FORCE_RES, a tie in n / r, a call whose only records above the
   threshold are unbound).
+* The restated bound-pass steps (orc_bound_pass: potential, potential-only
+  strip, sort by r2, calculate_corevel, compute_kinetic_energy) against the
+  reference's own functions run in the same sequence (ref_bound_pass):
+  kept counts, core / bulk velocity, their errors, n_core and ke bit for
+  bit, the sorted order wherever r2 has no ties, pe to f64 rounding.
 * The product's hp_mass_definitions (host code in libhalopot) against the
   reference's calc_mass_definition (src/groupies.c:102-142), exactly, and
   against closed forms at a = 1.
@@ -67,6 +72,37 @@ def test_restatement_matches_reference_on_a_catalogue(restatement, refprops):
     # the catalogue exercises every branch that matters
     assert (out["n_mdelta"] > 0).sum() > 100 and (out["vmax"] > 0).sum() > 100
     assert (out["n_half"] > 0).sum() > 100 and (out["energy"] < 0).sum() > 100
+
+
+def test_restated_bound_pass_matches_reference(restatement, refprops):
+    cat = catalog.config2()
+    prm = cat.params
+    n_ties = 0
+    for h in range(cat.n_halos):
+        f, c = int(cat.halos["first"][h]), int(cat.halos["count"][h])
+        recs = cat.po[f:f + c].copy()
+        recs["flags"] = np.arange(c, dtype=np.int32)
+        a, b = recs.copy(), recs.copy()
+        row = cat.halos[h:h + 1].copy()
+        ka, _ = restatement.bound_pass(a, row, prm)
+        kb, bh = refprops.bound_pass(b, cat.halos[h], prm)
+        assert ka == kb, h
+        for fld in ("pos", "corevel", "bulkvel", "min_vel_err", "min_bulkvel_err", "n_core"):
+            np.testing.assert_array_equal(row[0][fld], bh[fld], err_msg=f"call {h} {fld}")
+        # equal r2: the reference orders ties by its tree build's in-place
+        # swaps (src/fast3tree.c:751-774) before the stable sort
+        if np.unique(a["r2"][:ka]).size < ka:
+            n_ties += 1
+            ia, ib = np.argsort(a["flags"][:ka]), np.argsort(b["flags"][:kb])
+            np.testing.assert_array_equal(a["flags"][:ka][ia], b["flags"][:kb][ib])
+            a, b = a[:ka][ia], b[:kb][ib]
+        else:
+            np.testing.assert_array_equal(a["flags"][:ka], b["flags"][:kb], err_msg=f"call {h}")
+            a, b = a[:ka], b[:kb]
+        np.testing.assert_array_equal(a["ke"], b["ke"], err_msg=f"call {h}")
+        if ka:
+            assert np.max(np.abs(a["pe"] - b["pe"]) / np.abs(b["pe"])) < 1e-14, h
+    assert 0 < n_ties < cat.n_halos // 10
 
 
 def _records(r2, pe, ke, rng, scale=0.1):
