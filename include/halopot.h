@@ -170,6 +170,53 @@ typedef struct hp_tuning {
  * are not tuned (HP_EINVAL). */
 int hp_plan_tune(hp_plan *plan, struct potential *d_po, int reps, hp_tuning *out);
 
+/* ---------------------------------------------------- record assembly */
+
+/* The reference's particle copies of a FOF group (src/particle.h:5-8),
+ * 32 bytes: id at 0, pos[6] at 8. */
+#ifndef PARTICLE_H
+struct particle {
+    int64_t id;
+    float   pos[6];
+};
+#endif
+
+/* One halo of the hierarchy: the fields of struct halo (src/halo.h:8-15)
+ * and struct extra_halo_info (src/halo.h:37-41) that calc_particle_radii
+ * reads.  Indices refer to the same array; -1: none. */
+typedef struct hp_tree_halo {
+    int64_t p_start;             /* first particle in the copies array */
+    int64_t num_p;
+    int64_t num_child_particles;
+    int64_t child;               /* first child */
+    int64_t next_cochild;        /* next sibling */
+    int64_t sub_of;              /* parent, -1 for a host */
+    float   pos[6];              /* centre (pos[0..2] is used) */
+} hp_tree_halo;
+
+/* calc_particle_radii (src/groupies.c:212-268; replaces the record set-up of
+ * src/properties.c:526 and _reset_potentials, src/groupies.c:212-233) for
+ * the halos base[0..nbase): each call's records are the halo's particles,
+ * then its descendants' (depth first, children in sibling order), then --
+ * when the halo is a sub-halo with num_child_particles above
+ * include_host_ratio (INCLUDE_HOST_POTENTIAL_RATIO, src/config.template.h:
+ * 116, default 0.3) times its parent's -- the parent's subtree as potential-
+ * only records (ke = -1).  r2 is about the halo's pos[0..2]; flags = 1 for
+ * the halo's own particles, 2 for those of descendants with fewer than 3% of
+ * its particles.  A halo already on the current descent path is skipped, as
+ * the reference does.
+ *
+ * d_particles: the group's copies (device); the calls' records are written
+ * back to back into d_po (device, cap_records records).  calls[0..nbase) and
+ * *nrecords are written on the host: first, count, pos, is_sub (sub_of >
+ * -1); corevel and bulkvel are set to 0 and n_core, kept, n_bound to 0 -- the
+ * caller sets the prior corevel / bulkvel (src/properties.c:71-126) before
+ * hp_plan_run if it needs them.  Synchronous. */
+int hp_gather_calls(const hp_tree_halo *tree, int64_t nhalos, const int64_t *base, int64_t nbase,
+                    const struct particle *d_particles, int64_t nparticles,
+                    double include_host_ratio, struct potential *d_po, int64_t cap_records,
+                    hp_halo *calls, int64_t *nrecords);
+
 /* ------------------------------------------------------- halo properties */
 
 /* Density thresholds of the mass definitions, in particles per (Mpc/h)^3:

@@ -19,6 +19,7 @@
 #include "hp_build.hip"
 #include "hp_unbind.hip"
 #include "hp_props.hip"
+#include "hp_gather.hip"
 #include "hp_walk.hip"
 
 #include <algorithm>
@@ -264,6 +265,8 @@ struct Engine {
     DevBuf keys0, keys1, vals0, vals1, hist, hsum, srt, callinfo, tsum, tcand, tile_i32;
     // halo properties
     DevBuf pcnt, pscan, pcall, psums;
+    // record assembly
+    DevBuf gsegs, gtiles, gcen;
     int64_t cap_nodes = 0, cap_small = 0, cap_group = 0, cap_big = 0, cap_tiles = 0, cap_fat = 0;
     std::vector<int32_t> depth_end;  // node count after each depth's level
 
@@ -305,7 +308,7 @@ struct Engine {
         DevBuf *all[] = {&hfirst, &hcount, &root_tiles, &groups, &direct_blocks, &rec_call,
                          &scan_tiles, &call_tile0, &call_ntiles, &box, &bad, &badlist, &A, &B, &F,
                          &leaf_of, &nodes, &parent, &fat, &small0, &group0, &big, &tiles,
-                         &parts, &splitres, &dparts, &counters, &work, &depth_buf, &pcnt, &pscan, &pcall, &psums, &keys0, &keys1, &vals0,
+                         &parts, &splitres, &dparts, &counters, &work, &depth_buf, &pcnt, &pscan, &pcall, &psums, &gsegs, &gtiles, &gcen, &keys0, &keys1, &vals0,
                          &vals1, &hist, &hsum, &srt, &callinfo, &tsum, &tcand, &tile_i32};
         if (stream) {
             (void)hipStreamSynchronize(stream);
@@ -942,6 +945,107 @@ static Engine *thread_engine(int *err) {
     return e;
 }
 
+// calc_particle_radii (src/groupies.c:235-268) as a segment list: for each
+// base halo, the halos its recursion visits in order, each as a run of
+// particles [p_start, p_start + num_p) that becomes the next run of records.
+// The recursion is kept on an explicit stack; a halo already on the descent
+// path is skipped (src/groupies.c:240-246) and a sibling list that returns
+// to the first child is an error (src/groupies.c:254).
+static int plan_gather(const hp_tree_halo *tree, int64_t nh, const int64_t *base, int64_t nbase,
+                       int64_t nparticles, double ratio, std::vector<GatherSeg> &segs,
+                       std::vector<float> &cen, hp_halo *calls, int64_t *total) {
+    struct Frame {
+        int64_t h, child;
+        int64_t steps;      // sibling steps taken (a list longer than the tree is a cycle)
+        int32_t potential_only;
+        int32_t done;       // children visited (and the parent rule applied)
+    };
+    std::vector<Frame> st;
+    std::vector<int64_t> path;
+    int64_t pos = 0;
+    segs.clear();
+    cen.clear();
+    for (int64_t b = 0; b < nbase; b++) {
+        const int64_t bh = base[b];
+        if (bh < 0 || bh >= nh)
+            return set_error(HP_EINVAL, "hp_gather_calls: base[%lld] = %lld out of range",
+                             (long long)b, (long long)bh);
+        const int64_t first = pos;
+        const double small = (double)tree[bh].num_p * 0.03;
+        int r = HP_OK;
+        // src/groupies.c:212-233, 240-250: enter halo h unless it is on the path
+        auto enter = [&](int64_t h, int32_t potential_only) -> int {
+            if (h < 0 || h >= nh)
+                return set_error(HP_EINVAL, "hp_gather_calls: halo link %lld out of range",
+                                 (long long)h);
+            for (int64_t q : path)
+                if (q == h) return HP_OK;
+            const hp_tree_halo &t = tree[h];
+            if (t.num_p < 0 || t.p_start < 0 || t.p_start + t.num_p > nparticles)
+                return set_error(HP_EINVAL, "hp_gather_calls: halo %lld particles [%lld, +%lld) "
+                                 "outside the %lld copies", (long long)h, (long long)t.p_start,
+                                 (long long)t.num_p, (long long)nparticles);
+            int32_t flags = h == bh ? 1 : 0;
+            if (!potential_only && (double)t.num_p < small) flags = 2;
+            if (t.num_p > 0)
+                segs.push_back(GatherSeg{t.p_start, pos, (int32_t)t.num_p, (int32_t)b, flags,
+                                         potential_only});
+            pos += t.num_p;
+            if (pos > INT32_MAX / 2)
+                return set_error(HP_EINVAL, "hp_gather_calls: more than %d records", INT32_MAX / 2);
+            path.push_back(h);
+            st.push_back(Frame{h, t.child, 0, potential_only, 0});
+            return HP_OK;
+        };
+        if ((r = enter(bh, 0))) return r;
+        while (!st.empty()) {
+            Frame &f = st.back();
+            if (!f.done && f.child > -1) {
+                const int64_t c = f.child;
+                if (c >= nh)
+                    return set_error(HP_EINVAL, "hp_gather_calls: child link %lld out of range",
+                                     (long long)c);
+                f.child = tree[c].next_cochild;
+                if (f.child > -1 && f.child == tree[f.h].child)
+                    return set_error(HP_EINVAL, "hp_gather_calls: sibling list of halo %lld "
+                                     "returns to its first child", (long long)f.h);
+                if (++f.steps > nh)
+                    return set_error(HP_EINVAL, "hp_gather_calls: sibling list of halo %lld "
+                                     "does not end", (long long)f.h);
+                if ((r = enter(c, f.potential_only))) return r;
+                continue;
+            }
+            if (!f.done) {
+                f.done = 1;
+                // src/groupies.c:257-265: the host's subtree as potential-only
+                // records when this sub-halo holds a large enough share
+                const int64_t par = tree[f.h].sub_of;
+                if (f.h == bh && par > -1) {
+                    if (par >= nh)
+                        return set_error(HP_EINVAL, "hp_gather_calls: parent link %lld out of "
+                                         "range", (long long)par);
+                    if ((double)tree[par].num_child_particles * ratio <
+                        (double)tree[bh].num_child_particles) {
+                        if ((r = enter(par, 1))) return r;
+                        continue;
+                    }
+                }
+            }
+            path.pop_back();
+            st.pop_back();
+        }
+        for (int k = 0; k < 3; k++) cen.push_back(tree[bh].pos[k]);
+        hp_halo &hc = calls[b];
+        memset(&hc, 0, sizeof hc);
+        hc.first = first;
+        hc.count = pos - first;
+        for (int k = 0; k < 6; k++) hc.pos[k] = tree[bh].pos[k];
+        hc.is_sub = tree[bh].sub_of > -1 ? 1 : 0;
+    }
+    *total = pos;
+    return HP_OK;
+}
+
 static hp_options default_options() {
     hp_options o;
     memset(&o, 0, sizeof o);
@@ -1199,6 +1303,44 @@ __attribute__((visibility("default"))) int hp_plan_run(hp_plan *p, struct potent
         p->eng.st.ms_total = t;
     }
     HP_TRY(hipStreamSynchronize(p->eng.stream));
+    return HP_OK;
+}
+
+__attribute__((visibility("default"))) int hp_gather_calls(
+    const hp_tree_halo *tree, int64_t nhalos, const int64_t *base, int64_t nbase,
+    const struct particle *d_particles, int64_t nparticles, double include_host_ratio,
+    struct potential *d_po, int64_t cap_records, hp_halo *calls, int64_t *nrecords) {
+    if (nhalos < 0 || nbase < 0 || nparticles < 0 || cap_records < 0 || !nrecords ||
+        (nbase > 0 && (!tree || !base || !calls)) || (nparticles > 0 && !d_particles))
+        return set_error(HP_EINVAL, "hp_gather_calls: bad arguments");
+    std::vector<GatherSeg> segs;
+    std::vector<float> cen;
+    int64_t total = 0;
+    int r = plan_gather(tree, nhalos, base, nbase, nparticles, include_host_ratio, segs, cen,
+                        calls, &total);
+    if (r) return r;
+    if (total > cap_records)
+        return set_error(HP_EINVAL, "hp_gather_calls: %lld records do not fit the %lld given",
+                         (long long)total, (long long)cap_records);
+    if (total > 0 && !d_po) return set_error(HP_EINVAL, "hp_gather_calls: bad arguments");
+    *nrecords = total;
+    if (segs.empty()) return HP_OK;
+    std::vector<GatherTile> tiles;
+    for (size_t i = 0; i < segs.size(); i++)
+        for (int32_t off = 0; off < segs[i].n; off += kGatherTile)
+            tiles.push_back(GatherTile{(int32_t)i, off});
+    Engine *e = thread_engine(&r);
+    if (!e) return r;
+    HP_TRY(hipSetDevice(e->device));
+    if ((r = e->upload(e->gsegs, segs)) || (r = e->upload(e->gtiles, tiles)) ||
+        (r = e->upload(e->gcen, cen)))
+        return r;
+    const int32_t nt = (int32_t)tiles.size();
+    k_gather_records<<<std::min(nt, 8192), 256, 0, e->stream>>>(
+        reinterpret_cast<const GParticle *>(d_particles), e->gsegs.as<GatherSeg>(),
+        e->gtiles.as<GatherTile>(), nt, e->gcen.as<float>(), d_po);
+    HP_TRY(hipGetLastError());
+    HP_TRY(hipStreamSynchronize(e->stream));
     return HP_OK;
 }
 

@@ -9,7 +9,9 @@ Mirrors the reference's operator interface for this path:
   * ``set_config(params)`` -- the configuration globals the path reads
     (FORCE_RES, PARTICLE_MASS, Om, Ol, W0, WA, SCALE_NOW).
 plus the batched path (``Plan``, whose ``props`` runs the halo property pass
-of src/properties.c:384-510), ``mass_definitions`` and ``bound_pass_host``.
+of src/properties.c:384-510), ``gather_calls`` (the records of each call
+from the halo hierarchy, src/groupies.c:212-268), ``mass_definitions`` and
+``bound_pass_host``.
 
 There is no CPU fallback: if libhalopot.so is missing or no gfx950 device is
 usable, every call raises.
@@ -83,7 +85,7 @@ _LIB = None
 
 EXPORTS = ("compute_potential", "compute_kinetic_energy", "hp_plan_create", "hp_plan_run",
            "hp_plan_potential", "hp_plan_stats", "hp_plan_destroy", "hp_plan_tune",
-           "hp_mass_definitions", "hp_plan_props", "hp_bound_pass_host",
+           "hp_mass_definitions", "hp_plan_props", "hp_gather_calls", "hp_bound_pass_host",
            "hp_debug_tree", "hp_last_error", "hp_set_error_handler")
 GLOBALS = ("FORCE_RES", "PARTICLE_MASS", "Om", "Ol", "W0", "WA", "SCALE_NOW")
 
@@ -108,6 +110,9 @@ def lib() -> C.CDLL:
         L.hp_plan_destroy.argtypes = [C.c_void_p]
         L.hp_plan_tune.argtypes = [C.c_void_p, C.c_void_p, C.c_int, C.POINTER(Tuning)]
         L.hp_mass_definitions.argtypes = [C.POINTER(C.c_char_p), C.POINTER(MassDefs)]
+        L.hp_gather_calls.argtypes = [C.c_void_p, C.c_int64, C.c_void_p, C.c_int64, C.c_void_p,
+                                      C.c_int64, C.c_double, C.c_void_p, C.c_int64, C.c_void_p,
+                                      C.POINTER(C.c_int64)]
         L.hp_plan_props.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p, C.POINTER(MassDefs),
                                     C.c_void_p, C.c_void_p]
         L.hp_bound_pass_host.argtypes = [C.c_void_p, C.c_int64, C.c_void_p, C.c_int64,
@@ -168,6 +173,25 @@ def options(accumulate=ACC_COMPENSATED, method=METHOD_TREE, use_graph=0, timing=
     o.accumulate, o.method, o.use_graph, o.timing, o.count_work, o.per_call = (
         accumulate, method, use_graph, timing, count_work, per_call)
     return o
+
+
+def gather_calls(tree: np.ndarray, base, d_particles: int, nparticles: int, d_po: int,
+                 cap_records: int, include_host_ratio: float = 0.3):
+    """hp_gather_calls: calc_particle_radii (src/groupies.c:212-268) for the
+    halos `base` of the hierarchy `tree` (TREE_DTYPE), from the particle
+    copies at device address d_particles into records at d_po.  Returns the
+    call descriptors (HALO_DTYPE) and the record count."""
+    from .catalog import TREE_DTYPE
+
+    if tree.dtype != TREE_DTYPE or not tree.flags.c_contiguous:
+        raise TypeError("tree must be a C-contiguous TREE_DTYPE array")
+    b = np.ascontiguousarray(base, np.int64)
+    calls = np.zeros(b.shape[0], HALO_DTYPE)
+    n = C.c_int64()
+    _check(lib().hp_gather_calls(_ptr(tree), tree.shape[0], _ptr(b), b.shape[0],
+                                 C.c_void_p(d_particles), nparticles, float(include_host_ratio),
+                                 C.c_void_p(d_po), cap_records, _ptr(calls), C.byref(n)))
+    return calls, n.value
 
 
 def mass_definitions(defs=None) -> MassDefs:

@@ -56,6 +56,21 @@ PROPS_DTYPE = np.dtype(
     }
 )
 
+# struct particle (src/particle.h:5-8): a FOF group's particle copies
+PARTICLE_DTYPE = np.dtype({"names": ["id", "pos"], "formats": ["<i8", ("<f4", (6,))],
+                           "offsets": [0, 8], "itemsize": 32})
+
+# hp_tree_halo (include/halopot.h): one halo of the hierarchy
+TREE_DTYPE = np.dtype(
+    {
+        "names": ["p_start", "num_p", "num_child_particles", "child", "next_cochild", "sub_of",
+                  "pos"],
+        "formats": ["<i8"] * 6 + [("<f4", (6,))],
+        "offsets": [0, 8, 16, 24, 32, 40, 48],
+        "itemsize": 72,
+    }
+)
+
 # mass definitions of the halo finder's defaults (src/config.template.h:9-13)
 DEFAULT_MASS_DEFINITIONS = ("vir", "200b", "200c", "500c", "2500c")
 
@@ -75,7 +90,7 @@ HALO_DTYPE = np.dtype(
 )
 
 SEEDS = {"c1": 0x5EED0001, "c2": 0x5EED0002, "c3": 0x5EED0003, "c4": 0x5EED0001,
-         "c5": 0x5EED0005}
+         "c5": 0x5EED0005, "tree": 0x5EED0006}
 
 
 @dataclasses.dataclass
@@ -387,6 +402,93 @@ def config3(seed=SEEDS["c3"], n_host=2_000_000, n_subs=30, c_host=6.0, rvir_host
     halos = _halo_rows(firsts, counts, np.concatenate([cen, scen]), np.concatenate([bulk, sbulk]),
                        np.array([0] + [1] * n_subs, dtype=np.int32))
     return Catalogue("c3", prm, po, halos)
+
+
+@dataclasses.dataclass
+class Hierarchy:
+    """FOF groups as the halo finder holds them before the bound pass: the
+    particle copies, each halo's own particles contiguous, and the halo tree
+    (struct halo / extra_halo_info links, src/halo.h:8-41)."""
+
+    params: Params
+    particles: np.ndarray   # PARTICLE_DTYPE
+    tree: np.ndarray        # TREE_DTYPE
+    base: np.ndarray        # the halos that get a call: num_p >= 1 (src/properties.c:524)
+
+
+def hierarchy(seed=SEEDS["tree"], n_groups=200, lo=200, hi=50_000, mean_subs=1.5, depth=3,
+              big_sub=0.15, empty_sub=0.03) -> Hierarchy:
+    """Seeded FOF groups: an NFW host per group (n ~ n^-1.9 on [lo, hi]) with
+    sub-halos to the given depth (Poisson numbers of children, each with a
+    log-uniform 0.5%-50% of its parent's particles, or 45%-90% with
+    probability big_sub, so that the host's subtree joins its call as
+    potential-only records; some empty).  num_child_particles is each halo's
+    own count plus its descendants' (src/properties.c:139-153)."""
+    rng = np.random.Generator(np.random.Philox(seed))
+    prm = Params()
+    n_host = power_law_counts(rng, n_groups, lo, hi)
+    rows = []          # [num_p, sub_of, centre(3), bulk(3)]
+    children = []
+    def add(num_p, parent, cen, bulk):
+        rows.append([num_p, parent, cen, bulk])
+        children.append([])
+        if parent >= 0:
+            children[parent].append(len(rows) - 1)
+        return len(rows) - 1
+    def grow(h, level):
+        if level >= depth:
+            return
+        num_p, _, cen, bulk = rows[h]
+        for _ in range(min(6, rng.poisson(mean_subs / (level + 1)))):
+            if rng.uniform() < empty_sub:
+                n = 0
+            elif rng.uniform() < big_sub:
+                n = int(num_p * rng.uniform(0.45, 0.9))
+            else:
+                n = int(num_p * math.exp(rng.uniform(math.log(0.005), math.log(0.5))))
+            rv = float(virial_radius(prm, max(n, 1) * prm.particle_mass))
+            rparent = float(virial_radius(prm, max(num_p, 1) * prm.particle_mass))
+            off = rparent * rng.uniform(0.05, 0.8) * _unit_vectors(rng, 1)[0]
+            sb = bulk + float(sigma_vir(max(num_p, 1) * prm.particle_mass, rparent)) * \
+                rng.standard_normal(3)
+            c = add(n, h, cen + off, sb)
+            if n > 20:
+                grow(c, level + 1)
+    for g in range(n_groups):
+        cen = rng.uniform(0.0, 100.0, 3)
+        bulk = rng.normal(0.0, 200.0, 3)
+        grow(add(int(n_host[g]), -1, cen, bulk), 0)
+    nh = len(rows)
+    num_p = np.array([r[0] for r in rows], np.int64)
+    cen = np.array([r[2] for r in rows])
+    bulk = np.array([r[3] for r in rows])
+    mvir = np.maximum(num_p, 1) * prm.particle_mass
+    pos, vel = nfw_particles(rng, num_p, np.exp(rng.normal(math.log(8.0), 0.3, nh)),
+                             virial_radius(prm, mvir), mvir, cen, bulk)
+    parts = np.zeros(int(num_p.sum()), PARTICLE_DTYPE)
+    parts["id"] = np.arange(parts.shape[0])
+    parts["pos"][:, :3] = pos
+    parts["pos"][:, 3:] = vel
+    tree = np.zeros(nh, TREE_DTYPE)
+    tree["p_start"] = np.concatenate([[0], np.cumsum(num_p)[:-1]])
+    tree["num_p"] = num_p
+    tree["sub_of"] = [r[1] for r in rows]
+    tree["child"] = -1
+    tree["next_cochild"] = -1
+    for h in range(nh):
+        kids = children[h]
+        if kids:
+            tree["child"][h] = kids[0]
+            for a, b in zip(kids[:-1], kids[1:]):
+                tree["next_cochild"][a] = b
+    ncp = num_p.copy()
+    for h in range(nh - 1, -1, -1):  # children come after their parent
+        for c in children[h]:
+            ncp[h] += ncp[c]
+    tree["num_child_particles"] = ncp
+    tree["pos"][:, :3] = cen
+    tree["pos"][:, 3:] = bulk
+    return Hierarchy(prm, parts, tree, np.nonzero(num_p >= 1)[0].astype(np.int64))
 
 
 def config5(seed=SEEDS["c5"], total=16_777_216, box=250.0, n_halos=50_000, frac_halo=0.40,

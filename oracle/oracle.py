@@ -56,6 +56,13 @@ def build(quiet: bool = True) -> None:
         print(out.stdout)
 
 
+def _record_dtype():
+    """struct potential, 48 bytes (src/potential.h:7-21)."""
+    return np.dtype({"names": ["pos", "r2", "pe", "ke", "flags"],
+                     "formats": [("<f4", (6,)), "<f4", "<f8", "<f4", "<i4"],
+                     "offsets": [0, 24, 32, 40, 44], "itemsize": 48})
+
+
 def _p(a):
     return a.ctypes.data_as(C.c_void_p)
 
@@ -108,8 +115,11 @@ class Restatement:
                                      C.c_void_p, C.c_void_p]
         L.orc_halo_props_batch.argtypes = [C.c_void_p, C.c_void_p, C.c_int64, C.POINTER(_Params),
                                            C.c_void_p, C.c_void_p, C.c_int]
+        L.orc_particle_radii.restype = C.c_int64
+        L.orc_particle_radii.argtypes = [C.c_void_p, C.c_void_p, C.c_int64, C.c_int64, C.c_double,
+                                         C.c_void_p, C.c_int64]
         for f in ("orc_sizeof_node", "orc_sizeof_halo", "orc_sizeof_potential",
-                  "orc_sizeof_props"):
+                  "orc_sizeof_props", "orc_sizeof_tree_halo"):
             getattr(L, f).restype = C.c_int64
         assert L.orc_sizeof_node() == NODE_DTYPE.itemsize
         assert L.orc_sizeof_potential() == 48
@@ -212,6 +222,20 @@ class Restatement:
     def hubble_scaling(self, prm, z):
         return self.lib.orc_hubble_scaling(C.byref(_params(prm)), z)
 
+    def particle_radii(self, particles: np.ndarray, tree: np.ndarray, base: int, ratio: float,
+                       cap: int) -> np.ndarray:
+        """calc_particle_radii (src/groupies.c:212-268) for halo `base`: its
+        records (48-byte layout).  particles: 32-byte struct particle copies;
+        tree: rows of the product's hp_tree_halo layout (72 bytes)."""
+        assert particles.dtype.itemsize == 32 and particles.flags.c_contiguous
+        assert tree.dtype.itemsize == self.lib.orc_sizeof_tree_halo() and tree.flags.c_contiguous
+        po = np.zeros(max(1, cap), _record_dtype())
+        n = self.lib.orc_particle_radii(_p(particles), _p(tree), tree.shape[0], base, ratio,
+                                        _p(po), cap)
+        if n < 0:
+            raise ValueError("records do not fit, or a sibling list returns to its first child")
+        return po[:n]
+
     def halo_props(self, po: np.ndarray, kept: int, pos, prm, thresh, out: np.ndarray) -> None:
         """Property pass (src/properties.c:384-510, bound = 0 then 1) of one
         call's kept records, sorted by r2 with pe and ke; out: one record of
@@ -297,6 +321,9 @@ class ReferenceProps:
                                            C.c_int64, C.c_void_p, C.c_void_p, C.c_int]
         L.ref_bound_pass.restype = C.c_int64
         L.ref_bound_pass.argtypes = [C.c_void_p, C.c_int64, C.c_void_p]
+        L.ref_particle_radii.restype = C.c_int64
+        L.ref_particle_radii.argtypes = [C.c_void_p, C.c_void_p, C.c_int64, C.c_int64, C.c_double,
+                                         C.c_void_p]
 
     @staticmethod
     def available(path=None) -> bool:
@@ -319,6 +346,18 @@ class ReferenceProps:
                            "formats": [("<f4", (6,)), ("<f4", (3,)), ("<f4", (3,)), "<f4",
                                        "<f4", "<i8", "<i4"],
                            "offsets": [0, 24, 36, 48, 52, 56, 64], "itemsize": 72})
+
+    def particle_radii(self, particles: np.ndarray, tree: np.ndarray, base: int, ratio: float,
+                       cap: int) -> np.ndarray:
+        """The reference's own calc_particle_radii(h, h, h->pos, 0, 0, 0)
+        for halo `base` (cap: room for the records it writes)."""
+        assert particles.dtype.itemsize == 32 and particles.flags.c_contiguous
+        assert tree.dtype.itemsize == 72 and tree.flags.c_contiguous
+        po = np.zeros(max(1, cap), _record_dtype())
+        n = self.lib.ref_particle_radii(_p(particles), _p(tree), tree.shape[0], base, ratio,
+                                        _p(po))
+        assert n <= cap
+        return po[:n]
 
     def bound_pass(self, po: np.ndarray, halo_row, prm):
         """The reference's own bound-pass steps (src/properties.c:537-550:

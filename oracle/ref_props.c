@@ -12,6 +12,9 @@
  *                          calc_scale_radius, estimate_total_energy and
  *                          _calc_pseudo_evolution_masses inside)
  *   ref_mass_definition -> calc_mass_definition (src/groupies.c:124-142)
+ *   ref_particle_radii  -> calc_particle_radii(h, h, h->pos, 0, 0, 0)
+ *                          (src/groupies.c:235-268), the record set-up of
+ *                          calc_additional_halo_props (src/properties.c:526)
  *   ref_bound_pass      -> the bound-pass steps of calc_additional_halo_props
  *                          (src/properties.c:537-550) on prepared records:
  *                          compute_potential, the potential-only strip,
@@ -35,7 +38,9 @@
 #endif
 
 #include "config_vars.h"
+#include "groupies.h"
 #include "halo.h"
+#include "particle.h"
 #include "potential.h"
 
 extern struct potential *po; /* src/groupies.c:33 */
@@ -44,6 +49,12 @@ extern double particle_thresh_dens[5]; /* src/groupies.c:56 */
 extern double particle_rvir_dens;
 #pragma omp threadprivate(particle_thresh_dens, particle_rvir_dens)
 
+extern struct particle *copies; /* src/groupies.c:30 */
+#pragma omp threadprivate(copies)
+
+int64_t calc_particle_radii(struct halo *base_h, struct halo *h, float *cen, int64_t p_start,
+                            int64_t level, int64_t potential_only,
+                            const struct HaloInfo *haloinfo);
 void _calc_additional_halo_props(struct halo *h, int64_t total_p, int64_t bound);
 void calc_mass_definition(void);
 void calculate_corevel(struct halo *h, struct potential *po, int64_t total_p);
@@ -161,4 +172,37 @@ int64_t ref_bound_pass(struct potential *recs, int64_t total_p, ref_bound_halo *
     bh->min_bulkvel_err = h.min_bulkvel_err;
     bh->n_core = h.n_core;
     return total_p;
+}
+
+/* one halo of the hierarchy (the product's hp_tree_halo layout) */
+typedef struct ref_tree_halo {
+    int64_t p_start, num_p, num_child_particles;
+    int64_t child, next_cochild, sub_of;
+    float   pos[6];
+} ref_tree_halo;
+
+/* po must hold every record the recursion writes (the caller sizes it) */
+int64_t ref_particle_radii(struct particle *parts, const ref_tree_halo *tree, int64_t nh,
+                           int64_t base, double ratio, struct potential *out) {
+    struct halo *hs = calloc((size_t)nh, sizeof(struct halo));
+    struct extra_halo_info *xs = calloc((size_t)nh, sizeof(struct extra_halo_info));
+    for (int64_t i = 0; i < nh; i++) {
+        hs[i].p_start = tree[i].p_start;
+        hs[i].num_p = tree[i].num_p;
+        hs[i].num_child_particles = tree[i].num_child_particles;
+        memcpy(hs[i].pos, tree[i].pos, sizeof hs[i].pos);
+        xs[i].child = tree[i].child;
+        xs[i].next_cochild = tree[i].next_cochild;
+        xs[i].sub_of = tree[i].sub_of;
+    }
+    struct HaloInfo hi = {hs, xs, nh};
+    INCLUDE_HOST_POTENTIAL_RATIO = ratio;
+    copies = parts;
+    po = out;
+    int64_t n = calc_particle_radii(hs + base, hs + base, hs[base].pos, 0, 0, 0, &hi);
+    po = NULL;
+    copies = NULL;
+    free(hs);
+    free(xs);
+    return n;
 }

@@ -25,6 +25,9 @@
  *   - bound-pass driver .................. src/properties.c:537-550 and the
  *                                           bound test at src/properties.c:403
  *   - vir_density / particle_rvir_dens ... src/groupies.c:97-142
+ *   - record assembly .................... src/groupies.c:212-268
+ *                                           (_reset_potentials,
+ *                                           calc_particle_radii)
  *   - halo property pass ................. src/properties.c:127-137 (angular
  *                                           momentum), 328-349 (energy),
  *                                           384-510 (without calc_shape,
@@ -684,6 +687,94 @@ void orc_bound_pass_batch_fn(struct potential *po, const int64_t *first,
         kept[h] = orc_bound_pass_fn(po + first[h], count[h], halos + h, p, cp, ke);
 }
 
+/* ---------------------------------------------------- record assembly */
+
+/* One halo of the hierarchy, in the layout of the product's hp_tree_halo. */
+typedef struct orc_tree_halo {
+    int64_t p_start, num_p, num_child_particles;
+    int64_t child, next_cochild, sub_of;
+    float   pos[6];
+} orc_tree_halo;
+
+typedef struct orc_particle { /* src/particle.h:5-8 */
+    int64_t id;
+    float   pos[6];
+} orc_particle;
+
+typedef struct orc_gather {
+    const orc_particle  *parts;
+    const orc_tree_halo *tree;
+    int64_t              nh;
+    double               ratio;
+    struct potential    *po;
+    int64_t              cap;
+    int64_t             *path; /* halos on the current descent, by level */
+    int64_t              npath;
+    int                  err;
+} orc_gather;
+
+/* src/groupies.c:212-233 */
+static void orc_reset_records(orc_gather *g, int64_t base, int64_t h, const float *cen,
+                              int64_t p_start, int potential_only) {
+    const orc_tree_halo *t = g->tree + h;
+    if (p_start + t->num_p > g->cap) {
+        g->err = 1;
+        return;
+    }
+    memset(g->po + p_start, 0, sizeof(struct potential) * (size_t)t->num_p);
+    for (int64_t j = 0; j < t->num_p; j++) {
+        const orc_particle *q = g->parts + t->p_start + j;
+        struct potential *r = g->po + p_start + j;
+        float r2 = 0;
+        for (int k = 0; k < 3; k++) {
+            float dx = q->pos[k] - cen[k];
+            r2 += dx * dx;
+        }
+        r->r2 = r2;
+        memcpy(r->pos, q->pos, sizeof(float) * 6);
+        if (potential_only) r->ke = -1;
+        if (h == base) r->flags = 1;
+        if (!potential_only && (t->num_p < g->tree[base].num_p * 0.03)) r->flags = 2;
+    }
+}
+
+/* src/groupies.c:235-268 */
+static int64_t orc_radii(orc_gather *g, int64_t base, int64_t h, const float *cen,
+                         int64_t p_start, int64_t level, int potential_only) {
+    if (g->err) return p_start;
+    g->path[level] = h;
+    for (int64_t j = 0; j < level; j++)
+        if (g->path[j] == h) return p_start;
+    orc_reset_records(g, base, h, cen, p_start, potential_only);
+    int64_t total = p_start + g->tree[h].num_p;
+    int64_t first_child = g->tree[h].child, child = first_child;
+    while (child > -1) {
+        total = orc_radii(g, base, child, cen, total, level + 1, potential_only);
+        child = g->tree[child].next_cochild;
+        if (child == first_child) { /* src/groupies.c:254 asserts against it */
+            g->err = 2;
+            return total;
+        }
+    }
+    int64_t parent = g->tree[h].sub_of;
+    if (h == base && parent > -1 &&
+        g->tree[parent].num_child_particles * g->ratio < g->tree[h].num_child_particles)
+        total = orc_radii(g, base, parent, cen, total, level + 1, 1);
+    return total;
+}
+
+/* calc_particle_radii(h, h, h->pos, 0, 0, 0) for halo `base`: its records
+   into po (at most cap).  Returns the record count, or -1 when the records
+   do not fit or a sibling list returns to its first child. */
+int64_t orc_particle_radii(const orc_particle *parts, const orc_tree_halo *tree, int64_t nh,
+                           int64_t base, double ratio, struct potential *po, int64_t cap) {
+    orc_gather g = {parts, tree, nh, ratio, po, cap, NULL, 0, 0};
+    g.path = orc_grow(NULL, sizeof(int64_t) * (size_t)(nh + 2));
+    int64_t n = orc_radii(&g, base, base, tree[base].pos, 0, 0, 0);
+    free(g.path);
+    return g.err ? -1 : n;
+}
+
 /* ------------------------------------------------------ halo properties */
 
 /* Output of the property pass, in the layout of the product's
@@ -850,3 +941,4 @@ int64_t orc_sizeof_node(void) { return (int64_t)sizeof(orc_node); }
 int64_t orc_sizeof_halo(void) { return (int64_t)sizeof(orc_halo); }
 int64_t orc_sizeof_potential(void) { return (int64_t)sizeof(struct potential); }
 int64_t orc_sizeof_props(void) { return (int64_t)sizeof(orc_props); }
+int64_t orc_sizeof_tree_halo(void) { return (int64_t)sizeof(orc_tree_halo); }
