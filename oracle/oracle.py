@@ -324,6 +324,9 @@ class ReferenceProps:
         L.ref_particle_radii.restype = C.c_int64
         L.ref_particle_radii.argtypes = [C.c_void_p, C.c_void_p, C.c_int64, C.c_int64, C.c_double,
                                          C.c_void_p]
+        L.ref_particle_radii_batch.argtypes = [C.c_void_p, C.c_void_p, C.c_int64, C.c_void_p,
+                                               C.c_void_p, C.c_int64, C.c_double, C.c_void_p,
+                                               C.c_void_p, C.c_int]
 
     @staticmethod
     def available(path=None) -> bool:
@@ -358,6 +361,16 @@ class ReferenceProps:
                                         _p(po))
         assert n <= cap
         return po[:n]
+
+    def particle_radii_batch(self, particles, tree, base, offset, ratio, out, nthreads=0):
+        """particle_radii for every halo of `base`, its records at
+        out[offset[i]:], OpenMP dynamic over halos; returns the counts."""
+        b = np.ascontiguousarray(base, np.int64)
+        off = np.ascontiguousarray(offset, np.int64)
+        cnt = np.zeros(b.shape[0], np.int64)
+        self.lib.ref_particle_radii_batch(_p(particles), _p(tree), tree.shape[0], _p(b), _p(off),
+                                          b.shape[0], ratio, _p(out), _p(cnt), nthreads)
+        return cnt
 
     def bound_pass(self, po: np.ndarray, halo_row, prm):
         """The reference's own bound-pass steps (src/properties.c:537-550:

@@ -206,3 +206,19 @@ int64_t ref_particle_radii(struct particle *parts, const ref_tree_halo *tree, in
     free(xs);
     return n;
 }
+
+/* ref_particle_radii for every listed halo, OpenMP dynamic over halos (the
+   halo finder's per-thread copies / po / halo_ids are threadprivate); call
+   b's records go to out + offset[b] */
+void ref_particle_radii_batch(struct particle *parts, const ref_tree_halo *tree, int64_t nh,
+                              const int64_t *base, const int64_t *offset, int64_t nbase,
+                              double ratio, struct potential *out, int64_t *count, int nthreads) {
+#ifdef _OPENMP
+    if (nthreads > 0) omp_set_num_threads(nthreads);
+#else
+    (void)nthreads;
+#endif
+#pragma omp parallel for schedule(dynamic)
+    for (int64_t b = 0; b < nbase; b++)
+        count[b] = ref_particle_radii(parts, tree, nh, base[b], ratio, out + offset[b]);
+}
