@@ -220,11 +220,13 @@ int hp_gather_calls(const hp_tree_halo *tree, int64_t nhalos, const int64_t *bas
 /* ------------------------------------------------------- halo properties */
 
 /* Density thresholds of the mass definitions, in particles per (Mpc/h)^3:
- * particle_thresh_dens[0..4] and particle_rvir_dens as calc_mass_definition
- * leaves them (src/groupies.c:102-142; float-rounded values). */
+ * particle_thresh_dens[0..4], particle_rvir_dens and particle_rvir_dens_z0
+ * as calc_mass_definition leaves them (src/groupies.c:102-142; float-rounded
+ * values). */
 typedef struct hp_mass_defs {
     double thresh_dens[5];  /* MASS_DEFINITION, MASS_DEFINITION2..5 */
     double rvir_dens;       /* "vir" */
+    double rvir_dens_z0;    /* "vir" at a = 1 (pseudo-evolution masses) */
 } hp_mass_defs;
 
 /* calc_mass_definition (src/groupies.c:124-142) for the five definition
@@ -236,10 +238,12 @@ int hp_mass_definitions(const char *const defs[5], hp_mass_defs *out);
 
 /* One call's results of the halo property pass: src/properties.c:384-510
  * run for bound = 0 and then bound = 1 (src/properties.c:552-553), with the
- * defaults BOUND_PROPS = 1 and LIGHTCONE = 0.  The fields named after
- * struct halo (src/halo.h:8-35) are the values the reference stores there;
- * calc_shape, calc_scale_radius and _calc_pseudo_evolution_masses (shape,
- * rs, klypin_rs, m_pe_b, m_pe_d) are not computed.  Radii in kpc/h. */
+ * defaults BOUND_PROPS = 1, LIGHTCONE = 0, SHAPE_ITERATIONS = 10 and
+ * WEIGHTED_SHAPES = 1, calc_shape (src/properties.c:216-326),
+ * calc_scale_radius (src/nfw.c:130-173) and _calc_pseudo_evolution_masses
+ * (src/properties.c:351-382) included.  The fields are the values the
+ * reference stores in struct halo (src/halo.h:8-35, built with
+ * OUTPUT_NFW_CHI2 and OUTPUT_INERTIA_TENSOR).  Radii in kpc/h.  224 bytes. */
 typedef struct hp_halo_props {
     float   m;               /* bound = 0 pass: part_mdelta * PARTICLE_MASS */
     float   mgrav;           /* bound = 1 pass */
@@ -250,7 +254,11 @@ typedef struct hp_halo_props {
     float   J[3];
     float   energy, kin_to_pot, spin, bullock_spin;
     float   rvir;            /* the pass's virial radius (used by bullock_spin) */
-    int32_t pad;
+    float   rs, klypin_rs, chi2;
+    float   b_to_a, c_to_a, A[3];            /* within the main definition */
+    float   b_to_a2, c_to_a2, A2[3];         /* within MASS_DEFINITION4 */
+    float   inertia_tensor[6], inertia_tensor2[6];  /* xx, yy, zz, xy, yz, zx */
+    float   m_pe_b, m_pe_d;
     int64_t n_mdelta;        /* part_mdelta: bound records within the main definition */
     int64_t dens_tot;        /* records [0, dens_tot) enter the averages */
     int64_t n_vir;           /* bound records within the virial definition */

@@ -264,7 +264,7 @@ struct Engine {
     // unbind
     DevBuf keys0, keys1, vals0, vals1, hist, hsum, srt, callinfo, tsum, tcand, tile_i32;
     // halo properties
-    DevBuf pcnt, pscan, pcall, psums;
+    DevBuf pcnt, pscan, pcall, psums, shape_st, shape_part, shape_cd, scale_calls;
     // record assembly
     DevBuf gsegs, gtiles, gcen;
     int64_t cap_nodes = 0, cap_small = 0, cap_group = 0, cap_big = 0, cap_tiles = 0, cap_fat = 0;
@@ -308,7 +308,7 @@ struct Engine {
         DevBuf *all[] = {&hfirst, &hcount, &root_tiles, &groups, &direct_blocks, &rec_call,
                          &scan_tiles, &call_tile0, &call_ntiles, &box, &bad, &badlist, &A, &B, &F,
                          &leaf_of, &nodes, &parent, &fat, &small0, &group0, &big, &tiles,
-                         &parts, &splitres, &dparts, &counters, &work, &depth_buf, &pcnt, &pscan, &pcall, &psums, &gsegs, &gtiles, &gcen, &keys0, &keys1, &vals0,
+                         &parts, &splitres, &dparts, &counters, &work, &depth_buf, &pcnt, &pscan, &pcall, &psums, &shape_st, &shape_part, &shape_cd, &scale_calls, &gsegs, &gtiles, &gcen, &keys0, &keys1, &vals0,
                          &vals1, &hist, &hsum, &srt, &callinfo, &tsum, &tcand, &tile_i32};
         if (stream) {
             (void)hipStreamSynchronize(stream);
@@ -819,13 +819,19 @@ struct Engine {
         HP_ENSURE(pscan, sizeof(PropScan) * (nst + 1));
         HP_ENSURE(psums, sizeof(PropSums) * (nst + 1));
         HP_ENSURE(pcall, sizeof(PropCall) * (H + 1));
+        HP_ENSURE(shape_st, sizeof(ShapeState) * 2 * (H + 1));
+        HP_ENSURE(shape_part, sizeof(ShapePart) * 2 * (nst + 1));
+        HP_ENSURE(shape_cd, sizeof(float4) * (T.nrec + 1));
+        HP_ENSURE(scale_calls, sizeof(ScaleCall) * (H + 1));
         PropArgs pa;
         const double four_thirds_pi = 4.0 * M_PI / 3.0;  // src/properties.c:393-398
         pa.dens_thresh = md.thresh_dens[0] * four_thirds_pi;
         for (int k = 0; k < 4; k++) pa.d[k] = md.thresh_dens[k + 1] * four_thirds_pi;
         pa.rvir_thresh = md.rvir_dens * four_thirds_pi;
         pa.thresh0 = md.thresh_dens[0];
+        pa.thresh3 = md.thresh_dens[3];
         pa.rvir_dens = md.rvir_dens;
+        pa.rvir_dens_z0 = md.rvir_dens_z0;
         pa.force_res = FORCE_RES;
         pa.particle_mass = PARTICLE_MASS;
         pa.scale_now = (float)SCALE_NOW;
@@ -856,6 +862,40 @@ struct Engine {
         k_props_final<<<blocks(H, 256), 256, 0, s>>>(d_po, t0, tn, H, d_halos, hf,
                                                      pcall.as<PropCall>(), psums.as<PropSums>(),
                                                      pa, d_props);
+        HP_LAUNCHED();
+        // calc_shape in both windows, SHAPE_ITERATIONS rounds
+        ShapeState *ss = shape_st.as<ShapeState>();
+        k_shape_init<<<blocks(H, 256), 256, 0, s>>>(H, d_halos, hc, pcall.as<PropCall>(), d_props,
+                                                    pa, ss);
+        HP_LAUNCHED();
+        if (nst > 0) {
+            k_shape_prep<<<nst, kWave, 0, s>>>(d_po, tl, nst, d_halos, hf, ss,
+                                               shape_cd.as<float4>());
+            HP_LAUNCHED();
+        }
+        for (int it = 0; it < kShapeIterations; it++) {
+            if (nst > 0) {
+                k_shape_tiles<<<nst, kWave, 0, s>>>(tl, nst, hf, ss, shape_cd.as<float4>(),
+                                                    shape_part.as<ShapePart>());
+                HP_LAUNCHED();
+            }
+            k_shape_calls<<<blocks(2 * (int64_t)H, 128), 128, 0, s>>>(
+                H, it, t0, tn, shape_part.as<ShapePart>(), ss);
+            HP_LAUNCHED();
+        }
+        // calc_scale_radius and the pseudo-evolution masses
+        k_scale_init<<<blocks(H, 256), 256, 0, s>>>(H, d_po, hf, t0, tn, pcall.as<PropCall>(),
+                                                    psums.as<PropSums>(),
+                                                    scale_calls.as<ScaleCall>());
+        HP_LAUNCHED();
+        if (nst > 0) {
+            k_scale_bins<<<nst, kWave, 0, s>>>(d_po, tl, nst, hf, pcnt.as<PropCount>(),
+                                               scale_calls.as<ScaleCall>());
+            HP_LAUNCHED();
+        }
+        k_props_late<<<blocks(H, 64), 64, 0, s>>>(H, d_po, hf, t0, pcnt.as<PropCount>(),
+                                                  pcall.as<PropCall>(), ss,
+                                                  scale_calls.as<ScaleCall>(), pa, d_props);
         HP_LAUNCHED();
         return HP_OK;
     }
@@ -1353,6 +1393,10 @@ __attribute__((visibility("default"))) int hp_mass_definitions(const char *const
     for (int k = 0; k < 5; k++)
         out->thresh_dens[k] = mass_definition(defs && defs[k] ? defs[k] : dflt[k]);
     out->rvir_dens = mass_definition("vir");
+    // src/groupies.c:123: "vir" at a = 1 with the same float background
+    // density, kept in double there
+    const float cons = (float)(Om * 2.77519737e11 / PARTICLE_MASS);
+    out->rvir_dens_z0 = vir_density(1.0) * cons;
     return HP_OK;
 }
 

@@ -38,7 +38,7 @@
 
 namespace hp {
 
-static_assert(sizeof(hp_halo_props) == 120 && offsetof(hp_halo_props, n_mdelta) == 88,
+static_assert(sizeof(hp_halo_props) == 224 && offsetof(hp_halo_props, n_mdelta) == 192,
               "hp_halo_props layout");
 
 constexpr double kVmaxConst = 6.55833746e-5;  // src/universal_constants.h:6
@@ -50,7 +50,9 @@ struct PropArgs {
     double d[4];          // particle_thresh_dens[1..4] * 4pi/3
     double rvir_thresh;   // particle_rvir_dens * 4pi/3
     double thresh0;       // particle_thresh_dens[0]
+    double thresh3;       // particle_thresh_dens[3] (the second shape window)
     double rvir_dens;     // particle_rvir_dens
+    double rvir_dens_z0;  // particle_rvir_dens_z0
     double force_res, particle_mass;
     float scale_now;
 };
@@ -70,6 +72,7 @@ struct PropScan {
     int32_t j_all, j_after;
     double v_all;          // best n / r above rvir_thresh in the tile
     double v_after;        // the same from the tile's first index above dens_thresh on
+    double pe_b;           // max of n^2 / r^1.5 (src/properties.c:369-373)
 };
 
 // per call, walk 1 combined
@@ -82,6 +85,7 @@ struct PropCall {
     int32_t vir;
     int32_t j_vmax;        // -1: vmax stays 0
     double vmax;
+    double pe_b;
 };
 
 // walk 2 partial of one tile
@@ -189,6 +193,7 @@ __global__ __launch_bounds__(kWave) void k_props_scan(const struct potential *__
     s.vir = 0;
     s.v_all = s.v_after = -1.0;
     s.j_all = s.j_after = INT32_MAX;
+    s.pe_b = 0.0;
     bool seen0 = false;  // an index above dens_thresh earlier in this tile
     for (int32_t base = st.off; base < end; base += kWave) {
         const int32_t j = base + lane;
@@ -212,6 +217,11 @@ __global__ __launch_bounds__(kWave) void k_props_scan(const struct potential *__
         const bool after = seen0 || (m0 & (lane == 63 ? ~0ull : ((2ull << lane) - 1ull))) != 0;
         seen0 = seen0 || m0 != 0;
         if (b) {
+            // src/properties.c:365-373: unclamped radius
+            double r32 = sqrt(sqrt((double)r2));
+            r32 = r32 * r32 * r32;
+            const double q = (double)((int64_t)n * (int64_t)n) / r32;
+            s.pe_b = q > s.pe_b ? q : s.pe_b;
             if (c0) {
                 s.first0 = min(s.first0, j);
                 s.last0 = j;
@@ -236,6 +246,7 @@ __global__ __launch_bounds__(kWave) void k_props_scan(const struct potential *__
     s.vir = wave_max(s.vir);
     wave_arg_better(s.v_all, s.j_all);
     wave_arg_better(s.v_after, s.j_after);
+    s.pe_b = wave_max(s.pe_b);
     if (lane == 0) out[t] = s;
 }
 
@@ -250,6 +261,7 @@ __global__ void k_props_calls(const int32_t *__restrict__ call_tile0,
     PropCall c;
     c.kept = call_kept(halos, hcount, h);
     int32_t last_all = -1, last0 = -1, n0 = 0, tfirst = -1;
+    double pe_b = 0.0;
     for (int k = 0; k < 4; k++) c.alt[k] = 0;
     c.vir = 0;
     for (int32_t t = t0; t < t1; t++) {
@@ -262,7 +274,9 @@ __global__ void k_props_calls(const int32_t *__restrict__ call_tile0,
         }
         for (int k = 0; k < 4; k++) c.alt[k] = max(c.alt[k], ts[t].alt[k]);
         c.vir = max(c.vir, s.vir);
+        pe_b = s.pe_b > pe_b ? s.pe_b : pe_b;
     }
+    c.pe_b = pe_b;
     // vmax counts from the first index above dens_thresh on: the part of
     // that tile after it, then whole tiles
     double vm = -1.0;
@@ -425,11 +439,555 @@ __global__ void k_props_final(const struct potential *__restrict__ srt,
     o.bullock_spin = (float)(m > 0 ? Jh / (mvir * sqrt(2.0 * kGc * mvir * rvir *
                                                         (double)scale_now / 1e3))
                                    : 0.0);
-    o.pad = 0;
+    // filled by the shape, scale-radius and pseudo-evolution kernels
+    o.rs = o.klypin_rs = o.chi2 = 0.0f;
+    o.b_to_a = o.c_to_a = o.b_to_a2 = o.c_to_a2 = 0.0f;
+    for (int k = 0; k < 3; k++) o.A[k] = o.A2[k] = 0.0f;
+    for (int k = 0; k < 6; k++) o.inertia_tensor[k] = o.inertia_tensor2[k] = 0.0f;
+    o.m_pe_b = o.m_pe_d = 0.0f;
     o.n_mdelta = c.n0;
     o.dens_tot = c.dens_tot;
     o.n_vir = c.vir;
     o.n_half = ih;
+    props[h] = o;
+}
+
+// ------------------------------------------------------------- shapes
+//
+// calc_shape (src/properties.c:216-326) twice per call: window 0 over the
+// records [0, np_alt[2]) with h->r from MASS_DEFINITION4, window 1 over
+// [0, dens_tot) with the final h->r.  Each iteration is a tile pass (the
+// weighted second moments of the bound records inside the current
+// ellipsoid) and a per-(call, window) step: normalise, Jacobi, axis ratios,
+// convergence.  The host runs SHAPE_ITERATIONS iterations; a window that has
+// stopped is skipped.  The records' offsets from the centre (float
+// differences, as the reference takes them) and bound flags are compacted
+// once into 16 bytes per record.
+
+constexpr int kShapeIterations = 10;  // SHAPE_ITERATIONS, src/config.template.h:90
+
+struct ShapeState {
+    double orth[3][3];
+    double eig[3];
+    float r;           // h->r of the window, kpc/h
+    float min_r;
+    float b_to_a, c_to_a, A[3];
+    float inertia[6];
+    int32_t n;         // records [0, n)
+    int32_t limit;     // iterations to run; -1 until the first pass counted
+    int32_t done;
+    int32_t assigned;  // inertia written by this window
+};
+
+struct ShapePart {
+    double w;
+    double t[6];       // xx, yy, zz, yx, zx, zy
+    int32_t cnt;       // bound records of the window in the tile
+    int32_t pad;
+};
+
+__global__ void k_shape_init(int32_t nh, const hp_halo *__restrict__ halos,
+                             const int64_t *__restrict__ hcount, const PropCall *__restrict__ pc,
+                             const hp_halo_props *__restrict__ props, PropArgs pa,
+                             ShapeState *__restrict__ st) {
+    const int32_t h = blockIdx.x * blockDim.x + threadIdx.x;
+    if (h >= nh) return;
+    const PropCall c = pc[h];
+    for (int w = 0; w < 2; w++) {
+        ShapeState s;
+        s.r = w == 0 ? (float)(cbrt((3.0 / (4.0 * M_PI)) * (double)c.alt[2] / pa.thresh3) * 1e3)
+                     : props[h].r;
+        s.n = min(w == 0 ? c.alt[2] : c.dens_tot, c.kept);
+        float min_r = (float)(pa.force_res * pa.force_res);
+        min_r *= 1e6 / (s.r * s.r);
+        s.min_r = min_r;
+        s.b_to_a = s.c_to_a = 0.0f;
+        for (int k = 0; k < 3; k++) s.A[k] = 0.0f;
+        for (int k = 0; k < 6; k++) s.inertia[k] = 0.0f;
+        for (int k = 0; k < 3; k++) {
+            for (int l = 0; l < 3; l++) s.orth[k][l] = k == l ? 1.0 : 0.0;
+            s.eig[k] = (double)(s.r * s.r) * 1e-6;
+        }
+        s.limit = -1;
+        s.done = !(s.r > 0) || s.n <= 0;
+        s.assigned = 0;
+        st[2 * h + w] = s;
+    }
+}
+
+// x, y, z offsets from the centre and the bound flag of the records the
+// windows cover
+__global__ __launch_bounds__(kWave) void k_shape_prep(const struct potential *__restrict__ srt,
+                                                      const ScanTile *__restrict__ tiles,
+                                                      int32_t ntiles,
+                                                      const hp_halo *__restrict__ halos,
+                                                      const int64_t *__restrict__ hfirst,
+                                                      const ShapeState *__restrict__ st,
+                                                      float4 *__restrict__ cd) {
+    const int32_t t = blockIdx.x;
+    if (t >= ntiles) return;
+    const ScanTile tl = tiles[t];
+    const int32_t n = max(st[2 * tl.call].n, st[2 * tl.call + 1].n);
+    const int64_t first = hfirst[tl.call];
+    const float c0 = halos[tl.call].pos[0], c1 = halos[tl.call].pos[1], c2 = halos[tl.call].pos[2];
+    for (int32_t j = tl.off + lane_id(); j < min(tl.off + kScanTile, n); j += kWave) {
+        const struct potential &r = srt[first + j];
+        cd[first + j] = make_float4(r.pos[0] - c0, r.pos[1] - c1, r.pos[2] - c2,
+                                    !(r.pe < (double)r.ke) ? 1.0f : 0.0f);
+    }
+}
+
+// src/properties.c:243-268 for one tile, both windows
+__global__ __launch_bounds__(kWave) void k_shape_tiles(const ScanTile *__restrict__ tiles,
+                                                       int32_t ntiles,
+                                                       const int64_t *__restrict__ hfirst,
+                                                       const ShapeState *__restrict__ st,
+                                                       const float4 *__restrict__ cd,
+                                                       ShapePart *__restrict__ part) {
+    const int32_t t = blockIdx.x;
+    if (t >= ntiles) return;
+    const ScanTile tl = tiles[t];
+    const int64_t first = hfirst[tl.call];
+    for (int w = 0; w < 2; w++) {
+        const ShapeState &s = st[2 * tl.call + w];
+        double acc[7] = {0, 0, 0, 0, 0, 0, 0};
+        int32_t cnt = 0;
+        if (!s.done) {
+            const int32_t end = min(tl.off + kScanTile, s.n);
+            for (int32_t j = tl.off + lane_id(); j < end; j += kWave) {
+                const float4 q = cd[first + j];
+                if (q.w == 0.0f) continue;
+                cnt++;
+                const double d[3] = {(double)q.x, (double)q.y, (double)q.z};
+                double r = 0;
+                for (int k = 0; k < 3; k++) {
+                    double dr = 0;
+                    for (int l = 0; l < 3; l++) dr += s.orth[k][l] * d[l];
+                    r += dr * dr / s.eig[k];
+                }
+                if (r < (double)s.min_r) r = s.min_r;
+                if (!(r > 0 && r <= 1)) continue;
+                const double tw = 1.0 / r;  // WEIGHTED_SHAPES = 1
+                acc[0] += tw;
+                acc[1] += d[0] * d[0] * tw;
+                acc[2] += d[1] * d[1] * tw;
+                acc[3] += d[2] * d[2] * tw;
+                acc[4] += d[0] * d[1] * tw;
+                acc[5] += d[0] * d[2] * tw;
+                acc[6] += d[1] * d[2] * tw;
+            }
+        }
+        ShapePart pt;
+        pt.w = wave_sum(acc[0]);
+        for (int k = 0; k < 6; k++) pt.t[k] = wave_sum(acc[k + 1]);
+        pt.cnt = wave_sum(cnt);
+        pt.pad = 0;
+        if (lane_id() == 0) part[2 * t + w] = pt;
+    }
+}
+
+// |a| + sqrt(x), the sum and the root taken in extended precision and
+// rounded once (the reference evaluates both in x87 long double)
+__device__ __forceinline__ double add_sqrt_ext(double a_abs, double x) {
+    const double r = sqrt(x);
+    if (!(r > 0.0)) return a_abs + r;
+    const double e = fma(-r, r, x) / (2.0 * r);  // sqrt(x) = r + e to O(e^2)
+    const double s = a_abs + r;
+    const double bb = s - a_abs;
+    const double err = (a_abs - (s - bb)) + (r - bb);
+    return s + (err + e);
+}
+
+// The cyclic Jacobi eigenvalue iteration of src/jacobi.c:8-135 (largest
+// off-diagonal element first, each row's largest column cached, stop when
+// no eigenvalue changes by more than 1e-6 of itself).  orth's rows are the
+// eigenvectors; m is left symmetric.
+__device__ void jacobi3(double m[3][3], double eig[3], double orth[3][3]) {
+    int max_col[3], changed[3], state = 0;
+    for (int i = 0; i < 3; i++) {
+        double mx = 0;
+        max_col[i] = i + 1;
+        for (int j = i + 1; j < 3; j++)
+            if (fabs(m[i][j]) > mx) {
+                max_col[i] = j;
+                mx = fabs(m[i][j]);
+            }
+    }
+    for (int i = 0; i < 3; i++)
+        for (int j = 0; j < 3; j++) orth[i][j] = i == j ? 1.0 : 0.0;
+    for (int i = 0; i < 3; i++) {
+        eig[i] = m[i][i];
+        changed[i] = 1;
+        bool off = false;
+        for (int j = 0; j < 3; j++)
+            if (j != i && m[i][j] != 0.0) off = true;
+        if (!off) {
+            state--;
+            changed[i] = 0;
+        }
+    }
+    state += 3;
+    for (int guard = 0; state && guard < 1000; guard++) {
+        int k = 0;
+        double mx = fabs(m[0][max_col[0]]);
+        if (fabs(m[1][max_col[1]]) > mx) k = 1;
+        const int l = max_col[k];
+        const double v = m[k][l];
+        if (v == 0.0) break;
+        const double a = (eig[l] - eig[k]) * 0.5;
+        double t = add_sqrt_ext(fabs(a), v * v + a * a);
+        double s = add_sqrt_ext(0.0, v * v + t * t);
+        const double c = t / s;
+        s = v / s;
+        t = v * v / t;
+        if (a < 0) {
+            s = -s;
+            t = -t;
+        }
+        const int xs[2] = {k, l};
+        const double ys[2] = {-t, t};
+        for (int q = 0; q < 2; q++) {
+            const int x = xs[q];
+            const double y = ys[q];
+            eig[x] += y;
+            if (changed[x] && fabs(y) < 1e-6 * fabs(eig[x])) {
+                changed[x] = 0;
+                state--;
+            } else if (!changed[x] && fabs(y) > 1e-6 * fabs(eig[x])) {
+                changed[x] = 1;
+                state++;
+            }
+        }
+        m[k][l] = 0.0;
+        auto rot = [&](double (*M)[3], int w, int x, int y, int z, bool r) {
+            const double tt = M[w][x], uu = M[y][z];
+            M[w][x] = tt * c - s * uu;
+            M[y][z] = s * tt + c * uu;
+            if (r) {
+                if (fabs(M[w][x]) > fabs(M[w][max_col[w]])) max_col[w] = x;
+                if (y < 3 && fabs(M[y][z]) > fabs(M[y][max_col[y]])) max_col[y] = z;
+            }
+        };
+        for (int i = 0; i < k; i++) rot(m, i, k, i, l, true);
+        for (int i = k + 1; i < l; i++) rot(m, k, i, i, l, true);
+        for (int i = l + 1; i < 3; i++) rot(m, k, i, l, i, true);
+        for (int i = 0; i < 3; i++) rot(orth, k, i, l, i, false);
+    }
+    for (int k = 0; k < 2; k++)
+        for (int l = k + 1; l < 3; l++) m[k][l] = m[l][k];
+}
+
+// src/properties.c:270-325 for each (call, window) still running
+__global__ void k_shape_calls(int32_t nh, int32_t it, const int32_t *__restrict__ call_tile0,
+                              const int32_t *__restrict__ call_ntiles,
+                              const ShapePart *__restrict__ part, ShapeState *__restrict__ st) {
+    const int32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= 2 * nh) return;
+    const int32_t h = i >> 1, w = i & 1;
+    ShapeState s = st[i];
+    if (s.done) return;
+    double wt = 0.0, t6[6] = {0, 0, 0, 0, 0, 0};
+    int64_t cnt = 0;
+    for (int32_t t = call_tile0[h]; t < call_tile0[h] + call_ntiles[h]; t++) {
+        const ShapePart &pt = part[2 * t + w];
+        wt += pt.w;
+        for (int k = 0; k < 6; k++) t6[k] += pt.t[k];
+        cnt += pt.cnt;
+    }
+    if (s.limit < 0) {  // src/properties.c:228-236
+        if (cnt < 3) {
+            s.done = 1;
+            st[i] = s;
+            return;
+        }
+        s.limit = (int32_t)min((int64_t)kShapeIterations, cnt);
+    }
+    if (it >= s.limit || !(wt != 0.0)) {
+        s.done = 1;
+        st[i] = s;
+        return;
+    }
+    double m[3][3] = {{t6[0], t6[3], t6[4]}, {t6[3], t6[1], t6[5]}, {t6[4], t6[5], t6[2]}};
+    for (int k = 0; k < 3; k++)
+        for (int l = 0; l < 3; l++) m[k][l] /= wt;
+    const float inert[6] = {(float)m[0][0], (float)m[1][1], (float)m[2][2],
+                            (float)m[0][1], (float)m[1][2], (float)m[2][0]};
+    jacobi3(m, s.eig, s.orth);
+    int a = 0, b = 1, c = 2;
+    if (s.eig[1] > s.eig[0]) {
+        b = 0;
+        a = 1;
+    }
+    if (s.eig[2] > s.eig[b]) {
+        c = b;
+        b = 2;
+    }
+    if (s.eig[b] > s.eig[a]) {
+        const int q = a;
+        a = b;
+        b = q;
+    }
+    if (!(s.eig[a] != 0.0) || !(s.eig[b] != 0.0) || !(s.eig[c] != 0.0)) {
+        s.done = 1;
+        st[i] = s;
+        return;
+    }
+    const float b_to_a = (float)sqrt(s.eig[b] / s.eig[a]);
+    const float c_to_a = (float)sqrt(s.eig[c] / s.eig[a]);
+    if ((fabs((double)(b_to_a - s.b_to_a)) < 0.01 * (double)s.b_to_a) &&
+        (fabs((double)(c_to_a - s.c_to_a)) < 0.01 * (double)s.c_to_a)) {
+        s.done = 1;
+        st[i] = s;
+        return;
+    }
+    s.b_to_a = b_to_a > 0 ? b_to_a : 0.0f;
+    s.c_to_a = c_to_a > 0 ? c_to_a : 0.0f;
+    const double r = sqrt(s.eig[a]);
+    for (int k = 0; k < 3; k++) {
+        s.A[k] = (float)(1e3 * r * s.orth[a][k]);
+        s.eig[k] *= ((double)(s.r * s.r) * 1e-6) / (r * r);
+    }
+    for (int k = 0; k < 6; k++) s.inertia[k] = inert[k];
+    s.assigned = 1;
+    st[i] = s;
+}
+
+// ------------------------------------------- scale radius, pseudo-evolution
+
+constexpr int kScaleBins = 50;    // MAX_SCALE_BINS, src/nfw.c:9
+constexpr int kMinPerBin = 15;    // MIN_PART_PER_BIN
+constexpr int kMinScale = 100;    // MIN_SCALE_PART
+constexpr double kRmaxToRs = 2.1626;       // src/universal_constants.h:7
+constexpr double kRsConstant = 0.216216595;  // src/universal_constants.h:8
+
+struct ScaleCall {
+    int32_t ppbin;     // bound records per bin; 0: no fit (fewer than 100)
+    int32_t nbins;
+    int32_t end;       // bins come from records [0, end) = [0, dens_tot - 1)
+    int32_t pad;
+    float bin_r[kScaleBins + 1];
+};
+
+// src/nfw.c:134-152: the bound records of [0, dens_tot - 1) and the bin size
+__global__ void k_scale_init(int32_t nh, const struct potential *__restrict__ srt,
+                             const int64_t *__restrict__ hfirst,
+                             const int32_t *__restrict__ call_tile0,
+                             const int32_t *__restrict__ call_ntiles,
+                             const PropCall *__restrict__ pc, const PropSums *__restrict__ ts,
+                             ScaleCall *__restrict__ sc) {
+    const int32_t h = blockIdx.x * blockDim.x + threadIdx.x;
+    if (h >= nh) return;
+    const PropCall c = pc[h];
+    int64_t nb = 0;  // bound records in [0, dens_tot)
+    for (int32_t t = call_tile0[h]; t < call_tile0[h] + call_ntiles[h]; t++) nb += ts[t].n;
+    const int32_t end = c.dens_tot - 1;
+    if (end >= 0) {
+        const struct potential &r = srt[hfirst[h] + end];
+        if (!(r.pe < (double)r.ke)) nb--;
+    }
+    ScaleCall s;
+    s.end = end;
+    s.pad = 0;
+    s.nbins = 0;
+    s.ppbin = 0;
+    if (nb >= kMinScale) {
+        int64_t pp = (int64_t)ceil((double)nb / (double)kScaleBins);
+        if (pp < kMinPerBin) pp = nb / (nb / kMinPerBin);
+        s.ppbin = (int32_t)pp;
+        s.nbins = (int32_t)(nb / pp);
+    }
+    for (int k = 0; k <= kScaleBins; k++) s.bin_r[k] = 0.0f;
+    sc[h] = s;
+}
+
+// src/nfw.c:153-162: the outer edge of bin k is between the k * ppbin-th
+// bound record and the record after it
+__global__ __launch_bounds__(kWave) void k_scale_bins(const struct potential *__restrict__ srt,
+                                                      const ScanTile *__restrict__ tiles,
+                                                      int32_t ntiles,
+                                                      const int64_t *__restrict__ hfirst,
+                                                      const PropCount *__restrict__ pre,
+                                                      ScaleCall *__restrict__ sc) {
+    const int32_t t = blockIdx.x;
+    if (t >= ntiles) return;
+    const ScanTile tl = tiles[t];
+    const int32_t pp = sc[tl.call].ppbin, end = min(tl.off + kScanTile, sc[tl.call].end);
+    if (pp == 0 || tl.off >= end) return;
+    const int64_t first = hfirst[tl.call];
+    int32_t carry = pre[t].nb;
+    for (int32_t base = tl.off; base < end; base += kWave) {
+        const int32_t j = base + lane_id();
+        const bool v = j < end;
+        bool b = false;
+        if (v) b = !(srt[first + j].pe < (double)srt[first + j].ke);
+        const int32_t n = wave_scan_add(b ? 1 : 0) + carry;
+        carry = read_lane(n, kWave - 1);
+        if (b && n % pp == 0 && n / pp <= kScaleBins) {
+            const float r = (float)(1e3 * 0.5 * (sqrt((double)srt[first + j].r2) +
+                                                 sqrt((double)srt[first + j + 1].r2)));
+            sc[tl.call].bin_r[n / pp] = r;
+        }
+    }
+}
+
+__device__ double c_to_f(double c) {
+    const double cp1 = 1.0 + c;
+    return c * cp1 / (log1p(c) * cp1 - c);
+}
+
+// concentration with c_to_f(c) = f, src/nfw.c:18-33
+__device__ double f_to_c(double f) {
+    double c = f, tc = c_to_f(c);
+    for (int guard = 0; fabs((f - tc) / f) > 1e-7 && guard < 10000; guard++) {
+        const double tc2 = c_to_f(c + 0.1);
+        const double slope = (tc2 - tc) / 0.1;
+        const double nc = c + (f - tc) / slope;
+        if (nc < 0) c /= 2;
+        else c = nc;
+        tc = c_to_f(c);
+    }
+    return c;
+}
+
+// Klypin's rs from vmax, rvmax and the virial mass and radius, src/nfw.c:35-50
+__device__ float klypin_rs(float mvir, float rvir, float vmax, float rvmax, float scale) {
+    if (!(mvir != 0.0f) || !(rvir != 0.0f) || !(vmax != 0.0f))
+        return (float)(rvmax / kRmaxToRs);
+    float vm2 = (float)(vmax / kVmaxConst);
+    vm2 = vm2 * vm2 * scale;
+    const float f = (float)((rvir / 1.0e3) * vm2 / (mvir * kRsConstant));
+    if (f < 4.625) return (float)(rvmax / kRmaxToRs);
+    const float c = (float)f_to_c(f);
+    if (c <= 0) return (float)(rvmax / kRmaxToRs);
+    return rvir / c;
+}
+
+__device__ __forceinline__ float nfw_menc(float r, float rs) {
+    return (float)(log((double)((rs + r) / rs)) - (double)(r / (rs + r)));
+}
+
+__device__ float chi2_rs(float rs, const float *bin_r, const float *w, int nb) {
+    const float bin_mass = (float)(nfw_menc(bin_r[nb], rs) / (double)nb);
+    float last = 0, chi2 = 0;
+    for (int i = 0; i < nb; i++) {
+        const float menc = nfw_menc(bin_r[i + 1], rs);
+        const float dx = w[i] * ((menc - last) - bin_mass) / bin_mass;
+        chi2 += dx * dx;
+        last = menc;
+    }
+    return chi2;
+}
+
+// parabolic steps on chi2(rs), src/nfw.c:75-128
+__device__ float fit_rs(float rs, const float *bin_r, const float *w, int nb, float *out_chi2) {
+    if (!(rs > bin_r[0]) || !(rs < bin_r[nb])) rs = bin_r[(nb + 1) / 2];
+    const float initial = rs;
+    float chi2 = chi2_rs(rs, bin_r, w, nb);
+    float drs = (float)(rs / 10.0);
+    int iter = 0;
+    float last = 1e30f;
+    if (!(rs > 0)) return initial;
+    for (int guard = 0; fabs((double)(chi2 - last)) > 0.005 * chi2 && iter < kMinPerBin &&
+                        guard < 10000; guard++) {
+        last = chi2;
+        if (rs + drs > bin_r[nb]) drs = (float)(0.1 * (bin_r[nb] - rs));
+        if (rs - drs < bin_r[0]) drs = (float)(0.1 * (rs - bin_r[0]));
+        const float cr = chi2_rs(rs + drs, bin_r, w, nb);
+        const float cl = chi2_rs(rs - drs, bin_r, w, nb);
+        const float dx = (float)(0.5 * (cr - cl) / drs);
+        const float dx2 = (float)((cr + cl - 2.0 * chi2) / (drs * drs));
+        float move = 0;
+        if (dx2 != 0) move = -dx / dx2;
+        if (!(move != 0.0f)) return rs;
+        if (rs + move > 4 * rs) move = (float)(3.0 * rs);
+        if (rs + move < 0.25 * rs) move = (float)(-0.75 * rs);
+        float nrs = rs + move;
+        if (nrs > bin_r[nb]) nrs = (float)(rs + 0.8 * (bin_r[nb] - rs));
+        else if (nrs < bin_r[0]) nrs = (float)(rs + 0.8 * (bin_r[0] - rs));
+        float nchi2 = chi2_rs(nrs, bin_r, w, nb);
+        if (cr < nchi2) {
+            nchi2 = cr;
+            nrs = rs + drs;
+        }
+        if (cl < nchi2) {
+            nchi2 = cl;
+            nrs = rs - drs;
+        }
+        if (last < nchi2) {
+            drs *= 0.5f;
+            continue;
+        }
+        drs = (float)(fabs((double)(rs - nrs)) / 10.0);
+        rs = nrs;
+        chi2 = nchi2;
+        iter++;
+    }
+    *out_chi2 = chi2;
+    return rs;
+}
+
+// per call: the shape fields, the scale radius (src/nfw.c:130-173) and the
+// pseudo-evolution masses (src/properties.c:351-382)
+__global__ void k_props_late(int32_t nh, const struct potential *__restrict__ srt,
+                             const int64_t *__restrict__ hfirst,
+                             const int32_t *__restrict__ call_tile0,
+                             const PropCount *__restrict__ pre, const PropCall *__restrict__ pc,
+                             const ShapeState *__restrict__ st, const ScaleCall *__restrict__ sc,
+                             PropArgs pa, hp_halo_props *__restrict__ props) {
+    const int32_t h = blockIdx.x * blockDim.x + threadIdx.x;
+    if (h >= nh) return;
+    hp_halo_props o = props[h];
+    const PropCall c = pc[h];
+    const ShapeState &s0 = st[2 * h], &s1 = st[2 * h + 1];
+    o.b_to_a2 = s0.b_to_a;
+    o.c_to_a2 = s0.c_to_a;
+    o.b_to_a = s1.b_to_a;
+    o.c_to_a = s1.c_to_a;
+    for (int k = 0; k < 3; k++) {
+        o.A2[k] = s0.A[k];
+        o.A[k] = s1.A[k];
+    }
+    // the second calc_shape keeps the first's tensor until it writes its own
+    for (int k = 0; k < 6; k++) {
+        o.inertia_tensor2[k] = s0.inertia[k];
+        o.inertia_tensor[k] = s1.assigned ? s1.inertia[k] : s0.inertia[k];
+    }
+    const double m = (double)c.n0 * pa.particle_mass;
+    float rs = klypin_rs((float)m, o.r, o.vmax, o.rvmax, pa.scale_now);
+    o.klypin_rs = rs;
+    const ScaleCall &sk = sc[h];
+    if (sk.ppbin > 0) {
+        float w[kScaleBins];
+        for (int i = 0; i < kScaleBins; i++) w[i] = i < sk.nbins ? 1.0f : 0.0f;
+        for (int i = 0; i < sk.nbins; i++)
+            if (sk.bin_r[i] * 1e-3 < 3 * pa.force_res) w[i] = 0.1f;
+        float chi2 = 0.0f;
+        rs = fit_rs(rs, sk.bin_r, w, sk.nbins, &chi2);
+        o.chi2 = chi2;
+    }
+    o.rs = rs;
+    // src/properties.c:351-382: the bound records with r < r_pe_d come
+    // first (r2 is sorted), so their count is a prefix count
+    double r_pe_d = rs * 4.0;
+    if (r_pe_d < o.r / 5.0) r_pe_d = o.r / 5.0;
+    r_pe_d *= 1e-3;
+    const int64_t first = hfirst[h];
+    int32_t lo = 0, hi = c.kept;
+    while (lo < hi) {
+        const int32_t mid = (lo + hi) >> 1;
+        if (sqrt((double)srt[first + mid].r2) < r_pe_d) lo = mid + 1;
+        else hi = mid;
+    }
+    int64_t npd = 0;
+    if (lo > 0) {  // the tile holding record lo - 1: its prefix, then its records
+        const int32_t ti = (lo - 1) / kScanTile;
+        npd = pre[call_tile0[h] + ti].nb;
+        for (int32_t j = ti * kScanTile; j < lo; j++)
+            npd += !(srt[first + j].pe < (double)srt[first + j].ke) ? 1 : 0;
+    }
+    o.m_pe_d = (float)((double)npd * pa.particle_mass);
+    o.m_pe_b = (float)(pa.particle_mass * pow(c.pe_b, 2.0 / 3.0) /
+                       cbrt(4.0 * M_PI * pa.rvir_dens_z0 / 3.0));
     props[h] = o;
 }
 

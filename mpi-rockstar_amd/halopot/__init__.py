@@ -67,10 +67,12 @@ class Stats(C.Structure):
 
 
 class MassDefs(C.Structure):
-    _fields_ = [("thresh_dens", C.c_double * 5), ("rvir_dens", C.c_double)]
+    _fields_ = [("thresh_dens", C.c_double * 5), ("rvir_dens", C.c_double),
+                ("rvir_dens_z0", C.c_double)]
 
     def as_array(self) -> np.ndarray:
-        return np.array(list(self.thresh_dens) + [self.rvir_dens], np.float64)
+        """particle_thresh_dens[0..4], particle_rvir_dens, particle_rvir_dens_z0."""
+        return np.array(list(self.thresh_dens) + [self.rvir_dens, self.rvir_dens_z0], np.float64)
 
 
 class Tuning(C.Structure):
@@ -198,7 +200,7 @@ def mass_definitions(defs=None) -> MassDefs:
     """calc_mass_definition (src/groupies.c:124-142) with the current
     configuration globals: particle_thresh_dens[0..4] for the five mass
     definitions (None: the halo finder's defaults vir, 200b, 200c, 500c,
-    2500c) and particle_rvir_dens."""
+    2500c), particle_rvir_dens and particle_rvir_dens_z0."""
     arr = (C.c_char_p * 5)()
     for k in range(5):
         arr[k] = defs[k].encode() if defs is not None and defs[k] is not None else None

@@ -41,20 +41,17 @@ POTENTIAL_DTYPE = np.dtype(
 )
 
 # hp_halo_props (include/halopot.h): the halo property pass's results
-PROPS_FLOAT_FIELDS = ("m", "mgrav", "r", "vmax", "rvmax", "vrms", "Xoff", "Voff", "alt_m",
-                      "halfmass_radius", "J", "energy", "kin_to_pot", "spin", "bullock_spin",
-                      "rvir")
+_PROPS_LAYOUT = [("m", 1), ("mgrav", 1), ("r", 1), ("vmax", 1), ("rvmax", 1), ("vrms", 1),
+                 ("Xoff", 1), ("Voff", 1), ("alt_m", 4), ("halfmass_radius", 1), ("J", 3),
+                 ("energy", 1), ("kin_to_pot", 1), ("spin", 1), ("bullock_spin", 1),
+                 ("rvir", 1), ("rs", 1), ("klypin_rs", 1), ("chi2", 1), ("b_to_a", 1),
+                 ("c_to_a", 1), ("A", 3), ("b_to_a2", 1), ("c_to_a2", 1), ("A2", 3),
+                 ("inertia_tensor", 6), ("inertia_tensor2", 6), ("m_pe_b", 1), ("m_pe_d", 1)]
+PROPS_FLOAT_FIELDS = tuple(f for f, _ in _PROPS_LAYOUT)
 PROPS_INT_FIELDS = ("n_mdelta", "dens_tot", "n_vir", "n_half")
-PROPS_DTYPE = np.dtype(
-    {
-        "names": list(PROPS_FLOAT_FIELDS) + ["pad"] + list(PROPS_INT_FIELDS),
-        "formats": ["<f4"] * 8 + [("<f4", (4,)), "<f4", ("<f4", (3,))] + ["<f4"] * 5
-                   + ["<i4"] + ["<i8"] * 4,
-        "offsets": [0, 4, 8, 12, 16, 20, 24, 28, 32, 48, 52, 64, 68, 72, 76, 80, 84,
-                    88, 96, 104, 112],
-        "itemsize": 120,
-    }
-)
+PROPS_DTYPE = np.dtype([(f, "<f4") if n == 1 else (f, "<f4", (n,)) for f, n in _PROPS_LAYOUT]
+                       + [(f, "<i8") for f in PROPS_INT_FIELDS])
+assert PROPS_DTYPE.itemsize == 224
 
 # struct particle (src/particle.h:5-8): a FOF group's particle copies
 PARTICLE_DTYPE = np.dtype({"names": ["id", "pos"], "formats": ["<i8", ("<f4", (6,))],

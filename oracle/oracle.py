@@ -107,6 +107,7 @@ class Restatement:
                                               C.c_int64, C.POINTER(_Params), C.c_void_p, C.c_int,
                                               C.c_void_p, C.c_void_p]
         L.orc_set_never_accept.argtypes = [C.c_int]
+        L.orc_set_shape_reverse.argtypes = [C.c_int]
         L.orc_set_nmass_out.argtypes = [C.c_void_p]
         L.orc_hubble_scaling.restype = C.c_double
         L.orc_hubble_scaling.argtypes = [C.POINTER(_Params), C.c_double]
@@ -306,8 +307,9 @@ class ReferenceProps:
     FIELDS = ("m", "mgrav", "r", "vmax", "rvmax", "vrms", "Xoff", "Voff", "alt_m",
               "halfmass_radius", "J", "energy", "kin_to_pot", "spin", "bullock_spin",
               "rs", "klypin_rs", "b_to_a", "c_to_a", "A", "b_to_a2", "c_to_a2", "A2",
-              "m_pe_b", "m_pe_d")
+              "m_pe_b", "m_pe_d", "chi2", "inertia_tensor", "inertia_tensor2")
     DTYPE = np.dtype([(f, "<f4", (4,)) if f == "alt_m" else
+                      (f, "<f4", (6,)) if f.startswith("inertia") else
                       (f, "<f4", (3,)) if f in ("J", "A", "A2") else (f, "<f4") for f in FIELDS])
 
     def __init__(self, path: str | os.PathLike | None = None):
@@ -386,9 +388,10 @@ class ReferenceProps:
         return kept, bh[0]
 
     def mass_definition(self, prm, mass_defs) -> np.ndarray:
-        """particle_thresh_dens[0..4] and particle_rvir_dens."""
+        """particle_thresh_dens[0..4], particle_rvir_dens and
+        particle_rvir_dens_z0."""
         self.set_globals(prm, mass_defs)
-        out = np.zeros(6, np.float64)
+        out = np.zeros(7, np.float64)
         self.lib.ref_mass_definition(_p(out))
         return out
 

@@ -46,8 +46,8 @@
 extern struct potential *po; /* src/groupies.c:33 */
 #pragma omp threadprivate(po)
 extern double particle_thresh_dens[5]; /* src/groupies.c:56 */
-extern double particle_rvir_dens;
-#pragma omp threadprivate(particle_thresh_dens, particle_rvir_dens)
+extern double particle_rvir_dens, particle_rvir_dens_z0;
+#pragma omp threadprivate(particle_thresh_dens, particle_rvir_dens, particle_rvir_dens_z0)
 
 extern struct particle *copies; /* src/groupies.c:30 */
 #pragma omp threadprivate(copies)
@@ -65,19 +65,22 @@ typedef struct ref_props {
     float J[3], energy, kin_to_pot, spin, bullock_spin;
     float rs, klypin_rs, b_to_a, c_to_a, A[3], b_to_a2, c_to_a2, A2[3];
     float m_pe_b, m_pe_d;
+    float chi2, inertia_tensor[6], inertia_tensor2[6];
 } ref_props;
 
-void ref_mass_definition(double out[6]) {
+void ref_mass_definition(double out[7]) {
     calc_mass_definition();
     for (int k = 0; k < 5; k++) out[k] = particle_thresh_dens[k];
     out[5] = particle_rvir_dens;
+    out[6] = particle_rvir_dens_z0;
 }
 
 /* thresh: particle_thresh_dens[0..4] then particle_rvir_dens, as
    ref_mass_definition returns them (the halo finder computes them once per
-   snapshot, src/groupies.c:124) */
+   snapshot, src/groupies.c:124); particle_rvir_dens_z0 comes from this
+   thread's own calc_mass_definition */
 void ref_halo_props(struct potential *recs, int64_t total_p, const float pos[6],
-                    const double thresh[6], ref_props *out) {
+                    const double thresh[7], ref_props *out) {
     struct halo h;
     memset(&h, 0, sizeof h);
     memcpy(h.pos, pos, sizeof h.pos);
@@ -117,11 +120,14 @@ void ref_halo_props(struct potential *recs, int64_t total_p, const float pos[6],
     memcpy(o.A2, h.A2, sizeof o.A2);
     o.m_pe_b = h.m_pe_b;
     o.m_pe_d = h.m_pe_d;
+    o.chi2 = h.chi2;
+    memcpy(o.inertia_tensor, h.inertia_tensor, sizeof o.inertia_tensor);
+    memcpy(o.inertia_tensor2, h.inertia_tensor2, sizeof o.inertia_tensor2);
     *out = o;
 }
 
 void ref_halo_props_batch(struct potential *recs, const int64_t *first, const int64_t *kept,
-                          const float *pos6, int64_t nhalos, const double thresh[6],
+                          const float *pos6, int64_t nhalos, const double thresh[7],
                           ref_props *out, int nthreads) {
 #ifdef _OPENMP
     if (nthreads > 0) omp_set_num_threads(nthreads);

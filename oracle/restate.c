@@ -29,10 +29,11 @@
  *                                           (_reset_potentials,
  *                                           calc_particle_radii)
  *   - halo property pass ................. src/properties.c:127-137 (angular
- *                                           momentum), 328-349 (energy),
- *                                           384-510 (without calc_shape,
- *                                           calc_scale_radius and the
- *                                           pseudo-evolution masses)
+ *                                           momentum), 216-326 (calc_shape),
+ *                                           328-349 (energy), 351-382
+ *                                           (pseudo-evolution masses),
+ *                                           384-510; src/nfw.c:13-173
+ *                                           (scale radius); src/jacobi.c:8-135
  *
  * Deliberate differences from the reference, none of which changes a node's
  * membership, a walk decision or a value beyond f64 summation order:
@@ -786,11 +787,357 @@ typedef struct orc_props {
     float   J[3];
     float   energy, kin_to_pot, spin, bullock_spin;
     float   rvir;
-    int32_t pad;
+    float   rs, klypin_rs, chi2;
+    float   b_to_a, c_to_a, A[3], b_to_a2, c_to_a2, A2[3];
+    float   inertia_tensor[6], inertia_tensor2[6];
+    float   m_pe_b, m_pe_d;
     int64_t n_mdelta, dens_tot, n_vir, n_half;
 } orc_props;
 
-#define ORC_VMAX_CONST 6.55833746e-5 /* src/universal_constants.h:6 */
+#define ORC_VMAX_CONST       6.55833746e-5 /* src/universal_constants.h:6 */
+#define ORC_RMAX_TO_RS       2.1626        /* src/universal_constants.h:7 */
+#define ORC_RS_CONSTANT      0.216216595   /* src/universal_constants.h:8 */
+#define ORC_SHAPE_ITERATIONS 10            /* src/config.template.h:90 */
+#define ORC_WEIGHTED_SHAPES  1             /* src/config.template.h:91 */
+
+/* -------------------------------------------------- shape (calc_shape) */
+
+/* The cyclic Jacobi eigenvalue iteration of src/jacobi.c:8-135: rotate away
+   the largest off-diagonal element (rows cached by their largest column)
+   until no eigenvalue changes by more than 1e-6 of itself; the rotation's
+   t and s in long double as there.  m is left symmetric; orth's rows are
+   the eigenvectors. */
+static void orc_jacobi(double m[3][3], double eig[3], double orth[3][3]) {
+    const int n = 3;
+    int max_col[3], changed[3], state = 0;
+    double c, s, t, u, a;
+    for (int i = 0; i < n; i++) {
+        double mx = 0;
+        max_col[i] = i + 1;
+        for (int j = i + 1; j < n; j++)
+            if (fabs(m[i][j]) > mx) {
+                max_col[i] = j;
+                mx = fabs(m[i][j]);
+            }
+    }
+    for (int i = 0; i < n; i++)
+        for (int j = 0; j < n; j++) orth[i][j] = (i == j) ? 1 : 0;
+    for (int i = 0; i < n; i++) {
+        eig[i] = m[i][i];
+        changed[i] = 1;
+        int j;
+        for (j = 0; j < n; j++)
+            if (j != i && m[i][j]) break;
+        if (j == n) {
+            state--;
+            changed[i] = 0;
+        }
+    }
+    state += n;
+    while (state) {
+        double mx = fabs(m[0][max_col[0]]);
+        int k = 0;
+        for (int i = 1; i < n - 1; i++)
+            if (fabs(m[i][max_col[i]]) > mx) {
+                mx = fabs(m[i][max_col[i]]);
+                k = i;
+            }
+        int l = max_col[k];
+        double v = m[k][l];
+        if (v == 0) break;
+        a = (eig[l] - eig[k]) * 0.5;
+        t = fabsl(a) + sqrtl(v * v + a * a);
+        s = sqrtl(v * v + t * t);
+        c = t / s;
+        s = v / s;
+        t = v * v / t;
+        if (a < 0) {
+            s = -s;
+            t = -t;
+        }
+        const int xs[2] = {k, l};
+        const double ys[2] = {-t, t};
+        for (int q = 0; q < 2; q++) {
+            const int x = xs[q];
+            const double y = ys[q];
+            eig[x] += y;
+            if (changed[x] && (fabs(y) < 1e-6 * fabs(eig[x]))) {
+                changed[x] = 0;
+                state--;
+            } else if (!changed[x] && (fabs(y) > 1e-6 * fabs(eig[x]))) {
+                changed[x] = 1;
+                state++;
+            }
+        }
+        m[k][l] = 0;
+#define ORC_ROT(M, w, x, y, z, r)                                                   \
+    do {                                                                            \
+        t = M[w][x];                                                                \
+        u = M[y][z];                                                                \
+        M[w][x] = t * c - s * u;                                                    \
+        M[y][z] = s * t + c * u;                                                    \
+        if (r) {                                                                    \
+            if (fabs(M[w][x]) > fabs(M[w][max_col[w]])) max_col[w] = x;             \
+            if (y < 3 && fabs(M[y][z]) > fabs(M[y][max_col[y]])) max_col[y] = z;    \
+        }                                                                           \
+    } while (0)
+        for (int i = 0; i < k; i++) ORC_ROT(m, i, k, i, l, 1);
+        for (int i = k + 1; i < l; i++) ORC_ROT(m, k, i, i, l, 1);
+        for (int i = l + 1; i < n; i++) ORC_ROT(m, k, i, l, i, 1);
+        for (int i = 0; i < n; i++) ORC_ROT(orth, k, i, l, i, 0);
+#undef ORC_ROT
+    }
+    for (int k = 0; k < n - 1; k++)
+        for (int l = k + 1; l < n; l++) m[k][l] = m[l][k];
+}
+
+/* Test knob: walk the records of calc_shape's sums last to first.  Used to
+   show how much of the shape of a window of few records is set by the
+   summation order alone (the reference's own result moves with it). */
+static int orc_shape_reverse = 0;
+void orc_set_shape_reverse(int on) { orc_shape_reverse = on; }
+
+typedef struct orc_shape { /* the struct halo fields calc_shape reads and writes */
+    float r;               /* h->r, kpc/h */
+    float b_to_a, c_to_a, A[3];
+    float inertia[6];      /* kept from the previous call when this one stops early */
+} orc_shape;
+
+/* src/properties.c:216-326 over records [0, n) (bound: !(pe < ke)) */
+static void orc_calc_shape(orc_shape *h, const struct potential *po, int64_t n,
+                           const float cen[3], const orc_params *p) {
+    int64_t iter = ORC_SHAPE_ITERATIONS, analyze_p = 0;
+    float b_to_a, c_to_a, min_r = (float)(p->force_res * p->force_res);
+    double mass_t[3][3], orth[3][3], eig[3] = {0, 0, 0}, r = 0, dr, dr2, weight = 0;
+    h->b_to_a = h->c_to_a = 0;
+    memset(h->A, 0, sizeof h->A);
+    if (!(h->r > 0)) return;
+    min_r *= 1e6 / (h->r * h->r);
+    for (int64_t j = 0; j < n; j++)
+        if (!(po[j].pe < po[j].ke)) analyze_p++;
+    if (analyze_p < 3 || !(h->r > 0)) return;
+    if (analyze_p < iter) iter = analyze_p;
+    for (int i = 0; i < 3; i++) {
+        memset(orth[i], 0, sizeof(double) * 3);
+        orth[i][i] = 1;
+        eig[i] = (h->r * h->r) * 1e-6;
+    }
+    for (int64_t it = 0; it < iter; it++) {
+        for (int k = 0; k < 3; k++) memset(mass_t[k], 0, sizeof(double) * 3);
+        weight = 0;
+        for (int64_t jj = 0; jj < n; jj++) {
+            const int64_t j = orc_shape_reverse ? n - 1 - jj : jj;
+            if (po[j].pe < po[j].ke) continue;
+            r = 0;
+            for (int k = 0; k < 3; k++) {
+                dr = 0;
+                for (int l = 0; l < 3; l++) dr += orth[k][l] * (po[j].pos[l] - cen[l]);
+                r += dr * dr / eig[k];
+            }
+            if (r < min_r) r = min_r;
+            if (!(r > 0 && r <= 1)) continue;
+            double tw = ORC_WEIGHTED_SHAPES ? 1.0 / r : 1.0;
+            weight += tw;
+            for (int k = 0; k < 3; k++) {
+                dr = po[j].pos[k] - cen[k];
+                mass_t[k][k] += dr * dr * tw;
+                for (int l = 0; l < k; l++) {
+                    dr2 = po[j].pos[l] - cen[l];
+                    mass_t[k][l] += dr2 * dr * tw;
+                    mass_t[l][k] = mass_t[k][l];
+                }
+            }
+        }
+        if (!weight) return;
+        for (int k = 0; k < 3; k++)
+            for (int l = 0; l < 3; l++) mass_t[k][l] /= (double)weight;
+        double mt2[3][3];
+        memcpy(mt2, mass_t, sizeof mt2);
+        orc_jacobi(mass_t, eig, orth);
+        int a = 0, b = 1, c = 2;
+        if (eig[1] > eig[0]) {
+            b = 0;
+            a = 1;
+        }
+        if (eig[2] > eig[b]) {
+            c = b;
+            b = 2;
+        }
+        if (eig[b] > eig[a]) {
+            int q = a;
+            a = b;
+            b = q;
+        }
+        if (!eig[a] || !eig[b] || !eig[c]) return;
+        b_to_a = sqrt(eig[b] / eig[a]);
+        c_to_a = sqrt(eig[c] / eig[a]);
+        if ((fabs(b_to_a - h->b_to_a) < 0.01 * h->b_to_a) &&
+            (fabs(c_to_a - h->c_to_a) < 0.01 * h->c_to_a))
+            return;
+        h->b_to_a = (b_to_a > 0) ? b_to_a : 0;
+        h->c_to_a = (c_to_a > 0) ? c_to_a : 0;
+        r = sqrt(eig[a]);
+        for (int k = 0; k < 3; k++) {
+            h->A[k] = 1e3 * r * orth[a][k];
+            eig[k] *= (h->r * h->r * 1e-6) / (r * r);
+        }
+        h->inertia[0] = mt2[0][0];
+        h->inertia[1] = mt2[1][1];
+        h->inertia[2] = mt2[2][2];
+        h->inertia[3] = mt2[0][1];
+        h->inertia[4] = mt2[1][2];
+        h->inertia[5] = mt2[2][0];
+    }
+}
+
+/* ------------------------------------- NFW scale radius (src/nfw.c:13-173) */
+
+static double orc_c_to_f(double c) {
+    double cp1 = 1.0 + c;
+    return (c * cp1 / (log1p(c) * cp1 - c));
+}
+
+static double orc_f_to_c(double f) { /* Newton-like solve of c_to_f(c) = f */
+    double c = f, tc = orc_c_to_f(c), nc;
+    while (fabs((f - tc) / f) > 1e-7) {
+        double tc2 = orc_c_to_f(c + 0.1);
+        double slope = (tc2 - tc) / 0.1;
+        nc = c + (f - tc) / slope;
+        if (nc < 0) c /= 2;
+        else c = nc;
+        tc = orc_c_to_f(c);
+    }
+    return c;
+}
+
+/* Klypin's estimate from vmax, rvmax and the virial mass and radius */
+static float orc_klypin_rs(float mvir, float rvir, float vmax, float rvmax, float scale) {
+    float f, c, vm2;
+    if (!mvir || !rvir || !vmax) return (rvmax / ORC_RMAX_TO_RS);
+    vm2 = vmax / ORC_VMAX_CONST;
+    vm2 = vm2 * vm2 * scale;
+    f = (rvir / 1.0e3) * vm2 / (mvir * ORC_RS_CONSTANT);
+    if (f < 4.625) return (rvmax / ORC_RMAX_TO_RS);
+    c = orc_f_to_c(f);
+    if (c <= 0) return (rvmax / ORC_RMAX_TO_RS);
+    return (rvir / c);
+}
+
+static float orc_nfw_menc(float r, float rs) { return (log((rs + r) / rs) - r / (rs + r)); }
+
+static float orc_chi2(float rs, const float *bin_r, const float *w, int64_t nb) {
+    float bin_mass = orc_nfw_menc(bin_r[nb], rs) / ((double)nb);
+    float last = 0, chi2 = 0, dx;
+    for (int64_t i = 0; i < nb; i++) {
+        float menc = orc_nfw_menc(bin_r[i + 1], rs);
+        dx = w[i] * ((menc - last) - bin_mass) / bin_mass;
+        chi2 += dx * dx;
+        last = menc;
+    }
+    return chi2;
+}
+
+/* the minimisation of src/nfw.c:75-128: parabolic steps on chi2(rs) */
+static float orc_fit_rs(float rs, const float *bin_r, const float *w, int64_t nb, float *out_chi2) {
+    if (!(rs > bin_r[0]) || !(rs < bin_r[nb])) rs = bin_r[(nb + 1) / 2];
+    float initial = rs;
+    float chi2 = orc_chi2(rs, bin_r, w, nb);
+    float drs = rs / 10.0;
+    int64_t iter = 0;
+    float last = 1e30;
+    if (!(rs > 0)) return initial;
+    while (fabs(chi2 - last) > 0.005 * chi2 && iter < 15) {
+        last = chi2;
+        if (rs + drs > bin_r[nb]) drs = 0.1 * (bin_r[nb] - rs);
+        if (rs - drs < bin_r[0]) drs = 0.1 * (rs - bin_r[0]);
+        float cr = orc_chi2(rs + drs, bin_r, w, nb);
+        float cl = orc_chi2(rs - drs, bin_r, w, nb);
+        float dx = 0.5 * (cr - cl) / drs;
+        float dx2 = (cr + cl - 2.0 * chi2) / (drs * drs);
+        float move = 0;
+        if (dx2 != 0) move = -dx / dx2;
+        if (!move) return rs;
+        if (rs + move > 4 * rs) move = 3.0 * rs;
+        if (rs + move < 0.25 * rs) move = -0.75 * rs;
+        float nrs = rs + move;
+        if (nrs > bin_r[nb]) nrs = rs + 0.8 * (bin_r[nb] - rs);
+        else if (nrs < bin_r[0]) nrs = rs + 0.8 * (bin_r[0] - rs);
+        float nchi2 = orc_chi2(nrs, bin_r, w, nb);
+        if (cr < nchi2) {
+            nchi2 = cr;
+            nrs = rs + drs;
+        }
+        if (cl < nchi2) {
+            nchi2 = cl;
+            nrs = rs - drs;
+        }
+        if (last < nchi2) {
+            drs *= 0.5;
+            continue;
+        }
+        drs = fabs(rs - nrs) / 10.0;
+        rs = nrs;
+        chi2 = nchi2;
+        iter++;
+    }
+    *out_chi2 = chi2;
+    return rs;
+}
+
+/* src/nfw.c:130-173 over records [0, n): Klypin rs, then the fit to
+   equal-count radial bins (50 at most, 15 records at least) */
+static void orc_scale_radius(orc_props *out, float mvir, float rvir, float vmax, float rvmax,
+                             float scale, const struct potential *po, int64_t n,
+                             const orc_params *p) {
+    float bin_r[51] = {0}, w[50] = {0};
+    int64_t nb = 0, analyze_p = 0, ppbin;
+    float rs = out->klypin_rs = orc_klypin_rs(mvir, rvir, vmax, rvmax, scale);
+    for (int64_t i = 0; i < n - 1; i++)
+        if (!(po[i].pe < po[i].ke)) analyze_p++;
+    if (analyze_p < 100) {
+        out->rs = rs;
+        return;
+    }
+    ppbin = ceil(((double)analyze_p / (double)50));
+    if (ppbin < 15) ppbin = (analyze_p) / ((int64_t)(analyze_p / 15));
+    bin_r[0] = 0;
+    for (int64_t i = 0, j = 0; i < n - 1; i++) {
+        if (po[i].pe < po[i].ke) continue;
+        if (++j == ppbin) {
+            nb++;
+            bin_r[nb] = 1e3 * 0.5 * (sqrt(po[i].r2) + sqrt(po[i + 1].r2));
+            w[nb - 1] = 1;
+            j = 0;
+        }
+    }
+    for (int64_t i = 0; i < nb; i++)
+        if (bin_r[i] * 1e-3 < 3 * p->force_res) w[i] = 0.1;
+    float chi2 = 0.0;
+    out->rs = orc_fit_rs(rs, bin_r, w, nb, &chi2);
+    out->chi2 = chi2;
+}
+
+/* src/properties.c:351-382 over the bound records of [0, n) */
+static void orc_pseudo_evolution(orc_props *out, const struct potential *po, int64_t n,
+                                 float rs, float hr, double rvir_dens_z0, const orc_params *p) {
+    int64_t num_part = 0, num_part_pe_d = 0;
+    double r, r32, max_pe_b = 0;
+    double r_pe_d = rs * 4.0;
+    if (r_pe_d < hr / 5.0) r_pe_d = hr / 5.0;
+    r_pe_d *= 1e-3;
+    for (int64_t j = 0; j < n; j++) {
+        if (po[j].pe < po[j].ke) continue;
+        num_part++;
+        r = sqrt(po[j].r2);
+        r32 = sqrt(r);
+        r32 = r32 * r32 * r32;
+        if ((double)(num_part * num_part) / r32 > max_pe_b) max_pe_b = (double)(num_part * num_part) / r32;
+        if (r < r_pe_d) num_part_pe_d = num_part;
+    }
+    out->m_pe_d = (float)(num_part_pe_d * p->particle_mass);
+    out->m_pe_b = (float)(p->particle_mass * pow(max_pe_b, 2.0 / 3.0) /
+                          cbrt(4.0 * M_PI * rvir_dens_z0 / 3.0));
+}
 
 /* src/properties.c:328-349 over records [0, n), highest index first */
 static float orc_total_energy(const struct potential *po, int64_t n,
@@ -813,11 +1160,12 @@ static float orc_total_energy(const struct potential *po, int64_t n,
 
 /* src/properties.c:384-510 with the defaults BOUND_PROPS = 1 and LIGHTCONE
    = 0, run for bound = 0 and then bound = 1 as src/properties.c:552-553
-   does; calc_shape, calc_scale_radius and _calc_pseudo_evolution_masses are
-   not restated.  thresh[0..4] = particle_thresh_dens, thresh[5] =
-   particle_rvir_dens.  po: the kept records sorted by r2 with pe and ke. */
+   does, calc_shape, calc_scale_radius and _calc_pseudo_evolution_masses
+   included.  thresh[0..4] = particle_thresh_dens, thresh[5] =
+   particle_rvir_dens, thresh[6] = particle_rvir_dens_z0.  po: the kept
+   records sorted by r2 with pe and ke. */
 void orc_halo_props(const struct potential *po, int64_t total_p, const float pos[6],
-                    const orc_params *p, const double thresh[6], orc_props *out) {
+                    const orc_params *p, const double thresh[7], orc_props *out) {
     memset(out, 0, sizeof *out);
     const double four_thirds_pi = 4.0 * M_PI / 3.0;
     const float scale_now = (float)p->scale_now;
@@ -891,10 +1239,26 @@ void orc_halo_props(const struct potential *po, int64_t total_p, const float pos
         out->vmax = (float)(ORC_VMAX_CONST * sqrt(vmax * (p->particle_mass / scale_now)));
         out->rvmax = (float)(rvmax * 1e3);
         out->halfmass_radius = total_p > 0 ? (float)(sqrt(po[num_part_half].r2) * 1e3) : 0.0f;
-        out->r = (float)(cbrt((3.0 / (4.0 * M_PI)) * part_mdelta / thresh[0]) * 1e3);
+        /* src/properties.c:476-492: shape within the 4th definition, kept
+           as *2, then within the main one */
+        orc_shape sh;
+        memset(&sh, 0, sizeof sh);
+        sh.r = (float)(cbrt((3.0 / (4.0 * M_PI)) * np_alt[2] / thresh[3]) * 1e3);
+        orc_calc_shape(&sh, po, np_alt[2], pos, p);
+        out->b_to_a2 = sh.b_to_a;
+        out->c_to_a2 = sh.c_to_a;
+        memcpy(out->A2, sh.A, sizeof out->A2);
+        memcpy(out->inertia_tensor2, sh.inertia, sizeof out->inertia_tensor2);
+        sh.r = out->r = (float)(cbrt((3.0 / (4.0 * M_PI)) * part_mdelta / thresh[0]) * 1e3);
+        orc_calc_shape(&sh, po, dens_tot, pos, p);
+        out->b_to_a = sh.b_to_a;
+        out->c_to_a = sh.c_to_a;
+        memcpy(out->A, sh.A, sizeof out->A);
+        memcpy(out->inertia_tensor, sh.inertia, sizeof out->inertia_tensor);
         double rvir = cbrt((3.0 / (4.0 * M_PI)) * np_vir / thresh[5]) * 1e3;
         double mvir = np_vir * p->particle_mass;
         out->rvir = (float)rvir;
+        orc_scale_radius(out, (float)m, out->r, out->vmax, out->rvmax, scale_now, po, dens_tot, p);
         for (int k = 0; k < 3; k++) out->J[k] = (float)(p->particle_mass * scale_now * L[k]);
         out->energy = orc_total_energy(po, dens_tot, p, &out->kin_to_pot);
         double Jh = p->particle_mass * scale_now * sqrt(L[0] * L[0] + L[1] * L[1] + L[2] * L[2]);
@@ -906,11 +1270,12 @@ void orc_halo_props(const struct potential *po, int64_t total_p, const float pos
         out->dens_tot = dens_tot;
         out->n_vir = np_vir;
         out->n_half = num_part_half;
+        orc_pseudo_evolution(out, po, total_p, out->rs, out->r, thresh[6], p);
     }
 }
 
 void orc_halo_props_batch(const struct potential *po, const orc_halo *halos, int64_t nhalos,
-                          const orc_params *p, const double thresh[6], orc_props *out,
+                          const orc_params *p, const double thresh[7], orc_props *out,
                           int nthreads) {
 #ifdef _OPENMP
     if (nthreads > 0) omp_set_num_threads(nthreads);
