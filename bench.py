@@ -205,8 +205,9 @@ def run_props(plan, cat, d_po, d_h, args):
     res = {"ms_per_pass": 1e3 * e / reps, "value": kept * reps / e,
            "unit": "kept records/s through the property pass", "kept_records": kept,
            "calls": cat.n_halos, "passes": reps,
-           "scope": "src/properties.c:384-510 for bound = 0 and 1, without calc_shape, "
-                    "calc_scale_radius and the pseudo-evolution masses"}
+           "scope": "src/properties.c:384-510 for bound = 0 and 1: masses, vmax, offsets, "
+                    "J, energy, spins, calc_shape (both windows), calc_scale_radius, "
+                    "pseudo-evolution masses"}
     log(f"halo property pass: {res['ms_per_pass']:.3f} ms/pass, {res['value']:.4g} kept records/s")
     if not args.no_cpu:
         sys.path.insert(0, str(ROOT / "oracle"))
@@ -241,8 +242,8 @@ def run_props(plan, cat, d_po, d_h, args):
         log(f"halo property pass on the CPU ({kind}, {nt} threads): "
             f"{res['cpu_baseline']['ms_per_pass']:.1f} ms/pass")
         if kind == "reference":
-            # like for like: the restatement covers exactly the GPU's fields
-            # (the reference's pass also runs calc_shape and calc_scale_radius)
+            # the restatement of the same pass (the oracle's C code, pinned
+            # bit for bit to the reference's)
             O = oracle.Restatement()
             buf = np.zeros(cat.n_halos, catalog.PROPS_DTYPE)
             passes, busy = 0, 0.0
@@ -253,7 +254,7 @@ def run_props(plan, cat, d_po, d_h, args):
                 passes += 1
             res["cpu_port_same_fields"] = {"ms_per_pass": 1e3 * busy / passes, "cores": nt,
                                            "kind": "port", "passes": passes}
-            log(f"halo property pass, same fields, CPU restatement: {1e3 * busy / passes:.1f} ms/pass")
+            log(f"halo property pass, CPU restatement: {1e3 * busy / passes:.1f} ms/pass")
     d_p.free()
     return res
 
