@@ -264,7 +264,8 @@ struct Engine {
     // unbind
     DevBuf keys0, keys1, vals0, vals1, hist, hsum, srt, callinfo, tsum, tcand, tile_i32;
     // halo properties
-    DevBuf pcnt, pscan, pcall, psums, shape_st, shape_part, shape_cd, scale_calls;
+    DevBuf pcnt, pscan, pcall, psums, shape_st, shape_part, shape_cd, scale_calls, r_pe, pe_cnt,
+        fit_list, fit_n;
     // record assembly
     DevBuf gsegs, gtiles, gcen;
     int64_t cap_nodes = 0, cap_small = 0, cap_group = 0, cap_big = 0, cap_tiles = 0, cap_fat = 0;
@@ -308,7 +309,7 @@ struct Engine {
         DevBuf *all[] = {&hfirst, &hcount, &root_tiles, &groups, &direct_blocks, &rec_call,
                          &scan_tiles, &call_tile0, &call_ntiles, &box, &bad, &badlist, &A, &B, &F,
                          &leaf_of, &nodes, &parent, &fat, &small0, &group0, &big, &tiles,
-                         &parts, &splitres, &dparts, &counters, &work, &depth_buf, &pcnt, &pscan, &pcall, &psums, &shape_st, &shape_part, &shape_cd, &scale_calls, &gsegs, &gtiles, &gcen, &keys0, &keys1, &vals0,
+                         &parts, &splitres, &dparts, &counters, &work, &depth_buf, &pcnt, &pscan, &pcall, &psums, &shape_st, &shape_part, &shape_cd, &scale_calls, &r_pe, &pe_cnt, &fit_list, &fit_n, &gsegs, &gtiles, &gcen, &keys0, &keys1, &vals0,
                          &vals1, &hist, &hsum, &srt, &callinfo, &tsum, &tcand, &tile_i32};
         if (stream) {
             (void)hipStreamSynchronize(stream);
@@ -823,6 +824,10 @@ struct Engine {
         HP_ENSURE(shape_part, sizeof(ShapePart) * 2 * (nst + 1));
         HP_ENSURE(shape_cd, sizeof(float4) * (T.nrec + 1));
         HP_ENSURE(scale_calls, sizeof(ScaleCall) * (H + 1));
+        HP_ENSURE(r_pe, sizeof(double) * (H + 1));
+        HP_ENSURE(pe_cnt, sizeof(int32_t) * (nst + 1));
+        HP_ENSURE(fit_list, sizeof(int32_t) * (H + 1));
+        HP_ENSURE(fit_n, sizeof(int32_t) * 2);
         PropArgs pa;
         const double four_thirds_pi = 4.0 * M_PI / 3.0;  // src/properties.c:393-398
         pa.dens_thresh = md.thresh_dens[0] * four_thirds_pi;
@@ -875,8 +880,8 @@ struct Engine {
         }
         for (int it = 0; it < kShapeIterations; it++) {
             if (nst > 0) {
-                k_shape_tiles<<<nst, kWave, 0, s>>>(tl, nst, hf, ss, shape_cd.as<float4>(),
-                                                    shape_part.as<ShapePart>());
+                k_shape_tiles<<<std::min(nst, 8192), kWave, 0, s>>>(
+                    tl, nst, hf, ss, shape_cd.as<float4>(), shape_part.as<ShapePart>());
                 HP_LAUNCHED();
             }
             k_shape_calls<<<blocks(2 * (int64_t)H, 128), 128, 0, s>>>(
@@ -884,18 +889,32 @@ struct Engine {
             HP_LAUNCHED();
         }
         // calc_scale_radius and the pseudo-evolution masses
+        k_fill_i32<<<1, 64, 0, s>>>(fit_n.as<int32_t>(), 0, 1);
+        HP_LAUNCHED();
         k_scale_init<<<blocks(H, 256), 256, 0, s>>>(H, d_po, hf, t0, tn, pcall.as<PropCall>(),
                                                     psums.as<PropSums>(),
-                                                    scale_calls.as<ScaleCall>());
+                                                    scale_calls.as<ScaleCall>(),
+                                                    fit_list.as<int32_t>(), fit_n.as<int32_t>());
         HP_LAUNCHED();
         if (nst > 0) {
             k_scale_bins<<<nst, kWave, 0, s>>>(d_po, tl, nst, hf, pcnt.as<PropCount>(),
                                                scale_calls.as<ScaleCall>());
             HP_LAUNCHED();
         }
-        k_props_late<<<blocks(H, 64), 64, 0, s>>>(H, d_po, hf, t0, pcnt.as<PropCount>(),
-                                                  pcall.as<PropCall>(), ss,
-                                                  scale_calls.as<ScaleCall>(), pa, d_props);
+        k_scale_fit<<<blocks(H, 64), 64, 0, s>>>(fit_list.as<int32_t>(), fit_n.as<int32_t>(),
+                                                 pcall.as<PropCall>(), d_props, pa,
+                                                 scale_calls.as<ScaleCall>());
+        HP_LAUNCHED();
+        k_props_late<<<blocks(H, 64), 64, 0, s>>>(H, pcall.as<PropCall>(), ss,
+                                                  scale_calls.as<ScaleCall>(), pa,
+                                                  r_pe.as<double>(), d_props);
+        HP_LAUNCHED();
+        if (nst > 0) {
+            k_pe_tiles<<<nst, kWave, 0, s>>>(d_po, tl, nst, d_halos, hf, hc, r_pe.as<double>(),
+                                             pe_cnt.as<int32_t>());
+            HP_LAUNCHED();
+        }
+        k_pe_calls<<<blocks(H, 256), 256, 0, s>>>(H, t0, tn, pe_cnt.as<int32_t>(), pa, d_props);
         HP_LAUNCHED();
         return HP_OK;
     }

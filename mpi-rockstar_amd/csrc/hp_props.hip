@@ -544,8 +544,7 @@ __global__ __launch_bounds__(kWave) void k_shape_tiles(const ScanTile *__restric
                                                        const ShapeState *__restrict__ st,
                                                        const float4 *__restrict__ cd,
                                                        ShapePart *__restrict__ part) {
-    const int32_t t = blockIdx.x;
-    if (t >= ntiles) return;
+    for (int32_t t = blockIdx.x; t < ntiles; t += gridDim.x) {
     const ScanTile tl = tiles[t];
     const int64_t first = hfirst[tl.call];
     for (int w = 0; w < 2; w++) {
@@ -554,6 +553,11 @@ __global__ __launch_bounds__(kWave) void k_shape_tiles(const ScanTile *__restric
         int32_t cnt = 0;
         if (!s.done) {
             const int32_t end = min(tl.off + kScanTile, s.n);
+            double o[3][3], ie[3];
+            for (int k = 0; k < 3; k++) {
+                ie[k] = 1.0 / s.eig[k];  // the reference divides per record; within 1 ulp
+                for (int l = 0; l < 3; l++) o[k][l] = s.orth[k][l];
+            }
             for (int32_t j = tl.off + lane_id(); j < end; j += kWave) {
                 const float4 q = cd[first + j];
                 if (q.w == 0.0f) continue;
@@ -562,8 +566,8 @@ __global__ __launch_bounds__(kWave) void k_shape_tiles(const ScanTile *__restric
                 double r = 0;
                 for (int k = 0; k < 3; k++) {
                     double dr = 0;
-                    for (int l = 0; l < 3; l++) dr += s.orth[k][l] * d[l];
-                    r += dr * dr / s.eig[k];
+                    for (int l = 0; l < 3; l++) dr += o[k][l] * d[l];
+                    r += dr * dr * ie[k];
                 }
                 if (r < (double)s.min_r) r = s.min_r;
                 if (!(r > 0 && r <= 1)) continue;
@@ -584,6 +588,7 @@ __global__ __launch_bounds__(kWave) void k_shape_tiles(const ScanTile *__restric
         pt.pad = 0;
         if (lane_id() == 0) part[2 * t + w] = pt;
     }
+    }
 }
 
 // |a| + sqrt(x), the sum and the root taken in extended precision and
@@ -601,81 +606,95 @@ __device__ __forceinline__ double add_sqrt_ext(double a_abs, double x) {
 // The cyclic Jacobi eigenvalue iteration of src/jacobi.c:8-135 (largest
 // off-diagonal element first, each row's largest column cached, stop when
 // no eigenvalue changes by more than 1e-6 of itself).  orth's rows are the
-// eigenvectors; m is left symmetric.
-__device__ void jacobi3(double m[3][3], double eig[3], double orth[3][3]) {
-    int max_col[3], changed[3], state = 0;
-    for (int i = 0; i < 3; i++) {
-        double mx = 0;
-        max_col[i] = i + 1;
-        for (int j = i + 1; j < 3; j++)
-            if (fabs(m[i][j]) > mx) {
-                max_col[i] = j;
-                mx = fabs(m[i][j]);
-            }
-    }
-    for (int i = 0; i < 3; i++)
-        for (int j = 0; j < 3; j++) orth[i][j] = i == j ? 1.0 : 0.0;
-    for (int i = 0; i < 3; i++) {
-        eig[i] = m[i][i];
-        changed[i] = 1;
-        bool off = false;
-        for (int j = 0; j < 3; j++)
-            if (j != i && m[i][j] != 0.0) off = true;
-        if (!off) {
-            state--;
-            changed[i] = 0;
+// eigenvectors; m is left symmetric.  With three rows the pivot (k, l) is
+// (0, 1), (0, 2) or (1, 2); each case is spelled out with constant indices
+// so that the matrices stay in registers.
+struct Jacobi3 {
+    double m[3][3], eig[3], orth[3][3];
+    int max_col[3], changed[3], state;
+    double c, s;
+
+    template <int W, int X, int Y, int Z, bool R>
+    __device__ __forceinline__ void rot(double (&M)[3][3]) {
+        const double tt = M[W][X], uu = M[Y][Z];
+        M[W][X] = tt * c - s * uu;
+        M[Y][Z] = s * tt + c * uu;
+        if (R) {
+            if (fabs(M[W][X]) > fabs(col(M, W))) max_col[W] = X;
+            if (Y < 3 && fabs(M[Y][Z]) > fabs(col(M, Y))) max_col[Y] = Z;
         }
     }
-    state += 3;
-    for (int guard = 0; state && guard < 1000; guard++) {
-        int k = 0;
-        double mx = fabs(m[0][max_col[0]]);
-        if (fabs(m[1][max_col[1]]) > mx) k = 1;
-        const int l = max_col[k];
-        const double v = m[k][l];
-        if (v == 0.0) break;
-        const double a = (eig[l] - eig[k]) * 0.5;
+    // M[w][max_col[w]] for the rows that hold a cached column (0 and 1)
+    __device__ __forceinline__ double col(const double (&M)[3][3], int w) const {
+        const int j = max_col[w];
+        return w == 0 ? (j == 1 ? M[0][1] : M[0][2]) : M[1][2];
+    }
+    __device__ __forceinline__ void update(int x, double y) {
+        double &e = x == 0 ? eig[0] : (x == 1 ? eig[1] : eig[2]);
+        int &ch = x == 0 ? changed[0] : (x == 1 ? changed[1] : changed[2]);
+        e += y;
+        if (ch && fabs(y) < 1e-6 * fabs(e)) {
+            ch = 0;
+            state--;
+        } else if (!ch && fabs(y) > 1e-6 * fabs(e)) {
+            ch = 1;
+            state++;
+        }
+    }
+    template <int K, int L>
+    __device__ __forceinline__ bool step() {
+        const double v = m[K][L];
+        if (v == 0.0) return false;
+        const double a = (eig[L] - eig[K]) * 0.5;
         double t = add_sqrt_ext(fabs(a), v * v + a * a);
-        double s = add_sqrt_ext(0.0, v * v + t * t);
-        const double c = t / s;
+        s = add_sqrt_ext(0.0, v * v + t * t);
+        c = t / s;
         s = v / s;
         t = v * v / t;
         if (a < 0) {
             s = -s;
             t = -t;
         }
-        const int xs[2] = {k, l};
-        const double ys[2] = {-t, t};
-        for (int q = 0; q < 2; q++) {
-            const int x = xs[q];
-            const double y = ys[q];
-            eig[x] += y;
-            if (changed[x] && fabs(y) < 1e-6 * fabs(eig[x])) {
-                changed[x] = 0;
-                state--;
-            } else if (!changed[x] && fabs(y) > 1e-6 * fabs(eig[x])) {
-                changed[x] = 1;
-                state++;
-            }
-        }
-        m[k][l] = 0.0;
-        auto rot = [&](double (*M)[3], int w, int x, int y, int z, bool r) {
-            const double tt = M[w][x], uu = M[y][z];
-            M[w][x] = tt * c - s * uu;
-            M[y][z] = s * tt + c * uu;
-            if (r) {
-                if (fabs(M[w][x]) > fabs(M[w][max_col[w]])) max_col[w] = x;
-                if (y < 3 && fabs(M[y][z]) > fabs(M[y][max_col[y]])) max_col[y] = z;
-            }
-        };
-        for (int i = 0; i < k; i++) rot(m, i, k, i, l, true);
-        for (int i = k + 1; i < l; i++) rot(m, k, i, i, l, true);
-        for (int i = l + 1; i < 3; i++) rot(m, k, i, l, i, true);
-        for (int i = 0; i < 3; i++) rot(orth, k, i, l, i, false);
+        update(K, -t);
+        update(L, t);
+        m[K][L] = 0.0;
+        // src/jacobi.c:113-124: rows before k, between k and l, after l
+        if (K == 1) rot<0, 1, 0, L, true>(m);
+        if (K == 0 && L == 2) rot<0, 1, 1, 2, true>(m);
+        if (K == 0 && L == 1) rot<0, 2, 1, 2, true>(m);
+        rot<K, 0, L, 0, false>(orth);
+        rot<K, 1, L, 1, false>(orth);
+        rot<K, 2, L, 2, false>(orth);
+        return true;
     }
-    for (int k = 0; k < 2; k++)
-        for (int l = k + 1; l < 3; l++) m[k][l] = m[l][k];
-}
+    __device__ void run() {
+        max_col[0] = fabs(m[0][2]) > fabs(m[0][1]) ? 2 : 1;  // ties keep the first column
+        max_col[1] = 2;
+        max_col[2] = 3;
+        for (int i = 0; i < 3; i++)
+            for (int j = 0; j < 3; j++) orth[i][j] = i == j ? 1.0 : 0.0;
+        state = 0;
+        for (int i = 0; i < 3; i++) {
+            eig[i] = m[i][i];
+            changed[i] = 1;
+        }
+        if (m[0][1] == 0.0 && m[0][2] == 0.0) { state--; changed[0] = 0; }
+        if (m[1][0] == 0.0 && m[1][2] == 0.0) { state--; changed[1] = 0; }
+        if (m[2][0] == 0.0 && m[2][1] == 0.0) { state--; changed[2] = 0; }
+        state += 3;
+        for (int guard = 0; state && guard < 1000; guard++) {
+            const bool row1 = fabs(m[1][2]) > fabs(col(m, 0));
+            bool go;
+            if (row1) go = step<1, 2>();
+            else if (max_col[0] == 1) go = step<0, 1>();
+            else go = step<0, 2>();
+            if (!go) break;
+        }
+        m[0][1] = m[1][0];
+        m[0][2] = m[2][0];
+        m[1][2] = m[2][1];
+    }
+};
 
 // src/properties.c:270-325 for each (call, window) still running
 __global__ void k_shape_calls(int32_t nh, int32_t it, const int32_t *__restrict__ call_tile0,
@@ -683,9 +702,9 @@ __global__ void k_shape_calls(int32_t nh, int32_t it, const int32_t *__restrict_
                               const ShapePart *__restrict__ part, ShapeState *__restrict__ st) {
     const int32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= 2 * nh) return;
-    const int32_t h = i >> 1, w = i & 1;
-    ShapeState s = st[i];
+    ShapeState &s = st[i];
     if (s.done) return;
+    const int32_t h = i >> 1, w = i & 1;
     double wt = 0.0, t6[6] = {0, 0, 0, 0, 0, 0};
     int64_t cnt = 0;
     for (int32_t t = call_tile0[h]; t < call_tile0[h] + call_ntiles[h]; t++) {
@@ -697,59 +716,69 @@ __global__ void k_shape_calls(int32_t nh, int32_t it, const int32_t *__restrict_
     if (s.limit < 0) {  // src/properties.c:228-236
         if (cnt < 3) {
             s.done = 1;
-            st[i] = s;
             return;
         }
         s.limit = (int32_t)min((int64_t)kShapeIterations, cnt);
     }
     if (it >= s.limit || !(wt != 0.0)) {
         s.done = 1;
-        st[i] = s;
         return;
     }
-    double m[3][3] = {{t6[0], t6[3], t6[4]}, {t6[3], t6[1], t6[5]}, {t6[4], t6[5], t6[2]}};
-    for (int k = 0; k < 3; k++)
-        for (int l = 0; l < 3; l++) m[k][l] /= wt;
-    const float inert[6] = {(float)m[0][0], (float)m[1][1], (float)m[2][2],
-                            (float)m[0][1], (float)m[1][2], (float)m[2][0]};
-    jacobi3(m, s.eig, s.orth);
+    Jacobi3 jb;
+    jb.m[0][0] = t6[0] / wt;
+    jb.m[1][1] = t6[1] / wt;
+    jb.m[2][2] = t6[2] / wt;
+    jb.m[0][1] = jb.m[1][0] = t6[3] / wt;
+    jb.m[0][2] = jb.m[2][0] = t6[4] / wt;
+    jb.m[1][2] = jb.m[2][1] = t6[5] / wt;
+    const float inert[6] = {(float)jb.m[0][0], (float)jb.m[1][1], (float)jb.m[2][2],
+                            (float)jb.m[0][1], (float)jb.m[1][2], (float)jb.m[2][0]};
+    jb.run();
+    const double e0 = jb.eig[0], e1 = jb.eig[1], e2 = jb.eig[2];
+    auto E = [&](int q) { return q == 0 ? e0 : (q == 1 ? e1 : e2); };
     int a = 0, b = 1, c = 2;
-    if (s.eig[1] > s.eig[0]) {
+    if (e1 > e0) {
         b = 0;
         a = 1;
     }
-    if (s.eig[2] > s.eig[b]) {
+    if (e2 > E(b)) {
         c = b;
         b = 2;
     }
-    if (s.eig[b] > s.eig[a]) {
+    if (E(b) > E(a)) {
         const int q = a;
         a = b;
         b = q;
     }
-    if (!(s.eig[a] != 0.0) || !(s.eig[b] != 0.0) || !(s.eig[c] != 0.0)) {
+    const double ea = E(a), eb = E(b), ec = E(c);
+    for (int k = 0; k < 3; k++) {
+        s.eig[k] = jb.eig[k];
+        for (int l = 0; l < 3; l++) s.orth[k][l] = jb.orth[k][l];
+    }
+    if (!(ea != 0.0) || !(eb != 0.0) || !(ec != 0.0)) {
         s.done = 1;
-        st[i] = s;
         return;
     }
-    const float b_to_a = (float)sqrt(s.eig[b] / s.eig[a]);
-    const float c_to_a = (float)sqrt(s.eig[c] / s.eig[a]);
+    const float b_to_a = (float)sqrt(eb / ea);
+    const float c_to_a = (float)sqrt(ec / ea);
     if ((fabs((double)(b_to_a - s.b_to_a)) < 0.01 * (double)s.b_to_a) &&
         (fabs((double)(c_to_a - s.c_to_a)) < 0.01 * (double)s.c_to_a)) {
         s.done = 1;
-        st[i] = s;
         return;
     }
     s.b_to_a = b_to_a > 0 ? b_to_a : 0.0f;
     s.c_to_a = c_to_a > 0 ? c_to_a : 0.0f;
-    const double r = sqrt(s.eig[a]);
+    const double r = sqrt(ea);
+    const double oa[3] = {a == 0 ? jb.orth[0][0] : (a == 1 ? jb.orth[1][0] : jb.orth[2][0]),
+                          a == 0 ? jb.orth[0][1] : (a == 1 ? jb.orth[1][1] : jb.orth[2][1]),
+                          a == 0 ? jb.orth[0][2] : (a == 1 ? jb.orth[1][2] : jb.orth[2][2])};
+    const double sc = ((double)(s.r * s.r) * 1e-6) / (r * r);
     for (int k = 0; k < 3; k++) {
-        s.A[k] = (float)(1e3 * r * s.orth[a][k]);
-        s.eig[k] *= ((double)(s.r * s.r) * 1e-6) / (r * r);
+        s.A[k] = (float)(1e3 * r * oa[k]);
+        s.eig[k] = jb.eig[k] * sc;
     }
     for (int k = 0; k < 6; k++) s.inertia[k] = inert[k];
     s.assigned = 1;
-    st[i] = s;
 }
 
 // ------------------------------------------- scale radius, pseudo-evolution
@@ -766,6 +795,7 @@ struct ScaleCall {
     int32_t end;       // bins come from records [0, end) = [0, dens_tot - 1)
     int32_t pad;
     float bin_r[kScaleBins + 1];
+    float klypin, rs, chi2;  // results for the fitted calls
 };
 
 // src/nfw.c:134-152: the bound records of [0, dens_tot - 1) and the bin size
@@ -774,7 +804,8 @@ __global__ void k_scale_init(int32_t nh, const struct potential *__restrict__ sr
                              const int32_t *__restrict__ call_tile0,
                              const int32_t *__restrict__ call_ntiles,
                              const PropCall *__restrict__ pc, const PropSums *__restrict__ ts,
-                             ScaleCall *__restrict__ sc) {
+                             ScaleCall *__restrict__ sc, int32_t *__restrict__ fit_list,
+                             int32_t *__restrict__ fit_n) {
     const int32_t h = blockIdx.x * blockDim.x + threadIdx.x;
     if (h >= nh) return;
     const PropCall c = pc[h];
@@ -795,8 +826,10 @@ __global__ void k_scale_init(int32_t nh, const struct potential *__restrict__ sr
         if (pp < kMinPerBin) pp = nb / (nb / kMinPerBin);
         s.ppbin = (int32_t)pp;
         s.nbins = (int32_t)(nb / pp);
+        fit_list[atomicAdd(fit_n, 1)] = h;  // the fits run one call per lane, together
     }
     for (int k = 0; k <= kScaleBins; k++) s.bin_r[k] = 0.0f;
+    s.klypin = s.rs = s.chi2 = 0.0f;
     sc[h] = s;
 }
 
@@ -866,12 +899,17 @@ __device__ __forceinline__ float nfw_menc(float r, float rs) {
     return (float)(log((double)((rs + r) / rs)) - (double)(r / (rs + r)));
 }
 
-__device__ float chi2_rs(float rs, const float *bin_r, const float *w, int nb) {
+// bin weights, src/nfw.c:164-166: 0.1 for bins starting inside 3 FORCE_RES
+__device__ __forceinline__ float bin_weight(const float *bin_r, int i, double force_res) {
+    return bin_r[i] * 1e-3 < 3 * force_res ? 0.1f : 1.0f;
+}
+
+__device__ float chi2_rs(float rs, const float *bin_r, double force_res, int nb) {
     const float bin_mass = (float)(nfw_menc(bin_r[nb], rs) / (double)nb);
     float last = 0, chi2 = 0;
     for (int i = 0; i < nb; i++) {
         const float menc = nfw_menc(bin_r[i + 1], rs);
-        const float dx = w[i] * ((menc - last) - bin_mass) / bin_mass;
+        const float dx = bin_weight(bin_r, i, force_res) * ((menc - last) - bin_mass) / bin_mass;
         chi2 += dx * dx;
         last = menc;
     }
@@ -879,10 +917,11 @@ __device__ float chi2_rs(float rs, const float *bin_r, const float *w, int nb) {
 }
 
 // parabolic steps on chi2(rs), src/nfw.c:75-128
-__device__ float fit_rs(float rs, const float *bin_r, const float *w, int nb, float *out_chi2) {
+__device__ float fit_rs(float rs, const float *bin_r, double force_res, int nb,
+                       float *out_chi2) {
     if (!(rs > bin_r[0]) || !(rs < bin_r[nb])) rs = bin_r[(nb + 1) / 2];
     const float initial = rs;
-    float chi2 = chi2_rs(rs, bin_r, w, nb);
+    float chi2 = chi2_rs(rs, bin_r, force_res, nb);
     float drs = (float)(rs / 10.0);
     int iter = 0;
     float last = 1e30f;
@@ -892,8 +931,8 @@ __device__ float fit_rs(float rs, const float *bin_r, const float *w, int nb, fl
         last = chi2;
         if (rs + drs > bin_r[nb]) drs = (float)(0.1 * (bin_r[nb] - rs));
         if (rs - drs < bin_r[0]) drs = (float)(0.1 * (rs - bin_r[0]));
-        const float cr = chi2_rs(rs + drs, bin_r, w, nb);
-        const float cl = chi2_rs(rs - drs, bin_r, w, nb);
+        const float cr = chi2_rs(rs + drs, bin_r, force_res, nb);
+        const float cl = chi2_rs(rs - drs, bin_r, force_res, nb);
         const float dx = (float)(0.5 * (cr - cl) / drs);
         const float dx2 = (float)((cr + cl - 2.0 * chi2) / (drs * drs));
         float move = 0;
@@ -904,7 +943,7 @@ __device__ float fit_rs(float rs, const float *bin_r, const float *w, int nb, fl
         float nrs = rs + move;
         if (nrs > bin_r[nb]) nrs = (float)(rs + 0.8 * (bin_r[nb] - rs));
         else if (nrs < bin_r[0]) nrs = (float)(rs + 0.8 * (bin_r[0] - rs));
-        float nchi2 = chi2_rs(nrs, bin_r, w, nb);
+        float nchi2 = chi2_rs(nrs, bin_r, force_res, nb);
         if (cr < nchi2) {
             nchi2 = cr;
             nrs = rs + drs;
@@ -926,14 +965,31 @@ __device__ float fit_rs(float rs, const float *bin_r, const float *w, int nb, fl
     return rs;
 }
 
-// per call: the shape fields, the scale radius (src/nfw.c:130-173) and the
-// pseudo-evolution masses (src/properties.c:351-382)
-__global__ void k_props_late(int32_t nh, const struct potential *__restrict__ srt,
-                             const int64_t *__restrict__ hfirst,
-                             const int32_t *__restrict__ call_tile0,
-                             const PropCount *__restrict__ pre, const PropCall *__restrict__ pc,
+// the calls with a fit, one per lane: Klypin's estimate, then the fit
+__global__ void k_scale_fit(const int32_t *__restrict__ fit_list,
+                            const int32_t *__restrict__ fit_n, const PropCall *__restrict__ pc,
+                            const hp_halo_props *__restrict__ props, PropArgs pa,
+                            ScaleCall *__restrict__ sc) {
+    const int32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= *fit_n) return;
+    const int32_t h = fit_list[i];
+    const hp_halo_props &o = props[h];
+    const double m = (double)pc[h].n0 * pa.particle_mass;
+    ScaleCall &sk = sc[h];
+    const float rs0 = klypin_rs((float)m, o.r, o.vmax, o.rvmax, pa.scale_now);
+    float chi2 = 0.0f;
+    const float rs = fit_rs(rs0, sk.bin_r, pa.force_res, sk.nbins, &chi2);
+    sk.klypin = rs0;
+    sk.rs = rs;
+    sk.chi2 = chi2;
+}
+
+// per call: the shape fields, the scale radius (src/nfw.c:130-173), m_pe_b
+// and the radius that bounds m_pe_d (src/properties.c:351-382)
+__global__ void k_props_late(int32_t nh, const PropCall *__restrict__ pc,
                              const ShapeState *__restrict__ st, const ScaleCall *__restrict__ sc,
-                             PropArgs pa, hp_halo_props *__restrict__ props) {
+                             PropArgs pa, double *__restrict__ r_pe,
+                             hp_halo_props *__restrict__ props) {
     const int32_t h = blockIdx.x * blockDim.x + threadIdx.x;
     if (h >= nh) return;
     hp_halo_props o = props[h];
@@ -952,43 +1008,62 @@ __global__ void k_props_late(int32_t nh, const struct potential *__restrict__ sr
         o.inertia_tensor2[k] = s0.inertia[k];
         o.inertia_tensor[k] = s1.assigned ? s1.inertia[k] : s0.inertia[k];
     }
-    const double m = (double)c.n0 * pa.particle_mass;
-    float rs = klypin_rs((float)m, o.r, o.vmax, o.rvmax, pa.scale_now);
-    o.klypin_rs = rs;
     const ScaleCall &sk = sc[h];
-    if (sk.ppbin > 0) {
-        float w[kScaleBins];
-        for (int i = 0; i < kScaleBins; i++) w[i] = i < sk.nbins ? 1.0f : 0.0f;
-        for (int i = 0; i < sk.nbins; i++)
-            if (sk.bin_r[i] * 1e-3 < 3 * pa.force_res) w[i] = 0.1f;
-        float chi2 = 0.0f;
-        rs = fit_rs(rs, sk.bin_r, w, sk.nbins, &chi2);
-        o.chi2 = chi2;
+    float rs;
+    if (sk.ppbin > 0) {  // fitted by k_scale_fit
+        o.klypin_rs = sk.klypin;
+        rs = sk.rs;
+        o.chi2 = sk.chi2;
+    } else {  // fewer than 100 bound records: Klypin's estimate
+        const double m = (double)c.n0 * pa.particle_mass;
+        rs = klypin_rs((float)m, o.r, o.vmax, o.rvmax, pa.scale_now);
+        o.klypin_rs = rs;
     }
     o.rs = rs;
-    // src/properties.c:351-382: the bound records with r < r_pe_d come
-    // first (r2 is sorted), so their count is a prefix count
+    // src/properties.c:351-382: m_pe_d counts the bound records with
+    // r < r_pe_d; those come first (r2 is sorted): k_pe_tiles counts them
     double r_pe_d = rs * 4.0;
     if (r_pe_d < o.r / 5.0) r_pe_d = o.r / 5.0;
     r_pe_d *= 1e-3;
-    const int64_t first = hfirst[h];
-    int32_t lo = 0, hi = c.kept;
-    while (lo < hi) {
-        const int32_t mid = (lo + hi) >> 1;
-        if (sqrt((double)srt[first + mid].r2) < r_pe_d) lo = mid + 1;
-        else hi = mid;
-    }
-    int64_t npd = 0;
-    if (lo > 0) {  // the tile holding record lo - 1: its prefix, then its records
-        const int32_t ti = (lo - 1) / kScanTile;
-        npd = pre[call_tile0[h] + ti].nb;
-        for (int32_t j = ti * kScanTile; j < lo; j++)
-            npd += !(srt[first + j].pe < (double)srt[first + j].ke) ? 1 : 0;
-    }
-    o.m_pe_d = (float)((double)npd * pa.particle_mass);
+    r_pe[h] = r_pe_d;
     o.m_pe_b = (float)(pa.particle_mass * pow(c.pe_b, 2.0 / 3.0) /
                        cbrt(4.0 * M_PI * pa.rvir_dens_z0 / 3.0));
     props[h] = o;
+}
+
+// bound records with r < r_pe_d per tile
+__global__ __launch_bounds__(kWave) void k_pe_tiles(const struct potential *__restrict__ srt,
+                                                    const ScanTile *__restrict__ tiles,
+                                                    int32_t ntiles,
+                                                    const hp_halo *__restrict__ halos,
+                                                    const int64_t *__restrict__ hfirst,
+                                                    const int64_t *__restrict__ hcount,
+                                                    const double *__restrict__ r_pe,
+                                                    int32_t *__restrict__ cnt) {
+    const int32_t t = blockIdx.x;
+    if (t >= ntiles) return;
+    const ScanTile tl = tiles[t];
+    const int64_t first = hfirst[tl.call];
+    const double lim = r_pe[tl.call];
+    const int32_t end = min(tl.off + kScanTile, call_kept(halos, hcount, tl.call));
+    int32_t n = 0;
+    for (int32_t j = tl.off + lane_id(); j < end; j += kWave) {
+        const struct potential &r = srt[first + j];
+        n += !(r.pe < (double)r.ke) && sqrt((double)r.r2) < lim ? 1 : 0;
+    }
+    n = wave_sum(n);
+    if (lane_id() == 0) cnt[t] = n;
+}
+
+__global__ void k_pe_calls(int32_t nh, const int32_t *__restrict__ call_tile0,
+                           const int32_t *__restrict__ call_ntiles,
+                           const int32_t *__restrict__ cnt, PropArgs pa,
+                           hp_halo_props *__restrict__ props) {
+    const int32_t h = blockIdx.x * blockDim.x + threadIdx.x;
+    if (h >= nh) return;
+    int64_t n = 0;
+    for (int32_t t = call_tile0[h]; t < call_tile0[h] + call_ntiles[h]; t++) n += cnt[t];
+    props[h].m_pe_d = (float)((double)n * pa.particle_mass);
 }
 
 }  // namespace hp
