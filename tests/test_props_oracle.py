@@ -21,7 +21,8 @@ bound = 0 and then bound = 1 as src/properties.c:552-553 does).
   src/jacobi.c), calc_scale_radius (src/nfw.c) and the pseudo-evolution
   masses; every one of their fields is compared bit for bit above.  The
   shape of a window of few records is shown to be set by the order of the
-  sums alone.
+  sums alone.  Closed forms: an ellipsoidal cloud gives back its axis
+  ratios and major axis; an NFW cloud gives back its scale radius.
 """
 import math
 
@@ -206,6 +207,74 @@ def test_shape_of_few_records_is_set_by_summation_order(restatement):
                 moved += 1
                 assert nb < 32, (i, nb)
     assert moved > 0
+
+
+def _sorted_records(x, rng):
+    """Bound records at positions x about a centre at the origin, r2 from the
+    float positions in the assembly's order, sorted by r2."""
+    po = np.zeros(x.shape[0], catalog.POTENTIAL_DTYPE)
+    po["pos"][:, :3] = x
+    po["pos"][:, 3:] = rng.normal(0.0, 100.0, x.shape)
+    d = po["pos"][:, :3]
+    po["r2"] = (d[:, 0] * d[:, 0] + d[:, 1] * d[:, 1]) + d[:, 2] * d[:, 2]
+    po = po[np.argsort(po["r2"], kind="stable")]
+    po["pe"] = 2.0
+    po["ke"] = 1.0
+    return po
+
+
+def _directions(n, rng):
+    u = rng.normal(size=(n, 3))
+    return u / np.linalg.norm(u, axis=1)[:, None]
+
+
+@pytest.mark.parametrize("seed", [1, 2, 3])
+def test_shape_of_an_ellipsoidal_cloud(restatement, seed):
+    """Records stratified on similar ellipsoids with axes 1 : 0.6 : 0.35
+    (density ~ m^-2 in the ellipsoidal radius m), under a random rotation.
+    Any weight that depends on m alone makes the weighted second moments
+    over a similar ellipsoid proportional to diag(a^2, b^2, c^2), so the
+    shape iteration's fixed point is the drawn shape: b/a and c/a to the
+    sampling noise, and A along the drawn major axis.  A swapped eigenvalue
+    order or a transposed eigenvector matrix fails this."""
+    rng = np.random.default_rng(seed)
+    prm = catalog.Params()
+    halopot.set_config(prm)
+    th = halopot.mass_definitions().as_array()
+    rot, _ = np.linalg.qr(rng.normal(size=(3, 3)))
+    n = 40000
+    m = rng.uniform(0.0, 0.6, n)
+    x = (_directions(n, rng) * m[:, None] * np.array([1.0, 0.6, 0.35])) @ rot.T
+    po = _sorted_records(x, rng)
+    out = np.zeros(1, catalog.PROPS_DTYPE)
+    restatement.halo_props(po, n, np.zeros(6, np.float32), prm, th, out)
+    g = out[0]
+    for ba, ca, a in (("b_to_a", "c_to_a", "A"), ("b_to_a2", "c_to_a2", "A2")):
+        assert abs(float(g[ba]) - 0.6) < 0.03, (ba, g[ba])
+        assert abs(float(g[ca]) - 0.35) < 0.02, (ca, g[ca])
+        axis = np.asarray(g[a], np.float64)
+        assert abs(axis @ rot[:, 0]) > 0.995 * np.linalg.norm(axis), a
+
+
+@pytest.mark.parametrize("n,rs,rmax", [(20000, 0.02, 0.3), (5000, 0.05, 0.4),
+                                       (50000, 0.03, 0.3), (2000, 0.015, 0.2)])
+def test_scale_radius_of_an_nfw_cloud(restatement, n, rs, rmax):
+    """Radii drawn from an NFW profile with scale radius rs, truncated at
+    rmax.  calc_scale_radius fits the shape of the cumulative count inside
+    its outermost bin, which is independent of the normalisation and of the
+    truncation, so the fit returns rs (kpc/h) to the sampling noise."""
+    rng = np.random.default_rng(int(n))
+    prm = catalog.Params()
+    halopot.set_config(prm)
+    th = halopot.mass_definitions().as_array()
+    grid = np.linspace(0.0, rmax, 200001)
+    menc = np.log1p(grid / rs) - grid / (rs + grid)
+    r = np.interp(rng.uniform(0.0, menc[-1], n), menc, grid)
+    po = _sorted_records(_directions(n, rng) * r[:, None], rng)
+    out = np.zeros(1, catalog.PROPS_DTYPE)
+    restatement.halo_props(po, n, np.zeros(6, np.float32), prm, th, out)
+    assert abs(float(out[0]["rs"]) / (rs * 1e3) - 1.0) < 0.1, out[0]["rs"]
+    assert 0.0 < float(out[0]["chi2"]) < 2.0
 
 
 DEFS = [catalog.DEFAULT_MASS_DEFINITIONS, ("m200c", "Mvir", "178B", "1000C", "xyz")]
