@@ -86,7 +86,7 @@ HALO_DTYPE = np.dtype(
     }
 )
 
-SEEDS = {"c1": 0x5EED0001, "c2": 0x5EED0002, "c3": 0x5EED0003, "c4": 0x5EED0001,
+SEEDS = {"fof": 0x5EED0007, "c1": 0x5EED0001, "c2": 0x5EED0002, "c3": 0x5EED0003, "c4": 0x5EED0001,
          "c5": 0x5EED0005, "tree": 0x5EED0006}
 
 
@@ -486,6 +486,35 @@ def hierarchy(seed=SEEDS["tree"], n_groups=200, lo=200, hi=50_000, mean_subs=1.5
     tree["pos"][:, :3] = cen
     tree["pos"][:, 3:] = bulk
     return Hierarchy(prm, parts, tree, np.nonzero(num_p >= 1)[0].astype(np.int64))
+
+
+@dataclasses.dataclass
+class FofGroups:
+    """FOF groups as the sub-FOF step receives them: each group's particles
+    contiguous (struct particle, 32 B), in an arbitrary order."""
+
+    particles: np.ndarray
+    groups: np.ndarray        # FOF_GROUP_DTYPE rows: first, num_p
+    params: Params
+
+
+FOF_GROUP_DTYPE = np.dtype([("first", "<i8"), ("num_p", "<i8")])
+
+
+def fof_groups(seed=SEEDS["fof"], n_groups=200, lo=200, hi=50_000, **kw) -> FofGroups:
+    """The groups of hierarchy() as whole FOF groups (a host with its
+    sub-halos, distinct in phase space), each group's particles shuffled
+    (a FOF pass leaves them in no particular order)."""
+    h = hierarchy(seed=seed, n_groups=n_groups, lo=lo, hi=hi, **kw)
+    hosts = np.flatnonzero(h.tree["sub_of"] < 0)
+    g = np.zeros(hosts.shape[0], FOF_GROUP_DTYPE)
+    g["first"] = h.tree["p_start"][hosts]
+    g["num_p"] = h.tree["num_child_particles"][hosts]
+    rng = np.random.Generator(np.random.Philox(seed + 1))
+    parts = h.particles.copy()
+    for f, n in zip(g["first"], g["num_p"]):
+        parts[f:f + n] = parts[f:f + n][rng.permutation(int(n))]
+    return FofGroups(parts, g, h.params)
 
 
 def config5(seed=SEEDS["c5"], total=16_777_216, box=250.0, n_halos=50_000, frac_halo=0.40,

@@ -119,6 +119,14 @@ class Restatement:
         L.orc_particle_radii.restype = C.c_int64
         L.orc_particle_radii.argtypes = [C.c_void_p, C.c_void_p, C.c_int64, C.c_int64, C.c_double,
                                          C.c_void_p, C.c_int64]
+        L.orc_subfof.restype = C.c_int
+        L.orc_subfof.argtypes = [C.c_void_p, C.c_int64, C.c_float, C.c_int, C.c_double, C.c_int64,
+                                 C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]
+        L.orc_subfof_batch.argtypes = [C.c_void_p, C.c_void_p, C.c_int64, C.c_float, C.c_int,
+                                       C.c_double, C.c_int64, C.c_void_p, C.c_void_p, C.c_int]
+        L.orc_next_closest.argtypes = [C.c_void_p, C.c_int64, C.c_void_p, C.c_int64, C.c_void_p]
+        L.orc_find_median_r.restype = C.c_float
+        L.orc_find_median_r.argtypes = [C.c_void_p, C.c_int64, C.c_float, C.POINTER(C.c_uint)]
         for f in ("orc_sizeof_node", "orc_sizeof_halo", "orc_sizeof_potential",
                   "orc_sizeof_props", "orc_sizeof_tree_halo"):
             getattr(L, f).restype = C.c_int64
@@ -222,6 +230,51 @@ class Restatement:
 
     def hubble_scaling(self, prm, z):
         return self.lib.orc_hubble_scaling(C.byref(_params(prm)), z)
+
+    def subfof(self, parts: np.ndarray, thresh: float = 0.7, top: bool = True,
+               scaling: float = 1.0, exact_ll: int = 0) -> dict:
+        """The sub-FOF step of one group (orc_subfof): the particles
+        normalised and in tree order, the sample's next-closest distances,
+        the linking length and each particle's component label (smallest
+        tree position of its component, -1 without a neighbour)."""
+        assert parts.dtype.itemsize == 32
+        pp = np.ascontiguousarray(parts).copy()
+        n = pp.shape[0]
+        sample = np.zeros(max(n, 10000), np.float32)
+        label = np.zeros(max(1, n), np.int64)
+        tr = C.c_float()
+        ns = C.c_int64()
+        rc = self.lib.orc_subfof(_p(pp), n, thresh, int(top), scaling, exact_ll, C.byref(tr),
+                                 _p(sample), C.byref(ns), _p(label))
+        if rc:
+            raise ValueError("non-finite coordinate")
+        return {"tree_order": pp, "target_r": np.float32(tr.value),
+                "sample_r": sample[:ns.value], "label": label[:n]}
+
+    def subfof_batch(self, parts: np.ndarray, groups: np.ndarray, thresh: float = 0.7,
+                     top: bool = True, scaling: float = 1.0, exact_ll: int = 0,
+                     nthreads: int = 0):
+        """orc_subfof over every group (FOF_GROUP_DTYPE rows), in place on
+        parts; returns (target_r per group, label per particle)."""
+        assert parts.dtype.itemsize == 32 and parts.flags.c_contiguous
+        g = np.ascontiguousarray(groups)
+        tr = np.zeros(max(1, g.shape[0]), np.float32)
+        label = np.zeros(max(1, parts.shape[0]), np.int64)
+        self.lib.orc_subfof_batch(_p(parts), _p(g), g.shape[0], thresh, int(top), scaling,
+                                  exact_ll, _p(tr), _p(label), nthreads)
+        return tr[:g.shape[0]], label[:parts.shape[0]]
+
+    def next_closest(self, pts: np.ndarray, queries: np.ndarray) -> np.ndarray:
+        pp = np.ascontiguousarray(pts).copy()
+        q = np.ascontiguousarray(queries, np.float32).reshape(-1, 6)
+        out = np.zeros(q.shape[0], np.float32)
+        self.lib.orc_next_closest(_p(pp), pp.shape[0], _p(q), q.shape[0], _p(out))
+        return out
+
+    def find_median_r(self, rad, frac: float, seed: int) -> np.float32:
+        r = np.ascontiguousarray(rad, np.float32).copy()
+        sd = C.c_uint(seed)
+        return np.float32(self.lib.orc_find_median_r(_p(r), r.shape[0], frac, C.byref(sd)))
 
     def particle_radii(self, particles: np.ndarray, tree: np.ndarray, base: int, ratio: float,
                        cap: int) -> np.ndarray:
@@ -329,6 +382,12 @@ class ReferenceProps:
         L.ref_particle_radii_batch.argtypes = [C.c_void_p, C.c_void_p, C.c_int64, C.c_void_p,
                                                C.c_void_p, C.c_int64, C.c_double, C.c_void_p,
                                                C.c_void_p, C.c_int]
+        L.ref_subfof.restype = C.c_int64
+        L.ref_subfof.argtypes = [C.c_void_p, C.c_int64, C.c_float, C.c_int] + [C.c_void_p] * 7 + \
+            [C.c_int64] + [C.c_void_p] * 3
+        L.ref_next_closest.argtypes = [C.c_void_p, C.c_int64, C.c_void_p, C.c_int64, C.c_void_p]
+        L.ref_find_median_r.restype = C.c_float
+        L.ref_find_median_r.argtypes = [C.c_void_p, C.c_int64, C.c_float, C.c_uint]
 
     @staticmethod
     def available(path=None) -> bool:
@@ -386,6 +445,51 @@ class ReferenceProps:
             bh[f] = halo_row[f]
         kept = self.lib.ref_bound_pass(_p(po), po.shape[0], _p(bh))
         return kept, bh[0]
+
+    def set_subfof_config(self, min_halo_particles=10, exact_ll_calc=0,
+                          initial_metric_scaling=1.0) -> None:
+        """MIN_HALO_PARTICLES, EXACT_LL_CALC, INITIAL_METRIC_SCALING
+        (src/config.template.h:96,114,118)."""
+        C.c_int64.in_dll(self.lib, "MIN_HALO_PARTICLES").value = min_halo_particles
+        C.c_int64.in_dll(self.lib, "EXACT_LL_CALC").value = exact_ll_calc
+        C.c_double.in_dll(self.lib, "INITIAL_METRIC_SCALING").value = initial_metric_scaling
+
+    def subfof(self, parts: np.ndarray, thresh: float = 0.7, top: bool = True) -> dict:
+        """The reference's sub-FOF step on one group (ref_subfof): tree
+        order after norm_sd and the rebuild, the sample's next-closest
+        distances, the linking length, and _find_subfofs_better2's own
+        result (its particle array and sub-FOFs as (first, num_p)); also the
+        sub-FOFs of _find_subfofs_at_r at that length on the tree order."""
+        assert parts.dtype.itemsize == 32 and parts.flags.c_contiguous
+        n = parts.shape[0]
+        m = max(1, n)
+        tree = np.zeros(m, parts.dtype)
+        final = np.zeros(m, parts.dtype)
+        sample = np.zeros(max(n, 10000), np.float32)
+        ns = C.c_int64()
+        tr = C.c_float()
+        cap = n // 2 + 2
+        ff, fn, af, an = (np.zeros(cap, np.int64) for _ in range(4))
+        nat = C.c_int64()
+        nf = self.lib.ref_subfof(_p(parts), n, thresh, int(top), C.byref(tr), _p(tree), _p(sample),
+                                 C.byref(ns), _p(final), _p(ff), _p(fn), cap, _p(af), _p(an),
+                                 C.byref(nat))
+        return {"target_r": np.float32(tr.value), "tree_order": tree[:n],
+                "sample_r": sample[:ns.value], "final": final[:n],
+                "fofs": np.stack([ff[:nf], fn[:nf]], 1),
+                "fofs_at_r": np.stack([af[:nat.value], an[:nat.value]], 1)}
+
+    def next_closest(self, pts: np.ndarray, queries: np.ndarray) -> np.ndarray:
+        """fast3tree_find_next_closest_distance (src/fast3tree.c:992-1030)
+        of each query on the 6D GROUPIES tree of pts (rebuilt in place)."""
+        q = np.ascontiguousarray(queries, np.float32).reshape(-1, 6)
+        out = np.zeros(q.shape[0], np.float32)
+        self.lib.ref_next_closest(_p(pts), pts.shape[0], _p(q), q.shape[0], _p(out))
+        return out
+
+    def find_median_r(self, rad, frac: float, seed: int) -> np.float32:
+        r = np.ascontiguousarray(rad, np.float32)
+        return np.float32(self.lib.ref_find_median_r(_p(r), r.shape[0], frac, seed))
 
     def mass_definition(self, prm, mass_defs) -> np.ndarray:
         """particle_thresh_dens[0..4], particle_rvir_dens and

@@ -21,6 +21,11 @@
  *                          qsort with dist_compare, calculate_corevel,
  *                          compute_kinetic_energy -- the reference's own
  *                          functions; the harness only sequences them
+ *   ref_subfof          -> the phase-space sub-FOF step (_find_subfofs_better2,
+ *                          src/groupies.c:287-314), with the intermediate
+ *                          state its caller never sees (tree order, sample
+ *                          distances, linking length); ref_next_closest and
+ *                          ref_find_median_r expose its two search steps
  *   ref_halo_props_batch-> ref_halo_props for every call of a batch, OpenMP
  *                          dynamic over calls as the halo finder's loop over
  *                          halos (src/mpi_main.cpp:1377); the reference's
@@ -38,6 +43,7 @@
 #endif
 
 #include "config_vars.h"
+#include "fof.h"
 #include "groupies.h"
 #include "halo.h"
 #include "particle.h"
@@ -227,4 +233,122 @@ void ref_particle_radii_batch(struct particle *parts, const ref_tree_halo *tree,
 #pragma omp parallel for schedule(dynamic)
     for (int64_t b = 0; b < nbase; b++)
         count[b] = ref_particle_radii(parts, tree, nh, base[b], ratio, out + offset[b]);
+}
+
+/* ------------------------------------------------------------ sub-FOF ---
+ * The phase-space sub-FOF step of _find_subs (src/groupies.c:529):
+ * _find_subfofs_better2 (src/groupies.c:287-314) with norm_sd
+ * (src/groupies.c:788-830), the 6D GROUPIES fast3tree (leaf 40,
+ * src/groupies.c:24-28), find_median_r (769-786) and _find_subfofs_at_r
+ * (274-285) over src/fof.c. */
+struct fast3tree;
+struct fast3tree_results;
+extern struct fast3tree *phasetree; /* src/groupies.c:44 */
+extern struct fast3tree_results *res;
+extern float *particle_r; /* src/groupies.c:32 */
+extern int64_t num_copies;
+#pragma omp threadprivate(phasetree, res, particle_r, num_copies)
+struct fast3tree *GROUPIESfast3tree_init(int64_t n, struct particle *p);
+void GROUPIESfast3tree_rebuild(struct fast3tree *t, int64_t n, struct particle *p);
+struct fast3tree_results *GROUPIESfast3tree_results_init(void);
+float GROUPIESfast3tree_find_next_closest_distance(struct fast3tree *t,
+                                                   struct fast3tree_results *r, float c[6]);
+void _find_subfofs_better2(struct fof *f, float thresh, struct FOFInfo *fofinfo);
+void _find_subfofs_at_r(struct fof *f, float target_r, struct FOFInfo *fofinfo);
+
+static void sub_setup(int64_t n, int top) {
+    if (!phasetree) phasetree = GROUPIESfast3tree_init(0, NULL);
+    if (!res) res = GROUPIESfast3tree_results_init();
+    free(particle_r);
+    particle_r = malloc(sizeof(float) * (size_t)(n > 10000 ? n : 10000));
+    num_copies = top ? n : n + 1; /* norm_sd scales x when f->num_p == num_copies */
+}
+
+/*
+ * One group, two ways:
+ *  (1) the steps of _find_subfofs_better2 in its order, through the
+ *      reference's own functions, keeping what it does not return:
+ *      tree_order = the particles after norm_sd and fast3tree_rebuild (the
+ *      order the linking loop walks), *target_r = find_median_r's result,
+ *      sample_r = the next-closest distances of the sample, in sample order;
+ *  (2) _find_subfofs_better2 itself on a fresh copy: final = its particle
+ *      array (sorted by sub-FOF), fof_first / fof_num_p = its sub-FOFs.
+ * Also (3) _find_subfofs_at_r at (1)'s target_r on (1)'s tree order, whose
+ * sub-FOFs must equal (2)'s (returned in at_first / at_num_p). */
+int64_t ref_subfof(const struct particle *parts, int64_t n, float thresh, int top,
+                   float *target_r, struct particle *tree_order, float *sample_r,
+                   int64_t *nsample, struct particle *final, int64_t *fof_first,
+                   int64_t *fof_num_p, int64_t cap, int64_t *at_first, int64_t *at_num_p,
+                   int64_t *nat) {
+    sub_setup(n, top);
+    /* (1) */
+    struct particle *a = malloc(sizeof(struct particle) * (size_t)(n ? n : 1));
+    memcpy(a, parts, sizeof(struct particle) * (size_t)n);
+    struct fof fa = {n, a};
+    int64_t num_test = 10000;
+    if (EXACT_LL_CALC) num_test = n;
+    norm_sd(&fa, thresh);
+    GROUPIESfast3tree_rebuild(phasetree, n, a);
+    memcpy(tree_order, a, sizeof(struct particle) * (size_t)n);
+    if (num_test > n) num_test = n;
+    unsigned int seed = (unsigned int)n;
+    for (int64_t i = 0; i < num_test; i++) {
+        int64_t j;
+        if (num_test == n)
+            j = i;
+        else {
+            j = rand_r(&seed);
+            j <<= 31;
+            j += rand_r(&seed);
+            j %= n;
+        }
+        particle_r[i] = GROUPIESfast3tree_find_next_closest_distance(phasetree, res, a[j].pos);
+    }
+    memcpy(sample_r, particle_r, sizeof(float) * (size_t)num_test);
+    *nsample = num_test;
+    *target_r = find_median_r(particle_r, num_test, thresh, &seed);
+    /* (3) */
+    struct FOFInfo fi;
+    init_fofinfo(&fi);
+    _find_subfofs_at_r(&fa, *target_r, &fi);
+    *nat = fi.num_fofs;
+    for (int64_t i = 0; i < fi.num_fofs && i < cap; i++) {
+        at_first[i] = fi.fofs[i].particles - a;
+        at_num_p[i] = fi.fofs[i].num_p;
+    }
+    free_fofinfo(&fi);
+    /* (2) */
+    memcpy(final, parts, sizeof(struct particle) * (size_t)n);
+    struct fof fb = {n, final};
+    init_fofinfo(&fi);
+    _find_subfofs_better2(&fb, thresh, &fi);
+    int64_t nf = fi.num_fofs;
+    for (int64_t i = 0; i < nf && i < cap; i++) {
+        fof_first[i] = fi.fofs[i].particles - final;
+        fof_num_p[i] = fi.fofs[i].num_p;
+    }
+    free_fofinfo(&fi);
+    free(a);
+    return nf;
+}
+
+/* fast3tree_find_next_closest_distance for each of the given points on a
+   tree rebuilt over pts (in place, like src/groupies.c:294) */
+void ref_next_closest(struct particle *pts, int64_t n, const float *q, int64_t nq, float *out) {
+    sub_setup(n, 0);
+    GROUPIESfast3tree_rebuild(phasetree, n, pts);
+    for (int64_t i = 0; i < nq; i++) {
+        float c[6];
+        memcpy(c, q + 6 * i, sizeof c);
+        out[i] = GROUPIESfast3tree_find_next_closest_distance(phasetree, res, c);
+    }
+}
+
+/* find_median_r (src/groupies.c:769-786) on a copy of rad with the given seed */
+float ref_find_median_r(const float *rad, int64_t n, float frac, unsigned int seed) {
+    float *r = malloc(sizeof(float) * (size_t)(n ? n : 1));
+    memcpy(r, rad, sizeof(float) * (size_t)n);
+    float v = find_median_r(r, n, frac, &seed);
+    free(r);
+    return v;
 }

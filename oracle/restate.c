@@ -1307,3 +1307,396 @@ int64_t orc_sizeof_halo(void) { return (int64_t)sizeof(orc_halo); }
 int64_t orc_sizeof_potential(void) { return (int64_t)sizeof(struct potential); }
 int64_t orc_sizeof_props(void) { return (int64_t)sizeof(orc_props); }
 int64_t orc_sizeof_tree_halo(void) { return (int64_t)sizeof(orc_tree_halo); }
+
+/* ================================================================ sub-FOF
+ * The phase-space sub-FOF step of _find_subs: _find_subfofs_better2
+ * (src/groupies.c:287-314) over one group -- norm_sd (788-830) with
+ * calc_deviations (src/jacobi.c:137-175), the 6D GROUPIES fast3tree with 40
+ * points per leaf (src/groupies.c:24-28; src/fast3tree.c:736-907) rebuilt in
+ * place, next-closest distances of the sample (src/fast3tree.c:953-1030),
+ * find_median_r (src/groupies.c:744-786) and the linking at that length
+ * (_find_subfofs_at_r, src/groupies.c:274-285 with src/fof.c).
+ *
+ * Unlike the potential tree above, this one partitions in place exactly as
+ * the reference does: the sample indices and the linking loop both walk the
+ * array in the order the rebuild leaves it.  The linking result is given as
+ * components (label = smallest tree position in the component, -1 for a
+ * particle with no neighbour), not in the order of the reference's
+ * union-find roots. */
+#define ORC_LEAF6 40
+
+typedef struct orc_node6 {
+    float min[6], max[6];
+    int64_t start, n;
+    int32_t div, left, right, parent;
+} orc_node6;
+
+typedef struct orc_tree6 {
+    orc_node6 *nd;
+    int64_t nn, cap;
+    orc_particle *p;
+} orc_tree6;
+
+/* src/jacobi.c:137-151 */
+static double orc_set_eig(double in[3][3]) {
+    double out[3][3], e[3], m[3][3];
+    memcpy(m, in, sizeof m);
+    orc_jacobi(m, e, out);
+    double r = e[0];
+    if (e[1] < r) r = e[1];
+    if (e[2] < r) r = e[2];
+    if (r <= 0) return sqrt(in[0][0] + in[1][1] + in[2][2]);
+    return sqrt(r);
+}
+
+/* src/groupies.c:788-830 */
+static void orc_norm_sd(orc_particle *pp, int64_t n, int top, double scaling) {
+    double mean[6] = {0}, corr[6][6] = {{0}};
+    if (!n) return;
+    for (int64_t i = 0; i < n; i++)
+        for (int j = 0; j < 6; j++) mean[j] += pp[i].pos[j];
+    for (int j = 0; j < 6; j++) mean[j] /= (double)n;
+    for (int64_t i = 0; i < n; i++) {
+        float *x = pp[i].pos;
+        for (int j = 0; j < 6; j++) x[j] = (float)((double)x[j] - mean[j]);
+        for (int j = 0; j < 6; j++)
+            for (int k = j; k < 6; k++) {
+                float prod = x[j] * x[k];
+                corr[j][k] += prod;
+            }
+    }
+    for (int j = 0; j < 6; j++)
+        for (int k = j; k < 6; k++) corr[j][k] /= (double)n;
+    for (int j = 0; j < 6; j++)
+        for (int k = 0; k < j; k++) corr[j][k] = corr[k][j];
+    double a[3][3], b[3][3];
+    for (int j = 0; j < 3; j++)
+        for (int k = 0; k < 3; k++) {
+            a[j][k] = corr[j][k];
+            b[j][k] = corr[j + 3][k + 3];
+        }
+    double sx = orc_set_eig(a), sv = orc_set_eig(b);
+    if (top) sx *= scaling;
+    if (!sx || !sv) return;
+    for (int64_t i = 0; i < n; i++)
+        for (int j = 0; j < 6; j++)
+            pp[i].pos[j] = (float)((double)pp[i].pos[j] / (j < 3 ? sx : sv));
+}
+
+/* src/fast3tree.c:778-794 */
+static void orc6_bounds(orc_tree6 *t, orc_node6 *nd) {
+    const orc_particle *p = t->p + nd->start;
+    for (int j = 0; j < 6; j++) nd->min[j] = nd->max[j] = p[0].pos[j];
+    for (int64_t i = 1; i < nd->n; i++)
+        for (int j = 0; j < 6; j++) {
+            float x = p[i].pos[j];
+            if (x < nd->min[j]) nd->min[j] = x;
+            else if (x > nd->max[j]) nd->max[j] = x;
+        }
+}
+
+/* src/fast3tree.c:736-774: widest dimension (the last one first, a later one
+   only when strictly wider), float midpoint, in-place swap partition */
+static int64_t orc6_partition(orc_tree6 *t, orc_node6 *nd) {
+    int dim = 5;
+    float w = nd->max[5] - nd->min[5];
+    for (int i = 0; i < 5; i++) {
+        float w2 = nd->max[i] - nd->min[i];
+        if (w2 > w) {
+            w = w2;
+            dim = i;
+        }
+    }
+    nd->div = dim;
+    if (nd->max[dim] == nd->min[dim]) return nd->n;
+    const float lim = 0.5 * (nd->max[dim] + nd->min[dim]);
+    orc_particle *p = t->p + nd->start;
+    int64_t lo = 0, hi = nd->n - 1;
+    while (lo < hi) {
+        if (p[lo].pos[dim] > lim) {
+            orc_particle tmp = p[hi];
+            p[hi] = p[lo];
+            p[lo] = tmp;
+            hi--;
+        } else {
+            lo++;
+        }
+    }
+    if (lo == hi && !(p[lo].pos[dim] > lim)) lo++;
+    return lo;
+}
+
+/* src/fast3tree.c:798-846, left subtree before right */
+static void orc6_split(orc_tree6 *t, int64_t ni) {
+    int64_t nl = orc6_partition(t, t->nd + ni);
+    if (nl == t->nd[ni].n || nl == 0) {
+        t->nd[ni].div = -1;
+        return;
+    }
+    if (t->nn + 2 > t->cap) {
+        t->cap = t->cap * 2 + 16;
+        t->nd = realloc(t->nd, sizeof(orc_node6) * (size_t)t->cap);
+    }
+    int64_t l = t->nn, r = t->nn + 1;
+    t->nn += 2;
+    orc_node6 *nd = t->nd + ni;
+    nd->left = (int32_t)l;
+    nd->right = (int32_t)r;
+    orc_node6 *L = t->nd + l, *R = t->nd + r;
+    memset(L, 0, 2 * sizeof(orc_node6));
+    L->parent = R->parent = (int32_t)ni;
+    L->div = R->div = -1;
+    L->start = nd->start;
+    L->n = nl;
+    R->start = nd->start + nl;
+    R->n = nd->n - nl;
+    orc6_bounds(t, L);
+    orc6_bounds(t, R);
+    if (t->nd[l].n > ORC_LEAF6) orc6_split(t, l);
+    if (t->nd[r].n > ORC_LEAF6) orc6_split(t, r);
+}
+
+static void orc6_build(orc_tree6 *t, orc_particle *p, int64_t n) {
+    t->p = p;
+    t->cap = 3 + n / (ORC_LEAF6 / 2);
+    t->nd = calloc((size_t)t->cap, sizeof(orc_node6));
+    t->nn = 1;
+    t->nd[0].start = 0;
+    t->nd[0].n = n;
+    t->nd[0].div = -1;
+    t->nd[0].parent = -1;
+    if (n) orc6_bounds(t, t->nd);
+    if (n > ORC_LEAF6) orc6_split(t, 0);
+}
+
+/* src/fast3tree.c:294-309 */
+static int orc6_apart(const orc_node6 *nd, const float c[6], float r) {
+    float d = 0, e;
+    const float r2 = r * r;
+    for (int i = 0; i < 6; i++) {
+        if ((e = c[i] - nd->min[i]) < 0) {
+            d += e * e;
+            if (d >= r2) return 1;
+        } else if ((e = c[i] - nd->max[i]) > 0) {
+            d += e * e;
+            if (d >= r2) return 1;
+        }
+    }
+    return 0;
+}
+
+static float orc6_dist2(const float *a, const float *b) {
+    float d = 0;
+    for (int j = 0; j < 6; j++) {
+        float dx = a[j] - b[j];
+        d += dx * dx;
+    }
+    return d;
+}
+
+/* src/fast3tree.c:956-987 */
+static float orc6_nc(const orc_tree6 *t, int64_t ni, const float c[6], float r, int64_t skip) {
+    const orc_node6 *nd = t->nd + ni;
+    if (orc6_apart(nd, c, r) || ni == skip) return r;
+    if (nd->div < 0) {
+        float r2 = r * r;
+        for (int64_t i = 0; i < nd->n; i++) {
+            float d = orc6_dist2(c, t->p[nd->start + i].pos);
+            if (d < r2) r2 = d;
+        }
+        return (float)sqrt(r2);
+    }
+    int64_t near = nd->left, far = nd->right;
+    if (c[nd->div] > 0.5 * (nd->min[nd->div] + nd->max[nd->div])) {
+        near = nd->right;
+        far = nd->left;
+    }
+    r = orc6_nc(t, near, c, r, skip);
+    return orc6_nc(t, far, c, r, skip);
+}
+
+/* src/fast3tree.c:992-1030 */
+static float orc6_next_closest(const orc_tree6 *t, const float c[6]) {
+    int64_t ni = 0;
+    while (t->nd[ni].div >= 0) {
+        const orc_node6 *nd = t->nd + ni;
+        ni = (c[nd->div] <= t->nd[nd->left].max[nd->div]) ? nd->left : nd->right;
+    }
+    while (ni != 0 && t->nd[ni].min[t->nd[t->nd[ni].parent].div] ==
+                          t->nd[ni].max[t->nd[t->nd[ni].parent].div])
+        ni = t->nd[ni].parent;
+    const orc_node6 *nd = t->nd + ni;
+    float md = 0;
+    for (int j = 0; j < 6; j++) {
+        float dx = nd->max[j] - nd->min[j];
+        md += dx * dx;
+    }
+    for (int64_t i = 0; i < nd->n; i++) {
+        float d = orc6_dist2(c, t->p[nd->start + i].pos);
+        if (d && d < md) md = d;
+    }
+    return orc6_nc(t, 0, c, (float)sqrt(md), ni);
+}
+
+/* src/groupies.c:744-786: quickselect with rand_r pivots; it stops early when
+   the ends of the current range are equal, so its answer is not always the
+   k-th smallest value */
+static float orc_random_unit(unsigned int *seed) {
+    return ((float)(rand_r(seed) % (RAND_MAX)) / (float)(RAND_MAX));
+}
+
+static int64_t orc_rad_partition(float *v, int64_t left, int64_t right, int64_t pivot_at) {
+    const float pv = v[pivot_at];
+    float tmp;
+    tmp = v[pivot_at];
+    v[pivot_at] = v[right - 1];
+    v[right - 1] = tmp;
+    int64_t hi = right - 2, lo = left;
+    while (lo < hi) {
+        if (v[lo] > pv) {
+            tmp = v[lo];
+            v[lo] = v[hi];
+            v[hi] = tmp;
+            hi--;
+        } else {
+            lo++;
+        }
+    }
+    /* the reference reads v[hi] here even when the range holds one value
+       (hi = left - 1); the element before a range is never above its pivot */
+    if (hi < left || v[hi] <= pv) hi++;
+    tmp = v[right - 1];
+    v[right - 1] = v[hi];
+    v[hi] = tmp;
+    return hi;
+}
+
+float orc_find_median_r(float *v, int64_t n, float frac, unsigned int *seed) {
+    int64_t k = n * frac, left = 0, right = n;
+    if (n < 2) return v[0];
+    for (;;) {
+        int64_t at = left + orc_random_unit(seed) * (right - left);
+        int64_t p = orc_rad_partition(v, left, right, at);
+        if (k == p || v[left] == v[right - 1]) return v[k];
+        if (k < p)
+            right = p;
+        else
+            left = p + 1;
+    }
+}
+
+static int64_t orc_uf_find(int64_t *up, int64_t x) {
+    while (up[x] != x) {
+        up[x] = up[up[x]];
+        x = up[x];
+    }
+    return x;
+}
+
+/* every pair closer than r (float distance squared < r * r) joins one
+   component; label[i] = the smallest tree position of i's component, -1
+   when i has no such neighbour */
+static void orc6_link(const orc_tree6 *t, int64_t n, float r, int64_t *label) {
+    int64_t *up = malloc(sizeof(int64_t) * (size_t)(n ? n : 1));
+    char *has = calloc((size_t)(n ? n : 1), 1);
+    int64_t *stack = malloc(sizeof(int64_t) * (size_t)(2 * t->nn + 2));
+    for (int64_t i = 0; i < n; i++) up[i] = i;
+    const float r2 = r * r;
+    for (int64_t i = 0; i < n; i++) {
+        const float *c = t->p[i].pos;
+        int64_t sp = 0;
+        stack[sp++] = 0;
+        while (sp) {
+            const orc_node6 *nd = t->nd + stack[--sp];
+            if (nd->start + nd->n <= i || orc6_apart(nd, c, r)) continue;
+            if (nd->div >= 0) {
+                stack[sp++] = nd->right;
+                stack[sp++] = nd->left;
+                continue;
+            }
+            for (int64_t j = nd->start > i + 1 ? nd->start : i + 1; j < nd->start + nd->n; j++) {
+                if (!(orc6_dist2(c, t->p[j].pos) < r2)) continue;
+                has[i] = has[j] = 1;
+                int64_t a = orc_uf_find(up, i), b = orc_uf_find(up, j);
+                if (a < b) up[b] = a;
+                else if (b < a) up[a] = b;
+            }
+        }
+    }
+    for (int64_t i = 0; i < n; i++) label[i] = has[i] ? orc_uf_find(up, i) : -1;
+    free(stack);
+    free(has);
+    free(up);
+}
+
+/* the whole step for one group, in place on pp (normalised, tree order);
+   returns -1 (nothing done) when a coordinate is not finite */
+int orc_subfof(orc_particle *pp, int64_t n, float thresh, int top, double scaling,
+               int64_t exact_ll, float *target_r, float *sample_r, int64_t *nsample,
+               int64_t *label) {
+    for (int64_t i = 0; i < n; i++)
+        for (int j = 0; j < 6; j++)
+            if (!isfinite(pp[i].pos[j])) return -1;
+    *nsample = 0;
+    *target_r = 0;
+    if (!n) return 0;
+    orc_norm_sd(pp, n, top, scaling);
+    for (int64_t i = 0; i < n; i++)
+        for (int j = 0; j < 6; j++)
+            if (!isfinite(pp[i].pos[j])) return -1;
+    orc_tree6 t;
+    orc6_build(&t, pp, n);
+    int64_t ns = exact_ll ? n : 10000;
+    if (ns > n) ns = n;
+    unsigned int seed = (unsigned int)n;
+    for (int64_t i = 0; i < ns; i++) {
+        int64_t j = i;
+        if (ns != n) {
+            j = rand_r(&seed);
+            j <<= 31;
+            j += rand_r(&seed);
+            j %= n;
+        }
+        sample_r[i] = orc6_next_closest(&t, pp[j].pos);
+    }
+    *nsample = ns;
+    float *tmp = malloc(sizeof(float) * (size_t)ns);
+    memcpy(tmp, sample_r, sizeof(float) * (size_t)ns);
+    *target_r = orc_find_median_r(tmp, ns, thresh, &seed);
+    free(tmp);
+    orc6_link(&t, n, *target_r, label);
+    free(t.nd);
+    return 0;
+}
+
+/* next-closest distances of the queries on the tree rebuilt (in place) over p */
+void orc_next_closest(orc_particle *p, int64_t n, const float *q, int64_t nq, float *out) {
+    orc_tree6 t;
+    orc6_build(&t, p, n);
+    for (int64_t i = 0; i < nq; i++) out[i] = orc6_next_closest(&t, q + 6 * i);
+    free(t.nd);
+}
+
+/* orc_subfof over a batch of groups (first, num_p), OpenMP dynamic over
+   groups; label[] is indexed like the particles (group-local positions) */
+typedef struct orc_fof_group {
+    int64_t first, num_p;
+} orc_fof_group;
+
+void orc_subfof_batch(orc_particle *pp, const orc_fof_group *g, int64_t ng, float thresh,
+                      int top, double scaling, int64_t exact_ll, float *target_r,
+                      int64_t *label, int nthreads) {
+#ifdef _OPENMP
+    if (nthreads > 0) omp_set_num_threads(nthreads);
+#else
+    (void)nthreads;
+#endif
+#pragma omp parallel for schedule(dynamic)
+    for (int64_t i = 0; i < ng; i++) {
+        int64_t ns, n = g[i].num_p;
+        float *s = malloc(sizeof(float) * (size_t)(n > 10000 ? n : 10000));
+        orc_subfof(pp + g[i].first, n, thresh, top, scaling, exact_ll, target_r + i, s, &ns,
+                   label + g[i].first);
+        free(s);
+    }
+}
