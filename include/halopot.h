@@ -217,6 +217,49 @@ int hp_gather_calls(const hp_tree_halo *tree, int64_t nhalos, const int64_t *bas
                     double include_host_ratio, struct potential *d_po, int64_t cap_records,
                     hp_halo *calls, int64_t *nrecords);
 
+/* ------------------------------------------------------ phase-space sub-FOF */
+
+/* One FOF group of a batch: a struct fof (src/fof.h:7-10) given by index.
+   first/num_p/top in; target_r, first_sub, num_sub out. */
+typedef struct hp_fof_group {
+    int64_t first;      /* first particle in d_particles */
+    int64_t num_p;
+    int32_t top;        /* the whole FOF (norm_sd's num_p == num_copies,
+                           src/groupies.c:821): sigma_x *= INITIAL_METRIC_SCALING */
+    float target_r;     /* the linking length found (src/groupies.c:312) */
+    int64_t first_sub;  /* its sub-FOFs: subs[first_sub .. first_sub + num_sub) */
+    int64_t num_sub;
+} hp_fof_group;
+
+/* A sub-FOF: num_p particles from d_particles[first] (struct fof). */
+typedef struct hp_subfof {
+    int64_t first;
+    int64_t num_p;
+} hp_subfof;
+
+typedef struct hp_subfof_config {   /* src/config.template.h:96,112,114,118 */
+    double fof_fraction;            /* FOF_FRACTION (0.7) */
+    double initial_metric_scaling;  /* INITIAL_METRIC_SCALING (1) */
+    int64_t min_halo_particles;     /* MIN_HALO_PARTICLES (10) */
+    int32_t exact_ll_calc;          /* EXACT_LL_CALC (0) */
+    int32_t pad;
+} hp_subfof_config;
+
+/* Replaces _find_subfofs_better2(f, FOF_FRACTION, fofinfo)
+   (src/groupies.c:287-314, called by _find_subs at src/groupies.c:529) for
+   every group of a batch at once: norm_sd, the in-place 6D fast3tree
+   rebuild, the sample's next-closest distances, find_median_r and the
+   linking at that length.  On return each group's particles in d_particles
+   are normalised (as norm_sd leaves them) and grouped by sub-FOF; the
+   sub-FOFs of MIN_HALO_PARTICLES or more are listed in subs (in the order
+   of each component's first particle in the tree order; the reference lists
+   them in the order of its union-find roots).  Groups must not overlap.
+   NULL config: the reference's defaults.  Returns HP_OK or an error code
+   (non-finite coordinates, overlapping groups, too small a subs array). */
+int hp_find_subfofs(struct particle *d_particles, int64_t nparticles, hp_fof_group *groups,
+                    int64_t ngroups, const hp_subfof_config *config, hp_subfof *subs,
+                    int64_t cap_subs, int64_t *nsubs);
+
 /* ------------------------------------------------------- halo properties */
 
 /* Density thresholds of the mass definitions, in particles per (Mpc/h)^3:

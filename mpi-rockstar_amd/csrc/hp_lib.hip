@@ -20,6 +20,7 @@
 #include "hp_unbind.hip"
 #include "hp_props.hip"
 #include "hp_gather.hip"
+#include "hp_subfof.hip"
 #include "hp_walk.hip"
 
 #include <algorithm>
@@ -178,6 +179,18 @@ struct DevBuf {
         if (r_) return r_;                      \
     } while (0)
 
+// Device scratch of the sub-FOF step (hp_find_subfofs).
+struct SubBufs {
+    DevBuf groups, tiles, qtiles, drawn, psum, mean, pcorr, sig, pbox, gerr, w, w2, rk, bpos, spos,
+        nodes, act0, act1, counters, slot_group, dist, sel, sel2, target, up, has, csize, emit;
+    void release() {
+        for (DevBuf *b : {&groups, &tiles, &qtiles, &drawn, &psum, &mean, &pcorr, &sig, &pbox,
+                          &gerr, &w, &w2, &rk, &bpos, &spos, &nodes, &act0, &act1, &counters,
+                          &slot_group, &dist, &sel, &sel2, &target, &up, &has, &csize, &emit})
+            b->release();
+    }
+};
+
 // Host-side tables of one batch (depend only on the calls' first/count).
 struct Tables {
     int64_t nrec = 0, nhalos = 0;
@@ -268,6 +281,7 @@ struct Engine {
         fit_list, fit_n;
     // record assembly
     DevBuf gsegs, gtiles, gcen;
+    SubBufs sub;
     int64_t cap_nodes = 0, cap_small = 0, cap_group = 0, cap_big = 0, cap_tiles = 0, cap_fat = 0;
     std::vector<int32_t> depth_end;  // node count after each depth's level
 
@@ -315,6 +329,7 @@ struct Engine {
             (void)hipStreamSynchronize(stream);
             drop_graphs();
             for (DevBuf *b : all) b->release();
+            sub.release();
             for (auto &e : ev)
                 if (e) (void)hipEventDestroy(e);
             if (h_counters) (void)hipHostFree(h_counters);
@@ -1400,6 +1415,254 @@ __attribute__((visibility("default"))) int hp_gather_calls(
         e->gtiles.as<GatherTile>(), nt, e->gcen.as<float>(), d_po);
     HP_TRY(hipGetLastError());
     HP_TRY(hipStreamSynchronize(e->stream));
+    return HP_OK;
+}
+
+static_assert(sizeof(hp_fof_group) == 40 && offsetof(hp_fof_group, target_r) == 20 &&
+                  offsetof(hp_fof_group, first_sub) == 24,
+              "hp_fof_group layout");
+static_assert(sizeof(hp_subfof) == 16 && sizeof(hp_subfof_config) == 32, "sub-FOF layouts");
+
+__attribute__((visibility("default"))) int hp_find_subfofs(struct particle *d_particles,
+                                                           int64_t nparticles, hp_fof_group *groups,
+                                                           int64_t ngroups,
+                                                           const hp_subfof_config *config,
+                                                           hp_subfof *subs, int64_t cap_subs,
+                                                           int64_t *nsubs) {
+    if (nparticles < 0 || ngroups < 0 || cap_subs < 0 || !nsubs || (ngroups > 0 && !groups) ||
+        (nparticles > 0 && !d_particles) || (cap_subs > 0 && !subs))
+        return set_error(HP_EINVAL, "hp_find_subfofs: bad arguments");
+    const hp_subfof_config dflt = {0.7, 1.0, 10, 0, 0};
+    const hp_subfof_config cfg = config ? *config : dflt;
+    *nsubs = 0;
+    if (!(cfg.fof_fraction >= 0.0 && cfg.fof_fraction < 1.0) || cfg.min_halo_particles < 0)
+        return set_error(HP_EINVAL, "hp_find_subfofs: FOF_FRACTION must be in [0, 1)");
+    if (ngroups == 0) return HP_OK;
+    // host tables: working slots, norm_sd tiles, the sample of each group
+    const float thresh = (float)cfg.fof_fraction;
+    std::vector<SubGroup> G((size_t)ngroups);
+    std::vector<SubTile> tiles, qtiles;
+    std::vector<int32_t> drawn;
+    std::vector<std::pair<int64_t, int64_t>> ranges;
+    int64_t M = 0, S = 0;
+    for (int64_t g = 0; g < ngroups; g++) {
+        const int64_t n = groups[g].num_p, f = groups[g].first;
+        if (n < 0 || f < 0 || f + n > nparticles)
+            return set_error(HP_EINVAL, "hp_find_subfofs: group %lld particles [%lld, +%lld) "
+                             "outside the %lld given", (long long)g, (long long)f, (long long)n,
+                             (long long)nparticles);
+        if (M + n > (1 << 29)) return set_error(HP_EINVAL, "hp_find_subfofs: batch too large");
+        ranges.emplace_back(f, n);
+        SubGroup &sg = G[(size_t)g];
+        sg.first = f;
+        sg.off = (int32_t)M;
+        sg.n = (int32_t)n;
+        sg.top = groups[g].top ? 1 : 0;
+        // src/groupies.c:290-311: every particle when there are at most
+        // MAX_PARTICLES_TO_SAMPLE (or EXACT_LL_CALC), else rand_r draws
+        int64_t ns = cfg.exact_ll_calc ? n : kSampleMax;
+        if (ns > n) ns = n;
+        uint32_t seed = (uint32_t)n;
+        sg.nsample = (int32_t)ns;
+        sg.sample_off = (int32_t)S;
+        sg.drawn = ns != n;
+        drawn.resize((size_t)(S + ns));
+        for (int64_t i = 0; i < ns; i++) {
+            int64_t j = i;
+            if (sg.drawn) {
+                j = rand_r_glibc(&seed);
+                j <<= 31;
+                j += rand_r_glibc(&seed);
+                j %= n;
+            }
+            drawn[(size_t)(S + i)] = (int32_t)j;
+        }
+        sg.seed = seed;
+        sg.k = (int32_t)(int64_t)((float)ns * thresh);  // src/groupies.c:771
+        if (ns > 0 && sg.k >= ns)
+            return set_error(HP_EINVAL, "hp_find_subfofs: FOF_FRACTION selects past the sample");
+        sg.tile0 = (int32_t)tiles.size();
+        for (int64_t off = 0; off < n; off += kSubTile)
+            tiles.push_back(SubTile{(int32_t)g, (int32_t)off});
+        sg.ntiles = (int32_t)(tiles.size() - sg.tile0);
+        for (int64_t off = 0; off < ns; off += kWave)
+            qtiles.push_back(SubTile{(int32_t)g, (int32_t)off});
+        M += n;
+        S += ns;
+    }
+    std::sort(ranges.begin(), ranges.end());
+    for (size_t i = 1; i < ranges.size(); i++)
+        if (ranges[i].second && ranges[i - 1].first + ranges[i - 1].second > ranges[i].first)
+            return set_error(HP_EINVAL, "hp_find_subfofs: groups overlap at particle %lld",
+                             (long long)ranges[i].first);
+    int r;
+    Engine *e = thread_engine(&r);
+    if (!e) return r;
+    HP_TRY(hipSetDevice(e->device));
+    SubBufs &B = e->sub;
+    const int64_t nt = (int64_t)tiles.size(), ng = ngroups;
+    const int64_t node_cap = 2 * M + ng + 2;
+    const int64_t cap_emit = M / 2 + ng + 1;
+    HP_ENSURE(B.psum, sizeof(double) * 6 * (nt + 1));
+    HP_ENSURE(B.mean, sizeof(double) * 6 * ng);
+    HP_ENSURE(B.pcorr, sizeof(double) * 21 * (nt + 1));
+    HP_ENSURE(B.sig, sizeof(double) * 2 * ng);
+    HP_ENSURE(B.pbox, sizeof(float) * 12 * (nt + 1));
+    HP_ENSURE(B.gerr, sizeof(int32_t) * ng);
+    HP_ENSURE(B.w, sizeof(SParticle) * (M + 1));
+    HP_ENSURE(B.w2, sizeof(SParticle) * (M + 1));
+    HP_ENSURE(B.rk, sizeof(int32_t) * (M + 1));
+    HP_ENSURE(B.bpos, sizeof(int32_t) * (M + 1));
+    HP_ENSURE(B.spos, sizeof(int32_t) * (M + 1));
+    HP_ENSURE(B.nodes, sizeof(Node6) * node_cap);
+    HP_ENSURE(B.act0, sizeof(int32_t) * (M / (kLeaf6 + 1) + ng + 1));
+    HP_ENSURE(B.act1, sizeof(int32_t) * (M / (kLeaf6 + 1) + ng + 1));
+    HP_ENSURE(B.counters, sizeof(int32_t) * kSfCounters);
+    HP_ENSURE(B.slot_group, sizeof(int32_t) * (M + 1));
+    HP_ENSURE(B.dist, sizeof(float) * (S + 1));
+    HP_ENSURE(B.sel, sizeof(float) * (S + 1));
+    HP_ENSURE(B.sel2, sizeof(float) * (S + 1));
+    HP_ENSURE(B.target, sizeof(float) * ng);
+    HP_ENSURE(B.up, sizeof(int32_t) * (M + 1));
+    HP_ENSURE(B.has, M + 1);
+    HP_ENSURE(B.csize, sizeof(int32_t) * (M + 1));
+    HP_ENSURE(B.emit, sizeof(int4) * cap_emit);
+    const int32_t ntile_sort = (int32_t)((M + kSortTile - 1) / kSortTile);
+    HP_ENSURE(e->keys0, sizeof(uint64_t) * (M + 1));
+    HP_ENSURE(e->keys1, sizeof(uint64_t) * (M + 1));
+    HP_ENSURE(e->vals0, sizeof(int32_t) * (M + 1));
+    HP_ENSURE(e->vals1, sizeof(int32_t) * (M + 1));
+    HP_ENSURE(e->hist, sizeof(int32_t) * kRadix * (ntile_sort + 1));
+    HP_ENSURE(e->hsum, sizeof(int32_t) * (kRadix * (ntile_sort + 1) / kScanBlock + 2));
+    if ((r = e->upload(B.groups, G)) || (r = e->upload(B.tiles, tiles)) ||
+        (r = e->upload(B.qtiles, qtiles)) || (r = e->upload(B.drawn, drawn)))
+        return r;
+    hipStream_t s = e->stream;
+    const SubGroup *dg = B.groups.as<SubGroup>();
+    const SubTile *dt = B.tiles.as<SubTile>();
+    int32_t *cnt = B.counters.as<int32_t>();
+    SParticle *parts = reinterpret_cast<SParticle *>(d_particles);
+    k_fill_i32<<<1, 64, 0, s>>>(cnt, 0, kSfCounters);
+    HP_LAUNCHED();
+    k_fill_i32<<<Engine::blocks(ng, 256), 256, 0, s>>>(B.gerr.as<int32_t>(), 0, ng);
+    HP_LAUNCHED();
+    if (nt) {
+        // norm_sd
+        k_sub_load<<<(uint32_t)nt, kWave, 0, s>>>(parts, dg, dt, B.w.as<SParticle>(),
+                                                  B.slot_group.as<int32_t>(), B.psum.as<double>(),
+                                                  B.gerr.as<int32_t>());
+        HP_LAUNCHED();
+        k_sub_mean<<<Engine::blocks(ng, 64), 64, 0, s>>>((int32_t)ng, dg, B.psum.as<double>(),
+                                                 B.mean.as<double>());
+        HP_LAUNCHED();
+        k_sub_centre<<<(uint32_t)nt, kWave, 0, s>>>(dg, dt, B.mean.as<double>(),
+                                                    B.w.as<SParticle>(), B.pcorr.as<double>());
+        HP_LAUNCHED();
+        k_sub_sigma<<<Engine::blocks(ng, 64), 64, 0, s>>>((int32_t)ng, dg, B.pcorr.as<double>(),
+                                                  cfg.initial_metric_scaling, B.sig.as<double>());
+        HP_LAUNCHED();
+        k_sub_scale<<<(uint32_t)nt, kWave, 0, s>>>(dg, dt, B.sig.as<double>(), B.w.as<SParticle>(),
+                                                   B.pbox.as<float>(), B.gerr.as<int32_t>());
+        HP_LAUNCHED();
+    }
+    // the trees, level by level
+    k_fill_i32<<<1, 1, 0, s>>>(cnt + kSfNodes, (int32_t)ng, 1);
+    HP_LAUNCHED();
+    k_sub_roots<<<Engine::blocks(ng, 64), 64, 0, s>>>((int32_t)ng, dg, B.pbox.as<float>(),
+                                              B.nodes.as<Node6>(), B.act0.as<int32_t>(), cnt);
+    HP_LAUNCHED();
+    int32_t *act = B.act0.as<int32_t>(), *nxt = B.act1.as<int32_t>();
+    for (int level = 0;; level++) {
+        int32_t na = 0;
+        HP_TRY(hipMemcpyAsync(&na, cnt + kSfActive, sizeof na, hipMemcpyDeviceToHost, s));
+        HP_TRY(hipStreamSynchronize(s));
+        if (!na) break;
+        if (level > 4 * 64) return set_error(HP_EHIP, "hp_find_subfofs: tree too deep");
+        k_fill_i32<<<1, 1, 0, s>>>(cnt + kSfActive, 0, 1);
+        HP_LAUNCHED();
+        k_sub_split<<<(uint32_t)na, kSfThreads, 0, s>>>(
+            B.nodes.as<Node6>(), act, B.w.as<SParticle>(), B.w2.as<SParticle>(),
+            B.rk.as<int32_t>(), B.bpos.as<int32_t>(), B.spos.as<int32_t>(), nxt, cnt,
+            (int32_t)node_cap);
+        HP_LAUNCHED();
+        std::swap(act, nxt);
+    }
+    // the linking length of each group
+    if (!qtiles.empty()) {
+        k_sub_next_closest<<<(uint32_t)qtiles.size(), kWave, 0, s>>>(
+            dg, B.qtiles.as<SubTile>(), B.drawn.as<int32_t>(), B.w.as<SParticle>(),
+            B.nodes.as<Node6>(), B.dist.as<float>());
+        HP_LAUNCHED();
+        HP_TRY(hipMemcpyAsync(B.sel.p, B.dist.p, sizeof(float) * S, hipMemcpyDeviceToDevice, s));
+    }
+    k_sub_median<<<(uint32_t)ng, kSfThreads, 0, s>>>(dg, thresh, B.sel.as<float>(),
+                                                      B.sel2.as<float>(), B.rk.as<int32_t>(),
+                                                      B.bpos.as<int32_t>(), B.spos.as<int32_t>(),
+                                                      B.target.as<float>());
+    HP_LAUNCHED();
+    // the linking, then the particles back grouped by component
+    if (M) {
+        const int32_t m = (int32_t)M;
+        k_sub_init<<<Engine::blocks(M, 256), 256, 0, s>>>(m, B.up.as<int32_t>(), B.has.as<uint8_t>(),
+                                                  B.csize.as<int32_t>());
+        HP_LAUNCHED();
+        k_sub_link<<<Engine::blocks(M, kWave), kWave, 0, s>>>(m, B.slot_group.as<int32_t>(),
+                                                      B.target.as<float>(), B.w.as<SParticle>(),
+                                                      B.nodes.as<Node6>(), B.up.as<int32_t>(),
+                                                      B.has.as<uint8_t>());
+        HP_LAUNCHED();
+        k_sub_keys<<<Engine::blocks(M, 256), 256, 0, s>>>(m, B.slot_group.as<int32_t>(), dg,
+                                                  B.up.as<int32_t>(), B.has.as<uint8_t>(),
+                                                  B.csize.as<int32_t>(), e->keys0.as<uint64_t>(),
+                                                  e->vals0.as<int32_t>());
+        HP_LAUNCHED();
+        int bits = 1;
+        while ((int64_t)1 << bits <= M) bits++;
+        if ((r = e->enqueue_sort(M, bits))) return r;
+        k_sub_emit<<<Engine::blocks(M, 256), 256, 0, s>>>(
+            m, B.slot_group.as<int32_t>(), dg, e->sorted_keys, e->sorted_vals,
+            B.csize.as<int32_t>(), B.w.as<SParticle>(), parts, cfg.min_halo_particles,
+            B.emit.as<int4>(), (int32_t)cap_emit, cnt);
+        HP_LAUNCHED();
+    }
+    int32_t hc[kSfCounters];
+    std::vector<int32_t> gerr((size_t)ng);
+    std::vector<float> target((size_t)ng);
+    HP_TRY(hipMemcpyAsync(hc, cnt, sizeof hc, hipMemcpyDeviceToHost, s));
+    HP_TRY(hipMemcpyAsync(gerr.data(), B.gerr.p, sizeof(int32_t) * ng, hipMemcpyDeviceToHost, s));
+    HP_TRY(hipMemcpyAsync(target.data(), B.target.p, sizeof(float) * ng, hipMemcpyDeviceToHost, s));
+    HP_TRY(hipStreamSynchronize(s));
+    for (int64_t g = 0; g < ng; g++)
+        if (gerr[(size_t)g])
+            return set_error(HP_EINVAL, "hp_find_subfofs: group %lld has a non-finite coordinate "
+                             "(before or after norm_sd)", (long long)g);
+    if (hc[kSfErr])
+        return set_error(HP_EHIP, "hp_find_subfofs: scratch overflow (%d)", hc[kSfErr]);
+    const int32_t ne = hc[kSfEmit];
+    std::vector<int4> em((size_t)ne);
+    if (ne) {
+        HP_TRY(hipMemcpyAsync(em.data(), B.emit.p, sizeof(int4) * ne, hipMemcpyDeviceToHost, s));
+        HP_TRY(hipStreamSynchronize(s));
+    }
+    std::sort(em.begin(), em.end(), [](const int4 &a, const int4 &b) {
+        return a.x != b.x ? a.x < b.x : a.y < b.y;
+    });
+    *nsubs = ne;
+    if (ne > cap_subs)
+        return set_error(HP_EINVAL, "hp_find_subfofs: %d sub-FOFs do not fit the %lld given", ne,
+                         (long long)cap_subs);
+    for (int64_t g = 0; g < ng; g++) {
+        groups[g].target_r = target[(size_t)g];
+        groups[g].first_sub = 0;
+        groups[g].num_sub = 0;
+    }
+    for (int32_t i = 0; i < ne; i++) {
+        hp_fof_group &gg = groups[em[(size_t)i].x];
+        if (!gg.num_sub) gg.first_sub = i;
+        gg.num_sub++;
+        subs[i].first = gg.first + em[(size_t)i].y;
+        subs[i].num_p = em[(size_t)i].z;
+    }
     return HP_OK;
 }
 

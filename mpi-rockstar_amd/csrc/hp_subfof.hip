@@ -1,0 +1,820 @@
+// hp_subfof.hip -- the phase-space sub-FOF step of the halo finder's
+// _find_subs for a batch of FOF groups: _find_subfofs_better2
+// (src/groupies.c:287-314).  Per group:
+//   1. norm_sd (src/groupies.c:788-830): f64 mean, centring in float, the
+//      6x6 second moments of float products, calc_deviations
+//      (src/jacobi.c:137-175) on each 3x3 block, scaling by sigma_x and
+//      sigma_v;
+//   2. the 6D GROUPIES fast3tree (40 points per leaf, src/groupies.c:24-28)
+//      rebuilt in place: the tree order is the reference's exactly, because
+//      the sample and the linking both walk it;
+//   3. next-closest distances of the sample (src/fast3tree.c:953-1030);
+//   4. find_median_r (src/groupies.c:744-786) with the reference's rand_r
+//      pivots: the linking length;
+//   5. the linking (_find_subfofs_at_r, src/groupies.c:274-285 with
+//      src/fof.c): every pair closer than that length joins one component.
+//
+// The tree is built level by level over every group of the batch, one
+// workgroup per node that still splits.  fast3tree's in-place partition
+// (src/fast3tree.c:751-774) swaps each point above the midpoint with the
+// current last slot and examines the swapped-in point next.  Its result has
+// a closed form that a workgroup evaluates in parallel.  With the points
+// of a node at positions 0..n-1, bigs (above the midpoint) and smalls,
+// b_k the position of the k-th big from the left and r_k that of the k-th
+// small from the right (r_0 = n), K the number of k with b_k < r_k, and S
+// the number of points the loop reads from the left (b_{K+1} + 1 when that
+// big lies before r_K, else r_K):
+//   small at p < S: stays at p          big at p < S, c bigs before it: r_c - 1
+//   small at p >= S, c smalls after it: b_{c+1}        big at p >= S: p - 1
+// find_median_r's rad_partition is the same loop over [left, right - 1)
+// after the pivot moves to the end, so the selection uses the same form.
+
+#include "hp_internal.h"
+
+#pragma clang fp contract(off)
+
+namespace hp {
+
+constexpr int kSubTile = 2048;        // norm_sd tile (one wavefront)
+constexpr int kLeaf6 = 40;            // POINTS_PER_LEAF, src/groupies.c:25
+constexpr int kSfThreads = 256;      // split and selection workgroups
+constexpr int kSampleMax = 10000;     // MAX_PARTICLES_TO_SAMPLE, src/groupies.c:286
+
+struct SParticle {  // struct particle, src/particle.h:5-8
+    int64_t id;
+    float pos[6];
+};
+static_assert(sizeof(SParticle) == 32, "particle layout");
+
+struct SubGroup {
+    int64_t first;       // the caller's first particle
+    int32_t off, n;      // working slots [off, off + n)
+    int32_t top;         // a whole FOF group: sigma_x scaled (src/groupies.c:821-822)
+    int32_t nsample;     // distances drawn for find_median_r
+    int32_t sample_off;  // into the distance and drawn-position buffers
+    int32_t drawn;       // 1: sample positions from the drawn list, else 0..n-1
+    int32_t k;           // find_median_r's k
+    uint32_t seed;       // rand_r state when find_median_r starts
+    int32_t tile0, ntiles;
+};
+
+struct SubTile {
+    int32_t group, off;
+};
+
+struct Node6 {
+    float lo[6], hi[6];
+    int32_t start, n;    // working slots
+    int32_t div;         // split dimension, -1 for a leaf
+    int32_t left, right, parent, skip;  // skip: next node after this subtree, depth first
+    int32_t pad;
+};
+static_assert(sizeof(Node6) == 80, "node layout");
+
+enum SfCounter { kSfNodes = 0, kSfActive = 1, kSfEmit = 2, kSfErr = 3, kSfCounters = 4 };
+
+__device__ __forceinline__ float pcoord(const SParticle &p, int d) {
+    return d == 0 ? p.pos[0]
+                  : (d == 1 ? p.pos[1]
+                            : (d == 2 ? p.pos[2] : (d == 3 ? p.pos[3] : (d == 4 ? p.pos[4] : p.pos[5]))));
+}
+__device__ __forceinline__ float acoord(const float (&a)[6], int d) {
+    return d == 0 ? a[0] : (d == 1 ? a[1] : (d == 2 ? a[2] : (d == 3 ? a[3] : (d == 4 ? a[4] : a[5]))));
+}
+
+// glibc's rand_r: the reference draws its sample and its selection pivots
+// with it (src/groupies.c:300-304, 742).
+__host__ __device__ __forceinline__ int rand_r_glibc(uint32_t *seed) {
+    uint32_t next = *seed;
+    int32_t result;
+    next *= 1103515245u;
+    next += 12345u;
+    result = (int32_t)((next / 65536u) % 2048u);
+    next *= 1103515245u;
+    next += 12345u;
+    result <<= 10;
+    result ^= (int32_t)((next / 65536u) % 1024u);
+    next *= 1103515245u;
+    next += 12345u;
+    result <<= 10;
+    result ^= (int32_t)((next / 65536u) % 1024u);
+    *seed = next;
+    return result;
+}
+
+// ------------------------------------------------------------------ norm_sd
+
+// copy the group's particles into the working slots, f64 sums per tile,
+// non-finite input flagged (the reference's tree build asserts on it)
+__global__ __launch_bounds__(kWave) void k_sub_load(const SParticle *__restrict__ parts,
+                                                    const SubGroup *__restrict__ groups,
+                                                    const SubTile *__restrict__ tiles,
+                                                    SParticle *__restrict__ w,
+                                                    int32_t *__restrict__ slot_group,
+                                                    double *__restrict__ psum,
+                                                    int32_t *__restrict__ gerr) {
+    const SubTile tl = tiles[blockIdx.x];
+    const SubGroup g = groups[tl.group];
+    double s[6] = {0, 0, 0, 0, 0, 0};
+    bool bad = false;
+    const int32_t end = min(tl.off + kSubTile, g.n);
+    for (int32_t i = tl.off + lane_id(); i < end; i += kWave) {
+        const SParticle p = parts[g.first + i];
+        w[g.off + i] = p;
+        slot_group[g.off + i] = tl.group;
+        for (int j = 0; j < 6; j++) {
+            bad |= !__builtin_isfinite(p.pos[j]);
+            s[j] += (double)p.pos[j];
+        }
+    }
+    for (int j = 0; j < 6; j++) s[j] = wave_sum(s[j]);
+    if (ballot(bad) && lane_id() == 0) gerr[tl.group] = 1;
+    if (lane_id() == 0)
+        for (int j = 0; j < 6; j++) psum[6 * blockIdx.x + j] = s[j];
+}
+
+// src/groupies.c:797-802: the mean, tiles combined in order
+__global__ void k_sub_mean(int32_t ng, const SubGroup *__restrict__ groups,
+                           const double *__restrict__ psum, double *__restrict__ mean) {
+    const int32_t gi = blockIdx.x * blockDim.x + threadIdx.x;
+    if (gi >= ng) return;
+    const SubGroup g = groups[gi];
+    double s[6] = {0, 0, 0, 0, 0, 0};
+    for (int32_t t = g.tile0; t < g.tile0 + g.ntiles; t++)
+        for (int j = 0; j < 6; j++) s[j] += psum[6 * t + j];
+    for (int j = 0; j < 6; j++) mean[6 * gi + j] = g.n ? s[j] / (double)g.n : 0.0;
+}
+
+// src/groupies.c:806-813: centre in float, second moments of float products
+__global__ __launch_bounds__(kWave) void k_sub_centre(const SubGroup *__restrict__ groups,
+                                                      const SubTile *__restrict__ tiles,
+                                                      const double *__restrict__ mean,
+                                                      SParticle *__restrict__ w,
+                                                      double *__restrict__ pcorr) {
+    const SubTile tl = tiles[blockIdx.x];
+    const SubGroup g = groups[tl.group];
+    double mu[6], c[21];
+    for (int j = 0; j < 6; j++) mu[j] = mean[6 * tl.group + j];
+    for (int q = 0; q < 21; q++) c[q] = 0.0;
+    const int32_t end = min(tl.off + kSubTile, g.n);
+    for (int32_t i = tl.off + lane_id(); i < end; i += kWave) {
+        SParticle p = w[g.off + i];
+        for (int j = 0; j < 6; j++) p.pos[j] = (float)((double)p.pos[j] - mu[j]);
+        w[g.off + i] = p;
+        int q = 0;
+        for (int j = 0; j < 6; j++)
+            for (int k = j; k < 6; k++) {
+                const float prod = p.pos[j] * p.pos[k];
+                c[q++] += (double)prod;
+            }
+    }
+    for (int q = 0; q < 21; q++) c[q] = wave_sum(c[q]);
+    if (lane_id() == 0)
+        for (int q = 0; q < 21; q++) pcorr[21 * blockIdx.x + q] = c[q];
+}
+
+// src/jacobi.c:137-151: sqrt of the smallest eigenvalue, or of the trace
+__device__ double set_eig(const double (&in)[3][3]) {
+    Jacobi3 jb;
+    for (int i = 0; i < 3; i++)
+        for (int j = 0; j < 3; j++) jb.m[i][j] = in[i][j];
+    jb.run();
+    double r = jb.eig[0];
+    if (jb.eig[1] < r) r = jb.eig[1];
+    if (jb.eig[2] < r) r = jb.eig[2];
+    if (r <= 0) return sqrt(in[0][0] + in[1][1] + in[2][2]);
+    return sqrt(r);
+}
+
+// src/groupies.c:815-825 with calc_deviations (src/jacobi.c:154-175)
+__global__ void k_sub_sigma(int32_t ng, const SubGroup *__restrict__ groups,
+                            const double *__restrict__ pcorr, double scaling,
+                            double *__restrict__ sig) {
+    const int32_t gi = blockIdx.x * blockDim.x + threadIdx.x;
+    if (gi >= ng) return;
+    const SubGroup g = groups[gi];
+    double c[21];
+    for (int q = 0; q < 21; q++) c[q] = 0.0;
+    for (int32_t t = g.tile0; t < g.tile0 + g.ntiles; t++)
+        for (int q = 0; q < 21; q++) c[q] += pcorr[21 * t + q];
+    double m[6][6];
+    int q = 0;
+    for (int j = 0; j < 6; j++)
+        for (int k = j; k < 6; k++) {
+            m[j][k] = g.n ? c[q] / (double)g.n : 0.0;
+            m[k][j] = m[j][k];
+            q++;
+        }
+    double a[3][3], b[3][3];
+    for (int j = 0; j < 3; j++)
+        for (int k = 0; k < 3; k++) {
+            a[j][k] = m[j][k];
+            b[j][k] = m[j + 3][k + 3];
+        }
+    double sx = set_eig(a);
+    const double sv = set_eig(b);
+    if (g.top) sx *= scaling;
+    sig[2 * gi] = sx;
+    sig[2 * gi + 1] = sv;
+}
+
+// src/groupies.c:827-829, then the root box of each group's tree
+__global__ __launch_bounds__(kWave) void k_sub_scale(const SubGroup *__restrict__ groups,
+                                                     const SubTile *__restrict__ tiles,
+                                                     const double *__restrict__ sig,
+                                                     SParticle *__restrict__ w,
+                                                     float *__restrict__ pbox,
+                                                     int32_t *__restrict__ gerr) {
+    const SubTile tl = tiles[blockIdx.x];
+    const SubGroup g = groups[tl.group];
+    const double sx = sig[2 * tl.group], sv = sig[2 * tl.group + 1];
+    const bool scale = sx != 0.0 && sv != 0.0;
+    float lo[6], hi[6];
+    for (int j = 0; j < 6; j++) {
+        lo[j] = __builtin_inff();
+        hi[j] = -__builtin_inff();
+    }
+    bool bad = false;
+    const int32_t end = min(tl.off + kSubTile, g.n);
+    for (int32_t i = tl.off + lane_id(); i < end; i += kWave) {
+        SParticle p = w[g.off + i];
+        if (scale) {
+            for (int j = 0; j < 6; j++)
+                p.pos[j] = (float)((double)p.pos[j] / (j < 3 ? sx : sv));
+            w[g.off + i] = p;
+        }
+        for (int j = 0; j < 6; j++) {
+            bad |= !__builtin_isfinite(p.pos[j]);
+            lo[j] = p.pos[j] < lo[j] ? p.pos[j] : lo[j];
+            hi[j] = p.pos[j] > hi[j] ? p.pos[j] : hi[j];
+        }
+    }
+    for (int j = 0; j < 6; j++) {
+        lo[j] = wave_min(lo[j]);
+        hi[j] = wave_max(hi[j]);
+    }
+    if (ballot(bad) && lane_id() == 0) gerr[tl.group] = 1;
+    if (lane_id() == 0)
+        for (int j = 0; j < 6; j++) {
+            pbox[12 * blockIdx.x + j] = lo[j];
+            pbox[12 * blockIdx.x + 6 + j] = hi[j];
+        }
+}
+
+// root node of each group (node index = group index); roots that split go
+// on the first active list
+__global__ void k_sub_roots(int32_t ng, const SubGroup *__restrict__ groups,
+                            const float *__restrict__ pbox, Node6 *__restrict__ nodes,
+                            int32_t *__restrict__ active, int32_t *__restrict__ counters) {
+    const int32_t gi = blockIdx.x * blockDim.x + threadIdx.x;
+    if (gi >= ng) return;
+    const SubGroup g = groups[gi];
+    Node6 nd;
+    for (int j = 0; j < 6; j++) {
+        nd.lo[j] = __builtin_inff();
+        nd.hi[j] = -__builtin_inff();
+    }
+    for (int32_t t = g.tile0; t < g.tile0 + g.ntiles; t++)
+        for (int j = 0; j < 6; j++) {
+            const float l = pbox[12 * t + j], h = pbox[12 * t + 6 + j];
+            nd.lo[j] = l < nd.lo[j] ? l : nd.lo[j];
+            nd.hi[j] = h > nd.hi[j] ? h : nd.hi[j];
+        }
+    if (!g.n)
+        for (int j = 0; j < 6; j++) nd.lo[j] = nd.hi[j] = 0.0f;
+    nd.start = g.off;
+    nd.n = g.n;
+    nd.div = -1;
+    nd.left = nd.right = nd.parent = nd.skip = -1;
+    nd.pad = 0;
+    nodes[gi] = nd;
+    if (g.n > kLeaf6) active[atomicAdd(&counters[kSfActive], 1)] = gi;
+}
+
+// ------------------------------------------------------------- tree build
+
+// exclusive rank of b among the workgroup's threads, and the count
+__device__ __forceinline__ int block_rank(bool b, int *total, int *s_w) {
+    const uint64_t m = ballot(b);
+    const int w = threadIdx.x / kWave;
+    if (lane_id() == 0) s_w[w] = __popcll(m);
+    __syncthreads();
+    int off = 0, tot = 0;
+    for (int j = 0; j < kSfThreads / kWave; j++) {
+        off += j < w ? s_w[j] : 0;
+        tot += s_w[j];
+    }
+    __syncthreads();
+    *total = tot;
+    return off + rank_below(m);
+}
+
+// K and S of the closed form (file header) for one partition; b and r are
+// the positions of the bigs in order and of the smalls in order
+__device__ void partition_ks(const int32_t *__restrict__ bpos, const int32_t *__restrict__ spos,
+                             int32_t n, int32_t nbig, int32_t nsmall, int32_t *K, int32_t *S) {
+    // b_k = bpos[k - 1]; r_k = spos[nsmall - k]; b_k < r_k holds for k <= K
+    int32_t lo = 0, hi = min(nbig, nsmall);
+    while (lo < hi) {
+        const int32_t mid = (lo + hi + 1) / 2;
+        if (bpos[mid - 1] < spos[nsmall - mid]) lo = mid;
+        else hi = mid - 1;
+    }
+    const int32_t rK = lo == 0 ? n : spos[nsmall - lo];
+    *K = lo;
+    *S = (lo < nbig && bpos[lo] < rK) ? bpos[lo] + 1 : rK;
+}
+
+// destination of the element at p (class big, rank among its class) in the
+// closed form
+__device__ __forceinline__ int32_t partition_dest(int32_t p, bool big, int32_t rank, int32_t n,
+                                                  int32_t nsmall, int32_t S,
+                                                  const int32_t *__restrict__ bpos,
+                                                  const int32_t *__restrict__ spos) {
+    // the clamps never bind for a partition the form describes; they keep
+    // every read inside the node's lists whatever the input
+    if (p < S) {
+        if (!big) return p;
+        return (rank == 0 ? n : spos[max(nsmall - rank, 0)]) - 1;
+    }
+    if (!big) return bpos[min(max(nsmall - 1 - rank, 0), n - 1)];
+    return max(p - 1, 0);
+}
+
+// one level: every listed node splits (src/fast3tree.c:736-846), one
+// workgroup per node
+__global__ __launch_bounds__(kSfThreads) void k_sub_split(
+    Node6 *__restrict__ nodes, const int32_t *__restrict__ active, SParticle *__restrict__ w,
+    SParticle *__restrict__ w2, int32_t *__restrict__ rk, int32_t *__restrict__ bpos,
+    int32_t *__restrict__ spos, int32_t *__restrict__ next, int32_t *__restrict__ counters,
+    int32_t node_cap) {
+    __shared__ int s_w[kSfThreads / kWave];
+    __shared__ int32_t s_k[2];
+    __shared__ float s_box[kSfThreads / kWave][24];
+    const int32_t me = active[blockIdx.x];
+    const Node6 nd = nodes[me];
+    // widest dimension, the last one first (src/fast3tree.c:736-747)
+    int dim = 5;
+    float wd = nd.hi[5] - nd.lo[5];
+    for (int i = 0; i < 5; i++) {
+        const float e = nd.hi[i] - nd.lo[i];
+        if (e > wd) {
+            wd = e;
+            dim = i;
+        }
+    }
+    const float hd = acoord(nd.hi, dim), ld = acoord(nd.lo, dim);
+    if (hd == ld) return;  // no split, no reordering: stays a leaf
+    const float lim = 0.5 * (hd + ld);
+    const int32_t s0 = nd.start, n = nd.n;
+    // pass 1: classes, ranks within the class, positions of each class
+    int32_t nbig = 0;
+    for (int32_t base = 0; base < n; base += kSfThreads) {
+        const int32_t p = base + threadIdx.x;
+        const bool in = p < n;
+        const bool big = in && pcoord(w[s0 + p], dim) > lim;
+        int tot;
+        const int rb = block_rank(big, &tot, s_w);
+        if (in) {
+            const int32_t rank = big ? nbig + rb : p - (nbig + rb);
+            rk[s0 + p] = big ? ~rank : rank;
+            if (big) bpos[s0 + rank] = p;
+            else spos[s0 + rank] = p;
+        }
+        nbig += tot;
+    }
+    const int32_t nsmall = n - nbig;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        int32_t K, S;
+        partition_ks(bpos + s0, spos + s0, n, nbig, nsmall, &K, &S);
+        s_k[0] = S;
+    }
+    __syncthreads();
+    const int32_t S = s_k[0];
+    // pass 2: scatter, boxes of the two halves
+    float lo[2][6], hi[2][6];
+    for (int c = 0; c < 2; c++)
+        for (int j = 0; j < 6; j++) {
+            lo[c][j] = __builtin_inff();
+            hi[c][j] = -__builtin_inff();
+        }
+    for (int32_t p = threadIdx.x; p < n; p += kSfThreads) {
+        const int32_t r = rk[s0 + p];
+        const bool big = r < 0;
+        const int32_t d = partition_dest(p, big, big ? ~r : r, n, nsmall, S, bpos + s0, spos + s0);
+        const SParticle q = w[s0 + p];
+        w2[s0 + d] = q;
+        if (d < nsmall) {
+            for (int j = 0; j < 6; j++) {
+                lo[0][j] = q.pos[j] < lo[0][j] ? q.pos[j] : lo[0][j];
+                hi[0][j] = q.pos[j] > hi[0][j] ? q.pos[j] : hi[0][j];
+            }
+        } else {
+            for (int j = 0; j < 6; j++) {
+                lo[1][j] = q.pos[j] < lo[1][j] ? q.pos[j] : lo[1][j];
+                hi[1][j] = q.pos[j] > hi[1][j] ? q.pos[j] : hi[1][j];
+            }
+        }
+    }
+    for (int c = 0; c < 2; c++)
+        for (int j = 0; j < 6; j++) {
+            lo[c][j] = wave_min(lo[c][j]);
+            hi[c][j] = wave_max(hi[c][j]);
+        }
+    const int wv = threadIdx.x / kWave;
+    if (lane_id() == 0)
+        for (int c = 0; c < 2; c++)
+            for (int j = 0; j < 6; j++) {
+                s_box[wv][12 * c + j] = lo[c][j];
+                s_box[wv][12 * c + 6 + j] = hi[c][j];
+            }
+    __threadfence_block();
+    __syncthreads();
+    // pass 3: the reordered node back in place
+    for (int32_t p = threadIdx.x; p < n; p += kSfThreads) w[s0 + p] = w2[s0 + p];
+    if (threadIdx.x != 0) return;
+    Node6 me_n = nd;
+    if (nsmall == n || nsmall == 0) {  // src/fast3tree.c:804-808 (after the swaps)
+        nodes[me].div = -1;
+        return;
+    }
+    const int32_t l = atomicAdd(&counters[kSfNodes], 2);
+    if (l + 2 > node_cap) {
+        atomicOr(&counters[kSfErr], 1);
+        nodes[me].div = -1;
+        return;
+    }
+    Node6 L, R;
+    for (int j = 0; j < 6; j++) {
+        float a = __builtin_inff(), b = -__builtin_inff(), c = __builtin_inff(), e = -__builtin_inff();
+        for (int x = 0; x < kSfThreads / kWave; x++) {
+            a = s_box[x][j] < a ? s_box[x][j] : a;
+            b = s_box[x][6 + j] > b ? s_box[x][6 + j] : b;
+            c = s_box[x][12 + j] < c ? s_box[x][12 + j] : c;
+            e = s_box[x][18 + j] > e ? s_box[x][18 + j] : e;
+        }
+        L.lo[j] = a;
+        L.hi[j] = b;
+        R.lo[j] = c;
+        R.hi[j] = e;
+    }
+    L.start = s0;
+    L.n = nsmall;
+    R.start = s0 + nsmall;
+    R.n = n - nsmall;
+    L.div = R.div = -1;
+    L.left = L.right = R.left = R.right = -1;
+    L.parent = R.parent = me;
+    L.skip = l + 1;
+    R.skip = me_n.skip;
+    L.pad = R.pad = 0;
+    nodes[l] = L;
+    nodes[l + 1] = R;
+    me_n.div = dim;
+    me_n.left = l;
+    me_n.right = l + 1;
+    nodes[me] = me_n;
+    if (L.n > kLeaf6) next[atomicAdd(&counters[kSfActive], 1)] = l;
+    if (R.n > kLeaf6) next[atomicAdd(&counters[kSfActive], 1)] = l + 1;
+}
+
+// ---------------------------------------------------------- tree searches
+
+// src/fast3tree.c:294-309
+__device__ __forceinline__ bool box_apart(const Node6 &nd, const float (&c)[6], float r) {
+    float d = 0.0f, e;
+    const float r2 = r * r;
+#pragma unroll
+    for (int i = 0; i < 6; i++) {
+        if ((e = c[i] - nd.lo[i]) < 0) {
+            d += e * e;
+            if (d >= r2) return true;
+        } else if ((e = c[i] - nd.hi[i]) > 0) {
+            d += e * e;
+            if (d >= r2) return true;
+        }
+    }
+    return false;
+}
+
+__device__ __forceinline__ float dist6(const float (&c)[6], const SParticle &p) {
+    float d = 0.0f;
+    for (int j = 0; j < 6; j++) {
+        const float dx = c[j] - p.pos[j];
+        d += dx * dx;
+    }
+    return d;
+}
+
+// the child the reference visits first (src/fast3tree.c:980-984)
+__device__ __forceinline__ int32_t near_child(const Node6 &nd, const float (&c)[6]) {
+    const int d = nd.div;
+    const float mid = acoord(nd.lo, d) + acoord(nd.hi, d);
+    return (double)acoord(c, d) > 0.5 * (double)mid ? nd.right : nd.left;
+}
+
+// fast3tree_find_next_closest_distance (src/fast3tree.c:992-1030) for each
+// sampled particle, one per lane: the recursion of 956-987 as a walk over
+// parent links that keeps the reference's visiting order
+__global__ __launch_bounds__(kWave) void k_sub_next_closest(
+    const SubGroup *__restrict__ groups, const SubTile *__restrict__ qtiles,
+    const int32_t *__restrict__ drawn_pos, const SParticle *__restrict__ w,
+    const Node6 *__restrict__ nodes, float *__restrict__ dist) {
+    const SubTile tl = qtiles[blockIdx.x];
+    const SubGroup g = groups[tl.group];
+    const int32_t i = tl.off + lane_id();
+    if (i >= g.nsample) return;
+    const int32_t j = g.drawn ? drawn_pos[g.sample_off + i] : i;
+    float c[6];
+    {
+        const SParticle q = w[g.off + j];
+        for (int k = 0; k < 6; k++) c[k] = q.pos[k];
+    }
+    const int32_t root = tl.group;
+    int32_t nd = root;
+    while (nodes[nd].div >= 0) {
+        const Node6 &x = nodes[nd];
+        nd = acoord(c, x.div) <= acoord(nodes[x.left].hi, x.div) ? x.left : x.right;
+    }
+    while (nd != root) {
+        const int pd = nodes[nodes[nd].parent].div;
+        if (acoord(nodes[nd].lo, pd) != acoord(nodes[nd].hi, pd)) break;
+        nd = nodes[nd].parent;
+    }
+    const int32_t skip = nd;
+    float md = 0.0f;
+    {
+        const Node6 x = nodes[nd];
+        for (int k = 0; k < 6; k++) {
+            const float e = x.hi[k] - x.lo[k];
+            md += e * e;
+        }
+        for (int32_t p = 0; p < x.n; p++) {
+            const float d = dist6(c, w[x.start + p]);
+            if (d != 0.0f && d < md) md = d;
+        }
+    }
+    float r = (float)sqrt((double)md);
+    // depth-first from the root, near child first; every entry tests the
+    // box against the current r, a leaf narrows r
+    int32_t cur = root;
+    bool down = true;
+    for (;;) {
+        if (down) {
+            const Node6 x = nodes[cur];
+            if (cur == skip || box_apart(x, c, r)) {
+                down = false;
+            } else if (x.div < 0) {
+                float r2 = r * r;
+                for (int32_t p = 0; p < x.n; p++) {
+                    const float d = dist6(c, w[x.start + p]);
+                    if (d < r2) r2 = d;
+                }
+                r = (float)sqrt((double)r2);
+                down = false;
+            } else {
+                cur = near_child(x, c);
+                continue;
+            }
+        }
+        if (cur == root) break;
+        const int32_t par = nodes[cur].parent;
+        const Node6 px = nodes[par];
+        const int32_t nc = near_child(px, c);
+        if (cur == nc) {
+            cur = nc == px.left ? px.right : px.left;
+            down = true;
+        } else {
+            cur = par;
+        }
+    }
+    dist[g.sample_off + i] = r;
+}
+
+// ------------------------------------------------------ linking length
+
+// find_median_r (src/groupies.c:744-786) per group, one workgroup: the
+// reference's quickselect step by step, each rad_partition evaluated in
+// parallel with the closed form; buf holds the distances (a copy), buf2,
+// rk, bpos and spos are scratch of the same extent
+__global__ __launch_bounds__(kSfThreads) void k_sub_median(
+    const SubGroup *__restrict__ groups, float frac, float *__restrict__ buf,
+    float *__restrict__ buf2, int32_t *__restrict__ rk, int32_t *__restrict__ bpos,
+    int32_t *__restrict__ spos, float *__restrict__ target) {
+    __shared__ int s_w[kSfThreads / kWave];
+    __shared__ int32_t s_v[4];
+    __shared__ float s_piv;
+    const SubGroup g = groups[blockIdx.x];
+    const int32_t ns = g.nsample;
+    if (ns < 1) {
+        if (threadIdx.x == 0) target[blockIdx.x] = 0.0f;
+        return;
+    }
+    float *v = buf + g.sample_off;
+    float *v2 = buf2 + g.sample_off;
+    int32_t *rko = rk + g.sample_off, *bp = bpos + g.sample_off, *sp = spos + g.sample_off;
+    if (ns < 2) {
+        if (threadIdx.x == 0) target[blockIdx.x] = v[0];
+        return;
+    }
+    const int32_t k = g.k;
+    uint32_t seed = g.seed;  // thread 0's copy is the live one
+    int32_t left = 0, right = ns;
+    for (;;) {
+        if (threadIdx.x == 0) {
+            const float u = (float)(rand_r_glibc(&seed) % 2147483647) / (float)2147483647;
+            const float at_f = (float)left + u * (float)(right - left);
+            // u rounds to 1.0f for the top 63 of 2^31 draws; the reference
+            // then reads one past the range, here the last slot is taken
+            const int32_t at = min((int32_t)at_f, right - 1);
+            const float pv = v[at];
+            v[at] = v[right - 1];
+            v[right - 1] = pv;
+            s_piv = pv;
+        }
+        __threadfence_block();
+        __syncthreads();
+        const float pv = s_piv;
+        const int32_t m = right - 1 - left;  // the loop's range [left, right - 1)
+        int32_t nbig = 0;
+        for (int32_t base = 0; base < m; base += kSfThreads) {
+            const int32_t p = base + threadIdx.x;
+            const bool in = p < m;
+            const bool big = in && v[left + p] > pv;
+            int tot;
+            const int rb = block_rank(big, &tot, s_w);
+            if (in) {
+                const int32_t rank = big ? nbig + rb : p - (nbig + rb);
+                rko[left + p] = big ? ~rank : rank;
+                if (big) bp[left + rank] = p;
+                else sp[left + rank] = p;
+            }
+            nbig += tot;
+        }
+        const int32_t nsmall = m - nbig;
+        __threadfence_block();
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            int32_t K, S;
+            partition_ks(bp + left, sp + left, m, nbig, nsmall, &K, &S);
+            s_v[0] = S;
+        }
+        __syncthreads();
+        const int32_t S = s_v[0];
+        for (int32_t p = threadIdx.x; p < m; p += kSfThreads) {
+            const int32_t r = rko[left + p];
+            const bool big = r < 0;
+            v2[left + partition_dest(p, big, big ? ~r : r, m, nsmall, S, bp + left, sp + left)] =
+                v[left + p];
+        }
+        __threadfence_block();
+        __syncthreads();
+        for (int32_t p = threadIdx.x; p < m; p += kSfThreads) v[left + p] = v2[left + p];
+        __threadfence_block();
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            // rad_partition's end: the pivot moves to the first slot above it
+            const int32_t si = left + nsmall;
+            const float t = v[right - 1];
+            v[right - 1] = v[si];
+            v[si] = t;
+            int32_t done = 0;
+            if (k == si || v[left] == v[right - 1]) done = 1;
+            else if (k < si) right = si;
+            else left = si + 1;
+            s_v[1] = done;
+            s_v[2] = left;
+            s_v[3] = right;
+        }
+        __threadfence_block();
+        __syncthreads();
+        if (s_v[1]) break;
+        left = s_v[2];
+        right = s_v[3];
+    }
+    if (threadIdx.x == 0) target[blockIdx.x] = v[k];
+}
+
+// --------------------------------------------------------------- linking
+
+__global__ void k_sub_init(int32_t m, int32_t *__restrict__ up, uint8_t *__restrict__ has,
+                           int32_t *__restrict__ csize) {
+    const int32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= m) return;
+    up[i] = i;
+    has[i] = 0;
+    csize[i] = 0;
+}
+
+__device__ __forceinline__ int32_t uf_find(int32_t *up, int32_t x) {
+    for (;;) {
+        const int32_t y = __hip_atomic_load(up + x, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        if (y == x) return x;
+        x = y;
+    }
+}
+
+// hook the larger root under the smaller: a component's root is its
+// smallest slot
+__device__ __forceinline__ void uf_unite(int32_t *up, int32_t a, int32_t b) {
+    for (;;) {
+        a = uf_find(up, a);
+        b = uf_find(up, b);
+        if (a == b) return;
+        if (a > b) {
+            const int32_t t = a;
+            a = b;
+            b = t;
+        }
+        if (atomicCAS(up + b, b, a) == b) return;
+    }
+}
+
+// _find_subfofs_at_r (src/groupies.c:274-285): each particle's sphere search
+// over the later slots (fast3tree_find_sphere_skip, src/fast3tree.c:419-465)
+// and the union of every pair closer than the group's length
+__global__ __launch_bounds__(kWave) void k_sub_link(int32_t m, const int32_t *__restrict__ slot_group,
+                                                    const float *__restrict__ target,
+                                                    const SParticle *__restrict__ w,
+                                                    const Node6 *__restrict__ nodes,
+                                                    int32_t *__restrict__ up,
+                                                    uint8_t *__restrict__ has) {
+    const int32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= m) return;
+    const int32_t gi = slot_group[i];
+    const float r = target[gi];
+    const float r2 = r * r;
+    float c[6];
+    {
+        const SParticle q = w[i];
+        for (int k = 0; k < 6; k++) c[k] = q.pos[k];
+    }
+    bool any = false;
+    int32_t cur = gi;
+    while (cur >= 0) {
+        const Node6 x = nodes[cur];
+        if (x.start + x.n <= i || box_apart(x, c, r)) {
+            cur = x.skip;
+            continue;
+        }
+        if (x.div >= 0) {
+            cur = x.left;
+            continue;
+        }
+        for (int32_t j = max(x.start, i + 1); j < x.start + x.n; j++) {
+            if (!(dist6(c, w[j]) < r2)) continue;
+            any = true;
+            has[j] = 1;
+            uf_unite(up, i, j);
+        }
+        cur = x.skip;
+    }
+    if (any) has[i] = 1;
+}
+
+// sort keys: a group's particles with no neighbour first, then each
+// component in the order of its smallest slot; slots keep their order
+// inside each key (the sort is stable)
+__global__ void k_sub_keys(int32_t m, const int32_t *__restrict__ slot_group,
+                           const SubGroup *__restrict__ groups, int32_t *__restrict__ up,
+                           const uint8_t *__restrict__ has, int32_t *__restrict__ csize,
+                           uint64_t *__restrict__ keys, int32_t *__restrict__ vals) {
+    const int32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= m) return;
+    int32_t key;
+    if (has[i]) {
+        const int32_t r = uf_find(up, i);
+        atomicAdd(csize + r, 1);
+        key = r + 1;
+    } else {
+        key = groups[slot_group[i]].off;
+    }
+    keys[i] = (uint64_t)(uint32_t)key;
+    vals[i] = i;
+}
+
+// the particles to the caller's array in key order; one sub-FOF per
+// component of at least min_p particles
+__global__ void k_sub_emit(int32_t m, const int32_t *__restrict__ slot_group,
+                           const SubGroup *__restrict__ groups, const uint64_t *__restrict__ skeys,
+                           const int32_t *__restrict__ svals, const int32_t *__restrict__ csize,
+                           const SParticle *__restrict__ w, SParticle *__restrict__ parts,
+                           int64_t min_p, int4 *__restrict__ subs, int32_t cap,
+                           int32_t *__restrict__ counters) {
+    const int32_t q = blockIdx.x * blockDim.x + threadIdx.x;
+    if (q >= m) return;
+    const int32_t gi = slot_group[q];
+    const SubGroup g = groups[gi];
+    parts[g.first + (q - g.off)] = w[svals[q]];
+    const int32_t key = (int32_t)skeys[q];
+    if (key == g.off) return;  // no neighbour
+    if (q > g.off && (int32_t)skeys[q - 1] == key) return;
+    const int32_t size = csize[key - 1];
+    if (size < min_p) return;
+    const int32_t e = atomicAdd(&counters[kSfEmit], 1);
+    if (e < cap) subs[e] = make_int4(gi, q - g.off, size, 0);
+    else atomicOr(&counters[kSfErr], 2);
+}
+
+}  // namespace hp

@@ -87,7 +87,8 @@ _LIB = None
 
 EXPORTS = ("compute_potential", "compute_kinetic_energy", "hp_plan_create", "hp_plan_run",
            "hp_plan_potential", "hp_plan_stats", "hp_plan_destroy", "hp_plan_tune",
-           "hp_mass_definitions", "hp_plan_props", "hp_gather_calls", "hp_bound_pass_host",
+           "hp_mass_definitions", "hp_plan_props", "hp_gather_calls", "hp_find_subfofs",
+           "hp_bound_pass_host",
            "hp_debug_tree", "hp_last_error", "hp_set_error_handler")
 GLOBALS = ("FORCE_RES", "PARTICLE_MASS", "Om", "Ol", "W0", "WA", "SCALE_NOW")
 
@@ -114,6 +115,9 @@ def lib() -> C.CDLL:
         L.hp_mass_definitions.argtypes = [C.POINTER(C.c_char_p), C.POINTER(MassDefs)]
         L.hp_gather_calls.argtypes = [C.c_void_p, C.c_int64, C.c_void_p, C.c_int64, C.c_void_p,
                                       C.c_int64, C.c_double, C.c_void_p, C.c_int64, C.c_void_p,
+                                      C.POINTER(C.c_int64)]
+        L.hp_find_subfofs.argtypes = [C.c_void_p, C.c_int64, C.c_void_p, C.c_int64,
+                                      C.POINTER(SubfofConfig), C.c_void_p, C.c_int64,
                                       C.POINTER(C.c_int64)]
         L.hp_plan_props.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p, C.POINTER(MassDefs),
                                     C.c_void_p, C.c_void_p]
@@ -194,6 +198,40 @@ def gather_calls(tree: np.ndarray, base, d_particles: int, nparticles: int, d_po
                                  C.c_void_p(d_particles), nparticles, float(include_host_ratio),
                                  C.c_void_p(d_po), cap_records, _ptr(calls), C.byref(n)))
     return calls, n.value
+
+
+class SubfofConfig(C.Structure):
+    """hp_subfof_config: FOF_FRACTION, INITIAL_METRIC_SCALING,
+    MIN_HALO_PARTICLES, EXACT_LL_CALC (src/config.template.h)."""
+
+    _fields_ = [("fof_fraction", C.c_double), ("initial_metric_scaling", C.c_double),
+                ("min_halo_particles", C.c_int64), ("exact_ll_calc", C.c_int32),
+                ("pad", C.c_int32)]
+
+
+FOF_GROUP_DTYPE = np.dtype({"names": ["first", "num_p", "top", "target_r", "first_sub", "num_sub"],
+                            "formats": ["<i8", "<i8", "<i4", "<f4", "<i8", "<i8"],
+                            "offsets": [0, 8, 16, 20, 24, 32], "itemsize": 40})
+SUBFOF_DTYPE = np.dtype([("first", "<i8"), ("num_p", "<i8")])
+
+
+def find_subfofs(d_particles: int, nparticles: int, groups: np.ndarray, fof_fraction=0.7,
+                 min_halo_particles=10, exact_ll_calc=0, initial_metric_scaling=1.0):
+    """hp_find_subfofs: _find_subfofs_better2 (src/groupies.c:287-314) for
+    every group (rows with first, num_p, top) of the particles at device
+    address d_particles, in place.  Returns (groups with target_r, first_sub
+    and num_sub filled, the sub-FOFs as SUBFOF_DTYPE rows)."""
+    g = np.zeros(len(groups), FOF_GROUP_DTYPE)
+    for f in ("first", "num_p"):
+        g[f] = groups[f]
+    g["top"] = groups["top"] if "top" in (groups.dtype.names or ()) else 1
+    cap = int(np.sum(g["num_p"]) // 2 + len(g) + 1)
+    subs = np.zeros(cap, SUBFOF_DTYPE)
+    n = C.c_int64()
+    cfg = SubfofConfig(fof_fraction, initial_metric_scaling, min_halo_particles, exact_ll_calc, 0)
+    _check(lib().hp_find_subfofs(C.c_void_p(d_particles), nparticles, _ptr(g), g.shape[0],
+                                 C.byref(cfg), _ptr(subs), cap, C.byref(n)))
+    return g, subs[:n.value]
 
 
 def mass_definitions(defs=None) -> MassDefs:
