@@ -180,6 +180,79 @@ def run_gather(args):
     return res
 
 
+def run_subfof(args):
+    """The phase-space sub-FOF step (hp_find_subfofs, replacing
+    _find_subfofs_better2, src/groupies.c:287-314) over seeded FOF groups
+    (hosts with sub-halos distinct in phase space).  Each pass starts from
+    the pristine particles (restored device to device, untimed) because the
+    step normalises and reorders them in place.  Timed per call of the entry
+    point (synchronous: host sample draws, table upload, the level-by-level
+    tree build with its per-level count readback, results back); the
+    reference's own _find_subfofs_better2 beside it (oracle/_ref, OpenMP over
+    groups)."""
+    import numpy as np
+
+    import halopot
+    from halopot import catalog
+    from halopot.hipmem import DeviceArray, synchronize
+
+    fg = catalog.fof_groups(n_groups=args.subfof_groups)
+    n = int(fg.particles.shape[0])
+    d0 = DeviceArray.from_numpy(fg.particles)
+    d1 = DeviceArray.zeros(n * 32)
+    nbytes = n * 32
+    import ctypes as C
+
+    from halopot.hipmem import hip
+
+    def restore():  # device to device (hipMemcpyDeviceToDevice = 3)
+        assert hip().hipMemcpy(C.c_void_p(d1.address), C.c_void_p(d0.address), nbytes, 3) == 0
+
+    restore()
+    g, subs = halopot.find_subfofs(d1.address, n, fg.groups)  # warm
+    reps = max(5, args.steps // 2)
+    busy = 0.0
+    for _ in range(reps):
+        restore()
+        synchronize()
+        t0 = time.perf_counter()
+        g, subs = halopot.find_subfofs(d1.address, n, fg.groups)
+        busy += time.perf_counter() - t0
+    e = busy / reps
+    d0.free()
+    d1.free()
+    res = {"ms_per_pass": 1e3 * e, "value": n / e, "unit": "particles/s through the sub-FOF step",
+           "particles": n, "groups": int(fg.groups.shape[0]), "subfofs": int(subs.shape[0]),
+           "largest_group": int(fg.groups["num_p"].max()),
+           "workload": f"{args.subfof_groups} FOF groups (hosts n ~ n^-1.9 on [200, 50000] with "
+                       f"sub-halos to depth 3), seed 0x5EED0007"}
+    log(f"sub-FOF step: {1e3 * e:.3f} ms per pass, {n} particles in {fg.groups.shape[0]} groups, "
+        f"{subs.shape[0]} sub-FOFs, {n / e:.4g} particles/s")
+    if not args.no_cpu:
+        sys.path.insert(0, str(ROOT / "oracle"))
+        import oracle
+
+        if oracle.ReferenceProps.available():
+            R = oracle.ReferenceProps()
+            R.set_subfof_config()
+            nt = cpu_threads()
+            passes, busy = 0, 0.0
+            while busy < 2.0 or passes == 0:
+                parts = fg.particles.copy()
+                t0 = time.perf_counter()
+                ns = R.subfof_batch(parts, fg.groups, nthreads=nt)
+                busy += time.perf_counter() - t0
+                passes += 1
+            res["cpu_baseline"] = {"value": n * passes / busy, "unit": res["unit"], "cores": nt,
+                                   "kind": "reference", "ms_per_pass": 1e3 * busy / passes,
+                                   "sample": f"whole set x {passes} pass(es), {busy:.1f} s on "
+                                             f"{nt} threads (OpenMP dynamic over groups)"}
+            res["cpu_subfofs"] = int(ns)
+            log(f"sub-FOF step on the CPU (reference, {nt} threads): "
+                f"{1e3 * busy / passes:.1f} ms per pass, {ns} sub-FOFs")
+    return res
+
+
 def run_props(plan, cat, d_po, d_h, args):
     """The halo property pass (hp_plan_props, src/properties.c:384-510) on
     the last step's results: its own time per pass over the kept records and,
@@ -274,6 +347,8 @@ def main() -> int:
     ap.add_argument("--no-props", action="store_true", help="skip the halo property pass leg")
     ap.add_argument("--no-gather", action="store_true", help="skip the record assembly leg")
     ap.add_argument("--gather-groups", type=int, default=4000)
+    ap.add_argument("--no-subfof", action="store_true", help="skip the sub-FOF leg")
+    ap.add_argument("--subfof-groups", type=int, default=4000)
     ap.add_argument("--cpu-seconds", type=float, default=8.0)
     args = ap.parse_args()
     if args.gpus != 1:
@@ -391,6 +466,8 @@ def main() -> int:
         out["halo_props"] = props
     if not args.no_gather:
         out["record_assembly"] = run_gather(args)
+    if not args.no_subfof:
+        out["subfof"] = run_subfof(args)
 
     if not args.no_per_halo:
         plan_p = halopot.Plan(cat.halos, halopot.options(accumulate=acc, per_call=1))

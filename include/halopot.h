@@ -242,7 +242,9 @@ typedef struct hp_subfof_config {   /* src/config.template.h:96,112,114,118 */
     double initial_metric_scaling;  /* INITIAL_METRIC_SCALING (1) */
     int64_t min_halo_particles;     /* MIN_HALO_PARTICLES (10) */
     int32_t exact_ll_calc;          /* EXACT_LL_CALC (0) */
-    int32_t pad;
+    int32_t ordered_search;         /* 1: every next-closest search walks the tree in the
+                                       reference's order (a check of the default path,
+                                       which gives the same distances) */
 } hp_subfof_config;
 
 /* Replaces _find_subfofs_better2(f, FOF_FRACTION, fofinfo)

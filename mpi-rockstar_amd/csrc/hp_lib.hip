@@ -181,10 +181,10 @@ struct DevBuf {
 
 // Device scratch of the sub-FOF step (hp_find_subfofs).
 struct SubBufs {
-    DevBuf groups, tiles, qtiles, drawn, psum, mean, pcorr, sig, pbox, gerr, w, w2, rk, bpos, spos,
+    DevBuf groups, tiles, qtiles, ltiles, drawn, qidx, psum, mean, pcorr, sig, pbox, gerr, w, w2, rk, bpos, spos,
         nodes, act0, act1, counters, slot_group, dist, sel, sel2, target, up, has, csize, emit;
     void release() {
-        for (DevBuf *b : {&groups, &tiles, &qtiles, &drawn, &psum, &mean, &pcorr, &sig, &pbox,
+        for (DevBuf *b : {&groups, &tiles, &qtiles, &ltiles, &drawn, &qidx, &psum, &mean, &pcorr, &sig, &pbox,
                           &gerr, &w, &w2, &rk, &bpos, &spos, &nodes, &act0, &act1, &counters,
                           &slot_group, &dist, &sel, &sel2, &target, &up, &has, &csize, &emit})
             b->release();
@@ -1432,7 +1432,7 @@ __attribute__((visibility("default"))) int hp_find_subfofs(struct particle *d_pa
     if (nparticles < 0 || ngroups < 0 || cap_subs < 0 || !nsubs || (ngroups > 0 && !groups) ||
         (nparticles > 0 && !d_particles) || (cap_subs > 0 && !subs))
         return set_error(HP_EINVAL, "hp_find_subfofs: bad arguments");
-    const hp_subfof_config dflt = {0.7, 1.0, 10, 0, 0};
+    const hp_subfof_config dflt = {0.7, 1.0, 10, 0, 0};  // the reference's defaults
     const hp_subfof_config cfg = config ? *config : dflt;
     *nsubs = 0;
     if (!(cfg.fof_fraction >= 0.0 && cfg.fof_fraction < 1.0) || cfg.min_halo_particles < 0)
@@ -1441,8 +1441,8 @@ __attribute__((visibility("default"))) int hp_find_subfofs(struct particle *d_pa
     // host tables: working slots, norm_sd tiles, the sample of each group
     const float thresh = (float)cfg.fof_fraction;
     std::vector<SubGroup> G((size_t)ngroups);
-    std::vector<SubTile> tiles, qtiles;
-    std::vector<int32_t> drawn;
+    std::vector<SubTile> tiles, qtiles, ltiles;
+    std::vector<int32_t> drawn, qidx;
     std::vector<std::pair<int64_t, int64_t>> ranges;
     int64_t M = 0, S = 0;
     for (int64_t g = 0; g < ngroups; g++) {
@@ -1466,17 +1466,35 @@ __attribute__((visibility("default"))) int hp_find_subfofs(struct particle *d_pa
         sg.nsample = (int32_t)ns;
         sg.sample_off = (int32_t)S;
         sg.drawn = ns != n;
-        drawn.resize((size_t)(S + ns));
-        for (int64_t i = 0; i < ns; i++) {
-            int64_t j = i;
-            if (sg.drawn) {
-                j = rand_r_glibc(&seed);
+        // rand_r draws only for the groups that draw (the others search
+        // 0..n-1), kept sorted by position so that a wavefront's queries are
+        // neighbours in the tree; qidx remembers each draw's place in the
+        // sample, which is the order find_median_r sees
+        sg.draw_off = (int32_t)drawn.size();
+        if (sg.drawn) {
+            std::vector<int32_t> js((size_t)ns);
+            for (int64_t i = 0; i < ns; i++) {
+                int64_t j = rand_r_glibc(&seed);
                 j <<= 31;
                 j += rand_r_glibc(&seed);
                 j %= n;
+                js[(size_t)i] = (int32_t)j;
             }
-            drawn[(size_t)(S + i)] = (int32_t)j;
+            // counting sort by position, stable in draw order
+            std::vector<int32_t> start((size_t)n + 1, 0);
+            for (int64_t i = 0; i < ns; i++) start[(size_t)js[(size_t)i] + 1]++;
+            for (int64_t j = 0; j < n; j++) start[(size_t)j + 1] += start[(size_t)j];
+            const size_t base = drawn.size();
+            drawn.resize(base + (size_t)ns);
+            qidx.resize(base + (size_t)ns);
+            for (int64_t i = 0; i < ns; i++) {
+                const int32_t at = start[(size_t)js[(size_t)i]]++;
+                drawn[base + (size_t)at] = js[(size_t)i];
+                qidx[base + (size_t)at] = (int32_t)i;
+            }
         }
+        for (int64_t off = 0; off < n; off += kWave)
+            ltiles.push_back(SubTile{(int32_t)g, (int32_t)off});
         sg.seed = seed;
         sg.k = (int32_t)(int64_t)((float)ns * thresh);  // src/groupies.c:771
         if (ns > 0 && sg.k >= ns)
@@ -1535,7 +1553,8 @@ __attribute__((visibility("default"))) int hp_find_subfofs(struct particle *d_pa
     HP_ENSURE(e->hist, sizeof(int32_t) * kRadix * (ntile_sort + 1));
     HP_ENSURE(e->hsum, sizeof(int32_t) * (kRadix * (ntile_sort + 1) / kScanBlock + 2));
     if ((r = e->upload(B.groups, G)) || (r = e->upload(B.tiles, tiles)) ||
-        (r = e->upload(B.qtiles, qtiles)) || (r = e->upload(B.drawn, drawn)))
+        (r = e->upload(B.qtiles, qtiles)) || (r = e->upload(B.drawn, drawn)) ||
+        (r = e->upload(B.qidx, qidx)) || (r = e->upload(B.ltiles, ltiles)))
         return r;
     hipStream_t s = e->stream;
     const SubGroup *dg = B.groups.as<SubGroup>();
@@ -1580,18 +1599,36 @@ __attribute__((visibility("default"))) int hp_find_subfofs(struct particle *d_pa
         if (level > 4 * 64) return set_error(HP_EHIP, "hp_find_subfofs: tree too deep");
         k_fill_i32<<<1, 1, 0, s>>>(cnt + kSfActive, 0, 1);
         HP_LAUNCHED();
-        k_sub_split<<<(uint32_t)na, kSfThreads, 0, s>>>(
-            B.nodes.as<Node6>(), act, B.w.as<SParticle>(), B.w2.as<SParticle>(),
-            B.rk.as<int32_t>(), B.bpos.as<int32_t>(), B.spos.as<int32_t>(), nxt, cnt,
-            (int32_t)node_cap);
+        // the first levels hold few, large nodes: wider workgroups
+        if (level < 6)
+            k_sub_split<1024><<<(uint32_t)na, 1024, 0, s>>>(
+                B.nodes.as<Node6>(), act, B.w.as<SParticle>(), B.w2.as<SParticle>(),
+                B.rk.as<int32_t>(), B.bpos.as<int32_t>(), B.spos.as<int32_t>(), nxt, cnt,
+                (int32_t)node_cap);
+        else
+            k_sub_split<kSfThreads><<<(uint32_t)na, kSfThreads, 0, s>>>(
+                B.nodes.as<Node6>(), act, B.w.as<SParticle>(), B.w2.as<SParticle>(),
+                B.rk.as<int32_t>(), B.bpos.as<int32_t>(), B.spos.as<int32_t>(), nxt, cnt,
+                (int32_t)node_cap);
         HP_LAUNCHED();
         std::swap(act, nxt);
+    }
+    {
+        int32_t nn = 0;
+        HP_TRY(hipMemcpyAsync(&nn, cnt + kSfNodes, sizeof nn, hipMemcpyDeviceToHost, s));
+        HP_TRY(hipStreamSynchronize(s));
+        nn = (int32_t)std::min<int64_t>(nn, node_cap);
+        if (nn > 0) {
+            k_sub_collect<<<(uint32_t)nn, kWave, 0, s>>>(B.nodes.as<Node6>(), B.w.as<SParticle>(),
+                                                         B.w2.as<SParticle>());
+            HP_LAUNCHED();
+        }
     }
     // the linking length of each group
     if (!qtiles.empty()) {
         k_sub_next_closest<<<(uint32_t)qtiles.size(), kWave, 0, s>>>(
-            dg, B.qtiles.as<SubTile>(), B.drawn.as<int32_t>(), B.w.as<SParticle>(),
-            B.nodes.as<Node6>(), B.dist.as<float>());
+            dg, B.qtiles.as<SubTile>(), B.drawn.as<int32_t>(), B.qidx.as<int32_t>(),
+            B.w.as<SParticle>(), B.nodes.as<Node6>(), B.dist.as<float>(), cfg.ordered_search);
         HP_LAUNCHED();
         HP_TRY(hipMemcpyAsync(B.sel.p, B.dist.p, sizeof(float) * S, hipMemcpyDeviceToDevice, s));
     }
@@ -1606,10 +1643,9 @@ __attribute__((visibility("default"))) int hp_find_subfofs(struct particle *d_pa
         k_sub_init<<<Engine::blocks(M, 256), 256, 0, s>>>(m, B.up.as<int32_t>(), B.has.as<uint8_t>(),
                                                   B.csize.as<int32_t>());
         HP_LAUNCHED();
-        k_sub_link<<<Engine::blocks(M, kWave), kWave, 0, s>>>(m, B.slot_group.as<int32_t>(),
-                                                      B.target.as<float>(), B.w.as<SParticle>(),
-                                                      B.nodes.as<Node6>(), B.up.as<int32_t>(),
-                                                      B.has.as<uint8_t>());
+        k_sub_link<<<(uint32_t)ltiles.size(), kWave, 0, s>>>(
+            dg, B.ltiles.as<SubTile>(), B.target.as<float>(), B.w.as<SParticle>(),
+            B.nodes.as<Node6>(), B.up.as<int32_t>(), B.has.as<uint8_t>());
         HP_LAUNCHED();
         k_sub_keys<<<Engine::blocks(M, 256), 256, 0, s>>>(m, B.slot_group.as<int32_t>(), dg,
                                                   B.up.as<int32_t>(), B.has.as<uint8_t>(),

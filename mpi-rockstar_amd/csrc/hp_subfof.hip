@@ -56,6 +56,8 @@ struct SubGroup {
     int32_t k;           // find_median_r's k
     uint32_t seed;       // rand_r state when find_median_r starts
     int32_t tile0, ntiles;
+    int32_t draw_off;    // into the drawn-position lists (drawn groups only)
+    int32_t pad;
 };
 
 struct SubTile {
@@ -67,7 +69,7 @@ struct Node6 {
     int32_t start, n;    // working slots
     int32_t div;         // split dimension, -1 for a leaf
     int32_t left, right, parent, skip;  // skip: next node after this subtree, depth first
-    int32_t pad;
+    int32_t where;       // during the build: the buffer holding the node's points (0 or 1)
 };
 static_assert(sizeof(Node6) == 80, "node layout");
 
@@ -286,7 +288,7 @@ __global__ void k_sub_roots(int32_t ng, const SubGroup *__restrict__ groups,
     nd.n = g.n;
     nd.div = -1;
     nd.left = nd.right = nd.parent = nd.skip = -1;
-    nd.pad = 0;
+    nd.where = 0;
     nodes[gi] = nd;
     if (g.n > kLeaf6) active[atomicAdd(&counters[kSfActive], 1)] = gi;
 }
@@ -294,13 +296,14 @@ __global__ void k_sub_roots(int32_t ng, const SubGroup *__restrict__ groups,
 // ------------------------------------------------------------- tree build
 
 // exclusive rank of b among the workgroup's threads, and the count
+template <int NT = kSfThreads>
 __device__ __forceinline__ int block_rank(bool b, int *total, int *s_w) {
     const uint64_t m = ballot(b);
     const int w = threadIdx.x / kWave;
     if (lane_id() == 0) s_w[w] = __popcll(m);
     __syncthreads();
     int off = 0, tot = 0;
-    for (int j = 0; j < kSfThreads / kWave; j++) {
+    for (int j = 0; j < NT / kWave; j++) {
         off += j < w ? s_w[j] : 0;
         tot += s_w[j];
     }
@@ -342,17 +345,22 @@ __device__ __forceinline__ int32_t partition_dest(int32_t p, bool big, int32_t r
 }
 
 // one level: every listed node splits (src/fast3tree.c:736-846), one
-// workgroup per node
-__global__ __launch_bounds__(kSfThreads) void k_sub_split(
-    Node6 *__restrict__ nodes, const int32_t *__restrict__ active, SParticle *__restrict__ w,
-    SParticle *__restrict__ w2, int32_t *__restrict__ rk, int32_t *__restrict__ bpos,
+// workgroup per node.  A node's points are read from the buffer the last
+// level wrote them to and its halves go to the other one.
+template <int NT>
+__global__ __launch_bounds__(NT) void k_sub_split(
+    Node6 *__restrict__ nodes, const int32_t *__restrict__ active, SParticle *__restrict__ wa,
+    SParticle *__restrict__ wb, int32_t *__restrict__ rk, int32_t *__restrict__ bpos,
     int32_t *__restrict__ spos, int32_t *__restrict__ next, int32_t *__restrict__ counters,
     int32_t node_cap) {
-    __shared__ int s_w[kSfThreads / kWave];
+    constexpr int NW = NT / kWave;
+    __shared__ int s_w[NW];
     __shared__ int32_t s_k[2];
-    __shared__ float s_box[kSfThreads / kWave][24];
+    __shared__ float s_box[NW][24];
     const int32_t me = active[blockIdx.x];
     const Node6 nd = nodes[me];
+    const SParticle *__restrict__ w = nd.where ? wb : wa;
+    SParticle *__restrict__ w2 = nd.where ? wa : wb;
     // widest dimension, the last one first (src/fast3tree.c:736-747)
     int dim = 5;
     float wd = nd.hi[5] - nd.lo[5];
@@ -364,17 +372,17 @@ __global__ __launch_bounds__(kSfThreads) void k_sub_split(
         }
     }
     const float hd = acoord(nd.hi, dim), ld = acoord(nd.lo, dim);
-    if (hd == ld) return;  // no split, no reordering: stays a leaf
+    if (hd == ld) return;  // no split, no reordering: stays a leaf where it is
     const float lim = 0.5 * (hd + ld);
     const int32_t s0 = nd.start, n = nd.n;
     // pass 1: classes, ranks within the class, positions of each class
     int32_t nbig = 0;
-    for (int32_t base = 0; base < n; base += kSfThreads) {
+    for (int32_t base = 0; base < n; base += NT) {
         const int32_t p = base + threadIdx.x;
         const bool in = p < n;
         const bool big = in && pcoord(w[s0 + p], dim) > lim;
         int tot;
-        const int rb = block_rank(big, &tot, s_w);
+        const int rb = block_rank<NT>(big, &tot, s_w);
         if (in) {
             const int32_t rank = big ? nbig + rb : p - (nbig + rb);
             rk[s0 + p] = big ? ~rank : rank;
@@ -392,14 +400,14 @@ __global__ __launch_bounds__(kSfThreads) void k_sub_split(
     }
     __syncthreads();
     const int32_t S = s_k[0];
-    // pass 2: scatter, boxes of the two halves
+    // pass 2: scatter to the other buffer, boxes of the two halves
     float lo[2][6], hi[2][6];
     for (int c = 0; c < 2; c++)
         for (int j = 0; j < 6; j++) {
             lo[c][j] = __builtin_inff();
             hi[c][j] = -__builtin_inff();
         }
-    for (int32_t p = threadIdx.x; p < n; p += kSfThreads) {
+    for (int32_t p = threadIdx.x; p < n; p += NT) {
         const int32_t r = rk[s0 + p];
         const bool big = r < 0;
         const int32_t d = partition_dest(p, big, big ? ~r : r, n, nsmall, S, bpos + s0, spos + s0);
@@ -429,26 +437,26 @@ __global__ __launch_bounds__(kSfThreads) void k_sub_split(
                 s_box[wv][12 * c + j] = lo[c][j];
                 s_box[wv][12 * c + 6 + j] = hi[c][j];
             }
-    __threadfence_block();
     __syncthreads();
-    // pass 3: the reordered node back in place
-    for (int32_t p = threadIdx.x; p < n; p += kSfThreads) w[s0 + p] = w2[s0 + p];
     if (threadIdx.x != 0) return;
     Node6 me_n = nd;
+    me_n.where = 1 - nd.where;  // its points, reordered, are in the other buffer now
     if (nsmall == n || nsmall == 0) {  // src/fast3tree.c:804-808 (after the swaps)
-        nodes[me].div = -1;
+        me_n.div = -1;
+        nodes[me] = me_n;
         return;
     }
     const int32_t l = atomicAdd(&counters[kSfNodes], 2);
     if (l + 2 > node_cap) {
         atomicOr(&counters[kSfErr], 1);
-        nodes[me].div = -1;
+        me_n.div = -1;
+        nodes[me] = me_n;
         return;
     }
     Node6 L, R;
     for (int j = 0; j < 6; j++) {
         float a = __builtin_inff(), b = -__builtin_inff(), c = __builtin_inff(), e = -__builtin_inff();
-        for (int x = 0; x < kSfThreads / kWave; x++) {
+        for (int x = 0; x < NW; x++) {
             a = s_box[x][j] < a ? s_box[x][j] : a;
             b = s_box[x][6 + j] > b ? s_box[x][6 + j] : b;
             c = s_box[x][12 + j] < c ? s_box[x][12 + j] : c;
@@ -468,7 +476,7 @@ __global__ __launch_bounds__(kSfThreads) void k_sub_split(
     L.parent = R.parent = me;
     L.skip = l + 1;
     R.skip = me_n.skip;
-    L.pad = R.pad = 0;
+    L.where = R.where = me_n.where;
     nodes[l] = L;
     nodes[l + 1] = R;
     me_n.div = dim;
@@ -477,6 +485,16 @@ __global__ __launch_bounds__(kSfThreads) void k_sub_split(
     nodes[me] = me_n;
     if (L.n > kLeaf6) next[atomicAdd(&counters[kSfActive], 1)] = l;
     if (R.n > kLeaf6) next[atomicAdd(&counters[kSfActive], 1)] = l + 1;
+}
+
+// after the build: the leaves whose points the last write left in the
+// second buffer copy them to the first, which the searches read
+__global__ __launch_bounds__(kWave) void k_sub_collect(const Node6 *__restrict__ nodes,
+                                                       SParticle *__restrict__ wa,
+                                                       const SParticle *__restrict__ wb) {
+    const Node6 nd = nodes[blockIdx.x];
+    if (nd.div >= 0 || !nd.where) return;
+    for (int32_t p = lane_id(); p < nd.n; p += kWave) wa[nd.start + p] = wb[nd.start + p];
 }
 
 // ---------------------------------------------------------- tree searches
@@ -514,24 +532,14 @@ __device__ __forceinline__ int32_t near_child(const Node6 &nd, const float (&c)[
     return (double)acoord(c, d) > 0.5 * (double)mid ? nd.right : nd.left;
 }
 
-// fast3tree_find_next_closest_distance (src/fast3tree.c:992-1030) for each
-// sampled particle, one per lane: the recursion of 956-987 as a walk over
-// parent links that keeps the reference's visiting order
-__global__ __launch_bounds__(kWave) void k_sub_next_closest(
-    const SubGroup *__restrict__ groups, const SubTile *__restrict__ qtiles,
-    const int32_t *__restrict__ drawn_pos, const SParticle *__restrict__ w,
-    const Node6 *__restrict__ nodes, float *__restrict__ dist) {
-    const SubTile tl = qtiles[blockIdx.x];
-    const SubGroup g = groups[tl.group];
-    const int32_t i = tl.off + lane_id();
-    if (i >= g.nsample) return;
-    const int32_t j = g.drawn ? drawn_pos[g.sample_off + i] : i;
-    float c[6];
-    {
-        const SParticle q = w[g.off + j];
-        for (int k = 0; k < 6; k++) c[k] = q.pos[k];
-    }
-    const int32_t root = tl.group;
+// The reference's own-leaf step (src/fast3tree.c:999-1025): the leaf the
+// query descends to, climbed while its parent's split dimension is flat
+// there (that subtree, `skip`, is left out of the search), and the start
+// value: the smallest nonzero squared distance in it, or its squared
+// diagonal.
+__device__ __forceinline__ float nc_start(const float (&c)[6], int32_t root,
+                                          const SParticle *__restrict__ w,
+                                          const Node6 *__restrict__ nodes, int32_t *skip) {
     int32_t nd = root;
     while (nodes[nd].div >= 0) {
         const Node6 &x = nodes[nd];
@@ -542,22 +550,25 @@ __global__ __launch_bounds__(kWave) void k_sub_next_closest(
         if (acoord(nodes[nd].lo, pd) != acoord(nodes[nd].hi, pd)) break;
         nd = nodes[nd].parent;
     }
-    const int32_t skip = nd;
+    *skip = nd;
     float md = 0.0f;
-    {
-        const Node6 x = nodes[nd];
-        for (int k = 0; k < 6; k++) {
-            const float e = x.hi[k] - x.lo[k];
-            md += e * e;
-        }
-        for (int32_t p = 0; p < x.n; p++) {
-            const float d = dist6(c, w[x.start + p]);
-            if (d != 0.0f && d < md) md = d;
-        }
+    const Node6 x = nodes[nd];
+    for (int k = 0; k < 6; k++) {
+        const float e = x.hi[k] - x.lo[k];
+        md += e * e;
     }
-    float r = (float)sqrt((double)md);
-    // depth-first from the root, near child first; every entry tests the
-    // box against the current r, a leaf narrows r
+    for (int32_t p = 0; p < x.n; p++) {
+        const float d = dist6(c, w[x.start + p]);
+        if (d != 0.0f && d < md) md = d;
+    }
+    return md;
+}
+
+// The search of src/fast3tree.c:956-987 in the reference's visiting order
+// (near child first, every entry tests the box against the current r, a
+// leaf narrows r), as a walk over parent links.
+__device__ float nc_ordered(const float (&c)[6], int32_t root, int32_t skip, float r,
+                            const SParticle *__restrict__ w, const Node6 *__restrict__ nodes) {
     int32_t cur = root;
     bool down = true;
     for (;;) {
@@ -589,7 +600,109 @@ __global__ __launch_bounds__(kWave) void k_sub_next_closest(
             cur = par;
         }
     }
-    dist[g.sample_off + i] = r;
+    return r;
+}
+
+// a node whose box is at least thr (squared) from c: every point in it is
+__device__ __forceinline__ bool box_far(const Node6 &nd, const float (&c)[6], float thr) {
+    float d = 0.0f, e;
+#pragma unroll
+    for (int i = 0; i < 6; i++) {
+        if ((e = c[i] - nd.lo[i]) < 0) d += e * e;
+        else if ((e = c[i] - nd.hi[i]) > 0) d += e * e;
+    }
+    return d >= thr;
+}
+
+// fast3tree_find_next_closest_distance (src/fast3tree.c:992-1030) for 64
+// sampled particles per wavefront, neighbours in the tree order.
+//
+// The reference's answer depends on its visiting order only through
+// rounding: a leaf that improves nothing returns sqrt(r * r) = r (exact for
+// IEEE floats), so with T = fl(r0 * r0) from the start value and D the
+// smallest squared distance outside the skipped subtree, the answer is r0
+// when D >= T and sqrt(D) otherwise -- unless a value D' with
+// D < D' <= D (1 + 2^-21) was taken first, whose round trip
+// fl(fl(sqrt(D'))^2) can fall to D or below and reject it.  The wavefront
+// walks the tree once for all its lanes (a node is entered when any lane
+// may find a value below its bound there), each lane keeps the two smallest
+// distinct values, and a lane whose two are that close repeats the search
+// in the reference's own order.
+__global__ __launch_bounds__(kWave) void k_sub_next_closest(
+    const SubGroup *__restrict__ groups, const SubTile *__restrict__ qtiles,
+    const int32_t *__restrict__ qpos, const int32_t *__restrict__ qidx,
+    const SParticle *__restrict__ w, const Node6 *__restrict__ nodes, float *__restrict__ dist,
+    int ordered) {
+    const SubTile tl = qtiles[blockIdx.x];
+    const SubGroup g = groups[tl.group];
+    const int32_t q = tl.off + lane_id();
+    const bool valid = q < g.nsample;
+    const int32_t root = tl.group;
+    float c[6] = {0, 0, 0, 0, 0, 0};
+    int32_t skip = -1, s_lo = 0, s_hi = 0;
+    float r0 = 0.0f, T = 0.0f;
+    if (valid) {
+        const SParticle me = w[g.off + (g.drawn ? qpos[g.draw_off + q] : q)];
+        for (int k = 0; k < 6; k++) c[k] = me.pos[k];
+        r0 = (float)sqrt((double)nc_start(c, root, w, nodes, &skip));
+        T = r0 * r0;
+        s_lo = nodes[skip].start;
+        s_hi = s_lo + nodes[skip].n;
+    }
+    float m1 = __builtin_inff(), m2 = __builtin_inff();
+    float thr = valid ? T : 0.0f;  // nothing at or above T is ever taken
+    int32_t cur = ordered ? -1 : root;
+    while (cur >= 0) {
+        cur = uniform(cur);
+        const Node6 x = nodes[cur];
+        const bool inside = x.start >= s_lo && x.start + x.n <= s_hi;  // the skipped subtree
+        const bool need = valid && !inside && !box_far(x, c, thr);
+        if (!ballot(need)) {
+            cur = x.skip;
+            continue;
+        }
+        if (x.div >= 0) {
+            cur = x.left;
+            continue;
+        }
+        // a leaf whose split failed can hold more than a wavefront of points
+        for (int32_t base = 0; base < x.n; base += kWave) {
+            const int32_t cnt = min(x.n - base, kWave);
+            SParticle mine;
+            if (lane_id() < cnt) mine = w[x.start + base + lane_id()];
+            for (int32_t t = 0; t < cnt; t++) {
+                float pp[6];
+                for (int k = 0; k < 6; k++) pp[k] = lane_float(mine.pos[k], t);
+                float d = 0.0f;
+                for (int k = 0; k < 6; k++) {
+                    const float dx = c[k] - pp[k];
+                    d += dx * dx;
+                }
+                if (need) {
+                    if (d < m1) {
+                        m2 = m1;
+                        m1 = d;
+                    } else if (d > m1 && d < m2) {
+                        m2 = d;
+                    }
+                }
+            }
+        }
+        if (need && m1 < thr) thr = fminf(thr, m1 * 1.000002f);
+        cur = x.skip;
+    }
+    if (!valid) return;
+    float r;
+    if (ordered) {
+        r = nc_ordered(c, root, skip, r0, w, nodes);
+    } else if (!(m1 < T)) {
+        r = r0;
+    } else if (m2 > m1 * 1.000002f) {
+        r = (float)sqrt((double)m1);
+    } else {
+        r = nc_ordered(c, root, skip, r0, w, nodes);
+    }
+    dist[g.sample_off + (g.drawn ? qidx[g.draw_off + q] : q)] = r;
 }
 
 // ------------------------------------------------------ linking length
@@ -731,30 +844,39 @@ __device__ __forceinline__ void uf_unite(int32_t *up, int32_t a, int32_t b) {
     }
 }
 
-// _find_subfofs_at_r (src/groupies.c:274-285): each particle's sphere search
-// over the later slots (fast3tree_find_sphere_skip, src/fast3tree.c:419-465)
-// and the union of every pair closer than the group's length
-__global__ __launch_bounds__(kWave) void k_sub_link(int32_t m, const int32_t *__restrict__ slot_group,
+// _find_subfofs_at_r (src/groupies.c:274-285): each particle's sphere
+// search over the later slots (fast3tree_find_sphere_skip,
+// src/fast3tree.c:419-465) and the union of every pair closer than the
+// group's length.  64 consecutive slots per wavefront walk the tree
+// together: a node is entered when some lane's sphere reaches it and it
+// holds slots after that lane's, and each leaf's points are loaded once and
+// passed lane to lane.  Every pair the reference links is tested with the
+// same float distance, so the components are the reference's.
+__global__ __launch_bounds__(kWave) void k_sub_link(const SubGroup *__restrict__ groups,
+                                                    const SubTile *__restrict__ ltiles,
                                                     const float *__restrict__ target,
                                                     const SParticle *__restrict__ w,
                                                     const Node6 *__restrict__ nodes,
                                                     int32_t *__restrict__ up,
                                                     uint8_t *__restrict__ has) {
-    const int32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= m) return;
-    const int32_t gi = slot_group[i];
-    const float r = target[gi];
+    const SubTile tl = ltiles[blockIdx.x];
+    const SubGroup g = groups[tl.group];
+    const int32_t i = g.off + tl.off + lane_id();
+    const bool valid = tl.off + lane_id() < g.n;
+    const float r = target[tl.group];
     const float r2 = r * r;
-    float c[6];
-    {
+    float c[6] = {0, 0, 0, 0, 0, 0};
+    if (valid) {
         const SParticle q = w[i];
         for (int k = 0; k < 6; k++) c[k] = q.pos[k];
     }
     bool any = false;
-    int32_t cur = gi;
+    int32_t cur = tl.group;
     while (cur >= 0) {
+        cur = uniform(cur);
         const Node6 x = nodes[cur];
-        if (x.start + x.n <= i || box_apart(x, c, r)) {
+        const bool need = valid && x.start + x.n > i + 1 && !box_apart(x, c, r);
+        if (!ballot(need)) {
             cur = x.skip;
             continue;
         }
@@ -762,11 +884,25 @@ __global__ __launch_bounds__(kWave) void k_sub_link(int32_t m, const int32_t *__
             cur = x.left;
             continue;
         }
-        for (int32_t j = max(x.start, i + 1); j < x.start + x.n; j++) {
-            if (!(dist6(c, w[j]) < r2)) continue;
-            any = true;
-            has[j] = 1;
-            uf_unite(up, i, j);
+        for (int32_t base = 0; base < x.n; base += kWave) {
+            const int32_t cnt = min(x.n - base, kWave);
+            SParticle mine;
+            if (lane_id() < cnt) mine = w[x.start + base + lane_id()];
+            for (int32_t t = 0; t < cnt; t++) {
+                float pp[6];
+                for (int k = 0; k < 6; k++) pp[k] = lane_float(mine.pos[k], t);
+                float d = 0.0f;
+                for (int k = 0; k < 6; k++) {
+                    const float dx = c[k] - pp[k];
+                    d += dx * dx;
+                }
+                const int32_t j = x.start + base + t;
+                if (need && j > i && d < r2) {
+                    any = true;
+                    has[j] = 1;
+                    uf_unite(up, i, j);
+                }
+            }
         }
         cur = x.skip;
     }
@@ -785,6 +921,8 @@ __global__ void k_sub_keys(int32_t m, const int32_t *__restrict__ slot_group,
     int32_t key;
     if (has[i]) {
         const int32_t r = uf_find(up, i);
+        // every root is final here: shortens the finds still running
+        __hip_atomic_store(up + i, r, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
         atomicAdd(csize + r, 1);
         key = r + 1;
     } else {

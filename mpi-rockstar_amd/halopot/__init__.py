@@ -206,7 +206,7 @@ class SubfofConfig(C.Structure):
 
     _fields_ = [("fof_fraction", C.c_double), ("initial_metric_scaling", C.c_double),
                 ("min_halo_particles", C.c_int64), ("exact_ll_calc", C.c_int32),
-                ("pad", C.c_int32)]
+                ("ordered_search", C.c_int32)]
 
 
 FOF_GROUP_DTYPE = np.dtype({"names": ["first", "num_p", "top", "target_r", "first_sub", "num_sub"],
@@ -216,7 +216,8 @@ SUBFOF_DTYPE = np.dtype([("first", "<i8"), ("num_p", "<i8")])
 
 
 def find_subfofs(d_particles: int, nparticles: int, groups: np.ndarray, fof_fraction=0.7,
-                 min_halo_particles=10, exact_ll_calc=0, initial_metric_scaling=1.0):
+                 min_halo_particles=10, exact_ll_calc=0, initial_metric_scaling=1.0,
+                 ordered_search=0):
     """hp_find_subfofs: _find_subfofs_better2 (src/groupies.c:287-314) for
     every group (rows with first, num_p, top) of the particles at device
     address d_particles, in place.  Returns (groups with target_r, first_sub
@@ -225,10 +226,12 @@ def find_subfofs(d_particles: int, nparticles: int, groups: np.ndarray, fof_frac
     for f in ("first", "num_p"):
         g[f] = groups[f]
     g["top"] = groups["top"] if "top" in (groups.dtype.names or ()) else 1
-    cap = int(np.sum(g["num_p"]) // 2 + len(g) + 1)
-    subs = np.zeros(cap, SUBFOF_DTYPE)
+    # a sub-FOF holds at least min_halo_particles (and at least 2) particles
+    cap = int(np.sum(g["num_p"]) // max(2, min_halo_particles) + 1)
+    subs = np.empty(cap, SUBFOF_DTYPE)
     n = C.c_int64()
-    cfg = SubfofConfig(fof_fraction, initial_metric_scaling, min_halo_particles, exact_ll_calc, 0)
+    cfg = SubfofConfig(fof_fraction, initial_metric_scaling, min_halo_particles, exact_ll_calc,
+                       ordered_search)
     _check(lib().hp_find_subfofs(C.c_void_p(d_particles), nparticles, _ptr(g), g.shape[0],
                                  C.byref(cfg), _ptr(subs), cap, C.byref(n)))
     return g, subs[:n.value]

@@ -386,6 +386,9 @@ class ReferenceProps:
         L.ref_subfof.argtypes = [C.c_void_p, C.c_int64, C.c_float, C.c_int] + [C.c_void_p] * 7 + \
             [C.c_int64] + [C.c_void_p] * 3
         L.ref_next_closest.argtypes = [C.c_void_p, C.c_int64, C.c_void_p, C.c_int64, C.c_void_p]
+        L.ref_subfof_batch.restype = C.c_int64
+        L.ref_subfof_batch.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p, C.c_int64, C.c_float,
+                                       C.c_int, C.c_int]
         L.ref_find_median_r.restype = C.c_float
         L.ref_find_median_r.argtypes = [C.c_void_p, C.c_int64, C.c_float, C.c_uint]
 
@@ -478,6 +481,16 @@ class ReferenceProps:
                 "sample_r": sample[:ns.value], "final": final[:n],
                 "fofs": np.stack([ff[:nf], fn[:nf]], 1),
                 "fofs_at_r": np.stack([af[:nat.value], an[:nat.value]], 1)}
+
+    def subfof_batch(self, parts: np.ndarray, groups: np.ndarray, thresh: float = 0.7,
+                     top: bool = True, nthreads: int = 0) -> int:
+        """_find_subfofs_better2 on every group (first, num_p), in place,
+        OpenMP dynamic over groups; returns the number of sub-FOFs."""
+        assert parts.dtype.itemsize == 32 and parts.flags.c_contiguous
+        f = np.ascontiguousarray(groups["first"], np.int64)
+        n = np.ascontiguousarray(groups["num_p"], np.int64)
+        return self.lib.ref_subfof_batch(_p(parts), _p(f), _p(n), f.shape[0], thresh, int(top),
+                                         nthreads)
 
     def next_closest(self, pts: np.ndarray, queries: np.ndarray) -> np.ndarray:
         """fast3tree_find_next_closest_distance (src/fast3tree.c:992-1030)

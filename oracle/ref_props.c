@@ -352,3 +352,28 @@ float ref_find_median_r(const float *rad, int64_t n, float frac, unsigned int se
     free(r);
     return v;
 }
+
+/* _find_subfofs_better2 for every group of a batch (groups at parts +
+   first[g], num_p[g]), OpenMP dynamic over groups as the halo finder's loop
+   over FOF groups; the per-thread tree, results and sample buffer are the
+   reference's threadprivate globals.  Returns the total sub-FOF count. */
+int64_t ref_subfof_batch(struct particle *parts, const int64_t *first, const int64_t *num_p,
+                         int64_t ng, float thresh, int top, int nthreads) {
+    int64_t total = 0;
+#ifdef _OPENMP
+    if (nthreads > 0) omp_set_num_threads(nthreads);
+#else
+    (void)nthreads;
+#endif
+#pragma omp parallel for schedule(dynamic) reduction(+ : total)
+    for (int64_t g = 0; g < ng; g++) {
+        sub_setup(num_p[g], top);
+        struct fof f = {num_p[g], parts + first[g]};
+        struct FOFInfo fi;
+        init_fofinfo(&fi);
+        _find_subfofs_better2(&f, thresh, &fi);
+        total += fi.num_fofs;
+        free_fofinfo(&fi);
+    }
+    return total;
+}

@@ -103,6 +103,19 @@ def test_gpu_subfof_matches_restatement_on_fof_groups(restatement):
 
 
 @pytest.mark.gpu
+def test_gpu_subfof_shared_walk_equals_ordered_walk():
+    """The default next-closest search (one walk per wavefront, the rare
+    close calls repeated in the reference's order) against every search in
+    the reference's order: identical linking lengths and results."""
+    fg = catalog.fof_groups(seed=31, n_groups=80, lo=200, hi=40000)
+    parts, groups = fg.particles, fg.groups
+    a = _run_gpu(parts, groups)
+    b = _run_gpu(parts, groups, ordered_search=1)
+    np.testing.assert_array_equal(a[0]["target_r"], b[0]["target_r"])
+    assert a[1].tobytes() == b[1].tobytes() and a[2].tobytes() == b[2].tobytes()
+
+
+@pytest.mark.gpu
 def test_gpu_subfof_edge_groups(restatement):
     rng = np.random.default_rng(21)
 
