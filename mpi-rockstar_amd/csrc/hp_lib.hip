@@ -1533,8 +1533,8 @@ __attribute__((visibility("default"))) int hp_find_subfofs(struct particle *d_pa
     HP_ENSURE(B.bpos, sizeof(int32_t) * (M + 1));
     HP_ENSURE(B.spos, sizeof(int32_t) * (M + 1));
     HP_ENSURE(B.nodes, sizeof(Node6) * node_cap);
-    HP_ENSURE(B.act0, sizeof(int32_t) * (M / (kLeaf6 + 1) + ng + 1));
-    HP_ENSURE(B.act1, sizeof(int32_t) * (M / (kLeaf6 + 1) + ng + 1));
+    HP_ENSURE(B.act0, 2 * sizeof(int32_t) * (M / (kLeaf6 + 1) + ng + 1));
+    HP_ENSURE(B.act1, 2 * sizeof(int32_t) * (M / (kLeaf6 + 1) + ng + 1));
     HP_ENSURE(B.counters, sizeof(int32_t) * kSfCounters);
     HP_ENSURE(B.slot_group, sizeof(int32_t) * (M + 1));
     HP_ENSURE(B.dist, sizeof(float) * (S + 1));
@@ -1587,31 +1587,41 @@ __attribute__((visibility("default"))) int hp_find_subfofs(struct particle *d_pa
     // the trees, level by level
     k_fill_i32<<<1, 1, 0, s>>>(cnt + kSfNodes, (int32_t)ng, 1);
     HP_LAUNCHED();
-    k_sub_roots<<<Engine::blocks(ng, 64), 64, 0, s>>>((int32_t)ng, dg, B.pbox.as<float>(),
-                                              B.nodes.as<Node6>(), B.act0.as<int32_t>(), cnt);
-    HP_LAUNCHED();
+    // two lists per level: nodes above kSfBigNode points split with 1024
+    // threads, the others with kSfThreads
+    const int64_t act_cap = M / (kLeaf6 + 1) + ng + 1;
     int32_t *act = B.act0.as<int32_t>(), *nxt = B.act1.as<int32_t>();
+    int32_t *act_big = act + act_cap, *nxt_big = nxt + act_cap;
+    k_sub_roots<<<Engine::blocks(ng, 64), 64, 0, s>>>((int32_t)ng, dg, B.pbox.as<float>(),
+                                              B.nodes.as<Node6>(), act, act_big, cnt);
+    HP_LAUNCHED();
     for (int level = 0;; level++) {
-        int32_t na = 0;
-        HP_TRY(hipMemcpyAsync(&na, cnt + kSfActive, sizeof na, hipMemcpyDeviceToHost, s));
+        int32_t na[kSfCounters];
+        HP_TRY(hipMemcpyAsync(na, cnt, sizeof na, hipMemcpyDeviceToHost, s));
         HP_TRY(hipStreamSynchronize(s));
-        if (!na) break;
+        const int32_t ns_ = na[kSfActive], nb_ = na[kSfActiveBig];
+        if (!ns_ && !nb_) break;
         if (level > 4 * 64) return set_error(HP_EHIP, "hp_find_subfofs: tree too deep");
         k_fill_i32<<<1, 1, 0, s>>>(cnt + kSfActive, 0, 1);
         HP_LAUNCHED();
-        // the first levels hold few, large nodes: wider workgroups
-        if (level < 6)
-            k_sub_split<1024><<<(uint32_t)na, 1024, 0, s>>>(
-                B.nodes.as<Node6>(), act, B.w.as<SParticle>(), B.w2.as<SParticle>(),
-                B.rk.as<int32_t>(), B.bpos.as<int32_t>(), B.spos.as<int32_t>(), nxt, cnt,
-                (int32_t)node_cap);
-        else
-            k_sub_split<kSfThreads><<<(uint32_t)na, kSfThreads, 0, s>>>(
-                B.nodes.as<Node6>(), act, B.w.as<SParticle>(), B.w2.as<SParticle>(),
-                B.rk.as<int32_t>(), B.bpos.as<int32_t>(), B.spos.as<int32_t>(), nxt, cnt,
-                (int32_t)node_cap);
+        k_fill_i32<<<1, 1, 0, s>>>(cnt + kSfActiveBig, 0, 1);
         HP_LAUNCHED();
+        if (nb_) {
+            k_sub_split<1024><<<(uint32_t)nb_, 1024, 0, s>>>(
+                B.nodes.as<Node6>(), act_big, B.w.as<SParticle>(), B.w2.as<SParticle>(),
+                B.rk.as<int32_t>(), B.bpos.as<int32_t>(), B.spos.as<int32_t>(), nxt, nxt_big,
+                cnt, (int32_t)node_cap);
+            HP_LAUNCHED();
+        }
+        if (ns_) {
+            k_sub_split<kSfThreads><<<(uint32_t)ns_, kSfThreads, 0, s>>>(
+                B.nodes.as<Node6>(), act, B.w.as<SParticle>(), B.w2.as<SParticle>(),
+                B.rk.as<int32_t>(), B.bpos.as<int32_t>(), B.spos.as<int32_t>(), nxt, nxt_big,
+                cnt, (int32_t)node_cap);
+            HP_LAUNCHED();
+        }
         std::swap(act, nxt);
+        std::swap(act_big, nxt_big);
     }
     {
         int32_t nn = 0;

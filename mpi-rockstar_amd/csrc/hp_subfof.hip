@@ -73,7 +73,15 @@ struct Node6 {
 };
 static_assert(sizeof(Node6) == 80, "node layout");
 
-enum SfCounter { kSfNodes = 0, kSfActive = 1, kSfEmit = 2, kSfErr = 3, kSfCounters = 4 };
+enum SfCounter {
+    kSfNodes = 0,
+    kSfActive = 1,     // nodes to split next level with kSfThreads
+    kSfEmit = 2,
+    kSfErr = 3,
+    kSfActiveBig = 4,  // nodes above kSfBigNode points: split with 1024 threads
+    kSfCounters = 5
+};
+constexpr int kSfBigNode = 8192;
 
 __device__ __forceinline__ float pcoord(const SParticle &p, int d) {
     return d == 0 ? p.pos[0]
@@ -265,9 +273,18 @@ __global__ __launch_bounds__(kWave) void k_sub_scale(const SubGroup *__restrict_
 
 // root node of each group (node index = group index); roots that split go
 // on the first active list
+// the node goes on the list of the workgroup size that suits it
+__device__ __forceinline__ void push_active(int32_t id, int32_t n, int32_t *__restrict__ small,
+                                            int32_t *__restrict__ big,
+                                            int32_t *__restrict__ counters) {
+    if (n > kSfBigNode) big[atomicAdd(&counters[kSfActiveBig], 1)] = id;
+    else small[atomicAdd(&counters[kSfActive], 1)] = id;
+}
+
 __global__ void k_sub_roots(int32_t ng, const SubGroup *__restrict__ groups,
                             const float *__restrict__ pbox, Node6 *__restrict__ nodes,
-                            int32_t *__restrict__ active, int32_t *__restrict__ counters) {
+                            int32_t *__restrict__ active, int32_t *__restrict__ active_big,
+                            int32_t *__restrict__ counters) {
     const int32_t gi = blockIdx.x * blockDim.x + threadIdx.x;
     if (gi >= ng) return;
     const SubGroup g = groups[gi];
@@ -290,7 +307,7 @@ __global__ void k_sub_roots(int32_t ng, const SubGroup *__restrict__ groups,
     nd.left = nd.right = nd.parent = nd.skip = -1;
     nd.where = 0;
     nodes[gi] = nd;
-    if (g.n > kLeaf6) active[atomicAdd(&counters[kSfActive], 1)] = gi;
+    if (g.n > kLeaf6) push_active(gi, g.n, active, active_big, counters);
 }
 
 // ------------------------------------------------------------- tree build
@@ -351,8 +368,8 @@ template <int NT>
 __global__ __launch_bounds__(NT) void k_sub_split(
     Node6 *__restrict__ nodes, const int32_t *__restrict__ active, SParticle *__restrict__ wa,
     SParticle *__restrict__ wb, int32_t *__restrict__ rk, int32_t *__restrict__ bpos,
-    int32_t *__restrict__ spos, int32_t *__restrict__ next, int32_t *__restrict__ counters,
-    int32_t node_cap) {
+    int32_t *__restrict__ spos, int32_t *__restrict__ next, int32_t *__restrict__ next_big,
+    int32_t *__restrict__ counters, int32_t node_cap) {
     constexpr int NW = NT / kWave;
     __shared__ int s_w[NW];
     __shared__ int32_t s_k[2];
@@ -483,8 +500,8 @@ __global__ __launch_bounds__(NT) void k_sub_split(
     me_n.left = l;
     me_n.right = l + 1;
     nodes[me] = me_n;
-    if (L.n > kLeaf6) next[atomicAdd(&counters[kSfActive], 1)] = l;
-    if (R.n > kLeaf6) next[atomicAdd(&counters[kSfActive], 1)] = l + 1;
+    if (L.n > kLeaf6) push_active(l, L.n, next, next_big, counters);
+    if (R.n > kLeaf6) push_active(l + 1, R.n, next, next_big, counters);
 }
 
 // after the build: the leaves whose points the last write left in the
