@@ -671,7 +671,7 @@ __global__ __launch_bounds__(kWave) void k_sub_next_closest(
     int32_t cur = ordered ? -1 : root;
     while (cur >= 0) {
         cur = uniform(cur);
-        const Node6 x = nodes[cur];
+        const Node6 x = load_const(nodes + cur);
         const bool inside = x.start >= s_lo && x.start + x.n <= s_hi;  // the skipped subtree
         const bool need = valid && !inside && !box_far(x, c, thr);
         if (!ballot(need)) {
@@ -682,26 +682,21 @@ __global__ __launch_bounds__(kWave) void k_sub_next_closest(
             cur = x.left;
             continue;
         }
-        // a leaf whose split failed can hold more than a wavefront of points
-        for (int32_t base = 0; base < x.n; base += kWave) {
-            const int32_t cnt = min(x.n - base, kWave);
-            SParticle mine;
-            if (lane_id() < cnt) mine = w[x.start + base + lane_id()];
-            for (int32_t t = 0; t < cnt; t++) {
-                float pp[6];
-                for (int k = 0; k < 6; k++) pp[k] = lane_float(mine.pos[k], t);
-                float d = 0.0f;
-                for (int k = 0; k < 6; k++) {
-                    const float dx = c[k] - pp[k];
-                    d += dx * dx;
-                }
-                if (need) {
-                    if (d < m1) {
-                        m2 = m1;
-                        m1 = d;
-                    } else if (d > m1 && d < m2) {
-                        m2 = d;
-                    }
+        // the leaf's points at wavefront-uniform addresses: scalar loads
+        const int32_t x0 = uniform(x.start), xn = uniform(x.n);
+        for (int32_t t = 0; t < xn; t++) {
+            const SParticle pt = load_const(w + x0 + t);
+            float d = 0.0f;
+            for (int k = 0; k < 6; k++) {
+                const float dx = c[k] - pt.pos[k];
+                d += dx * dx;
+            }
+            if (need) {
+                if (d < m1) {
+                    m2 = m1;
+                    m1 = d;
+                } else if (d > m1 && d < m2) {
+                    m2 = d;
                 }
             }
         }
@@ -891,7 +886,7 @@ __global__ __launch_bounds__(kWave) void k_sub_link(const SubGroup *__restrict__
     int32_t cur = tl.group;
     while (cur >= 0) {
         cur = uniform(cur);
-        const Node6 x = nodes[cur];
+        const Node6 x = load_const(nodes + cur);
         const bool need = valid && x.start + x.n > i + 1 && !box_apart(x, c, r);
         if (!ballot(need)) {
             cur = x.skip;
@@ -901,16 +896,16 @@ __global__ __launch_bounds__(kWave) void k_sub_link(const SubGroup *__restrict__
             cur = x.left;
             continue;
         }
+        // the leaf's points, one per lane, passed lane to lane (a leaf whose
+        // split failed can hold more than a wavefront of points)
         for (int32_t base = 0; base < x.n; base += kWave) {
             const int32_t cnt = min(x.n - base, kWave);
             SParticle mine;
             if (lane_id() < cnt) mine = w[x.start + base + lane_id()];
             for (int32_t t = 0; t < cnt; t++) {
-                float pp[6];
-                for (int k = 0; k < 6; k++) pp[k] = lane_float(mine.pos[k], t);
                 float d = 0.0f;
                 for (int k = 0; k < 6; k++) {
-                    const float dx = c[k] - pp[k];
+                    const float dx = c[k] - lane_float(mine.pos[k], t);
                     d += dx * dx;
                 }
                 const int32_t j = x.start + base + t;
