@@ -832,11 +832,16 @@ __global__ void k_sub_init(int32_t m, int32_t *__restrict__ up, uint8_t *__restr
     csize[i] = 0;
 }
 
+// path halving: each visited slot is pointed at its grandparent.  A link
+// only ever moves to a smaller slot of the same tree (roots hook under
+// smaller roots), so a racing update never loses a union.
 __device__ __forceinline__ int32_t uf_find(int32_t *up, int32_t x) {
     for (;;) {
         const int32_t y = __hip_atomic_load(up + x, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
         if (y == x) return x;
-        x = y;
+        const int32_t z = __hip_atomic_load(up + y, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        if (z != y) __hip_atomic_store(up + x, z, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        x = z;
     }
 }
 
@@ -929,17 +934,26 @@ __global__ void k_sub_keys(int32_t m, const int32_t *__restrict__ slot_group,
                            const uint8_t *__restrict__ has, int32_t *__restrict__ csize,
                            uint64_t *__restrict__ keys, int32_t *__restrict__ vals) {
     const int32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= m) return;
-    int32_t key;
-    if (has[i]) {
-        const int32_t r = uf_find(up, i);
+    const bool in = i < m;
+    const bool linked = in && has[i];
+    int32_t r = -1;
+    if (linked) {
+        r = uf_find(up, i);
         // every root is final here: shortens the finds still running
         __hip_atomic_store(up + i, r, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        atomicAdd(csize + r, 1);
-        key = r + 1;
-    } else {
-        key = groups[slot_group[i]].off;
     }
+    // component sizes: one atomic per distinct root in the wavefront (the
+    // members of a large component would otherwise queue on one counter)
+    uint64_t todo = ballot(linked);
+    while (todo) {
+        const int lead = __builtin_ctzll(todo);
+        const int32_t rl = __builtin_amdgcn_readlane(r, lead);
+        const uint64_t same = ballot(linked && r == rl);
+        if (lane_id() == lead) atomicAdd(csize + rl, (int)__popcll(same));
+        todo &= ~same;
+    }
+    if (!in) return;
+    const int32_t key = linked ? r + 1 : groups[slot_group[i]].off;
     keys[i] = (uint64_t)(uint32_t)key;
     vals[i] = i;
 }
