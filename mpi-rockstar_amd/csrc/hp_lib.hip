@@ -32,6 +32,7 @@
 #include <cstring>
 #include <mutex>
 #include <string>
+#include <thread>
 #include <vector>
 
 // ----------------------------------------------------------- C globals
@@ -1443,6 +1444,8 @@ __attribute__((visibility("default"))) int hp_find_subfofs(struct particle *d_pa
     std::vector<SubGroup> G((size_t)ngroups);
     std::vector<SubTile> tiles, qtiles, ltiles;
     std::vector<int32_t> drawn, qidx;
+    std::vector<int64_t> drawn_groups;
+    int64_t ndrawn = 0;
     std::vector<std::pair<int64_t, int64_t>> ranges;
     int64_t M = 0, S = 0;
     for (int64_t g = 0; g < ngroups; g++) {
@@ -1467,35 +1470,15 @@ __attribute__((visibility("default"))) int hp_find_subfofs(struct particle *d_pa
         sg.sample_off = (int32_t)S;
         sg.drawn = ns != n;
         // rand_r draws only for the groups that draw (the others search
-        // 0..n-1), kept sorted by position so that a wavefront's queries are
-        // neighbours in the tree; qidx remembers each draw's place in the
-        // sample, which is the order find_median_r sees
-        sg.draw_off = (int32_t)drawn.size();
+        // 0..n-1); made below, several groups at a time
+        sg.draw_off = (int32_t)ndrawn;
         if (sg.drawn) {
-            std::vector<int32_t> js((size_t)ns);
-            for (int64_t i = 0; i < ns; i++) {
-                int64_t j = rand_r_glibc(&seed);
-                j <<= 31;
-                j += rand_r_glibc(&seed);
-                j %= n;
-                js[(size_t)i] = (int32_t)j;
-            }
-            // counting sort by position, stable in draw order
-            std::vector<int32_t> start((size_t)n + 1, 0);
-            for (int64_t i = 0; i < ns; i++) start[(size_t)js[(size_t)i] + 1]++;
-            for (int64_t j = 0; j < n; j++) start[(size_t)j + 1] += start[(size_t)j];
-            const size_t base = drawn.size();
-            drawn.resize(base + (size_t)ns);
-            qidx.resize(base + (size_t)ns);
-            for (int64_t i = 0; i < ns; i++) {
-                const int32_t at = start[(size_t)js[(size_t)i]]++;
-                drawn[base + (size_t)at] = js[(size_t)i];
-                qidx[base + (size_t)at] = (int32_t)i;
-            }
+            drawn_groups.push_back(g);
+            ndrawn += ns;
         }
+        sg.seed = seed;  // advanced past the draws below for the drawn groups
         for (int64_t off = 0; off < n; off += kWave)
             ltiles.push_back(SubTile{(int32_t)g, (int32_t)off});
-        sg.seed = seed;
         sg.k = (int32_t)(int64_t)((float)ns * thresh);  // src/groupies.c:771
         if (ns > 0 && sg.k >= ns)
             return set_error(HP_EINVAL, "hp_find_subfofs: FOF_FRACTION selects past the sample");
@@ -1507,6 +1490,50 @@ __attribute__((visibility("default"))) int hp_find_subfofs(struct particle *d_pa
             qtiles.push_back(SubTile{(int32_t)g, (int32_t)off});
         M += n;
         S += ns;
+    }
+    // src/groupies.c:300-304 for each drawn group: the sample positions in
+    // draw order (seed = num_p), then sorted by position (counting sort,
+    // stable in draw order) so that a wavefront's queries are neighbours in
+    // the tree; qidx keeps each draw's place in the sample, the order
+    // find_median_r sees.  Groups are independent: several host threads.
+    drawn.resize((size_t)ndrawn);
+    qidx.resize((size_t)ndrawn);
+    auto draw_group = [&](int64_t g) {
+        SubGroup &sg = G[(size_t)g];
+        const int64_t n = sg.n, ns = sg.nsample;
+        uint32_t seed = (uint32_t)n;
+        std::vector<int32_t> js((size_t)ns), start((size_t)n + 1, 0);
+        for (int64_t i = 0; i < ns; i++) {
+            int64_t j = rand_r_glibc(&seed);
+            j <<= 31;
+            j += rand_r_glibc(&seed);
+            j %= n;
+            js[(size_t)i] = (int32_t)j;
+            start[(size_t)j + 1]++;
+        }
+        for (int64_t j = 0; j < n; j++) start[(size_t)j + 1] += start[(size_t)j];
+        const size_t base = (size_t)sg.draw_off;
+        for (int64_t i = 0; i < ns; i++) {
+            const int32_t at = start[(size_t)js[(size_t)i]]++;
+            drawn[base + (size_t)at] = js[(size_t)i];
+            qidx[base + (size_t)at] = (int32_t)i;
+        }
+        sg.seed = seed;
+    };
+    {
+        const size_t nd = drawn_groups.size();
+        const size_t nthr = std::min<size_t>(nd, std::min(16u, std::max(1u, std::thread::hardware_concurrency())));
+        if (nthr <= 1) {
+            for (int64_t g : drawn_groups) draw_group(g);
+        } else {
+            std::atomic<size_t> next{0};
+            std::vector<std::thread> pool;
+            for (size_t w = 0; w < nthr; w++)
+                pool.emplace_back([&] {
+                    for (size_t k; (k = next.fetch_add(1)) < nd;) draw_group(drawn_groups[k]);
+                });
+            for (auto &th : pool) th.join();
+        }
     }
     std::sort(ranges.begin(), ranges.end());
     for (size_t i = 1; i < ranges.size(); i++)
