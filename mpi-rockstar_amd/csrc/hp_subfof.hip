@@ -876,6 +876,7 @@ __global__ __launch_bounds__(kWave) void k_sub_link(const SubGroup *__restrict__
                                                     const Node6 *__restrict__ nodes,
                                                     int32_t *__restrict__ up,
                                                     uint8_t *__restrict__ has) {
+    __shared__ float4 s_pts[kWave][2];
     const SubTile tl = ltiles[blockIdx.x];
     const SubGroup g = groups[tl.group];
     const int32_t i = g.off + tl.off + lane_id();
@@ -901,16 +902,23 @@ __global__ __launch_bounds__(kWave) void k_sub_link(const SubGroup *__restrict__
             cur = x.left;
             continue;
         }
-        // the leaf's points, one per lane, passed lane to lane (a leaf whose
-        // split failed can hold more than a wavefront of points)
+        // the leaf's points, one per lane, staged in LDS and read back by
+        // every lane at the same address (a broadcast; a leaf whose split
+        // failed can hold more than a wavefront of points)
         for (int32_t base = 0; base < x.n; base += kWave) {
             const int32_t cnt = min(x.n - base, kWave);
-            SParticle mine;
-            if (lane_id() < cnt) mine = w[x.start + base + lane_id()];
+            if (lane_id() < cnt) {
+                const SParticle q = w[x.start + base + lane_id()];
+                s_pts[lane_id()][0] = make_float4(q.pos[0], q.pos[1], q.pos[2], q.pos[3]);
+                s_pts[lane_id()][1] = make_float4(q.pos[4], q.pos[5], 0.0f, 0.0f);
+            }
+            __syncthreads();
             for (int32_t t = 0; t < cnt; t++) {
+                const float4 a = s_pts[t][0], b = s_pts[t][1];
+                const float pp[6] = {a.x, a.y, a.z, a.w, b.x, b.y};
                 float d = 0.0f;
                 for (int k = 0; k < 6; k++) {
-                    const float dx = c[k] - lane_float(mine.pos[k], t);
+                    const float dx = c[k] - pp[k];
                     d += dx * dx;
                 }
                 const int32_t j = x.start + base + t;
@@ -920,6 +928,7 @@ __global__ __launch_bounds__(kWave) void k_sub_link(const SubGroup *__restrict__
                     uf_unite(up, i, j);
                 }
             }
+            __syncthreads();
         }
         cur = x.skip;
     }
