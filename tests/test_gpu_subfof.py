@@ -139,6 +139,33 @@ def test_gpu_subfof_edge_groups(restatement):
 
 
 @pytest.mark.gpu
+def test_gpu_subfof_empty_groups_and_mixed_levels(restatement):
+    """Empty groups between others (linking length 0, no sub-FOF; the
+    reference asserts on an empty sample) and top-level and lower-level
+    groups in one batch (sigma_x scaled by INITIAL_METRIC_SCALING only for
+    the top-level ones)."""
+    fg = catalog.fof_groups(seed=8, n_groups=5, lo=300, hi=5000)
+    plist = [fg.particles[f:f + n].copy() for f, n in fg.groups]
+    empty = np.zeros(0, catalog.PARTICLE_DTYPE)
+    plist = [empty, plist[0], empty, plist[1], plist[2], empty, plist[3], plist[4]]
+    parts, groups = _batch(plist)
+    top = np.array([1, 1, 0, 0, 1, 1, 0, 1], np.int32)
+    g = np.zeros(len(plist), halopot.FOF_GROUP_DTYPE)
+    g["first"], g["num_p"], g["top"] = groups["first"], groups["num_p"], top
+    got, subs, out = _run_gpu(parts, g, initial_metric_scaling=1.3)
+    for i, p in enumerate(plist):
+        f, n = int(groups["first"][i]), int(groups["num_p"][i])
+        if n == 0:
+            assert got["target_r"][i] == 0.0 and got["num_sub"][i] == 0
+            continue
+        tr, arranged, want = _expected(restatement, p, bool(top[i]), 1.3, 0)
+        assert got["target_r"][i] == tr, i
+        assert out[f:f + n].tobytes() == arranged.tobytes(), i
+        sl = subs[got["first_sub"][i]:got["first_sub"][i] + got["num_sub"][i]]
+        assert [(int(x["first"]) - f, int(x["num_p"])) for x in sl] == want, i
+
+
+@pytest.mark.gpu
 def test_gpu_subfof_exact_ll_and_lower_levels(restatement):
     fg = catalog.fof_groups(seed=5, n_groups=6, lo=300, hi=12000)
     plist = [fg.particles[f:f + n].copy() for f, n in fg.groups]
