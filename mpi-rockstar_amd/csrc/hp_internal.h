@@ -19,7 +19,7 @@ constexpr int kDegenerate = 20;   // 2 * POINTS_PER_LEAF, src/potential.c:145
 constexpr int kWave = 64;
 constexpr int kGroup = 64;        // sink particles per walk wavefront (one per lane)
 constexpr int kTile = 1024;       // points per workgroup tile of a big node
-constexpr int kSmallMax = 512;    // largest subtree finished by one workgroup
+constexpr int kSmallMax = 1024;   // largest subtree finished by one workgroup
 constexpr int kSmallChunks = kSmallMax / 64;
 constexpr int kFatLeaf = 64;      // leaves above this size get a wavefront
 constexpr int kTileThreads = 256;

@@ -1269,7 +1269,7 @@ __attribute__((visibility("default"))) int hp_plan_tune(hp_plan *p, struct poten
     hp_options o = p->opt;
     o.timing = 0;
     o.count_work = 0;
-    const int32_t bts[] = {128, 256, 512};
+    const int32_t bts[] = {128, 256, 512, 1024};
     const int32_t wms[] = {kWaveMin, kWaveTask};
     const int32_t dms[] = {kLeafPoints, 0};
     const int ndm = p->opt.method == HP_METHOD_AUTO ? 2 : 1;
