@@ -183,11 +183,13 @@ struct DevBuf {
 // Device scratch of the sub-FOF step (hp_find_subfofs).
 struct SubBufs {
     DevBuf groups, tiles, qtiles, ltiles, drawn, qidx, psum, mean, pcorr, sig, pbox, gerr, w, w2, rk, bpos, spos,
-        nodes, act0, act1, counters, slot_group, dist, sel, sel2, target, up, has, csize, emit;
+        nodes, act0, act1, counters, slot_group, dist, sel, sel2, target, up, has, csize, emit,
+        fused;
     void release() {
         for (DevBuf *b : {&groups, &tiles, &qtiles, &ltiles, &drawn, &qidx, &psum, &mean, &pcorr, &sig, &pbox,
                           &gerr, &w, &w2, &rk, &bpos, &spos, &nodes, &act0, &act1, &counters,
-                          &slot_group, &dist, &sel, &sel2, &target, &up, &has, &csize, &emit})
+                          &slot_group, &dist, &sel, &sel2, &target, &up, &has, &csize, &emit,
+                          &fused})
             b->release();
     }
 };
@@ -1562,6 +1564,7 @@ __attribute__((visibility("default"))) int hp_find_subfofs(struct particle *d_pa
     HP_ENSURE(B.nodes, sizeof(Node6) * node_cap);
     HP_ENSURE(B.act0, 2 * sizeof(int32_t) * (M / (kLeaf6 + 1) + ng + 1));
     HP_ENSURE(B.act1, 2 * sizeof(int32_t) * (M / (kLeaf6 + 1) + ng + 1));
+    HP_ENSURE(B.fused, sizeof(int32_t) * (M / (kLeaf6 + 1) + ng + 1));
     HP_ENSURE(B.counters, sizeof(int32_t) * kSfCounters);
     HP_ENSURE(B.slot_group, sizeof(int32_t) * (M + 1));
     HP_ENSURE(B.dist, sizeof(float) * (S + 1));
@@ -1619,8 +1622,9 @@ __attribute__((visibility("default"))) int hp_find_subfofs(struct particle *d_pa
     const int64_t act_cap = M / (kLeaf6 + 1) + ng + 1;
     int32_t *act = B.act0.as<int32_t>(), *nxt = B.act1.as<int32_t>();
     int32_t *act_big = act + act_cap, *nxt_big = nxt + act_cap;
+    int32_t *fused = B.fused.as<int32_t>();
     k_sub_roots<<<Engine::blocks(ng, 64), 64, 0, s>>>((int32_t)ng, dg, B.pbox.as<float>(),
-                                              B.nodes.as<Node6>(), act, act_big, cnt);
+                                              B.nodes.as<Node6>(), act, act_big, cnt, fused);
     HP_LAUNCHED();
     for (int level = 0;; level++) {
         int32_t na[kSfCounters];
@@ -1637,20 +1641,31 @@ __attribute__((visibility("default"))) int hp_find_subfofs(struct particle *d_pa
             k_sub_split<1024><<<(uint32_t)nb_, 1024, 0, s>>>(
                 B.nodes.as<Node6>(), act_big, B.w.as<SParticle>(), B.w2.as<SParticle>(),
                 B.rk.as<int32_t>(), B.bpos.as<int32_t>(), B.spos.as<int32_t>(), nxt, nxt_big,
-                cnt, (int32_t)node_cap);
+                cnt, (int32_t)node_cap, fused);
             HP_LAUNCHED();
         }
         if (ns_) {
             k_sub_split<kSfThreads><<<(uint32_t)ns_, kSfThreads, 0, s>>>(
                 B.nodes.as<Node6>(), act, B.w.as<SParticle>(), B.w2.as<SParticle>(),
                 B.rk.as<int32_t>(), B.bpos.as<int32_t>(), B.spos.as<int32_t>(), nxt, nxt_big,
-                cnt, (int32_t)node_cap);
+                cnt, (int32_t)node_cap, fused);
             HP_LAUNCHED();
         }
         std::swap(act, nxt);
         std::swap(act_big, nxt_big);
     }
     {
+        // the nodes of at most kFuseMax points, each finished by one workgroup
+        int32_t nf = 0;
+        HP_TRY(hipMemcpyAsync(&nf, cnt + kSfFused, sizeof nf, hipMemcpyDeviceToHost, s));
+        HP_TRY(hipStreamSynchronize(s));
+        if (nf > 0) {
+            k_sub_fused<<<(uint32_t)nf, kSfThreads, 0, s>>>(B.nodes.as<Node6>(), fused,
+                                                           B.w.as<SParticle>(),
+                                                           B.w2.as<SParticle>(), cnt,
+                                                           (int32_t)node_cap);
+            HP_LAUNCHED();
+        }
         int32_t nn = 0;
         HP_TRY(hipMemcpyAsync(&nn, cnt + kSfNodes, sizeof nn, hipMemcpyDeviceToHost, s));
         HP_TRY(hipStreamSynchronize(s));

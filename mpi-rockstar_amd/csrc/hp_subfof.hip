@@ -79,7 +79,8 @@ enum SfCounter {
     kSfEmit = 2,
     kSfErr = 3,
     kSfActiveBig = 4,  // nodes above kSfBigNode points: split with 1024 threads
-    kSfCounters = 5
+    kSfFused = 5,      // nodes of at most kFuseMax points: finished by k_sub_fused
+    kSfCounters = 6
 };
 constexpr int kSfBigNode = 8192;
 
@@ -273,18 +274,23 @@ __global__ __launch_bounds__(kWave) void k_sub_scale(const SubGroup *__restrict_
 
 // root node of each group (node index = group index); roots that split go
 // on the first active list
-// the node goes on the list of the workgroup size that suits it
+// the node goes on the list that suits its size: split level by level
+// with 1024 or kSfThreads threads, or finished whole by k_sub_fused (the
+// fused list follows the two level lists in `big`'s buffer, at fused_at)
+constexpr int kFuseMaxNode = 1024;  // kFuseMax, declared with its kernel below
 __device__ __forceinline__ void push_active(int32_t id, int32_t n, int32_t *__restrict__ small,
                                             int32_t *__restrict__ big,
-                                            int32_t *__restrict__ counters) {
-    if (n > kSfBigNode) big[atomicAdd(&counters[kSfActiveBig], 1)] = id;
+                                            int32_t *__restrict__ counters,
+                                            int32_t *__restrict__ fused) {
+    if (n <= kFuseMaxNode) fused[atomicAdd(&counters[kSfFused], 1)] = id;
+    else if (n > kSfBigNode) big[atomicAdd(&counters[kSfActiveBig], 1)] = id;
     else small[atomicAdd(&counters[kSfActive], 1)] = id;
 }
 
 __global__ void k_sub_roots(int32_t ng, const SubGroup *__restrict__ groups,
                             const float *__restrict__ pbox, Node6 *__restrict__ nodes,
                             int32_t *__restrict__ active, int32_t *__restrict__ active_big,
-                            int32_t *__restrict__ counters) {
+                            int32_t *__restrict__ counters, int32_t *__restrict__ fused) {
     const int32_t gi = blockIdx.x * blockDim.x + threadIdx.x;
     if (gi >= ng) return;
     const SubGroup g = groups[gi];
@@ -307,7 +313,7 @@ __global__ void k_sub_roots(int32_t ng, const SubGroup *__restrict__ groups,
     nd.left = nd.right = nd.parent = nd.skip = -1;
     nd.where = 0;
     nodes[gi] = nd;
-    if (g.n > kLeaf6) push_active(gi, g.n, active, active_big, counters);
+    if (g.n > kLeaf6) push_active(gi, g.n, active, active_big, counters, fused);
 }
 
 // ------------------------------------------------------------- tree build
@@ -369,7 +375,7 @@ __global__ __launch_bounds__(NT) void k_sub_split(
     Node6 *__restrict__ nodes, const int32_t *__restrict__ active, SParticle *__restrict__ wa,
     SParticle *__restrict__ wb, int32_t *__restrict__ rk, int32_t *__restrict__ bpos,
     int32_t *__restrict__ spos, int32_t *__restrict__ next, int32_t *__restrict__ next_big,
-    int32_t *__restrict__ counters, int32_t node_cap) {
+    int32_t *__restrict__ counters, int32_t node_cap, int32_t *__restrict__ fused) {
     constexpr int NW = NT / kWave;
     __shared__ int s_w[NW];
     __shared__ int32_t s_k[2];
@@ -500,8 +506,275 @@ __global__ __launch_bounds__(NT) void k_sub_split(
     me_n.left = l;
     me_n.right = l + 1;
     nodes[me] = me_n;
-    if (L.n > kLeaf6) push_active(l, L.n, next, next_big, counters);
-    if (R.n > kLeaf6) push_active(l + 1, R.n, next, next_big, counters);
+    if (L.n > kLeaf6) push_active(l, L.n, next, next_big, counters, fused);
+    if (R.n > kLeaf6) push_active(l + 1, R.n, next, next_big, counters, fused);
+}
+
+// A node of at most kFuseMax points is split down to its leaves by one
+// workgroup, in LDS, all nodes of a level at once: the coordinates stay in
+// place and a permutation of local positions moves; one block-wide scan
+// ranks every node's points, one thread per node finds its K and S, the
+// children's boxes come from atomic min/max on order-preserving keys.  The
+// same splits as k_sub_split, node for node; the particles are written once,
+// to the other buffer, at the end.
+constexpr int kFuseMax = kFuseMaxNode;
+constexpr int kFuseList = 2 * (kFuseMax / (kLeaf6 + 1)) + 2;  // nodes of one level
+constexpr int kFusePer = kFuseMax / kSfThreads;                // positions per thread
+constexpr uint8_t kFuseDone = 0xff;
+static_assert(kFuseList < kFuseDone, "level entries fit a byte");
+
+struct FuseNode {
+    float lo[6], hi[6];
+    int32_t start, n;  // local positions
+    int32_t gid;       // global node id
+    int32_t dim;       // split dimension, -1: no split (a leaf)
+    float lim;
+    int32_t nsmall, S;
+    int32_t child;     // first of the children's entries in the next list, -1: none
+};
+
+__global__ __launch_bounds__(kSfThreads) void k_sub_fused(Node6 *__restrict__ nodes,
+                                                          const int32_t *__restrict__ roots,
+                                                          SParticle *__restrict__ wa,
+                                                          SParticle *__restrict__ wb,
+                                                          int32_t *__restrict__ counters,
+                                                          int32_t node_cap) {
+    __shared__ float X[6][kFuseMax];
+    __shared__ int16_t I[2][kFuseMax];
+    __shared__ uint8_t NO[2][kFuseMax];
+    __shared__ int32_t SC[kFuseMax + 1];
+    __shared__ int32_t BP[kFuseMax], SP[kFuseMax];
+    __shared__ FuseNode E[2][kFuseList];
+    __shared__ int32_t BK[kFuseList][2][12];
+    __shared__ int s_w[kSfThreads / kWave];
+    __shared__ int32_t s_cnt[2];
+    const int tid = threadIdx.x;
+    const int32_t root = roots[blockIdx.x];
+    const Node6 rn = nodes[root];
+    const int32_t s0 = rn.start, N = rn.n;
+    const SParticle *__restrict__ src = rn.where ? wb : wa;
+    SParticle *__restrict__ dst = rn.where ? wa : wb;
+    const int32_t wnew = 1 - rn.where;  // the buffer the subtree's points end in
+    for (int32_t p = tid; p < N; p += kSfThreads) {
+        const SParticle q = src[s0 + p];
+        for (int k = 0; k < 6; k++) X[k][p] = q.pos[k];
+        I[0][p] = (int16_t)p;
+        NO[0][p] = 0;
+    }
+    if (tid == 0) {
+        FuseNode &e = E[0][0];
+        for (int k = 0; k < 6; k++) {
+            e.lo[k] = rn.lo[k];
+            e.hi[k] = rn.hi[k];
+        }
+        e.start = 0;
+        e.n = N;
+        e.gid = root;
+        s_cnt[0] = 1;
+    }
+    __syncthreads();
+    int cur = 0;
+    for (int level = 0; level < kFuseMax; level++) {
+        const int32_t ne = s_cnt[cur];
+        if (ne == 0) break;
+        const int nxt = cur ^ 1;
+        // 1. each node: dimension and midpoint (src/fast3tree.c:736-760)
+        if (tid < ne) {
+            FuseNode &e = E[cur][tid];
+            int dim = 5;
+            float wd = e.hi[5] - e.lo[5];
+            for (int k = 0; k < 5; k++) {
+                const float w2 = e.hi[k] - e.lo[k];
+                if (w2 > wd) {
+                    wd = w2;
+                    dim = k;
+                }
+            }
+            const float hd = acoord(e.hi, dim), ld = acoord(e.lo, dim);
+            e.dim = hd == ld ? -1 : dim;  // a flat extent: a leaf, no reordering
+            e.lim = 0.5 * (hd + ld);
+            e.child = -1;
+        }
+        if (tid == 0) s_cnt[nxt] = 0;
+        for (int x = tid; x < kFuseList * 24; x += kSfThreads)
+            (&BK[0][0][0])[x] = (x % 12) < 6 ? 0x7fffffff : (int32_t)0x80000000;
+        __syncthreads();
+        // 2. classes and one scan of the bigs over every position
+        int32_t cnt_local = 0;
+        bool bigv[kFusePer];
+        for (int r = 0; r < kFusePer; r++) {
+            const int32_t p = tid * kFusePer + r;
+            bool b = false;
+            if (p < N) {
+                const uint8_t en = NO[cur][p];
+                if (en != kFuseDone) {
+                    const FuseNode &e = E[cur][en];
+                    if (e.dim >= 0) b = X[e.dim][I[cur][p]] > e.lim;
+                }
+            }
+            bigv[r] = b;
+            cnt_local += b ? 1 : 0;
+        }
+        // exclusive scan of the per-thread counts over the workgroup
+        int32_t incl = wave_scan_add(cnt_local);
+        if (lane_id() == kWave - 1) s_w[tid / kWave] = incl;
+        __syncthreads();
+        int32_t off = 0;
+        for (int j = 0; j < tid / kWave; j++) off += s_w[j];
+        int32_t run = off + incl - cnt_local;
+        for (int r = 0; r < kFusePer; r++) {
+            const int32_t p = tid * kFusePer + r;
+            if (p < N) SC[p] = run;
+            run += bigv[r] ? 1 : 0;
+        }
+        if (tid == kSfThreads - 1) SC[N] = run;  // the last thread ends with the total
+        __syncthreads();
+        // 3. ranks within the node and the two position lists
+        for (int r = 0; r < kFusePer; r++) {
+            const int32_t p = tid * kFusePer + r;
+            if (p >= N) continue;
+            const uint8_t en = NO[cur][p];
+            if (en == kFuseDone) continue;
+            const FuseNode &e = E[cur][en];
+            if (e.dim < 0) continue;
+            const int32_t rel = p - e.start;
+            const int32_t rb = SC[p] - SC[e.start];
+            if (bigv[r]) BP[e.start + rb] = rel;
+            else SP[e.start + (rel - rb)] = rel;
+        }
+        __syncthreads();
+        // 4. each node: K and S, its children's entries and global ids
+        if (tid < ne) {
+            FuseNode &e = E[cur][tid];
+            if (e.dim >= 0) {
+                const int32_t nbig = SC[e.start + e.n] - SC[e.start];
+                const int32_t nsm = e.n - nbig;
+                int32_t K, S;
+                partition_ks(BP + e.start, SP + e.start, e.n, nbig, nsm, &K, &S);
+                e.nsmall = nsm;
+                e.S = S;
+                if (nsm > 0 && nsm < e.n) {
+                    const int32_t g = atomicAdd(&counters[kSfNodes], 2);
+                    if (g + 2 <= node_cap) {
+                        const int32_t c = atomicAdd(&s_cnt[nxt], 2);
+                        e.child = c;
+                        FuseNode &L = E[nxt][c];
+                        FuseNode &R = E[nxt][c + 1];
+                        L.start = e.start;
+                        L.n = nsm;
+                        R.start = e.start + nsm;
+                        R.n = e.n - nsm;
+                        L.gid = g;
+                        R.gid = g + 1;
+                    } else {  // no room for the children: reordered, then a leaf
+                        atomicOr(&counters[kSfErr], 1);
+                    }
+                }
+            }
+        }
+        __syncthreads();
+        // 5. every position of a splitting node to its place, the children's
+        //    boxes
+        for (int r = 0; r < kFusePer; r++) {
+            const int32_t p = tid * kFusePer + r;
+            if (p >= N) continue;
+            const uint8_t en = NO[cur][p];
+            if (en == kFuseDone) {
+                I[nxt][p] = I[cur][p];
+                NO[nxt][p] = kFuseDone;
+                continue;
+            }
+            const FuseNode &e = E[cur][en];
+            if (e.dim < 0) {  // no split: a leaf where it is
+                I[nxt][p] = I[cur][p];
+                NO[nxt][p] = kFuseDone;
+                continue;
+            }
+            const int32_t rel = p - e.start;
+            const int32_t rb = SC[p] - SC[e.start];
+            const int32_t rank = bigv[r] ? rb : rel - rb;
+            const int32_t d = partition_dest(rel, bigv[r], rank, e.n, e.nsmall, e.S, BP + e.start,
+                                             SP + e.start);
+            const int32_t q = e.start + d;
+            const int16_t id = I[cur][p];
+            I[nxt][q] = id;
+            if (e.child < 0) {  // all on one side: reordered, then a leaf
+                NO[nxt][q] = kFuseDone;
+                continue;
+            }
+            const int side = d < e.nsmall ? 0 : 1;
+            const int32_t ce = e.child + side;
+            const int32_t cn = side ? e.n - e.nsmall : e.nsmall;
+            NO[nxt][q] = cn > kLeaf6 ? (uint8_t)ce : kFuseDone;
+            for (int k = 0; k < 6; k++) {
+                const int32_t key = float_key(X[k][id]);
+                atomicMin(&BK[en][side][k], key);
+                atomicMax(&BK[en][side][6 + k], key);
+            }
+        }
+        __syncthreads();
+        // 6. the node records (children as leaves or as next level's nodes)
+        if (tid < ne) {
+            const FuseNode &e = E[cur][tid];
+            Node6 me_n = nodes[e.gid];
+            me_n.where = wnew;
+            if (e.child >= 0) {
+                const int32_t l = E[nxt][e.child].gid;
+                Node6 C[2];
+                for (int c = 0; c < 2; c++) {
+                    FuseNode &ce = E[nxt][e.child + c];
+                    for (int k = 0; k < 6; k++) {
+                        ce.lo[k] = key_float(BK[tid][c][k]);
+                        ce.hi[k] = key_float(BK[tid][c][6 + k]);
+                        C[c].lo[k] = ce.lo[k];
+                        C[c].hi[k] = ce.hi[k];
+                    }
+                    C[c].start = s0 + ce.start;
+                    C[c].n = ce.n;
+                    C[c].div = -1;
+                    C[c].left = C[c].right = -1;
+                    C[c].parent = e.gid;
+                    C[c].where = wnew;
+                }
+                C[0].skip = l + 1;
+                C[1].skip = me_n.skip;
+                nodes[l] = C[0];
+                nodes[l + 1] = C[1];
+                me_n.div = e.dim;
+                me_n.left = l;
+                me_n.right = l + 1;
+            } else {
+                me_n.div = -1;
+            }
+            nodes[e.gid] = me_n;
+        }
+        __syncthreads();
+        // the next level holds only the children of more than kLeaf6 points:
+        // compact the entries, renumbering the positions' entries
+        if (tid == 0) {
+            int32_t m = 0;
+            const int32_t nn2 = s_cnt[nxt];
+            for (int32_t c = 0; c < nn2; c++) {
+                if (E[nxt][c].n > kLeaf6) {
+                    if (m != c) E[nxt][m] = E[nxt][c];
+                    BP[c] = m;  // old entry -> new entry (BP is free here)
+                    m++;
+                } else {
+                    BP[c] = -1;
+                }
+            }
+            s_cnt[nxt] = m;
+        }
+        __syncthreads();
+        for (int32_t p = tid; p < N; p += kSfThreads) {
+            const uint8_t en = NO[nxt][p];
+            if (en != kFuseDone) NO[nxt][p] = (uint8_t)BP[en];
+        }
+        __syncthreads();
+        cur = nxt;
+    }
+    // the particles in their final order, to the other buffer
+    for (int32_t k = tid; k < N; k += kSfThreads) dst[s0 + k] = src[s0 + I[cur][k]];
 }
 
 // after the build: the leaves whose points the last write left in the
