@@ -990,7 +990,7 @@ struct Engine {
         const double hub = 100.0 * hubble_scaling(z1 - 1.0);
         const double hub_a = hub * a;
         if (nst > 0) {
-            k_kinetic_tiles<<<nst, kWave, 0, s>>>(S, d_po, scan_tiles.as<ScanTile>(), nst, ci,
+            k_kinetic_tiles<<<nst, kKinThreads, 0, s>>>(S, d_po, scan_tiles.as<ScanTile>(), nst, ci,
                                                 d_halos, hub_a, conv, d_bound, ti32);
             HP_LAUNCHED();
         }

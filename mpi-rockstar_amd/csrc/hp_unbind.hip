@@ -534,7 +534,10 @@ __global__ void k_vel_finish(const struct potential *__restrict__ srt,
 // (src/properties.c:403).  One wavefront per scan tile; bound counts per
 // tile.  The records are written to their final place here, the potential-
 // only ones unchanged.
-__global__ __launch_bounds__(kWave) void k_kinetic_tiles(const struct potential *__restrict__ srt,
+// one tile of kScanTile records per workgroup of kKinThreads (several
+// wavefronts keep more loads in flight than one)
+constexpr int kKinThreads = 256;
+__global__ __launch_bounds__(kKinThreads) void k_kinetic_tiles(const struct potential *__restrict__ srt,
                                                          struct potential *__restrict__ out,
                                                          const ScanTile *__restrict__ tiles,
                                                          int32_t ntiles,
@@ -553,7 +556,7 @@ __global__ __launch_bounds__(kWave) void k_kinetic_tiles(const struct potential 
     const float cen_x[3] = {hh.pos[0], hh.pos[1], hh.pos[2]};
     const int32_t end = min(st.off + kScanTile, c.count);
     int32_t nb = 0;
-    for (int32_t j = st.off + lane_id(); j < end; j += kWave) {
+    for (int32_t j = st.off + (int32_t)threadIdx.x; j < end; j += kKinThreads) {
         struct potential rec = srt[c.first + j];
         if (j < c.kept) {
             float ke = 0.0f;
@@ -571,8 +574,15 @@ __global__ __launch_bounds__(kWave) void k_kinetic_tiles(const struct potential 
         }
         out[c.first + j] = rec;
     }
+    __shared__ int32_t s_nb[kKinThreads / kWave];
     nb = wave_sum(nb);
-    if (lane_id() == 0) tile_nb[t] = nb;
+    if (lane_id() == 0) s_nb[threadIdx.x / kWave] = nb;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        int32_t tot = 0;
+        for (int w = 0; w < kKinThreads / kWave; w++) tot += s_nb[w];
+        tile_nb[t] = tot;
+    }
 }
 
 // potential-only records: bound = 0
