@@ -482,13 +482,24 @@ constexpr int kWaveMin = 32;    // the smallest wavefront/workgroup boundary (wa
 constexpr int kRootSkip = -2;   // local skip pointer: the subtree root's own skip
 constexpr int kSubThreads = 256;
 constexpr int kSubWaves = kSubThreads / kWave;
-constexpr int kSubRounds = kSmallMax / kSubThreads;              // positions per thread
-constexpr int kSubChunks = kSmallMax / kWave;                    // ballot chunks
-constexpr int kSubList = 2 * (kSmallMax / (kLeafPoints + 1)) + 2;  // nodes of one level
-constexpr int kSubNodes = 2 * kSmallMax;                         // local ids (2n - 1 at most)
-constexpr int kSubPerThread = kSubNodes / kSubThreads;
 constexpr uint8_t kNoEntry = 0xff;
-static_assert(kSubList < kNoEntry, "level entries fit a byte");
+
+// The workgroup subtree kernel comes in three sizes, for nodes of at most
+// 256, 512 and 1,024 points: its LDS grows with the size, and the smaller
+// sizes fit more workgroups on a CU.  The tasks are split by size into one
+// list per kernel (k_group_classes) before the kernels run.
+constexpr int kSubClasses = 3;
+__host__ __device__ constexpr int sub_class(int32_t n) { return n > 512 ? 2 : (n > 256 ? 1 : 0); }
+template <int kCap>
+struct SubSize {
+    static constexpr int chunks = kCap / kWave;                     // ballot chunks
+    static constexpr int list = 2 * (kCap / (kLeafPoints + 1)) + 2; // nodes of one level
+    static constexpr int nodes = 2 * kCap;                          // local ids (2n - 1 at most)
+    static constexpr int per_thread = nodes / kSubThreads;
+    static_assert(list < kNoEntry, "level entries fit a byte");
+    static_assert(kCap >= kSubThreads && kCap <= kSmallMax && kCap % kSubThreads == 0,
+                  "a whole number of positions per thread");
+};
 
 __device__ __forceinline__ uint64_t lanes_below(int i) {
     return i >= 64 ? ~0ull : ((1ull << i) - 1ull);
@@ -530,31 +541,69 @@ struct SubEntry {       // a node of the level being split, 20 bytes
     uint8_t kind, dim;  // 1: split; split dimension
 };
 
+// Splits the workgroup subtree tasks by size into the lists of the three
+// kernel sizes: list c at out + c * cap_group, counted in counters[kClassNext + c].
+// Each task also gets its node ids here, 2n - 2 for n points (the most a
+// subtree can need; ids it does not use are never referenced), the first
+// kept in tile0: one atomic per wavefront instead of one per subtree.
+__global__ __launch_bounds__(256) void k_group_classes(const Task *__restrict__ in, BuildArgs a,
+                                                       Task *__restrict__ out) {
+    const int32_t n = min(a.counters[kGroupNext], a.cap_group);
+    const int32_t k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k - lane_id() >= n) return;  // whole wavefronts past the list
+    Task t;
+    int c = -1;
+    int32_t need = 0;
+    if (k < n) {
+        t = in[k];
+        c = sub_class(t.pad[1]);
+        need = 2 * t.pad[1] - 2;
+    }
+    const int32_t incl = wave_scan_add(need);
+    int32_t ids = 0;
+    if (lane_id() == kWave - 1)
+        ids = (int32_t)alloc_nodes_and_small(a, incl, 0);
+    t.tile0 = read_lane(ids, kWave - 1) + incl - need;
+    for (int q = 0; q < kSubClasses; q++) {
+        const uint64_t m = ballot(c == q);
+        if (!m) continue;
+        int32_t base = 0;
+        if (lane_id() == 0) base = atomicAdd(&a.counters[kClassNext + q], __popcll(m));
+        base = __shfl(base, 0, 64);
+        if (c == q) out[(int64_t)q * a.cap_group + base + rank_below(m)] = t;
+    }
+}
+
+// One workgroup per task of the list `cls` (nodes of at most kCap points).
+template <int kCap>
 __global__ __launch_bounds__(kSubThreads) void k_subtree(const float4 *__restrict__ A,
                                                          const float4 *__restrict__ B,
                                                          const Task *__restrict__ tasks,
-                                                         BuildArgs a, double particle_mass) {
-    __shared__ float4 P[2][kSmallMax];
-    __shared__ uint8_t Q[2][kSmallMax];       // level entry of each position
+                                                         BuildArgs a, double particle_mass,
+                                                         int cls) {
+    constexpr int kSubChunks = SubSize<kCap>::chunks, kSubList = SubSize<kCap>::list,
+                  kSubNodes = SubSize<kCap>::nodes, kSubPerThread = SubSize<kCap>::per_thread;
+    __shared__ float4 P[2][kCap];
+    __shared__ uint8_t Q[2][kCap];       // level entry of each position
     __shared__ SubEntry E[2][kSubList];
     __shared__ int32_t BOX[kSubList][6];      // next level's tight boxes, float keys
     __shared__ int16_t s_left[kSubNodes];     // left child of each local node, -1: leaf
-    __shared__ uint16_t s_lvl[kSmallMax + 2]; // first local id of each level
+    __shared__ uint16_t s_lvl[kCap + 2]; // first local id of each level
     __shared__ uint64_t s_mask[kSubChunks + 1];
     __shared__ int32_t s_pre[kSubChunks + 1];
     __shared__ int32_t s_nn[2], s_M, s_nlev, s_base, s_next;
     const int tid = threadIdx.x, lane = lane_id(), w = tid / kWave;
-    const int32_t ntasks = min(a.counters[kGroupNext], a.cap_group);
+    const int32_t ntasks = a.counters[kClassNext + cls];
     const float inf = __int_as_float(0x7f800000), ninf = __int_as_float((int)0xff800000);
     // A dynamic queue over the list: thread 0 takes the workgroup's next
     // entry when it starts the current one.  Entries are taken from the end
     // of the list: the largest subtrees (children of the big nodes of the
     // last split levels) are queued last and should start first.
-    if (tid == 0) s_next = atomicAdd(&a.counters[kGroupCursor], 1);
+    if (tid == 0) s_next = atomicAdd(&a.counters[kClassCursor + cls], 1);
     __syncthreads();
     for (int32_t k = s_next; k < ntasks; k = s_next) {
         int32_t k_after = 0;
-        if (tid == 0) k_after = atomicAdd(&a.counters[kGroupCursor], 1);
+        if (tid == 0) k_after = atomicAdd(&a.counters[kClassCursor + cls], 1);
         const Task t = tasks[ntasks - 1 - k];
         const int32_t first = t.pad[0], n = t.pad[1];
         const int32_t root_skip = a.nodes[t.node].skip;
@@ -584,7 +633,7 @@ __global__ __launch_bounds__(kSubThreads) void k_subtree(const float4 *__restric
             s_lvl[1] = 1;
             s_left[0] = -1;
             const int32_t R = 2 * n - 2;
-            int32_t base = (int32_t)alloc_nodes_and_small(a, R, 0);
+            int32_t base = t.tile0;  // R ids reserved by k_group_classes
             if (base + R > a.cap_nodes) {  // out of node ids: the root stays a leaf
                 atomicOr(&a.counters[kErrFlags], kErrCapacity);
                 base = -1;
@@ -782,12 +831,12 @@ __global__ __launch_bounds__(kSubThreads) void k_subtree(const float4 *__restric
         const float4 *F = P[buf];
         float4 *CM = P[buf ^ 1];
         auto cm_get = [&](int j) {
-            if (j < kSmallMax) return CM[j];
+            if (j < kCap) return CM[j];
             const Node *np = a.nodes + gid(j);
             return make_float4(np->com[0], np->com[1], np->com[2], np->mass);
         };
         auto cm_put = [&](int j, float4 v) {
-            if (j < kSmallMax) CM[j] = v;
+            if (j < kCap) CM[j] = v;
             Node *np = a.nodes + gid(j);
             np->com[0] = v.x;
             np->com[1] = v.y;
@@ -911,6 +960,49 @@ __global__ __launch_bounds__(kSubThreads) void k_subtree(const float4 *__restric
 
 constexpr int kWaveNodes = 2 * kWaveTask;  // a tree over n points has at most 2n - 1 nodes
 
+// Node ids of the wavefront subtrees, assigned before they are built: a
+// subtree of n points gets 2n - 2 ids (the most it can need; ids it does not
+// use are never referenced) from one block above every id in use, its first
+// kept in the task's tile0.  One workgroup scans the list, so the subtree
+// kernel needs no atomic per task.  Runs alone on the stream: nothing else
+// allocates node ids meanwhile.
+__host__ __device__ __forceinline__ int32_t wave_ids(int32_t n) { return 2 * n - 2; }
+__global__ __launch_bounds__(1024) void k_wave_ids(Task *__restrict__ tasks, BuildArgs a) {
+    __shared__ int32_t s_w[16];
+    const int32_t ntasks = min(a.counters[kSmallNext], a.cap_small);
+    const int32_t base0 = a.counters[kNodeCount];
+    const int w = threadIdx.x / kWave, lane = lane_id();
+    constexpr int kPer = 32;  // consecutive tasks per thread, loaded together
+    int32_t carry = 0;
+    for (int32_t k0 = 0; k0 < ntasks; k0 += 1024 * kPer) {
+        const int32_t kt = k0 + threadIdx.x * kPer;
+        int32_t need[kPer], mine = 0;
+#pragma unroll
+        for (int q = 0; q < kPer; q++) {  // clamped, unconditional: the loads issue together
+            const int32_t np = tasks[min(kt + q, ntasks - 1)].pad[1];
+            need[q] = kt + q < ntasks ? wave_ids(np) : 0;
+            mine += need[q];
+        }
+        const int32_t incl = wave_scan_add(mine);
+        if (lane == kWave - 1) s_w[w] = incl;
+        __syncthreads();
+        int32_t before = carry, total = carry;
+        for (int j = 0; j < 16; j++) {
+            before += j < w ? s_w[j] : 0;
+            total += s_w[j];
+        }
+        int32_t at = base0 + before + incl - mine;
+#pragma unroll
+        for (int q = 0; q < kPer; q++) {
+            if (kt + q < ntasks) tasks[kt + q].tile0 = at;
+            at += need[q];
+        }
+        carry = total;
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) a.counters[kNodeCount] = base0 + carry;
+}
+
 __global__ __launch_bounds__(kWave) void k_subtree_wave(const float4 *__restrict__ A,
                                                         const float4 *__restrict__ B,
                                                         const Task *__restrict__ tasks,
@@ -924,31 +1016,23 @@ __global__ __launch_bounds__(kWave) void k_subtree_wave(const float4 *__restrict
     __shared__ int16_t s_lvl[kWaveTask + 2];  // first local id of each level
     const int lane = lane_id();
     const int32_t ntasks = min(a.counters[kSmallNext], a.cap_small);
-    // a dynamic queue over the list: lane 0 takes the wavefront's next
-    // entry when it starts the current one
-    int32_t k = 0;
-    if (lane == 0) k = atomicAdd(&a.counters[kWaveCursor], 1);
-    k = read_lane(k, 0);
-    while (k < ntasks) {
-        int32_t k_after = 0;
-        if (lane == 0) k_after = atomicAdd(&a.counters[kWaveCursor], 1);
+    // tasks k = blockIdx.x, blockIdx.x + gridDim.x, ...: they are all small
+    // and alike, and a fixed stride needs no queue atomic per task
+    for (int32_t k = blockIdx.x; k < ntasks; k += gridDim.x) {
         const Task t = tasks[k];
         const int32_t n = t.pad[1];
         if (n > kWaveTask) {  // never queued here; guards the lane-per-point layout
             if (lane == 0) atomicOr(&a.counters[kErrFlags], kErrBadTree);
-            k = read_lane(k_after, 0);
             continue;
         }
-        const Node root = a.nodes[t.node];
+        // the task carries the root's range and split plane, so the points
+        // load at once; the root's node record is needed only at the end
         const float4 *src = (t.depth & 1) ? B : A;
-        const int32_t first = root.first;
+        const int32_t first = t.pad[0];
         const bool v = lane < n;
         float4 p = v ? src[first + lane] : make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+        const Node root = a.nodes[t.node];
         if (lane == 0) {
-            for (int d = 0; d < 3; d++) {
-                s_lo[d][0] = root.lo[d];
-                s_hi[d][0] = root.hi[d];
-            }
             s_lf[0] = 0;
             s_cnt[0] = (int16_t)n;
             s_left[0] = -1;
@@ -966,10 +1050,15 @@ __global__ __launch_bounds__(kWave) void k_subtree_wave(const float4 *__restrict
             if (pend) {
                 lf = s_lf[nid];
                 c = s_cnt[nid];
-                const float lo3[3] = {s_lo[0][nid], s_lo[1][nid], s_lo[2][nid]};
-                const float hi3[3] = {s_hi[0][nid], s_hi[1][nid], s_hi[2][nid]};
-                dim = split_dim(lo3, hi3);
-                lim = split_lim(lo3, hi3, dim);
+                if (nid == 0) {  // make_task's plane from the root's box
+                    dim = t.dim;
+                    lim = t.lim;
+                } else {
+                    const float lo3[3] = {s_lo[0][nid], s_lo[1][nid], s_lo[2][nid]};
+                    const float hi3[3] = {s_hi[0][nid], s_hi[1][nid], s_hi[2][nid]};
+                    dim = split_dim(lo3, hi3);
+                    lim = split_lim(lo3, hi3, dim);
+                }
             }
             // src/fast3tree.c:751-774 (partition), 805-809 (failed split)
             const bool gl = pend && comp(p, dim) <= lim;
@@ -1010,20 +1099,12 @@ __global__ __launch_bounds__(kWave) void k_subtree_wave(const float4 *__restrict
             // tight boxes of the new children: segmented scans over their runs
             const bool fresh = v && nid >= M0;
             const int seg = fresh ? s_lf[nid] : lane;
-            float l0 = p.x, l1 = p.y, l2 = p.z, h0 = p.x, h1 = p.y, h2 = p.z;
-            for (int o = 1; o < kWave; o <<= 1) {
-                const float u0 = __shfl_up(l0, o, 64), u1 = __shfl_up(l1, o, 64),
-                            u2 = __shfl_up(l2, o, 64), w0 = __shfl_up(h0, o, 64),
-                            w1 = __shfl_up(h1, o, 64), w2 = __shfl_up(h2, o, 64);
-                if (lane - o >= seg) {
-                    l0 = fminf(l0, u0);
-                    l1 = fminf(l1, u1);
-                    l2 = fminf(l2, u2);
-                    h0 = fmaxf(h0, w0);
-                    h1 = fmaxf(h1, w1);
-                    h2 = fmaxf(h2, w2);
-                }
-            }
+            auto mn = [](float x, float y) { return fminf(x, y); };
+            auto mx = [](float x, float y) { return fmaxf(x, y); };
+            const float l0 = seg_scan(p.x, seg, mn), l1 = seg_scan(p.y, seg, mn),
+                        l2 = seg_scan(p.z, seg, mn);
+            const float h0 = seg_scan(p.x, seg, mx), h1 = seg_scan(p.y, seg, mx),
+                        h2 = seg_scan(p.z, seg, mx);
             const int cnt_new = fresh ? s_cnt[nid] : 0;
             if (fresh && lane == seg + cnt_new - 1) {
                 s_lo[0][nid] = l0;
@@ -1040,16 +1121,19 @@ __global__ __launch_bounds__(kWave) void k_subtree_wave(const float4 *__restrict
             __syncthreads();
             pend = fresh && cnt_new > kLeafPoints;
         }
-        // every lane's node is a leaf now; reserve the subtree's node ids
-        int32_t base = 0;
-        if (lane == 0 && M > 1) {
-            base = (int32_t)alloc_nodes_and_small(a, M - 1, 0);
-            if (base + M - 1 > a.cap_nodes) {
-                atomicOr(&a.counters[kErrFlags], kErrCapacity);
-                base = -1;
+        // every lane's node is a leaf now; local id j > 0 is node base + j - 1
+        // of the ids k_wave_ids reserved
+        if (lane == 0) {
+            for (int d = 0; d < 3; d++) {
+                s_lo[d][0] = root.lo[d];
+                s_hi[d][0] = root.hi[d];
             }
         }
-        base = __shfl(base, 0, 64);
+        int32_t base = t.tile0;
+        if (M > 1 && base + wave_ids(n) > a.cap_nodes) {
+            if (lane == 0) atomicOr(&a.counters[kErrFlags], kErrCapacity);
+            base = -1;
+        }
         if (base < 0) {  // out of node ids: the root stays a leaf
             M = 1;
             nlev = 0;
@@ -1126,7 +1210,6 @@ __global__ __launch_bounds__(kWave) void k_subtree_wave(const float4 *__restrict
             if (j) a.parent[id] = s_par[j] ? base + s_par[j] - 1 : t.node;
         }
         __syncthreads();
-        k = read_lane(k_after, 0);
     }
 }
 

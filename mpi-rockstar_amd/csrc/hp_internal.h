@@ -101,9 +101,9 @@ enum Counter {
     kActive = 15,     // the current level has big tasks
     kBadRoots = 16,   // calls with non-finite points, listed for their own pass
     kGroupNext = 17,  // workgroup subtree tasks (wave_max < n <= big_threshold), append-only
-    kGroupCursor = 18,  // next workgroup subtree task to start (dynamic queue)
-    kWaveCursor = 19,   // next wavefront subtree task to start (dynamic queue)
-    kNumCounters = 20
+    kClassNext = 18,    // [18, 21): workgroup subtree tasks of each kernel size
+    kClassCursor = 21,  // [21, 24): next task of each size to start (dynamic queues)
+    kNumCounters = 24
 };
 
 enum ErrFlag { kErrWalkCap = 1, kErrBadTree = 2, kErrCapacity = 4 };

@@ -269,13 +269,13 @@ struct Engine {
     hipEvent_t ev[8] = {};
     int32_t big_threshold = kSmallMax;
     int32_t wave_max = kWaveTask;
-    int sub_grid = 0, wave_grid = 0;  // resident workgroups of the subtree kernels
+    int sub_grid[kSubClasses] = {}, wave_grid = 0;  // resident workgroups of the subtree kernels
 
     // batch tables on the device
     DevBuf hfirst, hcount, root_tiles, groups, direct_blocks, rec_call, scan_tiles, call_tile0,
         call_ntiles;
     // build
-    DevBuf box, bad, badlist, A, B, F, leaf_of, nodes, parent, fat, small0, group0, big, tiles, parts,
+    DevBuf box, bad, badlist, A, B, F, leaf_of, nodes, parent, fat, small0, group0, groupc, big, tiles, parts,
         splitres, dparts, counters, work, depth_buf;
     // unbind
     DevBuf keys0, keys1, vals0, vals1, hist, hsum, srt, callinfo, tsum, tcand, tile_i32;
@@ -312,8 +312,12 @@ struct Engine {
         // are empty; their grids are the workgroups that fit on the device
         // at once.
         int per_cu = 0;
-        HP_TRY(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_subtree, kSubThreads, 0));
-        sub_grid = std::max(1, per_cu) * prop.multiProcessorCount;
+        HP_TRY(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_subtree<256>, kSubThreads, 0));
+        sub_grid[0] = std::max(1, per_cu) * prop.multiProcessorCount;
+        HP_TRY(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_subtree<512>, kSubThreads, 0));
+        sub_grid[1] = std::max(1, per_cu) * prop.multiProcessorCount;
+        HP_TRY(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_subtree<1024>, kSubThreads, 0));
+        sub_grid[2] = std::max(1, per_cu) * prop.multiProcessorCount;
         HP_TRY(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_subtree_wave, kWave, 0));
         wave_grid = std::max(1, per_cu) * prop.multiProcessorCount;
         if (const char *s = getenv("HALOPOT_BIG_THRESHOLD")) big_threshold = atoi(s);
@@ -325,7 +329,7 @@ struct Engine {
     ~Engine() {
         DevBuf *all[] = {&hfirst, &hcount, &root_tiles, &groups, &direct_blocks, &rec_call,
                          &scan_tiles, &call_tile0, &call_ntiles, &box, &bad, &badlist, &A, &B, &F,
-                         &leaf_of, &nodes, &parent, &fat, &small0, &group0, &big, &tiles,
+                         &leaf_of, &nodes, &parent, &fat, &small0, &group0, &groupc, &big, &tiles,
                          &parts, &splitres, &dparts, &counters, &work, &depth_buf, &pcnt, &pscan, &pcall, &psums, &shape_st, &shape_part, &shape_cd, &scale_calls, &r_pe, &pe_cnt, &fit_list, &fit_n, &gsegs, &gtiles, &gcen, &keys0, &keys1, &vals0,
                          &vals1, &hist, &hsum, &srt, &callinfo, &tsum, &tcand, &tile_i32};
         if (stream) {
@@ -370,6 +374,7 @@ struct Engine {
         HP_ENSURE(small0, sizeof(Task) * cap_small);
         cap_group = N / (kWaveMin + 1) + H + 64;  // tasks of more than wave_max >= kWaveMin points
         HP_ENSURE(group0, sizeof(Task) * cap_group);
+        HP_ENSURE(groupc, sizeof(Task) * kSubClasses * cap_group);
         const int64_t bt = big_threshold > 0 ? big_threshold : 1;
         cap_big = 128 * (N / bt + 1) + H + 64;
         cap_tiles = 128 * (N / kTile + 2) + cap_big;
@@ -513,8 +518,24 @@ struct Engine {
     int enqueue_multipole(int levels) {
         hipStream_t s = stream;
         const int32_t *cnt = counters.as<int32_t>();
-        k_subtree<<<sub_grid, kSubThreads, 0, s>>>(A.as<float4>(), B.as<float4>(), group0.as<Task>(),
-                                               build_args(small0.as<Task>()), PARTICLE_MASS);
+        const BuildArgs ba = build_args(small0.as<Task>());
+        k_group_classes<<<blocks(cap_group, 256), 256, 0, s>>>(group0.as<Task>(), ba, groupc.as<Task>());
+        HP_LAUNCHED();
+        // largest first: the big subtrees hold most points and the fewest workgroups per CU
+        if (big_threshold > 512) {
+            k_subtree<1024><<<sub_grid[2], kSubThreads, 0, s>>>(
+                A.as<float4>(), B.as<float4>(), groupc.as<Task>() + 2 * cap_group, ba, PARTICLE_MASS, 2);
+            HP_LAUNCHED();
+        }
+        if (big_threshold > 256) {
+            k_subtree<512><<<sub_grid[1], kSubThreads, 0, s>>>(
+                A.as<float4>(), B.as<float4>(), groupc.as<Task>() + cap_group, ba, PARTICLE_MASS, 1);
+            HP_LAUNCHED();
+        }
+        k_subtree<256><<<sub_grid[0], kSubThreads, 0, s>>>(A.as<float4>(), B.as<float4>(),
+                                                           groupc.as<Task>(), ba, PARTICLE_MASS, 0);
+        HP_LAUNCHED();
+        k_wave_ids<<<1, 1024, 0, s>>>(small0.as<Task>(), ba);
         HP_LAUNCHED();
         k_subtree_wave<<<wave_grid, kWave, 0, s>>>(A.as<float4>(), B.as<float4>(), small0.as<Task>(),
                                               build_args(small0.as<Task>()), PARTICLE_MASS);
