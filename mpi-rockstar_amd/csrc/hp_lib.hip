@@ -982,7 +982,7 @@ struct Engine {
         HP_LAUNCHED();
         const double rvir_thresh = particle_rvir_dens() * (4.0 * M_PI / 3.0);
         if (nst > 0) {
-            k_rvir_tiles<<<nst, kWave, 0, s>>>(S, scan_tiles.as<ScanTile>(), nst, ci, rvir_thresh, ti32);
+            k_rvir_tiles<<<nst, kRvirThreads, 0, s>>>(S, scan_tiles.as<ScanTile>(), nst, ci, rvir_thresh, ti32);
             HP_LAUNCHED();
         }
         k_rvir_calls<<<blocks(H, 256), 256, 0, s>>>(S, call_tile0.as<int32_t>(),

@@ -547,7 +547,7 @@ __global__ __launch_bounds__(kSfThreads) void k_sub_fused(Node6 *__restrict__ no
     __shared__ FuseNode E[2][kFuseList];
     __shared__ int32_t BK[kFuseList][2][12];
     __shared__ int s_w[kSfThreads / kWave];
-    __shared__ int32_t s_cnt[2];
+    __shared__ int32_t s_cnt[2], s_g0;
     const int tid = threadIdx.x;
     const int32_t root = roots[blockIdx.x];
     const Node6 rn = nodes[root];
@@ -653,28 +653,31 @@ __global__ __launch_bounds__(kSfThreads) void k_sub_fused(Node6 *__restrict__ no
                 partition_ks(BP + e.start, SP + e.start, e.n, nbig, nsm, &K, &S);
                 e.nsmall = nsm;
                 e.S = S;
-                if (nsm > 0 && nsm < e.n) {
-                    const int32_t g = atomicAdd(&counters[kSfNodes], 2);
-                    if (g + 2 <= node_cap) {
-                        const int32_t c = atomicAdd(&s_cnt[nxt], 2);
-                        e.child = c;
-                        FuseNode &L = E[nxt][c];
-                        FuseNode &R = E[nxt][c + 1];
-                        L.start = e.start;
-                        L.n = nsm;
-                        R.start = e.start + nsm;
-                        R.n = e.n - nsm;
-                        L.gid = g;
-                        R.gid = g + 1;
-                    } else {  // no room for the children: reordered, then a leaf
-                        atomicOr(&counters[kSfErr], 1);
-                    }
+                if (nsm > 0 && nsm < e.n) {  // global ids after the level is counted
+                    const int32_t c = atomicAdd(&s_cnt[nxt], 2);
+                    e.child = c;
+                    FuseNode &L = E[nxt][c];
+                    FuseNode &R = E[nxt][c + 1];
+                    L.start = e.start;
+                    L.n = nsm;
+                    R.start = e.start + nsm;
+                    R.n = e.n - nsm;
                 }
             }
         }
         __syncthreads();
         // 5. every position of a splitting node to its place, the children's
-        //    boxes
+        //    boxes; the level's children take their global ids with one atomic
+        //    (child slot c is node s_g0 + c)
+        if (tid == 0) {
+            const int32_t nc = s_cnt[nxt];
+            int32_t g0 = nc ? atomicAdd(&counters[kSfNodes], nc) : 0;
+            if (g0 + nc > node_cap) {  // no room: the level's nodes stay leaves
+                atomicOr(&counters[kSfErr], 1);
+                g0 = -1;
+            }
+            s_g0 = g0;
+        }
         for (int r = 0; r < kFusePer; r++) {
             const int32_t p = tid * kFusePer + r;
             if (p >= N) continue;
@@ -718,8 +721,10 @@ __global__ __launch_bounds__(kSfThreads) void k_sub_fused(Node6 *__restrict__ no
             const FuseNode &e = E[cur][tid];
             Node6 me_n = nodes[e.gid];
             me_n.where = wnew;
-            if (e.child >= 0) {
-                const int32_t l = E[nxt][e.child].gid;
+            if (e.child >= 0 && s_g0 >= 0) {
+                const int32_t l = s_g0 + e.child;
+                E[nxt][e.child].gid = l;
+                E[nxt][e.child + 1].gid = l + 1;
                 Node6 C[2];
                 for (int c = 0; c < 2; c++) {
                     FuseNode &ce = E[nxt][e.child + c];
@@ -755,7 +760,7 @@ __global__ __launch_bounds__(kSfThreads) void k_sub_fused(Node6 *__restrict__ no
             int32_t m = 0;
             const int32_t nn2 = s_cnt[nxt];
             for (int32_t c = 0; c < nn2; c++) {
-                if (E[nxt][c].n > kLeaf6) {
+                if (E[nxt][c].n > kLeaf6 && s_g0 >= 0) {
                     if (m != c) E[nxt][m] = E[nxt][c];
                     BP[c] = m;  // old entry -> new entry (BP is free here)
                     m++;

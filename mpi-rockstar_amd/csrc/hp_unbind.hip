@@ -268,26 +268,34 @@ __device__ __forceinline__ int32_t block_sum_i32(int32_t v) {
 // rvir_max (src/properties.c:173-177): the largest j < kept with
 // j^2 > r2_j^3 (4pi/3 rho_vir)^2, r2^3 in float, the rest in double.
 // Per tile, then per call.
-__global__ __launch_bounds__(kWave) void k_rvir_tiles(const struct potential *__restrict__ srt,
-                                                      const ScanTile *__restrict__ tiles,
-                                                      int32_t ntiles,
-                                                      const CallInfo *__restrict__ ci,
-                                                      double rvir_thresh,
-                                                      int32_t *__restrict__ tile_rvir) {
+// four wavefronts per tile: more loads in flight than one
+constexpr int kRvirThreads = 256;
+__global__ __launch_bounds__(kRvirThreads) void k_rvir_tiles(const struct potential *__restrict__ srt,
+                                                             const ScanTile *__restrict__ tiles,
+                                                             int32_t ntiles,
+                                                             const CallInfo *__restrict__ ci,
+                                                             double rvir_thresh,
+                                                             int32_t *__restrict__ tile_rvir) {
+    __shared__ int32_t s_best[kRvirThreads / kWave];
     const int32_t t = blockIdx.x;
     if (t >= ntiles) return;
     const ScanTile st = tiles[t];
     const CallInfo c = ci[st.call];
     const int32_t end = min(st.off + kScanTile, c.kept);
     int32_t best = -1;
-    for (int32_t j = st.off + lane_id(); j < end; j += kWave) {
+    for (int32_t j = st.off + (int32_t)threadIdx.x; j < end; j += kRvirThreads) {
         const float r2 = srt[c.first + j].r2;
         const float r6 = (r2 * r2) * r2;
         if ((double)((int64_t)j * (int64_t)j) > (double)r6 * (rvir_thresh * rvir_thresh))
             best = j > best ? j : best;
     }
     best = wave_max(best);
-    if (lane_id() == 0) tile_rvir[t] = best;
+    if (lane_id() == 0) s_best[threadIdx.x / kWave] = best;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        for (int q = 1; q < kRvirThreads / kWave; q++) best = max(best, s_best[q]);
+        tile_rvir[t] = best;
+    }
 }
 
 // per call: rvir_max over the tiles, then core_max, the last j with
