@@ -204,6 +204,8 @@ struct Tables {
     std::vector<int32_t> rec_call;      // record -> call
     std::vector<ScanTile> scan_tiles;
     std::vector<int32_t> call_tile0, call_ntiles;
+    std::vector<int32_t> seg_wave, seg_block;  // calls sorted by one wavefront / workgroup
+    std::vector<SegTile> seg_tiles;            // tiles of the calls sorted by passes
     int method = HP_METHOD_TREE;
     int32_t direct_max = kLeafPoints;   // AUTO: direct at or below this count
 };
@@ -257,6 +259,30 @@ static int make_tables(Tables &T, const int64_t *first, const int64_t *count, in
                 T.scan_tiles.push_back(ScanTile{(int32_t)h, (int32_t)off});
             T.call_ntiles[h] = (int32_t)(T.scan_tiles.size() - T.call_tile0[h]);
         }
+        int64_t big_before = 0;  // records of the pass-sorted calls before this one
+        int32_t tiles_before = 0;
+        for (int64_t h = 0; h < nh; h++) {
+            const int64_t n = count[h];
+            if (n <= 0) continue;
+            if (n <= kWave) {
+                T.seg_wave.push_back((int32_t)h);
+            } else if (n <= kSegBlockMax) {
+                T.seg_block.push_back((int32_t)h);
+            } else {
+                const int32_t ntc = (int32_t)((n + kSortTile - 1) / kSortTile);
+                for (int32_t t = 0; t < ntc; t++) {
+                    SegTile st{};
+                    st.off = (int32_t)(first[h] + (int64_t)t * kSortTile);
+                    st.n = (int32_t)std::min<int64_t>(kSortTile, n - (int64_t)t * kSortTile);
+                    st.hpos = tiles_before * kSegRadix + t;
+                    st.ntc = ntc;
+                    st.delta = (int32_t)(first[h] - big_before);
+                    T.seg_tiles.push_back(st);
+                }
+                tiles_before += ntc;
+                big_before += n;
+            }
+        }
     }
     return HP_OK;
 }
@@ -279,6 +305,7 @@ struct Engine {
         splitres, dparts, counters, work, depth_buf;
     // unbind
     DevBuf keys0, keys1, vals0, vals1, hist, hsum, srt, callinfo, tsum, tcand, tile_i32;
+    DevBuf seg_wave_l, seg_block_l, seg_tiles_l, seg_hist, seg_hsum;
     // halo properties
     DevBuf pcnt, pscan, pcall, psums, shape_st, shape_part, shape_cd, scale_calls, r_pe, pe_cnt,
         fit_list, fit_n;
@@ -331,7 +358,8 @@ struct Engine {
                          &scan_tiles, &call_tile0, &call_ntiles, &box, &bad, &badlist, &A, &B, &F,
                          &leaf_of, &nodes, &parent, &fat, &small0, &group0, &groupc, &big, &tiles,
                          &parts, &splitres, &dparts, &counters, &work, &depth_buf, &pcnt, &pscan, &pcall, &psums, &shape_st, &shape_part, &shape_cd, &scale_calls, &r_pe, &pe_cnt, &fit_list, &fit_n, &gsegs, &gtiles, &gcen, &keys0, &keys1, &vals0,
-                         &vals1, &hist, &hsum, &srt, &callinfo, &tsum, &tcand, &tile_i32};
+                         &vals1, &hist, &hsum, &srt, &callinfo, &tsum, &tcand, &tile_i32,
+                         &seg_wave_l, &seg_block_l, &seg_tiles_l, &seg_hist, &seg_hsum};
         if (stream) {
             (void)hipStreamSynchronize(stream);
             drop_graphs();
@@ -399,6 +427,9 @@ struct Engine {
             HP_ENSURE(tsum, sizeof(Moments) * (T.scan_tiles.size() + 1));
             HP_ENSURE(tcand, sizeof(Cand) * 2 * (T.scan_tiles.size() + 1));
             HP_ENSURE(tile_i32, sizeof(int32_t) * (T.scan_tiles.size() + 1));
+            const int64_t m = (int64_t)T.seg_tiles.size() * kSegRadix;
+            HP_ENSURE(seg_hist, sizeof(int32_t) * (m + 1));
+            HP_ENSURE(seg_hsum, sizeof(int32_t) * (m / kScanBlock + 2));
         }
         return HP_OK;
     }
@@ -414,6 +445,9 @@ struct Engine {
         if ((r = upload(scan_tiles, T.scan_tiles))) return r;
         if ((r = upload(call_tile0, T.call_tile0))) return r;
         if ((r = upload(call_ntiles, T.call_ntiles))) return r;
+        if ((r = upload(seg_wave_l, T.seg_wave))) return r;
+        if ((r = upload(seg_block_l, T.seg_block))) return r;
+        if ((r = upload(seg_tiles_l, T.seg_tiles))) return r;
         return HP_OK;
     }
 
@@ -964,13 +998,49 @@ struct Engine {
         const int64_t N = T.nrec;
         const int32_t H = (int32_t)T.nhalos;
         if (N == 0 || H == 0) return HP_OK;
-        k_make_keys<<<blocks(N, 256), 256, 0, s>>>(d_po, rec_call.as<int32_t>(), (int32_t)N,
-                                                   keys0.as<uint64_t>(), vals0.as<int32_t>());
+        k_make_keys<<<blocks(N, 256), 256, 0, s>>>(d_po, (int32_t)N, keys0.as<uint64_t>(),
+                                                   vals0.as<int32_t>());
         HP_LAUNCHED();
-        int hbits = 1;
-        while ((1ll << hbits) < H) hbits++;
-        int r;
-        if ((r = enqueue_sort(N, 33 + hbits))) return r;
+        // each call's records sorted inside its own range, into keys1 / vals1
+        uint64_t *k0 = keys0.as<uint64_t>(), *k1 = keys1.as<uint64_t>();
+        int32_t *v0 = vals0.as<int32_t>(), *v1 = vals1.as<int32_t>();
+        const int32_t nw = (int32_t)T.seg_wave.size(), nb = (int32_t)T.seg_block.size();
+        const int32_t ntb = (int32_t)T.seg_tiles.size();
+        if (nw) {
+            k_seg_wave<<<blocks((int64_t)nw * kWave, 256), 256, 0, s>>>(
+                k0, k1, v1, seg_wave_l.as<int32_t>(), nw, hfirst.as<int64_t>(), hcount.as<int64_t>());
+            HP_LAUNCHED();
+        }
+        if (nb) {
+            k_seg_block<<<nb, kSortThreads, 0, s>>>(k0, k1, v1, seg_block_l.as<int32_t>(),
+                                                    hfirst.as<int64_t>(), hcount.as<int64_t>());
+            HP_LAUNCHED();
+        }
+        if (ntb) {
+            static_assert((33 + kSegBits - 1) / kSegBits == 3, "an odd number of passes ends in keys1");
+            const int32_t m = ntb * kSegRadix;
+            const int32_t nbk = (m + kScanBlock - 1) / kScanBlock;
+            uint64_t *ki = k0, *ko = k1;
+            int32_t *vi = v0, *vo = v1;
+            const SegTile *st = seg_tiles_l.as<SegTile>();
+            for (int shift = 0; shift < 33; shift += kSegBits) {
+                k_seg_hist<<<ntb, kSortThreads, 0, s>>>(ki, st, shift, seg_hist.as<int32_t>());
+                HP_LAUNCHED();
+                k_scan_blocks<<<nbk, 1024, 0, s>>>(seg_hist.as<int32_t>(), m, seg_hsum.as<int32_t>());
+                HP_LAUNCHED();
+                k_scan_top<<<1, 1024, 0, s>>>(seg_hsum.as<int32_t>(), nbk);
+                HP_LAUNCHED();
+                k_scan_add<<<nbk, 1024, 0, s>>>(seg_hist.as<int32_t>(), m, seg_hsum.as<int32_t>());
+                HP_LAUNCHED();
+                k_seg_scatter<<<ntb, kSortThreads, 0, s>>>(ki, vi, ko, vo, st, shift,
+                                                           seg_hist.as<int32_t>());
+                HP_LAUNCHED();
+                std::swap(ki, ko);
+                std::swap(vi, vo);
+            }
+        }
+        sorted_keys = k1;
+        sorted_vals = v1;
         struct potential *S = srt.as<struct potential>();
         k_gather_sorted<<<blocks(N, 256), 256, 0, s>>>(d_po, sorted_vals, (int32_t)N, S);
         HP_LAUNCHED();

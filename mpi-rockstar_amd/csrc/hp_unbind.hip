@@ -1,11 +1,16 @@
 // hp_unbind.hip -- the bound pass after the potential
 // (src/properties.c:539-550 and the bound test of src/properties.c:403),
 // for every call of a batch at once:
-//   1. keys: (call, potential-only flag, r2 bits).  A stable LSD radix sort
-//      (8-bit digits) orders every call's records by r2 with the potential-
-//      only records (ke < 0) last; ties keep their input order, which is what
-//      glibc's merge-sort qsort does with the reference's dist_compare
-//      (src/groupies.c:202-210).  r2 >= 0, so its bits sort as unsigned.
+//   1. keys: (potential-only flag, r2 bits), sorted inside each call (the
+//      calls tile the record array in order, so each call's sorted records
+//      stay in its own range): every call's records by r2 with the
+//      potential-only records (ke < 0) last; ties keep their input order,
+//      which is what glibc's merge-sort qsort does with the reference's
+//      dist_compare (src/groupies.c:202-210).  r2 >= 0, so its bits sort as
+//      unsigned.  Calls of at most 64 records are ranked by one wavefront,
+//      calls of at most 1,024 by one workgroup, larger ones by a stable LSD
+//      radix sort (3 passes of 11-bit digits) over tiles that never straddle
+//      two calls.
 //   2. gather the records into sorted order.
 //   3. core / bulk velocity (src/properties.c:165-214).  rvir_max is the
 //      largest j with j^2 > r2^3 (4pi/3 rho_vir)^2 and core_max the largest j
@@ -39,15 +44,14 @@ constexpr int kScanTile = 2048;
 
 // ---------------------------------------------------------------- radix sort
 
-__global__ void k_make_keys(const struct potential *__restrict__ po,
-                            const int32_t *__restrict__ rec_call, int32_t nrec,
+// key of one record inside its call: potential-only flag above the r2 bits
+__global__ void k_make_keys(const struct potential *__restrict__ po, int32_t nrec,
                             uint64_t *__restrict__ keys, int32_t *__restrict__ vals) {
     const int32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= nrec) return;
     const float r2 = po[i].r2;
     const uint64_t pot = po[i].ke < 0.0f ? 1ull : 0ull;
-    keys[i] = ((uint64_t)(uint32_t)rec_call[i] << 33) | (pot << 32) |
-              (uint64_t)__float_as_uint(r2);
+    keys[i] = (pot << 32) | (uint64_t)__float_as_uint(r2);
     vals[i] = i;
 }
 
@@ -185,6 +189,167 @@ __global__ __launch_bounds__(kSortThreads) void k_radix_scatter(
         }
         __syncthreads();
         // the next round starts after this round's keys of each digit
+        if (lead) {
+            atomicAdd(&s_base[dg], s_wc[w][dg]);
+            s_wc[w][dg] = 0;
+        }
+        __syncthreads();
+    }
+}
+
+// ------------------------------------------------------ sort inside calls
+
+// A call of at most 64 records, one wavefront: record i's place is the
+// number of records with a smaller (key, position).
+__global__ __launch_bounds__(256) void k_seg_wave(const uint64_t *__restrict__ kin,
+                                                  uint64_t *__restrict__ kout,
+                                                  int32_t *__restrict__ vout,
+                                                  const int32_t *__restrict__ calls,
+                                                  int32_t ncalls, const int64_t *__restrict__ hfirst,
+                                                  const int64_t *__restrict__ hcount) {
+    const int32_t wv = (int32_t)((blockIdx.x * blockDim.x + threadIdx.x) / kWave);
+    if (wv >= ncalls) return;  // whole wavefronts
+    const int lane = lane_id();
+    const int32_t h = calls[wv];
+    const int32_t first = (int32_t)hfirst[h], n = (int32_t)hcount[h];
+    const bool v = lane < n;
+    const uint64_t key = v ? kin[first + lane] : 0ull;
+    const uint64_t me = (key << 6) | (uint64_t)lane;  // 33-bit key, unique with the position
+    int32_t rank = 0;
+    for (int j = 0; j < n; j++) rank += ((read_lane(key, j) << 6) | (uint64_t)j) < me ? 1 : 0;
+    if (v) {
+        kout[first + rank] = key;
+        vout[first + rank] = first + lane;
+    }
+}
+
+// A call of 65..1024 records, one workgroup: the same ranking, the keys
+// broadcast from LDS.
+constexpr int kSegBlockMax = 1024;
+constexpr int kSegBlockPer = kSegBlockMax / kSortThreads;
+__global__ __launch_bounds__(kSortThreads) void k_seg_block(const uint64_t *__restrict__ kin,
+                                                            uint64_t *__restrict__ kout,
+                                                            int32_t *__restrict__ vout,
+                                                            const int32_t *__restrict__ calls,
+                                                            const int64_t *__restrict__ hfirst,
+                                                            const int64_t *__restrict__ hcount) {
+    __shared__ uint64_t s_k[kSegBlockMax];
+    const int32_t h = calls[blockIdx.x];
+    const int32_t first = (int32_t)hfirst[h], n = (int32_t)hcount[h];
+    // unique keys with the position; padded to a multiple of 8 with keys no
+    // record's key is above
+    const int32_t n8 = (n + 7) & ~7;
+    for (int32_t i = threadIdx.x; i < n8; i += kSortThreads)
+        s_k[i] = i < n ? (kin[first + i] << 10) | (uint64_t)i : ~0ull;
+    __syncthreads();
+    uint64_t mine[kSegBlockPer];
+    int32_t rank[kSegBlockPer];
+#pragma unroll
+    for (int q = 0; q < kSegBlockPer; q++) {
+        const int32_t i = threadIdx.x + q * kSortThreads;
+        mine[q] = i < n ? s_k[i] : ~0ull;
+        rank[q] = 0;
+    }
+    for (int32_t j = 0; j < n8; j += 8) {  // eight broadcast reads in flight
+        uint64_t kk[8];
+#pragma unroll
+        for (int u = 0; u < 8; u++) kk[u] = s_k[j + u];
+#pragma unroll
+        for (int u = 0; u < 8; u++)
+#pragma unroll
+            for (int q = 0; q < kSegBlockPer; q++) rank[q] += kk[u] < mine[q] ? 1 : 0;
+    }
+#pragma unroll
+    for (int q = 0; q < kSegBlockPer; q++) {
+        const int32_t i = threadIdx.x + q * kSortThreads;
+        if (i < n) {
+            kout[first + rank[q]] = mine[q] >> 10;
+            vout[first + rank[q]] = first + i;
+        }
+    }
+}
+
+// Calls of more than 1,024 records: tiles of up to kSortTile records that
+// never straddle two calls.  The digit counts of call c's tiles are stored
+// digit-major in their own block of the count array, entry (d, t) at
+// hpos(t) + d * ntiles(c), so one exclusive scan over the whole array gives
+// each (tile, digit) its place among the large calls' records; delta moves
+// that place into the call's own range.
+constexpr int kSegBits = 11;  // 3 passes cover the 33 key bits
+constexpr int kSegRadix = 1 << kSegBits;
+constexpr int kSegRadixPer = kSegRadix / kSortThreads;
+struct SegTile {
+    int32_t off;    // first record of the tile
+    int32_t n;      // records in the tile
+    int32_t hpos;   // count-array entry of digit 0
+    int32_t ntc;    // tiles of the call
+    int32_t delta;  // call's first record minus the large-call records before it
+    int32_t pad[3];
+};
+
+__global__ __launch_bounds__(kSortThreads) void k_seg_hist(const uint64_t *__restrict__ keys,
+                                                           const SegTile *__restrict__ tiles,
+                                                           int shift, int32_t *__restrict__ hist) {
+    __shared__ int32_t s_h[kSegRadix];
+    const SegTile st = tiles[blockIdx.x];
+    for (int q = 0; q < kSegRadixPer; q++) s_h[q * kSortThreads + threadIdx.x] = 0;
+    __syncthreads();
+    for (int r = 0; r < kSortItems; r++) {
+        const int32_t i = r * kSortThreads + threadIdx.x;
+        if (i < st.n) atomicAdd(&s_h[(keys[st.off + i] >> shift) & (kSegRadix - 1)], 1);
+    }
+    __syncthreads();
+    for (int q = 0; q < kSegRadixPer; q++) {
+        const int d = q * kSortThreads + threadIdx.x;
+        hist[st.hpos + d * st.ntc] = s_h[d];
+    }
+}
+
+// stable scatter of one tile, as k_radix_scatter, into its call's range
+__global__ __launch_bounds__(kSortThreads) void k_seg_scatter(
+    const uint64_t *__restrict__ kin, const int32_t *__restrict__ vin, uint64_t *__restrict__ kout,
+    int32_t *__restrict__ vout, const SegTile *__restrict__ tiles, int shift,
+    const int32_t *__restrict__ offs) {
+    __shared__ int32_t s_base[kSegRadix];
+    __shared__ int32_t s_wc[kSortThreads / kWave][kSegRadix];
+    const int w = threadIdx.x / kWave;
+    const SegTile st = tiles[blockIdx.x];
+    uint64_t key[kSortItems];
+    int32_t val[kSortItems];
+#pragma unroll
+    for (int r = 0; r < kSortItems; r++) {
+        const int32_t i = r * kSortThreads + threadIdx.x;
+        key[r] = i < st.n ? kin[st.off + i] : 0ull;
+        val[r] = i < st.n ? vin[st.off + i] : 0;
+    }
+    for (int q = 0; q < kSegRadixPer; q++) {
+        const int d = q * kSortThreads + threadIdx.x;
+        s_base[d] = offs[st.hpos + d * st.ntc] + st.delta;
+        for (int j = 0; j < kSortThreads / kWave; j++) s_wc[j][d] = 0;
+    }
+    __syncthreads();
+#pragma unroll
+    for (int r = 0; r < kSortItems; r++) {
+        if (r * kSortThreads >= st.n) break;  // block-uniform
+        const int32_t i = r * kSortThreads + threadIdx.x;
+        const bool v = i < st.n;
+        const int dg = (int)((key[r] >> shift) & (kSegRadix - 1));
+        uint64_t peers = ballot(v);
+        for (int b = 0; b < kSegBits; b++) {
+            const uint64_t m = ballot(((dg >> b) & 1) != 0);
+            peers &= ((dg >> b) & 1) ? m : ~m;
+        }
+        const int rk = rank_below(peers);
+        const bool lead = v && rk == 0;
+        if (lead) s_wc[w][dg] = __popcll(peers);
+        __syncthreads();
+        if (v) {
+            int32_t off = s_base[dg];
+            for (int j = 0; j < w; j++) off += s_wc[j][dg];
+            kout[off + rk] = key[r];
+            vout[off + rk] = val[r];
+        }
+        __syncthreads();
         if (lead) {
             atomicAdd(&s_base[dg], s_wc[w][dg]);
             s_wc[w][dg] = 0;
