@@ -106,3 +106,34 @@ def test_plain_kinetic_energy(restatement, request):
     r = po.copy()
     ref.compute_kinetic_energy(r, vc, pc, 1.0, prm)
     np.testing.assert_array_equal(g["ke"], r["ke"])
+
+
+def test_bound_sort_inside_calls_at_every_size():
+    """The sort by r2 inside each call at the sizes where its method changes
+    (one wavefront up to 64 records, one workgroup up to 1,024, radix passes
+    over tiles of 4,096 above), with r2 full of ties and potential-only
+    records spread through each call: the kept records come out in r2 order,
+    ties in their input order (a stable sort, as glibc's merge-sort qsort),
+    and the kept counts are exact."""
+    src = catalog.small_batch(seed=5, n_halos=40, lo=800, hi=4000, potential_only=0.1)
+    sizes = [1, 2, 63, 64, 65, 300, 1023, 1024, 1025, 4096, 4097, 9000]
+    assert sum(sizes) <= src.n_records
+    cat = src.copy()
+    first = np.concatenate([[0], np.cumsum(sizes)[:-1]])
+    halos = np.zeros(len(sizes), cat.halos.dtype)
+    for name in cat.halos.dtype.names:
+        halos[name] = cat.halos[name][:len(sizes)]
+    halos["first"], halos["count"] = first, sizes
+    cat.halos = halos
+    cat.po = np.ascontiguousarray(cat.po[:sum(sizes)])
+    rng = np.random.default_rng(7)
+    cat.po["r2"] = np.round(cat.po["r2"] * 8.0) / 8.0           # many exact ties
+    cat.po["ke"] = np.where(rng.random(cat.po.shape[0]) < 0.15, -1.0, cat.po["ke"])
+    cat.po["flags"] = np.arange(cat.po.shape[0], dtype=np.int32)
+    po_g, h_g, _, _ = run_plan(cat)
+    for h, (f, n) in enumerate(zip(first, sizes)):
+        seg = cat.po[f:f + n]
+        kept = seg["ke"] >= 0
+        want = seg["flags"][kept][np.argsort(seg["r2"][kept], kind="stable")]
+        assert int(h_g["kept"][h]) == want.shape[0], (h, n)
+        np.testing.assert_array_equal(po_g["flags"][f:f + want.shape[0]], want, err_msg=f"call {h} of {n}")

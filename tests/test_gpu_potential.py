@@ -69,10 +69,11 @@ def test_tree_identical(restatement, which):
     _compare_trees(gnodes, perm, onodes, opo)
 
 
-@pytest.mark.parametrize("n", [11, 37, 64, 65, 300])
+@pytest.mark.parametrize("n", [11, 37, 64, 65, 256, 257, 300, 512, 513, 1024, 1025])
 def test_tree_identical_small_calls(restatement, n):
-    """Calls around the size where subtrees move from one wavefront (<= 64
-    points) to a workgroup."""
+    """Calls around the sizes where subtrees move from one wavefront (<= 64
+    points) to a workgroup, between the workgroup kernel's three sizes (256,
+    512 and 1,024 points), and to the split levels above 1,024."""
     cat = catalog.config1()
     prm = cat.params
     halopot.set_config(prm)
