@@ -305,7 +305,7 @@ struct Engine {
         splitres, dparts, counters, work, depth_buf;
     // unbind
     DevBuf keys0, keys1, vals0, vals1, hist, hsum, srt, callinfo, tsum, tcand, tile_i32;
-    DevBuf seg_wave_l, seg_block_l, seg_tiles_l, seg_hist, seg_hsum;
+    DevBuf seg_wave_l, seg_block_l, seg_tiles_l, seg_hist, seg_hsum, srt_rv;
     // halo properties
     DevBuf pcnt, pscan, pcall, psums, shape_st, shape_part, shape_cd, scale_calls, r_pe, pe_cnt,
         fit_list, fit_n;
@@ -359,7 +359,7 @@ struct Engine {
                          &leaf_of, &nodes, &parent, &fat, &small0, &group0, &groupc, &big, &tiles,
                          &parts, &splitres, &dparts, &counters, &work, &depth_buf, &pcnt, &pscan, &pcall, &psums, &shape_st, &shape_part, &shape_cd, &scale_calls, &r_pe, &pe_cnt, &fit_list, &fit_n, &gsegs, &gtiles, &gcen, &keys0, &keys1, &vals0,
                          &vals1, &hist, &hsum, &srt, &callinfo, &tsum, &tcand, &tile_i32,
-                         &seg_wave_l, &seg_block_l, &seg_tiles_l, &seg_hist, &seg_hsum};
+                         &seg_wave_l, &seg_block_l, &seg_tiles_l, &seg_hist, &seg_hsum, &srt_rv};
         if (stream) {
             (void)hipStreamSynchronize(stream);
             drop_graphs();
@@ -423,6 +423,7 @@ struct Engine {
             HP_ENSURE(hist, sizeof(int32_t) * kRadix * nt);
             HP_ENSURE(hsum, sizeof(int32_t) * (kRadix * nt / kScanBlock + 2));
             HP_ENSURE(srt, sizeof(struct potential) * (N + 1));
+            HP_ENSURE(srt_rv, sizeof(float4) * (N + 1));
             HP_ENSURE(callinfo, sizeof(CallInfo) * (H + 1));
             HP_ENSURE(tsum, sizeof(Moments) * (T.scan_tiles.size() + 1));
             HP_ENSURE(tcand, sizeof(Cand) * 2 * (T.scan_tiles.size() + 1));
@@ -1042,7 +1043,8 @@ struct Engine {
         sorted_keys = k1;
         sorted_vals = v1;
         struct potential *S = srt.as<struct potential>();
-        k_gather_sorted<<<blocks(N, 256), 256, 0, s>>>(d_po, sorted_vals, (int32_t)N, S);
+        float4 *RV = srt_rv.as<float4>();
+        k_gather_sorted<<<blocks(N, 256), 256, 0, s>>>(d_po, sorted_vals, (int32_t)N, S, RV);
         HP_LAUNCHED();
         CallInfo *ci = callinfo.as<CallInfo>();
         const int32_t nst = (int32_t)T.scan_tiles.size();
@@ -1052,25 +1054,25 @@ struct Engine {
         HP_LAUNCHED();
         const double rvir_thresh = particle_rvir_dens() * (4.0 * M_PI / 3.0);
         if (nst > 0) {
-            k_rvir_tiles<<<nst, kRvirThreads, 0, s>>>(S, scan_tiles.as<ScanTile>(), nst, ci, rvir_thresh, ti32);
+            k_rvir_tiles<<<nst, kRvirThreads, 0, s>>>(RV, scan_tiles.as<ScanTile>(), nst, ci, rvir_thresh, ti32);
             HP_LAUNCHED();
         }
-        k_rvir_calls<<<blocks(H, 256), 256, 0, s>>>(S, call_tile0.as<int32_t>(),
+        k_rvir_calls<<<blocks(H, 256), 256, 0, s>>>(RV, call_tile0.as<int32_t>(),
                                                     call_ntiles.as<int32_t>(), ti32, H, ci);
         HP_LAUNCHED();
         if (nst > 0) {
-            k_vel_sums<<<nst, kWave, 0, s>>>(S, scan_tiles.as<ScanTile>(), nst, ci,
+            k_vel_sums<<<nst, kWave, 0, s>>>(RV, scan_tiles.as<ScanTile>(), nst, ci,
                                            tsum.as<Moments>());
             HP_LAUNCHED();
             k_vel_prefix<<<blocks(H, 256), 256, 0, s>>>(call_tile0.as<int32_t>(),
                                                         call_ntiles.as<int32_t>(), H,
                                                         tsum.as<Moments>());
             HP_LAUNCHED();
-            k_vel_select<<<nst, kWave, 0, s>>>(S, scan_tiles.as<ScanTile>(), nst, ci,
+            k_vel_select<<<nst, kWave, 0, s>>>(RV, scan_tiles.as<ScanTile>(), nst, ci,
                                              tsum.as<Moments>(), tcand.as<Cand>());
             HP_LAUNCHED();
         }
-        k_vel_finish<<<blocks(H, 256), 256, 0, s>>>(S, call_tile0.as<int32_t>(),
+        k_vel_finish<<<blocks(H, 256), 256, 0, s>>>(RV, call_tile0.as<int32_t>(),
                                                     call_ntiles.as<int32_t>(), H, ci,
                                                     tcand.as<Cand>(), d_halos);
         HP_LAUNCHED();

@@ -360,11 +360,16 @@ __global__ __launch_bounds__(kSortThreads) void k_seg_scatter(
 
 // ----------------------------------------------------------------- gather
 
+// the records in sorted order, and a compact copy of what the core-velocity
+// passes read (r2 and the velocity, 16 B instead of the 48-B record)
 __global__ void k_gather_sorted(const struct potential *__restrict__ po,
                                 const int32_t *__restrict__ order, int32_t nrec,
-                                struct potential *__restrict__ out) {
+                                struct potential *__restrict__ out, float4 *__restrict__ rv) {
     const int32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < nrec) out[i] = po[order[i]];
+    if (i >= nrec) return;
+    const struct potential rec = po[order[i]];
+    out[i] = rec;
+    rv[i] = make_float4(rec.r2, rec.pos[3], rec.pos[4], rec.pos[5]);
 }
 
 // ------------------------------------------------------------ core velocity
@@ -435,7 +440,7 @@ __device__ __forceinline__ int32_t block_sum_i32(int32_t v) {
 // Per tile, then per call.
 // four wavefronts per tile: more loads in flight than one
 constexpr int kRvirThreads = 256;
-__global__ __launch_bounds__(kRvirThreads) void k_rvir_tiles(const struct potential *__restrict__ srt,
+__global__ __launch_bounds__(kRvirThreads) void k_rvir_tiles(const float4 *__restrict__ rv,
                                                              const ScanTile *__restrict__ tiles,
                                                              int32_t ntiles,
                                                              const CallInfo *__restrict__ ci,
@@ -449,7 +454,7 @@ __global__ __launch_bounds__(kRvirThreads) void k_rvir_tiles(const struct potent
     const int32_t end = min(st.off + kScanTile, c.kept);
     int32_t best = -1;
     for (int32_t j = st.off + (int32_t)threadIdx.x; j < end; j += kRvirThreads) {
-        const float r2 = srt[c.first + j].r2;
+        const float r2 = rv[c.first + j].x;
         const float r6 = (r2 * r2) * r2;
         if ((double)((int64_t)j * (int64_t)j) > (double)r6 * (rvir_thresh * rvir_thresh))
             best = j > best ? j : best;
@@ -466,7 +471,7 @@ __global__ __launch_bounds__(kRvirThreads) void k_rvir_tiles(const struct potent
 // per call: rvir_max over the tiles, then core_max, the last j with
 // 100 r2_j < r2[rvir_max], by binary search over the sorted call
 // (src/properties.c:179-184), at least 100
-__global__ void k_rvir_calls(const struct potential *__restrict__ srt,
+__global__ void k_rvir_calls(const float4 *__restrict__ rv,
                              const int32_t *__restrict__ call_tile0,
                              const int32_t *__restrict__ call_ntiles,
                              const int32_t *__restrict__ tile_rvir, int32_t nh,
@@ -474,16 +479,16 @@ __global__ void k_rvir_calls(const struct potential *__restrict__ srt,
     const int32_t h = blockIdx.x * blockDim.x + threadIdx.x;
     if (h >= nh) return;
     CallInfo c = ci[h];
-    int32_t rv = -1;
+    int32_t rvm = -1;
     for (int32_t t = call_tile0[h]; t < call_tile0[h] + call_ntiles[h]; t++)
-        rv = tile_rvir[t] > rv ? tile_rvir[t] : rv;
-    c.rvir_max = rv;
-    if (rv >= 1) {
-        const float ref = srt[c.first + rv].r2;
+        rvm = tile_rvir[t] > rvm ? tile_rvir[t] : rvm;
+    c.rvir_max = rvm;
+    if (rvm >= 1) {
+        const float ref = rv[c.first + rvm].x;
         int32_t lo = 0, hi = c.kept;
         while (lo < hi) {
             const int32_t mid = (lo + hi) >> 1;
-            if ((double)srt[c.first + mid].r2 * 100.0 < (double)ref) lo = mid + 1;
+            if ((double)rv[c.first + mid].x * 100.0 < (double)ref) lo = mid + 1;
             else hi = mid;
         }
         const int32_t cm = lo - 1;
@@ -498,7 +503,7 @@ struct Moments {
 };
 
 // per-tile sums of v and v^2 over the tile's part of [0, rvir_max)
-__global__ __launch_bounds__(kWave) void k_vel_sums(const struct potential *__restrict__ srt,
+__global__ __launch_bounds__(kWave) void k_vel_sums(const float4 *__restrict__ rv,
                                                     const ScanTile *__restrict__ tiles,
                                                     int32_t ntiles,
                                                     const CallInfo *__restrict__ ci,
@@ -510,7 +515,8 @@ __global__ __launch_bounds__(kWave) void k_vel_sums(const struct potential *__re
     const int32_t end = min(st.off + kScanTile, c.rvir_max);
     double m[6] = {0, 0, 0, 0, 0, 0};
     for (int32_t j = st.off + lane_id(); j < end; j += kWave) {
-        const float *v = srt[c.first + j].pos + 3;
+        const float4 q = rv[c.first + j];
+        const float v[3] = {q.y, q.z, q.w};
         for (int k = 0; k < 3; k++) {
             const double x = (double)v[k];
             m[k] += x;
@@ -550,14 +556,14 @@ __global__ void k_vel_prefix(const int32_t *__restrict__ call_tile0,
 struct Cand {
     double val;
     int32_t idx;
-    float mean[3];
+    double s1[3];  // the prefix's velocity sums: the mean is s1 / (idx + 1), taken at the end
 };
 
 __device__ __forceinline__ Cand no_cand() {
     Cand c;
     c.val = __longlong_as_double(0x7ff0000000000000ll);
     c.idx = 0x7fffffff;
-    c.mean[0] = c.mean[1] = c.mean[2] = 0.0f;
+    c.s1[0] = c.s1[1] = c.s1[2] = 0.0;
     return c;
 }
 
@@ -572,9 +578,9 @@ __device__ __forceinline__ Cand wave_better(Cand c) {
         Cand d;
         d.val = __shfl_xor(c.val, o, 64);
         d.idx = __shfl_xor(c.idx, o, 64);
-        d.mean[0] = __shfl_xor(c.mean[0], o, 64);
-        d.mean[1] = __shfl_xor(c.mean[1], o, 64);
-        d.mean[2] = __shfl_xor(c.mean[2], o, 64);
+        d.s1[0] = __shfl_xor(c.s1[0], o, 64);
+        d.s1[1] = __shfl_xor(c.s1[1], o, 64);
+        d.s1[2] = __shfl_xor(c.s1[2], o, 64);
         c = better(c, d);
     }
     return c;
@@ -583,7 +589,7 @@ __device__ __forceinline__ Cand wave_better(Cand c) {
 // per tile: inclusive prefix moments per record -> var_i / ((i - 3) i), and
 // the tile's best candidates for the core window and the bulk window.  One
 // wavefront per tile, rounds of 64 records carried lane 63 to the next.
-__global__ __launch_bounds__(kWave) void k_vel_select(const struct potential *__restrict__ srt,
+__global__ __launch_bounds__(kWave) void k_vel_select(const float4 *__restrict__ rv,
                                                       const ScanTile *__restrict__ tiles,
                                                       int32_t ntiles,
                                                       const CallInfo *__restrict__ ci,
@@ -608,7 +614,8 @@ __global__ __launch_bounds__(kWave) void k_vel_select(const struct potential *__
         const bool v = j < end;
         double x[6] = {0, 0, 0, 0, 0, 0};
         if (v) {
-            const float *vel = srt[c.first + j].pos + 3;
+            const float4 q = rv[c.first + j];
+            const float vel[3] = {q.y, q.z, q.w};
             for (int k = 0; k < 3; k++) {
                 x[k] = (double)vel[k];
                 x[3 + k] = x[k] * x[k];
@@ -626,7 +633,7 @@ __global__ __launch_bounds__(kWave) void k_vel_select(const struct potential *__
             Cand me;
             me.val = j > 3 ? var / (double)((int64_t)(j - 3) * j) : 0.0;
             me.idx = j;
-            for (int k = 0; k < 3; k++) me.mean[k] = (float)(y[k] / n);
+            for (int k = 0; k < 3; k++) me.s1[k] = y[k];
             if (j < core_lim) bc = better(bc, me);
             bb = better(bb, me);
         }
@@ -640,11 +647,12 @@ __global__ __launch_bounds__(kWave) void k_vel_select(const struct potential *__
 }
 
 // mean and normalised variance of the first j + 1 records, j < 10, directly
-__device__ void small_prefix(const struct potential *__restrict__ srt, int32_t first, int32_t j,
-                             float *mean, double *val) {
+__device__ void small_prefix(const float4 *__restrict__ rv, int32_t first, int32_t j,
+                             double *sum, double *val) {
     double s[3] = {0, 0, 0}, q[3] = {0, 0, 0};
     for (int32_t i = 0; i <= j; i++) {
-        const float *v = srt[first + i].pos + 3;
+        const float4 w4 = rv[first + i];
+        const float v[3] = {w4.y, w4.z, w4.w};
         for (int k = 0; k < 3; k++) {
             s[k] += (double)v[k];
             q[k] += (double)v[k] * (double)v[k];
@@ -653,14 +661,14 @@ __device__ void small_prefix(const struct potential *__restrict__ srt, int32_t f
     const double n = (double)(j + 1);
     double var = 0.0;
     for (int k = 0; k < 3; k++) {
-        mean[k] = (float)(s[k] / n);
+        sum[k] = s[k];
         var += q[k] - s[k] * s[k] / n;
     }
     *val = j > 3 ? var / (double)((int64_t)(j - 3) * j) : 0.0;
 }
 
 // one thread per call: final selection, halo fields
-__global__ void k_vel_finish(const struct potential *__restrict__ srt,
+__global__ void k_vel_finish(const float4 *__restrict__ rv,
                              const int32_t *__restrict__ call_tile0,
                              const int32_t *__restrict__ call_ntiles, int32_t nh,
                              const CallInfo *__restrict__ ci, const Cand *__restrict__ tcand,
@@ -682,19 +690,20 @@ __global__ void k_vel_finish(const struct potential *__restrict__ srt,
         // windows of at most 10 records: the last forced update wins
         if (core_lim <= 10) {
             a.idx = core_lim - 1;
-            small_prefix(srt, c.first, a.idx, a.mean, &a.val);
+            small_prefix(rv, c.first, a.idx, a.s1, &a.val);
         }
         if (lim <= 10) {
             b.idx = lim - 1;
-            small_prefix(srt, c.first, b.idx, b.mean, &b.val);
+            small_prefix(rv, c.first, b.idx, b.s1, &b.val);
         }
         hh.n_core = a.idx;
         hh.min_vel_err = (float)a.val;
         hh.min_bulkvel_err = (float)b.val;
+        const double na = (double)a.idx + 1.0, nb = (double)b.idx + 1.0;
         for (int k = 0; k < 3; k++) {
-            hh.corevel[k] = a.mean[k];
-            hh.bulkvel[k] = b.mean[k];
-            hh.pos[3 + k] = a.mean[k];  // src/properties.c:213
+            hh.corevel[k] = (float)(a.s1[k] / na);
+            hh.bulkvel[k] = (float)(b.s1[k] / nb);
+            hh.pos[3 + k] = hh.corevel[k];  // src/properties.c:213
         }
     }
     halos[h] = hh;
