@@ -23,7 +23,8 @@
 //             child boxes) -> scatter pass; host-driven, several levels per
 //             counter read.
 //   subtrees  every node of at most kSmallMax points is finished in one go:
-//             65..kSmallMax points by a workgroup, point-parallel per level
+//             65..kSmallMax points by a workgroup (a kernel each for nodes of
+//             at most 256, 512 and 1,024 points), point-parallel per level
 //             (block prefix of the left flags, segmented DPP scans for the
 //             child boxes); at most 64 points by one wavefront, a point per
 //             lane.  Both keep the points and the node table in LDS, compute

@@ -11,7 +11,7 @@
 // Pipeline of one batch (every call of the batch together):
 //   gather + roots -> split levels until no node has more than 10 points ->
 //   centres of mass -> dmin -> walk (or the direct kernel) -> [bound pass:
-//   radix sort by (call, potential-only, r2) -> core/bulk velocity -> kinetic
+//   each call's records sorted by (potential-only, r2) -> core/bulk velocity -> kinetic
 //   energy + bound flags -> records written back in sorted order].
 // The split levels are host-driven: after each level the engine reads the
 // device counters (four integers) to size the next level's launches.
@@ -335,9 +335,9 @@ struct Engine {
         HP_TRY(hipHostMalloc((void **)&h_counters, sizeof(int32_t) * kNumCounters));
         HP_TRY(hipHostMalloc((void **)&h_work, sizeof(unsigned long long) * 4));
         for (auto &e : ev) HP_TRY(hipEventCreate(&e));
-        // The subtree kernels take tasks from dynamic queues until the lists
-        // are empty; their grids are the workgroups that fit on the device
-        // at once.
+        // The workgroup subtree kernels take tasks from dynamic queues, the
+        // wavefront one at a fixed stride, until the lists are empty; their
+        // grids are the workgroups that fit on the device at once.
         int per_cu = 0;
         HP_TRY(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_subtree<256>, kSubThreads, 0));
         sub_grid[0] = std::max(1, per_cu) * prop.multiProcessorCount;
@@ -391,8 +391,8 @@ struct Engine {
         HP_ENSURE(B, sizeof(float4) * (N + 1));
         HP_ENSURE(F, sizeof(float4) * (N + 1));
         HP_ENSURE(leaf_of, sizeof(int32_t) * (N + 1));
-        // split levels use at most about 0.1N + H ids; a workgroup subtree of
-        // n points reserves 2n - 2, a wavefront subtree what it uses
+        // split levels use at most about 0.1N + H ids; a subtree of n points
+        // (workgroup or wavefront) reserves 2n - 2
         cap_nodes = 5 * N + H + 1024;
         HP_ENSURE(nodes, sizeof(Node) * cap_nodes);
         HP_ENSURE(parent, sizeof(int32_t) * cap_nodes);
