@@ -576,8 +576,10 @@ __global__ __launch_bounds__(256) void k_group_classes(const Task *__restrict__ 
 }
 
 // One workgroup per task of the list `cls` (nodes of at most kCap points).
+// The smallest size asks the compiler for 7 wavefronts per SIMD (72 VGPRs):
+// its LDS would fit twice the workgroups its registers allowed.
 template <int kCap>
-__global__ __launch_bounds__(kSubThreads) void k_subtree(const float4 *__restrict__ A,
+__global__ __launch_bounds__(kSubThreads, kCap <= 256 ? 7 : 1) void k_subtree(const float4 *__restrict__ A,
                                                          const float4 *__restrict__ B,
                                                          const Task *__restrict__ tasks,
                                                          BuildArgs a, double particle_mass,
@@ -963,51 +965,31 @@ constexpr int kWaveNodes = 2 * kWaveTask;  // a tree over n points has at most 2
 
 // Node ids of the wavefront subtrees, assigned before they are built: a
 // subtree of n points gets 2n - 2 ids (the most it can need; ids it does not
-// use are never referenced) from one block above every id in use, its first
-// kept in the task's tile0.  One workgroup scans the list, so the subtree
-// kernel needs no atomic per task.  Runs alone on the stream: nothing else
-// allocates node ids meanwhile.
+// use are never referenced) from one block above every id in use.  Task k's
+// first id is counters[kWaveIdBase] plus the exclusive prefix of the needs
+// before it: k_wave_need, the library's exclusive scan, then k_wave_base.
+// The subtree kernel then needs no atomic per task.
 __host__ __device__ __forceinline__ int32_t wave_ids(int32_t n) { return 2 * n - 2; }
-__global__ __launch_bounds__(1024) void k_wave_ids(Task *__restrict__ tasks, BuildArgs a) {
-    __shared__ int32_t s_w[16];
+__global__ void k_wave_need(const Task *__restrict__ tasks, BuildArgs a, int32_t *__restrict__ need,
+                            int32_t m) {
+    const int32_t ntasks = min(a.counters[kSmallNext], a.cap_small);
+    for (int32_t k = blockIdx.x * blockDim.x + threadIdx.x; k < m; k += gridDim.x * blockDim.x)
+        need[k] = k < ntasks ? wave_ids(tasks[k].pad[1]) : 0;
+}
+// after the scan (off holds cap_small + 1 entries): the block of ids
+__global__ void k_wave_base(BuildArgs a, const int32_t *__restrict__ off) {
+    if (threadIdx.x != 0 || blockIdx.x != 0) return;
     const int32_t ntasks = min(a.counters[kSmallNext], a.cap_small);
     const int32_t base0 = a.counters[kNodeCount];
-    const int w = threadIdx.x / kWave, lane = lane_id();
-    constexpr int kPer = 32;  // consecutive tasks per thread, loaded together
-    int32_t carry = 0;
-    for (int32_t k0 = 0; k0 < ntasks; k0 += 1024 * kPer) {
-        const int32_t kt = k0 + threadIdx.x * kPer;
-        int32_t need[kPer], mine = 0;
-#pragma unroll
-        for (int q = 0; q < kPer; q++) {  // clamped, unconditional: the loads issue together
-            const int32_t np = tasks[min(kt + q, ntasks - 1)].pad[1];
-            need[q] = kt + q < ntasks ? wave_ids(np) : 0;
-            mine += need[q];
-        }
-        const int32_t incl = wave_scan_add(mine);
-        if (lane == kWave - 1) s_w[w] = incl;
-        __syncthreads();
-        int32_t before = carry, total = carry;
-        for (int j = 0; j < 16; j++) {
-            before += j < w ? s_w[j] : 0;
-            total += s_w[j];
-        }
-        int32_t at = base0 + before + incl - mine;
-#pragma unroll
-        for (int q = 0; q < kPer; q++) {
-            if (kt + q < ntasks) tasks[kt + q].tile0 = at;
-            at += need[q];
-        }
-        carry = total;
-        __syncthreads();
-    }
-    if (threadIdx.x == 0) a.counters[kNodeCount] = base0 + carry;
+    a.counters[kWaveIdBase] = base0;
+    a.counters[kNodeCount] = base0 + off[ntasks];
 }
 
 __global__ __launch_bounds__(kWave) void k_subtree_wave(const float4 *__restrict__ A,
                                                         const float4 *__restrict__ B,
                                                         const Task *__restrict__ tasks,
-                                                        BuildArgs a, double particle_mass) {
+                                                        BuildArgs a, double particle_mass,
+                                                        const int32_t *__restrict__ wave_off) {
     __shared__ float4 P[kWaveTask];
     __shared__ int32_t Q[kWaveTask];
     __shared__ float s_lo[3][kWaveNodes], s_hi[3][kWaveNodes];
@@ -1017,6 +999,7 @@ __global__ __launch_bounds__(kWave) void k_subtree_wave(const float4 *__restrict
     __shared__ int16_t s_lvl[kWaveTask + 2];  // first local id of each level
     const int lane = lane_id();
     const int32_t ntasks = min(a.counters[kSmallNext], a.cap_small);
+    const int32_t id_base = a.counters[kWaveIdBase];
     // tasks k = blockIdx.x, blockIdx.x + gridDim.x, ...: they are all small
     // and alike, and a fixed stride needs no queue atomic per task
     for (int32_t k = blockIdx.x; k < ntasks; k += gridDim.x) {
@@ -1123,14 +1106,14 @@ __global__ __launch_bounds__(kWave) void k_subtree_wave(const float4 *__restrict
             pend = fresh && cnt_new > kLeafPoints;
         }
         // every lane's node is a leaf now; local id j > 0 is node base + j - 1
-        // of the ids k_wave_ids reserved
+        // of the ids reserved for the task
         if (lane == 0) {
             for (int d = 0; d < 3; d++) {
                 s_lo[d][0] = root.lo[d];
                 s_hi[d][0] = root.hi[d];
             }
         }
-        int32_t base = t.tile0;
+        int32_t base = id_base + wave_off[k];
         if (M > 1 && base + wave_ids(n) > a.cap_nodes) {
             if (lane == 0) atomicOr(&a.counters[kErrFlags], kErrCapacity);
             base = -1;

@@ -103,7 +103,8 @@ enum Counter {
     kGroupNext = 17,  // workgroup subtree tasks (wave_max < n <= big_threshold), append-only
     kClassNext = 18,    // [18, 21): workgroup subtree tasks of each kernel size
     kClassCursor = 21,  // [21, 24): next task of each size to start (dynamic queues)
-    kNumCounters = 24
+    kWaveIdBase = 24,   // first node id of the wavefront subtrees' block
+    kNumCounters = 25
 };
 
 enum ErrFlag { kErrWalkCap = 1, kErrBadTree = 2, kErrCapacity = 4 };

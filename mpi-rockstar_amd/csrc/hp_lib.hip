@@ -301,7 +301,7 @@ struct Engine {
     DevBuf hfirst, hcount, root_tiles, groups, direct_blocks, rec_call, scan_tiles, call_tile0,
         call_ntiles;
     // build
-    DevBuf box, bad, badlist, A, B, F, leaf_of, nodes, parent, fat, small0, group0, groupc, big, tiles, parts,
+    DevBuf box, bad, badlist, A, B, F, leaf_of, nodes, parent, fat, small0, group0, groupc, wave_off, wave_hsum, big, tiles, parts,
         splitres, dparts, counters, work, depth_buf;
     // unbind
     DevBuf keys0, keys1, vals0, vals1, hist, hsum, srt, callinfo, tsum, tcand, tile_i32;
@@ -356,7 +356,7 @@ struct Engine {
     ~Engine() {
         DevBuf *all[] = {&hfirst, &hcount, &root_tiles, &groups, &direct_blocks, &rec_call,
                          &scan_tiles, &call_tile0, &call_ntiles, &box, &bad, &badlist, &A, &B, &F,
-                         &leaf_of, &nodes, &parent, &fat, &small0, &group0, &groupc, &big, &tiles,
+                         &leaf_of, &nodes, &parent, &fat, &small0, &group0, &groupc, &wave_off, &wave_hsum, &big, &tiles,
                          &parts, &splitres, &dparts, &counters, &work, &depth_buf, &pcnt, &pscan, &pcall, &psums, &shape_st, &shape_part, &shape_cd, &scale_calls, &r_pe, &pe_cnt, &fit_list, &fit_n, &gsegs, &gtiles, &gcen, &keys0, &keys1, &vals0,
                          &vals1, &hist, &hsum, &srt, &callinfo, &tsum, &tcand, &tile_i32,
                          &seg_wave_l, &seg_block_l, &seg_tiles_l, &seg_hist, &seg_hsum, &srt_rv};
@@ -400,6 +400,8 @@ struct Engine {
         HP_ENSURE(fat, sizeof(int32_t) * cap_fat);
         cap_small = N / (kLeafPoints + 1) + H + 64;
         HP_ENSURE(small0, sizeof(Task) * cap_small);
+        HP_ENSURE(wave_off, sizeof(int32_t) * (cap_small + 2));
+        HP_ENSURE(wave_hsum, sizeof(int32_t) * ((cap_small + 1) / kScanBlock + 2));
         cap_group = N / (kWaveMin + 1) + H + 64;  // tasks of more than wave_max >= kWaveMin points
         HP_ENSURE(group0, sizeof(Task) * cap_group);
         HP_ENSURE(groupc, sizeof(Task) * kSubClasses * cap_group);
@@ -570,10 +572,24 @@ struct Engine {
         k_subtree<256><<<sub_grid[0], kSubThreads, 0, s>>>(A.as<float4>(), B.as<float4>(),
                                                            groupc.as<Task>(), ba, PARTICLE_MASS, 0);
         HP_LAUNCHED();
-        k_wave_ids<<<1, 1024, 0, s>>>(small0.as<Task>(), ba);
-        HP_LAUNCHED();
+        {  // the wavefront subtrees' node ids: exclusive scan of 2n - 2 over the list
+            const int32_t m = (int32_t)cap_small + 1;
+            const int32_t nbk = (m + kScanBlock - 1) / kScanBlock;
+            int32_t *off = wave_off.as<int32_t>();
+            k_wave_need<<<std::min(blocks(m, 256), 4096), 256, 0, s>>>(small0.as<Task>(), ba, off, m);
+            HP_LAUNCHED();
+            k_scan_blocks<<<nbk, 1024, 0, s>>>(off, m, wave_hsum.as<int32_t>());
+            HP_LAUNCHED();
+            k_scan_top<<<1, 1024, 0, s>>>(wave_hsum.as<int32_t>(), nbk);
+            HP_LAUNCHED();
+            k_scan_add<<<nbk, 1024, 0, s>>>(off, m, wave_hsum.as<int32_t>());
+            HP_LAUNCHED();
+            k_wave_base<<<1, 64, 0, s>>>(ba, off);
+            HP_LAUNCHED();
+        }
         k_subtree_wave<<<wave_grid, kWave, 0, s>>>(A.as<float4>(), B.as<float4>(), small0.as<Task>(),
-                                              build_args(small0.as<Task>()), PARTICLE_MASS);
+                                              build_args(small0.as<Task>()), PARTICLE_MASS,
+                                              wave_off.as<int32_t>());
         HP_LAUNCHED();
         k_leaf_com<<<2048, 256, 0, s>>>(nodes.as<Node>(), F.as<float4>(), cnt, PARTICLE_MASS);
         HP_LAUNCHED();
