@@ -576,8 +576,9 @@ __global__ __launch_bounds__(256) void k_group_classes(const Task *__restrict__ 
 }
 
 // One workgroup per task of the list `cls` (nodes of at most kCap points).
-// The smallest size asks the compiler for 7 wavefronts per SIMD (72 VGPRs):
-// its LDS would fit twice the workgroups its registers allowed.
+// The smallest size asks the compiler for 7 wavefronts per SIMD (72 VGPRs,
+// no spills): its LDS fits more workgroups than its registers otherwise
+// allowed.
 template <int kCap>
 __global__ __launch_bounds__(kSubThreads, kCap <= 256 ? 7 : 1) void k_subtree(const float4 *__restrict__ A,
                                                          const float4 *__restrict__ B,
