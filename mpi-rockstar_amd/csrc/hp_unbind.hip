@@ -8,7 +8,8 @@
 //      which is what glibc's merge-sort qsort does with the reference's
 //      dist_compare (src/groupies.c:202-210).  r2 >= 0, so its bits sort as
 //      unsigned.  Calls of at most 64 records are ranked by one wavefront,
-//      calls of at most 1,024 by one workgroup, larger ones by a stable LSD
+//      calls of at most 1,024 sorted by one workgroup (bitonic, in LDS),
+//      larger ones by a stable LSD
 //      radix sort (3 passes of 11-bit digits) over tiles that never straddle
 //      two calls.
 //   2. gather the records into sorted order.
@@ -223,10 +224,10 @@ __global__ __launch_bounds__(256) void k_seg_wave(const uint64_t *__restrict__ k
     }
 }
 
-// A call of 65..1024 records, one workgroup: the same ranking, the keys
-// broadcast from LDS.
+// A call of 65..1024 records, one workgroup: a bitonic sort in LDS of the
+// (key, position) pairs, unique, so the order is the stable one; padded to
+// a power of two with keys above every record's.
 constexpr int kSegBlockMax = 1024;
-constexpr int kSegBlockPer = kSegBlockMax / kSortThreads;
 __global__ __launch_bounds__(kSortThreads) void k_seg_block(const uint64_t *__restrict__ kin,
                                                             uint64_t *__restrict__ kout,
                                                             int32_t *__restrict__ vout,
@@ -236,36 +237,30 @@ __global__ __launch_bounds__(kSortThreads) void k_seg_block(const uint64_t *__re
     __shared__ uint64_t s_k[kSegBlockMax];
     const int32_t h = calls[blockIdx.x];
     const int32_t first = (int32_t)hfirst[h], n = (int32_t)hcount[h];
-    // unique keys with the position; padded to a multiple of 8 with keys no
-    // record's key is above
-    const int32_t n8 = (n + 7) & ~7;
-    for (int32_t i = threadIdx.x; i < n8; i += kSortThreads)
+    int32_t np2 = kWave;
+    while (np2 < n) np2 <<= 1;  // n <= kSegBlockMax, so np2 <= kSegBlockMax
+    for (int32_t i = threadIdx.x; i < np2; i += kSortThreads)
         s_k[i] = i < n ? (kin[first + i] << 10) | (uint64_t)i : ~0ull;
     __syncthreads();
-    uint64_t mine[kSegBlockPer];
-    int32_t rank[kSegBlockPer];
-#pragma unroll
-    for (int q = 0; q < kSegBlockPer; q++) {
-        const int32_t i = threadIdx.x + q * kSortThreads;
-        mine[q] = i < n ? s_k[i] : ~0ull;
-        rank[q] = 0;
-    }
-    for (int32_t j = 0; j < n8; j += 8) {  // eight broadcast reads in flight
-        uint64_t kk[8];
-#pragma unroll
-        for (int u = 0; u < 8; u++) kk[u] = s_k[j + u];
-#pragma unroll
-        for (int u = 0; u < 8; u++)
-#pragma unroll
-            for (int q = 0; q < kSegBlockPer; q++) rank[q] += kk[u] < mine[q] ? 1 : 0;
-    }
-#pragma unroll
-    for (int q = 0; q < kSegBlockPer; q++) {
-        const int32_t i = threadIdx.x + q * kSortThreads;
-        if (i < n) {
-            kout[first + rank[q]] = mine[q] >> 10;
-            vout[first + rank[q]] = first + i;
+    for (int32_t k = 2; k <= np2; k <<= 1) {
+        for (int32_t j = k >> 1; j > 0; j >>= 1) {
+            for (int32_t i = threadIdx.x; i < np2; i += kSortThreads) {
+                const int32_t ixj = i ^ j;
+                if (ixj > i) {
+                    const uint64_t a = s_k[i], b = s_k[ixj];
+                    if ((a > b) == ((i & k) == 0)) {
+                        s_k[i] = b;
+                        s_k[ixj] = a;
+                    }
+                }
+            }
+            __syncthreads();
         }
+    }
+    for (int32_t i = threadIdx.x; i < n; i += kSortThreads) {
+        const uint64_t kk = s_k[i];
+        kout[first + i] = kk >> 10;
+        vout[first + i] = first + (int32_t)(kk & 1023);
     }
 }
 
