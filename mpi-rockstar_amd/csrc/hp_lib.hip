@@ -201,7 +201,6 @@ struct Tables {
     std::vector<RootTile> root_tiles;
     std::vector<Group> groups;          // walk wavefronts
     std::vector<int2> direct_blocks;    // direct-kernel workgroups
-    std::vector<int32_t> rec_call;      // record -> call
     std::vector<ScanTile> scan_tiles;
     std::vector<int32_t> call_tile0, call_ntiles;
     std::vector<int32_t> seg_wave, seg_block;  // calls sorted by one wavefront / workgroup
@@ -249,11 +248,9 @@ static int make_tables(Tables &T, const int64_t *first, const int64_t *count, in
         return count[x.root] > count[y.root];
     });
     if (want_unbind) {
-        T.rec_call.resize(pos);
         T.call_tile0.resize(nh);
         T.call_ntiles.resize(nh);
         for (int64_t h = 0; h < nh; h++) {
-            for (int64_t i = 0; i < count[h]; i++) T.rec_call[first[h] + i] = (int32_t)h;
             T.call_tile0[h] = (int32_t)T.scan_tiles.size();
             for (int64_t off = 0; off < count[h]; off += kScanTile)
                 T.scan_tiles.push_back(ScanTile{(int32_t)h, (int32_t)off});
@@ -298,7 +295,7 @@ struct Engine {
     int sub_grid[kSubClasses] = {}, wave_grid = 0;  // resident workgroups of the subtree kernels
 
     // batch tables on the device
-    DevBuf hfirst, hcount, root_tiles, groups, direct_blocks, rec_call, scan_tiles, call_tile0,
+    DevBuf hfirst, hcount, root_tiles, groups, direct_blocks, scan_tiles, call_tile0,
         call_ntiles;
     // build
     DevBuf box, bad, badlist, A, B, F, leaf_of, nodes, parent, fat, small0, group0, groupc, wave_off, wave_hsum, big, tiles, parts,
@@ -331,7 +328,10 @@ struct Engine {
         if (std::strncmp(prop.gcnArchName, "gfx950", 6) != 0)
             return set_error(HP_ENODEV, "device %d is %s; libhalopot is built for gfx950", device,
                              prop.gcnArchName);
-        HP_TRY(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking));
+        // A blocking stream: it waits for work the caller left on the null
+        // stream (a device-to-device hipMemcpy, for one, can return before
+        // the copy is done), so the records are complete when read.
+        HP_TRY(hipStreamCreateWithFlags(&stream, hipStreamDefault));
         HP_TRY(hipHostMalloc((void **)&h_counters, sizeof(int32_t) * kNumCounters));
         HP_TRY(hipHostMalloc((void **)&h_work, sizeof(unsigned long long) * 4));
         for (auto &e : ev) HP_TRY(hipEventCreate(&e));
@@ -354,7 +354,7 @@ struct Engine {
     }
 
     ~Engine() {
-        DevBuf *all[] = {&hfirst, &hcount, &root_tiles, &groups, &direct_blocks, &rec_call,
+        DevBuf *all[] = {&hfirst, &hcount, &root_tiles, &groups, &direct_blocks,
                          &scan_tiles, &call_tile0, &call_ntiles, &box, &bad, &badlist, &A, &B, &F,
                          &leaf_of, &nodes, &parent, &fat, &small0, &group0, &groupc, &wave_off, &wave_hsum, &big, &tiles,
                          &parts, &splitres, &dparts, &counters, &work, &depth_buf, &pcnt, &pscan, &pcall, &psums, &shape_st, &shape_part, &shape_cd, &scale_calls, &r_pe, &pe_cnt, &fit_list, &fit_n, &gsegs, &gtiles, &gcen, &keys0, &keys1, &vals0,
@@ -444,7 +444,6 @@ struct Engine {
         if ((r = upload(root_tiles, T.root_tiles))) return r;
         if ((r = upload(groups, T.groups))) return r;
         if ((r = upload(direct_blocks, T.direct_blocks))) return r;
-        if ((r = upload(rec_call, T.rec_call))) return r;
         if ((r = upload(scan_tiles, T.scan_tiles))) return r;
         if ((r = upload(call_tile0, T.call_tile0))) return r;
         if ((r = upload(call_ntiles, T.call_ntiles))) return r;
