@@ -5,9 +5,9 @@ A step is one hp_plan_run over the whole synthetic catalogue: the
 Barnes-Hut potential of every call (src/potential.c:183-192) and the bound
 pass after it (src/properties.c:539-550: drop potential-only records, sort by
 r2, core / bulk velocity, kinetic energy, bound flag).  Records and halo
-descriptors are resident in HBM before the timed region; each step first
-restores the pristine input with a device-to-device copy (the pass reorders
-records in place), which is included in the timing.
+descriptors are resident in HBM before the timed region; before each step
+the pristine input is restored with a device-to-device copy (the pass
+reorders records in place), waited for, and kept out of the step's clock.
 
 Default workload: the one-GPU catalogue, 16,777,216 particles in a
 250 Mpc/h box of which 40% (6,710,886) sit in 50,000 NFW halos with counts
