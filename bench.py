@@ -13,13 +13,17 @@ Default workload: the one-GPU catalogue, 16,777,216 particles in a
 250 Mpc/h box of which 40% (6,710,886) sit in 50,000 NFW halos with counts
 from n^-1.9 on [20, 500000] (seeded); every halo is one compute_potential
 call, all of them in one batched plan replayed from hipGraphs.  Beside it,
-the same catalogue through the plain per-halo GPU path (every call with its
-own launches), through the host-buffer entry point (PCIe copies included),
+the same catalogue through the plain per-halo GPU path (the drop-in
+compute_potential / compute_kinetic_energy called per halo from a pool of
+host threads, as the reference's OpenMP driver calls them, and a per-call
+plan on one stream), through the host-buffer entry point (PCIe included),
 and through the reference's CPU path (its own compute_potential /
 compute_kinetic_energy from oracle/_ref, OpenMP over halos) on a bounded
 sample of host time.  After the timed steps, the halo property pass
 (hp_plan_props) is timed on the last step's results, with the reference's
 own property code beside it (reported under "halo_props", not in "value").
+Outside the timed region, pe of the benched arithmetic is compared with the
+FP64 evaluation of the same trees ("accuracy"); the run fails above 1e-4.
 
 Prints ONE JSON line.
 """
@@ -37,8 +41,10 @@ sys.path.insert(0, str(ROOT / "mpi-rockstar_amd"))
 
 METRIC = "halo-particle potentials/s, batched vs per-halo GPU, ≤1e-4 rel err vs FP64"
 UNIT = "halo-particle potentials/s"
-FP32_PEAK_TFLOPS = 157.3   # MI355X FP32 (vector, dual-issue / packed), published
+FP32_PEAK_TFLOPS = 157.3   # MI355X FP32 vector, published (packed / dual-issue FP32)
+FP32_NONPACKED_TFLOPS = 78.6  # the same without packed FP32, which this build does not emit
 HBM_PEAK_GBS = 8000.0
+ACCURACY_BAR = 1e-4        # the metric's relative-error bar vs FP64
 
 WORKLOADS = {
     "c2": "batching: 2000 NFW halos, n ~ n^-1.9 on [20, 200000], c ~ lognormal(8, 0.3), "
@@ -55,6 +61,9 @@ def log(msg: str) -> None:
 
 
 def cpu_threads() -> int:
+    """Host threads for the CPU legs: OMP_NUM_THREADS when set (the GPU box
+    sets it to its CPU share for one GPU, 16), else the cores this process
+    may run on, at most 16."""
     env = os.environ.get("OMP_NUM_THREADS")
     if env and env.isdigit():
         return int(env)
@@ -62,6 +71,23 @@ def cpu_threads() -> int:
         return min(16, len(os.sched_getaffinity(0)))
     except AttributeError:
         return min(16, os.cpu_count() or 1)
+
+
+def cpu_model() -> dict:
+    """The host CPU as lscpu names it (from /proc/cpuinfo), and its counts."""
+    model = "unknown"
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                model = line.split(":", 1)[1].strip()
+                break
+    except OSError:
+        pass
+    try:
+        allowed = len(os.sched_getaffinity(0))
+    except AttributeError:
+        allowed = os.cpu_count() or 1
+    return {"model": model, "logical_cpus": os.cpu_count(), "cpus_allowed": allowed}
 
 
 def walk_traffic(workload: str):
@@ -81,6 +107,8 @@ def run_cpu_baseline(cat, seconds: float):
     """The reference's CPU path on the host cores: oracle/_ref's compiled
     compute_potential + compute_kinetic_energy inside the OpenMP-over-halos
     bound pass, repeated over the whole catalogue until `seconds` elapse."""
+    import numpy as np
+
     sys.path.insert(0, str(ROOT / "oracle"))
     import oracle
 
@@ -108,14 +136,97 @@ def run_cpu_baseline(cat, seconds: float):
         one()
         busy += time.perf_counter() - t0
         passes += 1
-    return {
+    out = {
         "value": passes * cat.n_records / busy,
         "unit": UNIT,
         "cores": nt,
         "kind": kind,
         "sample": f"whole catalogue x {passes} pass(es), {busy:.1f} s wall on {nt} threads "
                   f"(OpenMP dynamic over halos, as src/mpi_main.cpp:1377)",
+        "host": cpu_model(),
     }
+    # the same on one thread: a bounded sample of calls, every k-th call so
+    # that the sample keeps the catalogue's size distribution
+    k = max(1, int(round(nt * 1.5)))
+    sel = np.arange(0, cat.n_halos, k)
+    counts = cat.halos["count"][sel]
+    sub_h = cat.halos[sel].copy()
+    sub_h["first"] = np.concatenate([[0], np.cumsum(counts)[:-1]])
+    idx = np.concatenate([np.arange(f, f + c) for f, c in zip(cat.halos["first"][sel], counts)])
+    sub_po = cat.po[idx]
+    t0 = time.perf_counter()
+    if kind == "reference":
+        O.bound_pass_batch_with(sub_po.copy(), sub_h.copy(), cat.params, R, nthreads=1)
+    else:
+        O.bound_pass_batch(sub_po.copy(), sub_h.copy(), cat.params, nthreads=1)
+    e1 = time.perf_counter() - t0
+    out["one_thread"] = {"value": idx.shape[0] / e1, "unit": UNIT, "cores": 1,
+                         "sample": f"every {k}th call ({sel.shape[0]} calls, {idx.shape[0]} "
+                                   f"records), one pass, {e1:.1f} s"}
+    return out
+
+
+def run_drop_in(cat, seconds: float):
+    """The plain per-halo GPU path as the reference's driver calls it: the
+    drop-in compute_potential and compute_kinetic_energy (libhalopot's own
+    entry points, host records in and out) once per halo, from a pool of
+    host threads taking halos dynamically (src/mpi_main.cpp:1369-1402), each
+    thread with its own engine and stream.  Only the two library calls run
+    per halo (the caller's strip, sort and corevel scan are CPU work of the
+    reference, not of this path)."""
+    import threading
+
+    import numpy as np
+
+    import halopot
+
+    nt = cpu_threads()
+    L = halopot.lib()
+    po = np.ascontiguousarray(cat.po.copy())
+    first = cat.halos["first"].astype(np.int64)
+    count = cat.halos["count"].astype(np.int64)
+    centre = np.ascontiguousarray(cat.halos["pos"][:, :3], np.float32)
+    vel = np.ascontiguousarray(cat.halos["bulkvel"], np.float32)
+    base = po.ctypes.data
+    vbase, cbase = vel.ctypes.data, centre.ctypes.data  # float[3] per halo
+    rec = po.dtype.itemsize
+    scale = float(cat.params.scale_now)
+    lock = threading.Lock()
+    nxt = [0]
+
+    def worker():
+        while True:
+            with lock:
+                h = nxt[0]
+                nxt[0] += 1
+            if h >= cat.n_halos:
+                return
+            n = int(count[h])
+            if n <= 0:
+                continue
+            ptr = base + int(first[h]) * rec
+            if n > 1:
+                L.compute_potential(ptr, n)
+            L.compute_kinetic_energy(ptr, n, vbase + 12 * h, cbase + 12 * h, scale)
+
+    def one():
+        nxt[0] = 0
+        ts = [threading.Thread(target=worker) for _ in range(nt)]
+        for t in ts:
+            t.start()
+        for t in ts:
+            t.join()
+
+    one()  # warm: every thread's engine and scratch
+    passes, busy = 0, 0.0
+    while busy < seconds or passes == 0:
+        t0 = time.perf_counter()
+        one()
+        busy += time.perf_counter() - t0
+        passes += 1
+    return {"value": passes * cat.n_records / busy, "unit": UNIT,
+            "ms_per_pass": 1e3 * busy / passes, "passes": passes, "threads": nt,
+            "calls": "compute_potential + compute_kinetic_energy per halo, host buffers"}
 
 
 def run_gather(args):
@@ -425,6 +536,27 @@ def main() -> int:
             phase[k] += st["ms_" + k]
     props = None if args.no_props else run_props(plan, cat, d_po, d_h, args)
     plan.close()
+
+    # accuracy of the benched path (untimed): pe of the same plan's trees and
+    # decisions evaluated in FP64 (HP_ACC_F64) against the benched arithmetic
+    pes = {}
+    for a in (acc, halopot.ACC_F64):
+        pl = halopot.Plan(cat.halos, halopot.options(accumulate=a))
+        if tuning is not None:
+            pl.set_tuning(tuning["big_threshold"], tuning["wave_max"], tuning["direct_max"])
+        restore()
+        pl.potential(d_po.address)
+        synchronize()
+        pes[a] = d_po.download(np.empty_like(cat.po))["pe"]
+        pl.close()
+    live = pes[halopot.ACC_F64] != 0
+    rerr = np.abs(pes[acc][live] - pes[halopot.ACC_F64][live]) / np.abs(pes[halopot.ACC_F64][live])
+    accuracy = {"pe_vs_fp64_same_trees": {"median": float(np.median(rerr)),
+                                          "p99": float(np.quantile(rerr, 0.99)),
+                                          "max": float(rerr.max()), "records": int(live.sum())},
+                "bar": ACCURACY_BAR, "ok": bool(rerr.max() <= ACCURACY_BAR)}
+    log(f"accuracy: pe vs FP64 of the same trees, median {np.median(rerr):.3g}, max "
+        f"{rerr.max():.3g} (bar {ACCURACY_BAR:g})")
     ms_step = 1e3 * elapsed / args.steps
     value = cat.n_records * args.steps / elapsed
     log(f"batched: {ms_step:.3f} ms/step, {value:.4g} records/s; phases per step (ms): " +
@@ -433,15 +565,18 @@ def main() -> int:
     walk_avg = sum(walk_ms) / len(walk_ms)
     achieved = flops / (walk_avg * 1e-3) / 1e12
     roofline = {
-        "bound": "mfma",
+        "bound": "valu",
         "achieved": achieved,
         "peak": FP32_PEAK_TFLOPS,
         "unit": "TFLOP/s",
         "frac": achieved / FP32_PEAK_TFLOPS,
+        "frac_nonpacked": achieved / FP32_NONPACKED_TFLOPS,
+        "peak_nonpacked": FP32_NONPACKED_TFLOPS,
         "traffic": walk_traffic(args.workload),
         "kernel": "k_walk (Barnes-Hut group walk)",
         "bound_detail": "FP32 VALU + v_rsq issue (no dense contraction: MFMA unused); peak is "
-                        "the published FP32 vector figure",
+                        "the published FP32 vector figure (packed FP32), peak_nonpacked the "
+                        "rate without packed FP32, which this build does not emit",
         "flops_per_launch": flops,
         "flop_convention": "11 per P2P pair, 10 per P2M term, counts of the reference's walk",
         "walk_ms_avg": walk_avg,
@@ -466,6 +601,7 @@ def main() -> int:
                    "tuning": tuning,
                    "parallelism": "one GPU"},
         "roofline": roofline,
+        "accuracy": accuracy,
     }
 
     if props is not None:
@@ -490,11 +626,17 @@ def main() -> int:
         ep = time.perf_counter() - tp
         plan_p.close()
         vp = cat.n_records * reps / ep
-        out["per_halo_gpu"] = {"value": vp, "unit": UNIT, "ms_per_pass": 1e3 * ep / reps,
-                               "passes": reps}
-        out["batched_vs_per_halo"] = value / vp
-        log(f"per-halo GPU path: {1e3 * ep / reps:.1f} ms/pass, {vp:.4g} records/s; "
-            f"batched/per-halo = {value / vp:.1f}x")
+        out["per_halo_serial"] = {"value": vp, "unit": UNIT, "ms_per_pass": 1e3 * ep / reps,
+                                  "passes": reps,
+                                  "path": "per-call plan: every call its own launches, one "
+                                          "after another on one stream (device buffers)"}
+        log(f"per-halo GPU path, serial: {1e3 * ep / reps:.1f} ms/pass, {vp:.4g} records/s")
+        di = run_drop_in(cat, min(6.0, args.cpu_seconds))
+        out["per_halo_drop_in"] = di
+        out["batched_vs_per_halo"] = value / di["value"]
+        log(f"per-halo GPU path, drop-in from {di['threads']} host threads: "
+            f"{di['ms_per_pass']:.1f} ms/pass, {di['value']:.4g} records/s; "
+            f"batched/per-halo = {value / di['value']:.1f}x")
 
     if not args.no_host:
         # the same pass from pageable host buffers: H2D, plan, run, D2H
@@ -515,6 +657,10 @@ def main() -> int:
     for d in (d_src, d_hsrc, d_po, d_h, d_b):
         d.free()
     print(json.dumps(out), flush=True)
+    if not accuracy["ok"]:
+        log(f"FAILED: pe differs from FP64 of the same trees by {accuracy['pe_vs_fp64_same_trees']['max']:.3g} "
+            f"> {ACCURACY_BAR:g}")
+        return 1
     return 0
 
 

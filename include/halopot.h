@@ -135,8 +135,11 @@ int hp_plan_create(hp_plan **plan, const hp_halo *halos, int64_t nhalos,
 
 /* The bound pass of src/properties.c:537-550 for every call of the batch,
  * on device-resident data (d_po: the record array, d_halos: the
- * descriptors, d_bound: one byte per record or NULL; stream: a hipStream_t
- * or NULL for the plan's own).  Afterwards, per call, records
+ * descriptors, d_bound: one byte per record or NULL).  The pass runs on the
+ * plan's own stream, a blocking one (it follows the caller's null-stream
+ * work).  stream: a hipStream_t of the caller, or NULL; the pass starts after
+ * the work already queued on it, and work queued on it later starts after
+ * the pass.  The call returns when the pass is done.  Afterwards, per call, records
  * [first, first + kept) are the non-potential-only records sorted by r2
  * with pe and ke set, followed by the potential-only records with pe set;
  * d_bound[first + i] = !(pe < ke) for i < kept.  Returns 0 or an HP_E*
@@ -145,7 +148,7 @@ int hp_plan_run(hp_plan *plan, struct potential *d_po, hp_halo *d_halos,
                 uint8_t *d_bound, void *stream);
 
 /* compute_potential over every call of the batch (pe only, records keep
- * their places). */
+ * their places); stream as for hp_plan_run. */
 int hp_plan_potential(hp_plan *plan, struct potential *d_po, void *stream);
 
 int hp_plan_stats(const hp_plan *plan, hp_stats *out);
@@ -155,7 +158,7 @@ int hp_plan_destroy(hp_plan *plan);
  * walk decision or a sum: they only move work between kernels. */
 typedef struct hp_tuning {
     int32_t big_threshold;  /* nodes above this many points are split by the
-                               level passes (128, 256 or 512) */
+                               level passes (128, 256, 512 or 1024) */
     int32_t wave_max;       /* subtrees of at most this many points are built
                                by one wavefront (32 or 64) */
     int32_t direct_max;     /* HP_METHOD_AUTO: calls of at most this many
@@ -169,6 +172,12 @@ typedef struct hp_tuning {
  * chosen setting is returned in *out when out is not NULL.  Per-call plans
  * are not tuned (HP_EINVAL). */
 int hp_plan_tune(hp_plan *plan, struct potential *d_po, int reps, hp_tuning *out);
+
+/* Applies a setting hp_plan_tune returned (for this plan or another plan of
+ * the same kind of batch) without timing anything: big_threshold one of 128,
+ * 256, 512, 1024; wave_max 32 or 64; direct_max as hp_plan_tune allows it.
+ * HP_EINVAL for any other value and for per-call plans. */
+int hp_plan_set_tuning(hp_plan *plan, const hp_tuning *tuning);
 
 /* ---------------------------------------------------- record assembly */
 
@@ -314,7 +323,7 @@ typedef struct hp_halo_props {
  * last hp_plan_run of that plan: d_po and d_halos as hp_plan_run left them
  * (records sorted by r2 with pe and ke set; first, kept and the centre,
  * pos[3..5] = corevel).  d_props receives one hp_halo_props per call
- * (device memory).  Synchronous like hp_plan_run. */
+ * (device memory).  Synchronous, with stream as for hp_plan_run. */
 int hp_plan_props(hp_plan *plan, const struct potential *d_po, const hp_halo *d_halos,
                   const hp_mass_defs *defs, hp_halo_props *d_props, void *stream);
 

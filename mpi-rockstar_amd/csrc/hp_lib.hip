@@ -209,6 +209,12 @@ struct Tables {
     int32_t direct_max = kLeafPoints;   // AUTO: direct at or below this count
 };
 
+// Node ids, task slots and record positions are 32-bit: the node capacity
+// 5N + H + 1024 (Engine::reserve) must stay below INT32_MAX.
+constexpr int64_t kMaxCalls = 1 << 22;
+constexpr int64_t kMaxRecords = (INT32_MAX - kMaxCalls - 1024) / 5;
+static_assert(5 * kMaxRecords + kMaxCalls + 1024 <= INT32_MAX, "node ids fit 32 bits");
+
 static int make_tables(Tables &T, const int64_t *first, const int64_t *count, int64_t nh,
                        int method, bool want_unbind, int32_t direct_max = kLeafPoints) {
     T = Tables();
@@ -225,8 +231,10 @@ static int make_tables(Tables &T, const int64_t *first, const int64_t *count, in
                              (long long)count[h]);
         pos += count[h];
     }
-    if (pos > INT32_MAX / 2) return set_error(HP_EINVAL, "batch of %lld records is too large", (long long)pos);
-    if (nh > (1 << 22)) return set_error(HP_EINVAL, "too many calls in one batch");
+    if (nh > kMaxCalls) return set_error(HP_EINVAL, "too many calls in one batch");
+    if (pos > kMaxRecords)
+        return set_error(HP_EINVAL, "batch of %lld records is too large (at most %lld)",
+                         (long long)pos, (long long)kMaxRecords);
     T.nrec = pos;
     for (int64_t h = 0; h < nh; h++) {
         for (int64_t off = 0; off < count[h]; off += kTile)
@@ -450,6 +458,24 @@ struct Engine {
         if ((r = upload(seg_wave_l, T.seg_wave))) return r;
         if ((r = upload(seg_block_l, T.seg_block))) return r;
         if ((r = upload(seg_tiles_l, T.seg_tiles))) return r;
+        return HP_OK;
+    }
+
+    // Orders the engine's work after what the caller queued on `caller` (a
+    // hipStream_t; NULL: nothing beyond the null stream, which the blocking
+    // engine stream waits for by itself).
+    int join(void *caller) {
+        if (!caller) return HP_OK;
+        HP_TRY(hipSetDevice(device));
+        HP_TRY(hipEventRecord(ev[7], (hipStream_t)caller));
+        HP_TRY(hipStreamWaitEvent(stream, ev[7], 0));
+        return HP_OK;
+    }
+    // ... and the caller's later work on `caller` after the engine's.
+    int release(void *caller) {
+        if (!caller) return HP_OK;
+        HP_TRY(hipEventRecord(ev[7], stream));
+        HP_TRY(hipStreamWaitEvent((hipStream_t)caller, ev[7], 0));
         return HP_OK;
     }
 
@@ -757,10 +783,24 @@ struct Engine {
 
     // ------------------------------------------------------------ graphs
 
+    // The configuration globals the recorded kernels take as arguments
+    // (PARTICLE_MASS, FORCE_RES: multipoles and walk; SCALE_NOW and the
+    // cosmology: rvir threshold, Hubble term and KE factor of the bound pass).
+    struct Config {
+        double particle_mass, force_res, scale_now, om, ol, w0, wa;
+        static Config now() { return Config{PARTICLE_MASS, FORCE_RES, SCALE_NOW, Om, Ol, W0, WA}; }
+        bool operator==(const Config &o) const {
+            return particle_mass == o.particle_mass && force_res == o.force_res &&
+                   scale_now == o.scale_now && om == o.om && ol == o.ol && w0 == o.w0 &&
+                   wa == o.wa;
+        }
+    };
+
     struct Graphs {
         hipGraphExec_t build = nullptr, multipole = nullptr, walk = nullptr, unbind = nullptr;
         int levels = 0;  // split levels recorded in `build`
         const void *po = nullptr, *halos = nullptr, *bound = nullptr;
+        Config cfg{};    // the globals the graphs were recorded with
         bool ok = false;
     } g;
 
@@ -808,6 +848,7 @@ struct Engine {
         if ((r = capture(&g.unbind, [&] { return enqueue_unbind(T, d_po, d_halos, d_bound); })))
             return r;
         g.levels = levels;
+        g.cfg = Config::now();
         g.po = d_po;
         g.halos = d_halos;
         g.bound = d_bound;
@@ -817,12 +858,14 @@ struct Engine {
 
     // Graph driver of hp_plan_run.  A tree deeper than the recorded levels
     // is finished eagerly from the device state and the graphs re-recorded
-    // on the next call.
+    // on the next call.  The graphs are recorded again whenever the buffers
+    // or any configuration global they were recorded with changed.
     int run_graphs(const Tables &T, struct potential *d_po, hp_halo *d_halos, uint8_t *d_bound,
                    const hp_options &o) {
         hipStream_t s = stream;
         int r;
-        if (!g.ok || g.po != d_po || g.halos != d_halos || g.bound != d_bound) {
+        if (!g.ok || g.po != d_po || g.halos != d_halos || g.bound != d_bound ||
+            !(g.cfg == Config::now())) {
             // learn the depth eagerly, then record
             if ((r = potential(T, d_po, o))) return r;
             if ((r = enqueue_unbind(T, d_po, d_halos, d_bound))) return r;
@@ -1267,12 +1310,12 @@ __attribute__((visibility("default"))) void compute_potential(struct potential *
                                                                int64_t num_po) {
     if (num_po <= 0) return;
     int r;
-    Engine *e = thread_engine(&r);
-    if (!e) return fatal(r, __FILE__, __LINE__);
     Tables T;
     int64_t first = 0, count = num_po;
     if ((r = make_tables(T, &first, &count, 1, HP_METHOD_TREE, false)))
         return fatal(r, __FILE__, __LINE__);
+    Engine *e = thread_engine(&r);
+    if (!e) return fatal(r, __FILE__, __LINE__);
     hp_options o = default_options();
     if ((r = e->reserve(T, false)) || (r = e->load_tables(T))) return fatal(r, __FILE__, __LINE__);
     DevBuf &dpo = e->srt;  // record staging
@@ -1298,6 +1341,11 @@ __attribute__((visibility("default"))) void compute_kinetic_energy(struct potent
                                                                     float scale_now) {
     if (num_po <= 0) return;
     int r;
+    if (num_po > kMaxRecords || !vel_cen || !pos_cen) {
+        set_error(HP_EINVAL, "compute_kinetic_energy: bad arguments (%lld records)",
+                  (long long)num_po);
+        return fatal(HP_EINVAL, __FILE__, __LINE__);
+    }
     Engine *e = thread_engine(&r);
     if (!e) return fatal(r, __FILE__, __LINE__);
     DevBuf &dpo = e->srt;
@@ -1330,17 +1378,18 @@ __attribute__((visibility("default"))) int hp_plan_create(hp_plan **out, const h
         return set_error(HP_EINVAL, "hp_plan_create: bad arguments");
     hp_plan *p = new hp_plan();
     p->opt = opt ? *opt : default_options();
-    int r = p->eng.init();
+    // the calls are checked before any device work
+    std::vector<int64_t> f(nhalos), c(nhalos);
+    for (int64_t h = 0; h < nhalos; h++) {
+        f[h] = halos[h].first;
+        c[h] = halos[h].count;
+    }
+    int r = make_tables(p->T, f.data(), c.data(), nhalos, p->opt.method, true);
+    if (!r) r = p->eng.init();
     if (!r) {
-        std::vector<int64_t> f(nhalos), c(nhalos);
-        for (int64_t h = 0; h < nhalos; h++) {
-            f[h] = halos[h].first;
-            c[h] = halos[h].count;
-        }
-        r = make_tables(p->T, f.data(), c.data(), nhalos, p->opt.method, true);
         p->first = f;
         p->count = c;
-        if (!r && p->opt.per_call) {
+        if (p->opt.per_call) {
             p->per.resize(nhalos);
             int64_t big = 0, bigh = 0;
             for (int64_t h = 0; h < nhalos && !r; h++) {
@@ -1363,6 +1412,25 @@ __attribute__((visibility("default"))) int hp_plan_create(hp_plan **out, const h
         return r;
     }
     *out = p;
+    return HP_OK;
+}
+
+// Makes `t` the plan's launch setting (graphs are recorded again on the next
+// run).
+static int apply_tuning(hp_plan *p, const hp_tuning &t) {
+    Engine &e = p->eng;
+    int r;
+    e.big_threshold = t.big_threshold;
+    e.wave_max = t.wave_max;
+    e.drop_graphs();
+    if (p->opt.method == HP_METHOD_AUTO && t.direct_max != p->T.direct_max) {
+        if ((r = make_tables(p->T, p->first.data(), p->count.data(), (int64_t)p->first.size(),
+                             p->opt.method, true, t.direct_max)) ||
+            (r = e.load_tables(p->T)))
+            return r;
+    }
+    if ((r = e.reserve(p->T, true))) return r;
+    HP_TRY(hipStreamSynchronize(e.stream));
     return HP_OK;
 }
 
@@ -1419,19 +1487,22 @@ __attribute__((visibility("default"))) int hp_plan_tune(hp_plan *p, struct poten
             }
         }
     }
-    e.big_threshold = best.big_threshold;
-    e.wave_max = best.wave_max;
-    e.drop_graphs();
-    if (ndm > 1 && best.direct_max != p->T.direct_max) {
-        if ((r = make_tables(p->T, p->first.data(), p->count.data(), (int64_t)p->first.size(),
-                             p->opt.method, true, best.direct_max)) ||
-            (r = e.load_tables(p->T)))
-            return r;
-    }
-    if ((r = e.reserve(p->T, true))) return r;
-    HP_TRY(hipStreamSynchronize(e.stream));
+    if ((r = apply_tuning(p, best))) return r;
     if (out) *out = best;
     return HP_OK;
+}
+
+__attribute__((visibility("default"))) int hp_plan_set_tuning(hp_plan *p, const hp_tuning *t) {
+    if (!p || !t) return set_error(HP_EINVAL, "hp_plan_set_tuning: bad arguments");
+    if (p->opt.per_call) return set_error(HP_EINVAL, "hp_plan_set_tuning: per-call plans are not tuned");
+    const int32_t bt = t->big_threshold, wm = t->wave_max, dm = t->direct_max;
+    if ((bt != 128 && bt != 256 && bt != 512 && bt != 1024) || (wm != kWaveMin && wm != kWaveTask))
+        return set_error(HP_EINVAL, "hp_plan_set_tuning: big_threshold %d / wave_max %d is not a "
+                         "setting of the tuner", bt, wm);
+    if (p->opt.method == HP_METHOD_AUTO ? (dm != 0 && dm != kLeafPoints) : dm != p->T.direct_max)
+        return set_error(HP_EINVAL, "hp_plan_set_tuning: direct_max %d not allowed for this plan",
+                         dm);
+    return apply_tuning(p, *t);
 }
 
 // The plain per-halo path: every call with its own launches, one after
@@ -1453,20 +1524,23 @@ static int run_per_call(hp_plan *p, struct potential *d_po, hp_halo *d_halos, ui
     return HP_OK;
 }
 
-__attribute__((visibility("default"))) int hp_plan_potential(hp_plan *p, struct potential *d_po,
-                                                             void *stream) {
-    if (!p || (!d_po && p->T.nrec)) return set_error(HP_EINVAL, "hp_plan_potential: bad arguments");
-    (void)stream;
+static int plan_potential(hp_plan *p, struct potential *d_po) {
     if (p->opt.per_call) return run_per_call(p, d_po, nullptr, nullptr, false);
     return p->eng.potential(p->T, d_po, p->opt);
 }
 
-__attribute__((visibility("default"))) int hp_plan_run(hp_plan *p, struct potential *d_po,
-                                                       hp_halo *d_halos, uint8_t *d_bound,
-                                                       void *stream) {
-    if (!p || ((!d_po || !d_halos) && p->T.nrec))
-        return set_error(HP_EINVAL, "hp_plan_run: bad arguments");
-    (void)stream;
+// The entry points below run after the work queued on the caller's stream
+// and before the work the caller queues on it afterwards.
+__attribute__((visibility("default"))) int hp_plan_potential(hp_plan *p, struct potential *d_po,
+                                                             void *stream) {
+    if (!p || (!d_po && p->T.nrec)) return set_error(HP_EINVAL, "hp_plan_potential: bad arguments");
+    int r = p->eng.join(stream);
+    if (!r) r = plan_potential(p, d_po);
+    if (!r) r = p->eng.release(stream);
+    return r;
+}
+
+static int plan_run(hp_plan *p, struct potential *d_po, hp_halo *d_halos, uint8_t *d_bound) {
     if (p->opt.per_call) return run_per_call(p, d_po, d_halos, d_bound, true);
     int r;
     if (p->opt.use_graph && p->opt.method != HP_METHOD_DIRECT && !p->opt.count_work) {
@@ -1489,6 +1563,17 @@ __attribute__((visibility("default"))) int hp_plan_run(hp_plan *p, struct potent
     }
     HP_TRY(hipStreamSynchronize(p->eng.stream));
     return HP_OK;
+}
+
+__attribute__((visibility("default"))) int hp_plan_run(hp_plan *p, struct potential *d_po,
+                                                       hp_halo *d_halos, uint8_t *d_bound,
+                                                       void *stream) {
+    if (!p || ((!d_po || !d_halos) && p->T.nrec))
+        return set_error(HP_EINVAL, "hp_plan_run: bad arguments");
+    int r = p->eng.join(stream);
+    if (!r) r = plan_run(p, d_po, d_halos, d_bound);
+    if (!r) r = p->eng.release(stream);
+    return r;
 }
 
 __attribute__((visibility("default"))) int hp_gather_calls(
@@ -1884,11 +1969,12 @@ __attribute__((visibility("default"))) int hp_plan_props(hp_plan *p, const struc
                                                          hp_halo_props *d_props, void *stream) {
     if (!p || !defs || ((!d_po || !d_halos || !d_props) && p->T.nhalos))
         return set_error(HP_EINVAL, "hp_plan_props: bad arguments");
-    (void)stream;
     int r;
+    if ((r = p->eng.join(stream))) return r;
     // a per-call plan's runs load single-call tables; the pass needs the batch's
     if (p->opt.per_call && (r = p->eng.load_tables(p->T))) return r;
     if ((r = p->eng.props(p->T, d_po, d_halos, *defs, d_props))) return r;
+    if ((r = p->eng.release(stream))) return r;
     HP_TRY(hipStreamSynchronize(p->eng.stream));
     return HP_OK;
 }

@@ -87,6 +87,7 @@ _LIB = None
 
 EXPORTS = ("compute_potential", "compute_kinetic_energy", "hp_plan_create", "hp_plan_run",
            "hp_plan_potential", "hp_plan_stats", "hp_plan_destroy", "hp_plan_tune",
+           "hp_plan_set_tuning",
            "hp_mass_definitions", "hp_plan_props", "hp_gather_calls", "hp_find_subfofs",
            "hp_bound_pass_host",
            "hp_debug_tree", "hp_last_error", "hp_set_error_handler")
@@ -112,6 +113,7 @@ def lib() -> C.CDLL:
         L.hp_plan_stats.argtypes = [C.c_void_p, C.POINTER(Stats)]
         L.hp_plan_destroy.argtypes = [C.c_void_p]
         L.hp_plan_tune.argtypes = [C.c_void_p, C.c_void_p, C.c_int, C.POINTER(Tuning)]
+        L.hp_plan_set_tuning.argtypes = [C.c_void_p, C.POINTER(Tuning)]
         L.hp_mass_definitions.argtypes = [C.POINTER(C.c_char_p), C.POINTER(MassDefs)]
         L.hp_gather_calls.argtypes = [C.c_void_p, C.c_int64, C.c_void_p, C.c_int64, C.c_void_p,
                                       C.c_int64, C.c_double, C.c_void_p, C.c_int64, C.c_void_p,
@@ -128,6 +130,29 @@ def lib() -> C.CDLL:
         L.hp_last_error.restype = C.c_char_p
         _LIB = L
     return _LIB
+
+
+# hp_error_handler: void (*)(int code, const char *file, int line)
+ERROR_HANDLER = C.CFUNCTYPE(None, C.c_int, C.c_char_p, C.c_int)
+_HANDLER_REF = None
+
+
+def set_error_handler(fn) -> None:
+    """hp_set_error_handler: fn(code, file, line) is called on a fatal error
+    inside the void reference entry points (compute_potential,
+    compute_kinetic_energy) when the program does not link the reference's
+    rockstar_abort_impl (src/error.h:15).  None removes the handler."""
+    global _HANDLER_REF
+    L = lib()
+    L.hp_set_error_handler.argtypes = [ERROR_HANDLER]
+    L.hp_set_error_handler.restype = None
+    cb = ERROR_HANDLER(lambda code, f, line: fn(code, f.decode(), line)) if fn else ERROR_HANDLER()
+    L.hp_set_error_handler(cb)
+    _HANDLER_REF = cb  # the C side keeps the pointer: keep the thunk alive
+
+
+def last_error() -> str:
+    return lib().hp_last_error().decode()
 
 
 def _check(code: int) -> None:
@@ -299,20 +324,33 @@ class Plan:
         _check(lib().hp_plan_tune(self._h, C.c_void_p(d_po), int(reps), C.byref(t)))
         return t.as_dict()
 
-    def run(self, d_po: int, d_halos: int, d_bound: int | None = None) -> None:
+    def set_tuning(self, big_threshold: int, wave_max: int, direct_max: int | None = None) -> None:
+        """hp_plan_set_tuning: one of the tuner's settings, applied untimed."""
+        t = Tuning()
+        t.big_threshold, t.wave_max = big_threshold, wave_max
+        t.direct_max = 10 if direct_max is None else direct_max
+        _check(lib().hp_plan_set_tuning(self._h, C.byref(t)))
+
+    def run(self, d_po: int, d_halos: int, d_bound: int | None = None,
+            stream: int | None = None) -> None:
+        """hp_plan_run; stream: a hipStream_t whose queued work the pass
+        follows (and which waits for the pass), or None."""
         _check(lib().hp_plan_run(self._h, C.c_void_p(d_po), C.c_void_p(d_halos),
-                                 C.c_void_p(d_bound) if d_bound else None, None))
+                                 C.c_void_p(d_bound) if d_bound else None,
+                                 C.c_void_p(stream) if stream else None))
 
-    def potential(self, d_po: int) -> None:
-        _check(lib().hp_plan_potential(self._h, C.c_void_p(d_po), None))
+    def potential(self, d_po: int, stream: int | None = None) -> None:
+        _check(lib().hp_plan_potential(self._h, C.c_void_p(d_po),
+                                       C.c_void_p(stream) if stream else None))
 
-    def props(self, d_po: int, d_halos: int, d_props: int, defs: MassDefs | None = None) -> None:
+    def props(self, d_po: int, d_halos: int, d_props: int, defs: MassDefs | None = None,
+              stream: int | None = None) -> None:
         """hp_plan_props: the halo property pass (src/properties.c:384-510)
         on the results of the last run(); d_props receives one PROPS_DTYPE
         record per call."""
         md = defs if defs is not None else mass_definitions()
         _check(lib().hp_plan_props(self._h, C.c_void_p(d_po), C.c_void_p(d_halos), C.byref(md),
-                                   C.c_void_p(d_props), None))
+                                   C.c_void_p(d_props), C.c_void_p(stream) if stream else None))
 
     def stats(self) -> dict:
         s = Stats()

@@ -46,6 +46,37 @@ def synchronize() -> None:
     _ok(hip().hipDeviceSynchronize(), "hipDeviceSynchronize")
 
 
+class Stream:
+    """A non-blocking HIP stream at the default priority (it does not wait
+    for the null stream); .handle is the hipStream_t for the C ABI."""
+
+    def __init__(self):
+        h = hip()
+        h.hipStreamCreateWithFlags.argtypes = [C.POINTER(C.c_void_p), C.c_uint]
+        h.hipStreamSynchronize.argtypes = [C.c_void_p]
+        h.hipStreamDestroy.argtypes = [C.c_void_p]
+        h.hipMemcpyAsync.argtypes = [C.c_void_p, C.c_void_p, C.c_size_t, C.c_int, C.c_void_p]
+        self.ptr = C.c_void_p()
+        _ok(h.hipStreamCreateWithFlags(C.byref(self.ptr), 1), "hipStreamCreateWithFlags")
+
+    @property
+    def handle(self) -> int:
+        return int(self.ptr.value or 0)
+
+    def copy(self, dst: int, src: int, nbytes: int) -> None:
+        """Device-to-device copy queued on this stream (not waited for)."""
+        _ok(hip().hipMemcpyAsync(C.c_void_p(dst), C.c_void_p(src), nbytes, D2D, self.ptr),
+            "hipMemcpyAsync")
+
+    def synchronize(self) -> None:
+        _ok(hip().hipStreamSynchronize(self.ptr), "hipStreamSynchronize")
+
+    def destroy(self) -> None:
+        if self.ptr.value:
+            hip().hipStreamDestroy(self.ptr)
+            self.ptr = C.c_void_p()
+
+
 class DeviceArray:
     """A device allocation holding a copy of a numpy array's bytes."""
 

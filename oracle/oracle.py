@@ -106,6 +106,8 @@ class Restatement:
         L.orc_bound_pass_batch_fn.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p,
                                               C.c_int64, C.POINTER(_Params), C.c_void_p, C.c_int,
                                               C.c_void_p, C.c_void_p]
+        L.orc_potential_batch.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p, C.c_int64,
+                                          C.POINTER(_Params), C.c_void_p, C.c_void_p, C.c_int]
         L.orc_set_never_accept.argtypes = [C.c_int]
         L.orc_set_shape_reverse.argtypes = [C.c_int]
         L.orc_set_nmass_out.argtypes = [C.c_void_p]
@@ -154,6 +156,18 @@ class Restatement:
         if tree:
             out["tree"] = nodes[:nn]
         return out
+
+    def potential_batch(self, po: np.ndarray, halos: np.ndarray, prm, nthreads=0):
+        """compute_potential over every call (OpenMP over calls); po is permuted
+        within each call.  Returns per-call (p2p, p2m) counts of the walk."""
+        _check_records(po)
+        first = np.ascontiguousarray(halos["first"], np.int64)
+        count = np.ascontiguousarray(halos["count"], np.int64)
+        p2p = np.zeros(halos.shape[0], np.int64)
+        p2m = np.zeros(halos.shape[0], np.int64)
+        self.lib.orc_potential_batch(_p(po), _p(first), _p(count), halos.shape[0],
+                                     C.byref(_params(prm)), _p(p2p), _p(p2m), nthreads)
+        return p2p, p2m
 
     def compute_potential_mass(self, po: np.ndarray, prm):
         """compute_potential that also returns, per record in tree order, the

@@ -688,6 +688,29 @@ void orc_bound_pass_batch_fn(struct potential *po, const int64_t *first,
         kept[h] = orc_bound_pass_fn(po + first[h], count[h], halos + h, p, cp, ke);
 }
 
+/* compute_potential over every call of a batch (src/potential.c:183-192
+   per call, calls in parallel as src/mpi_main.cpp:1377 runs halos), with
+   each call's P2P pair and P2M term counts of the reference's walk.
+   Records are permuted within their calls as the restated build leaves
+   them. */
+void orc_potential_batch(struct potential *po, const int64_t *first,
+                         const int64_t *count, int64_t nhalos, const orc_params *p,
+                         int64_t *p2p, int64_t *p2m, int nthreads) {
+#ifdef _OPENMP
+    if (nthreads > 0) omp_set_num_threads(nthreads);
+#else
+    (void)nthreads;
+#endif
+#pragma omp parallel for schedule(dynamic)
+    for (int64_t h = 0; h < nhalos; h++) {
+        orc_walk_stats st;
+        memset(&st, 0, sizeof st);
+        orc_compute_potential(po + first[h], count[h], p, NULL, NULL, &st, NULL, 0);
+        p2p[h] = st.p2p;
+        p2m[h] = st.p2m;
+    }
+}
+
 /* ---------------------------------------------------- record assembly */
 
 /* One halo of the hierarchy, in the layout of the product's hp_tree_halo. */

@@ -52,6 +52,8 @@ def _check_batch(restatement, cat, opts=None):
         np.testing.assert_allclose(hg["bulkvel"], ho["bulkvel"], rtol=1e-6, atol=1e-4)
         np.testing.assert_allclose(hg["pos"], ho["pos"], rtol=1e-6, atol=1e-4)
         np.testing.assert_allclose(hg["min_vel_err"], ho["min_vel_err"], rtol=1e-5, atol=1e-6)
+        np.testing.assert_allclose(hg["min_bulkvel_err"], ho["min_bulkvel_err"], rtol=1e-5,
+                                   atol=1e-6)
         ke_g, ke_o = g["ke"][gi], o["ke"][oi]
         np.testing.assert_allclose(ke_g, ke_o, rtol=1e-5, atol=1e-3)
         bg = bound_g[f:f + k][gi].astype(bool)

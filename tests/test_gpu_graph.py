@@ -5,7 +5,11 @@ them.  Every replay is checked against the eager result (bit for bit: the
 kernels are the same and their sums are ordered by the tree, not by
 scheduling), with eager work and synchronisations in between.  A batch
 whose trees need more split levels than were recorded is finished eagerly
-from the device state and must give the eager answer too."""
+from the device state and must give the eager answer too.  The graphs are
+recorded again when a configuration global they take (PARTICLE_MASS,
+FORCE_RES, SCALE_NOW, the cosmology) changes between runs of one plan."""
+import dataclasses
+
 import numpy as np
 import pytest
 
@@ -96,3 +100,33 @@ def test_graph_deeper_tree_falls_back():
             _same(got, want)
     finally:
         plan.close()
+
+
+def test_graph_rerecords_on_config_change():
+    cat = catalog.small_batch(seed=43, n_halos=30, lo=5, hi=5000, potential_only=0.1)
+    cat.po["flags"] = np.arange(cat.n_records, dtype=np.int32)
+    p0 = cat.params
+    p1 = dataclasses.replace(p0, scale_now=0.5, particle_mass=2.5 * p0.particle_mass,
+                             force_res=0.7 * p0.force_res, Om=0.3, Ol=0.7)
+    other = cat.copy()
+    other.params = p1
+    want0, want1 = _eager(cat), _eager(other)
+    assert not np.array_equal(want0[0]["pe"], want1[0]["pe"])
+    assert not np.array_equal(want0[0]["ke"], want1[0]["ke"])
+    plan = halopot.Plan(cat.halos, halopot.options(use_graph=1))
+    d_po = DeviceArray.from_numpy(cat.po)
+    d_h = DeviceArray.from_numpy(cat.halos)
+    d_b = DeviceArray.zeros(cat.n_records)
+    try:
+        for prm, want in ((p0, want0), (p0, want0), (p1, want1), (p1, want1), (p0, want0)):
+            halopot.set_config(prm)
+            d_po.upload(cat.po)
+            d_h.upload(cat.halos)
+            plan.run(d_po.address, d_h.address, d_b.address)
+            synchronize()
+            got = (d_po.download(np.empty_like(cat.po)), d_h.download(np.empty_like(cat.halos)),
+                   d_b.download(np.empty(cat.n_records, np.uint8)))
+            _same(got, want)
+    finally:
+        plan.close()
+        halopot.set_config(p0)
