@@ -40,10 +40,12 @@ struct potential {
  * Writes pe for every record from the Barnes-Hut potential of the reference's
  * k-d tree (10 points per leaf, dmin acceptance with ERR_TOL 1, monopoles).
  * Non-finite positions get pe = 0 and are left out of the tree
- * (src/fast3tree.c:875-887).  Unlike the reference, the records are NOT
- * permuted: every record keeps its place; callers sort by r2 afterwards
- * either way (src/properties.c:545).  Reads the globals FORCE_RES and
- * PARTICLE_MASS at call time.  Safe to call from many host threads at once
+ * (src/fast3tree.c:875-887).  Like the reference, the call permutes po: the
+ * records end in the tree's order, the order the reference's in-place
+ * non-finite removal and partitions (src/fast3tree.c:762-773, 875-887)
+ * leave, every field travelling with its record; so the caller's strip and
+ * stable sort (src/properties.c:539-545) give the reference's order, ties
+ * included.  Reads the globals FORCE_RES and PARTICLE_MASS at call time.  Safe to call from many host threads at once
  * (src/mpi_main.cpp:1369-1402); each thread gets its own stream and device
  * scratch. */
 void compute_potential(struct potential *po, int64_t num_po);
@@ -141,14 +143,18 @@ int hp_plan_create(hp_plan **plan, const hp_halo *halos, int64_t nhalos,
  * the work already queued on it, and work queued on it later starts after
  * the pass.  The call returns when the pass is done.  Afterwards, per call, records
  * [first, first + kept) are the non-potential-only records sorted by r2
- * with pe and ke set, followed by the potential-only records with pe set;
+ * in the reference's order (records of equal r2 in the order its
+ * compute_potential and potential-only strip, src/properties.c:539-544,
+ * leave them before its stable qsort) with pe and ke set, followed by the
+ * potential-only records with pe set;
  * d_bound[first + i] = !(pe < ke) for i < kept.  Returns 0 or an HP_E*
  * code. */
 int hp_plan_run(hp_plan *plan, struct potential *d_po, hp_halo *d_halos,
                 uint8_t *d_bound, void *stream);
 
-/* compute_potential over every call of the batch (pe only, records keep
- * their places); stream as for hp_plan_run. */
+/* compute_potential over every call of the batch: pe, and each call's
+ * records permuted as compute_potential permutes them (HP_METHOD_DIRECT
+ * plans leave the records in place); stream as for hp_plan_run. */
 int hp_plan_potential(hp_plan *plan, struct potential *d_po, void *stream);
 
 int hp_plan_stats(const hp_plan *plan, hp_stats *out);

@@ -6,10 +6,12 @@
 // compute_potential call, splitting a node of more than 10 points at the float
 // midpoint of the widest side of its tight box (z first, x or y only when
 // strictly wider), points with pos[dim] <= lim going left.  A split that puts
-// every point on one side makes the node a leaf.  Node membership is a pure
-// function of the point set, so a stable partition reproduces the reference's
-// nodes exactly; only the order inside a node differs (the reference swaps in
-// place).
+// every point on one side makes the node a leaf.  Every partition puts the
+// points in the order the reference's in-place swap loop leaves them
+// (src/fast3tree.c:762-773, closed form in inplace_dest): each split lists
+// the node's points by side (the stable order), then moves every point to
+// its slot.  So the final tree order, and with it every leaf's summation
+// order, is the reference's.
 //
 // The build runs over the whole batch at once:
 //   roots     gather: records -> float4 (x, y, z, record index), tight box and
@@ -73,8 +75,8 @@ __global__ void k_fill_i32(int32_t *p, int32_t v, int64_t n) {
 }
 
 // box keys per halo: [lo0 lo1 lo2 hi0 hi1 hi2] as order-preserving ints,
-// bad: non-finite count
-__global__ void k_root_init(int32_t *box, int32_t *bad, int nhalos) {
+// bad: non-finite count, pot: potential-only records (ke < 0)
+__global__ void k_root_init(int32_t *box, int32_t *bad, int32_t *pot, int nhalos) {
     int h = blockIdx.x * blockDim.x + threadIdx.x;
     if (h >= nhalos) return;
     for (int k = 0; k < 3; k++) {
@@ -82,14 +84,16 @@ __global__ void k_root_init(int32_t *box, int32_t *bad, int nhalos) {
         box[6 * h + 3 + k] = float_key(__int_as_float((int)0xff800000));  // -inf
     }
     bad[h] = 0;
+    pot[h] = 0;
 }
 
 __global__ __launch_bounds__(kTileThreads) void k_gather(
     const struct potential *__restrict__ po, const int64_t *__restrict__ hfirst,
     const int64_t *__restrict__ hcount, const RootTile *__restrict__ tiles,
-    float4 *__restrict__ A, int32_t *__restrict__ box, int32_t *__restrict__ bad) {
+    float4 *__restrict__ A, int32_t *__restrict__ box, int32_t *__restrict__ bad,
+    int32_t *__restrict__ pot) {
     __shared__ float s_lo[3][kTileThreads / kWave], s_hi[3][kTileThreads / kWave];
-    __shared__ int s_bad[kTileThreads / kWave];
+    __shared__ int s_bad[kTileThreads / kWave], s_pot[kTileThreads / kWave];
     const RootTile t = tiles[blockIdx.x];
     const int64_t first = hfirst[t.halo];
     const int64_t end = first + hcount[t.halo];
@@ -98,12 +102,13 @@ __global__ __launch_bounds__(kTileThreads) void k_gather(
         lo[k] = __int_as_float(0x7f800000);
         hi[k] = __int_as_float((int)0xff800000);
     }
-    int nbad = 0;
+    int nbad = 0, npot = 0;
     for (int r = 0; r < kTile / kTileThreads; r++) {
         int64_t i = first + t.off + r * kTileThreads + threadIdx.x;
         if (i >= end) break;
         const float *q = po[i].pos;
         float x = q[0], y = q[1], z = q[2];
+        npot += po[i].ke < 0.0f ? 1 : 0;
         A[i] = make_float4(x, y, z, __int_as_float((int32_t)i));
         if (finite3(x, y, z)) {
             lo[0] = fminf(lo[0], x); hi[0] = fmaxf(hi[0], x);
@@ -121,12 +126,15 @@ __global__ __launch_bounds__(kTileThreads) void k_gather(
             s_hi[k][w] = b;
         }
     }
-    int nb = wave_sum(nbad);
-    if (lane_id() == 0) s_bad[w] = nb;
+    int nb = wave_sum(nbad), np_ = wave_sum(npot);
+    if (lane_id() == 0) {
+        s_bad[w] = nb;
+        s_pot[w] = np_;
+    }
     __syncthreads();
     if (threadIdx.x == 0) {
         float a[3], b[3];
-        int tb = 0;
+        int tb = 0, tp = 0;
         for (int k = 0; k < 3; k++) {
             a[k] = s_lo[k][0];
             b[k] = s_hi[k][0];
@@ -137,18 +145,23 @@ __global__ __launch_bounds__(kTileThreads) void k_gather(
                 b[k] = fmaxf(b[k], s_hi[k][j]);
             }
         }
-        for (int j = 0; j < kTileThreads / kWave; j++) tb += s_bad[j];
+        for (int j = 0; j < kTileThreads / kWave; j++) {
+            tb += s_bad[j];
+            tp += s_pot[j];
+        }
         for (int k = 0; k < 3; k++) {
             atomicMin(&box[6 * t.halo + k], float_key(a[k]));
             atomicMax(&box[6 * t.halo + 3 + k], float_key(b[k]));
         }
         if (tb) atomicAdd(&bad[t.halo], tb);
+        if (tp) atomicAdd(&pot[t.halo], tp);
     }
 }
 
 struct BuildArgs {
     Node *nodes;
     int32_t *parent;
+    int32_t *sides;         // per position: a partition's points listed by side
     int32_t *leaf_of;
     float4 *fin;            // final tree order
     Task *small_next;       // wavefront subtree tasks
@@ -348,9 +361,10 @@ __global__ __launch_bounds__(256) void k_roots(const int64_t *__restrict__ hfirs
     else if (gsmall) push_group_at(a, gslot0 + rank_below(mg), h, lo, hi, 0, first, nfin);
 }
 
-// The listed calls with non-finite points, one workgroup each: stable
-// removal of the non-finite records to the tail (src/fast3tree.c:875-887),
-// root node, root queued or finished as a leaf.
+// The listed calls with non-finite points, one workgroup each: removal of
+// the non-finite records to the tail in the order of the reference's swap
+// loop (src/fast3tree.c:875-887), root node, root queued or finished as a
+// leaf.
 __global__ __launch_bounds__(256) void k_roots_bad(const int64_t *__restrict__ hfirst,
                                                    const int64_t *__restrict__ hcount,
                                                    const int32_t *__restrict__ box,
@@ -371,40 +385,50 @@ __global__ __launch_bounds__(256) void k_roots_bad(const int64_t *__restrict__ h
             lo[k] = nfin ? key_float(box[6 * h + k]) : 0.0f;
             hi[k] = nfin ? key_float(box[6 * h + 3 + k]) : 0.0f;
         }
-        // stable partition finite | non-finite through the final array, then
-        // the finite part back into A for splitting
-        int32_t runf = 0, runb = 0;
+        // finite | non-finite in the order of the reference's swap loop
+        // (inplace_dest): pass 0 lists the positions by class in the stable
+        // order, pass 1 moves every point to its slot in the final array;
+        // then the finite part goes back into A for splitting
         const int w = threadIdx.x / kWave;
-        for (int32_t base = 0; base < count; base += 256) {
-            int32_t i = base + threadIdx.x;
-            bool vv = i < count;
-            float4 p = vv ? A[first + i] : make_float4(0, 0, 0, 0);
-            bool f = vv && finite3(p.x, p.y, p.z);
-            bool bb = vv && !f;
-            uint64_t mf = ballot(f), mb = ballot(bb);
-            if (lane_id() == 0) {
-                s_cnt[0][w] = __popcll(mf);
-                s_cnt[1][w] = __popcll(mb);
-            }
-            __syncthreads();
-            int of = 0, ob = 0, tf = 0, tb = 0;
-            for (int j = 0; j < 4; j++) {
-                if (j < w) {
-                    of += s_cnt[0][j];
-                    ob += s_cnt[1][j];
+        const int32_t *sd = a.sides + first;
+        for (int pass = 0; pass < 2; pass++) {
+            int32_t runf = 0, runb = 0;
+            for (int32_t base = 0; base < count; base += 256) {
+                int32_t i = base + threadIdx.x;
+                bool vv = i < count;
+                float4 p = vv ? A[first + i] : make_float4(0, 0, 0, 0);
+                bool f = vv && finite3(p.x, p.y, p.z);
+                bool bb = vv && !f;
+                uint64_t mf = ballot(f), mb = ballot(bb);
+                if (lane_id() == 0) {
+                    s_cnt[0][w] = __popcll(mf);
+                    s_cnt[1][w] = __popcll(mb);
                 }
-                tf += s_cnt[0][j];
-                tb += s_cnt[1][j];
+                __syncthreads();
+                int of = 0, ob = 0, tf = 0, tb = 0;
+                for (int j = 0; j < 4; j++) {
+                    if (j < w) {
+                        of += s_cnt[0][j];
+                        ob += s_cnt[1][j];
+                    }
+                    tf += s_cnt[0][j];
+                    tb += s_cnt[1][j];
+                }
+                if (vv) {
+                    const int32_t rank = f ? runf + of + rank_below(mf) : runb + ob + rank_below(mb);
+                    if (pass == 0) {
+                        a.sides[first + (f ? rank : nfin + rank)] = i;
+                    } else {
+                        const int32_t d = first + inplace_dest(i, f, rank, count, nfin,
+                                                               [&](int32_t k) { return sd[k]; });
+                        a.fin[d] = p;
+                        if (!f) a.leaf_of[d] = -1;
+                    }
+                }
+                runf += tf;
+                runb += tb;
+                __syncthreads();
             }
-            if (f) a.fin[first + runf + of + rank_below(mf)] = p;
-            if (bb) {
-                int32_t d = first + nfin + runb + ob + rank_below(mb);
-                a.fin[d] = p;
-                a.leaf_of[d] = -1;
-            }
-            runf += tf;
-            runb += tb;
-            __syncthreads();
         }
         __syncthreads();
         // A keeps the compact finite points of every call (the direct kernel
@@ -589,6 +613,7 @@ __global__ __launch_bounds__(kSubThreads, kCap <= 256 ? 7 : 1) void k_subtree(co
                   kSubNodes = SubSize<kCap>::nodes, kSubPerThread = SubSize<kCap>::per_thread;
     __shared__ float4 P[2][kCap];
     __shared__ uint8_t Q[2][kCap];       // level entry of each position
+    __shared__ int16_t ST[kCap];         // each split node's points listed by side
     __shared__ SubEntry E[2][kSubList];
     __shared__ int32_t BOX[kSubList][6];      // next level's tight boxes, float keys
     __shared__ int16_t s_left[kSubNodes];     // left child of each local node, -1: leaf
@@ -746,11 +771,25 @@ __global__ __launch_bounds__(kSubThreads, kCap <= 256 ? 7 : 1) void k_subtree(co
             }
             __syncthreads();
             // 3. scatter: split points to their child's place in the next
-            //    buffer; leaf points are in tree order already
+            //    buffer, in the order of the reference's in-place partition;
+            //    leaf points are in tree order already.  First each split
+            //    node's points are listed by side (the stable order), which
+            //    the closed form reads.
             auto Sx = [&](int x) {
                 const int c = x >> 6;
                 return s_pre[c] + __popcll(s_mask[c] & lanes_below(x & 63));
             };
+            for (int r = 0; r < rounds; r++) {
+                const int i = r * kSubThreads + tid;
+                if (i >= n) break;
+                const uint8_t e = Q[cur][i];
+                if (e == kNoEntry || !E[cur][e].kind) continue;
+                const SubEntry en = E[cur][e];
+                const bool gl = comp(P[buf][i], en.dim) <= en.lim;
+                const int sl = Sx(i) - Sx(en.lf);
+                ST[gl ? en.lf + sl : en.lf + en.nl + (i - en.lf - sl)] = (int16_t)(i - en.lf);
+            }
+            __syncthreads();
             for (int r = 0; r < rounds; r++) {
                 const int i = r * kSubThreads + tid;
                 if (i >= n) break;
@@ -763,7 +802,9 @@ __global__ __launch_bounds__(kSubThreads, kCap <= 256 ? 7 : 1) void k_subtree(co
                     if (en.kind) {
                         const bool gl = comp(p, en.dim) <= en.lim;
                         const int sl = Sx(i) - Sx(en.lf);
-                        dst = gl ? en.lf + sl : en.lf + en.nl + (i - en.lf - sl);
+                        const int lf = en.lf;
+                        dst = lf + inplace_dest(i - lf, gl, gl ? sl : i - lf - sl, en.n, en.nl,
+                                                [&](int k) { return (int32_t)ST[lf + k]; });
                         q = (uint8_t)(en.child - c0 + (gl ? 0 : 1));
                     } else {
                         a.fin[first + i] = p;
@@ -993,6 +1034,7 @@ __global__ __launch_bounds__(kWave) void k_subtree_wave(const float4 *__restrict
                                                         const int32_t *__restrict__ wave_off) {
     __shared__ float4 P[kWaveTask];
     __shared__ int32_t Q[kWaveTask];
+    __shared__ int8_t ST[kWaveTask];  // each split node's points listed by side
     __shared__ float s_lo[3][kWaveNodes], s_hi[3][kWaveNodes];
     __shared__ float s_com[3][kWaveNodes], s_mass[kWaveNodes];
     __shared__ int16_t s_lf[kWaveNodes], s_cnt[kWaveNodes], s_left[kWaveNodes];
@@ -1054,9 +1096,15 @@ __global__ __launch_bounds__(kWave) void k_subtree_wave(const float4 *__restrict
             const uint64_t mH = ballot(split && lane == lf);
             const int C = M + 2 * __popcll(mH & lanes_below(lf));
             int dst = lane, nn = nid;
+            // the in-place partition's order: the node's points listed by
+            // side (the stable order), then each point's closed-form slot
+            const uint64_t below = mN & lanes_below(lane);
+            const int rank = gl ? __popcll(mL & below) : __popcll(~mL & below);
+            if (split) ST[gl ? lf + rank : lf + nl + rank] = (int8_t)(lane - lf);
+            __syncthreads();
             if (split) {
-                const uint64_t below = mN & lanes_below(lane);
-                dst = gl ? lf + __popcll(mL & below) : lf + nl + __popcll(~mL & below);
+                dst = lf + inplace_dest(lane - lf, gl, rank, c, nl,
+                                        [&](int k) { return (int32_t)ST[lf + k]; });
                 nn = gl ? C : C + 1;
                 if (lane == lf) {
                     s_lf[C] = (int16_t)lf;
@@ -1351,32 +1399,44 @@ __global__ __launch_bounds__(256) void k_big_split(const Task *__restrict__ big,
     }
 }
 
-// Scatter pass over one tile of a big task: stable ranks in rounds of 256.
-__device__ void big_scatter_tile(const float4 *__restrict__ src, float4 *__restrict__ dst,
-                                 const int2 *__restrict__ tiles, const Task *__restrict__ big,
-                                 const TilePart *__restrict__ parts,
-                                 const BigSplit *__restrict__ res, const BuildArgs &a, int32_t ti);
+// The two passes over one tile of a big task, with the points' ranks on
+// their side of the plane in rounds of 256 (a block prefix of ballots):
+//   kSides:   lists the node's points by side, a.sides[first + k] = the
+//             position of the k-th left point, then of the right points
+//             (the stable order);
+//   kScatter: moves each point to the slot the reference's in-place
+//             partition gives it (inplace_dest over that list).
+enum { kSides = 0, kScatter = 1 };
 
-__global__ __launch_bounds__(kTileThreads) void k_big_scatter(
+template <int kPass>
+__device__ void big_tile_pass(const float4 *__restrict__ src, float4 *__restrict__ dst,
+                              const int2 *__restrict__ tiles, const Task *__restrict__ big,
+                              const TilePart *__restrict__ parts,
+                              const BigSplit *__restrict__ res, const BuildArgs &a, int32_t ti);
+
+template <int kPass>
+__global__ __launch_bounds__(kTileThreads) void k_big_tiles(
     const float4 *__restrict__ src, float4 *__restrict__ dst, const int2 *__restrict__ tiles,
     const Task *__restrict__ big, const TilePart *__restrict__ parts,
     const BigSplit *__restrict__ res, BuildArgs a) {
     const int32_t t1 = a.counters[kTileEnd];
     for (int32_t ti = a.counters[kTileBegin] + blockIdx.x; ti < t1; ti += gridDim.x)
-        big_scatter_tile(src, dst, tiles, big, parts, res, a, ti);
+        big_tile_pass<kPass>(src, dst, tiles, big, parts, res, a, ti);
 }
 
-__device__ void big_scatter_tile(const float4 *__restrict__ src, float4 *__restrict__ dst,
-                                 const int2 *__restrict__ tiles, const Task *__restrict__ big,
-                                 const TilePart *__restrict__ parts,
-                                 const BigSplit *__restrict__ res, const BuildArgs &a, int32_t ti) {
+template <int kPass>
+__device__ void big_tile_pass(const float4 *__restrict__ src, float4 *__restrict__ dst,
+                              const int2 *__restrict__ tiles, const Task *__restrict__ big,
+                              const TilePart *__restrict__ parts,
+                              const BigSplit *__restrict__ res, const BuildArgs &a, int32_t ti) {
     __shared__ int s_cnt[2][kTileThreads / kWave];
     const int2 tl = tiles[ti];
     const Task t = big[tl.x];
     const Node nd = a.nodes[t.node];
     const BigSplit r = res[tl.x];
     const int w = threadIdx.x / kWave;
-    if (r.left < 0) {  // failed split: the node is a leaf
+    if (r.left < 0) {  // failed split: the node is a leaf, its order unchanged
+        if (kPass == kSides) return;
         for (int j = threadIdx.x; j < kTile; j += kTileThreads) {
             int32_t i = tl.y + j;
             if (i >= nd.count) break;
@@ -1410,23 +1470,22 @@ __device__ void big_scatter_tile(const float4 *__restrict__ src, float4 *__restr
             tR += s_cnt[1][j];
         }
         if (v) {
-            int32_t d;
-            bool leaf;
-            int32_t id;
-            if (f) {
-                d = nd.first + lpre + runL + oL + rank_below(mL);
-                leaf = r.leafL;
-                id = r.left;
+            const int32_t rank = f ? lpre + runL + oL + rank_below(mL)
+                                   : rpre + runR + oR + rank_below(mR);
+            if (kPass == kSides) {
+                a.sides[nd.first + (f ? rank : r.nleft + rank)] = i;
             } else {
-                d = nd.first + r.nleft + rpre + runR + oR + rank_below(mR);
-                leaf = r.leafR;
-                id = r.left + 1;
-            }
-            if (leaf) {
-                a.fin[d] = p;
-                a.leaf_of[d] = id;
-            } else {
-                dst[d] = p;
+                const int32_t *sd = a.sides + nd.first;
+                const int32_t d = nd.first + inplace_dest(i, f, rank, nd.count, r.nleft,
+                                                          [&](int32_t k) { return sd[k]; });
+                const bool leaf = f ? r.leafL : r.leafR;  // left points land in [0, nleft)
+                const int32_t id = f ? r.left : r.left + 1;
+                if (leaf) {
+                    a.fin[d] = p;
+                    a.leaf_of[d] = id;
+                } else {
+                    dst[d] = p;
+                }
             }
         }
         runL += tL;

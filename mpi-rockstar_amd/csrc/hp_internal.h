@@ -255,6 +255,33 @@ __host__ __device__ __forceinline__ float split_lim(const float *lo, const float
     return 0.5f * (h + l);
 }
 
+// Where fast3tree's in-place partition (src/fast3tree.c:762-773) moves a
+// point.  The loop swaps every point it finds above the plane with the
+// current last slot and examines the swapped-in point next; the removal of
+// non-finite points (src/fast3tree.c:875-887) is the same loop, and the
+// strip of potential-only records (src/properties.c:539-544) moves the kept
+// records the same way.  For a node of n points of which ns stay ("left":
+// pos <= lim, finite, kept) the result has a closed form in the stable
+// listing st of the node's positions: st(k) for k < ns is the position of
+// the k-th left point, st(ns + k) that of the k-th right point (0-based,
+// from the left).  For the point at position p, the rank-th of its side:
+//   left,  p < ns:  stays at p;
+//   left,  p >= ns: with c = ns - 1 - rank left points after it, it fills the
+//                   slot of the (c + 1)-th right point, st(ns + c);
+//   right: with c = rank right points before it, r_c = n for c = 0, else
+//          the position of the c-th left point from the right, st(ns - c)
+//          (none, -1, when c > ns); it goes to r_c - 1 when p < r_c, else
+//          to p - 1.
+// Checked against the loop on random class patterns: every destination is
+// the loop's and each slot is used once.
+template <typename ST>
+__host__ __device__ __forceinline__ int32_t inplace_dest(int32_t p, bool left, int32_t rank, int32_t n,
+                                                         int32_t ns, ST st) {
+    if (left) return p < ns ? p : st(ns + (ns - 1 - rank));
+    const int32_t rc = rank == 0 ? n : (rank <= ns ? st(ns - rank) : -1);
+    return p < rc ? rc - 1 : p - 1;
+}
+
 // Order-preserving map of a float onto an int, for atomic min/max.
 __device__ __forceinline__ int float_key(float f) {
     int i = __float_as_int(f);

@@ -306,10 +306,11 @@ struct Engine {
     DevBuf hfirst, hcount, root_tiles, groups, direct_blocks, scan_tiles, call_tile0,
         call_ntiles;
     // build
-    DevBuf box, bad, badlist, A, B, F, leaf_of, nodes, parent, fat, small0, group0, groupc, wave_off, wave_hsum, big, tiles, parts,
+    DevBuf box, bad, pot, badlist, A, B, F, leaf_of, sides, nodes, parent, fat, small0, group0, groupc, wave_off, wave_hsum, big, tiles, parts,
         splitres, dparts, counters, work, depth_buf;
     // unbind
-    DevBuf keys0, keys1, vals0, vals1, hist, hsum, srt, callinfo, tsum, tcand, tile_i32;
+    DevBuf keys0, keys1, vals0, vals1, hist, hsum, srt, callinfo, tsum, tcand, tile_i32, strip_cnt;
+    DevBuf stage;  // records in tree order before they go back (compute_potential's permutation)
     DevBuf seg_wave_l, seg_block_l, seg_tiles_l, seg_hist, seg_hsum, srt_rv;
     // halo properties
     DevBuf pcnt, pscan, pcall, psums, shape_st, shape_part, shape_cd, scale_calls, r_pe, pe_cnt,
@@ -363,10 +364,10 @@ struct Engine {
 
     ~Engine() {
         DevBuf *all[] = {&hfirst, &hcount, &root_tiles, &groups, &direct_blocks,
-                         &scan_tiles, &call_tile0, &call_ntiles, &box, &bad, &badlist, &A, &B, &F,
-                         &leaf_of, &nodes, &parent, &fat, &small0, &group0, &groupc, &wave_off, &wave_hsum, &big, &tiles,
+                         &scan_tiles, &call_tile0, &call_ntiles, &box, &bad, &pot, &badlist, &A, &B, &F,
+                         &leaf_of, &sides, &nodes, &parent, &fat, &small0, &group0, &groupc, &wave_off, &wave_hsum, &big, &tiles,
                          &parts, &splitres, &dparts, &counters, &work, &depth_buf, &pcnt, &pscan, &pcall, &psums, &shape_st, &shape_part, &shape_cd, &scale_calls, &r_pe, &pe_cnt, &fit_list, &fit_n, &gsegs, &gtiles, &gcen, &keys0, &keys1, &vals0,
-                         &vals1, &hist, &hsum, &srt, &callinfo, &tsum, &tcand, &tile_i32,
+                         &vals1, &hist, &hsum, &srt, &callinfo, &tsum, &tcand, &tile_i32, &strip_cnt, &stage,
                          &seg_wave_l, &seg_block_l, &seg_tiles_l, &seg_hist, &seg_hsum, &srt_rv};
         if (stream) {
             (void)hipStreamSynchronize(stream);
@@ -394,11 +395,13 @@ struct Engine {
         HP_TRY(hipSetDevice(device));
         HP_ENSURE(box, sizeof(int32_t) * 6 * (H + 1));
         HP_ENSURE(bad, sizeof(int32_t) * (H + 1));
+        HP_ENSURE(pot, sizeof(int32_t) * (H + 1));
         HP_ENSURE(badlist, sizeof(int32_t) * (H + 1));
         HP_ENSURE(A, sizeof(float4) * (N + 1));
         HP_ENSURE(B, sizeof(float4) * (N + 1));
         HP_ENSURE(F, sizeof(float4) * (N + 1));
         HP_ENSURE(leaf_of, sizeof(int32_t) * (N + 1));
+        HP_ENSURE(sides, sizeof(int32_t) * (N + 1));
         // split levels use at most about 0.1N + H ids; a subtree of n points
         // (workgroup or wavefront) reserves 2n - 2
         cap_nodes = 5 * N + H + 1024;
@@ -438,6 +441,7 @@ struct Engine {
             HP_ENSURE(tsum, sizeof(Moments) * (T.scan_tiles.size() + 1));
             HP_ENSURE(tcand, sizeof(Cand) * 2 * (T.scan_tiles.size() + 1));
             HP_ENSURE(tile_i32, sizeof(int32_t) * (T.scan_tiles.size() + 1));
+            HP_ENSURE(strip_cnt, sizeof(int32_t) * (T.scan_tiles.size() + 1));
             const int64_t m = (int64_t)T.seg_tiles.size() * kSegRadix;
             HP_ENSURE(seg_hist, sizeof(int32_t) * (m + 1));
             HP_ENSURE(seg_hsum, sizeof(int32_t) * (m / kScanBlock + 2));
@@ -490,6 +494,7 @@ struct Engine {
         BuildArgs a;
         a.nodes = nodes.as<Node>();
         a.parent = parent.as<int32_t>();
+        a.sides = sides.as<int32_t>();
         a.leaf_of = leaf_of.as<int32_t>();
         a.fin = F.as<float4>();
         a.small_next = small_next;
@@ -521,13 +526,14 @@ struct Engine {
         const int32_t H = (int32_t)T.nhalos;
         hipStream_t s = stream;
         if (H > 0) {
-            k_root_init<<<blocks(H, 256), 256, 0, s>>>(box.as<int32_t>(), bad.as<int32_t>(), H);
+            k_root_init<<<blocks(H, 256), 256, 0, s>>>(box.as<int32_t>(), bad.as<int32_t>(),
+                                                       pot.as<int32_t>(), H);
             HP_LAUNCHED();
         }
         if (!T.root_tiles.empty()) {
             k_gather<<<(int)T.root_tiles.size(), kTileThreads, 0, s>>>(
                 d_po, hfirst.as<int64_t>(), hcount.as<int64_t>(), root_tiles.as<RootTile>(),
-                A.as<float4>(), box.as<int32_t>(), bad.as<int32_t>());
+                A.as<float4>(), box.as<int32_t>(), bad.as<int32_t>(), pot.as<int32_t>());
             HP_LAUNCHED();
         }
         k_fill_i32<<<1, 64, 0, s>>>(counters.as<int32_t>(), 0, kNumCounters);
@@ -566,9 +572,13 @@ struct Engine {
             k_big_split<<<1024, 256, 0, s>>>(big.as<Task>(), parts.as<TilePart>(),
                                           splitres.as<BigSplit>(), ba);
             HP_LAUNCHED();
-            k_big_scatter<<<2048, kTileThreads, 0, s>>>(src, dst, tiles.as<int2>(), big.as<Task>(),
-                                                        parts.as<TilePart>(),
-                                                        splitres.as<BigSplit>(), ba);
+            k_big_tiles<kSides><<<2048, kTileThreads, 0, s>>>(
+                src, dst, tiles.as<int2>(), big.as<Task>(), parts.as<TilePart>(),
+                splitres.as<BigSplit>(), ba);
+            HP_LAUNCHED();
+            k_big_tiles<kScatter><<<2048, kTileThreads, 0, s>>>(
+                src, dst, tiles.as<int2>(), big.as<Task>(), parts.as<TilePart>(),
+                splitres.as<BigSplit>(), ba);
             HP_LAUNCHED();
             k_level_advance<<<1, 1, 0, s>>>(counters.as<int32_t>(), depth_buf.as<int32_t>());
             HP_LAUNCHED();
@@ -740,8 +750,12 @@ struct Engine {
         return HP_OK;
     }
 
-    // The eager driver: learns the depth of the tree as it builds it.
-    int potential(const Tables &T, struct potential *d_po, const hp_options &o) {
+    // The eager driver: learns the depth of the tree as it builds it.  With
+    // `permute`, the records end in tree order, as the reference's
+    // compute_potential leaves them (the bound pass reads the tree order
+    // itself and runs without).
+    int potential(const Tables &T, struct potential *d_po, const hp_options &o,
+                  bool permute = false) {
         st = hp_stats();
         st.records = T.nrec;
         st.halos = T.nhalos;
@@ -763,6 +777,14 @@ struct Engine {
         if (o.count_work) HP_TRY(hipMemsetAsync(work.p, 0, sizeof(unsigned long long) * 4, s));
         if ((r = enqueue_walk(T, d_po, o))) return r;
         if (timing) HP_TRY(hipEventRecord(ev[3], s));
+        if (permute && T.method != HP_METHOD_DIRECT && T.nrec > 0) {
+            if ((r = stage.ensure(sizeof(struct potential) * T.nrec))) return r;
+            k_tree_records<<<blocks(T.nrec, 256), 256, 0, s>>>(d_po, F.as<float4>(), T.nrec,
+                                                               stage.as<struct potential>());
+            HP_LAUNCHED();
+            HP_TRY(hipMemcpyAsync(d_po, stage.p, sizeof(struct potential) * T.nrec,
+                                  hipMemcpyDeviceToDevice, s));
+        }
         if ((r = check_walk())) return r;
         st.walk_groups = (int64_t)T.groups.size();
         st.walk_launches = T.groups.empty() ? 0 : 1;
@@ -1057,9 +1079,34 @@ struct Engine {
         const int64_t N = T.nrec;
         const int32_t H = (int32_t)T.nhalos;
         if (N == 0 || H == 0) return HP_OK;
-        k_make_keys<<<blocks(N, 256), 256, 0, s>>>(d_po, (int32_t)N, keys0.as<uint64_t>(),
-                                                   vals0.as<int32_t>());
-        HP_LAUNCHED();
+        // sort input: each call's records in the order the reference's
+        // compute_potential and potential-only strip leave them
+        const int32_t nst_k = (int32_t)T.scan_tiles.size();
+        KeyArgs ka;
+        ka.po = d_po;
+        ka.fin = T.method == HP_METHOD_DIRECT ? nullptr : F.as<float4>();
+        ka.tiles = scan_tiles.as<ScanTile>();
+        ka.hfirst = hfirst.as<int64_t>();
+        ka.hcount = hcount.as<int64_t>();
+        ka.pot = pot.as<int32_t>();
+        ka.call_tile0 = call_tile0.as<int32_t>();
+        ka.call_ntiles = call_ntiles.as<int32_t>();
+        ka.tcnt = strip_cnt.as<int32_t>();
+        ka.bpos = sides.as<int32_t>();
+        ka.keys = keys0.as<uint64_t>();
+        ka.vals = vals0.as<int32_t>();
+        if (nst_k > 0) {
+            k_keys_tree<<<nst_k, kKeyThreads, 0, s>>>(ka);
+            HP_LAUNCHED();
+            k_strip_count<<<nst_k, kKeyThreads, 0, s>>>(ka);
+            HP_LAUNCHED();
+            k_strip_prefix<<<blocks(H, 256), 256, 0, s>>>(ka, H);
+            HP_LAUNCHED();
+            k_strip_list<<<nst_k, kKeyThreads, 0, s>>>(ka);
+            HP_LAUNCHED();
+            k_strip_keys<<<nst_k, kKeyThreads, 0, s>>>(ka);
+            HP_LAUNCHED();
+        }
         // each call's records sorted inside its own range, into keys1 / vals1
         uint64_t *k0 = keys0.as<uint64_t>(), *k1 = keys1.as<uint64_t>();
         int32_t *v0 = vals0.as<int32_t>(), *v1 = vals1.as<int32_t>();
@@ -1067,11 +1114,12 @@ struct Engine {
         const int32_t ntb = (int32_t)T.seg_tiles.size();
         if (nw) {
             k_seg_wave<<<blocks((int64_t)nw * kWave, 256), 256, 0, s>>>(
-                k0, k1, v1, seg_wave_l.as<int32_t>(), nw, hfirst.as<int64_t>(), hcount.as<int64_t>());
+                k0, v0, k1, v1, seg_wave_l.as<int32_t>(), nw, hfirst.as<int64_t>(),
+                hcount.as<int64_t>());
             HP_LAUNCHED();
         }
         if (nb) {
-            k_seg_block<<<nb, kSortThreads, 0, s>>>(k0, k1, v1, seg_block_l.as<int32_t>(),
+            k_seg_block<<<nb, kSortThreads, 0, s>>>(k0, v0, k1, v1, seg_block_l.as<int32_t>(),
                                                     hfirst.as<int64_t>(), hcount.as<int64_t>());
             HP_LAUNCHED();
         }
@@ -1325,7 +1373,8 @@ __attribute__((visibility("default"))) void compute_potential(struct potential *
         set_error(HP_EHIP, "copy of %lld records to the device failed", (long long)num_po);
         return fatal(HP_EHIP, __FILE__, __LINE__);
     }
-    if ((r = e->potential(T, dpo.as<struct potential>(), o))) return fatal(r, __FILE__, __LINE__);
+    if ((r = e->potential(T, dpo.as<struct potential>(), o, true)))
+        return fatal(r, __FILE__, __LINE__);
     if (hipMemcpyAsync(po, dpo.p, sizeof(struct potential) * num_po, hipMemcpyDeviceToHost,
                        e->stream) != hipSuccess ||
         hipStreamSynchronize(e->stream) != hipSuccess) {
@@ -1515,7 +1564,7 @@ static int run_per_call(hp_plan *p, struct potential *d_po, hp_halo *d_halos, ui
         if (t.nrec == 0) continue;
         struct potential *po_h = d_po + p->T.first[h];
         if ((r = p->eng.load_tables(t))) return r;
-        if ((r = p->eng.potential(t, po_h, p->opt))) return r;
+        if ((r = p->eng.potential(t, po_h, p->opt, !unbind))) return r;
         if (unbind && (r = p->eng.enqueue_unbind(t, po_h, d_halos + h,
                                                  d_bound ? d_bound + p->T.first[h] : nullptr)))
             return r;
@@ -1526,7 +1575,7 @@ static int run_per_call(hp_plan *p, struct potential *d_po, hp_halo *d_halos, ui
 
 static int plan_potential(hp_plan *p, struct potential *d_po) {
     if (p->opt.per_call) return run_per_call(p, d_po, nullptr, nullptr, false);
-    return p->eng.potential(p->T, d_po, p->opt);
+    return p->eng.potential(p->T, d_po, p->opt, true);
 }
 
 // The entry points below run after the work queued on the caller's stream

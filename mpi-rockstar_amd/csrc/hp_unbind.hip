@@ -4,10 +4,11 @@
 //   1. keys: (potential-only flag, r2 bits), sorted inside each call (the
 //      calls tile the record array in order, so each call's sorted records
 //      stay in its own range): every call's records by r2 with the
-//      potential-only records (ke < 0) last; ties keep their input order,
-//      which is what glibc's merge-sort qsort does with the reference's
-//      dist_compare (src/groupies.c:202-210).  r2 >= 0, so its bits sort as
-//      unsigned.  Calls of at most 64 records are ranked by one wavefront,
+//      potential-only records (ke < 0) last; ties keep the order the
+//      reference's compute_potential and potential-only strip leave (see
+//      "sort input" below), which is what glibc's merge-sort qsort does with
+//      the reference's dist_compare (src/groupies.c:202-210).  r2 >= 0, so
+//      its bits sort as unsigned.  Calls of at most 64 records are ranked by one wavefront,
 //      calls of at most 1,024 sorted by one workgroup (bitonic, in LDS),
 //      larger ones by a stable LSD
 //      radix sort (3 passes of 11-bit digits) over tiles that never straddle
@@ -43,17 +44,18 @@ constexpr int kRadix = 1 << kRadixBits;
 constexpr int kRadixPer = kRadix / kSortThreads;  // buckets per thread
 constexpr int kScanTile = 2048;
 
+// kScanTile consecutive records of one call, from record `off` of the call
+struct ScanTile {
+    int32_t call;
+    int32_t off;
+};
+
 // ---------------------------------------------------------------- radix sort
 
 // key of one record inside its call: potential-only flag above the r2 bits
-__global__ void k_make_keys(const struct potential *__restrict__ po, int32_t nrec,
-                            uint64_t *__restrict__ keys, int32_t *__restrict__ vals) {
-    const int32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= nrec) return;
-    const float r2 = po[i].r2;
-    const uint64_t pot = po[i].ke < 0.0f ? 1ull : 0ull;
-    keys[i] = (pot << 32) | (uint64_t)__float_as_uint(r2);
-    vals[i] = i;
+__device__ __forceinline__ uint64_t sort_key(const struct potential *__restrict__ r) {
+    const uint64_t pot = r->ke < 0.0f ? 1ull : 0ull;
+    return (pot << 32) | (uint64_t)__float_as_uint(r->r2);
 }
 
 // digit histogram of one tile: hist[digit * ntiles + tile]
@@ -201,8 +203,10 @@ __global__ __launch_bounds__(kSortThreads) void k_radix_scatter(
 // ------------------------------------------------------ sort inside calls
 
 // A call of at most 64 records, one wavefront: record i's place is the
-// number of records with a smaller (key, position).
+// number of records with a smaller (key, position); the value carried is
+// the record index at that position (vin).
 __global__ __launch_bounds__(256) void k_seg_wave(const uint64_t *__restrict__ kin,
+                                                  const int32_t *__restrict__ vin,
                                                   uint64_t *__restrict__ kout,
                                                   int32_t *__restrict__ vout,
                                                   const int32_t *__restrict__ calls,
@@ -220,7 +224,7 @@ __global__ __launch_bounds__(256) void k_seg_wave(const uint64_t *__restrict__ k
     for (int j = 0; j < n; j++) rank += ((read_lane(key, j) << 6) | (uint64_t)j) < me ? 1 : 0;
     if (v) {
         kout[first + rank] = key;
-        vout[first + rank] = first + lane;
+        vout[first + rank] = vin[first + lane];
     }
 }
 
@@ -229,6 +233,7 @@ __global__ __launch_bounds__(256) void k_seg_wave(const uint64_t *__restrict__ k
 // a power of two with keys above every record's.
 constexpr int kSegBlockMax = 1024;
 __global__ __launch_bounds__(kSortThreads) void k_seg_block(const uint64_t *__restrict__ kin,
+                                                            const int32_t *__restrict__ vin,
                                                             uint64_t *__restrict__ kout,
                                                             int32_t *__restrict__ vout,
                                                             const int32_t *__restrict__ calls,
@@ -260,7 +265,7 @@ __global__ __launch_bounds__(kSortThreads) void k_seg_block(const uint64_t *__re
     for (int32_t i = threadIdx.x; i < n; i += kSortThreads) {
         const uint64_t kk = s_k[i];
         kout[first + i] = kk >> 10;
-        vout[first + i] = first + (int32_t)(kk & 1023);
+        vout[first + i] = vin[first + (int32_t)(kk & 1023)];
     }
 }
 
@@ -353,6 +358,153 @@ __global__ __launch_bounds__(kSortThreads) void k_seg_scatter(
     }
 }
 
+// ------------------------------------------------------------ sort input
+//
+// The reference sorts each call's records (glibc's qsort: a stable merge
+// sort) in the order compute_potential and the potential-only strip leave
+// them: the call's tree order (the build reproduces the reference's
+// in-place partitions), then the strip of src/properties.c:539-544, which
+// overwrites each potential-only record with the current last one.  The
+// strip moves the kept records as the in-place partition moves its left
+// side (inplace_dest with ns = the kept count):
+//   kept record at tree position t < nk: stays at t;
+//   kept record at t >= nk, with c kept records after it: goes to the tree
+//   position of the (c + 1)-th potential-only record.
+// The keys are written at each record's place in that order, so that the
+// stable sort breaks ties as the reference does.  Calls without potential-
+// only records keep the tree order (k_keys_tree); the others go through
+// k_strip_count, k_strip_prefix, k_strip_list and k_strip_keys, which skip
+// every other call.  Potential-only records follow the kept ones.
+constexpr int kKeyThreads = 256;
+constexpr int kKeyPer = kScanTile / kKeyThreads;  // consecutive positions per thread
+
+struct KeyArgs {
+    const struct potential *po;
+    const float4 *fin;          // tree order (record index in w); null: the input order
+    const ScanTile *tiles;
+    const int64_t *hfirst, *hcount;
+    const int32_t *pot;         // potential-only records of each call
+    const int32_t *call_tile0, *call_ntiles;
+    int32_t *tcnt;              // per tile: its potential-only records, then their prefix
+    int32_t *bpos;              // per position: tree positions of the potential-only records
+    uint64_t *keys;
+    int32_t *vals;
+};
+
+__device__ __forceinline__ int32_t record_at(const KeyArgs &a, int64_t pos) {
+    return a.fin ? __float_as_int(a.fin[pos].w) : (int32_t)pos;
+}
+
+// calls without potential-only records: keys in tree order
+__global__ __launch_bounds__(kKeyThreads) void k_keys_tree(KeyArgs a) {
+    const ScanTile tl = a.tiles[blockIdx.x];
+    if (a.pot[tl.call] != 0) return;
+    const int64_t first = a.hfirst[tl.call], n = a.hcount[tl.call];
+    for (int j = threadIdx.x; j < kScanTile; j += kKeyThreads) {
+        const int64_t t = tl.off + j;
+        if (t >= n) break;
+        const int32_t idx = record_at(a, first + t);
+        a.keys[first + t] = sort_key(a.po + idx);
+        a.vals[first + t] = idx;
+    }
+}
+
+// Potential-only flags of this thread's kKeyPer consecutive positions of a
+// tile, their exclusive rank among the tile's and the tile's total.
+struct StripRun {
+    int32_t idx[kKeyPer];
+    uint32_t potm;   // bit j: position j of the run is potential-only
+    int32_t before;  // potential-only positions of the tile before the run
+    int32_t total;
+};
+
+__device__ StripRun strip_run(const KeyArgs &a, const ScanTile &tl, int64_t first, int64_t n) {
+    __shared__ int32_t s_w[kKeyThreads / kWave];
+    StripRun r;
+    r.potm = 0;
+    const int64_t t0 = tl.off + (int64_t)threadIdx.x * kKeyPer;
+#pragma unroll
+    for (int j = 0; j < kKeyPer; j++) {
+        r.idx[j] = -1;
+        if (t0 + j < n) {
+            r.idx[j] = record_at(a, first + t0 + j);
+            if (a.po[r.idx[j]].ke < 0.0f) r.potm |= 1u << j;
+        }
+    }
+    const int c = __popc(r.potm);
+    const int incl = wave_scan_add(c);
+    const int w = threadIdx.x / kWave;
+    if (lane_id() == kWave - 1) s_w[w] = incl;
+    __syncthreads();
+    int wb = 0, tot = 0;
+    for (int k = 0; k < kKeyThreads / kWave; k++) {
+        wb += k < w ? s_w[k] : 0;
+        tot += s_w[k];
+    }
+    __syncthreads();
+    r.before = wb + incl - c;
+    r.total = tot;
+    return r;
+}
+
+__global__ __launch_bounds__(kKeyThreads) void k_strip_count(KeyArgs a) {
+    const ScanTile tl = a.tiles[blockIdx.x];
+    if (a.pot[tl.call] == 0) return;
+    const StripRun r = strip_run(a, tl, a.hfirst[tl.call], a.hcount[tl.call]);
+    if (threadIdx.x == 0) a.tcnt[blockIdx.x] = r.total;
+}
+
+// per call with potential-only records: exclusive prefix of its tiles' counts
+__global__ void k_strip_prefix(KeyArgs a, int32_t ncalls) {
+    const int32_t h = blockIdx.x * blockDim.x + threadIdx.x;
+    if (h >= ncalls || a.pot[h] == 0) return;
+    int32_t run = 0;
+    for (int32_t k = a.call_tile0[h]; k < a.call_tile0[h] + a.call_ntiles[h]; k++) {
+        const int32_t c = a.tcnt[k];
+        a.tcnt[k] = run;
+        run += c;
+    }
+}
+
+// the tree positions of each call's potential-only records, in order
+__global__ __launch_bounds__(kKeyThreads) void k_strip_list(KeyArgs a) {
+    const ScanTile tl = a.tiles[blockIdx.x];
+    if (a.pot[tl.call] == 0) return;
+    const int64_t first = a.hfirst[tl.call];
+    const StripRun r = strip_run(a, tl, first, a.hcount[tl.call]);
+    int32_t b = a.tcnt[blockIdx.x] + r.before;
+    const int64_t t0 = tl.off + (int64_t)threadIdx.x * kKeyPer;
+#pragma unroll
+    for (int j = 0; j < kKeyPer; j++)
+        if (r.potm >> j & 1u) a.bpos[first + b++] = (int32_t)(t0 + j);
+}
+
+// keys at each record's place after the strip
+__global__ __launch_bounds__(kKeyThreads) void k_strip_keys(KeyArgs a) {
+    const ScanTile tl = a.tiles[blockIdx.x];
+    const int32_t npot = a.pot[tl.call];
+    if (npot == 0) return;
+    const int64_t first = a.hfirst[tl.call], n = a.hcount[tl.call];
+    const int64_t nk = n - npot;
+    const StripRun r = strip_run(a, tl, first, n);
+    int32_t b = a.tcnt[blockIdx.x] + r.before;  // potential-only positions before t
+    const int64_t t0 = tl.off + (int64_t)threadIdx.x * kKeyPer;
+#pragma unroll
+    for (int j = 0; j < kKeyPer; j++) {
+        const int64_t t = t0 + j;
+        if (t >= n) break;
+        int64_t q;
+        if (r.potm >> j & 1u) {
+            q = nk + b++;
+        } else {
+            const int64_t kept_rank = t - b;
+            q = t < nk ? t : (int64_t)a.bpos[first + (nk - 1 - kept_rank)];
+        }
+        a.keys[first + q] = sort_key(a.po + r.idx[j]);
+        a.vals[first + q] = r.idx[j];
+    }
+}
+
 // ----------------------------------------------------------------- gather
 
 // the records in sorted order, and a compact copy of what the core-velocity
@@ -402,10 +554,6 @@ __global__ void k_call_kept(const uint64_t *__restrict__ skeys, const int64_t *_
 
 // Scan tiles: tile t covers sorted records [first + off, first + off + 2048)
 // of one call; every kernel below clips it to the call's live prefix.
-struct ScanTile {
-    int32_t call;
-    int32_t off;
-};
 
 __device__ __forceinline__ int32_t block_max_i32(int32_t v) {
     __shared__ int32_t s_m[4];

@@ -340,6 +340,16 @@ __global__ __launch_bounds__(256) void k_direct(DirectArgs a) {
     if (live) a.po[__float_as_int(me.w)].pe = acc.value();
 }
 
+// The records in tree order, the order the reference's compute_potential
+// leaves them in (its tree build permutes po in place, src/fast3tree.c:
+// 762-773, 875-887): out[j] = po[record at tree position j].
+__global__ void k_tree_records(const struct potential *__restrict__ po,
+                               const float4 *__restrict__ fin, int64_t n,
+                               struct potential *__restrict__ out) {
+    const int64_t j = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (j < n) out[j] = po[__float_as_int(fin[j].w)];
+}
+
 // pe = 0 before the direct kernel, which writes finite records only
 __global__ void k_zero_pe(struct potential *po, int64_t n) {
     int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;

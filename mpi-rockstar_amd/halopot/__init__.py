@@ -3,7 +3,8 @@
 Mirrors the reference's operator interface for this path:
   * ``compute_potential(po)`` -- src/potential.h:24, on an array of
     ``struct potential`` records (``catalog.POTENTIAL_DTYPE``); pe is written in
-    place, records keep their places.
+    place and the records are permuted into tree order, as the reference's
+    compute_potential leaves them.
   * ``compute_kinetic_energy(po, vel_cen, pos_cen, scale_now)`` --
     src/potential.h:22-23.
   * ``set_config(params)`` -- the configuration globals the path reads
@@ -185,7 +186,8 @@ def _records(po: np.ndarray) -> np.ndarray:
 
 
 def compute_potential(po: np.ndarray) -> None:
-    """src/potential.c:183-192 on the GPU; pe written in place."""
+    """src/potential.c:183-192 on the GPU; pe written in place, records
+    permuted as the reference permutes them."""
     _records(po)
     lib().compute_potential(_ptr(po), po.shape[0])
 

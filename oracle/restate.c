@@ -35,15 +35,16 @@
  *                                           384-510; src/nfw.c:13-173
  *                                           (scale radius); src/jacobi.c:8-135
  *
- * Deliberate differences from the reference, none of which changes a node's
- * membership, a walk decision or a value beyond f64 summation order:
- *   - the partition inside a node is stable (left keeps input order, then
- *     right), where the reference swaps in place; node membership is the set
- *     {pos[dim] <= lim} either way;
- *   - non-finite points are moved to the tail stably;
- *   - nodes are numbered breadth-first, not depth-first.  The walk still
- *     visits left before right, so every sink leaf sees the same sequence of
- *     decisions.
+ * The record order follows the reference's in-place loops: the non-finite
+ * removal and the node partition swap with the current last slot
+ * (src/fast3tree.c:762-773, 875-887), the potential-only strip overwrites
+ * with the last record (src/properties.c:539-544), and the r2 sort is a
+ * stable merge sort (glibc's qsort).  So compute_potential permutes po, and
+ * the bound pass leaves it, exactly as the reference does.
+ * Deliberate difference, which changes no node, decision, value or order:
+ * nodes are numbered breadth-first, not depth-first.  The walk still visits
+ * left before right, so every sink leaf sees the same sequence of
+ * decisions.
  * Pinned by: the compiled reference (oracle/_ref) on the fixtures, the FP64
  * direct sum (N <= 10 is exact; the walk's mass bookkeeping is checked by an
  * interaction-count invariant) and closed forms for the kinetic energy.
@@ -168,18 +169,28 @@ static int orc_split_dim(const float lo[3], const float hi[3]) {
     return dim;
 }
 
+static void orc_swap(struct potential *a, struct potential *b) {
+    struct potential t = *a;
+    *a = *b;
+    *b = t;
+}
+
 /* Builds the tree over po[0..n); po is permuted so that every node owns a
-   contiguous range.  Returns the number of finite points. */
+   contiguous range, in the order the reference's in-place loops leave it.
+   Returns the number of finite points. */
 static int64_t orc_build(orc_tree *t, struct potential *po, int64_t n,
                          struct potential *scratch) {
-    /* stable move of non-finite points to the tail (src/fast3tree.c:875-887
-       drops them from the tree) */
-    int64_t nf = 0, nb = 0;
-    for (int64_t i = 0; i < n; i++) {
-        if (orc_finite3(po[i].pos)) po[nf++] = po[i];
-        else scratch[nb++] = po[i];
+    (void)scratch;
+    /* non-finite points are swapped with the current last one and the slot
+       is examined again (src/fast3tree.c:875-887 drops them from the tree) */
+    int64_t nf = n;
+    for (int64_t i = 0; i < nf; i++) {
+        if (!orc_finite3(po[i].pos)) {
+            nf--;
+            orc_swap(po + i, po + nf);
+            i--;
+        }
     }
-    for (int64_t i = 0; i < nb; i++) po[nf + i] = scratch[i];
 
     t->nnodes = 0;
     t->nfinite = nf;
@@ -200,12 +211,20 @@ static int64_t orc_build(orc_tree *t, struct potential *po, int64_t n,
         if (nd->hi[dim] == nd->lo[dim]) continue; /* src/fast3tree.c:760-761 */
         /* float sum, halved exactly (src/fast3tree.c:757) */
         float lim = 0.5f * (nd->hi[dim] + nd->lo[dim]);
-        int64_t nl = 0, nr = 0;
-        for (int64_t i = nd->first; i < nd->first + nd->n; i++) {
-            if (po[i].pos[dim] <= lim) po[nd->first + nl++] = po[i];
-            else scratch[nr++] = po[i];
+        /* in-place partition (src/fast3tree.c:762-773): a point above the
+           plane is swapped with the current last slot, which is examined
+           next; the loop ends where the two ends meet */
+        struct potential *q = po + nd->first;
+        int64_t i = 0, j = nd->n - 1;
+        for (; i < j; i++) {
+            if (q[i].pos[dim] > lim) {
+                orc_swap(q + i, q + j);
+                i--;
+                j--;
+            }
         }
-        for (int64_t i = 0; i < nr; i++) po[nd->first + nl + i] = scratch[i];
+        if (i == j && q[i].pos[dim] <= lim) i++;
+        const int64_t nl = i;
         if (nl == 0 || nl == nd->n) continue; /* src/fast3tree.c:805-809 */
         int64_t a = orc_new_node(t);
         int64_t b = orc_new_node(t);
@@ -599,6 +618,20 @@ static void orc_merge_sort_r2(struct potential *a, struct potential *tmp,
     }
 }
 
+/* Drops the potential-only records (ke < 0) the reference's way
+   (src/properties.c:539-544): the slot is overwritten with the current last
+   record and examined again.  Returns the records kept, in po[0..kept). */
+static int64_t orc_strip(struct potential *po, int64_t n) {
+    int64_t total = n;
+    for (int64_t j = 0; j < total; j++)
+        if (po[j].ke < 0) {
+            total--;
+            po[j] = po[total];
+            j--;
+        }
+    return total;
+}
+
 /* src/properties.c:537-550: potential, drop potential-only records, sort by
    radius, core velocity, kinetic energy about the core (sub-halos) or bulk
    (hosts) velocity.  Returns the kept record count; bound[i] = !(pe < ke)
@@ -606,9 +639,7 @@ static void orc_merge_sort_r2(struct potential *a, struct potential *tmp,
 int64_t orc_bound_pass(struct potential *po, int64_t n, orc_halo *h,
                        const orc_params *p, uint8_t *bound) {
     if (n > 1) orc_compute_potential(po, n, p, NULL, NULL, NULL, NULL, 0);
-    int64_t kept = 0;
-    for (int64_t i = 0; i < n; i++)
-        if (!(po[i].ke < 0)) po[kept++] = po[i];
+    int64_t kept = orc_strip(po, n);
     if (kept > 1) {
         struct potential *tmp = orc_grow(NULL, sizeof(struct potential) * (size_t)kept);
         orc_merge_sort_r2(po, tmp, kept);
@@ -657,9 +688,7 @@ typedef void (*orc_ke_fn)(struct potential *, int64_t, float *, float *, float);
 static int64_t orc_bound_pass_fn(struct potential *po, int64_t n, orc_halo *h,
                                  const orc_params *p, orc_cp_fn cp, orc_ke_fn ke) {
     if (n > 1) cp(po, n);
-    int64_t kept = 0;
-    for (int64_t i = 0; i < n; i++)
-        if (!(po[i].ke < 0)) po[kept++] = po[i];
+    int64_t kept = orc_strip(po, n);
     if (kept > 1) {
         struct potential *tmp = orc_grow(NULL, sizeof(struct potential) * (size_t)kept);
         orc_merge_sort_r2(po, tmp, kept);

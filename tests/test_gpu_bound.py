@@ -4,7 +4,9 @@ compute_kinetic_energy, bound test !(pe < ke) of src/properties.c:403)
 against the restatement of that driver, call by call.
 
 Tolerances:
-  * kept counts, the r2 order, the record set: identical;
+  * kept counts and the record order (ties in r2 included: the order the
+    reference's in-place tree build and potential-only strip leave before its
+    stable sort): identical;
   * n_core (the selected prefix): identical;
   * corevel / bulkvel: 1e-6 relative (the GPU restates the running Welford
     moments with exact prefix sums; the two differ in the last bits);
@@ -41,11 +43,10 @@ def _check_batch(restatement, cat, opts=None):
         g = po_g[f:f + k]
         o = po_o[f:f + k]
         np.testing.assert_array_equal(g["r2"], o["r2"])
-        gi = np.lexsort((g["flags"], g["r2"]))
-        oi = np.lexsort((o["flags"], o["r2"]))
-        np.testing.assert_array_equal(g["flags"][gi], o["flags"][oi])
+        # the same records in the same order, ties in r2 included
+        np.testing.assert_array_equal(g["flags"], o["flags"], err_msg=f"call {h}")
         if k > 1:
-            assert np.max(rel(g["pe"][gi], o["pe"][oi])) < 1e-5, h
+            assert np.max(rel(g["pe"], o["pe"])) < 1e-5, h
         hg, ho = h_g[h], h_o[h]
         assert hg["n_core"] == ho["n_core"], h
         np.testing.assert_allclose(hg["corevel"], ho["corevel"], rtol=1e-6, atol=1e-4)
@@ -54,13 +55,13 @@ def _check_batch(restatement, cat, opts=None):
         np.testing.assert_allclose(hg["min_vel_err"], ho["min_vel_err"], rtol=1e-5, atol=1e-6)
         np.testing.assert_allclose(hg["min_bulkvel_err"], ho["min_bulkvel_err"], rtol=1e-5,
                                    atol=1e-6)
-        ke_g, ke_o = g["ke"][gi], o["ke"][oi]
+        ke_g, ke_o = g["ke"], o["ke"]
         np.testing.assert_allclose(ke_g, ke_o, rtol=1e-5, atol=1e-3)
-        bg = bound_g[f:f + k][gi].astype(bool)
-        bo = bound_o[f:f + k][oi].astype(bool)
+        bg = bound_g[f:f + k].astype(bool)
+        bo = bound_o[f:f + k].astype(bool)
         mism = bg != bo
         if mism.any():
-            pe = o["pe"][oi][mism]
+            pe = o["pe"][mism]
             margin = np.abs(pe - ke_o[mism]) / np.maximum(pe, 1e-300)
             assert np.all(margin < MARGIN), (h, margin.max())
             n_margin += int(mism.sum())

@@ -4,8 +4,10 @@ was built, and the FP64 evaluations.
 
 Tolerances:
   * tree: identical (node boxes, counts, centres bit for bit; record order in
-    tree order identical; dmin within one float ulp, its f64 sum being
-    reduced in another order);
+    tree order identical, the order the reference's in-place partitions
+    leave; dmin within one float ulp, the f64 sums of the big nodes being
+    reduced by tiles);
+  * compute_potential permutes the records as the reference's does;
   * interaction counts (P2P pairs, P2M terms): identical to the restatement's,
     i.e. every acceptance decision is the reference's;
   * pe vs the restatement / reference Barnes-Hut: max relative 1e-5
@@ -102,7 +104,8 @@ def test_tree_identical_coincident_points(restatement):
     halopot.compute_potential(g)
     o = po.copy()
     restatement.compute_potential(o, prm)  # permutes o
-    np.testing.assert_allclose(g["pe"], by_id(o), rtol=1e-5)
+    np.testing.assert_array_equal(g["flags"], o["flags"])
+    np.testing.assert_allclose(g["pe"], o["pe"], rtol=1e-5)
 
 
 def test_c1_potential_parity(restatement, request):
@@ -112,15 +115,18 @@ def test_c1_potential_parity(restatement, request):
     po = with_ids(cat.po)
     g = po.copy()
     halopot.compute_potential(g)
-    np.testing.assert_array_equal(g["flags"], po["flags"])  # records keep their places
+    # the records are permuted as the reference's tree build permutes po
+    # (src/fast3tree.c:762-773), every field travelling with its record
     o = po.copy()
     restatement.compute_potential(o, prm)
-    po_ref = by_id(o)
-    r = rel(g["pe"], po_ref)
+    np.testing.assert_array_equal(g["flags"], o["flags"])
+    for f in ("pos", "r2", "ke"):
+        np.testing.assert_array_equal(g[f], po[f][g["flags"]])
+    r = rel(g["pe"], o["pe"])
     assert np.max(r) < 1e-5, np.max(r)
     b64 = po.copy()
     restatement.compute_potential_bh64(b64, prm)
-    r64 = rel(g["pe"], by_id(b64))
+    r64 = rel(by_id(g), by_id(b64))
     assert np.median(r64) < 1e-6 and np.max(r64) < 1e-5, (np.median(r64), np.max(r64))
     try:
         ref = request.getfixturevalue("reference")
@@ -128,7 +134,8 @@ def test_c1_potential_parity(restatement, request):
         return
     a = po.copy()
     ref.compute_potential(a, prm)
-    rr = rel(g["pe"], by_id(a))
+    np.testing.assert_array_equal(g["flags"], a["flags"])
+    rr = rel(g["pe"], a["pe"])
     assert np.max(rr) < 1e-5, np.max(rr)
 
 
@@ -136,6 +143,9 @@ def test_c1_potential_parity(restatement, request):
 def test_batch_counts_and_accumulation(restatement, acc):
     cat = catalog.small_batch(seed=7, n_halos=60, lo=2, hi=5000)
     prm = cat.params
+    for h in range(cat.n_halos):  # ids local to each call, as with_ids gives them
+        f, c = int(cat.halos["first"][h]), int(cat.halos["count"][h])
+        cat.po["flags"][f:f + c] = np.arange(c, dtype=np.int32)
     po_out, _, _, st = run_plan(cat, halopot.options(accumulate=acc, count_work=1),
                                 potential_only=True)
     want_p2p = want_p2m = 0
@@ -146,7 +156,9 @@ def test_batch_counts_and_accumulation(restatement, acc):
         want_p2p += s["p2p"]
         want_p2m += s["p2m"]
         f = int(cat.halos["first"][h])
-        got = po_out["pe"][f:f + seg.shape[0]]
+        out = po_out[f:f + seg.shape[0]]
+        np.testing.assert_array_equal(out["flags"], o["flags"])  # the reference's permutation
+        got = by_id(out)
         if acc == halopot.ACC_F64:
             # every term in FP64 on the same tree: the FP64 evaluation of the
             # reference's decisions (its own P2M quotient is a float)
@@ -194,7 +206,8 @@ def test_edge_cases(restatement):
         halopot.compute_potential(g)
         o = with_ids(c)
         restatement.compute_potential(o, prm)
-        np.testing.assert_allclose(g["pe"], by_id(o), rtol=1e-6, atol=0)
+        np.testing.assert_array_equal(g["flags"], o["flags"])
+        np.testing.assert_allclose(g["pe"], o["pe"], rtol=1e-6, atol=0)
     assert cases[0]["pe"].sum() == 0  # inputs untouched
     g = with_ids(cases[0])
     halopot.compute_potential(g)
@@ -207,4 +220,7 @@ def test_edge_cases(restatement):
     assert np.all(g["pe"] == 0)
     g = with_ids(cases[4])
     halopot.compute_potential(g)
-    assert g["pe"][3] == 0 and g["pe"][17] == 0 and np.all(g["pe"][[0, 1, 2, 4]] > 0)
+    pe = by_id(g)
+    assert pe[3] == 0 and pe[17] == 0 and np.all(pe[[0, 1, 2, 4]] > 0)
+    # non-finite records end at the tail (src/fast3tree.c:875-887)
+    assert set(g["flags"][-2:]) == {3, 17}

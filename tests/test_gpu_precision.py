@@ -70,7 +70,7 @@ def test_precision_sweep(c1_refs):
     failures = []
     for name, acc in MODES:
         po_g, _, _, _ = run_plan(cat, halopot.options(accumulate=acc), potential_only=True)
-        pe = po_g["pe"]  # records keep their places: index = record id
+        pe = by_id(po_g)  # records permuted into tree order: flags hold the record ids
         rd = rel(pe, direct)
         rs = rel(pe, bh64)
         rr = rel(pe, refbh)
@@ -79,11 +79,10 @@ def test_precision_sweep(c1_refs):
         assert int(h_b["kept"][0]) == kept
         g = po_b[:kept]
         o = po_o[:kept]
-        gi = np.lexsort((g["flags"], g["r2"]))
-        oi = np.lexsort((o["flags"], o["r2"]))
-        bg = bound_b[:kept][gi].astype(bool)
-        bo = bound_o[:kept][oi].astype(bool)
-        pe_o, ke_o = o["pe"][oi], o["ke"][oi].astype(np.float64)
+        np.testing.assert_array_equal(g["flags"], o["flags"])  # the reference's order
+        bg = bound_b[:kept].astype(bool)
+        bo = bound_o[:kept].astype(bool)
+        pe_o, ke_o = o["pe"], o["ke"].astype(np.float64)
         # physical energies: pe is stored positive (binding), bound <=> pe > ke
         margin = np.abs(ke_o - pe_o) / np.maximum(np.abs(pe_o), 1e-300)
         mism = bg != bo

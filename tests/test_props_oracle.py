@@ -95,20 +95,45 @@ def test_restated_bound_pass_matches_reference(restatement, refprops):
         assert ka == kb, h
         for fld in ("pos", "corevel", "bulkvel", "min_vel_err", "min_bulkvel_err", "n_core"):
             np.testing.assert_array_equal(row[0][fld], bh[fld], err_msg=f"call {h} {fld}")
-        # equal r2: the reference orders ties by its tree build's in-place
-        # swaps (src/fast3tree.c:751-774) before the stable sort
+        # the same records in the same order, ties in r2 included: the
+        # reference orders ties by its tree build's in-place swaps
+        # (src/fast3tree.c:762-773) before the stable sort, and so does the
+        # restatement
         if np.unique(a["r2"][:ka]).size < ka:
             n_ties += 1
-            ia, ib = np.argsort(a["flags"][:ka]), np.argsort(b["flags"][:kb])
-            np.testing.assert_array_equal(a["flags"][:ka][ia], b["flags"][:kb][ib])
-            a, b = a[:ka][ia], b[:kb][ib]
-        else:
-            np.testing.assert_array_equal(a["flags"][:ka], b["flags"][:kb], err_msg=f"call {h}")
-            a, b = a[:ka], b[:kb]
+        np.testing.assert_array_equal(a["flags"][:ka], b["flags"][:kb], err_msg=f"call {h}")
+        a, b = a[:ka], b[:kb]
         np.testing.assert_array_equal(a["ke"], b["ke"], err_msg=f"call {h}")
         if ka:
             assert np.max(np.abs(a["pe"] - b["pe"]) / np.abs(b["pe"])) < 1e-14, h
     assert 0 < n_ties < cat.n_halos // 10
+
+
+@pytest.mark.parametrize("seed", [3, 4])
+def test_restated_bound_pass_order_with_potential_only_records(restatement, refprops, seed):
+    """Calls with potential-only records (ke < 0) spread through the call:
+    the reference drops them by overwriting each with the current last record
+    (src/properties.c:539-544), which moves kept records before the stable
+    sort; ties in r2 are made common by rounding r2, so the order the strip
+    and the tree build leave decides the result."""
+    rng = np.random.default_rng(seed)
+    cat = catalog.small_batch(seed=seed, n_halos=40, lo=12, hi=4000, potential_only=0.0)
+    prm = cat.params
+    for h in range(cat.n_halos):
+        f, c = int(cat.halos["first"][h]), int(cat.halos["count"][h])
+        recs = cat.po[f:f + c].copy()
+        recs["flags"] = np.arange(c, dtype=np.int32)
+        recs["ke"][rng.random(c) < 0.3] = -1.0
+        recs["r2"] = np.round(recs["r2"], 2).astype(np.float32)   # many equal r2
+        a, b = recs.copy(), recs.copy()
+        row = cat.halos[h:h + 1].copy()
+        ka, _ = restatement.bound_pass(a, row, prm)
+        kb, bh = refprops.bound_pass(b, cat.halos[h], prm)
+        assert ka == kb == int((recs["ke"] >= 0).sum()), h
+        np.testing.assert_array_equal(a["flags"][:ka], b["flags"][:kb], err_msg=f"call {h}")
+        np.testing.assert_array_equal(a["ke"][:ka], b["ke"][:kb], err_msg=f"call {h}")
+        for fld in ("corevel", "bulkvel", "n_core"):
+            np.testing.assert_array_equal(row[0][fld], bh[fld], err_msg=f"call {h} {fld}")
 
 
 def _records(r2, pe, ke, rng, scale=0.1):
