@@ -613,7 +613,6 @@ __global__ __launch_bounds__(kSubThreads, kCap <= 256 ? 7 : 1) void k_subtree(co
                   kSubNodes = SubSize<kCap>::nodes, kSubPerThread = SubSize<kCap>::per_thread;
     __shared__ float4 P[2][kCap];
     __shared__ uint8_t Q[2][kCap];       // level entry of each position
-    __shared__ int16_t ST[kCap];         // each split node's points listed by side
     __shared__ SubEntry E[2][kSubList];
     __shared__ int32_t BOX[kSubList][6];      // next level's tight boxes, float keys
     __shared__ int16_t s_left[kSubNodes];     // left child of each local node, -1: leaf
@@ -771,25 +770,32 @@ __global__ __launch_bounds__(kSubThreads, kCap <= 256 ? 7 : 1) void k_subtree(co
             }
             __syncthreads();
             // 3. scatter: split points to their child's place in the next
-            //    buffer, in the order of the reference's in-place partition;
-            //    leaf points are in tree order already.  First each split
-            //    node's points are listed by side (the stable order), which
-            //    the closed form reads.
+            //    buffer, in the order of the reference's in-place partition
+            //    (inplace_dest); leaf points are in tree order already.  The
+            //    node's k-th left (right) point is the (Sx(lf) + k)-th set
+            //    (lf - Sx(lf) + k)-th clear bit of the left flags: a binary
+            //    search over the chunk prefixes, then a select in the word.
             auto Sx = [&](int x) {
                 const int c = x >> 6;
                 return s_pre[c] + __popcll(s_mask[c] & lanes_below(x & 63));
             };
-            for (int r = 0; r < rounds; r++) {
-                const int i = r * kSubThreads + tid;
-                if (i >= n) break;
-                const uint8_t e = Q[cur][i];
-                if (e == kNoEntry || !E[cur][e].kind) continue;
-                const SubEntry en = E[cur][e];
-                const bool gl = comp(P[buf][i], en.dim) <= en.lim;
-                const int sl = Sx(i) - Sx(en.lf);
-                ST[gl ? en.lf + sl : en.lf + en.nl + (i - en.lf - sl)] = (int16_t)(i - en.lf);
-            }
-            __syncthreads();
+            // r-th set / clear bit overall, searched among chunks [lo, hi]
+            auto nth_left = [&](int r, int lo, int hi) {
+                while (lo < hi) {
+                    const int mid = (lo + hi + 1) >> 1;
+                    if (s_pre[mid] <= r) lo = mid;
+                    else hi = mid - 1;
+                }
+                return lo * 64 + select_bit(s_mask[lo], r - s_pre[lo]);
+            };
+            auto nth_right = [&](int r, int lo, int hi) {
+                while (lo < hi) {
+                    const int mid = (lo + hi + 1) >> 1;
+                    if (64 * mid - s_pre[mid] <= r) lo = mid;
+                    else hi = mid - 1;
+                }
+                return lo * 64 + select_bit(~s_mask[lo], r - (64 * lo - s_pre[lo]));
+            };
             for (int r = 0; r < rounds; r++) {
                 const int i = r * kSubThreads + tid;
                 if (i >= n) break;
@@ -801,10 +807,14 @@ __global__ __launch_bounds__(kSubThreads, kCap <= 256 ? 7 : 1) void k_subtree(co
                     const SubEntry en = E[cur][e];
                     if (en.kind) {
                         const bool gl = comp(p, en.dim) <= en.lim;
-                        const int sl = Sx(i) - Sx(en.lf);
-                        const int lf = en.lf;
-                        dst = lf + inplace_dest(i - lf, gl, gl ? sl : i - lf - sl, en.n, en.nl,
-                                                [&](int k) { return (int32_t)ST[lf + k]; });
+                        const int lf = en.lf, s0 = Sx(lf), sl = Sx(i) - s0, nl = en.nl;
+                        const int clo = lf >> 6, chi = (lf + en.n - 1) >> 6;  // the node's chunks
+                        dst = lf + inplace_dest(i - lf, gl, gl ? sl : i - lf - sl, en.n, nl,
+                                                [&](int k) {
+                                                    return k < nl
+                                                        ? nth_left(s0 + k, clo, chi) - lf
+                                                        : nth_right(lf - s0 + k - nl, clo, chi) - lf;
+                                                });
                         q = (uint8_t)(en.child - c0 + (gl ? 0 : 1));
                     } else {
                         a.fin[first + i] = p;
@@ -1034,7 +1044,6 @@ __global__ __launch_bounds__(kWave) void k_subtree_wave(const float4 *__restrict
                                                         const int32_t *__restrict__ wave_off) {
     __shared__ float4 P[kWaveTask];
     __shared__ int32_t Q[kWaveTask];
-    __shared__ int8_t ST[kWaveTask];  // each split node's points listed by side
     __shared__ float s_lo[3][kWaveNodes], s_hi[3][kWaveNodes];
     __shared__ float s_com[3][kWaveNodes], s_mass[kWaveNodes];
     __shared__ int16_t s_lf[kWaveNodes], s_cnt[kWaveNodes], s_left[kWaveNodes];
@@ -1096,15 +1105,16 @@ __global__ __launch_bounds__(kWave) void k_subtree_wave(const float4 *__restrict
             const uint64_t mH = ballot(split && lane == lf);
             const int C = M + 2 * __popcll(mH & lanes_below(lf));
             int dst = lane, nn = nid;
-            // the in-place partition's order: the node's points listed by
-            // side (the stable order), then each point's closed-form slot
+            // the in-place partition's order (inplace_dest): the node's k-th
+            // left (right) point is the k-th set bit of its left (right)
+            // lanes' mask
             const uint64_t below = mN & lanes_below(lane);
             const int rank = gl ? __popcll(mL & below) : __popcll(~mL & below);
-            if (split) ST[gl ? lf + rank : lf + nl + rank] = (int8_t)(lane - lf);
-            __syncthreads();
             if (split) {
-                dst = lf + inplace_dest(lane - lf, gl, rank, c, nl,
-                                        [&](int k) { return (int32_t)ST[lf + k]; });
+                const uint64_t mLn = mL & mN, mRn = ~mL & mN;
+                dst = lf + inplace_dest(lane - lf, gl, rank, c, nl, [&](int k) {
+                    return (k < nl ? select_bit(mLn, k) : select_bit(mRn, k - nl)) - lf;
+                });
                 nn = gl ? C : C + 1;
                 if (lane == lf) {
                     s_lf[C] = (int16_t)lf;

@@ -269,17 +269,35 @@ __host__ __device__ __forceinline__ float split_lim(const float *lo, const float
 //   left,  p >= ns: with c = ns - 1 - rank left points after it, it fills the
 //                   slot of the (c + 1)-th right point, st(ns + c);
 //   right: with c = rank right points before it, r_c = n for c = 0, else
-//          the position of the c-th left point from the right, st(ns - c)
-//          (none, -1, when c > ns); it goes to r_c - 1 when p < r_c, else
-//          to p - 1.
-// Checked against the loop on random class patterns: every destination is
-// the loop's and each slot is used once.
+//          the position of the c-th left point from the right, st(ns - c);
+//          it goes to r_c - 1 when p < r_c, else to p - 1.  There are
+//          p - c left points before p, so ns - p + c after it, and p < r_c
+//          exactly when p <= ns: a right point beyond ns needs no lookup.
+// Only the points on the wrong side of ns look anything up.  Checked
+// against the loop on random class patterns: every destination is the
+// loop's and each slot is used once.
 template <typename ST>
 __host__ __device__ __forceinline__ int32_t inplace_dest(int32_t p, bool left, int32_t rank, int32_t n,
                                                          int32_t ns, ST st) {
     if (left) return p < ns ? p : st(ns + (ns - 1 - rank));
-    const int32_t rc = rank == 0 ? n : (rank <= ns ? st(ns - rank) : -1);
-    return p < rc ? rc - 1 : p - 1;
+    if (p > ns) return p - 1;
+    return (rank == 0 ? n : st(ns - rank)) - 1;
+}
+
+// Position of the k-th (0-based) set bit of m; k < popcount(m).  A binary
+// search over halves, quarters, ... of the word with popcounts.
+__host__ __device__ __forceinline__ int select_bit(uint64_t m, int k) {
+    int pos = 0;
+#pragma unroll
+    for (int w = 32; w >= 1; w >>= 1) {
+        const int c = __builtin_popcountll(m & ((1ull << w) - 1ull));
+        if (k >= c) {
+            k -= c;
+            m >>= w;
+            pos += w;
+        }
+    }
+    return pos;
 }
 
 // Order-preserving map of a float onto an int, for atomic min/max.

@@ -199,7 +199,8 @@ struct Tables {
     int64_t nrec = 0, nhalos = 0;
     std::vector<int64_t> first, count;
     std::vector<RootTile> root_tiles;
-    std::vector<Group> groups;          // walk wavefronts
+    std::vector<Group> groups;          // walk wavefronts, 64 sink points each (k_walk)
+    std::vector<Group> windows;         // walk wavefronts, kWinPoints positions each (k_walk_lists)
     std::vector<int2> direct_blocks;    // direct-kernel workgroups
     std::vector<ScanTile> scan_tiles;
     std::vector<int32_t> call_tile0, call_ntiles;
@@ -248,13 +249,17 @@ static int make_tables(Tables &T, const int64_t *first, const int64_t *count, in
             for (int64_t off = 0; off < count[h]; off += kGroup)
                 T.groups.push_back(Group{(int32_t)h, (int32_t)(first[h] + off),
                                          (int32_t)(first[h] + count[h]), 0});
+            for (int64_t off = 0; off < count[h]; off += kWinPoints)
+                T.windows.push_back(Group{(int32_t)h, (int32_t)(first[h] + off),
+                                          (int32_t)(first[h] + std::min(count[h], off + kWinPoints)),
+                                          0});
         }
     }
     // the walk's wavefronts of the largest calls first: they walk the
     // deepest trees, and starting them early shortens the launch's tail
-    std::stable_sort(T.groups.begin(), T.groups.end(), [&](const Group &x, const Group &y) {
-        return count[x.root] > count[y.root];
-    });
+    auto larger = [&](const Group &x, const Group &y) { return count[x.root] > count[y.root]; };
+    std::stable_sort(T.groups.begin(), T.groups.end(), larger);
+    std::stable_sort(T.windows.begin(), T.windows.end(), larger);
     if (want_unbind) {
         T.call_tile0.resize(nh);
         T.call_ntiles.resize(nh);
@@ -300,10 +305,11 @@ struct Engine {
     hipEvent_t ev[8] = {};
     int32_t big_threshold = kSmallMax;
     int32_t wave_max = kWaveTask;
+    bool point_walk = false;  // HALOPOT_WALK=points: the walk with lanes = sink points
     int sub_grid[kSubClasses] = {}, wave_grid = 0;  // resident workgroups of the subtree kernels
 
     // batch tables on the device
-    DevBuf hfirst, hcount, root_tiles, groups, direct_blocks, scan_tiles, call_tile0,
+    DevBuf hfirst, hcount, root_tiles, groups, windows, direct_blocks, scan_tiles, call_tile0,
         call_ntiles;
     // build
     DevBuf box, bad, pot, badlist, A, B, F, leaf_of, sides, nodes, parent, fat, small0, group0, groupc, wave_off, wave_hsum, big, tiles, parts,
@@ -357,13 +363,14 @@ struct Engine {
         HP_TRY(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_subtree_wave, kWave, 0));
         wave_grid = std::max(1, per_cu) * prop.multiProcessorCount;
         if (const char *s = getenv("HALOPOT_BIG_THRESHOLD")) big_threshold = atoi(s);
+        if (const char *s = getenv("HALOPOT_WALK")) point_walk = std::strcmp(s, "points") == 0;
         if (big_threshold > kSmallMax) big_threshold = kSmallMax;
         if (big_threshold < kLeafPoints) big_threshold = kLeafPoints;
         return HP_OK;
     }
 
     ~Engine() {
-        DevBuf *all[] = {&hfirst, &hcount, &root_tiles, &groups, &direct_blocks,
+        DevBuf *all[] = {&hfirst, &hcount, &root_tiles, &groups, &windows, &direct_blocks,
                          &scan_tiles, &call_tile0, &call_ntiles, &box, &bad, &pot, &badlist, &A, &B, &F,
                          &leaf_of, &sides, &nodes, &parent, &fat, &small0, &group0, &groupc, &wave_off, &wave_hsum, &big, &tiles,
                          &parts, &splitres, &dparts, &counters, &work, &depth_buf, &pcnt, &pscan, &pcall, &psums, &shape_st, &shape_part, &shape_cd, &scale_calls, &r_pe, &pe_cnt, &fit_list, &fit_n, &gsegs, &gtiles, &gcen, &keys0, &keys1, &vals0,
@@ -455,6 +462,7 @@ struct Engine {
         if ((r = upload(hcount, T.count))) return r;
         if ((r = upload(root_tiles, T.root_tiles))) return r;
         if ((r = upload(groups, T.groups))) return r;
+        if ((r = upload(windows, T.windows))) return r;
         if ((r = upload(direct_blocks, T.direct_blocks))) return r;
         if ((r = upload(scan_tiles, T.scan_tiles))) return r;
         if ((r = upload(call_tile0, T.call_tile0))) return r;
@@ -663,12 +671,24 @@ struct Engine {
             a.po = d_po;
             a.err = counters.as<int32_t>() + kErrFlags;
             a.work = work.as<unsigned long long>();
-            const int nb = blocks((int64_t)T.groups.size() * kWave, 64);
             const bool cnt = o.count_work != 0;
+            int nb;
+            if (point_walk) {  // the walk with lanes = sink points
+                nb = blocks((int64_t)T.groups.size() * kWave, 64);
+            } else {           // lanes = sink leaves, lists evaluated per point
+                a.groups = windows.as<Group>();
+                a.ngroups = (int32_t)T.windows.size();
+                nb = (int)T.windows.size();
+            }
 #define HP_WALK(ACC)                                                                   \
     do {                                                                               \
-        if (cnt) k_walk<ACC, true><<<nb, 64, 0, s>>>(a);                              \
-        else k_walk<ACC, false><<<nb, 64, 0, s>>>(a);                                 \
+        if (point_walk) {                                                              \
+            if (cnt) k_walk<ACC, true><<<nb, 64, 0, s>>>(a);                          \
+            else k_walk<ACC, false><<<nb, 64, 0, s>>>(a);                             \
+        } else {                                                                       \
+            if (cnt) k_walk_lists<ACC, true><<<nb, 64, 0, s>>>(a);                    \
+            else k_walk_lists<ACC, false><<<nb, 64, 0, s>>>(a);                       \
+        }                                                                              \
     } while (0)
             if (o.accumulate == HP_ACC_NAIVE) HP_WALK(HP_ACC_NAIVE);
             else if (o.accumulate == HP_ACC_F64) HP_WALK(HP_ACC_F64);

@@ -38,7 +38,7 @@ def test_plan_reads_records_after_a_null_stream_copy():
             synchronize()
             got = d_dst.download(np.empty_like(po))
             np.testing.assert_array_equal(got["pe"], want["pe"])
-            np.testing.assert_array_equal(got["pos"], po["pos"])
+            np.testing.assert_array_equal(got["pos"], want["pos"])  # the same permutation
     finally:
         plan.close()
         d_src.free()
