@@ -8,6 +8,7 @@
 #pragma once
 
 #include <hip/hip_runtime.h>
+#include <stddef.h>
 #include <stdint.h>
 
 #include "../../include/halopot.h"
@@ -29,11 +30,13 @@ constexpr int kMaxLevels = 4096;  // depth_end entries kept per build
 //   [first, first + count) is the node's range in tree order (batch-global);
 //   left is the first child (the right child is left + 1) or -1 for a leaf;
 //   skip is the node that follows the subtree in depth-first order (-1: end).
+// The centre of mass and the mass come first: the list walk's evaluation
+// reads them as one aligned float4 per monopole.
 struct __attribute__((aligned(16))) Node {
-    float lo[3];
-    float hi[3];
     float com[3];
     float mass;
+    float lo[3];
+    float hi[3];
     float dmin2;     // float(dmin) * float(dmin), src/potential.c:128
     int32_t first;
     int32_t count;
@@ -41,7 +44,8 @@ struct __attribute__((aligned(16))) Node {
     int32_t skip;
     float dmin;
 };
-static_assert(sizeof(Node) == 64, "node layout");
+static_assert(sizeof(Node) == 64 && offsetof(Node, com) == 0 && offsetof(Node, mass) == 12,
+              "node layout");
 
 // A node waiting to be split at the current level.
 struct Task {

@@ -199,8 +199,7 @@ struct Tables {
     int64_t nrec = 0, nhalos = 0;
     std::vector<int64_t> first, count;
     std::vector<RootTile> root_tiles;
-    std::vector<Group> groups;          // walk wavefronts, 64 sink points each (k_walk)
-    std::vector<Group> windows;         // walk wavefronts, kWinPoints positions each (k_walk_lists)
+    std::vector<Group> groups;          // walk wavefronts, 64 sink points each
     std::vector<int2> direct_blocks;    // direct-kernel workgroups
     std::vector<ScanTile> scan_tiles;
     std::vector<int32_t> call_tile0, call_ntiles;
@@ -215,6 +214,34 @@ struct Tables {
 constexpr int64_t kMaxCalls = 1 << 22;
 constexpr int64_t kMaxRecords = (INT32_MAX - kMaxCalls - 1024) / 5;
 static_assert(5 * kMaxRecords + kMaxCalls + 1024 <= INT32_MAX, "node ids fit 32 bits");
+
+// Workgroups of a launch are dispatched to the XCDs in turn (workgroup b to
+// XCD b % xcds), and each XCD has its own L2.  The walk reads a call's tree
+// over and over, so the wavefronts of one call should run on one XCD: the
+// calls, in the list's order, go to the XCD with the fewest wavefronts so
+// far, and the list is rewritten so that entry b is the next of XCD b %
+// xcds's wavefronts; XCDs that run out get empty entries (root -1).  A call
+// with more than 1/(8 xcds) of all the wavefronts is cut into pieces of that
+// size, placed one by one, so that one large call still runs on every XCD.
+static void place_on_xcds(std::vector<Group> &g, int xcds) {
+    if (xcds <= 1 || g.empty()) return;
+    const size_t piece = std::max<size_t>(1, g.size() / (8 * (size_t)xcds));
+    std::vector<std::vector<Group>> per(xcds);
+    for (size_t i = 0; i < g.size();) {
+        size_t j = i;
+        while (j < g.size() && g[j].root == g[i].root && j - i < piece) j++;
+        int best = 0;
+        for (int x = 1; x < xcds; x++)
+            if (per[x].size() < per[best].size()) best = x;
+        per[best].insert(per[best].end(), g.begin() + i, g.begin() + j);
+        i = j;
+    }
+    size_t rows = 0;
+    for (auto &v : per) rows = std::max(rows, v.size());
+    g.assign(rows * xcds, Group{-1, 0, 0, 0});
+    for (int x = 0; x < xcds; x++)
+        for (size_t k = 0; k < per[x].size(); k++) g[k * xcds + x] = per[x][k];
+}
 
 static int make_tables(Tables &T, const int64_t *first, const int64_t *count, int64_t nh,
                        int method, bool want_unbind, int32_t direct_max = kLeafPoints) {
@@ -249,17 +276,12 @@ static int make_tables(Tables &T, const int64_t *first, const int64_t *count, in
             for (int64_t off = 0; off < count[h]; off += kGroup)
                 T.groups.push_back(Group{(int32_t)h, (int32_t)(first[h] + off),
                                          (int32_t)(first[h] + count[h]), 0});
-            for (int64_t off = 0; off < count[h]; off += kWinPoints)
-                T.windows.push_back(Group{(int32_t)h, (int32_t)(first[h] + off),
-                                          (int32_t)(first[h] + std::min(count[h], off + kWinPoints)),
-                                          0});
         }
     }
     // the walk's wavefronts of the largest calls first: they walk the
     // deepest trees, and starting them early shortens the launch's tail
     auto larger = [&](const Group &x, const Group &y) { return count[x.root] > count[y.root]; };
     std::stable_sort(T.groups.begin(), T.groups.end(), larger);
-    std::stable_sort(T.windows.begin(), T.windows.end(), larger);
     if (want_unbind) {
         T.call_tile0.resize(nh);
         T.call_ntiles.resize(nh);
@@ -305,11 +327,12 @@ struct Engine {
     hipEvent_t ev[8] = {};
     int32_t big_threshold = kSmallMax;
     int32_t wave_max = kWaveTask;
-    bool point_walk = false;  // HALOPOT_WALK=points: the walk with lanes = sink points
+    int xcds = 1;             // XCDs of the device (each with its own L2)
+    int32_t walk_groups = 0;  // entries of the loaded walk table (XCD layout)
     int sub_grid[kSubClasses] = {}, wave_grid = 0;  // resident workgroups of the subtree kernels
 
     // batch tables on the device
-    DevBuf hfirst, hcount, root_tiles, groups, windows, direct_blocks, scan_tiles, call_tile0,
+    DevBuf hfirst, hcount, root_tiles, groups, direct_blocks, scan_tiles, call_tile0,
         call_ntiles;
     // build
     DevBuf box, bad, pot, badlist, A, B, F, leaf_of, sides, nodes, parent, fat, small0, group0, groupc, wave_off, wave_hsum, big, tiles, parts,
@@ -347,6 +370,12 @@ struct Engine {
         // stream (a device-to-device hipMemcpy, for one, can return before
         // the copy is done), so the records are complete when read.
         HP_TRY(hipStreamCreateWithFlags(&stream, hipStreamDefault));
+        if (hipDeviceGetAttribute(&xcds, hipDeviceAttributeNumberOfXccs, device) != hipSuccess ||
+            xcds < 1) {
+            (void)hipGetLastError();
+            xcds = 1;
+        }
+        if (const char *s = getenv("HALOPOT_XCDS")) xcds = std::max(1, atoi(s));
         HP_TRY(hipHostMalloc((void **)&h_counters, sizeof(int32_t) * kNumCounters));
         HP_TRY(hipHostMalloc((void **)&h_work, sizeof(unsigned long long) * 4));
         for (auto &e : ev) HP_TRY(hipEventCreate(&e));
@@ -363,14 +392,13 @@ struct Engine {
         HP_TRY(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_subtree_wave, kWave, 0));
         wave_grid = std::max(1, per_cu) * prop.multiProcessorCount;
         if (const char *s = getenv("HALOPOT_BIG_THRESHOLD")) big_threshold = atoi(s);
-        if (const char *s = getenv("HALOPOT_WALK")) point_walk = std::strcmp(s, "points") == 0;
         if (big_threshold > kSmallMax) big_threshold = kSmallMax;
         if (big_threshold < kLeafPoints) big_threshold = kLeafPoints;
         return HP_OK;
     }
 
     ~Engine() {
-        DevBuf *all[] = {&hfirst, &hcount, &root_tiles, &groups, &windows, &direct_blocks,
+        DevBuf *all[] = {&hfirst, &hcount, &root_tiles, &groups, &direct_blocks,
                          &scan_tiles, &call_tile0, &call_ntiles, &box, &bad, &pot, &badlist, &A, &B, &F,
                          &leaf_of, &sides, &nodes, &parent, &fat, &small0, &group0, &groupc, &wave_off, &wave_hsum, &big, &tiles,
                          &parts, &splitres, &dparts, &counters, &work, &depth_buf, &pcnt, &pscan, &pcall, &psums, &shape_st, &shape_part, &shape_cd, &scale_calls, &r_pe, &pe_cnt, &fit_list, &fit_n, &gsegs, &gtiles, &gcen, &keys0, &keys1, &vals0,
@@ -461,8 +489,10 @@ struct Engine {
         if ((r = upload(hfirst, T.first))) return r;
         if ((r = upload(hcount, T.count))) return r;
         if ((r = upload(root_tiles, T.root_tiles))) return r;
-        if ((r = upload(groups, T.groups))) return r;
-        if ((r = upload(windows, T.windows))) return r;
+        std::vector<Group> g = T.groups;
+        place_on_xcds(g, xcds);
+        walk_groups = (int32_t)g.size();
+        if ((r = upload(groups, g))) return r;
         if ((r = upload(direct_blocks, T.direct_blocks))) return r;
         if ((r = upload(scan_tiles, T.scan_tiles))) return r;
         if ((r = upload(call_tile0, T.call_tile0))) return r;
@@ -660,7 +690,7 @@ struct Engine {
             a.fin = F.as<float4>();
             a.leaf_of = leaf_of.as<int32_t>();
             a.groups = groups.as<Group>();
-            a.ngroups = (int32_t)T.groups.size();
+            a.ngroups = walk_groups;
             a.counters = counters.as<int32_t>();
             a.nrec = (int32_t)T.nrec;
             a.cap_nodes = (int32_t)cap_nodes;
@@ -671,24 +701,12 @@ struct Engine {
             a.po = d_po;
             a.err = counters.as<int32_t>() + kErrFlags;
             a.work = work.as<unsigned long long>();
+            const int nb = blocks((int64_t)walk_groups * kWave, 64);
             const bool cnt = o.count_work != 0;
-            int nb;
-            if (point_walk) {  // the walk with lanes = sink points
-                nb = blocks((int64_t)T.groups.size() * kWave, 64);
-            } else {           // lanes = sink leaves, lists evaluated per point
-                a.groups = windows.as<Group>();
-                a.ngroups = (int32_t)T.windows.size();
-                nb = (int)T.windows.size();
-            }
 #define HP_WALK(ACC)                                                                   \
     do {                                                                               \
-        if (point_walk) {                                                              \
-            if (cnt) k_walk<ACC, true><<<nb, 64, 0, s>>>(a);                          \
-            else k_walk<ACC, false><<<nb, 64, 0, s>>>(a);                             \
-        } else {                                                                       \
-            if (cnt) k_walk_lists<ACC, true><<<nb, 64, 0, s>>>(a);                    \
-            else k_walk_lists<ACC, false><<<nb, 64, 0, s>>>(a);                       \
-        }                                                                              \
+        if (cnt) k_walk<ACC, true><<<nb, 64, 0, s>>>(a);                              \
+        else k_walk<ACC, false><<<nb, 64, 0, s>>>(a);                                 \
     } while (0)
             if (o.accumulate == HP_ACC_NAIVE) HP_WALK(HP_ACC_NAIVE);
             else if (o.accumulate == HP_ACC_F64) HP_WALK(HP_ACC_F64);

@@ -125,6 +125,7 @@ __global__ __launch_bounds__(256) void k_walk(WalkArgs a) {
     if (gid >= a.ngroups) return;
     const int lane = lane_id();
     const Group g = load_const(a.groups + gid);
+    if (g.root < 0) return;  // an empty entry of the XCD layout
     const int32_t nnodes = min(load_const(a.counters + kNodeCount), a.cap_nodes);
     const int32_t root = uniform(g.root);
     const Node rn = load_const(a.nodes + root);
@@ -280,243 +281,6 @@ __global__ __launch_bounds__(256) void k_walk(WalkArgs a) {
     }
 }
 
-// ------------------------------------------------------------ list walk
-//
-// The same decisions as k_walk, taken once per sink leaf instead of once
-// per sink point.  One wavefront per window of kWinPoints consecutive tree
-// positions of one call.  Its lanes are the call's sink leaves that start
-// in the window, up to 64 at a time (a round).  The wavefront walks the
-// union of their traversals in depth-first order (stackless, resume
-// pointers, as k_walk), every lane taking its own leaf's decisions
-// (src/potential.c:126-168), and appends:
-//   * each node its leaf accepts (a monopole, P2M) to its P2M list;
-//   * each leaf whose pairs it sums (not accepted, or its own leaf of at
-//     most 20 points) to its P2P list.
-// The lists live in LDS.  They are evaluated with lanes = sink points: each
-// point runs its leaf's P2M list, then its P2P list (a leaf's sources
-// loaded per lane, its own position skipped).  A round's lists are
-// evaluated when one of them fills (the partial sums go to pe) and when its
-// walk ends.  A point's terms are summed in list order; the reference sums
-// them in visit order, which differs only in rounding.
-constexpr int kWinPoints = 384;  // tree positions per walk window
-constexpr int kListM = 32;       // P2M entries per sink leaf between evaluations
-constexpr int kListP = 8;        // P2P entries per sink leaf between evaluations
-
-template <int ACC, bool COUNT>
-__global__ __launch_bounds__(kWave) void k_walk_lists(WalkArgs a) {
-    __shared__ int32_t LM[kListM][kWave];
-    __shared__ int32_t LP[kListP][kWave];
-    __shared__ int32_t s_start[kWave], s_cntM[kWave], s_cntP[kWave];
-    const int lane = lane_id();
-    const Group g = load_const(a.groups + blockIdx.x);
-    const int32_t nnodes = min(load_const(a.counters + kNodeCount), a.cap_nodes);
-    const int32_t root = uniform(g.root);
-    const Node rn = load_const(a.nodes + root);
-    const int32_t fend = rn.first + rn.count;  // tree positions past fend hold non-finite records
-    const float pm = a.pm, inv_fr = a.inv_fr;
-    const double pm_d = a.pm_d, fr_d = a.fr_d;
-    const float4 *__restrict__ fin = a.fin;
-    const Node *__restrict__ nodes = a.nodes;
-    unsigned long long n_vis = 0, n_p2p = 0, n_p2m = 0;
-
-    // non-finite records of the window: pe = 0, they are left out of the tree
-    for (int32_t p = max(g.p0, fend) + lane; p < g.pend; p += kWave) {
-        const int32_t idx = __float_as_int(fin[p].w);
-        if ((uint32_t)idx < (uint32_t)a.nrec) a.po[idx].pe = 0.0;
-        else atomicOr(a.err, kErrBadTree);
-    }
-    const int32_t scan_end = min(g.pend, fend);
-    int32_t scan = g.p0;
-    while (scan < scan_end) {
-        // 1. the next round: up to 64 sink leaves starting at or after scan
-        int nl = 0;
-        while (nl < kWave && scan < scan_end) {
-            const int32_t p = scan + lane;
-            bool st = false;
-            if (p < scan_end) {
-                const int32_t lf = a.leaf_of[p];
-                st = p == rn.first || a.leaf_of[p - 1] != lf;
-            }
-            const uint64_t m = ballot(st);
-            const int cnt = __popcll(m), take = min(cnt, kWave - nl), r = rank_below(m);
-            if (st && r < take) s_start[nl + r] = p;
-            if (take < cnt) {
-                scan += select_bit(m, take);  // the first start left for the next round
-            } else {
-                scan = min(scan + kWave, scan_end);
-            }
-            nl += take;
-        }
-        __syncthreads();
-        if (nl == 0) break;  // the rest of the window lies inside a leaf started before it
-        const bool live = lane < nl;
-        int32_t lfirst = 0, lcount = 0, leaf = 0;
-        float lc0 = 0.0f, lc1 = 0.0f, lc2 = 0.0f;
-        bool bad = false;
-        if (live) {
-            const int32_t p = s_start[lane];
-            leaf = a.leaf_of[p];
-            bad = (uint32_t)leaf >= (uint32_t)nnodes;
-            const Node *ln = nodes + (bad ? 0 : leaf);
-            lfirst = ln->first;
-            lcount = ln->count;
-            lc0 = ln->com[0];
-            lc1 = ln->com[1];
-            lc2 = ln->com[2];
-            bad = bad || lfirst != p;
-        }
-        if (ballot(bad)) {
-            if (lane == 0) atomicOr(a.err, kErrBadTree);
-            return;
-        }
-        const int32_t P0 = read_lane(lfirst, 0);
-        const int32_t P1 = read_lane(lfirst + lcount, nl - 1);  // the round's points: [P0, P1)
-        s_start[lane] = live ? lfirst : 0x7fffffff;
-        int cntM = 0, cntP = 0;
-        bool first_eval = true;
-
-        // evaluation of the lists gathered so far, lanes = sink points
-        auto evaluate = [&]() {
-            s_cntM[lane] = cntM;
-            s_cntP[lane] = cntP;
-            __syncthreads();
-            for (int32_t q0 = P0; q0 < P1; q0 += kWave) {
-                const int32_t p = q0 + lane;
-                const bool v = p < P1;
-                int li = 0;  // the point's leaf: the last round leaf starting at or before p
-                for (int step = 32; step > 0; step >>= 1)
-                    if (li + step < nl && s_start[li + step] <= p) li += step;
-                const float4 me = v ? fin[p] : make_float4(0.0f, 0.0f, 0.0f, 0.0f);
-                Acc<ACC> acc;
-                int since = 0;
-                const int nm = v ? s_cntM[li] : 0, np = v ? s_cntP[li] : 0;
-                // monopoles, unsoftened (src/potential.c:162-166)
-                for (int k = 0; k < nm; k++) {
-                    const Node *q = nodes + LM[k][li];
-                    const float cx = q->com[0], cy = q->com[1], cz = q->com[2], m = q->mass;
-                    if constexpr (ACC == HP_ACC_F64) {
-                        const double ex = (double)me.x - (double)cx;
-                        const double ey = (double)me.y - (double)cy;
-                        const double ez = (double)me.z - (double)cz;
-                        acc.add((double)m / sqrt(ex * ex + ey * ey + ez * ez));
-                    } else {
-                        const float ex = me.x - cx, ey = me.y - cy, ez = me.z - cz;
-                        acc.add(m * rsq(fmaf(ez, ez, fmaf(ey, ey, ex * ex))));
-                        if constexpr (ACC == HP_ACC_COMPENSATED) {
-                            if (++since == 16) {
-                                acc.fold();
-                                since = 0;
-                            }
-                        }
-                    }
-                }
-                if constexpr (ACC == HP_ACC_COMPENSATED) acc.fold();
-                // particle pairs, softened (src/potential.c:38-55); the own
-                // leaf's pairs skip the point itself
-                for (int k = 0; k < np; k++) {
-                    const Node *q = nodes + LP[k][li];
-                    const int32_t f0 = q->first, f1 = f0 + q->count;
-                    if constexpr (COUNT) n_p2p += (unsigned long long)(f1 - f0 - (p >= f0 && p < f1 ? 1 : 0));
-                    for (int32_t j0 = f0; j0 < f1; j0 += 16) {
-                        const int32_t j1 = min(j0 + 16, f1);
-                        if constexpr (ACC == HP_ACC_F64) {
-                            double sum = 0.0;
-                            for (int32_t j = j0; j < j1; j++) {
-                                const float4 s4 = fin[j];
-                                const double ex = (double)(me.x - s4.x), ey = (double)(me.y - s4.y),
-                                             ez = (double)(me.z - s4.z);
-                                double r = sqrt((ex * ex + ey * ey) + ez * ez);
-                                r = r < fr_d ? fr_d : r;
-                                sum += j != p ? pm_d * (1.0 / r) : 0.0;
-                            }
-                            acc.add_block(sum);
-                        } else {
-                            float sum = 0.0f;
-                            for (int32_t j = j0; j < j1; j++) {
-                                const float4 s4 = fin[j];
-                                const float ex = me.x - s4.x, ey = me.y - s4.y, ez = me.z - s4.z;
-                                const float t = fminf(rsq(fmaf(ez, ez, fmaf(ey, ey, ex * ex))), inv_fr);
-                                sum += j != p ? t : 0.0f;
-                            }
-                            acc.add_block(pm * sum);
-                        }
-                    }
-                }
-                if constexpr (COUNT) n_p2m += (unsigned long long)nm;
-                if (v) {
-                    const int32_t idx = __float_as_int(me.w);
-                    if ((uint32_t)idx < (uint32_t)a.nrec) {
-                        const double val = acc.value();
-                        a.po[idx].pe = first_eval ? val : a.po[idx].pe + val;
-                    } else {
-                        atomicOr(a.err, kErrBadTree);
-                    }
-                }
-            }
-            __syncthreads();
-            cntM = cntP = 0;
-            first_eval = false;
-        };
-
-        // 2. the walk, decisions per sink leaf
-        int32_t resume = live ? root : -2;
-        int32_t c = root;
-        int32_t guard = 0;
-        const int32_t cap = 2 * rn.count + 4;
-        while (c >= 0) {
-            if (++guard > cap) {
-                if (lane == 0) atomicOr(a.err, kErrWalkCap);
-                break;
-            }
-            c = uniform(c);
-            if ((uint32_t)c >= (uint32_t)nnodes) {
-                if (lane == 0) atomicOr(a.err, kErrBadTree);
-                break;
-            }
-            const Node nd = load_const(nodes + c);
-            if constexpr (COUNT) n_vis += 1;
-            const bool act = resume == c;
-            const bool inr = (uint32_t)(lfirst - nd.first) < (uint32_t)nd.count;
-            const bool cand = act & !inr;
-            // acceptance, src/potential.c:126-131
-            const float dx = lc0 - nd.com[0], dy = lc1 - nd.com[1], dz = lc2 - nd.com[2];
-            const float f = fmaf(dz, dz, fmaf(dy, dy, dx * dx));
-            bool acc_ok = f > nd.dmin2;
-            const bool amb = cand & (fabsf(f - nd.dmin2) <= nd.dmin2 * 0x1p-20f);
-            if (__builtin_expect(ballot(amb) != 0, 0)) {
-                const bool ex = accept_exact(lc0, lc1, lc2, nd.com, nd.dmin2);
-                acc_ok = amb ? ex : acc_ok;
-            }
-            const bool p2m = cand & acc_ok;
-            if (p2m) LM[cntM++][lane] = c;
-            if (nd.left >= 0) {
-                // an internal node: opened by a lane whose leaf lies inside
-                // it or that does not accept it
-                const bool open = act & (inr | !acc_ok);
-                if (act) resume = open ? nd.left : nd.skip;
-                c = ballot(open) != 0 ? nd.left : nd.skip;
-            } else {
-                // a leaf: pair sums when not accepted, or the own leaf of at
-                // most 20 points (src/potential.c:144-146)
-                const bool p2p = act & ((inr & (nd.count <= kDegenerate)) | (!inr & !acc_ok));
-                if (p2p) LP[cntP++][lane] = c;
-                if (act) resume = nd.skip;
-                c = nd.skip;
-            }
-            if (ballot(cntM == kListM || cntP == kListP)) evaluate();
-        }
-        evaluate();
-    }
-    if constexpr (COUNT) {
-        const unsigned long long vv = n_vis, pp = wave_sum(n_p2p), mm = wave_sum(n_p2m);
-        if (lane == 0) {
-            atomicAdd(&a.work[0], vv);
-            atomicAdd(&a.work[1], pp);
-            atomicAdd(&a.work[2], mm);
-        }
-    }
-}
-
 // Exact pairwise sum over a halo's finite points, source tiles staged in LDS.
 // One workgroup of 256 sinks per block entry (halo root, first sink).  For a
 // halo of at most 10 finite points this is the reference's answer: its tree
@@ -605,12 +369,6 @@ template __global__ void k_walk<HP_ACC_NAIVE, false>(WalkArgs);
 template __global__ void k_walk<HP_ACC_NAIVE, true>(WalkArgs);
 template __global__ void k_walk<HP_ACC_F64, false>(WalkArgs);
 template __global__ void k_walk<HP_ACC_F64, true>(WalkArgs);
-template __global__ void k_walk_lists<HP_ACC_COMPENSATED, false>(WalkArgs);
-template __global__ void k_walk_lists<HP_ACC_COMPENSATED, true>(WalkArgs);
-template __global__ void k_walk_lists<HP_ACC_NAIVE, false>(WalkArgs);
-template __global__ void k_walk_lists<HP_ACC_NAIVE, true>(WalkArgs);
-template __global__ void k_walk_lists<HP_ACC_F64, false>(WalkArgs);
-template __global__ void k_walk_lists<HP_ACC_F64, true>(WalkArgs);
 template __global__ void k_direct<HP_ACC_COMPENSATED>(DirectArgs);
 template __global__ void k_direct<HP_ACC_NAIVE>(DirectArgs);
 template __global__ void k_direct<HP_ACC_F64>(DirectArgs);
