@@ -58,7 +58,22 @@ void compute_potential(struct potential *po, int64_t num_po);
 void compute_kinetic_energy(struct potential *po, int64_t num_po,
                             float *vel_cen, float *pos_cen, float scale_now);
 
-/* Configuration globals read at call time (src/config.template.h:7,16,20,
+/* No counterpart in the reference: compute_potential for callers that cannot
+ * allocate device memory (code under HIP graph capture, tight serving loops,
+ * harnesses that forbid hipMalloc).  The same tree, decisions, values and
+ * permutation of the records as compute_potential, on device-resident
+ * records d_po[0..num_po), at most 32768 records (HP_EINVAL above).  It
+ * calls no hipMalloc, hipFree, hipHostMalloc and creates no stream or event:
+ * its scratch is one static device array of 20 MiB per GPU, held until the
+ * process ends, whose address is looked up on the first call on each device.
+ * Every call on a GPU shares that GPU's array, so calls that overlap on one
+ * GPU (from several host threads or streams) race: serialise them.  Runs on
+ * `stream` (a hipStream_t, or NULL for the null stream) and returns when the
+ * potential is done (it reads the tree's level counters back to the host
+ * between levels).  Returns 0 or an HP_E* code. */
+int hp_compute_potential_noalloc(struct potential *d_po, int64_t num_po, void *stream);
+
+/* Configuration globals read at call time (src/config.template.h:7,16,20,/* Configuration globals read at call time (src/config.template.h:7,16,20,
  * 22-25 through src/config_vars.h).  libhalopot carries weak definitions with
  * the reference's defaults, so the program that defines the real ones (the
  * halo finder) overrides them and stand-alone callers can set them here.

@@ -150,8 +150,10 @@ static double mass_definition(const char *md) {
 struct DevBuf {
     void *p = nullptr;
     size_t bytes = 0;
+    bool carved = false;  // a slice of a region the buffer does not own (never freed)
     int ensure(size_t n) {
         if (n <= bytes) return HP_OK;
+        if (carved) return set_error(HP_EINVAL, "carved buffer of %zu bytes asked for %zu", bytes, n);
         if (p) (void)hipFree(p);
         p = nullptr;
         bytes = 0;
@@ -164,9 +166,10 @@ struct DevBuf {
         return HP_OK;
     }
     void release() {
-        if (p) (void)hipFree(p);
+        if (p && !carved) (void)hipFree(p);
         p = nullptr;
         bytes = 0;
+        carved = false;
     }
     template <typename T>
     T *as() const {
@@ -177,6 +180,14 @@ struct DevBuf {
 #define HP_ENSURE(buf, n)                       \
     do {                                        \
         int r_ = (buf).ensure(n);               \
+        if (r_) return r_;                      \
+    } while (0)
+
+// Inside the engine: from the engine's carve region when it has one, else
+// hipMalloc'd (DevBuf::ensure).
+#define HP_GRAB(buf, n)                         \
+    do {                                        \
+        int r_ = grab((buf), (n));              \
         if (r_) return r_;                      \
     } while (0)
 
@@ -397,17 +408,89 @@ struct Engine {
         return HP_OK;
     }
 
-    ~Engine() {
+    // The engine of hp_compute_potential_noalloc: every device buffer is a
+    // slice of a caller-given region (carve), the stream is the caller's, and
+    // the level counters are read into host memory of the engine itself, so
+    // that a run allocates nothing.
+    uint8_t *carve = nullptr;
+    size_t carve_cap = 0, carve_off = 0;
+    bool own_stream = true;
+    int32_t host_counters[kNumCounters] = {};
+    unsigned long long host_work[4] = {};
+
+    int grab(DevBuf &b, size_t n) {
+        if (!carve) return b.ensure(n);
+        if (b.carved && b.bytes >= n) return HP_OK;
+        const size_t a = (n + 255) & ~(size_t)255;
+        if (carve_off + a > carve_cap)
+            return set_error(HP_EINVAL, "scratch of %zu bytes is too small (%zu more needed)",
+                             carve_cap, carve_off + a - carve_cap);
+        b.p = carve + carve_off;
+        b.bytes = a;
+        b.carved = true;
+        carve_off += a;
+        return HP_OK;
+    }
+
+    template <class Fn>
+    void for_each_buf(Fn &&f) {
         DevBuf *all[] = {&hfirst, &hcount, &root_tiles, &groups, &direct_blocks,
                          &scan_tiles, &call_tile0, &call_ntiles, &box, &bad, &pot, &badlist, &A, &B, &F,
                          &leaf_of, &sides, &nodes, &parent, &fat, &small0, &group0, &groupc, &wave_off, &wave_hsum, &big, &tiles,
                          &parts, &splitres, &dparts, &counters, &work, &depth_buf, &pcnt, &pscan, &pcall, &psums, &shape_st, &shape_part, &shape_cd, &scale_calls, &r_pe, &pe_cnt, &fit_list, &fit_n, &gsegs, &gtiles, &gcen, &keys0, &keys1, &vals0,
                          &vals1, &hist, &hsum, &srt, &callinfo, &tsum, &tcand, &tile_i32, &strip_cnt, &stage,
                          &seg_wave_l, &seg_block_l, &seg_tiles_l, &seg_hist, &seg_hsum, &srt_rv};
-        if (stream) {
+        for (DevBuf *b : all) f(*b);
+    }
+
+    // Every buffer is carved again from the start of the region.
+    void reset_carve() {
+        carve_off = 0;
+        for_each_buf([](DevBuf &b) { b.release(); });
+    }
+
+    // Set-up of the engine of hp_compute_potential_noalloc on the current
+    // device: the checks and launch sizes of init(), no stream, no pinned
+    // memory, no events.
+    int init_static(void *region, size_t bytes) {
+        int n = 0;
+        if (hipGetDeviceCount(&n) != hipSuccess || n < 1) {
+            (void)hipGetLastError();
+            return set_error(HP_ENODEV, "no HIP device available (libhalopot has no CPU path)");
+        }
+        HP_TRY(hipGetDevice(&device));
+        hipDeviceProp_t prop;
+        HP_TRY(hipGetDeviceProperties(&prop, device));
+        if (std::strncmp(prop.gcnArchName, "gfx950", 6) != 0)
+            return set_error(HP_ENODEV, "device %d is %s; libhalopot is built for gfx950", device,
+                             prop.gcnArchName);
+        if (hipDeviceGetAttribute(&xcds, hipDeviceAttributeNumberOfXccs, device) != hipSuccess ||
+            xcds < 1) {
+            (void)hipGetLastError();
+            xcds = 1;
+        }
+        int per_cu = 0;
+        HP_TRY(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_subtree<256>, kSubThreads, 0));
+        sub_grid[0] = std::max(1, per_cu) * prop.multiProcessorCount;
+        HP_TRY(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_subtree<512>, kSubThreads, 0));
+        sub_grid[1] = std::max(1, per_cu) * prop.multiProcessorCount;
+        HP_TRY(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_subtree<1024>, kSubThreads, 0));
+        sub_grid[2] = std::max(1, per_cu) * prop.multiProcessorCount;
+        HP_TRY(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_subtree_wave, kWave, 0));
+        wave_grid = std::max(1, per_cu) * prop.multiProcessorCount;
+        carve = static_cast<uint8_t *>(region);
+        carve_cap = bytes;
+        own_stream = false;
+        h_counters = host_counters;
+        h_work = host_work;
+        return HP_OK;
+    }
+
+    ~Engine() {
+        if (stream && own_stream) {
             (void)hipStreamSynchronize(stream);
             drop_graphs();
-            for (DevBuf *b : all) b->release();
+            for_each_buf([](DevBuf &b) { b.release(); });
             sub.release();
             for (auto &e : ev)
                 if (e) (void)hipEventDestroy(e);
@@ -419,7 +502,7 @@ struct Engine {
 
     template <typename T>
     int upload(DevBuf &b, const std::vector<T> &v) {
-        HP_ENSURE(b, sizeof(T) * (v.size() ? v.size() : 1));
+        HP_GRAB(b, sizeof(T) * (v.size() ? v.size() : 1));
         if (!v.empty())
             HP_TRY(hipMemcpyAsync(b.p, v.data(), sizeof(T) * v.size(), hipMemcpyHostToDevice, stream));
         return HP_OK;
@@ -428,58 +511,58 @@ struct Engine {
     int reserve(const Tables &T, bool unbind) {
         const int64_t N = T.nrec, H = T.nhalos;
         HP_TRY(hipSetDevice(device));
-        HP_ENSURE(box, sizeof(int32_t) * 6 * (H + 1));
-        HP_ENSURE(bad, sizeof(int32_t) * (H + 1));
-        HP_ENSURE(pot, sizeof(int32_t) * (H + 1));
-        HP_ENSURE(badlist, sizeof(int32_t) * (H + 1));
-        HP_ENSURE(A, sizeof(float4) * (N + 1));
-        HP_ENSURE(B, sizeof(float4) * (N + 1));
-        HP_ENSURE(F, sizeof(float4) * (N + 1));
-        HP_ENSURE(leaf_of, sizeof(int32_t) * (N + 1));
-        HP_ENSURE(sides, sizeof(int32_t) * (N + 1));
+        HP_GRAB(box, sizeof(int32_t) * 6 * (H + 1));
+        HP_GRAB(bad, sizeof(int32_t) * (H + 1));
+        HP_GRAB(pot, sizeof(int32_t) * (H + 1));
+        HP_GRAB(badlist, sizeof(int32_t) * (H + 1));
+        HP_GRAB(A, sizeof(float4) * (N + 1));
+        HP_GRAB(B, sizeof(float4) * (N + 1));
+        HP_GRAB(F, sizeof(float4) * (N + 1));
+        HP_GRAB(leaf_of, sizeof(int32_t) * (N + 1));
+        HP_GRAB(sides, sizeof(int32_t) * (N + 1));
         // split levels use at most about 0.1N + H ids; a subtree of n points
         // (workgroup or wavefront) reserves 2n - 2
         cap_nodes = 5 * N + H + 1024;
-        HP_ENSURE(nodes, sizeof(Node) * cap_nodes);
-        HP_ENSURE(parent, sizeof(int32_t) * cap_nodes);
+        HP_GRAB(nodes, sizeof(Node) * cap_nodes);
+        HP_GRAB(parent, sizeof(int32_t) * cap_nodes);
         cap_fat = N / (kFatLeaf + 1) + 16;
-        HP_ENSURE(fat, sizeof(int32_t) * cap_fat);
+        HP_GRAB(fat, sizeof(int32_t) * cap_fat);
         cap_small = N / (kLeafPoints + 1) + H + 64;
-        HP_ENSURE(small0, sizeof(Task) * cap_small);
-        HP_ENSURE(wave_off, sizeof(int32_t) * (cap_small + 2));
-        HP_ENSURE(wave_hsum, sizeof(int32_t) * ((cap_small + 1) / kScanBlock + 2));
+        HP_GRAB(small0, sizeof(Task) * cap_small);
+        HP_GRAB(wave_off, sizeof(int32_t) * (cap_small + 2));
+        HP_GRAB(wave_hsum, sizeof(int32_t) * ((cap_small + 1) / kScanBlock + 2));
         cap_group = N / (kWaveMin + 1) + H + 64;  // tasks of more than wave_max >= kWaveMin points
-        HP_ENSURE(group0, sizeof(Task) * cap_group);
-        HP_ENSURE(groupc, sizeof(Task) * kSubClasses * cap_group);
+        HP_GRAB(group0, sizeof(Task) * cap_group);
+        HP_GRAB(groupc, sizeof(Task) * kSubClasses * cap_group);
         const int64_t bt = big_threshold > 0 ? big_threshold : 1;
         cap_big = 128 * (N / bt + 1) + H + 64;
         cap_tiles = 128 * (N / kTile + 2) + cap_big;
-        HP_ENSURE(big, sizeof(Task) * cap_big);
-        HP_ENSURE(splitres, sizeof(BigSplit) * cap_big);
-        HP_ENSURE(tiles, sizeof(int2) * cap_tiles);
-        HP_ENSURE(parts, sizeof(TilePart) * cap_tiles);
-        HP_ENSURE(dparts, sizeof(double2) * cap_tiles);
-        HP_ENSURE(counters, sizeof(int32_t) * kNumCounters);
-        HP_ENSURE(depth_buf, sizeof(int32_t) * kMaxLevels);
-        HP_ENSURE(work, sizeof(unsigned long long) * 4);
+        HP_GRAB(big, sizeof(Task) * cap_big);
+        HP_GRAB(splitres, sizeof(BigSplit) * cap_big);
+        HP_GRAB(tiles, sizeof(int2) * cap_tiles);
+        HP_GRAB(parts, sizeof(TilePart) * cap_tiles);
+        HP_GRAB(dparts, sizeof(double2) * cap_tiles);
+        HP_GRAB(counters, sizeof(int32_t) * kNumCounters);
+        HP_GRAB(depth_buf, sizeof(int32_t) * kMaxLevels);
+        HP_GRAB(work, sizeof(unsigned long long) * 4);
         if (unbind) {
-            HP_ENSURE(keys0, sizeof(uint64_t) * (N + 1));
-            HP_ENSURE(keys1, sizeof(uint64_t) * (N + 1));
-            HP_ENSURE(vals0, sizeof(int32_t) * (N + 1));
-            HP_ENSURE(vals1, sizeof(int32_t) * (N + 1));
+            HP_GRAB(keys0, sizeof(uint64_t) * (N + 1));
+            HP_GRAB(keys1, sizeof(uint64_t) * (N + 1));
+            HP_GRAB(vals0, sizeof(int32_t) * (N + 1));
+            HP_GRAB(vals1, sizeof(int32_t) * (N + 1));
             const int64_t nt = (N + kSortTile - 1) / kSortTile + 1;
-            HP_ENSURE(hist, sizeof(int32_t) * kRadix * nt);
-            HP_ENSURE(hsum, sizeof(int32_t) * (kRadix * nt / kScanBlock + 2));
-            HP_ENSURE(srt, sizeof(struct potential) * (N + 1));
-            HP_ENSURE(srt_rv, sizeof(float4) * (N + 1));
-            HP_ENSURE(callinfo, sizeof(CallInfo) * (H + 1));
-            HP_ENSURE(tsum, sizeof(Moments) * (T.scan_tiles.size() + 1));
-            HP_ENSURE(tcand, sizeof(Cand) * 2 * (T.scan_tiles.size() + 1));
-            HP_ENSURE(tile_i32, sizeof(int32_t) * (T.scan_tiles.size() + 1));
-            HP_ENSURE(strip_cnt, sizeof(int32_t) * (T.scan_tiles.size() + 1));
+            HP_GRAB(hist, sizeof(int32_t) * kRadix * nt);
+            HP_GRAB(hsum, sizeof(int32_t) * (kRadix * nt / kScanBlock + 2));
+            HP_GRAB(srt, sizeof(struct potential) * (N + 1));
+            HP_GRAB(srt_rv, sizeof(float4) * (N + 1));
+            HP_GRAB(callinfo, sizeof(CallInfo) * (H + 1));
+            HP_GRAB(tsum, sizeof(Moments) * (T.scan_tiles.size() + 1));
+            HP_GRAB(tcand, sizeof(Cand) * 2 * (T.scan_tiles.size() + 1));
+            HP_GRAB(tile_i32, sizeof(int32_t) * (T.scan_tiles.size() + 1));
+            HP_GRAB(strip_cnt, sizeof(int32_t) * (T.scan_tiles.size() + 1));
             const int64_t m = (int64_t)T.seg_tiles.size() * kSegRadix;
-            HP_ENSURE(seg_hist, sizeof(int32_t) * (m + 1));
-            HP_ENSURE(seg_hsum, sizeof(int32_t) * (m / kScanBlock + 2));
+            HP_GRAB(seg_hist, sizeof(int32_t) * (m + 1));
+            HP_GRAB(seg_hsum, sizeof(int32_t) * (m / kScanBlock + 2));
         }
         return HP_OK;
     }
@@ -816,7 +899,7 @@ struct Engine {
         if ((r = enqueue_walk(T, d_po, o))) return r;
         if (timing) HP_TRY(hipEventRecord(ev[3], s));
         if (permute && T.method != HP_METHOD_DIRECT && T.nrec > 0) {
-            if ((r = stage.ensure(sizeof(struct potential) * T.nrec))) return r;
+            if ((r = grab(stage, sizeof(struct potential) * T.nrec))) return r;
             k_tree_records<<<blocks(T.nrec, 256), 256, 0, s>>>(d_po, F.as<float4>(), T.nrec,
                                                                stage.as<struct potential>());
             HP_LAUNCHED();
@@ -1370,6 +1453,57 @@ static hp_options default_options() {
 
 }  // namespace hp
 
+namespace hp {
+
+// ----------------------------------------------------------- no allocation
+//
+// hp_compute_potential_noalloc works in one file-scope device array per GPU:
+// every buffer its run needs (points, tree, tasks, counters, the tree-order
+// staging of the records) is carved from it, sized for at most
+// kNoAllocMaxRecords records.  Host code never passes the array's name to a
+// kernel: its address on each device comes from hipGetSymbolAddress, taken
+// once per device with that device current, and kept.
+constexpr int64_t kNoAllocMaxRecords = 32768;
+constexpr size_t kNoAllocScratchBytes = (size_t)20 << 20;  // needs about 16 MiB at the maximum
+static __device__ __align__(256) uint8_t scratch[kNoAllocScratchBytes];
+
+constexpr int kNoAllocDevices = 64;
+static std::mutex g_noalloc_mu;
+static Engine *g_noalloc[kNoAllocDevices];  // per device; kept until the process ends
+
+// The engine of the current device, set up on the first call there.
+static Engine *noalloc_engine(int *err) {
+    int dev = -1;
+    if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= kNoAllocDevices) {
+        (void)hipGetLastError();
+        *err = set_error(HP_ENODEV, "hp_compute_potential_noalloc: no usable current device");
+        return nullptr;
+    }
+    std::lock_guard<std::mutex> lock(g_noalloc_mu);
+    if (!g_noalloc[dev]) {
+        void *ptr = nullptr;
+        const hipError_t e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+        if (e != hipSuccess || !ptr) {
+            (void)hipGetLastError();
+            *err = set_error(HP_EHIP, "hipGetSymbolAddress of the scratch array failed on "
+                             "device %d: %s", dev, hipGetErrorString(e));
+            return nullptr;
+        }
+        Engine *en = new Engine();
+        const int r = en->init_static(ptr, kNoAllocScratchBytes);
+        if (r) {
+            delete en;
+            *err = r;
+            return nullptr;
+        }
+        g_noalloc[dev] = en;
+    }
+    *err = HP_OK;
+    return g_noalloc[dev];
+}
+
+}  // namespace hp
+
 // ------------------------------------------------------------------ C ABI
 
 struct hp_plan {
@@ -1419,6 +1553,28 @@ __attribute__((visibility("default"))) void compute_potential(struct potential *
         set_error(HP_EHIP, "copy of %lld records from the device failed", (long long)num_po);
         return fatal(HP_EHIP, __FILE__, __LINE__);
     }
+}
+
+__attribute__((visibility("default"))) int hp_compute_potential_noalloc(struct potential *d_po,
+                                                                        int64_t num_po,
+                                                                        void *stream) {
+    if (num_po < 0 || (num_po > 0 && !d_po))
+        return set_error(HP_EINVAL, "hp_compute_potential_noalloc: bad arguments");
+    if (num_po > kNoAllocMaxRecords)
+        return set_error(HP_EINVAL, "hp_compute_potential_noalloc: %lld records; at most %lld "
+                         "without allocating (use compute_potential or a plan)",
+                         (long long)num_po, (long long)kNoAllocMaxRecords);
+    if (num_po == 0) return HP_OK;
+    int r;
+    Tables T;
+    int64_t first = 0, count = num_po;
+    if ((r = make_tables(T, &first, &count, 1, HP_METHOD_TREE, false))) return r;
+    Engine *e = noalloc_engine(&r);
+    if (!e) return r;
+    e->stream = static_cast<hipStream_t>(stream);
+    e->reset_carve();
+    if ((r = e->reserve(T, false)) || (r = e->load_tables(T))) return r;
+    return e->potential(T, d_po, default_options(), true);
 }
 
 __attribute__((visibility("default"))) void compute_kinetic_energy(struct potential *po,

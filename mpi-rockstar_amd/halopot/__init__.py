@@ -88,7 +88,7 @@ _LIB = None
 
 EXPORTS = ("compute_potential", "compute_kinetic_energy", "hp_plan_create", "hp_plan_run",
            "hp_plan_potential", "hp_plan_stats", "hp_plan_destroy", "hp_plan_tune",
-           "hp_plan_set_tuning",
+           "hp_plan_set_tuning", "hp_compute_potential_noalloc",
            "hp_mass_definitions", "hp_plan_props", "hp_gather_calls", "hp_find_subfofs",
            "hp_bound_pass_host",
            "hp_debug_tree", "hp_last_error", "hp_set_error_handler")
@@ -115,6 +115,7 @@ def lib() -> C.CDLL:
         L.hp_plan_destroy.argtypes = [C.c_void_p]
         L.hp_plan_tune.argtypes = [C.c_void_p, C.c_void_p, C.c_int, C.POINTER(Tuning)]
         L.hp_plan_set_tuning.argtypes = [C.c_void_p, C.POINTER(Tuning)]
+        L.hp_compute_potential_noalloc.argtypes = [C.c_void_p, C.c_int64, C.c_void_p]
         L.hp_mass_definitions.argtypes = [C.POINTER(C.c_char_p), C.POINTER(MassDefs)]
         L.hp_gather_calls.argtypes = [C.c_void_p, C.c_int64, C.c_void_p, C.c_int64, C.c_void_p,
                                       C.c_int64, C.c_double, C.c_void_p, C.c_int64, C.c_void_p,
@@ -190,6 +191,17 @@ def compute_potential(po: np.ndarray) -> None:
     permuted as the reference permutes them."""
     _records(po)
     lib().compute_potential(_ptr(po), po.shape[0])
+
+
+NOALLOC_MAX_RECORDS = 32768
+
+
+def compute_potential_noalloc(d_po: int, n: int, stream: int | None = None) -> None:
+    """hp_compute_potential_noalloc: compute_potential on n device-resident
+    records at d_po, with no device allocation (at most NOALLOC_MAX_RECORDS
+    records; overlapping calls on one GPU race)."""
+    _check(lib().hp_compute_potential_noalloc(C.c_void_p(d_po), int(n),
+                                              C.c_void_p(stream) if stream else None))
 
 
 def compute_kinetic_energy(po: np.ndarray, vel_cen, pos_cen, scale_now: float) -> None:
