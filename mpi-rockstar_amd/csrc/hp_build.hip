@@ -611,10 +611,32 @@ __global__ __launch_bounds__(kSubThreads, kCap <= 256 ? 7 : 1) void k_subtree(co
                                                          int cls) {
     constexpr int kSubChunks = SubSize<kCap>::chunks, kSubList = SubSize<kCap>::list,
                   kSubNodes = SubSize<kCap>::nodes, kSubPerThread = SubSize<kCap>::per_thread;
-    __shared__ float4 P[2][kCap];
-    __shared__ uint8_t Q[2][kCap];       // level entry of each position
-    __shared__ SubEntry E[2][kSubList];
-    __shared__ int32_t BOX[kSubList][6];      // next level's tight boxes, float keys
+    // The points stay where they were loaded (P); the order is a 16-bit
+    // permutation, PM[buf][i] = the point at position i.  Once the levels are
+    // done, the final order is moved to PM[0] and the local nodes' centres of
+    // mass (CM) take the room of the level arrays: the LDS fits four
+    // workgroups per CU at 1,024 points.
+    struct Levels {
+        int16_t PM[2][kCap];
+        uint8_t Q[2][kCap];               // level entry of each position
+        SubEntry E[2][kSubList];
+        int32_t BOX[kSubList][6];         // next level's tight boxes, float keys
+    };
+    constexpr int kCM = (int)((sizeof(Levels) - sizeof(int16_t) * kCap) / sizeof(float4));
+    struct Centres {
+        int16_t PMF[kCap];                // the final order
+        float4 CM[kCM];                   // centres of mass of local nodes j < kCM
+    };
+    static_assert(sizeof(Centres) <= sizeof(Levels) && kCM >= kCap / 2, "centre room");
+    __shared__ float4 P[kCap];
+    __shared__ union Pool {
+        Levels lv;
+        Centres cm;
+    } pool;
+    auto &PM = pool.lv.PM;
+    auto &Q = pool.lv.Q;
+    auto &E = pool.lv.E;
+    auto &BOX = pool.lv.BOX;
     __shared__ int16_t s_left[kSubNodes];     // left child of each local node, -1: leaf
     __shared__ uint16_t s_lvl[kCap + 2]; // first local id of each level
     __shared__ uint64_t s_mask[kSubChunks + 1];
@@ -637,7 +659,8 @@ __global__ __launch_bounds__(kSubThreads, kCap <= 256 ? 7 : 1) void k_subtree(co
         const int32_t root_skip = a.nodes[t.node].skip;
         const float4 *src = (t.depth & 1) ? B : A;
         for (int i = tid; i < n; i += kSubThreads) {
-            P[0][i] = src[first + i];
+            P[i] = src[first + i];
+            PM[0][i] = (int16_t)i;
             Q[0][i] = 0;
         }
         if (tid <= kSubChunks) s_mask[tid] = 0;  // chunks past n stay empty
@@ -686,7 +709,7 @@ __global__ __launch_bounds__(kSubThreads, kCap <= 256 ? 7 : 1) void k_subtree(co
                 if (i < n) {
                     const uint8_t e = Q[cur][i];
                     if (e != kNoEntry && E[cur][e].n > kLeafPoints)
-                        gl = comp(P[buf][i], E[cur][e].dim) <= E[cur][e].lim;
+                        gl = comp(P[PM[buf][i]], E[cur][e].dim) <= E[cur][e].lim;
                 }
                 const uint64_t m = ballot(gl);
                 if (lane == 0) s_mask[r * kSubWaves + w] = m;
@@ -799,7 +822,8 @@ __global__ __launch_bounds__(kSubThreads, kCap <= 256 ? 7 : 1) void k_subtree(co
             for (int r = 0; r < rounds; r++) {
                 const int i = r * kSubThreads + tid;
                 if (i >= n) break;
-                const float4 p = P[buf][i];
+                const int16_t pi = PM[buf][i];
+                const float4 p = P[pi];
                 const uint8_t e = Q[cur][i];
                 int dst = i;
                 uint8_t q = kNoEntry;
@@ -821,7 +845,7 @@ __global__ __launch_bounds__(kSubThreads, kCap <= 256 ? 7 : 1) void k_subtree(co
                         a.leaf_of[first + i] = gid(en.loc);
                     }
                 }
-                P[buf ^ 1][dst] = p;
+                PM[buf ^ 1][dst] = pi;
                 Q[nxt][dst] = q;
             }
             __syncthreads();
@@ -833,7 +857,7 @@ __global__ __launch_bounds__(kSubThreads, kCap <= 256 ? 7 : 1) void k_subtree(co
                     const uint8_t q = i < n ? Q[nxt][i] : kNoEntry;
                     const bool fresh = q != kNoEntry;
                     if (!ballot(fresh)) continue;
-                    const float4 p = fresh ? P[buf ^ 1][i] : make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+                    const float4 p = fresh ? P[PM[buf ^ 1][i]] : make_float4(0.0f, 0.0f, 0.0f, 0.0f);
                     const int lf = fresh ? E[nxt][q].lf : i;
                     const int chunk0 = i - lane;
                     const int seg = lf > chunk0 ? lf - chunk0 : 0;
@@ -876,22 +900,28 @@ __global__ __launch_bounds__(kSubThreads, kCap <= 256 ? 7 : 1) void k_subtree(co
         }
         if (base < 0) {  // the root stays a leaf over the points in input order
             for (int i = tid; i < n; i += kSubThreads) {
-                a.fin[first + i] = P[0][i];
+                a.fin[first + i] = P[i];
                 a.leaf_of[first + i] = t.node;
             }
         }
-        // centres of mass and dmin (src/potential.c:71-124); P[buf] holds the
-        // final order, P[buf ^ 1] is free for com (x, y, z) and mass
+        // centres of mass and dmin (src/potential.c:71-124): the final order
+        // to PM[0] (PMF), the rest of the level arrays' room for com (x, y,
+        // z) and mass
+        if (buf) {
+            for (int i = tid; i < n; i += kSubThreads) PM[0][i] = PM[1][i];
+            __syncthreads();
+        }
         const int M = s_M, nlev = s_nlev;
-        const float4 *F = P[buf];
-        float4 *CM = P[buf ^ 1];
+        const int16_t *PMF = pool.cm.PMF;
+        float4 *CM = pool.cm.CM;
+        auto F = [&](int i) { return P[PMF[i]]; };
         auto cm_get = [&](int j) {
-            if (j < kCap) return CM[j];
+            if (j < kCM) return CM[j];
             const Node *np = a.nodes + gid(j);
             return make_float4(np->com[0], np->com[1], np->com[2], np->mass);
         };
         auto cm_put = [&](int j, float4 v) {
-            if (j < kCap) CM[j] = v;
+            if (j < kCM) CM[j] = v;
             Node *np = a.nodes + gid(j);
             np->com[0] = v.x;
             np->com[1] = v.y;
@@ -916,7 +946,7 @@ __global__ __launch_bounds__(kSubThreads, kCap <= 256 ? 7 : 1) void k_subtree(co
             if (j >= M || s_left[j] >= 0) continue;
             double sx = 0.0, sy = 0.0, sz = 0.0;
             for (int i = my_lf[q]; i < my_lf[q] + my_n[q]; i++) {
-                const float4 p = F[i];
+                const float4 p = F(i);
                 sx += (double)p.x;
                 sy += (double)p.y;
                 sz += (double)p.z;
@@ -965,7 +995,7 @@ __global__ __launch_bounds__(kSubThreads, kCap <= 256 ? 7 : 1) void k_subtree(co
             const float com[3] = {c4.x, c4.y, c4.z};
             double big = 0.0, sum = 0.0;
             for (int i = my_lf[q]; i < my_lf[q] + my_n[q]; i++) {
-                const double d = d2_ref(com, F[i]);
+                const double d = d2_ref(com, F(i));
                 big = d > big ? d : big;
                 sum += d;
             }
@@ -987,7 +1017,7 @@ __global__ __launch_bounds__(kSubThreads, kCap <= 256 ? 7 : 1) void k_subtree(co
                 const float com[3] = {c4.x, c4.y, c4.z};
                 double big = 0.0, sum = 0.0;
                 for (int32_t i = lane; i < qn; i += kWave) {
-                    const double d = d2_ref(com, F[ql + i]);
+                    const double d = d2_ref(com, F(ql + i));
                     big = d > big ? d : big;
                     sum += d;
                 }
