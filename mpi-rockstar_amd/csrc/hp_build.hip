@@ -75,9 +75,11 @@ __global__ void k_fill_i32(int32_t *p, int32_t v, int64_t n) {
 }
 
 // box keys per halo: [lo0 lo1 lo2 hi0 hi1 hi2] as order-preserving ints,
-// bad: non-finite count, pot: potential-only records (ke < 0)
+// bad: non-finite count, pot: potential-only records (ke < 0) per call, and
+// pot[nhalos] > 0 when some call has any
 __global__ void k_root_init(int32_t *box, int32_t *bad, int32_t *pot, int nhalos) {
     int h = blockIdx.x * blockDim.x + threadIdx.x;
+    if (h == 0) pot[nhalos] = 0;
     if (h >= nhalos) return;
     for (int k = 0; k < 3; k++) {
         box[6 * h + k] = float_key(__int_as_float(0x7f800000));       // +inf
@@ -87,11 +89,15 @@ __global__ void k_root_init(int32_t *box, int32_t *bad, int32_t *pot, int nhalos
     pot[h] = 0;
 }
 
+// The records into float4 (x, y, z, record index), each call's tight box and
+// non-finite and potential-only counts; and each record's sort key for the
+// bound pass (potential-only flag above the r2 bits), read here where the
+// records stream in order.
 __global__ __launch_bounds__(kTileThreads) void k_gather(
     const struct potential *__restrict__ po, const int64_t *__restrict__ hfirst,
     const int64_t *__restrict__ hcount, const RootTile *__restrict__ tiles,
     float4 *__restrict__ A, int32_t *__restrict__ box, int32_t *__restrict__ bad,
-    int32_t *__restrict__ pot) {
+    int32_t *__restrict__ pot, int32_t nhalos, uint64_t *__restrict__ rkey) {
     __shared__ float s_lo[3][kTileThreads / kWave], s_hi[3][kTileThreads / kWave];
     __shared__ int s_bad[kTileThreads / kWave], s_pot[kTileThreads / kWave];
     const RootTile t = tiles[blockIdx.x];
@@ -108,7 +114,9 @@ __global__ __launch_bounds__(kTileThreads) void k_gather(
         if (i >= end) break;
         const float *q = po[i].pos;
         float x = q[0], y = q[1], z = q[2];
-        npot += po[i].ke < 0.0f ? 1 : 0;
+        const bool potonly = po[i].ke < 0.0f;
+        npot += potonly ? 1 : 0;
+        rkey[i] = ((uint64_t)(potonly ? 1 : 0) << 32) | (uint64_t)__float_as_uint(po[i].r2);
         A[i] = make_float4(x, y, z, __int_as_float((int32_t)i));
         if (finite3(x, y, z)) {
             lo[0] = fminf(lo[0], x); hi[0] = fmaxf(hi[0], x);
@@ -154,7 +162,10 @@ __global__ __launch_bounds__(kTileThreads) void k_gather(
             atomicMax(&box[6 * t.halo + 3 + k], float_key(b[k]));
         }
         if (tb) atomicAdd(&bad[t.halo], tb);
-        if (tp) atomicAdd(&pot[t.halo], tp);
+        if (tp) {
+            atomicAdd(&pot[t.halo], tp);
+            atomicOr(&pot[nhalos], 1);
+        }
     }
 }
 

@@ -346,7 +346,7 @@ struct Engine {
     DevBuf hfirst, hcount, root_tiles, groups, direct_blocks, scan_tiles, call_tile0,
         call_ntiles;
     // build
-    DevBuf box, bad, pot, badlist, A, B, F, leaf_of, sides, nodes, parent, fat, small0, group0, groupc, wave_off, wave_hsum, big, tiles, parts,
+    DevBuf box, bad, pot, badlist, A, B, F, leaf_of, sides, rkey, nodes, parent, fat, small0, group0, groupc, wave_off, wave_hsum, big, tiles, parts,
         splitres, dparts, counters, work, depth_buf;
     // unbind
     DevBuf keys0, keys1, vals0, vals1, hist, hsum, srt, callinfo, tsum, tcand, tile_i32, strip_cnt;
@@ -435,7 +435,7 @@ struct Engine {
     template <class Fn>
     void for_each_buf(Fn &&f) {
         DevBuf *all[] = {&hfirst, &hcount, &root_tiles, &groups, &direct_blocks,
-                         &scan_tiles, &call_tile0, &call_ntiles, &box, &bad, &pot, &badlist, &A, &B, &F,
+                         &scan_tiles, &call_tile0, &call_ntiles, &box, &bad, &pot, &rkey, &badlist, &A, &B, &F,
                          &leaf_of, &sides, &nodes, &parent, &fat, &small0, &group0, &groupc, &wave_off, &wave_hsum, &big, &tiles,
                          &parts, &splitres, &dparts, &counters, &work, &depth_buf, &pcnt, &pscan, &pcall, &psums, &shape_st, &shape_part, &shape_cd, &scale_calls, &r_pe, &pe_cnt, &fit_list, &fit_n, &gsegs, &gtiles, &gcen, &keys0, &keys1, &vals0,
                          &vals1, &hist, &hsum, &srt, &callinfo, &tsum, &tcand, &tile_i32, &strip_cnt, &stage,
@@ -520,6 +520,7 @@ struct Engine {
         HP_GRAB(F, sizeof(float4) * (N + 1));
         HP_GRAB(leaf_of, sizeof(int32_t) * (N + 1));
         HP_GRAB(sides, sizeof(int32_t) * (N + 1));
+        HP_GRAB(rkey, sizeof(uint64_t) * (N + 1));
         // split levels use at most about 0.1N + H ids; a subtree of n points
         // (workgroup or wavefront) reserves 2n - 2
         cap_nodes = 5 * N + H + 1024;
@@ -654,7 +655,8 @@ struct Engine {
         if (!T.root_tiles.empty()) {
             k_gather<<<(int)T.root_tiles.size(), kTileThreads, 0, s>>>(
                 d_po, hfirst.as<int64_t>(), hcount.as<int64_t>(), root_tiles.as<RootTile>(),
-                A.as<float4>(), box.as<int32_t>(), bad.as<int32_t>(), pot.as<int32_t>());
+                A.as<float4>(), box.as<int32_t>(), bad.as<int32_t>(), pot.as<int32_t>(), H,
+                rkey.as<uint64_t>());
             HP_LAUNCHED();
         }
         k_fill_i32<<<1, 64, 0, s>>>(counters.as<int32_t>(), 0, kNumCounters);
@@ -1204,7 +1206,8 @@ struct Engine {
         // compute_potential and potential-only strip leave them
         const int32_t nst_k = (int32_t)T.scan_tiles.size();
         KeyArgs ka;
-        ka.po = d_po;
+        ka.rkey = rkey.as<uint64_t>();
+        ka.ncalls = H;
         ka.fin = T.method == HP_METHOD_DIRECT ? nullptr : F.as<float4>();
         ka.tiles = scan_tiles.as<ScanTile>();
         ka.hfirst = hfirst.as<int64_t>();
@@ -1221,7 +1224,7 @@ struct Engine {
             HP_LAUNCHED();
             k_strip_count<<<nst_k, kKeyThreads, 0, s>>>(ka);
             HP_LAUNCHED();
-            k_strip_prefix<<<blocks(H, 256), 256, 0, s>>>(ka, H);
+            k_strip_prefix<<<blocks(H, 256), 256, 0, s>>>(ka);
             HP_LAUNCHED();
             k_strip_list<<<nst_k, kKeyThreads, 0, s>>>(ka);
             HP_LAUNCHED();

@@ -52,11 +52,6 @@ struct ScanTile {
 
 // ---------------------------------------------------------------- radix sort
 
-// key of one record inside its call: potential-only flag above the r2 bits
-__device__ __forceinline__ uint64_t sort_key(const struct potential *__restrict__ r) {
-    const uint64_t pot = r->ke < 0.0f ? 1ull : 0ull;
-    return (pot << 32) | (uint64_t)__float_as_uint(r->r2);
-}
 
 // digit histogram of one tile: hist[digit * ntiles + tile]
 __global__ __launch_bounds__(kSortThreads) void k_radix_hist(const uint64_t *__restrict__ keys,
@@ -379,11 +374,12 @@ constexpr int kKeyThreads = 256;
 constexpr int kKeyPer = kScanTile / kKeyThreads;  // consecutive positions per thread
 
 struct KeyArgs {
-    const struct potential *po;
+    const uint64_t *rkey;       // each record's sort key, in input order
     const float4 *fin;          // tree order (record index in w); null: the input order
     const ScanTile *tiles;
     const int64_t *hfirst, *hcount;
-    const int32_t *pot;         // potential-only records of each call
+    const int32_t *pot;         // potential-only records of each call; pot[ncalls] > 0 if any
+    int32_t ncalls;
     const int32_t *call_tile0, *call_ntiles;
     int32_t *tcnt;              // per tile: its potential-only records, then their prefix
     int32_t *bpos;              // per position: tree positions of the potential-only records
@@ -404,7 +400,7 @@ __global__ __launch_bounds__(kKeyThreads) void k_keys_tree(KeyArgs a) {
         const int64_t t = tl.off + j;
         if (t >= n) break;
         const int32_t idx = record_at(a, first + t);
-        a.keys[first + t] = sort_key(a.po + idx);
+        a.keys[first + t] = a.rkey[idx];
         a.vals[first + t] = idx;
     }
 }
@@ -428,7 +424,7 @@ __device__ StripRun strip_run(const KeyArgs &a, const ScanTile &tl, int64_t firs
         r.idx[j] = -1;
         if (t0 + j < n) {
             r.idx[j] = record_at(a, first + t0 + j);
-            if (a.po[r.idx[j]].ke < 0.0f) r.potm |= 1u << j;
+            if (a.rkey[r.idx[j]] >> 32) r.potm |= 1u << j;
         }
     }
     const int c = __popc(r.potm);
@@ -448,6 +444,7 @@ __device__ StripRun strip_run(const KeyArgs &a, const ScanTile &tl, int64_t firs
 }
 
 __global__ __launch_bounds__(kKeyThreads) void k_strip_count(KeyArgs a) {
+    if (a.pot[a.ncalls] == 0) return;  // no call has potential-only records
     const ScanTile tl = a.tiles[blockIdx.x];
     if (a.pot[tl.call] == 0) return;
     const StripRun r = strip_run(a, tl, a.hfirst[tl.call], a.hcount[tl.call]);
@@ -455,9 +452,9 @@ __global__ __launch_bounds__(kKeyThreads) void k_strip_count(KeyArgs a) {
 }
 
 // per call with potential-only records: exclusive prefix of its tiles' counts
-__global__ void k_strip_prefix(KeyArgs a, int32_t ncalls) {
+__global__ void k_strip_prefix(KeyArgs a) {
     const int32_t h = blockIdx.x * blockDim.x + threadIdx.x;
-    if (h >= ncalls || a.pot[h] == 0) return;
+    if (a.pot[a.ncalls] == 0 || h >= a.ncalls || a.pot[h] == 0) return;
     int32_t run = 0;
     for (int32_t k = a.call_tile0[h]; k < a.call_tile0[h] + a.call_ntiles[h]; k++) {
         const int32_t c = a.tcnt[k];
@@ -468,6 +465,7 @@ __global__ void k_strip_prefix(KeyArgs a, int32_t ncalls) {
 
 // the tree positions of each call's potential-only records, in order
 __global__ __launch_bounds__(kKeyThreads) void k_strip_list(KeyArgs a) {
+    if (a.pot[a.ncalls] == 0) return;  // no call has potential-only records
     const ScanTile tl = a.tiles[blockIdx.x];
     if (a.pot[tl.call] == 0) return;
     const int64_t first = a.hfirst[tl.call];
@@ -481,6 +479,7 @@ __global__ __launch_bounds__(kKeyThreads) void k_strip_list(KeyArgs a) {
 
 // keys at each record's place after the strip
 __global__ __launch_bounds__(kKeyThreads) void k_strip_keys(KeyArgs a) {
+    if (a.pot[a.ncalls] == 0) return;  // no call has potential-only records
     const ScanTile tl = a.tiles[blockIdx.x];
     const int32_t npot = a.pot[tl.call];
     if (npot == 0) return;
@@ -500,7 +499,7 @@ __global__ __launch_bounds__(kKeyThreads) void k_strip_keys(KeyArgs a) {
             const int64_t kept_rank = t - b;
             q = t < nk ? t : (int64_t)a.bpos[first + (nk - 1 - kept_rank)];
         }
-        a.keys[first + q] = sort_key(a.po + r.idx[j]);
+        a.keys[first + q] = a.rkey[r.idx[j]];
         a.vals[first + q] = r.idx[j];
     }
 }
