@@ -121,6 +121,12 @@ enum {
                                faster AND the halo has at most 10 points */
 };
 
+enum {
+    HP_WALK_PAIRS = 0,      /* both children of an opened node decided in one
+                               step (a stack of pending right children) */
+    HP_WALK_NODES = 1       /* one node per step on skip pointers (no stack) */
+};
+
 typedef struct hp_options {
     int32_t accumulate;     /* HP_ACC_* */
     int32_t method;         /* HP_METHOD_* */
@@ -129,7 +135,8 @@ typedef struct hp_options {
     int32_t count_work;     /* count node visits / interactions (slower) */
     int32_t per_call;       /* 1: run the calls one after another, each with
                                its own launches (the plain per-halo path) */
-    int32_t reserved[2];
+    int32_t walk;           /* HP_WALK_PAIRS (default) or HP_WALK_NODES */
+    int32_t reserved;
 } hp_options;
 
 typedef struct hp_plan hp_plan;

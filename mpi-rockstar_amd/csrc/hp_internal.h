@@ -22,29 +22,43 @@ constexpr int kGroup = 64;        // sink particles per walk wavefront (one per 
 constexpr int kTile = 1024;       // points per workgroup tile of a big node
 constexpr int kSmallMax = 1024;   // largest subtree finished by one workgroup
 constexpr int kSmallChunks = kSmallMax / 64;
+constexpr int kSrcBatch = 8;      // leaf points the walk reads per scalar batch
 constexpr int kFatLeaf = 64;      // leaves above this size get a wavefront
 constexpr int kTileThreads = 256;
 constexpr int kMaxLevels = 4096;  // depth_end entries kept per build
 
-// Tree node, 64 bytes, read by the walk with one scalar load.
+// Tree node, 64 bytes, read by the walk with scalar loads.
 //   [first, first + count) is the node's range in tree order (batch-global);
 //   left is the first child (the right child is left + 1) or -1 for a leaf;
 //   skip is the node that follows the subtree in depth-first order (-1: end).
-// The centre of mass and the mass come first: the list walk's evaluation
-// reads them as one aligned float4 per monopole.
+// What the walk reads of a node comes first (NodeHead, 32 bytes), so that a
+// pair of siblings is read as two 32-byte loads from one 128-byte line.
 struct __attribute__((aligned(16))) Node {
     float com[3];
     float mass;
-    float lo[3];
-    float hi[3];
     float dmin2;     // float(dmin) * float(dmin), src/potential.c:128
     int32_t first;
     int32_t count;
     int32_t left;
+    float lo[3];
+    float hi[3];
     int32_t skip;
     float dmin;
 };
-static_assert(sizeof(Node) == 64 && offsetof(Node, com) == 0 && offsetof(Node, mass) == 12,
+struct NodeHead {
+    float com[3];
+    float mass;
+    float dmin2;
+    int32_t first;
+    int32_t count;
+    int32_t left;
+};
+static_assert(sizeof(Node) == 64 && sizeof(NodeHead) == 32 && offsetof(Node, com) == 0 &&
+                  offsetof(Node, mass) == offsetof(NodeHead, mass) &&
+                  offsetof(Node, dmin2) == offsetof(NodeHead, dmin2) &&
+                  offsetof(Node, first) == offsetof(NodeHead, first) &&
+                  offsetof(Node, count) == offsetof(NodeHead, count) &&
+                  offsetof(Node, left) == offsetof(NodeHead, left),
               "node layout");
 
 // A node waiting to be split at the current level.

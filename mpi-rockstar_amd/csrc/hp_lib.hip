@@ -517,7 +517,7 @@ struct Engine {
         HP_GRAB(badlist, sizeof(int32_t) * (H + 1));
         HP_GRAB(A, sizeof(float4) * (N + 1));
         HP_GRAB(B, sizeof(float4) * (N + 1));
-        HP_GRAB(F, sizeof(float4) * (N + 1));
+        HP_GRAB(F, sizeof(float4) * (N + kSrcBatch));  // the walk reads leaf points in batches
         HP_GRAB(leaf_of, sizeof(int32_t) * (N + 1));
         HP_GRAB(sides, sizeof(int32_t) * (N + 1));
         HP_GRAB(rkey, sizeof(uint64_t) * (N + 1));
@@ -779,6 +779,7 @@ struct Engine {
             a.counters = counters.as<int32_t>();
             a.nrec = (int32_t)T.nrec;
             a.cap_nodes = (int32_t)cap_nodes;
+            a.pairs = o.walk != HP_WALK_NODES;
             a.pm = (float)PARTICLE_MASS;
             a.inv_fr = (float)(1.0 / FORCE_RES);
             a.pm_d = PARTICLE_MASS;
@@ -1622,6 +1623,9 @@ __attribute__((visibility("default"))) int hp_plan_create(hp_plan **out, const h
                                                           const hp_options *opt) {
     if (!out || (nhalos > 0 && !halos) || nhalos < 0)
         return set_error(HP_EINVAL, "hp_plan_create: bad arguments");
+    if (opt && ((uint32_t)opt->accumulate > HP_ACC_F64 || (uint32_t)opt->method > HP_METHOD_AUTO ||
+                (uint32_t)opt->walk > HP_WALK_NODES))
+        return set_error(HP_EINVAL, "hp_plan_create: bad options");
     hp_plan *p = new hp_plan();
     p->opt = opt ? *opt : default_options();
     // the calls are checked before any device work

@@ -31,6 +31,7 @@ PKG = pathlib.Path(__file__).resolve().parent
 LIB_PATH = PKG / "libhalopot.so"
 
 ACC_COMPENSATED, ACC_NAIVE, ACC_F64 = 0, 1, 2
+WALK_PAIRS, WALK_NODES = 0, 1
 METHOD_TREE, METHOD_DIRECT, METHOD_AUTO = 0, 1, 2
 
 NODE_OUT_DTYPE = np.dtype(
@@ -52,7 +53,7 @@ class HalopotError(RuntimeError):
 class Options(C.Structure):
     _fields_ = [("accumulate", C.c_int32), ("method", C.c_int32), ("use_graph", C.c_int32),
                 ("timing", C.c_int32), ("count_work", C.c_int32), ("per_call", C.c_int32),
-                ("reserved", C.c_int32 * 2)]
+                ("walk", C.c_int32), ("reserved", C.c_int32)]
 
 
 class Stats(C.Structure):
@@ -213,10 +214,10 @@ def compute_kinetic_energy(po: np.ndarray, vel_cen, pos_cen, scale_now: float) -
 
 
 def options(accumulate=ACC_COMPENSATED, method=METHOD_TREE, use_graph=0, timing=0,
-            count_work=0, per_call=0) -> Options:
+            count_work=0, per_call=0, walk=WALK_PAIRS) -> Options:
     o = Options()
-    o.accumulate, o.method, o.use_graph, o.timing, o.count_work, o.per_call = (
-        accumulate, method, use_graph, timing, count_work, per_call)
+    o.accumulate, o.method, o.use_graph, o.timing, o.count_work, o.per_call, o.walk = (
+        accumulate, method, use_graph, timing, count_work, per_call, walk)
     return o
 
 

@@ -10,7 +10,8 @@ Checked against the oracle:
   * walk decisions: the batch's P2P pair and P2M term counts equal the
     restatement's (summed over its per-call walks) under every launch
     setting the autotuner can choose (the trees, and so every decision, do
-    not depend on the setting);
+    not depend on the setting), with the sibling-pair walk and with the
+    node-by-node walk;
   * the bound pass against the reference's own compute_potential and
     compute_kinetic_energy (oracle/_ref) inside the restated driver of
     src/properties.c:537-550: kept counts and n_core identical; the kept
@@ -86,16 +87,19 @@ def test_walk_counts_every_setting(restatement, cat):
     want = (int(p2p.sum()), int(p2m.sum()))
     print(f"restatement: {want[0]} P2P pairs, {want[1]} P2M terms", flush=True)
     dev = Device(cat)
-    plan = halopot.Plan(cat.halos, halopot.options(count_work=1))
     try:
-        for bt, wm in SETTINGS:
-            plan.set_tuning(bt, wm)
-            dev.restore()
-            plan.potential(dev.po.address)
-            st = plan.stats()
-            assert (st["p2p"], st["p2m"]) == want, (bt, wm, st["p2p"], st["p2m"])
+        for walk in (halopot.WALK_PAIRS, halopot.WALK_NODES):
+            plan = halopot.Plan(cat.halos, halopot.options(count_work=1, walk=walk))
+            try:
+                for bt, wm in SETTINGS if walk == halopot.WALK_PAIRS else SETTINGS[-1:]:
+                    plan.set_tuning(bt, wm)
+                    dev.restore()
+                    plan.potential(dev.po.address)
+                    st = plan.stats()
+                    assert (st["p2p"], st["p2m"]) == want, (walk, bt, wm, st["p2p"], st["p2m"])
+            finally:
+                plan.close()
     finally:
-        plan.close()
         dev.free()
 
 

@@ -224,3 +224,86 @@ def test_edge_cases(restatement):
     assert pe[3] == 0 and pe[17] == 0 and np.all(pe[[0, 1, 2, 4]] > 0)
     # non-finite records end at the tail (src/fast3tree.c:875-887)
     assert set(g["flags"][-2:]) == {3, 17}
+
+
+def _shell_chain(shells=100, per_shell=24, core=8, seed=3):
+    """Points on nested shells along x: shell k holds per_shell points with x
+    in [0.52, 0.98] * 2^(k - 49) (one at 0.98), y and z in +-0.48 of that
+    scale; a core of `core` points lies below shell 0.  Every split of the
+    chain is the plane between two shells, so the tree is shells + a few
+    levels deep, and each shell node is close enough to the core to be
+    opened by the core's leaves: the walk of the first 64 points keeps
+    every shell pending at once."""
+    rng = np.random.default_rng(seed)
+    s0 = 2.0 ** -50
+    parts = [np.column_stack([rng.uniform(0.05, 0.45, core) * s0,
+                              rng.uniform(-0.2, 0.2, (core, 2)) * s0])]
+    for k in range(shells):
+        s = 2.0 ** (k + 1) * s0
+        x = rng.uniform(0.52, 0.95, per_shell) * s
+        x[0] = 0.98 * s
+        parts.append(np.column_stack([x, rng.uniform(-0.48, 0.48, (per_shell, 2)) * s]))
+    pos = np.concatenate(parts)
+    po = catalog._records(pos, np.zeros_like(pos), np.zeros(3))
+    return with_ids(po)
+
+
+def _pending_pairs(nodes, npts=64):
+    """Deepest stack of pending sibling pairs in the pair walk of the first
+    npts points in tree order (the restatement's tree; acceptance in FP64,
+    the shells being far from the threshold)."""
+    leaves = np.nonzero(nodes["left"] < 0)[0]
+    leaf_of = np.empty(int(nodes["n"][0]), np.int64)
+    for lf in leaves:
+        leaf_of[nodes["first"][lf]:nodes["first"][lf] + nodes["n"][lf]] = lf
+    lanes = leaf_of[:npts]
+    com = nodes["com"].astype(np.float64)
+    dmin2 = nodes["dmin"].astype(np.float64) ** 2
+
+    def opened(i, m):
+        if nodes["left"][i] < 0:
+            return set()
+        out = set()
+        for ln in m:
+            lf = lanes[ln]
+            inside = nodes["first"][i] <= nodes["first"][lf] < nodes["first"][i] + nodes["n"][i]
+            if inside or not np.sum((com[lf] - com[i]) ** 2) > dmin2[i]:
+                out.add(ln)
+        return out
+
+    stack, deepest = [], 0
+    c, m = int(nodes["left"][0]), set(range(npts))
+    while True:
+        o0, o1 = opened(c, m), opened(c + 1, m)
+        if o1:
+            stack.append((int(nodes["left"][c + 1]), o1))
+            deepest = max(deepest, len(stack))
+        if o0:
+            c, m = int(nodes["left"][c]), o0
+        elif stack:
+            c, m = stack.pop()
+        else:
+            return deepest
+
+
+@pytest.mark.parametrize("walk", [halopot.WALK_PAIRS, halopot.WALK_NODES])
+def test_deep_tree_walks(restatement, walk):
+    """A tree about 100 levels deep whose first wavefront keeps more sibling
+    pairs pending than the pair walk's 64-entry stack holds: that wavefront
+    finishes on the node-by-node walk, the others on pairs.  Both walks take
+    the restatement's decisions (equal P2P and P2M counts) and give its pe."""
+    prm = catalog.Params(particle_mass=1e9)
+    halopot.set_config(prm)
+    po = _shell_chain()
+    o = po.copy()
+    s = restatement.compute_potential(o, prm, tree=True)
+    nodes = s["tree"]
+    assert int(nodes["depth"].max()) > 90
+    assert _pending_pairs(nodes) > 64
+    cat = catalog.Catalogue("shell_chain", prm, po.copy(),
+                            catalog._halo_rows([0], [po.shape[0]], [[0, 0, 0]], [[0, 0, 0]], [0]))
+    out, _, _, st = run_plan(cat, halopot.options(count_work=1, walk=walk), potential_only=True)
+    assert (st["p2p"], st["p2m"]) == (s["p2p"], s["p2m"])
+    np.testing.assert_array_equal(out["flags"], o["flags"])
+    r = rel(out["pe"], o["pe"])
+    assert np.max(r) < 1e-5, np.max(r)
