@@ -349,7 +349,8 @@ struct Engine {
     DevBuf box, bad, pot, badlist, A, B, F, leaf_of, sides, rkey, nodes, parent, fat, small0, group0, groupc, wave_off, wave_hsum, big, tiles, parts,
         splitres, dparts, counters, work, depth_buf;
     // unbind
-    DevBuf keys0, keys1, vals0, vals1, hist, hsum, srt, callinfo, tsum, tcand, tile_i32, strip_cnt;
+    DevBuf keys0, keys1, vals0, vals1, hist, hsum, srt, callinfo, tsum, tcand, tile_i32, rec_call,
+        strip_cnt;
     DevBuf stage;  // records in tree order before they go back (compute_potential's permutation)
     DevBuf seg_wave_l, seg_block_l, seg_tiles_l, seg_hist, seg_hsum, srt_rv;
     // halo properties
@@ -438,7 +439,7 @@ struct Engine {
                          &scan_tiles, &call_tile0, &call_ntiles, &box, &bad, &pot, &rkey, &badlist, &A, &B, &F,
                          &leaf_of, &sides, &nodes, &parent, &fat, &small0, &group0, &groupc, &wave_off, &wave_hsum, &big, &tiles,
                          &parts, &splitres, &dparts, &counters, &work, &depth_buf, &pcnt, &pscan, &pcall, &psums, &shape_st, &shape_part, &shape_cd, &scale_calls, &r_pe, &pe_cnt, &fit_list, &fit_n, &gsegs, &gtiles, &gcen, &keys0, &keys1, &vals0,
-                         &vals1, &hist, &hsum, &srt, &callinfo, &tsum, &tcand, &tile_i32, &strip_cnt, &stage,
+                         &vals1, &hist, &hsum, &srt, &callinfo, &tsum, &tcand, &tile_i32, &rec_call, &strip_cnt, &stage,
                          &seg_wave_l, &seg_block_l, &seg_tiles_l, &seg_hist, &seg_hsum, &srt_rv};
         for (DevBuf *b : all) f(*b);
     }
@@ -560,6 +561,7 @@ struct Engine {
             HP_GRAB(tsum, sizeof(Moments) * (T.scan_tiles.size() + 1));
             HP_GRAB(tcand, sizeof(Cand) * 2 * (T.scan_tiles.size() + 1));
             HP_GRAB(tile_i32, sizeof(int32_t) * (T.scan_tiles.size() + 1));
+            HP_GRAB(rec_call, sizeof(int32_t) * (N + 1));
             HP_GRAB(strip_cnt, sizeof(int32_t) * (T.scan_tiles.size() + 1));
             const int64_t m = (int64_t)T.seg_tiles.size() * kSegRadix;
             HP_GRAB(seg_hist, sizeof(int32_t) * (m + 1));
@@ -584,6 +586,11 @@ struct Engine {
         if ((r = upload(seg_wave_l, T.seg_wave))) return r;
         if ((r = upload(seg_block_l, T.seg_block))) return r;
         if ((r = upload(seg_tiles_l, T.seg_tiles))) return r;
+        if (!T.scan_tiles.empty() && rec_call.p && T.nhalos > 0) {  // each record's call
+            k_rec_calls<<<(int)T.nhalos, 256, 0, stream>>>(hfirst.as<int64_t>(), hcount.as<int64_t>(),
+                                                          (int32_t)T.nhalos, rec_call.as<int32_t>());
+            HP_LAUNCHED();
+        }
         return HP_OK;
     }
 
@@ -1221,15 +1228,16 @@ struct Engine {
         ka.keys = keys0.as<uint64_t>();
         ka.vals = vals0.as<int32_t>();
         if (nst_k > 0) {
-            k_keys_tree<<<nst_k, kKeyThreads, 0, s>>>(ka);
+            const int sg = std::min(nst_k, 2048);  // strip kernels: a tile per workgroup, grid-stride
+            k_keys_tree<<<blocks(N, 256), 256, 0, s>>>(ka, rec_call.as<int32_t>(), N);
             HP_LAUNCHED();
-            k_strip_count<<<nst_k, kKeyThreads, 0, s>>>(ka);
+            k_strip_count<<<sg, kKeyThreads, 0, s>>>(ka, nst_k);
             HP_LAUNCHED();
             k_strip_prefix<<<blocks(H, 256), 256, 0, s>>>(ka);
             HP_LAUNCHED();
-            k_strip_list<<<nst_k, kKeyThreads, 0, s>>>(ka);
+            k_strip_list<<<sg, kKeyThreads, 0, s>>>(ka, nst_k);
             HP_LAUNCHED();
-            k_strip_keys<<<nst_k, kKeyThreads, 0, s>>>(ka);
+            k_strip_keys<<<sg, kKeyThreads, 0, s>>>(ka, nst_k);
             HP_LAUNCHED();
         }
         // each call's records sorted inside its own range, into keys1 / vals1

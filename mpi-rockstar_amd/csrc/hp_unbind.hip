@@ -50,6 +50,14 @@ struct ScanTile {
     int32_t off;
 };
 
+// Each record's call, filled once per plan (per-record kernels read it).
+__global__ void k_rec_calls(const int64_t *__restrict__ hfirst, const int64_t *__restrict__ hcount,
+                            int32_t nh, int32_t *__restrict__ rec_call) {
+    const int32_t h = blockIdx.x;
+    if (h >= nh) return;
+    for (int64_t j = threadIdx.x; j < hcount[h]; j += blockDim.x) rec_call[hfirst[h] + j] = h;
+}
+
 // ---------------------------------------------------------------- radix sort
 
 
@@ -391,18 +399,14 @@ __device__ __forceinline__ int32_t record_at(const KeyArgs &a, int64_t pos) {
     return a.fin ? __float_as_int(a.fin[pos].w) : (int32_t)pos;
 }
 
-// calls without potential-only records: keys in tree order
-__global__ __launch_bounds__(kKeyThreads) void k_keys_tree(KeyArgs a) {
-    const ScanTile tl = a.tiles[blockIdx.x];
-    if (a.pot[tl.call] != 0) return;
-    const int64_t first = a.hfirst[tl.call], n = a.hcount[tl.call];
-    for (int j = threadIdx.x; j < kScanTile; j += kKeyThreads) {
-        const int64_t t = tl.off + j;
-        if (t >= n) break;
-        const int32_t idx = record_at(a, first + t);
-        a.keys[first + t] = a.rkey[idx];
-        a.vals[first + t] = idx;
-    }
+// calls without potential-only records: keys in tree order, a thread per
+// position
+__global__ void k_keys_tree(KeyArgs a, const int32_t *__restrict__ rec_call, int64_t n) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n || a.pot[rec_call[i]] != 0) return;
+    const int32_t idx = record_at(a, i);
+    a.keys[i] = a.rkey[idx];
+    a.vals[i] = idx;
 }
 
 // Potential-only flags of this thread's kKeyPer consecutive positions of a
@@ -443,12 +447,16 @@ __device__ StripRun strip_run(const KeyArgs &a, const ScanTile &tl, int64_t firs
     return r;
 }
 
-__global__ __launch_bounds__(kKeyThreads) void k_strip_count(KeyArgs a) {
+// (the strip kernels take a tile per workgroup, the grid striding over the
+// tiles; they return at once when no call has potential-only records)
+__global__ __launch_bounds__(kKeyThreads) void k_strip_count(KeyArgs a, int32_t ntiles) {
     if (a.pot[a.ncalls] == 0) return;  // no call has potential-only records
-    const ScanTile tl = a.tiles[blockIdx.x];
-    if (a.pot[tl.call] == 0) return;
-    const StripRun r = strip_run(a, tl, a.hfirst[tl.call], a.hcount[tl.call]);
-    if (threadIdx.x == 0) a.tcnt[blockIdx.x] = r.total;
+    for (int32_t k = blockIdx.x; k < ntiles; k += gridDim.x) {
+        const ScanTile tl = a.tiles[k];
+        if (a.pot[tl.call] == 0) continue;
+        const StripRun r = strip_run(a, tl, a.hfirst[tl.call], a.hcount[tl.call]);
+        if (threadIdx.x == 0) a.tcnt[k] = r.total;
+    }
 }
 
 // per call with potential-only records: exclusive prefix of its tiles' counts
@@ -464,47 +472,73 @@ __global__ void k_strip_prefix(KeyArgs a) {
 }
 
 // the tree positions of each call's potential-only records, in order
-__global__ __launch_bounds__(kKeyThreads) void k_strip_list(KeyArgs a) {
+__global__ __launch_bounds__(kKeyThreads) void k_strip_list(KeyArgs a, int32_t ntiles) {
     if (a.pot[a.ncalls] == 0) return;  // no call has potential-only records
-    const ScanTile tl = a.tiles[blockIdx.x];
-    if (a.pot[tl.call] == 0) return;
-    const int64_t first = a.hfirst[tl.call];
-    const StripRun r = strip_run(a, tl, first, a.hcount[tl.call]);
-    int32_t b = a.tcnt[blockIdx.x] + r.before;
-    const int64_t t0 = tl.off + (int64_t)threadIdx.x * kKeyPer;
+    for (int32_t k = blockIdx.x; k < ntiles; k += gridDim.x) {
+        const ScanTile tl = a.tiles[k];
+        if (a.pot[tl.call] == 0) continue;
+        const int64_t first = a.hfirst[tl.call];
+        const StripRun r = strip_run(a, tl, first, a.hcount[tl.call]);
+        int32_t b = a.tcnt[k] + r.before;
+        const int64_t t0 = tl.off + (int64_t)threadIdx.x * kKeyPer;
 #pragma unroll
-    for (int j = 0; j < kKeyPer; j++)
-        if (r.potm >> j & 1u) a.bpos[first + b++] = (int32_t)(t0 + j);
+        for (int j = 0; j < kKeyPer; j++)
+            if (r.potm >> j & 1u) a.bpos[first + b++] = (int32_t)(t0 + j);
+    }
 }
 
 // keys at each record's place after the strip
-__global__ __launch_bounds__(kKeyThreads) void k_strip_keys(KeyArgs a) {
+__global__ __launch_bounds__(kKeyThreads) void k_strip_keys(KeyArgs a, int32_t ntiles) {
     if (a.pot[a.ncalls] == 0) return;  // no call has potential-only records
-    const ScanTile tl = a.tiles[blockIdx.x];
-    const int32_t npot = a.pot[tl.call];
-    if (npot == 0) return;
-    const int64_t first = a.hfirst[tl.call], n = a.hcount[tl.call];
-    const int64_t nk = n - npot;
-    const StripRun r = strip_run(a, tl, first, n);
-    int32_t b = a.tcnt[blockIdx.x] + r.before;  // potential-only positions before t
-    const int64_t t0 = tl.off + (int64_t)threadIdx.x * kKeyPer;
+    for (int32_t k = blockIdx.x; k < ntiles; k += gridDim.x) {
+        const ScanTile tl = a.tiles[k];
+        const int32_t npot = a.pot[tl.call];
+        if (npot == 0) continue;
+        const int64_t first = a.hfirst[tl.call], n = a.hcount[tl.call];
+        const int64_t nk = n - npot;
+        const StripRun r = strip_run(a, tl, first, n);
+        int32_t b = a.tcnt[k] + r.before;  // potential-only positions before t
+        const int64_t t0 = tl.off + (int64_t)threadIdx.x * kKeyPer;
 #pragma unroll
-    for (int j = 0; j < kKeyPer; j++) {
-        const int64_t t = t0 + j;
-        if (t >= n) break;
-        int64_t q;
-        if (r.potm >> j & 1u) {
-            q = nk + b++;
-        } else {
-            const int64_t kept_rank = t - b;
-            q = t < nk ? t : (int64_t)a.bpos[first + (nk - 1 - kept_rank)];
+        for (int j = 0; j < kKeyPer; j++) {
+            const int64_t t = t0 + j;
+            if (t >= n) break;
+            int64_t q;
+            if (r.potm >> j & 1u) {
+                q = nk + b++;
+            } else {
+                const int64_t kept_rank = t - b;
+                q = t < nk ? t : (int64_t)a.bpos[first + (nk - 1 - kept_rank)];
+            }
+            a.keys[first + q] = a.rkey[r.idx[j]];
+            a.vals[first + q] = r.idx[j];
         }
-        a.keys[first + q] = a.rkey[r.idx[j]];
-        a.vals[first + q] = r.idx[j];
     }
 }
 
 // ----------------------------------------------------------------- gather
+
+// A record moved as three 16-byte words (three loads and three stores
+// instead of one per field group).
+struct RecWords {
+    float4 w[3];
+};
+static_assert(sizeof(RecWords) == sizeof(struct potential), "48-byte records");
+__device__ __forceinline__ struct potential load_rec(const struct potential *p) {
+    const float4 *q = reinterpret_cast<const float4 *>(p);
+    RecWords u;
+    u.w[0] = q[0];
+    u.w[1] = q[1];
+    u.w[2] = q[2];
+    return __builtin_bit_cast(struct potential, u);
+}
+__device__ __forceinline__ void store_rec(struct potential *p, const struct potential &r) {
+    const RecWords u = __builtin_bit_cast(RecWords, r);
+    float4 *q = reinterpret_cast<float4 *>(p);
+    q[0] = u.w[0];
+    q[1] = u.w[1];
+    q[2] = u.w[2];
+}
 
 // the records in sorted order, and a compact copy of what the core-velocity
 // passes read (r2 and the velocity, 16 B instead of the 48-B record)
@@ -513,8 +547,8 @@ __global__ void k_gather_sorted(const struct potential *__restrict__ po,
                                 struct potential *__restrict__ out, float4 *__restrict__ rv) {
     const int32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= nrec) return;
-    const struct potential rec = po[order[i]];
-    out[i] = rec;
+    const struct potential rec = load_rec(po + order[i]);
+    store_rec(out + i, rec);
     rv[i] = make_float4(rec.r2, rec.pos[3], rec.pos[4], rec.pos[5]);
 }
 
@@ -881,7 +915,7 @@ __global__ __launch_bounds__(kKinThreads) void k_kinetic_tiles(const struct pote
     const int32_t end = min(st.off + kScanTile, c.count);
     int32_t nb = 0;
     for (int32_t j = st.off + (int32_t)threadIdx.x; j < end; j += kKinThreads) {
-        struct potential rec = srt[c.first + j];
+        struct potential rec = load_rec(srt + c.first + j);
         if (j < c.kept) {
             float ke = 0.0f;
             for (int k = 0; k < 3; k++) {
@@ -896,7 +930,7 @@ __global__ __launch_bounds__(kKinThreads) void k_kinetic_tiles(const struct pote
             if (bound) bound[c.first + j] = b ? 1 : 0;
             nb += b ? 1 : 0;
         }
-        out[c.first + j] = rec;
+        store_rec(out + c.first + j, rec);
     }
     __shared__ int32_t s_nb[kKinThreads / kWave];
     nb = wave_sum(nb);
