@@ -13,7 +13,7 @@ bound = 0 and then bound = 1 as src/properties.c:552-553 does).
   strip, sort by r2, calculate_corevel, compute_kinetic_energy) against the
   reference's own functions run in the same sequence (ref_bound_pass):
   kept counts, core / bulk velocity, their errors, n_core and ke bit for
-  bit, the sorted order wherever r2 has no ties, pe to f64 rounding.
+  bit, the sorted order (r2 ties included), pe to f64 rounding.
 * The product's hp_mass_definitions (host code in libhalopot) against the
   reference's calc_mass_definition (src/groupies.c:102-142), exactly, and
   against closed forms at a = 1.
