@@ -106,6 +106,11 @@ __device__ __forceinline__ bool accept_exact(float a0, float a1, float a2, const
 
 __device__ __forceinline__ float rsq(float x) { return __builtin_amdgcn_rsqf(x); }
 
+// v[l] = x for the lane l (wavefront-uniform); no lane for l outside [0, 64).
+__device__ __forceinline__ void write_lane(int32_t &v, int32_t x, int32_t l) {
+    v = lane_id() == l ? x : v;
+}
+
 struct WalkArgs {
     const Node *nodes;
     const float4 *fin;
@@ -273,17 +278,12 @@ struct Walker {
         int32_t sp = 0;
         uint64_t m = live;
         int32_t c = rn.left;
-        int32_t guard = 0, fail = 0;
-        bool deep = false;
-        for (;;) {
-            if (++guard > cap) {
-                fail = kErrWalkCap;
-                break;
-            }
-            if ((uint32_t)c >= (uint32_t)(nnodes - 1)) {
-                fail = kErrBadTree;
-                break;
-            }
+        int32_t guard = cap, fail = 0;
+        while (c >= 0) {
+            if (sp > kWave) break;  // a pending pair was lost: the caller walks node by node
+            --guard;
+            fail = (uint32_t)c >= (uint32_t)(nnodes - 1) ? kErrBadTree : (guard < 0 ? kErrWalkCap : 0);
+            if (fail != 0) break;
             const NodeHead n0 = load_const(reinterpret_cast<const NodeHead *>(nodes + c));
             const NodeHead n1 = load_const(reinterpret_cast<const NodeHead *>(nodes + c + 1));
             // both loads are issued before either node is used
@@ -291,30 +291,27 @@ struct Walker {
             const uint64_t m0 = visit(n0, m);
             const uint64_t m1 = visit(n1, m);
             if (m1) {
-                if (sp == kWave) {
-                    deep = true;
-                    break;
-                }
-                const bool here = lane == sp;
-                st_c = here ? n1.left : st_c;
-                st_lo = here ? (int32_t)(uint32_t)m1 : st_lo;
-                st_hi = here ? (int32_t)(uint32_t)(m1 >> 32) : st_hi;
+                // entry sp of the stack: lane sp of each register (a lane
+                // past 63 selects none, and the check above ends the walk)
+                write_lane(st_c, n1.left, sp);
+                write_lane(st_lo, (int32_t)(uint32_t)m1, sp);
+                write_lane(st_hi, (int32_t)(uint32_t)(m1 >> 32), sp);
                 sp++;
             }
             if (m0) {
                 c = n0.left;
                 m = m0;
-            } else if (sp > 0) {
+            } else if (sp > 0 && sp <= kWave) {
                 sp--;
                 c = __builtin_amdgcn_readlane(st_c, sp);
                 m = ((uint64_t)(uint32_t)__builtin_amdgcn_readlane(st_hi, sp) << 32) |
                     (uint32_t)__builtin_amdgcn_readlane(st_lo, sp);
             } else {
-                break;
+                c = -1;
             }
         }
         if (fail && lane == 0) atomicOr(err, fail);
-        return !deep;
+        return sp <= kWave;
     }
 
     // Node-by-node walk.  The wavefront visits the union of the lanes'
