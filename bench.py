@@ -511,8 +511,9 @@ def main() -> int:
     if not args.no_tune:
         restore()
         tuning = plan.tune(d_po.address, reps=2)
-        log(f"autotuner: big threshold {tuning['big_threshold']}, wavefront subtrees up to "
-            f"{tuning['wave_max']} points ({tuning['ms_potential']:.3f} ms per potential pass)")
+        log(f"autotuner, per call-size bucket (<= 1024, 8192, 65536, more records): big "
+            f"threshold {tuning['big_threshold']}, wavefront subtrees up to {tuning['wave_max']} "
+            f"points ({tuning['ms_potential']:.3f} ms per potential pass)")
     for i in range(args.warmup):
         restore()
         plan.run(d_po.address, d_h.address, d_b.address)

@@ -184,11 +184,17 @@ int hp_plan_destroy(hp_plan *plan);
 
 /* Launch settings an autotuned plan uses.  None of them changes a tree, a
  * walk decision or a sum: they only move work between kernels. */
+/* The launch setting is chosen per call-size bucket: calls of at most
+ * 1,024, 8,192, 65,536 and more records. */
+#define HP_TUNE_BUCKETS 4
+
 typedef struct hp_tuning {
-    int32_t big_threshold;  /* nodes above this many points are split by the
-                               level passes (128, 256, 512 or 1024) */
-    int32_t wave_max;       /* subtrees of at most this many points are built
-                               by one wavefront (32 or 64) */
+    int32_t big_threshold[HP_TUNE_BUCKETS];  /* per bucket: nodes above this many
+                                                points are split by the level
+                                                passes (128, 256, 512 or 1024) */
+    int32_t wave_max[HP_TUNE_BUCKETS];       /* per bucket: subtrees of at most
+                                                this many points are built by
+                                                one wavefront (32 or 64) */
     int32_t direct_max;     /* HP_METHOD_AUTO: calls of at most this many
                                points use the direct kernel (0 or 10) */
     float   ms_potential;   /* the chosen setting's time for the potential */
@@ -196,15 +202,18 @@ typedef struct hp_tuning {
 
 /* Autotuner: runs the plan's potential pass (compute_potential of every
  * call, pe overwritten) reps times per candidate setting on the caller's
- * device records and keeps the fastest for later runs of the plan.  The
- * chosen setting is returned in *out when out is not NULL.  Per-call plans
- * are not tuned (HP_EINVAL). */
+ * device records and keeps the fastest for later runs of the plan: first one
+ * setting for all buckets, then each bucket that holds calls in turn with
+ * the others fixed.  No setting changes a tree, a decision or a sum order
+ * of the reference's.  The chosen setting is returned in *out when out is
+ * not NULL.  Per-call plans are not tuned (HP_EINVAL). */
 int hp_plan_tune(hp_plan *plan, struct potential *d_po, int reps, hp_tuning *out);
 
 /* Applies a setting hp_plan_tune returned (for this plan or another plan of
- * the same kind of batch) without timing anything: big_threshold one of 128,
- * 256, 512, 1024; wave_max 32 or 64; direct_max as hp_plan_tune allows it.
- * HP_EINVAL for any other value and for per-call plans. */
+ * the same kind of batch) without timing anything: each bucket's
+ * big_threshold one of 128, 256, 512, 1024 and wave_max 32 or 64;
+ * direct_max as hp_plan_tune allows it.  HP_EINVAL for any other value and
+ * for per-call plans. */
 int hp_plan_set_tuning(hp_plan *plan, const hp_tuning *tuning);
 
 /* ---------------------------------------------------- record assembly */

@@ -181,8 +181,11 @@ struct BuildArgs {
     int2 *tiles;            // append-only (task index, offset)
     int32_t *fat;           // leaves of more than kFatLeaf points
     int32_t *counters;
-    int32_t big_threshold;
-    int32_t wave_max;       // subtrees of at most this many points: one wavefront (32..64)
+    // per call-size bucket (tune_bucket): nodes above bucket_big points are
+    // split by the level passes; subtrees of at most bucket_wave points are
+    // built by one wavefront (32 or 64)
+    int32_t bucket_big[kTuneBuckets];
+    int32_t bucket_wave[kTuneBuckets];
     int32_t cap_small, cap_group, cap_big, cap_tiles, cap_nodes, cap_fat;
 };
 
@@ -215,14 +218,24 @@ __device__ Task make_task(int32_t id, const float *lo, const float *hi, int32_t 
     return t;
 }
 
-// Subtree tasks: a node of more than 10 and at most wave_max points is built
-// by one wavefront, a larger one of at most big_threshold points by one
-// workgroup.  Each kind has its own list.
-__device__ __forceinline__ bool wave_sized(const BuildArgs &a, int32_t n) {
-    return n > kLeafPoints && n <= a.wave_max && n <= a.big_threshold;
+// The launch setting of a call's nodes (its size bucket's).
+struct Sizes {
+    int32_t big;   // nodes above this many points: split levels
+    int32_t wave;  // subtrees of at most this many points: one wavefront
+};
+__device__ __forceinline__ Sizes sizes_of(const BuildArgs &a, int32_t bucket) {
+    bucket = min(max(bucket, 0), kTuneBuckets - 1);
+    return Sizes{a.bucket_big[bucket], a.bucket_wave[bucket]};
 }
-__device__ __forceinline__ bool group_sized(const BuildArgs &a, int32_t n) {
-    return n > a.wave_max && n <= a.big_threshold;
+
+// Subtree tasks: a node of more than 10 and at most s.wave points is built
+// by one wavefront, a larger one of at most s.big points by one workgroup.
+// Each kind has its own list.
+__device__ __forceinline__ bool wave_sized(const Sizes &s, int32_t n) {
+    return n > kLeafPoints && n <= s.wave && n <= s.big;
+}
+__device__ __forceinline__ bool group_sized(const Sizes &s, int32_t n) {
+    return n > s.wave && n <= s.big;
 }
 
 // Queues node `id` (tight box known) at a slot already allocated in the
@@ -259,10 +272,12 @@ __device__ __forceinline__ uint64_t alloc_big_and_tiles(const BuildArgs &a, int3
                      1ull | ((unsigned long long)ntiles << 32));
 }
 
-// Queues a node of more than big_threshold points with its tiles.
+// Queues a node of more than its bucket's split size with its tiles; the
+// task keeps the bucket (pad[0]) for its children.
 __device__ void push_big(const BuildArgs &a, int32_t id, const float *lo, const float *hi,
-                         int32_t count, int32_t depth) {
+                         int32_t count, int32_t depth, int32_t bucket) {
     Task t = make_task(id, lo, hi, depth);
+    t.pad[0] = bucket;
     const int32_t nt = (count + kTile - 1) / kTile;
     const uint64_t al = alloc_big_and_tiles(a, nt);
     const int32_t bi = (int32_t)al, t0 = (int32_t)(al >> 32);
@@ -279,7 +294,7 @@ __device__ void push_big(const BuildArgs &a, int32_t id, const float *lo, const 
 // push_big made by a whole wavefront with wavefront-uniform arguments: lane 0
 // reserves the slots, the lanes write the tiles.
 __device__ void push_big_wave(const BuildArgs &a, int32_t id, const float *lo, const float *hi,
-                              int32_t count, int32_t depth) {
+                              int32_t count, int32_t depth, int32_t bucket) {
     const int lane = lane_id();
     const int32_t nt = (count + kTile - 1) / kTile;
     uint64_t al = 0;
@@ -294,6 +309,7 @@ __device__ void push_big_wave(const BuildArgs &a, int32_t id, const float *lo, c
         Task t = make_task(id, lo, hi, depth);
         t.tile0 = t0;
         t.ntiles = nt;
+        t.pad[0] = bucket;
         a.big[bi] = t;
     }
     for (int32_t j = lane; j < nt; j += kWave) a.tiles[t0 + j] = make_int2(bi, j * kTile);
@@ -330,7 +346,7 @@ __global__ __launch_bounds__(256) void k_roots(const int64_t *__restrict__ hfirs
                                                BuildArgs a) {
     const int h = blockIdx.x * blockDim.x + threadIdx.x;
     const bool v = h < nhalos;
-    int32_t first = 0, nfin = 0;
+    int32_t first = 0, nfin = 0, bucket = 0;
     bool isbad = false;
     if (v) {
         first = (int32_t)hfirst[h];
@@ -338,9 +354,11 @@ __global__ __launch_bounds__(256) void k_roots(const int64_t *__restrict__ hfirs
         const int32_t nbad = bad[h];
         nfin = count - nbad;
         isbad = nbad > 0;
+        bucket = tune_bucket(count);
     }
     const bool ok = v && !isbad;
-    const bool wsmall = ok && wave_sized(a, nfin), gsmall = ok && group_sized(a, nfin);
+    const Sizes sz = sizes_of(a, bucket);
+    const bool wsmall = ok && wave_sized(sz, nfin), gsmall = ok && group_sized(sz, nfin);
     const uint64_t ms = ballot(wsmall), mg = ballot(gsmall);
     int32_t slot0 = 0, gslot0 = 0;
     if (lane_id() == 0) {
@@ -367,7 +385,7 @@ __global__ __launch_bounds__(256) void k_roots(const int64_t *__restrict__ hfirs
     }
     write_node(a.nodes + h, lo, hi, first, nfin, -1);
     a.parent[h] = -1;
-    if (nfin > a.big_threshold) push_big(a, h, lo, hi, nfin, 0);
+    if (nfin > sz.big) push_big(a, h, lo, hi, nfin, 0, bucket);
     else if (wsmall) push_small_at(a, slot0 + rank_below(ms), h, lo, hi, 0, first, nfin);
     else if (gsmall) push_group_at(a, gslot0 + rank_below(mg), h, lo, hi, 0, first, nfin);
 }
@@ -450,12 +468,14 @@ __global__ __launch_bounds__(256) void k_roots_bad(const int64_t *__restrict__ h
         if (threadIdx.x == 0) {
             write_node(a.nodes + h, lo, hi, first, nfin, -1);
             a.parent[h] = -1;
-            if (nfin > a.big_threshold) {
-                push_big(a, h, lo, hi, nfin, 0);
-            } else if (wave_sized(a, nfin)) {
+            const int32_t bucket = tune_bucket((int32_t)hcount[h]);
+            const Sizes sz = sizes_of(a, bucket);
+            if (nfin > sz.big) {
+                push_big(a, h, lo, hi, nfin, 0, bucket);
+            } else if (wave_sized(sz, nfin)) {
                 const uint64_t old = alloc_nodes_and_small(a, 0, 1);
                 push_small_at(a, (int32_t)(old >> 32), h, lo, hi, 0, first, nfin);
-            } else if (group_sized(a, nfin)) {
+            } else if (group_sized(sz, nfin)) {
                 push_group_at(a, atomicAdd(&a.counters[kGroupNext], 1), h, lo, hi, 0, first, nfin);
             }
         }
@@ -474,25 +494,26 @@ __device__ void make_children(const BuildArgs &a, const Task &t, const Node &nd,
                               int32_t P, int32_t G, int32_t nl, const float *loL, const float *hiL,
                               const float *loR, const float *hiR) {
     const int32_t nr = nd.count - nl;
+    const Sizes sz = sizes_of(a, t.pad[0]);
     if (lane_id() == 0) {
         write_node(a.nodes + L, loL, hiL, nd.first, nl, L + 1);
         write_node(a.nodes + L + 1, loR, hiR, nd.first + nl, nr, nd.skip);
         a.parent[L] = a.parent[L + 1] = t.node;
         a.nodes[t.node].left = L;
-        if (wave_sized(a, nl)) push_small_at(a, P++, L, loL, hiL, t.depth + 1, nd.first, nl);
-        if (group_sized(a, nl)) push_group_at(a, G++, L, loL, hiL, t.depth + 1, nd.first, nl);
-        if (wave_sized(a, nr)) push_small_at(a, P, L + 1, loR, hiR, t.depth + 1, nd.first + nl, nr);
-        if (group_sized(a, nr)) push_group_at(a, G, L + 1, loR, hiR, t.depth + 1, nd.first + nl, nr);
+        if (wave_sized(sz, nl)) push_small_at(a, P++, L, loL, hiL, t.depth + 1, nd.first, nl);
+        if (group_sized(sz, nl)) push_group_at(a, G++, L, loL, hiL, t.depth + 1, nd.first, nl);
+        if (wave_sized(sz, nr)) push_small_at(a, P, L + 1, loR, hiR, t.depth + 1, nd.first + nl, nr);
+        if (group_sized(sz, nr)) push_group_at(a, G, L + 1, loR, hiR, t.depth + 1, nd.first + nl, nr);
     }
-    if (nl > a.big_threshold) push_big_wave(a, L, loL, hiL, nl, t.depth + 1);
-    if (nr > a.big_threshold) push_big_wave(a, L + 1, loR, hiR, nr, t.depth + 1);
+    if (nl > sz.big) push_big_wave(a, L, loL, hiL, nl, t.depth + 1, t.pad[0]);
+    if (nr > sz.big) push_big_wave(a, L + 1, loR, hiR, nr, t.depth + 1, t.pad[0]);
 }
 
-__device__ __forceinline__ int32_t wave_pushes(const BuildArgs &a, int32_t nl, int32_t nr) {
-    return (wave_sized(a, nl) ? 1 : 0) + (wave_sized(a, nr) ? 1 : 0);
+__device__ __forceinline__ int32_t wave_pushes(const Sizes &s, int32_t nl, int32_t nr) {
+    return (wave_sized(s, nl) ? 1 : 0) + (wave_sized(s, nr) ? 1 : 0);
 }
-__device__ __forceinline__ int32_t group_pushes(const BuildArgs &a, int32_t nl, int32_t nr) {
-    return (group_sized(a, nl) ? 1 : 0) + (group_sized(a, nr) ? 1 : 0);
+__device__ __forceinline__ int32_t group_pushes(const Sizes &s, int32_t nl, int32_t nr) {
+    return (group_sized(s, nl) ? 1 : 0) + (group_sized(s, nr) ? 1 : 0);
 }
 
 // ------------------------------------------------------------- subtrees
@@ -1430,8 +1451,9 @@ __global__ __launch_bounds__(256) void k_big_split(const Task *__restrict__ big,
             uint64_t al = 0;
             int32_t G = 0;
             if (lane == 0) {
-                al = alloc_nodes_and_small(a, 2, wave_pushes(a, nl, nr));
-                const int32_t ng = group_pushes(a, nl, nr);
+                const Sizes sz = sizes_of(a, t.pad[0]);
+                al = alloc_nodes_and_small(a, 2, wave_pushes(sz, nl, nr));
+                const int32_t ng = group_pushes(sz, nl, nr);
                 if (ng) G = atomicAdd(&a.counters[kGroupNext], ng);
             }
             al = read_lane(al, 0);

@@ -23,6 +23,12 @@ constexpr int kTile = 1024;       // points per workgroup tile of a big node
 constexpr int kSmallMax = 1024;   // largest subtree finished by one workgroup
 constexpr int kSmallChunks = kSmallMax / 64;
 constexpr int kSrcBatch = 8;      // leaf points the walk reads per scalar batch
+// Call-size buckets of the launch setting: calls of at most 1,024, 8,192,
+// 65,536 and more records (HP_TUNE_BUCKETS in the header).
+constexpr int kTuneBuckets = 4;
+__host__ __device__ __forceinline__ int tune_bucket(int64_t count) {
+    return count <= 1024 ? 0 : (count <= 8192 ? 1 : (count <= 65536 ? 2 : 3));
+}
 constexpr int kFatLeaf = 64;      // leaves above this size get a wavefront
 constexpr int kTileThreads = 256;
 constexpr int kMaxLevels = 4096;  // depth_end entries kept per build

@@ -336,8 +336,11 @@ struct Engine {
     int32_t *h_counters = nullptr;  // pinned mirror
     unsigned long long *h_work = nullptr;
     hipEvent_t ev[8] = {};
-    int32_t big_threshold = kSmallMax;
-    int32_t wave_max = kWaveTask;
+    // launch setting per call-size bucket (tune_bucket)
+    int32_t bucket_big[kTuneBuckets] = {kSmallMax, kSmallMax, kSmallMax, kSmallMax};
+    int32_t bucket_wave[kTuneBuckets] = {kWaveTask, kWaveTask, kWaveTask, kWaveTask};
+    int32_t big_max() const { return *std::max_element(bucket_big, bucket_big + kTuneBuckets); }
+    int32_t big_min() const { return *std::min_element(bucket_big, bucket_big + kTuneBuckets); }
     int xcds = 1;             // XCDs of the device (each with its own L2)
     int32_t walk_groups = 0;  // entries of the loaded walk table (XCD layout)
     int sub_grid[kSubClasses] = {}, wave_grid = 0;  // resident workgroups of the subtree kernels
@@ -403,9 +406,10 @@ struct Engine {
         sub_grid[2] = std::max(1, per_cu) * prop.multiProcessorCount;
         HP_TRY(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_subtree_wave, kWave, 0));
         wave_grid = std::max(1, per_cu) * prop.multiProcessorCount;
-        if (const char *s = getenv("HALOPOT_BIG_THRESHOLD")) big_threshold = atoi(s);
-        if (big_threshold > kSmallMax) big_threshold = kSmallMax;
-        if (big_threshold < kLeafPoints) big_threshold = kLeafPoints;
+        if (const char *s = getenv("HALOPOT_BIG_THRESHOLD")) {
+            const int32_t bt = std::min(std::max(atoi(s), kLeafPoints), kSmallMax);
+            for (int b = 0; b < kTuneBuckets; b++) bucket_big[b] = bt;
+        }
         return HP_OK;
     }
 
@@ -536,7 +540,7 @@ struct Engine {
         cap_group = N / (kWaveMin + 1) + H + 64;  // tasks of more than wave_max >= kWaveMin points
         HP_GRAB(group0, sizeof(Task) * cap_group);
         HP_GRAB(groupc, sizeof(Task) * kSubClasses * cap_group);
-        const int64_t bt = big_threshold > 0 ? big_threshold : 1;
+        const int64_t bt = std::max(big_min(), 1);
         cap_big = 128 * (N / bt + 1) + H + 64;
         cap_tiles = 128 * (N / kTile + 2) + cap_big;
         HP_GRAB(big, sizeof(Task) * cap_big);
@@ -632,8 +636,10 @@ struct Engine {
         a.tiles = tiles.as<int2>();
         a.fat = fat.as<int32_t>();
         a.counters = counters.as<int32_t>();
-        a.big_threshold = big_threshold;
-        a.wave_max = wave_max;
+        for (int b = 0; b < kTuneBuckets; b++) {
+            a.bucket_big[b] = bucket_big[b];
+            a.bucket_wave[b] = bucket_wave[b];
+        }
         a.cap_small = (int32_t)cap_small;
         a.cap_group = (int32_t)cap_group;
         a.cap_big = (int32_t)cap_big;
@@ -724,12 +730,12 @@ struct Engine {
         k_group_classes<<<blocks(cap_group, 256), 256, 0, s>>>(group0.as<Task>(), ba, groupc.as<Task>());
         HP_LAUNCHED();
         // largest first: the big subtrees hold most points and the fewest workgroups per CU
-        if (big_threshold > 512) {
+        if (big_max() > 512) {
             k_subtree<1024><<<sub_grid[2], kSubThreads, 0, s>>>(
                 A.as<float4>(), B.as<float4>(), groupc.as<Task>() + 2 * cap_group, ba, PARTICLE_MASS, 2);
             HP_LAUNCHED();
         }
-        if (big_threshold > 256) {
+        if (big_max() > 256) {
             k_subtree<512><<<sub_grid[1], kSubThreads, 0, s>>>(
                 A.as<float4>(), B.as<float4>(), groupc.as<Task>() + cap_group, ba, PARTICLE_MASS, 1);
             HP_LAUNCHED();
@@ -1678,8 +1684,10 @@ __attribute__((visibility("default"))) int hp_plan_create(hp_plan **out, const h
 static int apply_tuning(hp_plan *p, const hp_tuning &t) {
     Engine &e = p->eng;
     int r;
-    e.big_threshold = t.big_threshold;
-    e.wave_max = t.wave_max;
+    for (int b = 0; b < kTuneBuckets; b++) {
+        e.bucket_big[b] = t.big_threshold[b];
+        e.bucket_wave[b] = t.wave_max[b];
+    }
     e.drop_graphs();
     if (p->opt.method == HP_METHOD_AUTO && t.direct_max != p->T.direct_max) {
         if ((r = make_tables(p->T, p->first.data(), p->count.data(), (int64_t)p->first.size(),
@@ -1692,11 +1700,13 @@ static int apply_tuning(hp_plan *p, const hp_tuning &t) {
     return HP_OK;
 }
 
-// Times the plan's potential pass on the caller's records for each setting
-// that changes no tree, decision or sum, and keeps the fastest: the size
-// above which nodes are split by the level passes, the size up to which a
-// subtree is built by one wavefront and, for HP_METHOD_AUTO plans, whether
-// calls of at most 10 points use the direct kernel.
+// Times the plan's potential pass on the caller's records for settings that
+// change no tree, decision or sum, and keeps the fastest: per call-size
+// bucket, the size above which nodes are split by the level passes and the
+// size up to which a subtree is built by one wavefront; for HP_METHOD_AUTO
+// plans, whether calls of at most 10 points use the direct kernel.  First
+// one setting for every bucket, then each bucket that holds calls in turn,
+// the others fixed.
 __attribute__((visibility("default"))) int hp_plan_tune(hp_plan *p, struct potential *d_po,
                                                         int reps, hp_tuning *out) {
     if (!p || !d_po || reps < 1) return set_error(HP_EINVAL, "hp_plan_tune: bad arguments");
@@ -1709,10 +1719,30 @@ __attribute__((visibility("default"))) int hp_plan_tune(hp_plan *p, struct poten
     const int32_t wms[] = {kWaveMin, kWaveTask};
     const int32_t dms[] = {kLeafPoints, 0};
     const int ndm = p->opt.method == HP_METHOD_AUTO ? 2 : 1;
-    hp_tuning best{};
-    float best_ms = -1.0f;
     int r;
     hipEvent_t e0 = e.ev[5], e1 = e.ev[6];
+    // the potential pass under setting t (its direct crossover already loaded)
+    auto time_setting = [&](const hp_tuning &t, float *ms) -> int {
+        for (int b = 0; b < kTuneBuckets; b++) {
+            e.bucket_big[b] = t.big_threshold[b];
+            e.bucket_wave[b] = t.wave_max[b];
+        }
+        e.drop_graphs();
+        int rr;
+        if ((rr = e.reserve(p->T, true)) || (rr = e.potential(p->T, d_po, o))) return rr;
+        HP_TRY(hipEventRecord(e0, e.stream));
+        for (int k = 0; k < reps; k++)
+            if ((rr = e.potential(p->T, d_po, o))) return rr;
+        HP_TRY(hipEventRecord(e1, e.stream));
+        HP_TRY(hipEventSynchronize(e1));
+        HP_TRY(hipEventElapsedTime(ms, e0, e1));
+        *ms /= (float)reps;
+        return HP_OK;
+    };
+    bool used[kTuneBuckets] = {};
+    for (int64_t c : p->T.count) used[tune_bucket(c)] = true;
+    hp_tuning best{};
+    float best_ms = -1.0f;
     for (int di = 0; di < ndm; di++) {
         if (ndm > 1) {
             if ((r = make_tables(p->T, p->first.data(), p->count.data(), (int64_t)p->first.size(),
@@ -1720,30 +1750,45 @@ __attribute__((visibility("default"))) int hp_plan_tune(hp_plan *p, struct poten
                 (r = e.load_tables(p->T)))
                 return r;
         }
-        for (int32_t bt : bts) {
-            if (bt > kSmallMax) continue;
+        for (int32_t bt : bts)
             for (int32_t wm : wms) {
-                e.big_threshold = bt;
-                e.wave_max = wm;
-                e.drop_graphs();
-                if ((r = e.reserve(p->T, true)) || (r = e.potential(p->T, d_po, o))) return r;
-                HP_TRY(hipEventRecord(e0, e.stream));
-                for (int k = 0; k < reps; k++)
-                    if ((r = e.potential(p->T, d_po, o))) return r;
-                HP_TRY(hipEventRecord(e1, e.stream));
-                HP_TRY(hipEventSynchronize(e1));
+                hp_tuning t{};
+                for (int b = 0; b < kTuneBuckets; b++) {
+                    t.big_threshold[b] = bt;
+                    t.wave_max[b] = wm;
+                }
+                t.direct_max = ndm > 1 ? dms[di] : p->T.direct_max;
                 float ms = 0.0f;
-                HP_TRY(hipEventElapsedTime(&ms, e0, e1));
-                ms /= (float)reps;
+                if ((r = time_setting(t, &ms))) return r;
                 if (best_ms < 0.0f || ms < best_ms) {
                     best_ms = ms;
-                    best.big_threshold = bt;
-                    best.wave_max = wm;
-                    best.direct_max = ndm > 1 ? dms[di] : p->T.direct_max;
+                    best = t;
                     best.ms_potential = ms;
                 }
             }
-        }
+    }
+    if (ndm > 1 && best.direct_max != p->T.direct_max) {
+        if ((r = make_tables(p->T, p->first.data(), p->count.data(), (int64_t)p->first.size(),
+                             p->opt.method, true, best.direct_max)) ||
+            (r = e.load_tables(p->T)))
+            return r;
+    }
+    for (int b = 0; b < kTuneBuckets; b++) {
+        if (!used[b]) continue;
+        for (int32_t bt : bts)
+            for (int32_t wm : wms) {
+                if (bt == best.big_threshold[b] && wm == best.wave_max[b]) continue;
+                hp_tuning t = best;
+                t.big_threshold[b] = bt;
+                t.wave_max[b] = wm;
+                float ms = 0.0f;
+                if ((r = time_setting(t, &ms))) return r;
+                if (ms < best_ms) {
+                    best_ms = ms;
+                    best = t;
+                    best.ms_potential = ms;
+                }
+            }
     }
     if ((r = apply_tuning(p, best))) return r;
     if (out) *out = best;
@@ -1753,10 +1798,13 @@ __attribute__((visibility("default"))) int hp_plan_tune(hp_plan *p, struct poten
 __attribute__((visibility("default"))) int hp_plan_set_tuning(hp_plan *p, const hp_tuning *t) {
     if (!p || !t) return set_error(HP_EINVAL, "hp_plan_set_tuning: bad arguments");
     if (p->opt.per_call) return set_error(HP_EINVAL, "hp_plan_set_tuning: per-call plans are not tuned");
-    const int32_t bt = t->big_threshold, wm = t->wave_max, dm = t->direct_max;
-    if ((bt != 128 && bt != 256 && bt != 512 && bt != 1024) || (wm != kWaveMin && wm != kWaveTask))
-        return set_error(HP_EINVAL, "hp_plan_set_tuning: big_threshold %d / wave_max %d is not a "
-                         "setting of the tuner", bt, wm);
+    for (int b = 0; b < kTuneBuckets; b++) {
+        const int32_t bt = t->big_threshold[b], wm = t->wave_max[b];
+        if ((bt != 128 && bt != 256 && bt != 512 && bt != 1024) || (wm != kWaveMin && wm != kWaveTask))
+            return set_error(HP_EINVAL, "hp_plan_set_tuning: bucket %d: big_threshold %d / wave_max "
+                             "%d is not a setting of the tuner", b, bt, wm);
+    }
+    const int32_t dm = t->direct_max;
     if (p->opt.method == HP_METHOD_AUTO ? (dm != 0 && dm != kLeafPoints) : dm != p->T.direct_max)
         return set_error(HP_EINVAL, "hp_plan_set_tuning: direct_max %d not allowed for this plan",
                          dm);

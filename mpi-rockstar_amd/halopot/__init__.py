@@ -77,12 +77,25 @@ class MassDefs(C.Structure):
         return np.array(list(self.thresh_dens) + [self.rvir_dens, self.rvir_dens_z0], np.float64)
 
 
+TUNE_BUCKETS = 4
+TUNE_BUCKET_MAX = (1024, 8192, 65536)  # calls of at most this many records; the last: more
+
+
+def tune_bucket(count: int) -> int:
+    """The call-size bucket of a call of `count` records."""
+    for b, m in enumerate(TUNE_BUCKET_MAX):
+        if count <= m:
+            return b
+    return len(TUNE_BUCKET_MAX)
+
+
 class Tuning(C.Structure):
-    _fields_ = [("big_threshold", C.c_int32), ("wave_max", C.c_int32),
+    _fields_ = [("big_threshold", C.c_int32 * TUNE_BUCKETS), ("wave_max", C.c_int32 * TUNE_BUCKETS),
                 ("direct_max", C.c_int32), ("ms_potential", C.c_float)]
 
     def as_dict(self):
-        return {f: getattr(self, f) for f, _ in self._fields_}
+        return {"big_threshold": list(self.big_threshold), "wave_max": list(self.wave_max),
+                "direct_max": self.direct_max, "ms_potential": self.ms_potential}
 
 
 _LIB = None
@@ -333,16 +346,25 @@ class Plan:
                                     C.byref(self.opts)))
 
     def tune(self, d_po: int, reps: int = 3) -> dict:
-        """hp_plan_tune: times the potential pass (pe overwritten) for each
-        result-neutral launch setting and keeps the fastest."""
+        """hp_plan_tune: times the potential pass (pe overwritten) for the
+        result-neutral launch settings, per call-size bucket, and keeps the
+        fastest.  Returns the per-bucket choice (lists indexed by bucket)."""
         t = Tuning()
         _check(lib().hp_plan_tune(self._h, C.c_void_p(d_po), int(reps), C.byref(t)))
         return t.as_dict()
 
-    def set_tuning(self, big_threshold: int, wave_max: int, direct_max: int | None = None) -> None:
-        """hp_plan_set_tuning: one of the tuner's settings, applied untimed."""
+    def set_tuning(self, big_threshold, wave_max, direct_max: int | None = None) -> None:
+        """hp_plan_set_tuning: one of the tuner's settings, applied untimed.
+        big_threshold and wave_max: one value for every bucket, or one per
+        bucket."""
+        def per_bucket(v):
+            v = list(v) if hasattr(v, "__len__") else [v] * TUNE_BUCKETS
+            if len(v) != TUNE_BUCKETS:
+                raise ValueError(f"{TUNE_BUCKETS} values, one per bucket")
+            return v
         t = Tuning()
-        t.big_threshold, t.wave_max = big_threshold, wave_max
+        t.big_threshold[:] = per_bucket(big_threshold)
+        t.wave_max[:] = per_bucket(wave_max)
         t.direct_max = 10 if direct_max is None else direct_max
         _check(lib().hp_plan_set_tuning(self._h, C.byref(t)))
 
