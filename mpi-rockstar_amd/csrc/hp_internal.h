@@ -131,7 +131,11 @@ enum Counter {
     kNumCounters = 25
 };
 
-enum ErrFlag { kErrWalkCap = 1, kErrBadTree = 2, kErrCapacity = 4 };
+// kErrDeep: the split levels recorded in a graph did not finish the trees;
+// every kernel that writes the caller's records, halos or bound flags then
+// returns at once, so the caller's buffers are untouched and the step can be
+// run again.
+enum ErrFlag { kErrWalkCap = 1, kErrBadTree = 2, kErrCapacity = 4, kErrDeep = 8 };
 
 // ----------------------------------------------------------- wave helpers
 

@@ -335,6 +335,8 @@ struct Engine {
     hipStream_t stream = nullptr;
     int32_t *h_counters = nullptr;  // pinned mirror
     unsigned long long *h_work = nullptr;
+    uint64_t *h_stamps = nullptr;   // pinned: wall-clock stamps between a graph step's phases
+    double wall_khz = 1.0e5;        // the device wall clock's rate
     hipEvent_t ev[8] = {};
     // launch setting per call-size bucket (tune_bucket)
     int32_t bucket_big[kTuneBuckets] = {kSmallMax, kSmallMax, kSmallMax, kSmallMax};
@@ -393,6 +395,13 @@ struct Engine {
         if (const char *s = getenv("HALOPOT_XCDS")) xcds = std::max(1, atoi(s));
         HP_TRY(hipHostMalloc((void **)&h_counters, sizeof(int32_t) * kNumCounters));
         HP_TRY(hipHostMalloc((void **)&h_work, sizeof(unsigned long long) * 4));
+        HP_TRY(hipHostMalloc((void **)&h_stamps, sizeof(uint64_t) * 8));
+        int khz = 0;
+        if (hipDeviceGetAttribute(&khz, hipDeviceAttributeWallClockRate, device) == hipSuccess &&
+            khz > 0)
+            wall_khz = (double)khz;
+        else
+            (void)hipGetLastError();
         for (auto &e : ev) HP_TRY(hipEventCreate(&e));
         // The workgroup subtree kernels take tasks from dynamic queues, the
         // wavefront one at a fixed stride, until the lists are empty; their
@@ -501,6 +510,7 @@ struct Engine {
                 if (e) (void)hipEventDestroy(e);
             if (h_counters) (void)hipHostFree(h_counters);
             if (h_work) (void)hipHostFree(h_work);
+            if (h_stamps) (void)hipHostFree(h_stamps);
             (void)hipStreamDestroy(stream);
         }
     }
@@ -819,6 +829,7 @@ struct Engine {
             d.fin = T.method == HP_METHOD_DIRECT ? A.as<float4>() : F.as<float4>();
             d.blocks = direct_blocks.as<int2>();
             d.nblocks = (int32_t)T.direct_blocks.size();
+            d.flags = counters.as<int32_t>() + kErrFlags;
             d.pm = (float)PARTICLE_MASS;
             d.inv_fr = (float)(1.0 / FORCE_RES);
             d.pm_d = PARTICLE_MASS;
@@ -828,7 +839,8 @@ struct Engine {
                 k_zero_pe<<<blocks(T.nrec, 256), 256, 0, s>>>(d_po, T.nrec);
             } else {
                 k_zero_pe_calls<<<(int)T.direct_blocks.size(), 256, 0, s>>>(
-                    d_po, hfirst.as<int64_t>(), hcount.as<int64_t>(), direct_blocks.as<int2>());
+                    d_po, hfirst.as<int64_t>(), hcount.as<int64_t>(), direct_blocks.as<int2>(),
+                    counters.as<int32_t>() + kErrFlags);
             }
             HP_LAUNCHED();
             const int nbk = (int)T.direct_blocks.size();
@@ -956,19 +968,18 @@ struct Engine {
     };
 
     struct Graphs {
-        hipGraphExec_t build = nullptr, multipole = nullptr, walk = nullptr, unbind = nullptr;
-        int levels = 0;  // split levels recorded in `build`
+        hipGraphExec_t step = nullptr;  // the whole hp_plan_run, one launch
+        int levels = 0;  // split levels recorded in `step`
         const void *po = nullptr, *halos = nullptr, *bound = nullptr;
         Config cfg{};    // the globals the graphs were recorded with
         bool ok = false;
     } g;
 
     void drop_graphs() {
-        for (hipGraphExec_t *e : {&g.build, &g.multipole, &g.walk, &g.unbind})
-            if (*e) {
-                (void)hipGraphExecDestroy(*e);
-                *e = nullptr;
-            }
+        if (g.step) {
+            (void)hipGraphExecDestroy(g.step);
+            g.step = nullptr;
+        }
         g.ok = false;
     }
 
@@ -991,20 +1002,41 @@ struct Engine {
         return HP_OK;
     }
 
-    // Records the four stages for this batch and these buffers, with the
-    // number of split levels the last eager build needed.
+    // Records the whole step as one graph: the roots, `levels` split levels
+    // (the depth learned eagerly plus kLevelMargin; levels with nothing left
+    // to split cost their empty launches only), the depth check, the
+    // multipoles, the walk and the bound pass, with a wall-clock stamp
+    // before, between and after the phases (events recorded inside a graph
+    // are not usable for elapsed times once the graph is recorded again).
+    static constexpr int kLevelMargin = 2;
     int capture_all(const Tables &T, struct potential *d_po, hp_halo *d_halos, uint8_t *d_bound,
                     const hp_options &o, int levels) {
         drop_graphs();
         int r;
-        if ((r = capture(&g.build, [&] {
-                 int q = enqueue_roots(T, d_po);
-                 return q ? q : enqueue_levels(1, levels);
+        const bool tree = T.method != HP_METHOD_DIRECT;
+        if ((r = capture(&g.step, [&]() -> int {
+                 int q;
+                 k_stamp<<<1, 1, 0, stream>>>(h_stamps, 0);
+                 HP_LAUNCHED();
+                 if ((q = enqueue_roots(T, d_po))) return q;
+                 if (tree) {
+                     if ((q = enqueue_levels(1, levels))) return q;
+                     k_depth_check<<<1, 1, 0, stream>>>(counters.as<int32_t>());
+                     HP_LAUNCHED();
+                 }
+                 k_stamp<<<1, 1, 0, stream>>>(h_stamps, 1);
+                 HP_LAUNCHED();
+                 if (tree && (q = enqueue_multipole(levels))) return q;
+                 k_stamp<<<1, 1, 0, stream>>>(h_stamps, 2);
+                 HP_LAUNCHED();
+                 if ((q = enqueue_walk(T, d_po, o))) return q;
+                 k_stamp<<<1, 1, 0, stream>>>(h_stamps, 3);
+                 HP_LAUNCHED();
+                 if ((q = enqueue_unbind(T, d_po, d_halos, d_bound))) return q;
+                 k_stamp<<<1, 1, 0, stream>>>(h_stamps, 4);
+                 HP_LAUNCHED();
+                 return HP_OK;
              })))
-            return r;
-        if ((r = capture(&g.multipole, [&] { return enqueue_multipole(levels); }))) return r;
-        if ((r = capture(&g.walk, [&] { return enqueue_walk(T, d_po, o); }))) return r;
-        if ((r = capture(&g.unbind, [&] { return enqueue_unbind(T, d_po, d_halos, d_bound); })))
             return r;
         g.levels = levels;
         g.cfg = Config::now();
@@ -1015,10 +1047,13 @@ struct Engine {
         return HP_OK;
     }
 
-    // Graph driver of hp_plan_run.  A tree deeper than the recorded levels
-    // is finished eagerly from the device state and the graphs re-recorded
-    // on the next call.  The graphs are recorded again whenever the buffers
-    // or any configuration global they were recorded with changed.
+    // Graph driver of hp_plan_run: one graph launch per step, with no host
+    // read between its phases.  The split levels are driven on the device;
+    // a tree deeper than the recorded levels raises kErrDeep, under which
+    // nothing of the caller's is written, and the step is then run eagerly
+    // (the graph is recorded again on the next call).  The graph is recorded
+    // again whenever the buffers or any configuration global it was
+    // recorded with changed.
     int run_graphs(const Tables &T, struct potential *d_po, hp_halo *d_halos, uint8_t *d_bound,
                    const hp_options &o) {
         hipStream_t s = stream;
@@ -1029,44 +1064,32 @@ struct Engine {
             if ((r = potential(T, d_po, o))) return r;
             if ((r = enqueue_unbind(T, d_po, d_halos, d_bound))) return r;
             HP_TRY(hipStreamSynchronize(s));
-            return capture_all(T, d_po, d_halos, d_bound, o, (int)st.levels + 1);
+            return capture_all(T, d_po, d_halos, d_bound, o, (int)st.levels + 1 + kLevelMargin);
         }
         st = hp_stats();
         st.records = T.nrec;
         st.halos = T.nhalos;
-        HP_TRY(hipEventRecord(ev[0], s));
-        HP_TRY(hipGraphLaunch(g.build, s));
+        HP_TRY(hipGraphLaunch(g.step, s));
         if ((r = read_counters())) return r;
-        bool fresh = true;
-        if (h_counters[kActive] && !h_counters[kErrFlags]) {
-            int next = 0;
-            if ((r = finish_levels(g.levels + 1, &next))) return r;
-            fresh = false;
+        if (h_counters[kErrFlags] & kErrDeep) {
+            g.ok = false;
+            if ((r = potential(T, d_po, o))) return r;
+            if ((r = enqueue_unbind(T, d_po, d_halos, d_bound))) return r;
+            HP_TRY(hipStreamSynchronize(s));
+            return HP_OK;
         }
         if ((r = check_build())) return r;
-        HP_TRY(hipEventRecord(ev[1], s));
-        if (fresh) HP_TRY(hipGraphLaunch(g.multipole, s));
-        else if ((r = enqueue_multipole((int)st.levels))) return r;
-        HP_TRY(hipEventRecord(ev[2], s));
-        HP_TRY(hipGraphLaunch(g.walk, s));
-        HP_TRY(hipEventRecord(ev[3], s));
-        HP_TRY(hipGraphLaunch(g.unbind, s));
-        HP_TRY(hipEventRecord(ev[4], s));
         if ((r = check_walk())) return r;
-        if (!fresh) g.ok = false;  // record again with more levels next time
         st.walk_groups = (int64_t)T.groups.size();
         st.walk_launches = T.groups.empty() ? 0 : 1;
-        float t;
-        HP_TRY(hipEventElapsedTime(&t, ev[0], ev[1]));
-        st.ms_build = t;
-        HP_TRY(hipEventElapsedTime(&t, ev[1], ev[2]));
-        st.ms_multipole = t;
-        HP_TRY(hipEventElapsedTime(&t, ev[2], ev[3]));
-        st.ms_walk = t;
-        HP_TRY(hipEventElapsedTime(&t, ev[3], ev[4]));
-        st.ms_unbind = t;
-        HP_TRY(hipEventElapsedTime(&t, ev[0], ev[4]));
-        st.ms_total = t;
+        auto ms = [&](int i, int j) {
+            return (float)((double)(int64_t)(h_stamps[j] - h_stamps[i]) / wall_khz);
+        };
+        st.ms_build = ms(0, 1);
+        st.ms_multipole = ms(1, 2);
+        st.ms_walk = ms(2, 3);
+        st.ms_unbind = ms(3, 4);
+        st.ms_total = ms(0, 4);
         return HP_OK;
     }
 
@@ -1317,9 +1340,10 @@ struct Engine {
                                              tsum.as<Moments>(), tcand.as<Cand>());
             HP_LAUNCHED();
         }
+        const int32_t *flags = counters.as<int32_t>() + kErrFlags;
         k_vel_finish<<<blocks(H, 256), 256, 0, s>>>(RV, call_tile0.as<int32_t>(),
                                                     call_ntiles.as<int32_t>(), H, ci,
-                                                    tcand.as<Cand>(), d_halos);
+                                                    tcand.as<Cand>(), d_halos, flags);
         HP_LAUNCHED();
         // kinetic energy (src/potential.c:194-211 with scale_now = SCALE_NOW)
         const float a = (float)SCALE_NOW;
@@ -1329,15 +1353,17 @@ struct Engine {
         const double hub_a = hub * a;
         if (nst > 0) {
             k_kinetic_tiles<<<nst, kKinThreads, 0, s>>>(S, d_po, scan_tiles.as<ScanTile>(), nst, ci,
-                                                d_halos, hub_a, conv, d_bound, ti32);
+                                                d_halos, hub_a, conv, d_bound, ti32, flags);
             HP_LAUNCHED();
         }
         if (d_bound) {
-            k_bound_tail<<<H, 256, 0, s>>>(hfirst.as<int64_t>(), hcount.as<int64_t>(), ci, H, d_bound);
+            k_bound_tail<<<H, 256, 0, s>>>(hfirst.as<int64_t>(), hcount.as<int64_t>(), ci, H, d_bound,
+                                           flags);
             HP_LAUNCHED();
         }
         k_nbound_calls<<<blocks(H, 256), 256, 0, s>>>(call_tile0.as<int32_t>(),
-                                                      call_ntiles.as<int32_t>(), ti32, H, d_halos);
+                                                      call_ntiles.as<int32_t>(), ti32, H, d_halos,
+                                                      flags);
         HP_LAUNCHED();
         return HP_OK;
     }

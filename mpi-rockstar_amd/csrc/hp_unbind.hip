@@ -848,9 +848,9 @@ __global__ void k_vel_finish(const float4 *__restrict__ rv,
                              const int32_t *__restrict__ call_tile0,
                              const int32_t *__restrict__ call_ntiles, int32_t nh,
                              const CallInfo *__restrict__ ci, const Cand *__restrict__ tcand,
-                             hp_halo *__restrict__ halos) {
+                             hp_halo *__restrict__ halos, const int32_t *__restrict__ flags) {
     const int32_t h = blockIdx.x * blockDim.x + threadIdx.x;
-    if (h >= nh) return;
+    if (h >= nh || (*flags & kErrDeep)) return;
     const CallInfo c = ci[h];
     hp_halo hh = halos[h];
     hh.kept = c.kept;
@@ -903,9 +903,10 @@ __global__ __launch_bounds__(kKinThreads) void k_kinetic_tiles(const struct pote
                                                          const hp_halo *__restrict__ halos,
                                                          double hub_a, double conv,
                                                          uint8_t *__restrict__ bound,
-                                                         int32_t *__restrict__ tile_nb) {
+                                                         int32_t *__restrict__ tile_nb,
+                                                         const int32_t *__restrict__ flags) {
     const int32_t t = blockIdx.x;
-    if (t >= ntiles) return;
+    if (t >= ntiles || (*flags & kErrDeep)) return;
     const ScanTile st = tiles[t];
     const CallInfo c = ci[st.call];
     const hp_halo &hh = halos[st.call];
@@ -945,9 +946,10 @@ __global__ __launch_bounds__(kKinThreads) void k_kinetic_tiles(const struct pote
 
 // potential-only records: bound = 0
 __global__ void k_bound_tail(const int64_t *__restrict__ hfirst, const int64_t *__restrict__ hcount,
-                             const CallInfo *__restrict__ ci, int32_t nh, uint8_t *__restrict__ bound) {
+                             const CallInfo *__restrict__ ci, int32_t nh, uint8_t *__restrict__ bound,
+                             const int32_t *__restrict__ flags) {
     const int32_t h = blockIdx.x;
-    if (h >= nh) return;
+    if (h >= nh || (*flags & kErrDeep)) return;
     const CallInfo c = ci[h];
     for (int64_t j = c.kept + threadIdx.x; j < hcount[h]; j += blockDim.x) bound[hfirst[h] + j] = 0;
 }
@@ -955,9 +957,9 @@ __global__ void k_bound_tail(const int64_t *__restrict__ hfirst, const int64_t *
 __global__ void k_nbound_calls(const int32_t *__restrict__ call_tile0,
                                const int32_t *__restrict__ call_ntiles,
                                const int32_t *__restrict__ tile_nb, int32_t nh,
-                               hp_halo *__restrict__ halos) {
+                               hp_halo *__restrict__ halos, const int32_t *__restrict__ flags) {
     const int32_t h = blockIdx.x * blockDim.x + threadIdx.x;
-    if (h >= nh) return;
+    if (h >= nh || (*flags & kErrDeep)) return;
     int64_t n = 0;
     for (int32_t t = call_tile0[h]; t < call_tile0[h] + call_ntiles[h]; t++) n += tile_nb[t];
     halos[h].n_bound = n;

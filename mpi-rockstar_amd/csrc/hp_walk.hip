@@ -348,7 +348,7 @@ struct Walker {
 template <int ACC, bool COUNT>
 __global__ __launch_bounds__(64) void k_walk(WalkArgs a) {
     const int32_t gid = (int32_t)((blockIdx.x * blockDim.x + threadIdx.x) / kWave);
-    if (gid >= a.ngroups) return;
+    if (gid >= a.ngroups || (*a.err & kErrDeep)) return;
     const int lane = lane_id();
     const Group g = load_const(a.groups + gid);
     if (g.root < 0) return;  // an empty entry of the XCD layout
@@ -416,6 +416,7 @@ struct DirectArgs {
     const float4 *fin;
     const int2 *blocks;   // (root node, first sink offset)
     int32_t nblocks;
+    const int32_t *flags;  // error flags (nothing is written under kErrDeep)
     float pm;
     float inv_fr;
     double pm_d;
@@ -426,6 +427,7 @@ struct DirectArgs {
 template <int ACC>
 __global__ __launch_bounds__(256) void k_direct(DirectArgs a) {
     __shared__ float4 s_src[256];
+    if (*a.flags & kErrDeep) return;
     const int2 b = a.blocks[blockIdx.x];
     const Node rn = a.nodes[b.x];
     const int32_t i = b.y + threadIdx.x;
@@ -483,7 +485,8 @@ __global__ void k_zero_pe(struct potential *po, int64_t n) {
     if (i < n) po[i].pe = 0.0;
 }
 __global__ void k_zero_pe_calls(struct potential *po, const int64_t *hfirst,
-                                const int64_t *hcount, const int2 *blocks) {
+                                const int64_t *hcount, const int2 *blocks, const int32_t *flags) {
+    if (*flags & kErrDeep) return;
     const int2 b = blocks[blockIdx.x];
     const int64_t i = b.y + threadIdx.x;
     if (i < hcount[b.x]) po[hfirst[b.x] + i].pe = 0.0;
