@@ -477,6 +477,8 @@ struct ShapeState {
     int32_t limit;     // iterations to run; -1 until the first pass counted
     int32_t done;
     int32_t assigned;  // inertia written by this window
+    int32_t nbound;    // bound records in [0, n)
+    int32_t pad;
 };
 
 struct ShapePart {
@@ -511,30 +513,52 @@ __global__ void k_shape_init(int32_t nh, const hp_halo *__restrict__ halos,
         s.limit = -1;
         s.done = !(s.r > 0) || s.n <= 0;
         s.assigned = 0;
+        s.nbound = 0;
+        s.pad = 0;
         st[2 * h + w] = s;
     }
 }
 
 // x, y, z offsets from the centre and the bound flag of the records the
-// windows cover
+// windows cover; each window's bound records are counted
 __global__ __launch_bounds__(kWave) void k_shape_prep(const struct potential *__restrict__ srt,
                                                       const ScanTile *__restrict__ tiles,
                                                       int32_t ntiles,
                                                       const hp_halo *__restrict__ halos,
                                                       const int64_t *__restrict__ hfirst,
-                                                      const ShapeState *__restrict__ st,
+                                                      ShapeState *__restrict__ st,
                                                       float4 *__restrict__ cd) {
     const int32_t t = blockIdx.x;
     if (t >= ntiles) return;
     const ScanTile tl = tiles[t];
-    const int32_t n = max(st[2 * tl.call].n, st[2 * tl.call + 1].n);
+    const int32_t n0 = st[2 * tl.call].n, n1 = st[2 * tl.call + 1].n;
+    const int32_t n = max(n0, n1);
     const int64_t first = hfirst[tl.call];
     const float c0 = halos[tl.call].pos[0], c1 = halos[tl.call].pos[1], c2 = halos[tl.call].pos[2];
+    int32_t b0 = 0, b1 = 0;
     for (int32_t j = tl.off + lane_id(); j < min(tl.off + kScanTile, n); j += kWave) {
         const struct potential &r = srt[first + j];
-        cd[first + j] = make_float4(r.pos[0] - c0, r.pos[1] - c1, r.pos[2] - c2,
-                                    !(r.pe < (double)r.ke) ? 1.0f : 0.0f);
+        const bool b = !(r.pe < (double)r.ke);
+        cd[first + j] = make_float4(r.pos[0] - c0, r.pos[1] - c1, r.pos[2] - c2, b ? 1.0f : 0.0f);
+        b0 += (b && j < n0) ? 1 : 0;
+        b1 += (b && j < n1) ? 1 : 0;
     }
+    b0 = wave_sum(b0);
+    b1 = wave_sum(b1);
+    if (lane_id() == 0) {
+        if (b0) atomicAdd(&st[2 * tl.call].nbound, b0);
+        if (b1) atomicAdd(&st[2 * tl.call + 1].nbound, b1);
+    }
+}
+
+// Windows summed in the reference's order, record by record (the sums of a
+// window of few records decide which records the next ellipsoid holds, and
+// a change in their last bit can change the result): those of at most
+// kShapeExactRecords records or fewer than kShapeExactBound bound records.
+constexpr int32_t kShapeExactRecords = 256;
+constexpr int32_t kShapeExactBound = 64;
+__device__ __forceinline__ bool shape_exact(const ShapeState &s) {
+    return s.n <= kShapeExactRecords || s.nbound < kShapeExactBound;
 }
 
 // src/properties.c:243-268 for one tile, both windows
@@ -544,6 +568,7 @@ __global__ __launch_bounds__(kWave) void k_shape_tiles(const ScanTile *__restric
                                                        const ShapeState *__restrict__ st,
                                                        const float4 *__restrict__ cd,
                                                        ShapePart *__restrict__ part) {
+    __shared__ double s_term[kWave][8];
     for (int32_t t = blockIdx.x; t < ntiles; t += gridDim.x) {
     const ScanTile tl = tiles[t];
     const int64_t first = hfirst[tl.call];
@@ -551,6 +576,62 @@ __global__ __launch_bounds__(kWave) void k_shape_tiles(const ScanTile *__restric
         const ShapeState &s = st[2 * tl.call + w];
         double acc[7] = {0, 0, 0, 0, 0, 0, 0};
         int32_t cnt = 0;
+        if (!s.done && shape_exact(s)) {
+            // the window's first tile sums the whole window in record order
+            // (src/properties.c:243-268: r with the divisions by eig, then
+            // the running sums); the other tiles add nothing
+            if (tl.off == 0) {
+                const int lane = lane_id();
+                for (int32_t base = 0; base < s.n; base += kWave) {
+                    const int32_t j = base + lane;
+                    bool take = false;
+                    if (j < s.n) {
+                        const float4 q = cd[first + j];
+                        if (q.w != 0.0f) {
+                            cnt++;
+                            const double d[3] = {(double)q.x, (double)q.y, (double)q.z};
+                            double r = 0;
+                            for (int k = 0; k < 3; k++) {
+                                double dr = 0;
+                                for (int l = 0; l < 3; l++) dr += s.orth[k][l] * d[l];
+                                r += dr * dr / s.eig[k];
+                            }
+                            if (r < (double)s.min_r) r = s.min_r;
+                            if (r > 0 && r <= 1) {
+                                take = true;
+                                const double tw = 1.0 / r;  // WEIGHTED_SHAPES = 1
+                                s_term[lane][0] = tw;
+                                s_term[lane][1] = d[0] * d[0] * tw;
+                                s_term[lane][2] = d[1] * d[1] * tw;
+                                s_term[lane][3] = d[2] * d[2] * tw;
+                                s_term[lane][4] = d[0] * d[1] * tw;
+                                s_term[lane][5] = d[0] * d[2] * tw;
+                                s_term[lane][6] = d[1] * d[2] * tw;
+                            }
+                        }
+                    }
+                    uint64_t m = ballot(take);
+                    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+                    __builtin_amdgcn_wave_barrier();
+                    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+                    while (m) {  // every lane keeps the same running sums
+                        const int i = __builtin_ctzll(m);
+                        m &= m - 1;
+                        for (int k = 0; k < 7; k++) acc[k] += s_term[i][k];
+                    }
+                    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+                    __builtin_amdgcn_wave_barrier();
+                    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+                }
+            }
+            ShapePart pt;
+            pt.w = acc[0];
+            for (int k = 0; k < 6; k++) pt.t[k] = acc[k + 1];
+            pt.cnt = wave_sum(cnt);
+            pt.pad = 0;
+            if (lane_id() == 0) part[2 * t + w] = pt;
+            continue;
+        }
         if (!s.done) {
             const int32_t end = min(tl.off + kScanTile, s.n);
             double o[3][3], ie[3];

@@ -13,11 +13,14 @@ Both sides read the same input: the GPU bound pass's own output records
     spin, bullock_spin): 1e-5 relative, the reference's running means and
     sequential sums being restated as sums of exact differences;
   * NFW scale radii and chi2, pseudo-evolution masses: 1e-5 relative;
-  * shapes (b_to_a, c_to_a, A, inertia tensor, in both windows): 1e-4
-    relative where the window holds at least 32 bound records.  With fewer,
-    the iteration is ill-conditioned: the reference's own result moves when
-    only the order of its sums changes (tests/test_props_oracle.py), and the
-    device sums in another order.
+  * shapes (b_to_a, c_to_a, A, inertia tensor, in both windows), every
+    window: the device sums a window of at most 256 records or fewer than 64
+    bound records in the reference's order, record by record (the iteration
+    is ill-conditioned there: the reference's own result moves when only the
+    order of its sums changes, tests/test_props_oracle.py), and those agree
+    to 1e-6 of the field's scale (axis ratios: absolute, the near-zero ratio
+    of a flat window being the last bits of a vanishing eigenvalue); larger
+    windows are summed per tile and agree to 1e-4 of the field's scale.
 """
 import numpy as np
 import pytest
@@ -34,7 +37,8 @@ SUMS = ("J", "Xoff", "Voff", "vrms", "energy", "kin_to_pot", "spin", "bullock_sp
 FITS = ("rs", "klypin_rs", "chi2", "m_pe_b", "m_pe_d")
 SHAPE = {1: ("b_to_a", "c_to_a", "A", "inertia_tensor"),
          0: ("b_to_a2", "c_to_a2", "A2", "inertia_tensor2")}
-WELL_CONDITIONED = 32  # bound records in a shape window
+EXACT_RECORDS, EXACT_BOUND = 256, 64  # windows the device sums in the reference's order
+RATIOS = ("b_to_a", "c_to_a", "b_to_a2", "c_to_a2")
 
 
 def run_props(cat, defs=None, opts=None):
@@ -62,16 +66,18 @@ def run_props(cat, defs=None, opts=None):
 
 
 def _window_counts(po, halos, props, particle_mass):
-    """Bound records of each call's two shape windows: [0, np_alt[2]) and
-    [0, dens_tot) (alt_m[2] = np_alt[2] * PARTICLE_MASS)."""
-    out = np.zeros((halos.shape[0], 2), np.int64)
+    """Records and bound records of each call's two shape windows:
+    [0, np_alt[2]) and [0, dens_tot), each cut at the kept count
+    (alt_m[2] = np_alt[2] * PARTICLE_MASS)."""
+    size = np.zeros((halos.shape[0], 2), np.int64)
+    bound = np.zeros((halos.shape[0], 2), np.int64)
     for h in range(halos.shape[0]):
         f, k = int(halos["first"][h]), int(halos["kept"][h])
         b = ~(po["pe"][f:f + k] < po["ke"][f:f + k].astype(np.float64))
         n0 = int(round(float(props["alt_m"][h][2]) / particle_mass))
-        out[h, 0] = int(b[:min(n0, k)].sum())
-        out[h, 1] = int(b[:int(props["dens_tot"][h])].sum())
-    return out
+        size[h] = min(n0, k), min(int(props["dens_tot"][h]), k)
+        bound[h] = int(b[:size[h, 0]].sum()), int(b[:size[h, 1]].sum())
+    return size, bound
 
 
 def check(restatement, cat, defs=None, opts=None):
@@ -80,14 +86,18 @@ def check(restatement, cat, defs=None, opts=None):
     restatement.halo_props_batch(po_g, h_g, cat.params, th, want, nthreads=8)
     for f in FITS:
         np.testing.assert_allclose(got[f], want[f], rtol=1e-5, atol=0, err_msg=f)
-    win = _window_counts(po_g, h_g, want, cat.params.particle_mass)
+    size, nbound = _window_counts(po_g, h_g, want, cat.params.particle_mass)
+    exact = (size <= EXACT_RECORDS) | (nbound < EXACT_BOUND)
     for w, fields in SHAPE.items():
-        ok = win[:, w] >= WELL_CONDITIONED
+        tol = np.where(exact[:, w], 1e-6, 1e-4)[:, None]
         for f in fields:
-            a = got[f][ok].reshape(int(ok.sum()), -1).astype(np.float64)
-            b = want[f][ok].reshape(int(ok.sum()), -1).astype(np.float64)
+            a = got[f].reshape(cat.n_halos, -1).astype(np.float64)
+            b = want[f].reshape(cat.n_halos, -1).astype(np.float64)
             scale = np.maximum(np.max(np.abs(b), axis=1, keepdims=True), 1e-30)
-            assert np.all(np.abs(a - b) <= 1e-4 * scale), f
+            if f in RATIOS:
+                scale = np.ones_like(scale)
+            bad = np.nonzero(np.any(np.abs(a - b) > tol * scale, axis=1))[0]
+            assert bad.size == 0, (f, bad[:5], a[bad[:5]], b[bad[:5]])
         assert np.all(np.isfinite(got[SHAPE[w][0]]))
     for f in catalog.PROPS_INT_FIELDS:
         np.testing.assert_array_equal(got[f], want[f], err_msg=f)
