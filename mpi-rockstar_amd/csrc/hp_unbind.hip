@@ -308,34 +308,32 @@ __global__ __launch_bounds__(kSortThreads) void k_seg_hist(const uint64_t *__res
     }
 }
 
-// stable scatter of one tile, as k_radix_scatter, into its call's range
+// Stable scatter of one tile into its call's range.  Wavefront w takes the
+// tile's w-th quarter (rounds of 64 consecutive positions); a record's place
+// is its digit's base for the tile (the scanned counts), plus the records of
+// that digit in the earlier quarters, plus those before it in its own
+// quarter.  The ranks come from ballots (the lanes of a round with the same
+// digit), counted per wavefront in LDS, so the tile needs two workgroup
+// barriers instead of three per round.
+constexpr int kSortWaves = kSortThreads / kWave;
 __global__ __launch_bounds__(kSortThreads) void k_seg_scatter(
     const uint64_t *__restrict__ kin, const int32_t *__restrict__ vin, uint64_t *__restrict__ kout,
     int32_t *__restrict__ vout, const SegTile *__restrict__ tiles, int shift,
     const int32_t *__restrict__ offs) {
-    __shared__ int32_t s_base[kSegRadix];
-    __shared__ int32_t s_wc[kSortThreads / kWave][kSegRadix];
-    const int w = threadIdx.x / kWave;
+    __shared__ int32_t s_cnt[kSortWaves][kSegRadix];
+    const int w = threadIdx.x / kWave, lane = lane_id();
     const SegTile st = tiles[blockIdx.x];
-    uint64_t key[kSortItems];
-    int32_t val[kSortItems];
-#pragma unroll
-    for (int r = 0; r < kSortItems; r++) {
-        const int32_t i = r * kSortThreads + threadIdx.x;
-        key[r] = i < st.n ? kin[st.off + i] : 0ull;
-        val[r] = i < st.n ? vin[st.off + i] : 0;
-    }
-    for (int q = 0; q < kSegRadixPer; q++) {
-        const int d = q * kSortThreads + threadIdx.x;
-        s_base[d] = offs[st.hpos + d * st.ntc] + st.delta;
-        for (int j = 0; j < kSortThreads / kWave; j++) s_wc[j][d] = 0;
-    }
+    for (int q = threadIdx.x; q < kSortWaves * kSegRadix; q += kSortThreads) (&s_cnt[0][0])[q] = 0;
     __syncthreads();
+    const int32_t q0 = w * kSortItems * kWave;  // this wavefront's quarter
+    uint64_t key[kSortItems];
+    int32_t val[kSortItems], pos[kSortItems];
 #pragma unroll
     for (int r = 0; r < kSortItems; r++) {
-        if (r * kSortThreads >= st.n) break;  // block-uniform
-        const int32_t i = r * kSortThreads + threadIdx.x;
+        const int32_t i = q0 + r * kWave + lane;
         const bool v = i < st.n;
+        key[r] = v ? kin[st.off + i] : 0ull;
+        val[r] = v ? vin[st.off + i] : 0;
         const int dg = (int)((key[r] >> shift) & (kSegRadix - 1));
         uint64_t peers = ballot(v);
         for (int b = 0; b < kSegBits; b++) {
@@ -343,21 +341,35 @@ __global__ __launch_bounds__(kSortThreads) void k_seg_scatter(
             peers &= ((dg >> b) & 1) ? m : ~m;
         }
         const int rk = rank_below(peers);
-        const bool lead = v && rk == 0;
-        if (lead) s_wc[w][dg] = __popcll(peers);
-        __syncthreads();
-        if (v) {
-            int32_t off = s_base[dg];
-            for (int j = 0; j < w; j++) off += s_wc[j][dg];
-            kout[off + rk] = key[r];
-            vout[off + rk] = val[r];
+        // the count so far is read by every lane of the digit before its
+        // first lane adds the round's records (program order in the wavefront)
+        pos[r] = v ? s_cnt[w][dg] + rk : -1;
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+        if (v && rk == 0) s_cnt[w][dg] += __popcll(peers);
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+    }
+    __syncthreads();
+    // each digit's place for each quarter: the tile's base, then the quarters in order
+    for (int d = threadIdx.x; d < kSegRadix; d += kSortThreads) {
+        int32_t run = offs[st.hpos + d * st.ntc] + st.delta;
+        for (int q = 0; q < kSortWaves; q++) {
+            const int32_t c = s_cnt[q][d];
+            s_cnt[q][d] = run;
+            run += c;
         }
-        __syncthreads();
-        if (lead) {
-            atomicAdd(&s_base[dg], s_wc[w][dg]);
-            s_wc[w][dg] = 0;
-        }
-        __syncthreads();
+    }
+    __syncthreads();
+#pragma unroll
+    for (int r = 0; r < kSortItems; r++) {
+        if (pos[r] < 0) continue;
+        const int dg = (int)((key[r] >> shift) & (kSegRadix - 1));
+        const int32_t dst = s_cnt[w][dg] + pos[r];
+        kout[dst] = key[r];
+        vout[dst] = val[r];
     }
 }
 
