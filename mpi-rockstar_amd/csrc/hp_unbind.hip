@@ -554,14 +554,26 @@ __device__ __forceinline__ void store_rec(struct potential *p, const struct pote
 
 // the records in sorted order, and a compact copy of what the core-velocity
 // passes read (r2 and the velocity, 16 B instead of the 48-B record)
-__global__ void k_gather_sorted(const struct potential *__restrict__ po,
-                                const int32_t *__restrict__ order, int32_t nrec,
-                                struct potential *__restrict__ out, float4 *__restrict__ rv) {
-    const int32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= nrec) return;
-    const struct potential rec = load_rec(po + order[i]);
-    store_rec(out + i, rec);
-    rv[i] = make_float4(rec.r2, rec.pos[3], rec.pos[4], rec.pos[5]);
+__global__ __launch_bounds__(256) void k_gather_sorted(const struct potential *__restrict__ po,
+                                                       const int32_t *__restrict__ order,
+                                                       int32_t nrec,
+                                                       struct potential *__restrict__ out,
+                                                       float4 *__restrict__ rv) {
+    // each workgroup's 256 records go out through LDS as 16-byte words, so
+    // that the stores of a wavefront are contiguous (the loads are a gather)
+    __shared__ float4 s_rec[3 * 256];
+    const int32_t i0 = blockIdx.x * 256;
+    const int32_t i = i0 + threadIdx.x;
+    const int32_t m = min(256, nrec - i0);
+    if (i < nrec) {
+        const struct potential rec = load_rec(po + order[i]);
+        const RecWords u = __builtin_bit_cast(RecWords, rec);
+        for (int q = 0; q < 3; q++) s_rec[3 * threadIdx.x + q] = u.w[q];
+        rv[i] = make_float4(rec.r2, rec.pos[3], rec.pos[4], rec.pos[5]);
+    }
+    __syncthreads();
+    float4 *dst = reinterpret_cast<float4 *>(out + i0);
+    for (int q = threadIdx.x; q < 3 * m; q += 256) dst[q] = s_rec[q];
 }
 
 // ------------------------------------------------------------ core velocity
@@ -927,9 +939,19 @@ __global__ __launch_bounds__(kKinThreads) void k_kinetic_tiles(const struct pote
     const float cen_x[3] = {hh.pos[0], hh.pos[1], hh.pos[2]};
     const int32_t end = min(st.off + kScanTile, c.count);
     int32_t nb = 0;
-    for (int32_t j = st.off + (int32_t)threadIdx.x; j < end; j += kKinThreads) {
-        struct potential rec = load_rec(srt + c.first + j);
-        if (j < c.kept) {
+    // chunks of kKinThreads records moved through LDS as 16-byte words, so
+    // that the loads and stores of a wavefront are contiguous
+    __shared__ float4 s_rec[3 * kKinThreads];
+    for (int32_t j0 = st.off; j0 < end; j0 += kKinThreads) {
+        const int32_t m = min(kKinThreads, end - j0);
+        const float4 *src = reinterpret_cast<const float4 *>(srt + c.first + j0);
+        for (int q = threadIdx.x; q < 3 * m; q += kKinThreads) s_rec[q] = src[q];
+        __syncthreads();
+        const int32_t j = j0 + (int32_t)threadIdx.x;
+        if ((int32_t)threadIdx.x < m && j < c.kept) {
+            RecWords u;
+            for (int q = 0; q < 3; q++) u.w[q] = s_rec[3 * threadIdx.x + q];
+            struct potential rec = __builtin_bit_cast(struct potential, u);
             float ke = 0.0f;
             for (int k = 0; k < 3; k++) {
                 const float dvp = rec.pos[k + 3] - cen_v[k];
@@ -942,8 +964,13 @@ __global__ __launch_bounds__(kKinThreads) void k_kinetic_tiles(const struct pote
             const bool b = !(rec.pe < (double)ke);
             if (bound) bound[c.first + j] = b ? 1 : 0;
             nb += b ? 1 : 0;
+            u = __builtin_bit_cast(RecWords, rec);
+            s_rec[3 * threadIdx.x + 2] = u.w[2];  // ke and flags: the third word
         }
-        store_rec(out + c.first + j, rec);
+        __syncthreads();
+        float4 *dst = reinterpret_cast<float4 *>(out + c.first + j0);
+        for (int q = threadIdx.x; q < 3 * m; q += kKinThreads) dst[q] = s_rec[q];
+        __syncthreads();
     }
     __shared__ int32_t s_nb[kKinThreads / kWave];
     nb = wave_sum(nb);
