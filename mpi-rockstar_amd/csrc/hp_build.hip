@@ -44,13 +44,15 @@
 namespace hp {
 
 __device__ __forceinline__ double d2_ref(const float *c, const float4 &p) {
-    // src/potential.c:22-29: float differences, double squares, summed in order
-    double dx = (double)(c[0] - p.x);
-    double dy = (double)(c[1] - p.y);
-    double dz = (double)(c[2] - p.z);
+    // src/potential.c:22-29: float differences, double squares, summed in
+    // order (a float times a float is exact in double, so each fma rounds
+    // as the separate product and sum do)
+    const double dx = (double)(c[0] - p.x);
+    const double dy = (double)(c[1] - p.y);
+    const double dz = (double)(c[2] - p.z);
     double s = dx * dx;
-    s = s + dy * dy;
-    s = s + dz * dz;
+    s = fma(dy, dy, s);
+    s = fma(dz, dz, s);
     return s;
 }
 
@@ -1026,6 +1028,7 @@ __global__ __launch_bounds__(kSubThreads, kCap <= 256 ? 7 : 1) void k_subtree(co
             const float4 c4 = cm_get(j);
             const float com[3] = {c4.x, c4.y, c4.z};
             double big = 0.0, sum = 0.0;
+#pragma unroll 4
             for (int i = my_lf[q]; i < my_lf[q] + my_n[q]; i++) {
                 const double d = d2_ref(com, F(i));
                 big = d > big ? d : big;

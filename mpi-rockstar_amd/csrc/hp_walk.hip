@@ -471,12 +471,23 @@ __global__ __launch_bounds__(256) void k_direct(DirectArgs a) {
 
 // The records in tree order, the order the reference's compute_potential
 // leaves them in (its tree build permutes po in place, src/fast3tree.c:
-// 762-773, 875-887): out[j] = po[record at tree position j].
-__global__ void k_tree_records(const struct potential *__restrict__ po,
-                               const float4 *__restrict__ fin, int64_t n,
-                               struct potential *__restrict__ out) {
-    const int64_t j = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (j < n) out[j] = po[__float_as_int(fin[j].w)];
+// 762-773, 875-887): out[j] = po[record at tree position j].  Each
+// workgroup's 256 records go out through LDS as 16-byte words, so that the
+// stores of a wavefront are contiguous (the loads are a gather).
+__global__ __launch_bounds__(256) void k_tree_records(const struct potential *__restrict__ po,
+                                                      const float4 *__restrict__ fin, int64_t n,
+                                                      struct potential *__restrict__ out) {
+    __shared__ float4 s_rec[3 * 256];
+    const int64_t j0 = (int64_t)blockIdx.x * 256;
+    const int64_t j = j0 + threadIdx.x;
+    const int m = (int)min((int64_t)256, n - j0);
+    if (j < n) {
+        const float4 *src = reinterpret_cast<const float4 *>(po + __float_as_int(fin[j].w));
+        for (int q = 0; q < 3; q++) s_rec[3 * threadIdx.x + q] = src[q];
+    }
+    __syncthreads();
+    float4 *dst = reinterpret_cast<float4 *>(out + j0);
+    for (int q = threadIdx.x; q < 3 * m; q += 256) dst[q] = s_rec[q];
 }
 
 // pe = 0 before the direct kernel, which writes finite records only
