@@ -73,7 +73,7 @@ void compute_kinetic_energy(struct potential *po, int64_t num_po,
  * between levels).  Returns 0 or an HP_E* code. */
 int hp_compute_potential_noalloc(struct potential *d_po, int64_t num_po, void *stream);
 
-/* Configuration globals read at call time (src/config.template.h:7,16,20,/* Configuration globals read at call time (src/config.template.h:7,16,20,
+/* Configuration globals read at call time (src/config.template.h:7,16,20,
  * 22-25 through src/config_vars.h).  libhalopot carries weak definitions with
  * the reference's defaults, so the program that defines the real ones (the
  * halo finder) overrides them and stand-alone callers can set them here.

@@ -638,7 +638,7 @@ __global__ __launch_bounds__(256) void k_group_classes(const Task *__restrict__ 
 // no spills): its LDS fits more workgroups than its registers otherwise
 // allowed.
 template <int kCap>
-__global__ __launch_bounds__(kSubThreads, kCap <= 256 ? 7 : 1) void k_subtree(const float4 *__restrict__ A,
+__global__ __launch_bounds__(kSubThreads, kCap <= 256 ? 7 : (kCap <= 512 ? 6 : 1)) void k_subtree(const float4 *__restrict__ A,
                                                          const float4 *__restrict__ B,
                                                          const Task *__restrict__ tasks,
                                                          BuildArgs a, double particle_mass,
@@ -1362,15 +1362,16 @@ __device__ void big_count_tile(const float4 *__restrict__ src, const int2 *__res
                                const Task *__restrict__ big, const Node *__restrict__ nodes,
                                TilePart *__restrict__ parts, int32_t ti);
 
-// Count pass over one tile of a big task.
+// Count pass over the tiles of split level `level`.
 __global__ __launch_bounds__(kTileThreads) void k_big_count(const float4 *__restrict__ src,
                                                             const int2 *__restrict__ tiles,
                                                             const Task *__restrict__ big,
                                                             const Node *__restrict__ nodes,
                                                             TilePart *__restrict__ parts,
-                                                            const int32_t *__restrict__ counters) {
-    const int32_t t1 = counters[kTileEnd];
-    for (int32_t ti = counters[kTileBegin] + blockIdx.x; ti < t1; ti += gridDim.x)
+                                                            const int32_t *__restrict__ lv,
+                                                            int32_t level) {
+    const int32_t t1 = lvl_tile(lv)[level];
+    for (int32_t ti = lvl_tile(lv)[level - 1] + blockIdx.x; ti < t1; ti += gridDim.x)
         big_count_tile(src, tiles, big, nodes, parts, ti);
 }
 
@@ -1400,128 +1401,138 @@ __device__ void big_count_tile(const float4 *__restrict__ src, const int2 *__res
     block_box_reduce(lo, hi, nl, parts + ti);
 }
 
-// One thread per big task of the level: scan of the tile counts, children.
-// One wavefront per big task of the level: exclusive scan of the tiles'
-// left counts (lanes over tiles), the children's boxes, then the split.
-__global__ __launch_bounds__(256) void k_big_split(const Task *__restrict__ big,
-                                                   TilePart *__restrict__ parts,
-                                                   BigSplit *__restrict__ res, BuildArgs a) {
-    const int32_t b1 = a.counters[kBigEnd];
-    const int32_t nw = (int32_t)(gridDim.x * blockDim.x / kWave);
+// The split of big task bi, by one wavefront once its tiles are counted:
+// the left count and the children's boxes from the tiles' parts (lanes over
+// tiles), then the children.
+__device__ void split_task(const TilePart *__restrict__ parts, BigSplit *__restrict__ res,
+                           const BuildArgs &a, int32_t bi) {
     const int lane = lane_id();
-    for (int32_t bi = a.counters[kBigBegin] + (int32_t)((blockIdx.x * blockDim.x + threadIdx.x) / kWave);
-         bi < b1; bi += nw) {
-        const Task t = big[bi];
-        const Node nd = a.nodes[t.node];
-        const float inf = __int_as_float(0x7f800000), ninf = __int_as_float((int)0xff800000);
-        float lo[2][3] = {{inf, inf, inf}, {inf, inf, inf}};
-        float hi[2][3] = {{ninf, ninf, ninf}, {ninf, ninf, ninf}};
-        int32_t nl = 0;
-        for (int32_t j0 = 0; j0 < t.ntiles; j0 += kWave) {
-            const int32_t jj = j0 + lane;
-            const bool v = jj < t.ntiles;
-            TilePart *p = parts + t.tile0 + (v ? jj : 0);
-            const int32_t c = v ? p->nleft : 0;
-            int32_t incl = c;
-            for (int o = 1; o < kWave; o <<= 1) {
-                const int32_t u = __shfl_up(incl, o, 64);
-                if (lane >= o) incl += u;
-            }
-            if (v) {
-                p->pad[0] = nl + incl - c;  // left points in the task's earlier tiles
-                for (int s = 0; s < 2; s++)
-                    for (int d = 0; d < 3; d++) {
-                        lo[s][d] = fminf(lo[s][d], p->lo[s][d]);
-                        hi[s][d] = fmaxf(hi[s][d], p->hi[s][d]);
-                    }
-            }
-            nl += __shfl(incl, kWave - 1, 64);
-        }
+    const Task t = a.big[bi];
+    const Node nd = a.nodes[t.node];
+    const float inf = __int_as_float(0x7f800000), ninf = __int_as_float((int)0xff800000);
+    float lo[2][3] = {{inf, inf, inf}, {inf, inf, inf}};
+    float hi[2][3] = {{ninf, ninf, ninf}, {ninf, ninf, ninf}};
+    int32_t nl = 0;
+    for (int32_t j = lane; j < t.ntiles; j += kWave) {
+        const TilePart *p = parts + t.tile0 + j;
+        nl += p->nleft;
         for (int s = 0; s < 2; s++)
             for (int d = 0; d < 3; d++) {
-                lo[s][d] = wave_min(lo[s][d]);
-                hi[s][d] = wave_max(hi[s][d]);
+                lo[s][d] = fminf(lo[s][d], p->lo[s][d]);
+                hi[s][d] = fmaxf(hi[s][d], p->hi[s][d]);
             }
-        // nl, nd and the reduced boxes are wavefront-uniform from here on
-        BigSplit r;
-        r.nleft = nl;
-        r.left = -1;
-        r.leafL = r.leafR = 0;
-        if (nl == 0 || nl == nd.count) {
-            if (lane == 0) note_fat_leaf(a, t.node, nd.count);
-        } else {
-            const int32_t nr = nd.count - nl;
-            uint64_t al = 0;
-            int32_t G = 0;
-            if (lane == 0) {
-                const Sizes sz = sizes_of(a, t.pad[0]);
-                al = alloc_nodes_and_small(a, 2, wave_pushes(sz, nl, nr));
-                const int32_t ng = group_pushes(sz, nl, nr);
-                if (ng) G = atomicAdd(&a.counters[kGroupNext], ng);
-            }
-            al = read_lane(al, 0);
-            G = read_lane(G, 0);
-            const int32_t L = (int32_t)al, P = (int32_t)(al >> 32);
-            if (L + 2 > a.cap_nodes) {
-                if (lane == 0) atomicOr(&a.counters[kErrFlags], kErrCapacity);
-            } else {
-                r.left = L;
-                r.leafL = nl <= kLeafPoints;
-                r.leafR = nr <= kLeafPoints;
-                make_children(a, t, nd, L, P, G, nl, lo[0], hi[0], lo[1], hi[1]);
-            }
-        }
-        if (lane == 0) res[bi] = r;
     }
+    nl = wave_sum(nl);
+    for (int s = 0; s < 2; s++)
+        for (int d = 0; d < 3; d++) {
+            lo[s][d] = wave_min(lo[s][d]);
+            hi[s][d] = wave_max(hi[s][d]);
+        }
+    // nl, nd and the reduced boxes are wavefront-uniform from here on
+    BigSplit r;
+    r.nleft = nl;
+    r.left = -1;
+    r.leafL = r.leafR = 0;
+    if (nl == 0 || nl == nd.count) {
+        if (lane == 0) note_fat_leaf(a, t.node, nd.count);
+    } else {
+        const int32_t nr = nd.count - nl;
+        uint64_t al = 0;
+        int32_t G = 0;
+        if (lane == 0) {
+            const Sizes sz = sizes_of(a, t.pad[0]);
+            al = alloc_nodes_and_small(a, 2, wave_pushes(sz, nl, nr));
+            const int32_t ng = group_pushes(sz, nl, nr);
+            if (ng) G = atomicAdd(&a.counters[kGroupNext], ng);
+        }
+        al = read_lane(al, 0);
+        G = read_lane(G, 0);
+        const int32_t L = (int32_t)al, P = (int32_t)(al >> 32);
+        if (L + 2 > a.cap_nodes) {
+            if (lane == 0) atomicOr(&a.counters[kErrFlags], kErrCapacity);
+        } else {
+            r.left = L;
+            r.leafL = nl <= kLeafPoints;
+            r.leafR = nr <= kLeafPoints;
+            make_children(a, t, nd, L, P, G, nl, lo[0], hi[0], lo[1], hi[1]);
+        }
+    }
+    if (lane == 0) res[bi] = r;
 }
 
 // The two passes over one tile of a big task, with the points' ranks on
 // their side of the plane in rounds of 256 (a block prefix of ballots):
 //   kSides:   lists the node's points by side, a.sides[first + k] = the
 //             position of the k-th left point, then of the right points
-//             (the stable order);
+//             (the stable order).  Each workgroup takes its tile's left
+//             prefix and the task's left total from the count pass's parts
+//             (and keeps the prefix in its part for the scatter pass); the
+//             workgroup of the task's first tile also splits the task.
 //   kScatter: moves each point to the slot the reference's in-place
-//             partition gives it (inplace_dest over that list).
+//             partition gives it (inplace_dest over that list).  Its first
+//             workgroup moves the level state forward (level_done).
 enum { kSides = 0, kScatter = 1 };
 
 template <int kPass>
 __device__ void big_tile_pass(const float4 *__restrict__ src, float4 *__restrict__ dst,
                               const int2 *__restrict__ tiles, const Task *__restrict__ big,
-                              const TilePart *__restrict__ parts,
-                              const BigSplit *__restrict__ res, const BuildArgs &a, int32_t ti);
+                              TilePart *__restrict__ parts, BigSplit *__restrict__ res,
+                              const BuildArgs &a, int32_t ti);
+__device__ void level_done(int32_t *c, int32_t *lv, int32_t level);
 
 template <int kPass>
 __global__ __launch_bounds__(kTileThreads) void k_big_tiles(
     const float4 *__restrict__ src, float4 *__restrict__ dst, const int2 *__restrict__ tiles,
-    const Task *__restrict__ big, const TilePart *__restrict__ parts,
-    const BigSplit *__restrict__ res, BuildArgs a) {
-    const int32_t t1 = a.counters[kTileEnd];
-    for (int32_t ti = a.counters[kTileBegin] + blockIdx.x; ti < t1; ti += gridDim.x)
+    const Task *__restrict__ big, TilePart *__restrict__ parts, BigSplit *__restrict__ res,
+    BuildArgs a, int32_t *__restrict__ lv, int32_t level) {
+    // (the scatter pass makes no pushes: the counters it reads here are final)
+    if (kPass == kScatter && blockIdx.x == 0 && threadIdx.x == 0) level_done(a.counters, lv, level);
+    const int32_t t1 = lvl_tile(lv)[level];
+    for (int32_t ti = lvl_tile(lv)[level - 1] + blockIdx.x; ti < t1; ti += gridDim.x)
         big_tile_pass<kPass>(src, dst, tiles, big, parts, res, a, ti);
 }
 
 template <int kPass>
 __device__ void big_tile_pass(const float4 *__restrict__ src, float4 *__restrict__ dst,
                               const int2 *__restrict__ tiles, const Task *__restrict__ big,
-                              const TilePart *__restrict__ parts,
-                              const BigSplit *__restrict__ res, const BuildArgs &a, int32_t ti) {
+                              TilePart *__restrict__ parts, BigSplit *__restrict__ res,
+                              const BuildArgs &a, int32_t ti) {
     __shared__ int s_cnt[2][kTileThreads / kWave];
     const int2 tl = tiles[ti];
     const Task t = big[tl.x];
     const Node nd = a.nodes[t.node];
-    const BigSplit r = res[tl.x];
     const int w = threadIdx.x / kWave;
-    if (r.left < 0) {  // failed split: the node is a leaf, its order unchanged
-        if (kPass == kSides) return;
-        for (int j = threadIdx.x; j < kTile; j += kTileThreads) {
-            int32_t i = tl.y + j;
-            if (i >= nd.count) break;
-            a.fin[nd.first + i] = src[nd.first + i];
-            a.leaf_of[nd.first + i] = t.node;
+    int32_t lpre, nleft;
+    BigSplit r;
+    if (kPass == kSides) {
+        // left points of the task's earlier tiles and of all its tiles (every
+        // wavefront sums them: no barrier needed to share them)
+        const int32_t me = tl.y / kTile;
+        int32_t before = 0, all = 0;
+        for (int32_t j = lane_id(); j < t.ntiles; j += kWave) {
+            const int32_t c = parts[t.tile0 + j].nleft;
+            before += j < me ? c : 0;
+            all += c;
         }
-        return;
+        lpre = wave_sum(before);
+        nleft = wave_sum(all);
+        if (threadIdx.x == 0) parts[ti].pad[0] = lpre;
+        if (me == 0 && w == 0) split_task(parts, res, a, tl.x);
+        if (nleft == 0 || nleft == nd.count) return;  // failed split: no listing
+    } else {
+        r = res[tl.x];
+        nleft = r.nleft;
+        if (r.left < 0) {  // failed split: the node is a leaf, its order unchanged
+            for (int j = threadIdx.x; j < kTile; j += kTileThreads) {
+                int32_t i = tl.y + j;
+                if (i >= nd.count) break;
+                a.fin[nd.first + i] = src[nd.first + i];
+                a.leaf_of[nd.first + i] = t.node;
+            }
+            return;
+        }
+        lpre = parts[ti].pad[0];
     }
-    const int32_t lpre = parts[ti].pad[0];
     const int32_t rpre = tl.y - lpre;
     int32_t runL = 0, runR = 0;
     for (int rr = 0; rr < kTile / kTileThreads; rr++) {
@@ -1549,10 +1560,10 @@ __device__ void big_tile_pass(const float4 *__restrict__ src, float4 *__restrict
             const int32_t rank = f ? lpre + runL + oL + rank_below(mL)
                                    : rpre + runR + oR + rank_below(mR);
             if (kPass == kSides) {
-                a.sides[nd.first + (f ? rank : r.nleft + rank)] = i;
+                a.sides[nd.first + (f ? rank : nleft + rank)] = i;
             } else {
                 const int32_t *sd = a.sides + nd.first;
-                const int32_t d = nd.first + inplace_dest(i, f, rank, nd.count, r.nleft,
+                const int32_t d = nd.first + inplace_dest(i, f, rank, nd.count, nleft,
                                                           [&](int32_t k) { return sd[k]; });
                 const bool leaf = f ? r.leafL : r.leafR;  // left points land in [0, nleft)
                 const int32_t id = f ? r.left : r.left + 1;
@@ -1570,25 +1581,33 @@ __device__ void big_tile_pass(const float4 *__restrict__ src, float4 *__restrict
     }
 }
 
-// Moves the level state forward on the device: the pushes of the level just
-// run become the next level's work; depth_end[level] records the node count
-// (node ids grow with depth).
-__global__ void k_level_advance(int32_t *__restrict__ c, int32_t *__restrict__ depth_end) {
-    if (threadIdx.x != 0 || blockIdx.x != 0) return;
-    c[kGlobalNodes] = c[kNodeCount];
-    c[kBigBegin] = c[kBigEnd];
-    c[kBigEnd] = c[kBigNext];
-    c[kTileBegin] = c[kTileEnd];
-    c[kTileEnd] = c[kTileNext];
-    const int32_t lvl = c[kLevel];
-    if (lvl < kMaxLevels) depth_end[lvl] = c[kNodeCount];
-    else c[kErrFlags] |= kErrCapacity;
-    c[kLevel] = lvl + 1;
-    if (c[kErrFlags] & kErrCapacity) {  // a push was dropped: no further level
-        c[kBigEnd] = c[kBigBegin];
-        c[kTileEnd] = c[kTileBegin];
+// The level state on the device.  After the roots, level 1's big tasks and
+// tiles are the pushes so far; after split level L's scatter pass (whose
+// counters are final: the split happened in the sides pass), level L + 1's
+// are the pushes of level L.  depth_end[L] records the node count (node
+// ids grow with depth).  A dropped push (capacity) ends the levels.
+__device__ void level_done(int32_t *c, int32_t *lv, int32_t level) {
+    const int32_t nodes = c[kNodeCount];
+    int32_t err = c[kErrFlags];
+    if (level >= kMaxLevels) {
+        atomicOr(&c[kErrFlags], kErrCapacity);
+        err |= kErrCapacity;
+    } else {
+        lv[level] = nodes;
     }
-    c[kActive] = c[kBigEnd] > c[kBigBegin] ? 1 : 0;
+    const bool stop = (err & kErrCapacity) != 0;
+    int32_t *big = lvl_big(lv), *tile = lvl_tile(lv);
+    big[level + 1] = stop ? big[level] : c[kBigNext];
+    tile[level + 1] = stop ? tile[level] : c[kTileNext];
+    c[kGlobalNodes] = nodes;
+    c[kLevel] = level + 1;
+    c[kActive] = big[level + 1] > big[level] ? 1 : 0;
+}
+__global__ void k_levels_init(int32_t *__restrict__ c, int32_t *__restrict__ lv) {
+    if (threadIdx.x != 0 || blockIdx.x != 0) return;
+    lvl_big(lv)[0] = 0;
+    lvl_tile(lv)[0] = 0;
+    level_done(c, lv, 0);
 }
 
 // --------------------------------------------------------------- multipoles

@@ -32,6 +32,13 @@ __host__ __device__ __forceinline__ int tune_bucket(int64_t count) {
 constexpr int kFatLeaf = 64;      // leaves above this size get a wavefront
 constexpr int kTileThreads = 256;
 constexpr int kMaxLevels = 4096;  // depth_end entries kept per build
+// The level buffer: depth_end[kMaxLevels], then the split levels' ranges.
+// The big tasks of split level L are [lvl_big[L - 1], lvl_big[L]) and their
+// tiles [lvl_tile[L - 1], lvl_tile[L]); entry 0 is 0.
+constexpr int kLevelWords = 3 * kMaxLevels + 4;
+__host__ __device__ inline int32_t *lvl_big(int32_t *lv) { return lv + kMaxLevels; }
+__host__ __device__ inline int32_t *lvl_tile(int32_t *lv) { return lv + 2 * kMaxLevels + 2; }
+__host__ __device__ inline const int32_t *lvl_tile(const int32_t *lv) { return lv + 2 * kMaxLevels + 2; }
 
 // Tree node, 64 bytes, read by the walk with scalar loads.
 //   [first, first + count) is the node's range in tree order (batch-global);
@@ -117,10 +124,7 @@ enum Counter {
     kFatCount = 8,    // leaves of more than kFatLeaf points (failed splits)
     // level state, advanced on the device between levels
     kGlobalNodes = 9, // nodes made by the split levels (subtree nodes follow)
-    kBigBegin = 10,   // big tasks of the current level: [kBigBegin, kBigEnd)
-    kBigEnd = 11,
-    kTileBegin = 12,  // their tiles: [kTileBegin, kTileEnd)
-    kTileEnd = 13,
+    // 10-13 unused (the split levels' ranges are in the level buffer)
     kLevel = 14,      // levels advanced so far
     kActive = 15,     // the current level has big tasks
     kBadRoots = 16,   // calls with non-finite points, listed for their own pass

@@ -559,7 +559,7 @@ struct Engine {
         HP_GRAB(parts, sizeof(TilePart) * cap_tiles);
         HP_GRAB(dparts, sizeof(double2) * cap_tiles);
         HP_GRAB(counters, sizeof(int32_t) * kNumCounters);
-        HP_GRAB(depth_buf, sizeof(int32_t) * kMaxLevels);
+        HP_GRAB(depth_buf, sizeof(int32_t) * kLevelWords);
         HP_GRAB(work, sizeof(unsigned long long) * 4);
         if (unbind) {
             HP_GRAB(keys0, sizeof(uint64_t) * (N + 1));
@@ -697,7 +697,7 @@ struct Engine {
                                            badlist.as<int32_t>(), build_args(small0.as<Task>()));
             HP_LAUNCHED();
         }
-        k_level_advance<<<1, 1, 0, s>>>(counters.as<int32_t>(), depth_buf.as<int32_t>());
+        k_levels_init<<<1, 1, 0, s>>>(counters.as<int32_t>(), depth_buf.as<int32_t>());
         HP_LAUNCHED();
         return HP_OK;
     }
@@ -707,26 +707,24 @@ struct Engine {
     // and queues children of at most big_threshold points as subtree tasks
     int enqueue_levels(int first, int n) {
         hipStream_t s = stream;
+        if (first < 1 || first + n - 1 >= kMaxLevels)
+            return set_error(HP_EINVAL, "split levels %d..%d out of range", first, first + n - 1);
+        int32_t *lv = depth_buf.as<int32_t>();
         for (int level = first; level < first + n; level++) {
             float4 *src = (level & 1) ? A.as<float4>() : B.as<float4>();
             float4 *dst = (level & 1) ? B.as<float4>() : A.as<float4>();
             const BuildArgs ba = build_args(small0.as<Task>());
             k_big_count<<<2048, kTileThreads, 0, s>>>(src, tiles.as<int2>(), big.as<Task>(),
-                                                      nodes.as<Node>(), parts.as<TilePart>(),
-                                                      counters.as<int32_t>());
-            HP_LAUNCHED();
-            k_big_split<<<1024, 256, 0, s>>>(big.as<Task>(), parts.as<TilePart>(),
-                                          splitres.as<BigSplit>(), ba);
+                                                      nodes.as<Node>(), parts.as<TilePart>(), lv,
+                                                      level);
             HP_LAUNCHED();
             k_big_tiles<kSides><<<2048, kTileThreads, 0, s>>>(
                 src, dst, tiles.as<int2>(), big.as<Task>(), parts.as<TilePart>(),
-                splitres.as<BigSplit>(), ba);
+                splitres.as<BigSplit>(), ba, lv, level);
             HP_LAUNCHED();
             k_big_tiles<kScatter><<<2048, kTileThreads, 0, s>>>(
                 src, dst, tiles.as<int2>(), big.as<Task>(), parts.as<TilePart>(),
-                splitres.as<BigSplit>(), ba);
-            HP_LAUNCHED();
-            k_level_advance<<<1, 1, 0, s>>>(counters.as<int32_t>(), depth_buf.as<int32_t>());
+                splitres.as<BigSplit>(), ba, lv, level);
             HP_LAUNCHED();
         }
         return HP_OK;
