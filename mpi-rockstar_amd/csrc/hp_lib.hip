@@ -601,6 +601,9 @@ struct Engine {
         if ((r = upload(seg_block_l, T.seg_block))) return r;
         if ((r = upload(seg_tiles_l, T.seg_tiles))) return r;
         if (!T.scan_tiles.empty() && rec_call.p && T.nhalos > 0) {  // each record's call
+            // sized by these tables: a per-call plan reserves for its largest
+            // call but loads the whole batch's tables too
+            HP_GRAB(rec_call, sizeof(int32_t) * (T.nrec + 1));
             k_rec_calls<<<(int)T.nhalos, 256, 0, stream>>>(hfirst.as<int64_t>(), hcount.as<int64_t>(),
                                                           (int32_t)T.nhalos, rec_call.as<int32_t>());
             HP_LAUNCHED();

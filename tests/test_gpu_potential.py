@@ -307,3 +307,20 @@ def test_deep_tree_walks(restatement, walk):
     np.testing.assert_array_equal(out["flags"], o["flags"])
     r = rel(out["pe"], o["pe"])
     assert np.max(r) < 1e-5, np.max(r)
+
+
+def test_per_call_plan_on_many_small_calls():
+    """A per-call plan reserves its buffers for its largest call but loads
+    the whole batch's tables as well (each record's call is written for the
+    whole batch): 400 calls of 20 to 200 records, 100 times the largest
+    call.  The per-call run gives the batched plan's record order, bound
+    flags and counts, and its potentials to float rounding."""
+    cat = catalog.small_batch(seed=91, n_halos=400, lo=20, hi=200, potential_only=0.05)
+    cat.po["flags"] = np.arange(cat.n_records, dtype=np.int32)
+    assert cat.n_records > 50 * int(cat.halos["count"].max())
+    want = run_plan(cat)
+    got = run_plan(cat, halopot.options(per_call=1))
+    np.testing.assert_array_equal(got[0]["flags"], want[0]["flags"])
+    np.testing.assert_array_equal(got[2], want[2])
+    np.testing.assert_array_equal(got[1]["n_bound"], want[1]["n_bound"])
+    np.testing.assert_allclose(got[0]["pe"], want[0]["pe"], rtol=1e-6, atol=0)
