@@ -634,11 +634,12 @@ __global__ __launch_bounds__(256) void k_group_classes(const Task *__restrict__ 
 }
 
 // One workgroup per task of the list `cls` (nodes of at most kCap points).
-// The smallest size asks the compiler for 7 wavefronts per SIMD (72 VGPRs,
-// no spills): its LDS fits more workgroups than its registers otherwise
-// allowed.
+// The two smaller sizes ask the compiler for 7 wavefronts per SIMD (72
+// VGPRs, a few values spilled outside the level loop): their LDS fits more
+// workgroups than their registers otherwise allowed, and the latency-bound
+// levels gain more from the extra wavefronts than the spills cost.
 template <int kCap>
-__global__ __launch_bounds__(kSubThreads, kCap <= 256 ? 7 : (kCap <= 512 ? 6 : 1)) void k_subtree(const float4 *__restrict__ A,
+__global__ __launch_bounds__(kSubThreads, kCap <= 512 ? 7 : 1) void k_subtree(const float4 *__restrict__ A,
                                                          const float4 *__restrict__ B,
                                                          const Task *__restrict__ tasks,
                                                          BuildArgs a, double particle_mass,

@@ -789,7 +789,9 @@ __device__ __forceinline__ Cand wave_better(Cand c) {
 // per tile: inclusive prefix moments per record -> var_i / ((i - 3) i), and
 // the tile's best candidates for the core window and the bulk window.  One
 // wavefront per tile, rounds of 64 records carried lane 63 to the next.
-__global__ __launch_bounds__(kWave) void k_vel_select(const float4 *__restrict__ rv,
+// (6 wavefronts per SIMD asked of the compiler, 80 VGPRs: the kernel is a
+// latency chain of f64 scans and divisions per round)
+__global__ __launch_bounds__(kWave, 6) void k_vel_select(const float4 *__restrict__ rv,
                                                       const ScanTile *__restrict__ tiles,
                                                       int32_t ntiles,
                                                       const CallInfo *__restrict__ ci,
@@ -846,9 +848,8 @@ __global__ __launch_bounds__(kWave) void k_vel_select(const float4 *__restrict__
     }
 }
 
-// mean and normalised variance of the first j + 1 records, j < 10, directly
-__device__ void small_prefix(const float4 *__restrict__ rv, int32_t first, int32_t j,
-                             double *sum, double *val) {
+// sum and normalised variance of the first j + 1 records, j < 10, directly
+__device__ __forceinline__ Cand small_prefix(const float4 *__restrict__ rv, int32_t first, int32_t j) {
     double s[3] = {0, 0, 0}, q[3] = {0, 0, 0};
     for (int32_t i = 0; i <= j; i++) {
         const float4 w4 = rv[first + i];
@@ -860,11 +861,14 @@ __device__ void small_prefix(const float4 *__restrict__ rv, int32_t first, int32
     }
     const double n = (double)(j + 1);
     double var = 0.0;
+    Cand c;
     for (int k = 0; k < 3; k++) {
-        sum[k] = s[k];
+        c.s1[k] = s[k];
         var += q[k] - s[k] * s[k] / n;
     }
-    *val = j > 3 ? var / (double)((int64_t)(j - 3) * j) : 0.0;
+    c.val = j > 3 ? var / (double)((int64_t)(j - 3) * j) : 0.0;
+    c.idx = j;
+    return c;
 }
 
 // one thread per call: final selection, halo fields
@@ -876,9 +880,11 @@ __global__ void k_vel_finish(const float4 *__restrict__ rv,
     const int32_t h = blockIdx.x * blockDim.x + threadIdx.x;
     if (h >= nh || (*flags & kErrDeep)) return;
     const CallInfo c = ci[h];
-    hp_halo hh = halos[h];
-    hh.kept = c.kept;
-    hh.status = 0;
+    // the fields are written one by one (a copy of the whole record would
+    // sit in scratch)
+    hp_halo *hh = halos + h;
+    hh->kept = c.kept;
+    hh->status = 0;
     if (c.rvir_max >= 1) {
         const int32_t lim = c.rvir_max;
         const int32_t core_lim = min(c.core_max, lim);
@@ -888,25 +894,19 @@ __global__ void k_vel_finish(const float4 *__restrict__ rv,
             b = better(b, tcand[2 * t + 1]);
         }
         // windows of at most 10 records: the last forced update wins
-        if (core_lim <= 10) {
-            a.idx = core_lim - 1;
-            small_prefix(rv, c.first, a.idx, a.s1, &a.val);
-        }
-        if (lim <= 10) {
-            b.idx = lim - 1;
-            small_prefix(rv, c.first, b.idx, b.s1, &b.val);
-        }
-        hh.n_core = a.idx;
-        hh.min_vel_err = (float)a.val;
-        hh.min_bulkvel_err = (float)b.val;
+        if (core_lim <= 10) a = small_prefix(rv, c.first, core_lim - 1);
+        if (lim <= 10) b = small_prefix(rv, c.first, lim - 1);
+        hh->n_core = a.idx;
+        hh->min_vel_err = (float)a.val;
+        hh->min_bulkvel_err = (float)b.val;
         const double na = (double)a.idx + 1.0, nb = (double)b.idx + 1.0;
         for (int k = 0; k < 3; k++) {
-            hh.corevel[k] = (float)(a.s1[k] / na);
-            hh.bulkvel[k] = (float)(b.s1[k] / nb);
-            hh.pos[3 + k] = hh.corevel[k];  // src/properties.c:213
+            const float cv = (float)(a.s1[k] / na);
+            hh->corevel[k] = cv;
+            hh->bulkvel[k] = (float)(b.s1[k] / nb);
+            hh->pos[3 + k] = cv;  // src/properties.c:213
         }
     }
-    halos[h] = hh;
 }
 
 // ------------------------------------------------------- kinetic energy
