@@ -1687,10 +1687,6 @@ __global__ __launch_bounds__(256) void k_fat_leaf_com(Node *__restrict__ nodes,
     }
 }
 
-// Internal nodes of depth d: float mass and centre from the two children,
-// right term first (src/potential.c:115-122).  Node ids grow with depth:
-// depth d holds [depth_end[d - 1], depth_end[d]), depth 0 the roots; the
-// host runs the depths deepest first.
 // The device wall clock at this point of the stream.
 __global__ void k_stamp(uint64_t *t, int i) {
     if (threadIdx.x == 0 && blockIdx.x == 0) t[i] = (uint64_t)wall_clock64();
@@ -1702,6 +1698,10 @@ __global__ void k_depth_check(int32_t *__restrict__ c) {
     if (threadIdx.x == 0 && blockIdx.x == 0 && c[kActive]) c[kErrFlags] |= kErrDeep;
 }
 
+// Internal nodes of depth d: float mass and centre from the two children,
+// right term first (src/potential.c:115-122).  Node ids grow with depth:
+// depth d holds [depth_end[d - 1], depth_end[d]), depth 0 the roots; the
+// host runs the depths deepest first.
 __global__ void k_internal_com(Node *__restrict__ nodes, const int32_t *__restrict__ depth_end,
                                int32_t d) {
     const int32_t begin = d ? depth_end[d - 1] : 0, end = depth_end[d];
