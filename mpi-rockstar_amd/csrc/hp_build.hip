@@ -1108,12 +1108,16 @@ __global__ __launch_bounds__(kWave) void k_subtree_wave(const float4 *__restrict
                                                         const Task *__restrict__ tasks,
                                                         BuildArgs a, double particle_mass,
                                                         const int32_t *__restrict__ wave_off) {
+    // under 5 KB of LDS, so that 8 wavefronts fit per SIMD: the nodes' boxes
+    // go to their global records as they are found (only each node's split
+    // plane stays), parents are written from the parent's side
     __shared__ float4 P[kWaveTask];
-    __shared__ int32_t Q[kWaveTask];
-    __shared__ float s_lo[3][kWaveNodes], s_hi[3][kWaveNodes];
+    __shared__ int16_t Q[kWaveTask];
+    __shared__ float s_lim[kWaveNodes];
+    __shared__ uint8_t s_dim[kWaveNodes];
     __shared__ float s_com[3][kWaveNodes], s_mass[kWaveNodes];
     __shared__ int16_t s_lf[kWaveNodes], s_cnt[kWaveNodes], s_left[kWaveNodes];
-    __shared__ int16_t s_par[kWaveNodes], s_skip[kWaveNodes];
+    __shared__ int16_t s_skip[kWaveNodes];
     __shared__ int16_t s_lvl[kWaveTask + 2];  // first local id of each level
     const int lane = lane_id();
     const int32_t ntasks = min(a.counters[kSmallNext], a.cap_small);
@@ -1134,11 +1138,14 @@ __global__ __launch_bounds__(kWave) void k_subtree_wave(const float4 *__restrict
         const bool v = lane < n;
         float4 p = v ? src[first + lane] : make_float4(0.0f, 0.0f, 0.0f, 0.0f);
         const Node root = a.nodes[t.node];
+        // the task's node ids (2n - 2 from the scan); a subtree that does not
+        // fit keeps its root as a leaf (checked once the levels are done)
+        const int32_t base0 = id_base + wave_off[k];
+        const bool fits = base0 + wave_ids(n) <= a.cap_nodes;
         if (lane == 0) {
             s_lf[0] = 0;
             s_cnt[0] = (int16_t)n;
             s_left[0] = -1;
-            s_par[0] = -1;
             s_skip[0] = kRootSkip;
             s_lvl[0] = 0;
             s_lvl[1] = 1;
@@ -1156,10 +1163,8 @@ __global__ __launch_bounds__(kWave) void k_subtree_wave(const float4 *__restrict
                     dim = t.dim;
                     lim = t.lim;
                 } else {
-                    const float lo3[3] = {s_lo[0][nid], s_lo[1][nid], s_lo[2][nid]};
-                    const float hi3[3] = {s_hi[0][nid], s_hi[1][nid], s_hi[2][nid]};
-                    dim = split_dim(lo3, hi3);
-                    lim = split_lim(lo3, hi3, dim);
+                    dim = s_dim[nid];
+                    lim = s_lim[nid];
                 }
             }
             // src/fast3tree.c:751-774 (partition), 805-809 (failed split)
@@ -1188,7 +1193,6 @@ __global__ __launch_bounds__(kWave) void k_subtree_wave(const float4 *__restrict
                     s_lf[C + 1] = (int16_t)(lf + nl);
                     s_cnt[C + 1] = (int16_t)(c - nl);
                     s_left[C] = s_left[C + 1] = -1;
-                    s_par[C] = s_par[C + 1] = (int16_t)nid;
                     s_skip[C] = (int16_t)(C + 1);
                     s_skip[C + 1] = s_skip[nid];
                     s_left[nid] = (int16_t)C;
@@ -1216,12 +1220,17 @@ __global__ __launch_bounds__(kWave) void k_subtree_wave(const float4 *__restrict
                         h2 = seg_scan(p.z, seg, mx);
             const int cnt_new = fresh ? s_cnt[nid] : 0;
             if (fresh && lane == seg + cnt_new - 1) {
-                s_lo[0][nid] = l0;
-                s_lo[1][nid] = l1;
-                s_lo[2][nid] = l2;
-                s_hi[0][nid] = h0;
-                s_hi[1][nid] = h1;
-                s_hi[2][nid] = h2;
+                const float lo3[3] = {l0, l1, l2}, hi3[3] = {h0, h1, h2};
+                const int dm = split_dim(lo3, hi3);
+                s_dim[nid] = (uint8_t)dm;
+                s_lim[nid] = split_lim(lo3, hi3, dm);
+                if (fits) {
+                    Node *g = a.nodes + base0 + nid - 1;
+                    for (int d = 0; d < 3; d++) {
+                        g->lo[d] = lo3[d];
+                        g->hi[d] = hi3[d];
+                    }
+                }
             }
             if (M > M0) {
                 nlev++;
@@ -1231,15 +1240,9 @@ __global__ __launch_bounds__(kWave) void k_subtree_wave(const float4 *__restrict
             pend = fresh && cnt_new > kLeafPoints;
         }
         // every lane's node is a leaf now; local id j > 0 is node base + j - 1
-        // of the ids reserved for the task
-        if (lane == 0) {
-            for (int d = 0; d < 3; d++) {
-                s_lo[d][0] = root.lo[d];
-                s_hi[d][0] = root.hi[d];
-            }
-        }
-        int32_t base = id_base + wave_off[k];
-        if (M > 1 && base + wave_ids(n) > a.cap_nodes) {
+        // of the ids reserved for the task (its box is there already)
+        int32_t base = base0;
+        if (M > 1 && !fits) {
             if (lane == 0) atomicOr(&a.counters[kErrFlags], kErrCapacity);
             base = -1;
         }
@@ -1302,11 +1305,7 @@ __global__ __launch_bounds__(kWave) void k_subtree_wave(const float4 *__restrict
                 sum += d;
             }
             Node nd;
-            for (int d = 0; d < 3; d++) {
-                nd.lo[d] = s_lo[d][j];
-                nd.hi[d] = s_hi[d][j];
-                nd.com[d] = com[d];
-            }
+            for (int d = 0; d < 3; d++) nd.com[d] = com[d];
             nd.mass = s_mass[j];
             nd.first = first + lf;
             nd.count = c;
@@ -1315,8 +1314,18 @@ __global__ __launch_bounds__(kWave) void k_subtree_wave(const float4 *__restrict
             nd.skip = sk == kRootSkip ? root.skip : base + sk - 1;
             finish_dmin(&nd, big, sum, c);
             const int32_t id = j ? base + j - 1 : t.node;
-            a.nodes[id] = nd;
-            if (j) a.parent[id] = s_par[j] ? base + s_par[j] - 1 : t.node;
+            // every field but the box (the root's is the task's, the others
+            // were written with their level)
+            Node *g = a.nodes + id;
+            for (int d = 0; d < 3; d++) g->com[d] = nd.com[d];
+            g->mass = nd.mass;
+            g->dmin2 = nd.dmin2;
+            g->first = nd.first;
+            g->count = nd.count;
+            g->left = nd.left;
+            g->skip = nd.skip;
+            g->dmin = nd.dmin;
+            if (cl >= 0) a.parent[base + cl - 1] = a.parent[base + cl] = id;
         }
         __syncthreads();
     }
