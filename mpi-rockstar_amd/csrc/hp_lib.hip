@@ -1323,16 +1323,14 @@ struct Engine {
         HP_LAUNCHED();
         const double rvir_thresh = particle_rvir_dens() * (4.0 * M_PI / 3.0);
         if (nst > 0) {
-            k_rvir_tiles<<<nst, kRvirThreads, 0, s>>>(RV, scan_tiles.as<ScanTile>(), nst, ci, rvir_thresh, ti32);
+            k_rvir_vel_tiles<<<nst, kWave, 0, s>>>(RV, scan_tiles.as<ScanTile>(), nst, ci,
+                                                   rvir_thresh, ti32, tsum.as<Moments>());
             HP_LAUNCHED();
         }
         k_rvir_calls<<<blocks(H, 256), 256, 0, s>>>(RV, call_tile0.as<int32_t>(),
                                                     call_ntiles.as<int32_t>(), ti32, H, ci);
         HP_LAUNCHED();
         if (nst > 0) {
-            k_vel_sums<<<nst, kWave, 0, s>>>(RV, scan_tiles.as<ScanTile>(), nst, ci,
-                                           tsum.as<Moments>());
-            HP_LAUNCHED();
             k_vel_prefix<<<blocks(H, 256), 256, 0, s>>>(call_tile0.as<int32_t>(),
                                                         call_ntiles.as<int32_t>(), H,
                                                         tsum.as<Moments>());

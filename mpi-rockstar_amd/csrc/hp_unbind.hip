@@ -637,40 +637,7 @@ __device__ __forceinline__ int32_t block_sum_i32(int32_t v) {
 
 // rvir_max (src/properties.c:173-177): the largest j < kept with
 // j^2 > r2_j^3 (4pi/3 rho_vir)^2, r2^3 in float, the rest in double.
-// Per tile, then per call.
-// four wavefronts per tile: more loads in flight than one
-constexpr int kRvirThreads = 256;
-__global__ __launch_bounds__(kRvirThreads) void k_rvir_tiles(const float4 *__restrict__ rv,
-                                                             const ScanTile *__restrict__ tiles,
-                                                             int32_t ntiles,
-                                                             const CallInfo *__restrict__ ci,
-                                                             double rvir_thresh,
-                                                             int32_t *__restrict__ tile_rvir) {
-    __shared__ int32_t s_best[kRvirThreads / kWave];
-    const int32_t t = blockIdx.x;
-    if (t >= ntiles) return;
-    const ScanTile st = tiles[t];
-    const CallInfo c = ci[st.call];
-    const int32_t end = min(st.off + kScanTile, c.kept);
-    int32_t best = -1;
-    for (int32_t j = st.off + (int32_t)threadIdx.x; j < end; j += kRvirThreads) {
-        const float r2 = rv[c.first + j].x;
-        const float r6 = (r2 * r2) * r2;
-        if ((double)((int64_t)j * (int64_t)j) > (double)r6 * (rvir_thresh * rvir_thresh))
-            best = j > best ? j : best;
-    }
-    best = wave_max(best);
-    if (lane_id() == 0) s_best[threadIdx.x / kWave] = best;
-    __syncthreads();
-    if (threadIdx.x == 0) {
-        for (int q = 1; q < kRvirThreads / kWave; q++) best = max(best, s_best[q]);
-        tile_rvir[t] = best;
-    }
-}
-
-// per call: rvir_max over the tiles, then core_max, the last j with
-// 100 r2_j < r2[rvir_max], by binary search over the sorted call
-// (src/properties.c:179-184), at least 100
+// Per tile (k_rvir_vel_tiles, below), then per call.
 __global__ void k_rvir_calls(const float4 *__restrict__ rv,
                              const int32_t *__restrict__ call_tile0,
                              const int32_t *__restrict__ call_ntiles,
@@ -702,20 +669,30 @@ struct Moments {
     double s2[3];
 };
 
-// per-tile sums of v and v^2 over the tile's part of [0, rvir_max)
-__global__ __launch_bounds__(kWave) void k_vel_sums(const float4 *__restrict__ rv,
-                                                    const ScanTile *__restrict__ tiles,
-                                                    int32_t ntiles,
-                                                    const CallInfo *__restrict__ ci,
-                                                    Moments *__restrict__ tsum) {
+// One pass over each tile of the kept records (one wavefront per tile): the
+// last record inside the virial density threshold (rvir_max above) and the
+// tile's sums of v and v^2.  The sums cover the whole tile; only the tiles
+// wholly below rvir_max feed the core-velocity prefixes, so the records
+// past rvir_max in its own tile do not matter.
+__global__ __launch_bounds__(kWave) void k_rvir_vel_tiles(const float4 *__restrict__ rv,
+                                                          const ScanTile *__restrict__ tiles,
+                                                          int32_t ntiles,
+                                                          const CallInfo *__restrict__ ci,
+                                                          double rvir_thresh,
+                                                          int32_t *__restrict__ tile_rvir,
+                                                          Moments *__restrict__ tsum) {
     const int32_t t = blockIdx.x;
     if (t >= ntiles) return;
     const ScanTile st = tiles[t];
     const CallInfo c = ci[st.call];
-    const int32_t end = min(st.off + kScanTile, c.rvir_max);
+    const int32_t end = min(st.off + kScanTile, c.kept);
+    int32_t best = -1;
     double m[6] = {0, 0, 0, 0, 0, 0};
     for (int32_t j = st.off + lane_id(); j < end; j += kWave) {
         const float4 q = rv[c.first + j];
+        const float r6 = (q.x * q.x) * q.x;
+        if ((double)((int64_t)j * (int64_t)j) > (double)r6 * (rvir_thresh * rvir_thresh))
+            best = j > best ? j : best;
         const float v[3] = {q.y, q.z, q.w};
         for (int k = 0; k < 3; k++) {
             const double x = (double)v[k];
@@ -723,8 +700,10 @@ __global__ __launch_bounds__(kWave) void k_vel_sums(const float4 *__restrict__ r
             m[3 + k] += x * x;
         }
     }
+    best = wave_max(best);
     for (int k = 0; k < 6; k++) m[k] = wave_sum(m[k]);
     if (lane_id() == 0) {
+        tile_rvir[t] = best;
         for (int k = 0; k < 3; k++) {
             tsum[t].s1[k] = m[k];
             tsum[t].s2[k] = m[3 + k];
@@ -732,7 +711,6 @@ __global__ __launch_bounds__(kWave) void k_vel_sums(const float4 *__restrict__ r
     }
 }
 
-// per call: exclusive prefix of the tile sums, in tile order
 __global__ void k_vel_prefix(const int32_t *__restrict__ call_tile0,
                              const int32_t *__restrict__ call_ntiles, int32_t nh,
                              Moments *__restrict__ tsum) {
