@@ -866,10 +866,22 @@ __global__ void k_vel_finish(const float4 *__restrict__ rv,
     if (c.rvir_max >= 1) {
         const int32_t lim = c.rvir_max;
         const int32_t core_lim = min(c.core_max, lim);
+        // the best of the tiles' candidates, kept in scalars (a select of
+        // whole Cand records would go through scratch memory)
         Cand a = no_cand(), b = no_cand();
         for (int32_t t = call_tile0[h]; t < call_tile0[h] + call_ntiles[h]; t++) {
-            a = better(a, tcand[2 * t]);
-            b = better(b, tcand[2 * t + 1]);
+            const Cand ca = tcand[2 * t], cb = tcand[2 * t + 1];
+            const bool ta = ca.val < a.val || (ca.val == a.val && ca.idx < a.idx);
+            const bool tb = cb.val < b.val || (cb.val == b.val && cb.idx < b.idx);
+            a.val = ta ? ca.val : a.val;
+            a.idx = ta ? ca.idx : a.idx;
+            b.val = tb ? cb.val : b.val;
+            b.idx = tb ? cb.idx : b.idx;
+#pragma unroll
+            for (int k = 0; k < 3; k++) {
+                a.s1[k] = ta ? ca.s1[k] : a.s1[k];
+                b.s1[k] = tb ? cb.s1[k] : b.s1[k];
+            }
         }
         // windows of at most 10 records: the last forced update wins
         if (core_lim <= 10) a = small_prefix(rv, c.first, core_lim - 1);
