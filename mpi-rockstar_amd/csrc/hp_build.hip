@@ -293,16 +293,14 @@ __device__ void push_big(const BuildArgs &a, int32_t id, const float *lo, const 
     for (int32_t j = 0; j < nt; j++) a.tiles[t0 + j] = make_int2(bi, j * kTile);
 }
 
-// push_big made by a whole wavefront with wavefront-uniform arguments: lane 0
-// reserves the slots, the lanes write the tiles.
-__device__ void push_big_wave(const BuildArgs &a, int32_t id, const float *lo, const float *hi,
-                              int32_t count, int32_t depth, int32_t bucket) {
+// A big task queued by a whole wavefront (wavefront-uniform arguments) at a
+// task slot and tiles its caller reserved (alloc_big_and_tiles); the lanes
+// write the tiles.
+__device__ void push_big_wave_at(const BuildArgs &a, int32_t bi, int32_t t0, int32_t id,
+                                 const float *lo, const float *hi, int32_t count, int32_t depth,
+                                 int32_t bucket) {
     const int lane = lane_id();
     const int32_t nt = (count + kTile - 1) / kTile;
-    uint64_t al = 0;
-    if (lane == 0) al = alloc_big_and_tiles(a, nt);
-    al = read_lane(al, 0);
-    const int32_t bi = (int32_t)al, t0 = (int32_t)(al >> 32);
     if (bi >= a.cap_big || t0 + nt > a.cap_tiles) {
         if (lane == 0) atomicOr(&a.counters[kErrFlags], kErrCapacity);
         return;
@@ -489,12 +487,14 @@ __global__ __launch_bounds__(256) void k_roots_bad(const int64_t *__restrict__ h
 
 // Writes the two children of a split node and queues those that need a
 // split: small ones at slots P, P + 1 (allocated with the node ids), big ones
-// with their tiles.
-// Called by a whole wavefront with wavefront-uniform arguments.  P and G
-// are the slots reserved in the wavefront and workgroup subtree lists.
+// at task slots B, B + 1 with their tiles from T0.
+// Called by a whole wavefront with wavefront-uniform arguments.  P, G, B
+// and T0 are reserved by the caller in the wavefront and workgroup subtree
+// lists and the big-task list.
 __device__ void make_children(const BuildArgs &a, const Task &t, const Node &nd, int32_t L,
-                              int32_t P, int32_t G, int32_t nl, const float *loL, const float *hiL,
-                              const float *loR, const float *hiR) {
+                              int32_t P, int32_t G, int32_t B, int32_t T0, int32_t nl,
+                              const float *loL, const float *hiL, const float *loR,
+                              const float *hiR) {
     const int32_t nr = nd.count - nl;
     const Sizes sz = sizes_of(a, t.pad[0]);
     if (lane_id() == 0) {
@@ -507,8 +507,11 @@ __device__ void make_children(const BuildArgs &a, const Task &t, const Node &nd,
         if (wave_sized(sz, nr)) push_small_at(a, P, L + 1, loR, hiR, t.depth + 1, nd.first + nl, nr);
         if (group_sized(sz, nr)) push_group_at(a, G, L + 1, loR, hiR, t.depth + 1, nd.first + nl, nr);
     }
-    if (nl > sz.big) push_big_wave(a, L, loL, hiL, nl, t.depth + 1, t.pad[0]);
-    if (nr > sz.big) push_big_wave(a, L + 1, loR, hiR, nr, t.depth + 1, t.pad[0]);
+    // big children at the task slots from B and the tiles from T0 (left first)
+    if (nl > sz.big) push_big_wave_at(a, B, T0, L, loL, hiL, nl, t.depth + 1, t.pad[0]);
+    if (nr > sz.big)
+        push_big_wave_at(a, B + (nl > sz.big ? 1 : 0), T0 + (nl > sz.big ? (nl + kTile - 1) / kTile : 0),
+                         L + 1, loR, hiR, nr, t.depth + 1, t.pad[0]);
 }
 
 __device__ __forceinline__ int32_t wave_pushes(const Sizes &s, int32_t nl, int32_t nr) {
@@ -1411,14 +1414,14 @@ __device__ void big_count_tile(const float4 *__restrict__ src, const int2 *__res
     block_box_reduce(lo, hi, nl, parts + ti);
 }
 
-// The split of big task bi, by one wavefront once its tiles are counted:
-// the left count and the children's boxes from the tiles' parts (lanes over
-// tiles), then the children.
+// The split of big task bi (t, its node nd), by one wavefront once its
+// tiles are counted: the left count and the children's boxes from the tiles'
+// parts (lanes over tiles), then the children.  The node ids and subtree
+// slots, the workgroup-subtree slots and the big children's task slots and
+// tiles are reserved by three lanes at once (one round trip).
 __device__ void split_task(const TilePart *__restrict__ parts, BigSplit *__restrict__ res,
-                           const BuildArgs &a, int32_t bi) {
+                           const BuildArgs &a, int32_t bi, const Task &t, const Node &nd) {
     const int lane = lane_id();
-    const Task t = a.big[bi];
-    const Node nd = a.nodes[t.node];
     const float inf = __int_as_float(0x7f800000), ninf = __int_as_float((int)0xff800000);
     float lo[2][3] = {{inf, inf, inf}, {inf, inf, inf}};
     float hi[2][3] = {{ninf, ninf, ninf}, {ninf, ninf, ninf}};
@@ -1447,16 +1450,21 @@ __device__ void split_task(const TilePart *__restrict__ parts, BigSplit *__restr
         if (lane == 0) note_fat_leaf(a, t.node, nd.count);
     } else {
         const int32_t nr = nd.count - nl;
-        uint64_t al = 0;
+        const Sizes sz = sizes_of(a, t.pad[0]);
+        const int32_t ng = group_pushes(sz, nl, nr);
+        const int32_t ntl = nl > sz.big ? (nl + kTile - 1) / kTile : 0;
+        const int32_t ntr = nr > sz.big ? (nr + kTile - 1) / kTile : 0;
+        const int32_t nbig = (ntl ? 1 : 0) + (ntr ? 1 : 0);
+        uint64_t al = 0, bal = 0;
         int32_t G = 0;
-        if (lane == 0) {
-            const Sizes sz = sizes_of(a, t.pad[0]);
-            al = alloc_nodes_and_small(a, 2, wave_pushes(sz, nl, nr));
-            const int32_t ng = group_pushes(sz, nl, nr);
-            if (ng) G = atomicAdd(&a.counters[kGroupNext], ng);
-        }
+        if (lane == 0) al = alloc_nodes_and_small(a, 2, wave_pushes(sz, nl, nr));
+        if (lane == 1 && ng) G = atomicAdd(&a.counters[kGroupNext], ng);
+        if (lane == 2 && nbig)
+            bal = atomicAdd(reinterpret_cast<unsigned long long *>(a.counters + kBigNext),
+                            (unsigned long long)nbig | ((unsigned long long)(ntl + ntr) << 32));
         al = read_lane(al, 0);
-        G = read_lane(G, 0);
+        G = read_lane(G, 1);
+        bal = read_lane(bal, 2);
         const int32_t L = (int32_t)al, P = (int32_t)(al >> 32);
         if (L + 2 > a.cap_nodes) {
             if (lane == 0) atomicOr(&a.counters[kErrFlags], kErrCapacity);
@@ -1464,7 +1472,8 @@ __device__ void split_task(const TilePart *__restrict__ parts, BigSplit *__restr
             r.left = L;
             r.leafL = nl <= kLeafPoints;
             r.leafR = nr <= kLeafPoints;
-            make_children(a, t, nd, L, P, G, nl, lo[0], hi[0], lo[1], hi[1]);
+            make_children(a, t, nd, L, P, G, (int32_t)bal, (int32_t)(bal >> 32), nl, lo[0], hi[0],
+                          lo[1], hi[1]);
         }
     }
     if (lane == 0) res[bi] = r;
@@ -1527,7 +1536,7 @@ __device__ void big_tile_pass(const float4 *__restrict__ src, float4 *__restrict
         lpre = wave_sum(before);
         nleft = wave_sum(all);
         if (threadIdx.x == 0) parts[ti].pad[0] = lpre;
-        if (me == 0 && w == 0) split_task(parts, res, a, tl.x);
+        if (me == 0 && w == 0) split_task(parts, res, a, tl.x, t, nd);
         if (nleft == 0 || nleft == nd.count) return;  // failed split: no listing
     } else {
         r = res[tl.x];
