@@ -894,6 +894,14 @@ struct Engine {
         return HP_OK;
     }
 
+    // whether some call of the tables has more points than its size bucket's
+    // split size (the root gather counts only finite points, never more)
+    bool may_split(const Tables &T) const {
+        for (int64_t h = 0; h < T.nhalos; h++)
+            if (T.count[h] > bucket_big[tune_bucket(T.count[h])]) return true;
+        return false;
+    }
+
     int check_walk() {
         int r;
         if ((r = read_counters())) return r;
@@ -918,10 +926,18 @@ struct Engine {
         int r;
         if (timing) HP_TRY(hipEventRecord(ev[0], s));
         if ((r = enqueue_roots(T, d_po))) return r;
+        bool build_checked = true;
         if (T.method != HP_METHOD_DIRECT) {
-            int next = 1;
-            if ((r = finish_levels(1, &next))) return r;
-            if ((r = check_build())) return r;
+            if (may_split(T)) {
+                int next = 1;
+                if ((r = finish_levels(1, &next))) return r;
+                if ((r = check_build())) return r;
+            } else {
+                // no call above its split size: no split level has work, so
+                // none is run, and the build's state is read after the walk
+                st.levels = 0;
+                build_checked = false;
+            }
             if (timing) HP_TRY(hipEventRecord(ev[1], s));
             if ((r = enqueue_multipole((int)st.levels))) return r;
         } else if (timing) {
@@ -939,6 +955,7 @@ struct Engine {
             HP_TRY(hipMemcpyAsync(d_po, stage.p, sizeof(struct potential) * T.nrec,
                                   hipMemcpyDeviceToDevice, s));
         }
+        if (!build_checked && ((r = read_counters()) || (r = check_build()))) return r;
         if ((r = check_walk())) return r;
         st.walk_groups = (int64_t)T.groups.size();
         st.walk_launches = T.groups.empty() ? 0 : 1;
