@@ -584,8 +584,10 @@ struct Engine {
         return HP_OK;
     }
 
+    int64_t max_count = 0;  // the largest call of the loaded tables
     int load_tables(const Tables &T) {
         int r;
+        max_count = T.count.empty() ? 0 : *std::max_element(T.count.begin(), T.count.end());
         if ((r = upload(hfirst, T.first))) return r;
         if ((r = upload(hcount, T.count))) return r;
         if ((r = upload(root_tiles, T.root_tiles))) return r;
@@ -742,22 +744,32 @@ struct Engine {
         hipStream_t s = stream;
         const int32_t *cnt = counters.as<int32_t>();
         const BuildArgs ba = build_args(small0.as<Task>());
-        k_group_classes<<<blocks(cap_group, 256), 256, 0, s>>>(group0.as<Task>(), ba, groupc.as<Task>());
-        HP_LAUNCHED();
-        // largest first: the big subtrees hold most points and the fewest workgroups per CU
-        if (big_max() > 512) {
+        // A workgroup subtree kernel (and the sorting of its tasks by size) is
+        // launched only if a task of its size can exist: a workgroup task
+        // holds more than the wavefront size and at most both the split size
+        // and the largest call.  Largest first: the big subtrees hold most
+        // points and the fewest workgroups per CU.
+        const int64_t gmax = std::min<int64_t>(max_count, big_max());
+        if (gmax > kWaveMin) {
+            k_group_classes<<<blocks(cap_group, 256), 256, 0, s>>>(group0.as<Task>(), ba,
+                                                                   groupc.as<Task>());
+            HP_LAUNCHED();
+        }
+        if (gmax > 512) {
             k_subtree<1024><<<sub_grid[2], kSubThreads, 0, s>>>(
                 A.as<float4>(), B.as<float4>(), groupc.as<Task>() + 2 * cap_group, ba, PARTICLE_MASS, 2);
             HP_LAUNCHED();
         }
-        if (big_max() > 256) {
+        if (gmax > 256) {
             k_subtree<512><<<sub_grid[1], kSubThreads, 0, s>>>(
                 A.as<float4>(), B.as<float4>(), groupc.as<Task>() + cap_group, ba, PARTICLE_MASS, 1);
             HP_LAUNCHED();
         }
-        k_subtree<256><<<sub_grid[0], kSubThreads, 0, s>>>(A.as<float4>(), B.as<float4>(),
-                                                           groupc.as<Task>(), ba, PARTICLE_MASS, 0);
-        HP_LAUNCHED();
+        if (gmax > kWaveMin) {
+            k_subtree<256><<<sub_grid[0], kSubThreads, 0, s>>>(A.as<float4>(), B.as<float4>(),
+                                                               groupc.as<Task>(), ba, PARTICLE_MASS, 0);
+            HP_LAUNCHED();
+        }
         {  // the wavefront subtrees' node ids: exclusive scan of 2n - 2 over the list
             const int32_t m = (int32_t)cap_small + 1;
             const int32_t nbk = (m + kScanBlock - 1) / kScanBlock;
