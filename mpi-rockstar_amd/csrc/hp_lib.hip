@@ -789,8 +789,13 @@ struct Engine {
                                               build_args(small0.as<Task>()), PARTICLE_MASS,
                                               wave_off.as<int32_t>());
         HP_LAUNCHED();
+        // the roots that stay leaves (at most 10 points, or as many finite)
         k_leaf_com<<<2048, 256, 0, s>>>(nodes.as<Node>(), F.as<float4>(), cnt, PARTICLE_MASS);
         HP_LAUNCHED();
+        // Without split levels every other node is a subtree's, and the
+        // subtree kernels finished it (its root included): the fat leaves,
+        // the internal nodes and dmin of the levels' nodes have no work.
+        if (levels == 0) return HP_OK;
         k_fat_leaf_com<<<256, 256, 0, s>>>(nodes.as<Node>(), fat.as<int32_t>(), cnt,
                                            (int32_t)cap_fat, F.as<float4>(), PARTICLE_MASS);
         HP_LAUNCHED();
