@@ -333,6 +333,10 @@ static int make_tables(Tables &T, const int64_t *first, const int64_t *count, in
 struct Engine {
     int device = -1;
     hipStream_t stream = nullptr;
+    // streams (default priority) on which a recorded step runs the subtree
+    // kernels of the other sizes beside the largest, and their events
+    hipStream_t side[3] = {nullptr, nullptr, nullptr};
+    hipEvent_t fork_ev = nullptr, join_ev[3] = {nullptr, nullptr, nullptr};
     int32_t *h_counters = nullptr;  // pinned mirror
     unsigned long long *h_work = nullptr;
     uint64_t *h_stamps = nullptr;   // pinned: wall-clock stamps between a graph step's phases
@@ -403,6 +407,11 @@ struct Engine {
         else
             (void)hipGetLastError();
         for (auto &e : ev) HP_TRY(hipEventCreate(&e));
+        for (int b = 0; b < 3; b++) {
+            HP_TRY(hipStreamCreateWithFlags(&side[b], hipStreamDefault));
+            HP_TRY(hipEventCreateWithFlags(&join_ev[b], hipEventDisableTiming));
+        }
+        HP_TRY(hipEventCreateWithFlags(&fork_ev, hipEventDisableTiming));
         // The workgroup subtree kernels take tasks from dynamic queues, the
         // wavefront one at a fixed stride, until the lists are empty; their
         // grids are the workgroups that fit on the device at once.
@@ -511,6 +520,14 @@ struct Engine {
             if (h_counters) (void)hipHostFree(h_counters);
             if (h_work) (void)hipHostFree(h_work);
             if (h_stamps) (void)hipHostFree(h_stamps);
+            if (fork_ev) (void)hipEventDestroy(fork_ev);
+            for (int b = 0; b < 3; b++) {
+                if (join_ev[b]) (void)hipEventDestroy(join_ev[b]);
+                if (side[b]) {
+                    (void)hipStreamSynchronize(side[b]);
+                    (void)hipStreamDestroy(side[b]);
+                }
+            }
             (void)hipStreamDestroy(stream);
         }
     }
@@ -740,8 +757,9 @@ struct Engine {
     }
 
     // subtrees, then the multipoles of the nodes of `levels` split levels
-    int enqueue_multipole(int levels) {
+    int enqueue_multipole(int levels, bool fork = false) {
         hipStream_t s = stream;
+        fork = fork && side[0];
         const int32_t *cnt = counters.as<int32_t>();
         const BuildArgs ba = build_args(small0.as<Task>());
         // A workgroup subtree kernel (and the sorting of its tasks by size) is
@@ -753,21 +771,6 @@ struct Engine {
         if (gmax > kWaveMin) {
             k_group_classes<<<blocks(cap_group, 256), 256, 0, s>>>(group0.as<Task>(), ba,
                                                                    groupc.as<Task>());
-            HP_LAUNCHED();
-        }
-        if (gmax > 512) {
-            k_subtree<1024><<<sub_grid[2], kSubThreads, 0, s>>>(
-                A.as<float4>(), B.as<float4>(), groupc.as<Task>() + 2 * cap_group, ba, PARTICLE_MASS, 2);
-            HP_LAUNCHED();
-        }
-        if (gmax > 256) {
-            k_subtree<512><<<sub_grid[1], kSubThreads, 0, s>>>(
-                A.as<float4>(), B.as<float4>(), groupc.as<Task>() + cap_group, ba, PARTICLE_MASS, 1);
-            HP_LAUNCHED();
-        }
-        if (gmax > kWaveMin) {
-            k_subtree<256><<<sub_grid[0], kSubThreads, 0, s>>>(A.as<float4>(), B.as<float4>(),
-                                                               groupc.as<Task>(), ba, PARTICLE_MASS, 0);
             HP_LAUNCHED();
         }
         {  // the wavefront subtrees' node ids: exclusive scan of 2n - 2 over the list
@@ -785,10 +788,43 @@ struct Engine {
             k_wave_base<<<1, 64, 0, s>>>(ba, off);
             HP_LAUNCHED();
         }
-        k_subtree_wave<<<wave_grid, kWave, 0, s>>>(A.as<float4>(), B.as<float4>(), small0.as<Task>(),
+        // recorded steps: the other sizes and the wavefront subtrees on their
+        // own streams, beside the largest (their task lists and node ids are
+        // independent once the ids are reserved), so that each kernel's last
+        // tasks overlap the others' instead of leaving the device idle
+        hipStream_t s1 = s, s2 = s, s3 = s;
+        if (fork) {
+            HP_TRY(hipEventRecord(fork_ev, s));
+            for (int b = 0; b < 3; b++) HP_TRY(hipStreamWaitEvent(side[b], fork_ev, 0));
+            s1 = side[0];
+            s2 = side[1];
+            s3 = side[2];
+        }
+        if (gmax > 512) {
+            k_subtree<1024><<<sub_grid[2], kSubThreads, 0, s>>>(
+                A.as<float4>(), B.as<float4>(), groupc.as<Task>() + 2 * cap_group, ba, PARTICLE_MASS, 2);
+            HP_LAUNCHED();
+        }
+        if (gmax > 256) {
+            k_subtree<512><<<sub_grid[1], kSubThreads, 0, s1>>>(
+                A.as<float4>(), B.as<float4>(), groupc.as<Task>() + cap_group, ba, PARTICLE_MASS, 1);
+            HP_LAUNCHED();
+        }
+        if (gmax > kWaveMin) {
+            k_subtree<256><<<sub_grid[0], kSubThreads, 0, s2>>>(A.as<float4>(), B.as<float4>(),
+                                                               groupc.as<Task>(), ba, PARTICLE_MASS, 0);
+            HP_LAUNCHED();
+        }
+        k_subtree_wave<<<wave_grid, kWave, 0, s3>>>(A.as<float4>(), B.as<float4>(), small0.as<Task>(),
                                               build_args(small0.as<Task>()), PARTICLE_MASS,
                                               wave_off.as<int32_t>());
         HP_LAUNCHED();
+        if (fork) {
+            for (int b = 0; b < 3; b++) {
+                HP_TRY(hipEventRecord(join_ev[b], side[b]));
+                HP_TRY(hipStreamWaitEvent(s, join_ev[b], 0));
+            }
+        }
         // the roots that stay leaves (at most 10 points, or as many finite)
         k_leaf_com<<<2048, 256, 0, s>>>(nodes.as<Node>(), F.as<float4>(), cnt, PARTICLE_MASS);
         HP_LAUNCHED();
@@ -1065,7 +1101,7 @@ struct Engine {
                  }
                  k_stamp<<<1, 1, 0, stream>>>(h_stamps, 1);
                  HP_LAUNCHED();
-                 if (tree && (q = enqueue_multipole(levels))) return q;
+                 if (tree && (q = enqueue_multipole(levels, true))) return q;
                  k_stamp<<<1, 1, 0, stream>>>(h_stamps, 2);
                  HP_LAUNCHED();
                  if ((q = enqueue_walk(T, d_po, o))) return q;
