@@ -123,7 +123,8 @@ class Restatement:
                                          C.c_void_p, C.c_int64]
         L.orc_subfof.restype = C.c_int
         L.orc_subfof.argtypes = [C.c_void_p, C.c_int64, C.c_float, C.c_int, C.c_double, C.c_int64,
-                                 C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]
+                                 C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p,
+                                 C.c_int64, C.c_void_p, C.c_void_p, C.c_int64, C.c_void_p]
         L.orc_subfof_batch.argtypes = [C.c_void_p, C.c_void_p, C.c_int64, C.c_float, C.c_int,
                                        C.c_double, C.c_int64, C.c_void_p, C.c_void_p, C.c_int]
         L.orc_next_closest.argtypes = [C.c_void_p, C.c_int64, C.c_void_p, C.c_int64, C.c_void_p]
@@ -246,24 +247,34 @@ class Restatement:
         return self.lib.orc_hubble_scaling(C.byref(_params(prm)), z)
 
     def subfof(self, parts: np.ndarray, thresh: float = 0.7, top: bool = True,
-               scaling: float = 1.0, exact_ll: int = 0) -> dict:
+               scaling: float = 1.0, exact_ll: int = 0, min_p: int = 10) -> dict:
         """The sub-FOF step of one group (orc_subfof): the particles
         normalised and in tree order, the sample's next-closest distances,
-        the linking length and each particle's component label (smallest
-        tree position of its component, -1 without a neighbour)."""
+        the linking length, each particle's component label (smallest tree
+        position of its component, -1 without a neighbour), and the
+        reference's own output: the particles sorted by its union-find roots
+        ("final") and its sub-FOFs of at least min_p particles in root order
+        ("fofs", rows of first and num_p)."""
         assert parts.dtype.itemsize == 32
         pp = np.ascontiguousarray(parts).copy()
         n = pp.shape[0]
         sample = np.zeros(max(n, 10000), np.float32)
         label = np.zeros(max(1, n), np.int64)
+        final = np.zeros(max(1, n), pp.dtype)
+        cap = n // 2 + 2
+        ff, fn = np.zeros(cap, np.int64), np.zeros(cap, np.int64)
+        nf = C.c_int64()
         tr = C.c_float()
         ns = C.c_int64()
         rc = self.lib.orc_subfof(_p(pp), n, thresh, int(top), scaling, exact_ll, C.byref(tr),
-                                 _p(sample), C.byref(ns), _p(label))
+                                 _p(sample), C.byref(ns), _p(label), _p(final), min_p, _p(ff),
+                                 _p(fn), cap, C.byref(nf))
         if rc:
             raise ValueError("non-finite coordinate")
+        k = min(nf.value, cap)
         return {"tree_order": pp, "target_r": np.float32(tr.value),
-                "sample_r": sample[:ns.value], "label": label[:n]}
+                "sample_r": sample[:ns.value], "label": label[:n], "final": final[:n],
+                "fofs": np.stack([ff[:k], fn[:k]], 1)}
 
     def subfof_batch(self, parts: np.ndarray, groups: np.ndarray, thresh: float = 0.7,
                      top: bool = True, scaling: float = 1.0, exact_ll: int = 0,

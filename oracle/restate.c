@@ -1373,8 +1373,9 @@ int64_t orc_sizeof_tree_halo(void) { return (int64_t)sizeof(orc_tree_halo); }
  * the reference does: the sample indices and the linking loop both walk the
  * array in the order the rebuild leaves it.  The linking result is given as
  * components (label = smallest tree position in the component, -1 for a
- * particle with no neighbour), not in the order of the reference's
- * union-find roots. */
+ * particle with no neighbour) and, by orc6_fofs, as the reference's own
+ * output: the particles sorted by its union-find roots and the sub-FOFs in
+ * root order. */
 #define ORC_LEAF6 40
 
 typedef struct orc_node6 {
@@ -1681,11 +1682,153 @@ static void orc6_link(const orc_tree6 *t, int64_t n, float r, int64_t *label) {
     free(up);
 }
 
+/* --- the reference's own output order: the linking loop of
+ * _find_subfofs_at_r (src/groupies.c:274-285) and build_fullfofs
+ * (src/fof.c:223-252), step by step.  A particle's sphere search
+ * (fast3tree_find_sphere_skip, src/fast3tree.c:419-465) lists the particle
+ * itself and then every later particle closer than r, in array order (the
+ * depth-first walk meets the leaves in array order).  Small FOFs are
+ * numbered as they are created; sf_up[] is the reference's root field. */
+static void orc_sf_collapse(int64_t *sf_up, int64_t f) {  /* src/fof.c:58-70 */
+    if (sf_up[sf_up[f]] == sf_up[f]) return;
+    int64_t top = sf_up[f];
+    while (sf_up[top] != top) top = sf_up[top];
+    while (f != top) {
+        const int64_t next = sf_up[f];
+        sf_up[f] = top;
+        f = next;
+    }
+}
+
+/* merge_smallfofs(f1, f2), src/fof.c:72-89: f2's tree ends under f1's root */
+static void orc_sf_merge(int64_t *sf_up, int64_t f1, int64_t f2) {
+    if (sf_up[f1] == sf_up[f2]) return;
+    orc_sf_collapse(sf_up, f1);
+    const int64_t top = sf_up[f1];
+    if (sf_up[f2] == f2) {
+        sf_up[f2] = top;
+        return;
+    }
+    for (;;) {  /* every link on f2's path to its root is pointed at top */
+        const int64_t next = sf_up[f2];
+        sf_up[f2] = top;
+        if (next == top) break;
+        f2 = next;
+    }
+}
+
+/* partition_sort_particles, src/fof.c:184-221: recursive in-place
+   partitions about the middle of each range's assignment values */
+static void orc_sf_partition(orc_particle *p, int64_t *a, int64_t lo, int64_t hi) {
+    while (hi - lo >= 2) {
+        int64_t vmin = a[lo], vmax = a[lo];
+        for (int64_t i = lo + 1; i < hi; i++) {
+            if (a[i] > vmax) vmax = a[i];
+            if (a[i] < vmin) vmin = a[i];
+        }
+        if (vmin == vmax) return;
+        const int64_t pivot = vmin + (vmax - vmin) / 2;
+        int64_t i = lo, last = hi - 1;
+        while (i < last) {
+            if (a[i] > pivot) {  /* swap with the last unexamined slot, look again */
+                const orc_particle tp = p[i];
+                p[i] = p[last];
+                p[last] = tp;
+                const int64_t ta = a[i];
+                a[i] = a[last];
+                a[last] = ta;
+                last--;
+            } else {
+                i++;
+            }
+        }
+        if (a[last] <= pivot) last++;
+        orc_sf_partition(p, a, lo, last);
+        lo = last;
+    }
+}
+
+/* The linking at r on the tree order, then out = the tree order's particles
+   sorted by root as the reference leaves them, and its sub-FOFs of at least
+   min_p particles in root order; returns their count (at most cap are
+   written). */
+static int64_t orc6_fofs(const orc_tree6 *t, int64_t n, float r, int64_t min_p,
+                         orc_particle *out, int64_t *fof_first, int64_t *fof_num, int64_t cap) {
+    int64_t *sf = malloc(sizeof(int64_t) * (size_t)(n ? n : 1));   /* particle_smallfofs */
+    int64_t *sf_up = malloc(sizeof(int64_t) * (size_t)(n ? n : 1));
+    int64_t *nb = malloc(sizeof(int64_t) * (size_t)(n ? n : 1));
+    int64_t *stack = malloc(sizeof(int64_t) * (size_t)(2 * t->nn + 2));
+    int64_t nsf = 0;
+    const float r2 = r * r;
+    for (int64_t i = 0; i < n; i++) sf[i] = -1;
+    for (int64_t i = 0; i < n; i++) {
+        /* the search result after the particle itself: later points, in order */
+        int64_t k = 0, sp = 0;
+        const float *c = t->p[i].pos;
+        stack[sp++] = 0;
+        while (sp) {
+            const orc_node6 *nd = t->nd + stack[--sp];
+            if (nd->start + nd->n <= i || orc6_apart(nd, c, r)) continue;
+            if (nd->div >= 0) {
+                stack[sp++] = nd->right;
+                stack[sp++] = nd->left;
+                continue;
+            }
+            for (int64_t j = nd->start > i + 1 ? nd->start : i + 1; j < nd->start + nd->n; j++)
+                if (orc6_dist2(c, t->p[j].pos) < r2) nb[k++] = j;
+        }
+        if (!k) continue;  /* fewer than two points: src/fof.c:117 */
+        /* link_particle_to_fof, src/fof.c:113-137 */
+        int64_t f = sf[i];
+        if (f < 0)
+            for (int64_t q = 0; q < k && f < 0; q++) f = sf[nb[q]];
+        if (f < 0) {
+            sf_up[nsf] = nsf;
+            f = nsf++;
+        }
+        for (int64_t q = -1; q < k; q++) {
+            const int64_t x = q < 0 ? i : nb[q];
+            if (sf[x] < 0) sf[x] = f;
+            else orc_sf_merge(sf_up, sf[x], f);
+        }
+    }
+    /* collapse_smallfofs, src/fof.c:159-170 */
+    for (int64_t s = 0; s < nsf; s++) orc_sf_collapse(sf_up, s);
+    for (int64_t i = 0; i < n; i++)
+        if (sf[i] >= 0) sf[i] = sf_up[sf[i]];
+    memcpy(out, t->p, sizeof(orc_particle) * (size_t)n);
+    orc_sf_partition(out, sf, 0, n);
+    /* build_fullfofs: one per run of equal roots, kept at min_p or more */
+    int64_t nf = 0;
+    for (int64_t i = 0; i < n;) {
+        int64_t e = i + 1;
+        while (e < n && sf[e] == sf[i]) e++;
+        if (sf[i] >= 0 && e - i >= min_p) {
+            if (nf < cap) {
+                fof_first[nf] = i;
+                fof_num[nf] = e - i;
+            }
+            nf++;
+        }
+        i = e;
+    }
+    free(stack);
+    free(nb);
+    free(sf_up);
+    free(sf);
+    return nf;
+}
+
 /* the whole step for one group, in place on pp (normalised, tree order);
-   returns -1 (nothing done) when a coordinate is not finite */
+   returns -1 (nothing done) when a coordinate is not finite.  With final
+   given: final = the particles as the reference leaves them (sorted by
+   union-find root) and its sub-FOFs (first, num_p) of at least min_p
+   particles, *nfof of them (at most cap written). */
 int orc_subfof(orc_particle *pp, int64_t n, float thresh, int top, double scaling,
                int64_t exact_ll, float *target_r, float *sample_r, int64_t *nsample,
-               int64_t *label) {
+               int64_t *label, orc_particle *final, int64_t min_p, int64_t *fof_first,
+               int64_t *fof_num, int64_t cap, int64_t *nfof) {
+    if (nfof) *nfof = 0;
     for (int64_t i = 0; i < n; i++)
         for (int j = 0; j < 6; j++)
             if (!isfinite(pp[i].pos[j])) return -1;
@@ -1717,6 +1860,10 @@ int orc_subfof(orc_particle *pp, int64_t n, float thresh, int top, double scalin
     *target_r = orc_find_median_r(tmp, ns, thresh, &seed);
     free(tmp);
     orc6_link(&t, n, *target_r, label);
+    if (final) {
+        const int64_t nf = orc6_fofs(&t, n, *target_r, min_p, final, fof_first, fof_num, cap);
+        if (nfof) *nfof = nf;
+    }
     free(t.nd);
     return 0;
 }
@@ -1748,7 +1895,7 @@ void orc_subfof_batch(orc_particle *pp, const orc_fof_group *g, int64_t ng, floa
         int64_t ns, n = g[i].num_p;
         float *s = malloc(sizeof(float) * (size_t)(n > 10000 ? n : 10000));
         orc_subfof(pp + g[i].first, n, thresh, top, scaling, exact_ll, target_r + i, s, &ns,
-                   label + g[i].first);
+                   label + g[i].first, NULL, 0, NULL, NULL, 0, NULL);
         free(s);
     }
 }

@@ -56,6 +56,10 @@ def _compare(ref, restatement, p, **kw):
     assert r["target_r"] == o["target_r"]
     assert _ref_sets(r) == _sets_from_labels(o["label"], o["tree_order"]["id"])
     assert np.array_equal(r["fofs"], r["fofs_at_r"])   # the harness's own sequence
+    # the reference's output order: particles sorted by union-find root, the
+    # sub-FOFs in root order
+    assert r["final"].tobytes() == o["final"].tobytes()
+    assert np.array_equal(r["fofs"], o["fofs"])
     return r, o
 
 
@@ -88,6 +92,8 @@ def test_restatement_matches_reference_exact_ll_and_lower_levels(ref, restatemen
             assert r["tree_order"].tobytes() == o["tree_order"].tobytes()
             assert r["target_r"] == o["target_r"] and r["sample_r"].shape[0] == n
             assert _ref_sets(r) == _sets_from_labels(o["label"], o["tree_order"]["id"])
+            assert r["final"].tobytes() == o["final"].tobytes()
+            assert np.array_equal(r["fofs"], o["fofs"])
     finally:
         ref.set_subfof_config()
     for f, n in fg.groups[:3]:
