@@ -298,10 +298,11 @@ typedef struct hp_subfof_config {   /* src/config.template.h:96,112,114,118 */
    every group of a batch at once: norm_sd, the in-place 6D fast3tree
    rebuild, the sample's next-closest distances, find_median_r and the
    linking at that length.  On return each group's particles in d_particles
-   are normalised (as norm_sd leaves them) and grouped by sub-FOF; the
-   sub-FOFs of MIN_HALO_PARTICLES or more are listed in subs (in the order
-   of each component's first particle in the tree order; the reference lists
-   them in the order of its union-find roots).  Groups must not overlap.
+   are normalised (as norm_sd leaves them) and in the reference's order:
+   sorted by its union-find roots with partition_sort_particles
+   (src/fof.c:184-252), the particles with no neighbour first; the sub-FOFs
+   of MIN_HALO_PARTICLES or more are listed in subs in the reference's root
+   order (build_fullfofs).  Groups must not overlap.
    NULL config: the reference's defaults.  Returns HP_OK or an error code
    (non-finite coordinates, overlapping groups, too small a subs array). */
 int hp_find_subfofs(struct particle *d_particles, int64_t nparticles, hp_fof_group *groups,

@@ -194,13 +194,20 @@ struct DevBuf {
 // Device scratch of the sub-FOF step (hp_find_subfofs).
 struct SubBufs {
     DevBuf groups, tiles, qtiles, ltiles, drawn, qidx, psum, mean, pcorr, sig, pbox, gerr, w, w2, rk, bpos, spos,
-        nodes, act0, act1, counters, slot_group, dist, sel, sel2, target, up, has, csize, emit,
-        fused;
+        nodes, act0, act1, counters, slot_group, dist, sel, sel2, target, csize, emit, fused;
+    // the linking and the output order: each particle's later neighbours
+    // (appended, then listed in place), earliest earlier neighbour, pointer,
+    // number, the roots of the numbers, the sort values and slots (two
+    // buffers), the ranges still to sort, scan block sums, the merge pairs
+    // (counts, then the list)
+    DevBuf edges, deg, adj, kmin, par, first_label, lab, root, v0, v1, x0, x1, rng0, rng1, rsmall,
+        ssum, ecnt, evt;
     void release() {
         for (DevBuf *b : {&groups, &tiles, &qtiles, &ltiles, &drawn, &qidx, &psum, &mean, &pcorr, &sig, &pbox,
                           &gerr, &w, &w2, &rk, &bpos, &spos, &nodes, &act0, &act1, &counters,
-                          &slot_group, &dist, &sel, &sel2, &target, &up, &has, &csize, &emit,
-                          &fused})
+                          &slot_group, &dist, &sel, &sel2, &target, &csize, &emit, &fused,
+                          &edges, &deg, &adj, &kmin, &par, &first_label, &lab, &root, &v0, &v1,
+                          &x0, &x1, &rng0, &rng1, &rsmall, &ssum, &ecnt, &evt})
             b->release();
     }
 };
@@ -2169,17 +2176,20 @@ __attribute__((visibility("default"))) int hp_find_subfofs(struct particle *d_pa
     HP_ENSURE(B.sel, sizeof(float) * (S + 1));
     HP_ENSURE(B.sel2, sizeof(float) * (S + 1));
     HP_ENSURE(B.target, sizeof(float) * ng);
-    HP_ENSURE(B.up, sizeof(int32_t) * (M + 1));
-    HP_ENSURE(B.has, M + 1);
     HP_ENSURE(B.csize, sizeof(int32_t) * (M + 1));
     HP_ENSURE(B.emit, sizeof(int4) * cap_emit);
-    const int32_t ntile_sort = (int32_t)((M + kSortTile - 1) / kSortTile);
-    HP_ENSURE(e->keys0, sizeof(uint64_t) * (M + 1));
-    HP_ENSURE(e->keys1, sizeof(uint64_t) * (M + 1));
-    HP_ENSURE(e->vals0, sizeof(int32_t) * (M + 1));
-    HP_ENSURE(e->vals1, sizeof(int32_t) * (M + 1));
-    HP_ENSURE(e->hist, sizeof(int32_t) * kRadix * (ntile_sort + 1));
-    HP_ENSURE(e->hsum, sizeof(int32_t) * (kRadix * (ntile_sort + 1) / kScanBlock + 2));
+    // the linking's pair list starts at 8 per particle and grows to what the
+    // walk reports
+    int64_t cap_edges = std::max<int64_t>((int64_t)(B.edges.bytes / sizeof(EdgeRec)), 8 * M + 1024);
+    HP_ENSURE(B.edges, sizeof(EdgeRec) * cap_edges);
+    for (DevBuf *b : {&B.deg, &B.kmin, &B.par, &B.first_label, &B.lab, &B.root, &B.v0, &B.v1, &B.x0,
+                      &B.x1, &B.ecnt})
+        HP_ENSURE(*b, sizeof(int32_t) * (M + 2));
+    const int64_t cap_rng = M / (kSortSmall + 1) + ng + 1, cap_rsmall = M / 2 + ng + 1;
+    HP_ENSURE(B.rng0, sizeof(SortRange) * cap_rng);
+    HP_ENSURE(B.rng1, sizeof(SortRange) * cap_rng);
+    HP_ENSURE(B.rsmall, sizeof(SortRange) * cap_rsmall);
+    HP_ENSURE(B.ssum, sizeof(int32_t) * ((M + 2) / kScanBlock + 2));
     if ((r = e->upload(B.groups, G)) || (r = e->upload(B.tiles, tiles)) ||
         (r = e->upload(B.qtiles, qtiles)) || (r = e->upload(B.drawn, drawn)) ||
         (r = e->upload(B.qidx, qidx)) || (r = e->upload(B.ltiles, ltiles)))
@@ -2287,28 +2297,110 @@ __attribute__((visibility("default"))) int hp_find_subfofs(struct particle *d_pa
                                                       B.bpos.as<int32_t>(), B.spos.as<int32_t>(),
                                                       B.target.as<float>());
     HP_LAUNCHED();
-    // the linking, then the particles back grouped by component
+    // the linking, the reference's numbers and roots, then its output order
     if (M) {
         const int32_t m = (int32_t)M;
-        k_sub_init<<<Engine::blocks(M, 256), 256, 0, s>>>(m, B.up.as<int32_t>(), B.has.as<uint8_t>(),
-                                                  B.csize.as<int32_t>());
+        int32_t *deg = B.deg.as<int32_t>(), *kmin = B.kmin.as<int32_t>();
+        int32_t *flab = B.first_label.as<int32_t>();
+        auto scan = [&](int32_t *v, int32_t n) {  // exclusive, in place
+            const int32_t nbk = (n + kScanBlock - 1) / kScanBlock;
+            k_scan_blocks<<<nbk, 1024, 0, s>>>(v, n, B.ssum.as<int32_t>());
+            k_scan_top<<<1, 1024, 0, s>>>(B.ssum.as<int32_t>(), nbk);
+            k_scan_add<<<nbk, 1024, 0, s>>>(v, n, B.ssum.as<int32_t>());
+        };
+        k_sub_init<<<Engine::blocks(M + 1, 256), 256, 0, s>>>(m, kmin, deg, B.csize.as<int32_t>());
         HP_LAUNCHED();
-        k_sub_link<<<(uint32_t)ltiles.size(), kWave, 0, s>>>(
-            dg, B.ltiles.as<SubTile>(), B.target.as<float>(), B.w.as<SParticle>(),
-            B.nodes.as<Node6>(), B.up.as<int32_t>(), B.has.as<uint8_t>());
+        unsigned long long ne = 0;
+        for (int pass = 0; pass < 2; pass++) {
+            k_fill_i32<<<1, 64, 0, s>>>(cnt + kSfEdges, 0, 2);
+            HP_LAUNCHED();
+            k_sub_link<<<(uint32_t)ltiles.size(), kWave, 0, s>>>(
+                dg, B.ltiles.as<SubTile>(), B.target.as<float>(), B.w.as<SParticle>(),
+                B.nodes.as<Node6>(), B.edges.as<EdgeRec>(), cap_edges, deg, cnt);
+            HP_LAUNCHED();
+            HP_TRY(hipMemcpyAsync(&ne, cnt + kSfEdges, sizeof ne, hipMemcpyDeviceToHost, s));
+            HP_TRY(hipStreamSynchronize(s));
+            if ((int64_t)ne <= cap_edges) break;
+            if (pass || ne >= (unsigned long long)INT32_MAX)
+                return set_error(HP_EINVAL, "hp_find_subfofs: %llu linked pairs (at most %d)", ne,
+                                 INT32_MAX - 1);
+            cap_edges = (int64_t)ne;  // the walk again, with room for every pair
+            HP_ENSURE(B.edges, sizeof(EdgeRec) * cap_edges);
+        }
+        HP_ENSURE(B.adj, sizeof(int32_t) * (ne + 1));
+        scan(deg, m + 1);
         HP_LAUNCHED();
-        k_sub_keys<<<Engine::blocks(M, 256), 256, 0, s>>>(m, B.slot_group.as<int32_t>(), dg,
-                                                  B.up.as<int32_t>(), B.has.as<uint8_t>(),
-                                                  B.csize.as<int32_t>(), e->keys0.as<uint64_t>(),
-                                                  e->vals0.as<int32_t>());
+        if (ne) {
+            k_sub_adjacent<<<(uint32_t)((ne + 255) / 256), 256, 0, s>>>(
+                (int64_t)ne, B.edges.as<EdgeRec>(), deg, B.adj.as<int32_t>(), kmin);
+            HP_LAUNCHED();
+        }
+        k_sub_parent<<<Engine::blocks(M + 1, 256), 256, 0, s>>>(m, deg, B.adj.as<int32_t>(), kmin,
+                                                        B.par.as<int32_t>(), flab);
         HP_LAUNCHED();
-        int bits = 1;
-        while ((int64_t)1 << bits <= M) bits++;
-        if ((r = e->enqueue_sort(M, bits))) return r;
+        scan(flab, m + 1);
+        HP_LAUNCHED();
+        k_sub_label<<<Engine::blocks(M, 256), 256, 0, s>>>(m, B.par.as<int32_t>(), flab,
+                                                   B.lab.as<int32_t>());
+        HP_LAUNCHED();
+        // the pairs that can merge, staged in the pair list's place (free now)
+        HP_ENSURE(B.evt, sizeof(int2) * (ne + 1));
+        int32_t *ecnt = B.ecnt.as<int32_t>();
+        k_sub_events<<<Engine::blocks(M + 1, 256), 256, 0, s>>>(m, deg, B.adj.as<int32_t>(), kmin,
+                                                        B.lab.as<int32_t>(),
+                                                        B.edges.as<int2>(), ecnt);
+        HP_LAUNCHED();
+        scan(ecnt, m + 1);
+        HP_LAUNCHED();
+        k_sub_compact<<<Engine::blocks(M, 256), 256, 0, s>>>(m, deg, B.edges.as<int2>(), ecnt,
+                                                     B.evt.as<int2>());
+        HP_LAUNCHED();
+        k_sub_tokens<<<(uint32_t)ng, kTokThreads, 0, s>>>(dg, ecnt, B.evt.as<int2>(), flab,
+                                                          B.root.as<int32_t>());
+        HP_LAUNCHED();
+        int32_t *v0 = B.v0.as<int32_t>(), *v1 = B.v1.as<int32_t>();
+        int32_t *x0 = B.x0.as<int32_t>(), *x1 = B.x1.as<int32_t>();
+        k_sub_values<<<Engine::blocks(M, 256), 256, 0, s>>>(m, B.slot_group.as<int32_t>(), dg,
+                                                    B.lab.as<int32_t>(), flab,
+                                                    B.root.as<int32_t>(), v0, x0,
+                                                    B.csize.as<int32_t>());
+        HP_LAUNCHED();
+        // partition_sort_particles, level by level
+        SortRange *ra = B.rng0.as<SortRange>(), *rb = B.rng1.as<SortRange>();
+        SortRange *rs = B.rsmall.as<SortRange>();
+        k_fill_i32<<<1, 64, 0, s>>>(cnt + kSfRange, 0, 2);
+        HP_LAUNCHED();
+        k_sort_roots<<<Engine::blocks(ng, 64), 64, 0, s>>>((int32_t)ng, dg, ra, rs, cnt,
+                                                   (int32_t)cap_rng, (int32_t)cap_rsmall);
+        HP_LAUNCHED();
+        int32_t nsmall = 0;
+        for (int level = 0;; level++) {
+            int32_t na[kSfCounters];
+            HP_TRY(hipMemcpyAsync(na, cnt, sizeof na, hipMemcpyDeviceToHost, s));
+            HP_TRY(hipStreamSynchronize(s));
+            if (na[kSfErr])
+                return set_error(HP_EHIP, "hp_find_subfofs: scratch overflow (%d)", na[kSfErr]);
+            const int32_t nr = na[kSfRange];
+            nsmall = na[kSfRangeSmall];
+            if (!nr) break;
+            // each level halves the value span of every range it splits
+            if (level > 40) return set_error(HP_EHIP, "hp_find_subfofs: output sort too deep");
+            k_fill_i32<<<1, 1, 0, s>>>(cnt + kSfRange, 0, 1);
+            HP_LAUNCHED();
+            k_sort_split<kSortThreads1><<<(uint32_t)nr, kSortThreads1, 0, s>>>(
+                ra, v0, v1, x0, x1, B.rk.as<int32_t>(), B.bpos.as<int32_t>(),
+                B.spos.as<int32_t>(), rb, rs, cnt, (int32_t)cap_rng, (int32_t)cap_rsmall);
+            HP_LAUNCHED();
+            std::swap(ra, rb);
+        }
+        if (nsmall > 0) {
+            k_sort_small<<<(uint32_t)nsmall, kWave, 0, s>>>(rs, v0, v1, x0, x1);
+            HP_LAUNCHED();
+        }
         k_sub_emit<<<Engine::blocks(M, 256), 256, 0, s>>>(
-            m, B.slot_group.as<int32_t>(), dg, e->sorted_keys, e->sorted_vals,
-            B.csize.as<int32_t>(), B.w.as<SParticle>(), parts, cfg.min_halo_particles,
-            B.emit.as<int4>(), (int32_t)cap_emit, cnt);
+            m, B.slot_group.as<int32_t>(), dg, v0, x0, flab, B.csize.as<int32_t>(),
+            B.w.as<SParticle>(), parts, cfg.min_halo_particles, B.emit.as<int4>(),
+            (int32_t)cap_emit, cnt);
         HP_LAUNCHED();
     }
     int32_t hc[kSfCounters];

@@ -12,7 +12,10 @@
 //   4. find_median_r (src/groupies.c:744-786) with the reference's rand_r
 //      pivots: the linking length;
 //   5. the linking (_find_subfofs_at_r, src/groupies.c:274-285 with
-//      src/fof.c): every pair closer than that length joins one component.
+//      src/fof.c): every pair closer than that length joins one component;
+//   6. the output as the reference leaves it: the particles sorted by its
+//      union-find roots (collapse_smallfofs, partition_sort_particles) and
+//      the sub-FOFs in root order (build_fullfofs, src/fof.c:159-252).
 //
 // The tree is built level by level over every group of the batch, one
 // workgroup per node that still splits.  fast3tree's in-place partition
@@ -80,7 +83,10 @@ enum SfCounter {
     kSfErr = 3,
     kSfActiveBig = 4,  // nodes above kSfBigNode points: split with 1024 threads
     kSfFused = 5,      // nodes of at most kFuseMax points: finished by k_sub_fused
-    kSfCounters = 6
+    kSfEdges = 6,      // 64-bit (6 and 7): pairs the linking walk listed
+    kSfRange = 8,      // output-order ranges to split next level
+    kSfRangeSmall = 9, // output-order ranges finished by one wavefront
+    kSfCounters = 10
 };
 constexpr int kSfBigNode = 8192;
 
@@ -1100,61 +1106,63 @@ __global__ __launch_bounds__(kSfThreads) void k_sub_median(
 }
 
 // --------------------------------------------------------------- linking
+//
+// _find_subfofs_at_r (src/groupies.c:274-285) visits the particles in tree
+// order.  Particle i's sphere search (fast3tree_find_sphere_skip,
+// src/fast3tree.c:419-465) lists i itself, then every later particle closer
+// than the length, in array order, and link_particle_to_fof
+// (src/fof.c:113-137) gives the listed particles a small FOF or merges
+// theirs.  The output is sorted by the final union-find roots
+// (build_fullfofs, src/fof.c:223-252), so the roots' numbers fix it.  They
+// are found here without replaying the loop particle by particle:
+//   * particle x gets its small FOF at the first step that lists it: kmin(x),
+//     its earliest earlier neighbour, or its own step when it has none;
+//   * step i works with i's small FOF when i has an earlier neighbour, else
+//     with that of its first listed particle j numbered before (kmin(j) < i),
+//     else it creates the next number.  These pointers lead to earlier
+//     steps and end at the creating one: the numbers are a pointer chase and
+//     a count of the creating steps;
+//   * a merge (merge_smallfofs, src/fof.c:72-89) hangs the root of step i's
+//     small FOF under the root of the listed particle's.  Only pairs (i, x)
+//     with x numbered before step i under another number can merge.  One
+//     workgroup per group replays them in step order, a chunk of pairs at a
+//     time, keeping only the pairs whose roots still differ when the chunk
+//     starts (a joined pair stays joined).
 
-__global__ void k_sub_init(int32_t m, int32_t *__restrict__ up, uint8_t *__restrict__ has,
+constexpr int32_t kNoEarlier = 0x7fffffff;
+
+struct EdgeRec {
+    int32_t i, j, rank;  // step i lists j as its rank-th later particle
+};
+constexpr int kLinkStage = 256;  // pairs a wavefront stages before one append
+
+__global__ void k_sub_init(int32_t m, int32_t *__restrict__ kmin, int32_t *__restrict__ deg,
                            int32_t *__restrict__ csize) {
     const int32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= m) return;
-    up[i] = i;
-    has[i] = 0;
+    if (i > m) return;  // m + 1 entries: the scans' totals
+    kmin[i] = kNoEarlier;
+    deg[i] = 0;
     csize[i] = 0;
 }
 
-// path halving: each visited slot is pointed at its grandparent.  A link
-// only ever moves to a smaller slot of the same tree (roots hook under
-// smaller roots), so a racing update never loses a union.
-__device__ __forceinline__ int32_t uf_find(int32_t *up, int32_t x) {
-    for (;;) {
-        const int32_t y = __hip_atomic_load(up + x, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        if (y == x) return x;
-        const int32_t z = __hip_atomic_load(up + y, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        if (z != y) __hip_atomic_store(up + x, z, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        x = z;
-    }
-}
-
-// hook the larger root under the smaller: a component's root is its
-// smallest slot
-__device__ __forceinline__ void uf_unite(int32_t *up, int32_t a, int32_t b) {
-    for (;;) {
-        a = uf_find(up, a);
-        b = uf_find(up, b);
-        if (a == b) return;
-        if (a > b) {
-            const int32_t t = a;
-            a = b;
-            b = t;
-        }
-        if (atomicCAS(up + b, b, a) == b) return;
-    }
-}
-
-// _find_subfofs_at_r (src/groupies.c:274-285): each particle's sphere
-// search over the later slots (fast3tree_find_sphere_skip,
-// src/fast3tree.c:419-465) and the union of every pair closer than the
-// group's length.  64 consecutive slots per wavefront walk the tree
+// Each particle's sphere search over the later slots: the later particles
+// closer than the group's length, appended to `edges` in the order the
+// search lists them.  64 consecutive slots per wavefront walk the tree
 // together: a node is entered when some lane's sphere reaches it and it
 // holds slots after that lane's, and each leaf's points are loaded once and
-// passed lane to lane.  Every pair the reference links is tested with the
-// same float distance, so the components are the reference's.
+// passed lane to lane.  Every pair is tested with the reference's float
+// distance.  The append counter keeps counting past cap_edges (nothing is
+// written there), so the host can size the list and run again.
 __global__ __launch_bounds__(kWave) void k_sub_link(const SubGroup *__restrict__ groups,
                                                     const SubTile *__restrict__ ltiles,
                                                     const float *__restrict__ target,
                                                     const SParticle *__restrict__ w,
                                                     const Node6 *__restrict__ nodes,
-                                                    int32_t *__restrict__ up,
-                                                    uint8_t *__restrict__ has) {
+                                                    EdgeRec *__restrict__ edges, int64_t cap_edges,
+                                                    int32_t *__restrict__ deg,
+                                                    int32_t *__restrict__ counters) {
     __shared__ float4 s_pts[kWave][2];
+    __shared__ EdgeRec s_e[kLinkStage];
     const SubTile tl = ltiles[blockIdx.x];
     const SubGroup g = groups[tl.group];
     const int32_t i = g.off + tl.off + lane_id();
@@ -1166,7 +1174,20 @@ __global__ __launch_bounds__(kWave) void k_sub_link(const SubGroup *__restrict__
         const SParticle q = w[i];
         for (int k = 0; k < 6; k++) c[k] = q.pos[k];
     }
-    bool any = false;
+    unsigned long long *ecount = reinterpret_cast<unsigned long long *>(counters + kSfEdges);
+    int32_t listed = 0;  // this lane's later particles so far
+    int32_t staged = 0;  // pairs in s_e (wavefront-uniform)
+    // the staged pairs to the list with one append
+    auto flush = [&]() {
+        unsigned long long at = 0;
+        if (lane_id() == 0) at = atomicAdd(ecount, (unsigned long long)staged);
+        const int64_t pos = (int64_t)read_lane(at, 0);
+        __syncthreads();
+        for (int32_t k = lane_id(); k < staged; k += kWave)
+            if (pos + k < cap_edges) edges[pos + k] = s_e[k];
+        __syncthreads();
+        staged = 0;
+    };
     int32_t cur = tl.group;
     while (cur >= 0) {
         cur = uniform(cur);
@@ -1191,6 +1212,7 @@ __global__ __launch_bounds__(kWave) void k_sub_link(const SubGroup *__restrict__
                 s_pts[lane_id()][1] = make_float4(q.pos[4], q.pos[5], 0.0f, 0.0f);
             }
             __syncthreads();
+            uint64_t hit = 0;
             for (int32_t t = 0; t < cnt; t++) {
                 const float4 a = s_pts[t][0], b = s_pts[t][1];
                 const float pp[6] = {a.x, a.y, a.z, a.w, b.x, b.y};
@@ -1200,68 +1222,459 @@ __global__ __launch_bounds__(kWave) void k_sub_link(const SubGroup *__restrict__
                     d += dx * dx;
                 }
                 const int32_t j = x.start + base + t;
-                if (need && j > i && d < r2) {
-                    any = true;
-                    has[j] = 1;
-                    uf_unite(up, i, j);
-                }
+                if (need && j > i && d < r2) hit |= 1ull << t;
             }
             __syncthreads();
+            // the chunk's pairs: lanes in order, each lane's in order, staged
+            // in LDS (appended directly when they alone overfill it)
+            const int32_t nh = __popcll(hit);
+            const int32_t incl = wave_scan_add(nh);
+            const int32_t tot = __builtin_amdgcn_readlane(incl, kWave - 1);
+            if (tot) {
+                if (staged + tot > kLinkStage) flush();
+                int32_t rank = listed;
+                if (tot > kLinkStage) {
+                    unsigned long long at = 0;
+                    if (lane_id() == 0) at = atomicAdd(ecount, (unsigned long long)tot);
+                    int64_t pos = (int64_t)read_lane(at, 0) + (incl - nh);
+                    while (hit) {
+                        const int t = __builtin_ctzll(hit);
+                        hit &= hit - 1;
+                        if (pos < cap_edges) edges[pos] = EdgeRec{i, x.start + base + t, rank};
+                        pos++;
+                        rank++;
+                    }
+                } else {
+                    int32_t at = staged + incl - nh;
+                    while (hit) {
+                        const int t = __builtin_ctzll(hit);
+                        hit &= hit - 1;
+                        s_e[at++] = EdgeRec{i, x.start + base + t, rank++};
+                    }
+                    staged += tot;
+                }
+                listed += nh;
+            }
         }
         cur = x.skip;
     }
-    if (any) has[i] = 1;
+    if (staged) flush();
+    if (valid) deg[i] = listed;
 }
 
-// sort keys: a group's particles with no neighbour first, then each
-// component in the order of its smallest slot; slots keep their order
-// inside each key (the sort is stable)
-__global__ void k_sub_keys(int32_t m, const int32_t *__restrict__ slot_group,
-                           const SubGroup *__restrict__ groups, int32_t *__restrict__ up,
-                           const uint8_t *__restrict__ has, int32_t *__restrict__ csize,
-                           uint64_t *__restrict__ keys, int32_t *__restrict__ vals) {
+// the lists in place (off: exclusive scan of the counts) and each
+// particle's earliest earlier neighbour
+__global__ void k_sub_adjacent(int64_t ne, const EdgeRec *__restrict__ edges,
+                               const int32_t *__restrict__ off, int32_t *__restrict__ adj,
+                               int32_t *__restrict__ kmin) {
+    const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= ne) return;
+    const EdgeRec r = edges[e];
+    adj[off[r.i] + r.rank] = r.j;
+    atomicMin(kmin + r.j, r.i);
+}
+
+// the step whose small FOF each particle takes (-1: never listed; a creating
+// step points at itself), and created[i] = 1 for a creating step
+__global__ void k_sub_parent(int32_t m, const int32_t *__restrict__ off,
+                             const int32_t *__restrict__ adj, const int32_t *__restrict__ kmin,
+                             int32_t *__restrict__ par, int32_t *__restrict__ created) {
     const int32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-    const bool in = i < m;
-    const bool linked = in && has[i];
-    int32_t r = -1;
-    if (linked) {
-        r = uf_find(up, i);
-        // every root is final here: shortens the finds still running
-        __hip_atomic_store(up + i, r, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    if (i > m) return;
+    if (i == m) {
+        created[m] = 0;
+        return;
     }
-    // component sizes: one atomic per distinct root in the wavefront (the
-    // members of a large component would otherwise queue on one counter)
+    const int32_t k = kmin[i];
+    int32_t p = -1;
+    if (k != kNoEarlier) {
+        p = k;
+    } else if (off[i + 1] > off[i]) {
+        p = i;
+        for (int32_t e = off[i]; e < off[i + 1]; e++) {
+            const int32_t kj = kmin[adj[e]];
+            if (kj < i) {
+                p = kj;
+                break;
+            }
+        }
+    }
+    par[i] = p;
+    created[i] = p == i;
+}
+
+// each particle's small FOF number: that of the creating step its pointers
+// lead to (first_label: exclusive count of creating steps, over the batch).
+// Path halving: every pointer only ever moves to an earlier step of the same
+// chain, so racing updates keep each chain intact.
+__global__ void k_sub_label(int32_t m, int32_t *__restrict__ par,
+                            const int32_t *__restrict__ first_label, int32_t *__restrict__ lab) {
+    const int32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= m) return;
+    int32_t x = __hip_atomic_load(par + i, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    if (x < 0) {
+        lab[i] = -1;
+        return;
+    }
+    x = i;
+    for (;;) {
+        const int32_t y = __hip_atomic_load(par + x, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        if (y == x) break;
+        const int32_t z = __hip_atomic_load(par + y, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        if (z != y) __hip_atomic_store(par + x, z, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        x = z;
+    }
+    lab[i] = first_label[x];
+}
+
+// The pairs of step i that can merge, (i's number, x's number) for each
+// listed x numbered before step i under another number, in list order.  A
+// number met earlier in the same step is dropped: the first merge with it
+// joined the two.  Written at the step's own list positions (stage), counted
+// in ecnt.
+__global__ void k_sub_events(int32_t m, const int32_t *__restrict__ off,
+                             const int32_t *__restrict__ adj, const int32_t *__restrict__ kmin,
+                             const int32_t *__restrict__ lab, int2 *__restrict__ stage,
+                             int32_t *__restrict__ ecnt) {
+    const int32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i > m) return;
+    if (i == m) {
+        ecnt[m] = 0;
+        return;
+    }
+    const int32_t e0 = off[i], e1 = off[i + 1];
+    int32_t k = 0;
+    if (e1 > e0) {
+        const int32_t f = lab[i];
+        for (int32_t e = e0; e < e1; e++) {
+            const int32_t x = adj[e];
+            if (kmin[x] >= i) continue;
+            const int32_t lx = lab[x];
+            if (lx == f) continue;
+            bool seen = false;
+            for (int32_t q = 0; q < k && !seen; q++) seen = stage[e0 + q].y == lx;
+            if (!seen) stage[e0 + k++] = make_int2(f, lx);
+        }
+    }
+    ecnt[i] = k;
+}
+
+// the pairs of every step in one list, in step order (eoff: exclusive scan
+// of ecnt)
+__global__ void k_sub_compact(int32_t m, const int32_t *__restrict__ off,
+                              const int2 *__restrict__ stage, const int32_t *__restrict__ eoff,
+                              int2 *__restrict__ ev) {
+    const int32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= m) return;
+    const int32_t k = eoff[i + 1] - eoff[i];
+    for (int32_t q = 0; q < k; q++) ev[eoff[i] + q] = stage[off[i] + q];
+}
+
+constexpr int kTokThreads = 1024;
+constexpr int kTokLds = 16384;  // a group's numbers held in LDS up to this many
+
+// root of x with path halving; u is the group's table (LDS or global, one
+// workgroup only)
+__device__ __forceinline__ int32_t tok_find(int32_t *u, int32_t x) {
+    for (;;) {
+        const int32_t y = __hip_atomic_load(u + x, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        if (y == x) return x;
+        const int32_t z = __hip_atomic_load(u + y, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        if (z != y) __hip_atomic_store(u + x, z, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        x = z;
+    }
+}
+
+// The merges of one group in step order; root[L0 + k] = the root of number
+// k (group-local numbers).  A chunk of kTokThreads pairs at a time: every
+// thread finds its pair's two roots as the chunk starts, and one thread
+// replays, in order, the pairs whose roots differ then.  A group with more
+// than kTokLds numbers keeps its table in root itself.
+__global__ __launch_bounds__(kTokThreads) void k_sub_tokens(
+    const SubGroup *__restrict__ groups, const int32_t *__restrict__ eoff,
+    const int2 *__restrict__ ev, const int32_t *__restrict__ first_label,
+    int32_t *__restrict__ root) {
+    __shared__ int32_t s_u[kTokLds];
+    __shared__ int2 s_pair[kTokThreads];
+    __shared__ int s_w[kTokThreads / kWave];
+    const SubGroup g = groups[blockIdx.x];
+    if (g.n < 2) return;
+    const int32_t L0 = first_label[g.off];
+    const int32_t nl = first_label[g.off + g.n] - L0;
+    if (nl <= 0) return;
+    int32_t *u = nl <= kTokLds ? s_u : root + L0;
+    for (int32_t k = threadIdx.x; k < nl; k += kTokThreads) u[k] = k;
+    __syncthreads();
+    const int32_t q0 = eoff[g.off], q1 = eoff[g.off + g.n];
+    for (int32_t base = q0; base < q1; base += kTokThreads) {
+        const int32_t q = base + threadIdx.x;
+        int32_t ra = 0, rb = 0;
+        if (q < q1) {
+            const int2 p = ev[q];
+            ra = tok_find(u, p.x - L0);
+            rb = tok_find(u, p.y - L0);
+        }
+        int tot;
+        const int at = block_rank<kTokThreads>(ra != rb, &tot, s_w);
+        if (ra != rb) s_pair[at] = make_int2(ra, rb);
+        __syncthreads();
+        if (threadIdx.x == 0)
+            for (int k = 0; k < tot; k++) {
+                const int32_t a = tok_find(u, s_pair[k].x), b = tok_find(u, s_pair[k].y);
+                if (a != b) __hip_atomic_store(u + a, b, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            }
+        __syncthreads();
+    }
+    for (int32_t k = threadIdx.x; k < nl; k += kTokThreads) {
+        const int32_t rt = tok_find(u, k);
+        __hip_atomic_store(root + L0 + k, rt, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    }
+}
+
+// each particle's sort value (its root's number in the group, -1 without a
+// neighbour; collapse_smallfofs, src/fof.c:159-170), its slot, and the size
+// of each root's set
+__global__ void k_sub_values(int32_t m, const int32_t *__restrict__ slot_group,
+                             const SubGroup *__restrict__ groups, const int32_t *__restrict__ lab,
+                             const int32_t *__restrict__ first_label,
+                             const int32_t *__restrict__ root, int32_t *__restrict__ val,
+                             int32_t *__restrict__ idx, int32_t *__restrict__ csize) {
+    const int32_t q = blockIdx.x * blockDim.x + threadIdx.x;
+    const bool in = q < m;
+    int32_t v = -1, key = 0;
+    if (in) {
+        const int32_t l = lab[q];
+        if (l >= 0) {
+            v = root[l];
+            key = first_label[groups[slot_group[q]].off] + v;
+        }
+        val[q] = v;
+        idx[q] = q;
+    }
+    // one atomic per distinct root in the wavefront
+    const bool linked = v >= 0;
     uint64_t todo = ballot(linked);
     while (todo) {
         const int lead = __builtin_ctzll(todo);
-        const int32_t rl = __builtin_amdgcn_readlane(r, lead);
-        const uint64_t same = ballot(linked && r == rl);
-        if (lane_id() == lead) atomicAdd(csize + rl, (int)__popcll(same));
+        const int32_t kl = __builtin_amdgcn_readlane(key, lead);
+        const uint64_t same = ballot(linked && key == kl);
+        if (lane_id() == lead) atomicAdd(csize + kl, (int)__popcll(same));
         todo &= ~same;
     }
-    if (!in) return;
-    const int32_t key = linked ? r + 1 : groups[slot_group[i]].off;
-    keys[i] = (uint64_t)(uint32_t)key;
-    vals[i] = i;
 }
 
-// the particles to the caller's array in key order; one sub-FOF per
-// component of at least min_p particles
+// --------------------------------------------------------- the output order
+//
+// partition_sort_particles (src/fof.c:184-221) on the sort values: each range
+// splits about the middle of its values with the loop of the tree build
+// (the closed form in the file header, `big` = value above the middle),
+// level by level over every group.  A range of at most kSortSmall runs the
+// loop itself in one wavefront.  Ranges move between two buffers of values
+// and slots; every finished range ends in the first.
+
+constexpr int kSortSmall = 128;
+constexpr int kSortThreads1 = 1024;
+
+struct SortRange {
+    int32_t start, n, where, pad;
+};
+
+__device__ __forceinline__ void push_range(SortRange r, SortRange *__restrict__ next,
+                                           SortRange *__restrict__ small,
+                                           int32_t *__restrict__ counters, int32_t cap_next,
+                                           int32_t cap_small) {
+    if (r.n > kSortSmall) {
+        const int32_t k = atomicAdd(&counters[kSfRange], 1);
+        if (k < cap_next) next[k] = r;
+        else atomicOr(&counters[kSfErr], 4);
+    } else {
+        const int32_t k = atomicAdd(&counters[kSfRangeSmall], 1);
+        if (k < cap_small) small[k] = r;
+        else atomicOr(&counters[kSfErr], 4);
+    }
+}
+
+__global__ void k_sort_roots(int32_t ng, const SubGroup *__restrict__ groups,
+                             SortRange *__restrict__ next, SortRange *__restrict__ small,
+                             int32_t *__restrict__ counters, int32_t cap_next, int32_t cap_small) {
+    const int32_t gi = blockIdx.x * blockDim.x + threadIdx.x;
+    if (gi >= ng) return;
+    const SubGroup g = groups[gi];
+    if (g.n >= 2) push_range(SortRange{g.off, g.n, 0, 0}, next, small, counters, cap_next, cap_small);
+}
+
+template <int NT>
+__device__ __forceinline__ void block_minmax(int32_t &lo, int32_t &hi, int32_t *s_w) {
+    constexpr int NW = NT / kWave;
+    lo = wave_min(lo);
+    hi = wave_max(hi);
+    const int wv = threadIdx.x / kWave;
+    if (lane_id() == 0) {
+        s_w[wv] = lo;
+        s_w[NW + wv] = hi;
+    }
+    __syncthreads();
+    for (int k = 0; k < NW; k++) {
+        lo = min(lo, s_w[k]);
+        hi = max(hi, s_w[NW + k]);
+    }
+    __syncthreads();
+}
+
+// one level: every listed range splits, one workgroup per range
+template <int NT>
+__global__ __launch_bounds__(NT) void k_sort_split(
+    const SortRange *__restrict__ ranges, int32_t *__restrict__ v0, int32_t *__restrict__ v1,
+    int32_t *__restrict__ x0, int32_t *__restrict__ x1, int32_t *__restrict__ rk,
+    int32_t *__restrict__ bpos, int32_t *__restrict__ spos, SortRange *__restrict__ next,
+    SortRange *__restrict__ small, int32_t *__restrict__ counters, int32_t cap_next,
+    int32_t cap_small) {
+    __shared__ int s_w[2 * (NT / kWave)];
+    __shared__ int32_t s_k[1];
+    const SortRange sr = ranges[blockIdx.x];
+    const int32_t s0 = sr.start, n = sr.n;
+    const int32_t *__restrict__ vs = sr.where ? v1 : v0;
+    const int32_t *__restrict__ xs = sr.where ? x1 : x0;
+    int32_t *__restrict__ vd = sr.where ? v0 : v1;
+    int32_t *__restrict__ xd = sr.where ? x0 : x1;
+    int32_t lo = 0x7fffffff, hi = -0x7fffffff - 1;
+    for (int32_t p = threadIdx.x; p < n; p += NT) {
+        const int32_t v = vs[s0 + p];
+        lo = min(lo, v);
+        hi = max(hi, v);
+    }
+    block_minmax<NT>(lo, hi, s_w);
+    if (lo == hi) {  // one value: in order already, to the first buffer
+        if (sr.where)
+            for (int32_t p = threadIdx.x; p < n; p += NT) {
+                v0[s0 + p] = v1[s0 + p];
+                x0[s0 + p] = x1[s0 + p];
+            }
+        return;
+    }
+    const int32_t pivot = (int32_t)((int64_t)lo + ((int64_t)hi - (int64_t)lo) / 2);
+    // classes, ranks within the class, positions of each class
+    int32_t nbig = 0;
+    for (int32_t base = 0; base < n; base += NT) {
+        const int32_t p = base + threadIdx.x;
+        const bool in = p < n;
+        const bool big = in && vs[s0 + p] > pivot;
+        int tot;
+        const int rb = block_rank<NT>(big, &tot, s_w);
+        if (in) {
+            const int32_t rank = big ? nbig + rb : p - (nbig + rb);
+            rk[s0 + p] = big ? ~rank : rank;
+            if (big) bpos[s0 + rank] = p;
+            else spos[s0 + rank] = p;
+        }
+        nbig += tot;
+    }
+    const int32_t nsmall = n - nbig;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        int32_t K, S;
+        partition_ks(bpos + s0, spos + s0, n, nbig, nsmall, &K, &S);
+        s_k[0] = S;
+    }
+    __syncthreads();
+    const int32_t S = s_k[0];
+    for (int32_t p = threadIdx.x; p < n; p += NT) {
+        const int32_t r = rk[s0 + p];
+        const bool big = r < 0;
+        const int32_t d = partition_dest(p, big, big ? ~r : r, n, nsmall, S, bpos + s0, spos + s0);
+        vd[s0 + d] = vs[s0 + p];
+        xd[s0 + d] = xs[s0 + p];
+    }
+    __syncthreads();
+    if (threadIdx.x != 0) return;
+    const int32_t wh = 1 - sr.where;
+    const SortRange L{s0, nsmall, wh, 0}, R{s0 + nsmall, n - nsmall, wh, 0};
+    for (const SortRange &c : {L, R}) {
+        if (c.n >= 2) {
+            push_range(c, next, small, counters, cap_next, cap_small);
+        } else if (c.n == 1 && wh) {
+            v0[c.start] = v1[c.start];
+            x0[c.start] = x1[c.start];
+        }
+    }
+}
+
+// a range of at most kSortSmall: the reference's loop itself, by one lane,
+// on a copy in LDS; the result to the first buffer
+__global__ __launch_bounds__(kWave) void k_sort_small(const SortRange *__restrict__ ranges,
+                                                      int32_t *__restrict__ v0,
+                                                      int32_t *__restrict__ v1,
+                                                      int32_t *__restrict__ x0,
+                                                      int32_t *__restrict__ x1) {
+    __shared__ int32_t s_v[kSortSmall], s_x[kSortSmall];
+    __shared__ int2 s_stack[kSortSmall];
+    const SortRange sr = ranges[blockIdx.x];
+    const int32_t s0 = sr.start, n = min(sr.n, kSortSmall);
+    const int32_t *__restrict__ vs = sr.where ? v1 : v0;
+    const int32_t *__restrict__ xs = sr.where ? x1 : x0;
+    for (int32_t p = lane_id(); p < n; p += kWave) {
+        s_v[p] = vs[s0 + p];
+        s_x[p] = xs[s0 + p];
+    }
+    __syncthreads();
+    if (lane_id() == 0) {
+        // ranges still to sort; each pop pushes at most two ranges of at
+        // least two values each, disjoint, so the stack never holds more
+        // than n / 2 of them
+        int32_t sp = 0;
+        s_stack[sp++] = make_int2(0, n);
+        while (sp) {
+            const int2 rg = s_stack[--sp];
+            const int32_t a = rg.x, b = rg.y;
+            int32_t lo = s_v[a], hi = s_v[a];
+            for (int32_t p = a + 1; p < b; p++) {
+                lo = min(lo, s_v[p]);
+                hi = max(hi, s_v[p]);
+            }
+            if (lo == hi) continue;
+            const int32_t pivot = (int32_t)((int64_t)lo + ((int64_t)hi - (int64_t)lo) / 2);
+            int32_t p = a, last = b - 1;
+            while (p < last) {
+                if (s_v[p] > pivot) {  // swap with the last unexamined slot, look again
+                    const int32_t tv = s_v[p], tx = s_x[p];
+                    s_v[p] = s_v[last];
+                    s_x[p] = s_x[last];
+                    s_v[last] = tv;
+                    s_x[last] = tx;
+                    last--;
+                } else {
+                    p++;
+                }
+            }
+            if (s_v[last] <= pivot) last++;
+            if (last - a >= 2) s_stack[sp++] = make_int2(a, last);
+            if (b - last >= 2) s_stack[sp++] = make_int2(last, b);
+        }
+    }
+    __syncthreads();
+    for (int32_t p = lane_id(); p < n; p += kWave) {
+        v0[s0 + p] = s_v[p];
+        x0[s0 + p] = s_x[p];
+    }
+}
+
+// the particles to the caller's array in the sorted order; one sub-FOF per
+// run of a root of at least min_p particles (build_fullfofs)
 __global__ void k_sub_emit(int32_t m, const int32_t *__restrict__ slot_group,
-                           const SubGroup *__restrict__ groups, const uint64_t *__restrict__ skeys,
-                           const int32_t *__restrict__ svals, const int32_t *__restrict__ csize,
-                           const SParticle *__restrict__ w, SParticle *__restrict__ parts,
-                           int64_t min_p, int4 *__restrict__ subs, int32_t cap,
-                           int32_t *__restrict__ counters) {
+                           const SubGroup *__restrict__ groups, const int32_t *__restrict__ val,
+                           const int32_t *__restrict__ idx, const int32_t *__restrict__ first_label,
+                           const int32_t *__restrict__ csize, const SParticle *__restrict__ w,
+                           SParticle *__restrict__ parts, int64_t min_p, int4 *__restrict__ subs,
+                           int32_t cap, int32_t *__restrict__ counters) {
     const int32_t q = blockIdx.x * blockDim.x + threadIdx.x;
     if (q >= m) return;
     const int32_t gi = slot_group[q];
     const SubGroup g = groups[gi];
-    parts[g.first + (q - g.off)] = w[svals[q]];
-    const int32_t key = (int32_t)skeys[q];
-    if (key == g.off) return;  // no neighbour
-    if (q > g.off && (int32_t)skeys[q - 1] == key) return;
-    const int32_t size = csize[key - 1];
+    parts[g.first + (q - g.off)] = w[idx[q]];
+    const int32_t v = val[q];
+    if (v < 0) return;  // no neighbour
+    if (q > g.off && val[q - 1] == v) return;
+    const int32_t size = csize[first_label[g.off] + v];
     if (size < min_p) return;
     const int32_t e = atomicAdd(&counters[kSfEmit], 1);
     if (e < cap) subs[e] = make_int4(gi, q - g.off, size, 0);

@@ -5,14 +5,14 @@ compiled code, and against that compiled code directly.
 
 Every comparison is exact:
   * the linking length of each group (find_median_r's result);
-  * each group's particles as the call leaves them: normalised by norm_sd
-    and grouped -- the particles with no neighbour first, then each
-    component in the order of its first particle in the reference's tree
-    order, particles inside a component in that tree order.  Byte for byte
-    against the restatement's tree order sorted the same way, which checks
-    the normalisation, the in-place tree order and the components at once;
-  * the sub-FOFs (components of at least MIN_HALO_PARTICLES) as sets of
-    particle ids, against the reference's _find_subfofs_better2.
+  * each group's particles as the call leaves them, byte for byte: normalised
+    by norm_sd, in the reference's tree order, linked, and sorted by the
+    reference's union-find roots with its partition sort (the particles with
+    no neighbour first).  This checks the normalisation, the in-place tree
+    order, the components, the roots' numbers and the partition at once;
+  * the sub-FOFs (components of at least MIN_HALO_PARTICLES) as (first,
+    num_p) in the reference's root order, against the restatement and
+    against the reference's own _find_subfofs_better2.
 The host-side argument checks run without a GPU.
 """
 import ctypes as C
@@ -50,38 +50,25 @@ def _run_gpu(parts, groups, **kw):
 
 
 def _expected(restatement, p, top=True, scaling=1.0, exact_ll=0, min_p=10):
-    o = restatement.subfof(p, top=top, scaling=scaling, exact_ll=exact_ll)
-    lab = o["label"]
-    order = np.argsort(lab, kind="stable")
-    arranged = o["tree_order"][order]
-    subs = []
-    slab = lab[order]
-    q = 0
-    while q < slab.shape[0]:
-        e = q
-        while e < slab.shape[0] and slab[e] == slab[q]:
-            e += 1
-        if slab[q] >= 0 and e - q >= min_p:
-            subs.append((q, e - q))
-        q = e
-    return o["target_r"], arranged, subs
+    o = restatement.subfof(p, top=top, scaling=scaling, exact_ll=exact_ll, min_p=min_p)
+    return o["target_r"], o["final"], [(int(a), int(b)) for a, b in o["fofs"]]
 
 
-def _check_batch(restatement, parts_list, top=True, scaling=1.0, exact_ll=0):
+def _check_batch(restatement, parts_list, top=True, scaling=1.0, exact_ll=0, min_p=10):
     parts, groups = _batch(parts_list)
     g, subs, out = _run_gpu(parts, groups, exact_ll_calc=exact_ll,
-                            initial_metric_scaling=scaling)
+                            initial_metric_scaling=scaling, min_halo_particles=min_p)
     g["top"] = int(top)
     if not top:
         groups_nt = groups.copy()
         g2 = np.zeros(len(groups_nt), halopot.FOF_GROUP_DTYPE)
         g2["first"], g2["num_p"], g2["top"] = groups_nt["first"], groups_nt["num_p"], 0
         g, subs, out = _run_gpu(parts, g2, exact_ll_calc=exact_ll,
-                                initial_metric_scaling=scaling)
+                                initial_metric_scaling=scaling, min_halo_particles=min_p)
     nsub = 0
     for i, p in enumerate(parts_list):
         f, n = int(groups["first"][i]), int(groups["num_p"][i])
-        tr, arranged, want = _expected(restatement, p, top, scaling, exact_ll)
+        tr, arranged, want = _expected(restatement, p, top, scaling, exact_ll, min_p)
         assert g["target_r"][i] == tr, (i, n)
         assert out[f:f + n].tobytes() == arranged.tobytes(), (i, n)
         got = [(int(s["first"]) - f, int(s["num_p"]))
@@ -100,6 +87,16 @@ def test_gpu_subfof_matches_restatement_on_fof_groups(restatement):
     plist += [big.particles[f:f + n].copy() for f, n in big.groups]
     g, subs, _ = _check_batch(restatement, plist)
     assert subs.shape[0] > len(plist)
+
+
+@pytest.mark.gpu
+def test_gpu_subfof_small_min_halo_particles(restatement):
+    """MIN_HALO_PARTICLES 2: every linked component is a sub-FOF, so the
+    list holds many small roots in the reference's order."""
+    fg = catalog.fof_groups(seed=17, n_groups=12, lo=200, hi=8000)
+    plist = [fg.particles[f:f + n].copy() for f, n in fg.groups]
+    _, subs, _ = _check_batch(restatement, plist, min_p=2)
+    assert subs.shape[0] > 10 * len(plist)
 
 
 @pytest.mark.gpu
@@ -175,8 +172,9 @@ def test_gpu_subfof_exact_ll_and_lower_levels(restatement):
 
 @pytest.mark.gpu
 def test_gpu_subfof_matches_reference():
-    """Against the reference's own _find_subfofs_better2 (oracle/_ref):
-    the linking length and every sub-FOF as a set of ids."""
+    """Against the reference's own _find_subfofs_better2 (oracle/_ref): the
+    linking length, the particle array it leaves byte for byte and its
+    sub-FOF list in its order."""
     if not oracle.ReferenceProps.available():
         pytest.skip("compiled reference (oracle/_ref) not present")
     ref = oracle.ReferenceProps()
@@ -189,14 +187,10 @@ def test_gpu_subfof_matches_reference():
         r = ref.subfof(p)
         assert g["target_r"][i] == r["target_r"], i
         f = int(groups["first"][i])
-        mine = sorted(tuple(sorted(out["id"][s["first"]:s["first"] + s["num_p"]]))
-                      for s in subs[g["first_sub"][i]:g["first_sub"][i] + g["num_sub"][i]])
-        theirs = sorted(tuple(sorted(r["final"]["id"][a:a + b])) for a, b in r["fofs"])
-        assert mine == theirs, i
-        # the particles themselves: the same normalised values by id
-        a = np.sort(out[f:f + p.shape[0]], order="id")
-        b = np.sort(r["final"], order="id")
-        assert a.tobytes() == b.tobytes(), i
+        mine = [(int(s["first"]) - f, int(s["num_p"]))
+                for s in subs[g["first_sub"][i]:g["first_sub"][i] + g["num_sub"][i]]]
+        assert mine == [(int(a), int(b)) for a, b in r["fofs"]], i
+        assert out[f:f + p.shape[0]].tobytes() == r["final"].tobytes(), i
 
 
 @pytest.mark.gpu
