@@ -2185,9 +2185,11 @@ __attribute__((visibility("default"))) int hp_find_subfofs(struct particle *d_pa
     for (DevBuf *b : {&B.deg, &B.kmin, &B.par, &B.first_label, &B.lab, &B.root, &B.v0, &B.v1, &B.x0,
                       &B.x1, &B.ecnt})
         HP_ENSURE(*b, sizeof(int32_t) * (M + 2));
+    // one level's ranges: the large ones from the front, the mid-sized ones
+    // from the back
     const int64_t cap_rng = M / (kSortSmall + 1) + ng + 1, cap_rsmall = M / 2 + ng + 1;
-    HP_ENSURE(B.rng0, sizeof(SortRange) * cap_rng);
-    HP_ENSURE(B.rng1, sizeof(SortRange) * cap_rng);
+    HP_ENSURE(B.rng0, 2 * sizeof(SortRange) * cap_rng);
+    HP_ENSURE(B.rng1, 2 * sizeof(SortRange) * cap_rng);
     HP_ENSURE(B.rsmall, sizeof(SortRange) * cap_rsmall);
     HP_ENSURE(B.ssum, sizeof(int32_t) * ((M + 2) / kScanBlock + 2));
     if ((r = e->upload(B.groups, G)) || (r = e->upload(B.tiles, tiles)) ||
@@ -2368,7 +2370,7 @@ __attribute__((visibility("default"))) int hp_find_subfofs(struct particle *d_pa
         // partition_sort_particles, level by level
         SortRange *ra = B.rng0.as<SortRange>(), *rb = B.rng1.as<SortRange>();
         SortRange *rs = B.rsmall.as<SortRange>();
-        k_fill_i32<<<1, 64, 0, s>>>(cnt + kSfRange, 0, 2);
+        k_fill_i32<<<1, 64, 0, s>>>(cnt + kSfRange, 0, 3);
         HP_LAUNCHED();
         k_sort_roots<<<Engine::blocks(ng, 64), 64, 0, s>>>((int32_t)ng, dg, ra, rs, cnt,
                                                    (int32_t)cap_rng, (int32_t)cap_rsmall);
@@ -2380,17 +2382,28 @@ __attribute__((visibility("default"))) int hp_find_subfofs(struct particle *d_pa
             HP_TRY(hipStreamSynchronize(s));
             if (na[kSfErr])
                 return set_error(HP_EHIP, "hp_find_subfofs: scratch overflow (%d)", na[kSfErr]);
-            const int32_t nr = na[kSfRange];
+            const int32_t nr = na[kSfRange], nm = na[kSfRangeMid];
             nsmall = na[kSfRangeSmall];
-            if (!nr) break;
+            if (!nr && !nm) break;
             // each level halves the value span of every range it splits
             if (level > 40) return set_error(HP_EHIP, "hp_find_subfofs: output sort too deep");
             k_fill_i32<<<1, 1, 0, s>>>(cnt + kSfRange, 0, 1);
             HP_LAUNCHED();
-            k_sort_split<kSortThreads1><<<(uint32_t)nr, kSortThreads1, 0, s>>>(
-                ra, v0, v1, x0, x1, B.rk.as<int32_t>(), B.bpos.as<int32_t>(),
-                B.spos.as<int32_t>(), rb, rs, cnt, (int32_t)cap_rng, (int32_t)cap_rsmall);
+            k_fill_i32<<<1, 1, 0, s>>>(cnt + kSfRangeMid, 0, 1);
             HP_LAUNCHED();
+            if (nr) {
+                k_sort_split<kSortThreads1><<<(uint32_t)nr, kSortThreads1, 0, s>>>(
+                    ra, v0, v1, x0, x1, B.rk.as<int32_t>(), B.bpos.as<int32_t>(),
+                    B.spos.as<int32_t>(), rb, rs, cnt, (int32_t)cap_rng, (int32_t)cap_rsmall);
+                HP_LAUNCHED();
+            }
+            if (nm) {
+                k_sort_split<kSortThreads2><<<(uint32_t)nm, kSortThreads2, 0, s>>>(
+                    ra + 2 * cap_rng - 1, v0, v1, x0, x1, B.rk.as<int32_t>(),
+                    B.bpos.as<int32_t>(), B.spos.as<int32_t>(), rb, rs, cnt, (int32_t)cap_rng,
+                    (int32_t)cap_rsmall);
+                HP_LAUNCHED();
+            }
             std::swap(ra, rb);
         }
         if (nsmall > 0) {

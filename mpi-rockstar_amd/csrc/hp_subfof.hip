@@ -84,9 +84,10 @@ enum SfCounter {
     kSfActiveBig = 4,  // nodes above kSfBigNode points: split with 1024 threads
     kSfFused = 5,      // nodes of at most kFuseMax points: finished by k_sub_fused
     kSfEdges = 6,      // 64-bit (6 and 7): pairs the linking walk listed
-    kSfRange = 8,      // output-order ranges to split next level
+    kSfRange = 8,      // output-order ranges above kSortMid values to split next level
     kSfRangeSmall = 9, // output-order ranges finished by one wavefront
-    kSfCounters = 10
+    kSfRangeMid = 10,  // output-order ranges of at most kSortMid values to split next level
+    kSfCounters = 11
 };
 constexpr int kSfBigNode = 8192;
 
@@ -1371,64 +1372,120 @@ __global__ void k_sub_compact(int32_t m, const int32_t *__restrict__ off,
 }
 
 constexpr int kTokThreads = 1024;
-constexpr int kTokLds = 16384;  // a group's numbers held in LDS up to this many
+constexpr int kTokLds = 12288;  // a group's numbers held in LDS up to this many
+constexpr int kTokHash = 1024;  // one chunk's root pairs, deduplicated
 
-// root of x with path halving; u is the group's table (LDS or global, one
-// workgroup only)
+// root of x with path halving, in the group's table u (LDS, or global when
+// the group has more than kTokLds numbers; one workgroup only)
+template <bool kLds>
 __device__ __forceinline__ int32_t tok_find(int32_t *u, int32_t x) {
-    for (;;) {
-        const int32_t y = __hip_atomic_load(u + x, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        if (y == x) return x;
-        const int32_t z = __hip_atomic_load(u + y, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        if (z != y) __hip_atomic_store(u + x, z, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        x = z;
+    if constexpr (kLds) {
+        volatile int32_t *v = u;
+        for (;;) {
+            const int32_t y = v[x];
+            if (y == x) return x;
+            const int32_t z = v[y];
+            if (z != y) v[x] = z;
+            x = z;
+        }
+    } else {
+        for (;;) {
+            const int32_t y = __hip_atomic_load(u + x, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            if (y == x) return x;
+            const int32_t z = __hip_atomic_load(u + y, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            if (z != y) __hip_atomic_store(u + x, z, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            x = z;
+        }
     }
 }
 
-// The merges of one group in step order; root[L0 + k] = the root of number
-// k (group-local numbers).  A chunk of kTokThreads pairs at a time: every
-// thread finds its pair's two roots as the chunk starts, and one thread
-// replays, in order, the pairs whose roots differ then.  A group with more
-// than kTokLds numbers keeps its table in root itself.
-__global__ __launch_bounds__(kTokThreads) void k_sub_tokens(
-    const SubGroup *__restrict__ groups, const int32_t *__restrict__ eoff,
-    const int2 *__restrict__ ev, const int32_t *__restrict__ first_label,
-    int32_t *__restrict__ root) {
-    __shared__ int32_t s_u[kTokLds];
-    __shared__ int2 s_pair[kTokThreads];
-    __shared__ int s_w[kTokThreads / kWave];
-    const SubGroup g = groups[blockIdx.x];
-    if (g.n < 2) return;
-    const int32_t L0 = first_label[g.off];
-    const int32_t nl = first_label[g.off + g.n] - L0;
-    if (nl <= 0) return;
-    int32_t *u = nl <= kTokLds ? s_u : root + L0;
-    for (int32_t k = threadIdx.x; k < nl; k += kTokThreads) u[k] = k;
+template <bool kLds>
+__device__ __forceinline__ void tok_set(int32_t *u, int32_t x, int32_t v) {
+    if constexpr (kLds) *(volatile int32_t *)(u + x) = v;
+    else __hip_atomic_store(u + x, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+struct TokShared {
+    int2 pair[kTokThreads];
+    unsigned long long key[kTokHash];
+    int32_t first[kTokHash];
+    int w[kTokThreads / kWave];
+};
+
+// One group's merges, a chunk of kTokThreads pairs at a time.  Every thread
+// finds its pair's two roots as the chunk starts; a pair whose roots are
+// equal then, or equal as a set to an earlier pair's of the chunk, cannot
+// merge.  One thread replays the others in order: the root of step i's small
+// FOF under the root of the listed particle's.
+template <bool kLds>
+__device__ void tok_group(int32_t *u, int32_t nl, int32_t q0, int32_t q1, int32_t L0,
+                          const int2 *__restrict__ ev, int32_t *__restrict__ root,
+                          TokShared &S) {
+    for (int32_t k = threadIdx.x; k < nl; k += kTokThreads) tok_set<kLds>(u, k, k);
     __syncthreads();
-    const int32_t q0 = eoff[g.off], q1 = eoff[g.off + g.n];
     for (int32_t base = q0; base < q1; base += kTokThreads) {
         const int32_t q = base + threadIdx.x;
         int32_t ra = 0, rb = 0;
         if (q < q1) {
             const int2 p = ev[q];
-            ra = tok_find(u, p.x - L0);
-            rb = tok_find(u, p.y - L0);
+            ra = tok_find<kLds>(u, p.x - L0);
+            rb = tok_find<kLds>(u, p.y - L0);
         }
+        for (int k = threadIdx.x; k < kTokHash; k += kTokThreads) {
+            S.key[k] = 0ull;
+            S.first[k] = kTokThreads;
+        }
+        __syncthreads();
+        // the earliest pair of each root set claims its slot (open
+        // addressing; at most kTokThreads of kTokHash slots are taken)
+        int h = 0;
+        if (ra != rb) {
+            const uint32_t lo = (uint32_t)min(ra, rb), hi = (uint32_t)max(ra, rb);
+            const unsigned long long key = ((unsigned long long)lo << 32) | hi;
+            h = (int)((lo * 0x9E3779B1u) ^ (hi * 0x85EBCA77u)) & (kTokHash - 1);
+            for (;;) {
+                const unsigned long long was = atomicCAS(&S.key[h], 0ull, key);
+                if (was == 0ull || was == key) break;
+                h = (h + 1) & (kTokHash - 1);
+            }
+            atomicMin(&S.first[h], (int)threadIdx.x);
+        }
+        __syncthreads();
+        const bool keep = ra != rb && S.first[h] == (int)threadIdx.x;
         int tot;
-        const int at = block_rank<kTokThreads>(ra != rb, &tot, s_w);
-        if (ra != rb) s_pair[at] = make_int2(ra, rb);
+        const int at = block_rank<kTokThreads>(keep, &tot, S.w);
+        if (keep) S.pair[at] = make_int2(ra, rb);
         __syncthreads();
         if (threadIdx.x == 0)
             for (int k = 0; k < tot; k++) {
-                const int32_t a = tok_find(u, s_pair[k].x), b = tok_find(u, s_pair[k].y);
-                if (a != b) __hip_atomic_store(u + a, b, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                const int32_t a = tok_find<kLds>(u, S.pair[k].x), b = tok_find<kLds>(u, S.pair[k].y);
+                if (a != b) tok_set<kLds>(u, a, b);
             }
         __syncthreads();
     }
     for (int32_t k = threadIdx.x; k < nl; k += kTokThreads) {
-        const int32_t rt = tok_find(u, k);
+        const int32_t rt = tok_find<kLds>(u, k);
         __hip_atomic_store(root + L0 + k, rt, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     }
+}
+
+// The merges of each group in step order: root[L0 + k] = the root of number
+// k (group-local numbers).  A group with more than kTokLds numbers keeps its
+// table in root itself.
+__global__ __launch_bounds__(kTokThreads) void k_sub_tokens(
+    const SubGroup *__restrict__ groups, const int32_t *__restrict__ eoff,
+    const int2 *__restrict__ ev, const int32_t *__restrict__ first_label,
+    int32_t *__restrict__ root) {
+    __shared__ int32_t s_u[kTokLds];
+    __shared__ TokShared S;
+    const SubGroup g = groups[blockIdx.x];
+    if (g.n < 2) return;
+    const int32_t L0 = first_label[g.off];
+    const int32_t nl = first_label[g.off + g.n] - L0;
+    if (nl <= 0) return;
+    const int32_t q0 = eoff[g.off], q1 = eoff[g.off + g.n];
+    if (nl <= kTokLds) tok_group<true>(s_u, nl, q0, q1, L0, ev, root, S);
+    else tok_group<false>(root + L0, nl, q0, q1, L0, ev, root, S);
 }
 
 // each particle's sort value (its root's number in the group, -1 without a
@@ -1473,19 +1530,29 @@ __global__ void k_sub_values(int32_t m, const int32_t *__restrict__ slot_group,
 // and slots; every finished range ends in the first.
 
 constexpr int kSortSmall = 128;
+constexpr int kSortMid = 4096;  // ranges up to this size split with 256 threads
 constexpr int kSortThreads1 = 1024;
+constexpr int kSortThreads2 = 256;
 
 struct SortRange {
-    int32_t start, n, where, pad;
+    int32_t start, n, where;
+    int32_t lo, hi;  // the range's least and greatest values (lo > hi: not known yet)
+    int32_t pad;
 };
 
+// a range goes on the list that suits its size; next holds the large ones
+// from 0 and the mid-sized ones from cap_next down
 __device__ __forceinline__ void push_range(SortRange r, SortRange *__restrict__ next,
                                            SortRange *__restrict__ small,
                                            int32_t *__restrict__ counters, int32_t cap_next,
                                            int32_t cap_small) {
-    if (r.n > kSortSmall) {
+    if (r.n > kSortMid) {
         const int32_t k = atomicAdd(&counters[kSfRange], 1);
         if (k < cap_next) next[k] = r;
+        else atomicOr(&counters[kSfErr], 4);
+    } else if (r.n > kSortSmall) {
+        const int32_t k = atomicAdd(&counters[kSfRangeMid], 1);
+        if (k < cap_next) next[2 * cap_next - 1 - k] = r;
         else atomicOr(&counters[kSfErr], 4);
     } else {
         const int32_t k = atomicAdd(&counters[kSfRangeSmall], 1);
@@ -1500,7 +1567,26 @@ __global__ void k_sort_roots(int32_t ng, const SubGroup *__restrict__ groups,
     const int32_t gi = blockIdx.x * blockDim.x + threadIdx.x;
     if (gi >= ng) return;
     const SubGroup g = groups[gi];
-    if (g.n >= 2) push_range(SortRange{g.off, g.n, 0, 0}, next, small, counters, cap_next, cap_small);
+    if (g.n >= 2)
+        push_range(SortRange{g.off, g.n, 0, 1, 0, 0}, next, small, counters, cap_next, cap_small);
+}
+
+// exclusive prefix sum over the workgroup's threads, and the total
+template <int NT>
+__device__ __forceinline__ int block_excl_scan(int x, int *total, int *s_w) {
+    constexpr int NW = NT / kWave;
+    const int incl = wave_scan_add(x);
+    const int wv = threadIdx.x / kWave;
+    if (lane_id() == kWave - 1) s_w[wv] = incl;
+    __syncthreads();
+    int off = 0, tot = 0;
+    for (int j = 0; j < NW; j++) {
+        off += j < wv ? s_w[j] : 0;
+        tot += s_w[j];
+    }
+    __syncthreads();
+    *total = tot;
+    return off + incl - x;
 }
 
 template <int NT>
@@ -1521,7 +1607,9 @@ __device__ __forceinline__ void block_minmax(int32_t &lo, int32_t &hi, int32_t *
     __syncthreads();
 }
 
-// one level: every listed range splits, one workgroup per range
+// one level: every listed range splits, one workgroup per range: 1024
+// threads (four values each per round) for the large ranges, 256 (one value
+// each) for the mid-sized ones, whose list runs down from `ranges`
 template <int NT>
 __global__ __launch_bounds__(NT) void k_sort_split(
     const SortRange *__restrict__ ranges, int32_t *__restrict__ v0, int32_t *__restrict__ v1,
@@ -1529,21 +1617,26 @@ __global__ __launch_bounds__(NT) void k_sort_split(
     int32_t *__restrict__ bpos, int32_t *__restrict__ spos, SortRange *__restrict__ next,
     SortRange *__restrict__ small, int32_t *__restrict__ counters, int32_t cap_next,
     int32_t cap_small) {
-    __shared__ int s_w[2 * (NT / kWave)];
+    static_assert(NT == 1024 || NT == 256, "launch shapes");
+    __shared__ int s_w[4 * (NT / kWave)];
     __shared__ int32_t s_k[1];
-    const SortRange sr = ranges[blockIdx.x];
+    const SortRange sr = NT == 1024 ? ranges[blockIdx.x] : *(ranges - blockIdx.x);
     const int32_t s0 = sr.start, n = sr.n;
     const int32_t *__restrict__ vs = sr.where ? v1 : v0;
     const int32_t *__restrict__ xs = sr.where ? x1 : x0;
     int32_t *__restrict__ vd = sr.where ? v0 : v1;
     int32_t *__restrict__ xd = sr.where ? x0 : x1;
-    int32_t lo = 0x7fffffff, hi = -0x7fffffff - 1;
-    for (int32_t p = threadIdx.x; p < n; p += NT) {
-        const int32_t v = vs[s0 + p];
-        lo = min(lo, v);
-        hi = max(hi, v);
+    int32_t lo = sr.lo, hi = sr.hi;
+    if (lo > hi) {  // a group's whole range: its values' span first
+        lo = 0x7fffffff;
+        hi = -0x7fffffff - 1;
+        for (int32_t p = threadIdx.x; p < n; p += NT) {
+            const int32_t v = vs[s0 + p];
+            lo = min(lo, v);
+            hi = max(hi, v);
+        }
+        block_minmax<NT>(lo, hi, s_w);
     }
-    block_minmax<NT>(lo, hi, s_w);
     if (lo == hi) {  // one value: in order already, to the first buffer
         if (sr.where)
             for (int32_t p = threadIdx.x; p < n; p += NT) {
@@ -1555,17 +1648,27 @@ __global__ __launch_bounds__(NT) void k_sort_split(
     const int32_t pivot = (int32_t)((int64_t)lo + ((int64_t)hi - (int64_t)lo) / 2);
     // classes, ranks within the class, positions of each class
     int32_t nbig = 0;
-    for (int32_t base = 0; base < n; base += NT) {
-        const int32_t p = base + threadIdx.x;
-        const bool in = p < n;
-        const bool big = in && vs[s0 + p] > pivot;
+    for (int32_t base = 0; base < n; base += 4 * NT) {
+        const int32_t p0 = base + 4 * (int32_t)threadIdx.x;
+        int nb = 0;
+        bool big[4];
+        for (int k = 0; k < 4; k++) {
+            big[k] = p0 + k < n && vs[s0 + p0 + k] > pivot;
+            nb += big[k];
+        }
         int tot;
-        const int rb = block_rank<NT>(big, &tot, s_w);
-        if (in) {
-            const int32_t rank = big ? nbig + rb : p - (nbig + rb);
-            rk[s0 + p] = big ? ~rank : rank;
-            if (big) bpos[s0 + rank] = p;
-            else spos[s0 + rank] = p;
+        int32_t run = nbig + block_excl_scan<NT>(nb, &tot, s_w);  // bigs before p0
+        for (int k = 0; k < 4; k++) {
+            const int32_t p = p0 + k;
+            if (p >= n) break;
+            if (big[k]) {
+                rk[s0 + p] = ~run;
+                bpos[s0 + run] = p;
+                run++;
+            } else {
+                rk[s0 + p] = p - run;
+                spos[s0 + p - run] = p;
+            }
         }
         nbig += tot;
     }
@@ -1578,17 +1681,43 @@ __global__ __launch_bounds__(NT) void k_sort_split(
     }
     __syncthreads();
     const int32_t S = s_k[0];
-    for (int32_t p = threadIdx.x; p < n; p += NT) {
-        const int32_t r = rk[s0 + p];
-        const bool big = r < 0;
-        const int32_t d = partition_dest(p, big, big ? ~r : r, n, nsmall, S, bpos + s0, spos + s0);
-        vd[s0 + d] = vs[s0 + p];
-        xd[s0 + d] = xs[s0 + p];
+    // the scatter, and the value spans of the two halves
+    int32_t llo = 0x7fffffff, lhi = -0x7fffffff - 1, rlo = llo, rhi = lhi;
+    for (int32_t base = 0; base < n; base += 4 * NT) {
+        // four independent positions per thread, their loads in flight together
+        int32_t r[4], v[4], x[4], d[4];
+        for (int k = 0; k < 4; k++) {
+            const int32_t p = base + k * NT + threadIdx.x;
+            r[k] = p < n ? rk[s0 + p] : 0;
+            v[k] = p < n ? vs[s0 + p] : 0;
+            x[k] = p < n ? xs[s0 + p] : 0;
+        }
+        for (int k = 0; k < 4; k++) {
+            const int32_t p = base + k * NT + threadIdx.x;
+            const bool big = r[k] < 0;
+            d[k] = p < n ? partition_dest(p, big, big ? ~r[k] : r[k], n, nsmall, S, bpos + s0,
+                                          spos + s0)
+                         : 0;
+        }
+        for (int k = 0; k < 4; k++) {
+            const int32_t p = base + k * NT + threadIdx.x;
+            if (p >= n) break;
+            vd[s0 + d[k]] = v[k];
+            xd[s0 + d[k]] = x[k];
+            if (r[k] < 0) {
+                rlo = min(rlo, v[k]);
+                rhi = max(rhi, v[k]);
+            } else {
+                llo = min(llo, v[k]);
+                lhi = max(lhi, v[k]);
+            }
+        }
     }
-    __syncthreads();
+    block_minmax<NT>(llo, lhi, s_w);
+    block_minmax<NT>(rlo, rhi, s_w + 2 * (NT / kWave));
     if (threadIdx.x != 0) return;
     const int32_t wh = 1 - sr.where;
-    const SortRange L{s0, nsmall, wh, 0}, R{s0 + nsmall, n - nsmall, wh, 0};
+    const SortRange L{s0, nsmall, wh, llo, lhi, 0}, R{s0 + nsmall, n - nsmall, wh, rlo, rhi, 0};
     for (const SortRange &c : {L, R}) {
         if (c.n >= 2) {
             push_range(c, next, small, counters, cap_next, cap_small);
