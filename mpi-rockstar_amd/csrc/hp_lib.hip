@@ -344,6 +344,9 @@ struct Engine {
     // kernels of the other sizes beside the largest, and their events
     hipStream_t side[3] = {nullptr, nullptr, nullptr};
     hipEvent_t fork_ev = nullptr, join_ev[3] = {nullptr, nullptr, nullptr};
+    // a recorded step's bound-pass sort branch (forked after the subtrees)
+    hipEvent_t sort_fork_ev = nullptr, sort_join_ev = nullptr;
+    bool sort_forked = false;
     int32_t *h_counters = nullptr;  // pinned mirror
     unsigned long long *h_work = nullptr;
     uint64_t *h_stamps = nullptr;   // pinned: wall-clock stamps between a graph step's phases
@@ -419,6 +422,8 @@ struct Engine {
             HP_TRY(hipEventCreateWithFlags(&join_ev[b], hipEventDisableTiming));
         }
         HP_TRY(hipEventCreateWithFlags(&fork_ev, hipEventDisableTiming));
+        HP_TRY(hipEventCreateWithFlags(&sort_fork_ev, hipEventDisableTiming));
+        HP_TRY(hipEventCreateWithFlags(&sort_join_ev, hipEventDisableTiming));
         // The workgroup subtree kernels take tasks from dynamic queues, the
         // wavefront one at a fixed stride, until the lists are empty; their
         // grids are the workgroups that fit on the device at once.
@@ -528,6 +533,8 @@ struct Engine {
             if (h_work) (void)hipHostFree(h_work);
             if (h_stamps) (void)hipHostFree(h_stamps);
             if (fork_ev) (void)hipEventDestroy(fork_ev);
+            if (sort_fork_ev) (void)hipEventDestroy(sort_fork_ev);
+            if (sort_join_ev) (void)hipEventDestroy(sort_join_ev);
             for (int b = 0; b < 3; b++) {
                 if (join_ev[b]) (void)hipEventDestroy(join_ev[b]);
                 if (side[b]) {
@@ -764,7 +771,7 @@ struct Engine {
     }
 
     // subtrees, then the multipoles of the nodes of `levels` split levels
-    int enqueue_multipole(int levels, bool fork = false) {
+    int enqueue_multipole(int levels, bool fork = false, const Tables *sort_T = nullptr) {
         hipStream_t s = stream;
         fork = fork && side[0];
         const int32_t *cnt = counters.as<int32_t>();
@@ -830,6 +837,16 @@ struct Engine {
             for (int b = 0; b < 3; b++) {
                 HP_TRY(hipEventRecord(join_ev[b], side[b]));
                 HP_TRY(hipStreamWaitEvent(s, join_ev[b], 0));
+            }
+            // the tree order is final: the bound pass's sort on a branch
+            // that the bound pass joins
+            if (sort_T) {
+                HP_TRY(hipEventRecord(sort_fork_ev, s));
+                HP_TRY(hipStreamWaitEvent(side[0], sort_fork_ev, 0));
+                int r;
+                if ((r = enqueue_bound_sort(*sort_T, side[0]))) return r;
+                HP_TRY(hipEventRecord(sort_join_ev, side[0]));
+                sort_forked = true;
             }
         }
         // the roots that stay leaves (at most 10 points, or as many finite)
@@ -1070,6 +1087,7 @@ struct Engine {
         hipGraph_t graph = nullptr;
         HP_TRY(hipStreamBeginCapture(stream, hipStreamCaptureModeThreadLocal));
         const int r = enqueue();
+        sort_forked = false;
         const hipError_t e = hipStreamEndCapture(stream, &graph);
         if (r) {
             if (graph) (void)hipGraphDestroy(graph);
@@ -1108,7 +1126,8 @@ struct Engine {
                  }
                  k_stamp<<<1, 1, 0, stream>>>(h_stamps, 1);
                  HP_LAUNCHED();
-                 if (tree && (q = enqueue_multipole(levels, true))) return q;
+                 sort_forked = false;
+                 if (tree && (q = enqueue_multipole(levels, true, &T))) return q;
                  k_stamp<<<1, 1, 0, stream>>>(h_stamps, 2);
                  HP_LAUNCHED();
                  if ((q = enqueue_walk(T, d_po, o))) return q;
@@ -1315,9 +1334,10 @@ struct Engine {
         return HP_OK;
     }
 
-    int enqueue_unbind(const Tables &T, struct potential *d_po, hp_halo *d_halos,
-                       uint8_t *d_bound) {
-        hipStream_t s = stream;
+    // The bound pass's sort: each call's records by r2 inside the call.  It
+    // needs the tree order only, not the potentials, so a recorded step runs
+    // it on a branch of its own beside the multipoles and the walk.
+    int enqueue_bound_sort(const Tables &T, hipStream_t s) {
         const int64_t N = T.nrec;
         const int32_t H = (int32_t)T.nhalos;
         if (N == 0 || H == 0) return HP_OK;
@@ -1392,6 +1412,22 @@ struct Engine {
         }
         sorted_keys = k1;
         sorted_vals = v1;
+        return HP_OK;
+    }
+
+    int enqueue_unbind(const Tables &T, struct potential *d_po, hp_halo *d_halos,
+                       uint8_t *d_bound) {
+        hipStream_t s = stream;
+        const int64_t N = T.nrec;
+        const int32_t H = (int32_t)T.nhalos;
+        if (N == 0 || H == 0) return HP_OK;
+        int r;
+        if (sort_forked) {  // the sort ran on its branch: join it
+            sort_forked = false;
+            HP_TRY(hipStreamWaitEvent(s, sort_join_ev, 0));
+        } else if ((r = enqueue_bound_sort(T, s))) {
+            return r;
+        }
         struct potential *S = srt.as<struct potential>();
         float4 *RV = srt_rv.as<float4>();
         k_gather_sorted<<<blocks(N, 256), 256, 0, s>>>(d_po, sorted_vals, (int32_t)N, S, RV);
