@@ -347,6 +347,9 @@ struct Engine {
     // a recorded step's bound-pass sort branch (forked after the subtrees)
     hipEvent_t sort_fork_ev = nullptr, sort_join_ev = nullptr;
     bool sort_forked = false;
+    // and its walk branch for the calls without split levels
+    hipEvent_t walk_fork_ev = nullptr, walk_join_ev = nullptr;
+    bool walk_forked = false;
     int32_t *h_counters = nullptr;  // pinned mirror
     unsigned long long *h_work = nullptr;
     uint64_t *h_stamps = nullptr;   // pinned: wall-clock stamps between a graph step's phases
@@ -359,6 +362,13 @@ struct Engine {
     int32_t big_min() const { return *std::min_element(bucket_big, bucket_big + kTuneBuckets); }
     int xcds = 1;             // XCDs of the device (each with its own L2)
     int32_t walk_groups = 0;  // entries of the loaded walk table (XCD layout)
+    // The walk table holds the wavefronts of the calls with split levels
+    // first (walk_groups_lv entries), then the others, each part laid out on
+    // the XCDs by itself: a recorded step walks the second part as soon as
+    // those calls' trees are done.  walk_layout_big: the thresholds the split
+    // was made with.
+    int32_t walk_groups_lv = 0;
+    int32_t walk_layout_big[kTuneBuckets] = {-1, -1, -1, -1};
     int sub_grid[kSubClasses] = {}, wave_grid = 0;  // resident workgroups of the subtree kernels
 
     // batch tables on the device
@@ -424,6 +434,8 @@ struct Engine {
         HP_TRY(hipEventCreateWithFlags(&fork_ev, hipEventDisableTiming));
         HP_TRY(hipEventCreateWithFlags(&sort_fork_ev, hipEventDisableTiming));
         HP_TRY(hipEventCreateWithFlags(&sort_join_ev, hipEventDisableTiming));
+        HP_TRY(hipEventCreateWithFlags(&walk_fork_ev, hipEventDisableTiming));
+        HP_TRY(hipEventCreateWithFlags(&walk_join_ev, hipEventDisableTiming));
         // The workgroup subtree kernels take tasks from dynamic queues, the
         // wavefront one at a fixed stride, until the lists are empty; their
         // grids are the workgroups that fit on the device at once.
@@ -535,6 +547,8 @@ struct Engine {
             if (fork_ev) (void)hipEventDestroy(fork_ev);
             if (sort_fork_ev) (void)hipEventDestroy(sort_fork_ev);
             if (sort_join_ev) (void)hipEventDestroy(sort_join_ev);
+            if (walk_fork_ev) (void)hipEventDestroy(walk_fork_ev);
+            if (walk_join_ev) (void)hipEventDestroy(walk_join_ev);
             for (int b = 0; b < 3; b++) {
                 if (join_ev[b]) (void)hipEventDestroy(join_ev[b]);
                 if (side[b]) {
@@ -615,6 +629,23 @@ struct Engine {
         return HP_OK;
     }
 
+    // A call has split levels when it holds more points than its bucket's
+    // threshold (the device splits on its finite points, never more).
+    int layout_walk(const Tables &T) {
+        std::vector<Group> lv, rest;
+        for (const Group &x : T.groups) {
+            const int64_t n = T.count[(size_t)x.root];
+            (n > bucket_big[tune_bucket(n)] ? lv : rest).push_back(x);
+        }
+        place_on_xcds(lv, xcds);
+        place_on_xcds(rest, xcds);
+        walk_groups_lv = (int32_t)lv.size();
+        lv.insert(lv.end(), rest.begin(), rest.end());
+        walk_groups = (int32_t)lv.size();
+        std::copy(bucket_big, bucket_big + kTuneBuckets, walk_layout_big);
+        return upload(groups, lv);
+    }
+
     int64_t max_count = 0;  // the largest call of the loaded tables
     int load_tables(const Tables &T) {
         int r;
@@ -622,10 +653,7 @@ struct Engine {
         if ((r = upload(hfirst, T.first))) return r;
         if ((r = upload(hcount, T.count))) return r;
         if ((r = upload(root_tiles, T.root_tiles))) return r;
-        std::vector<Group> g = T.groups;
-        place_on_xcds(g, xcds);
-        walk_groups = (int32_t)g.size();
-        if ((r = upload(groups, g))) return r;
+        if ((r = layout_walk(T))) return r;
         if ((r = upload(direct_blocks, T.direct_blocks))) return r;
         if ((r = upload(scan_tiles, T.scan_tiles))) return r;
         if ((r = upload(call_tile0, T.call_tile0))) return r;
@@ -771,7 +799,12 @@ struct Engine {
     }
 
     // subtrees, then the multipoles of the nodes of `levels` split levels
-    int enqueue_multipole(int levels, bool fork = false, const Tables *sort_T = nullptr) {
+    // fork: a recorded step's branches (subtree sizes; with sort_T the bound
+    // pass's sort; with walk_po and walk_o the walk of the calls without
+    // split levels)
+    int enqueue_multipole(int levels, bool fork = false, const Tables *sort_T = nullptr,
+                          struct potential *walk_po = nullptr,
+                          const hp_options *walk_o = nullptr) {
         hipStream_t s = stream;
         fork = fork && side[0];
         const int32_t *cnt = counters.as<int32_t>();
@@ -852,6 +885,17 @@ struct Engine {
         // the roots that stay leaves (at most 10 points, or as many finite)
         k_leaf_com<<<2048, 256, 0, s>>>(nodes.as<Node>(), F.as<float4>(), cnt, PARTICLE_MASS);
         HP_LAUNCHED();
+        // the calls without split levels have their whole trees now: their
+        // walk on a branch beside the split levels' nodes (joined after the
+        // walk of the others)
+        if (fork && sort_T && walk_o) {
+            HP_TRY(hipEventRecord(walk_fork_ev, s));
+            HP_TRY(hipStreamWaitEvent(side[1], walk_fork_ev, 0));
+            int r;
+            if ((r = enqueue_walk(*sort_T, walk_po, *walk_o, 2, side[1]))) return r;
+            HP_TRY(hipEventRecord(walk_join_ev, side[1]));
+            walk_forked = true;
+        }
         // Without split levels every other node is a subtree's, and the
         // subtree kernels finished it (its root included): the fat leaves,
         // the internal nodes and dmin of the levels' nodes have no work.
@@ -872,15 +916,21 @@ struct Engine {
         return HP_OK;
     }
 
-    int enqueue_walk(const Tables &T, struct potential *d_po, const hp_options &o) {
-        hipStream_t s = stream;
-        if (!T.groups.empty()) {
+    // part 0: every call; 1: the calls with split levels; 2: the others
+    // and the direct-sum calls (on stream st)
+    int enqueue_walk(const Tables &T, struct potential *d_po, const hp_options &o, int part = 0,
+                     hipStream_t st = nullptr) {
+        hipStream_t s = st ? st : stream;
+        const int32_t g0 = part == 2 ? walk_groups_lv : 0;
+        const int32_t ng = part == 0 ? walk_groups
+                                     : (part == 1 ? walk_groups_lv : walk_groups - walk_groups_lv);
+        if (!T.groups.empty() && ng > 0) {
             WalkArgs a;
             a.nodes = nodes.as<Node>();
             a.fin = F.as<float4>();
             a.leaf_of = leaf_of.as<int32_t>();
-            a.groups = groups.as<Group>();
-            a.ngroups = walk_groups;
+            a.groups = groups.as<Group>() + g0;
+            a.ngroups = ng;
             a.counters = counters.as<int32_t>();
             a.nrec = (int32_t)T.nrec;
             a.cap_nodes = (int32_t)cap_nodes;
@@ -892,7 +942,7 @@ struct Engine {
             a.po = d_po;
             a.err = counters.as<int32_t>() + kErrFlags;
             a.work = work.as<unsigned long long>();
-            const int nb = blocks((int64_t)walk_groups * kWave, 64);
+            const int nb = blocks((int64_t)ng * kWave, 64);
             const bool cnt = o.count_work != 0;
 #define HP_WALK(ACC)                                                                   \
     do {                                                                               \
@@ -905,7 +955,7 @@ struct Engine {
 #undef HP_WALK
             HP_LAUNCHED();
         }
-        if (!T.direct_blocks.empty()) {
+        if (!T.direct_blocks.empty() && part != 1) {
             DirectArgs d;
             d.nodes = nodes.as<Node>();
             d.fin = T.method == HP_METHOD_DIRECT ? A.as<float4>() : F.as<float4>();
@@ -1087,7 +1137,7 @@ struct Engine {
         hipGraph_t graph = nullptr;
         HP_TRY(hipStreamBeginCapture(stream, hipStreamCaptureModeThreadLocal));
         const int r = enqueue();
-        sort_forked = false;
+        sort_forked = walk_forked = false;
         const hipError_t e = hipStreamEndCapture(stream, &graph);
         if (r) {
             if (graph) (void)hipGraphDestroy(graph);
@@ -1114,6 +1164,9 @@ struct Engine {
         drop_graphs();
         int r;
         const bool tree = T.method != HP_METHOD_DIRECT;
+        if (tree && !std::equal(bucket_big, bucket_big + kTuneBuckets, walk_layout_big) &&
+            (r = layout_walk(T)))
+            return r;
         if ((r = capture(&g.step, [&]() -> int {
                  int q;
                  k_stamp<<<1, 1, 0, stream>>>(h_stamps, 0);
@@ -1126,11 +1179,15 @@ struct Engine {
                  }
                  k_stamp<<<1, 1, 0, stream>>>(h_stamps, 1);
                  HP_LAUNCHED();
-                 sort_forked = false;
-                 if (tree && (q = enqueue_multipole(levels, true, &T))) return q;
+                 sort_forked = walk_forked = false;
+                 if (tree && (q = enqueue_multipole(levels, true, &T, d_po, &o))) return q;
                  k_stamp<<<1, 1, 0, stream>>>(h_stamps, 2);
                  HP_LAUNCHED();
-                 if ((q = enqueue_walk(T, d_po, o))) return q;
+                 if ((q = enqueue_walk(T, d_po, o, walk_forked ? 1 : 0))) return q;
+                 if (walk_forked) {
+                     walk_forked = false;
+                     HP_TRY(hipStreamWaitEvent(stream, walk_join_ev, 0));
+                 }
                  k_stamp<<<1, 1, 0, stream>>>(h_stamps, 3);
                  HP_LAUNCHED();
                  if ((q = enqueue_unbind(T, d_po, d_halos, d_bound))) return q;
