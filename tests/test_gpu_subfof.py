@@ -100,6 +100,28 @@ def test_gpu_subfof_small_min_halo_particles(restatement):
 
 
 @pytest.mark.gpu
+def test_gpu_subfof_large_group_and_dense_clump(restatement):
+    """Two paths the seeded batches do not reach.  A group of 150,000
+    particles creates 13,647 small FOFs, more than the merge replay holds in
+    LDS (12,288), so its union-find table lives in global memory.  A clump of
+    6,000 particles inside a sparse cloud links every pair of the clump
+    (about 18M pairs, and thousands per leaf chunk of the walk): more than
+    the pair list's first allocation (8 per particle) and more than any
+    earlier batch of this file left it, so the linking walk runs again with
+    room for every pair."""
+    big = catalog.fof_groups(seed=41, n_groups=1, lo=150000, hi=150001)
+    assert big.particles.shape[0] == 150000
+    _check_batch(restatement, [big.particles.copy()])
+    rng = np.random.default_rng(5)
+    x = np.concatenate([rng.normal(0, 1.0, (14700, 6)), rng.normal(0, 1e-3, (6000, 6))])
+    clump = np.zeros(len(x), catalog.PARTICLE_DTYPE)
+    clump["id"] = np.arange(len(x))
+    clump["pos"] = x.astype(np.float32)
+    _, subs, _ = _check_batch(restatement, [clump])
+    assert subs["num_p"].max() >= 6000
+
+
+@pytest.mark.gpu
 def test_gpu_subfof_shared_walk_equals_ordered_walk():
     """The default next-closest search (one walk per wavefront, the rare
     close calls repeated in the reference's order) against every search in
