@@ -1328,6 +1328,8 @@ __global__ void k_sub_label(int32_t m, int32_t *__restrict__ par,
     lab[i] = first_label[x];
 }
 
+constexpr int32_t kEventDedup = 64;
+
 // The pairs of step i that can merge, (i's number, x's number) for each
 // listed x numbered before step i under another number, in list order.  A
 // number met earlier in the same step is dropped: the first merge with it
@@ -1352,8 +1354,10 @@ __global__ void k_sub_events(int32_t m, const int32_t *__restrict__ off,
             if (kmin[x] >= i) continue;
             const int32_t lx = lab[x];
             if (lx == f) continue;
+            // checked against the step's first kEventDedup pairs only: a
+            // repeat kept past them is one more pair that cannot merge
             bool seen = false;
-            for (int32_t q = 0; q < k && !seen; q++) seen = stage[e0 + q].y == lx;
+            for (int32_t q = 0; q < min(k, kEventDedup) && !seen; q++) seen = stage[e0 + q].y == lx;
             if (!seen) stage[e0 + k++] = make_int2(f, lx);
         }
     }
