@@ -824,16 +824,22 @@ struct Engine {
             const int32_t m = (int32_t)cap_small + 1;
             const int32_t nbk = (m + kScanBlock - 1) / kScanBlock;
             int32_t *off = wave_off.as<int32_t>();
-            k_wave_need<<<std::min(blocks(m, 256), 4096), 256, 0, s>>>(small0.as<Task>(), ba, off, m);
-            HP_LAUNCHED();
-            k_scan_blocks<<<nbk, 1024, 0, s>>>(off, m, wave_hsum.as<int32_t>());
-            HP_LAUNCHED();
-            k_scan_top<<<1, 1024, 0, s>>>(wave_hsum.as<int32_t>(), nbk);
-            HP_LAUNCHED();
-            k_scan_add<<<nbk, 1024, 0, s>>>(off, m, wave_hsum.as<int32_t>());
-            HP_LAUNCHED();
-            k_wave_base<<<1, 64, 0, s>>>(ba, off);
-            HP_LAUNCHED();
+            if (nbk == 1) {
+                k_wave_ids_one<<<1, 1024, 0, s>>>(small0.as<Task>(), ba, off, m);
+                HP_LAUNCHED();
+            } else {
+                k_wave_need<<<std::min(blocks(m, 256), 4096), 256, 0, s>>>(small0.as<Task>(), ba,
+                                                                           off, m);
+                HP_LAUNCHED();
+                k_scan_blocks<<<nbk, 1024, 0, s>>>(off, m, wave_hsum.as<int32_t>());
+                HP_LAUNCHED();
+                k_scan_top<<<1, 1024, 0, s>>>(wave_hsum.as<int32_t>(), nbk);
+                HP_LAUNCHED();
+                k_scan_add<<<nbk, 1024, 0, s>>>(off, m, wave_hsum.as<int32_t>());
+                HP_LAUNCHED();
+                k_wave_base<<<1, 64, 0, s>>>(ba, off);
+                HP_LAUNCHED();
+            }
         }
         // recorded steps: the other sizes and the wavefront subtrees on their
         // own streams, beside the largest (their task lists and node ids are
@@ -1029,9 +1035,9 @@ struct Engine {
         return false;
     }
 
-    int check_walk() {
+    int check_walk(bool read = true) {
         int r;
-        if ((r = read_counters())) return r;
+        if (read && (r = read_counters())) return r;
         if (h_counters[kErrFlags])
             return set_error(HP_EINVAL, "walk stopped early (flags %d): corrupt tree",
                              h_counters[kErrFlags]);
@@ -1043,8 +1049,10 @@ struct Engine {
     // `permute`, the records end in tree order, as the reference's
     // compute_potential leaves them (the bound pass reads the tree order
     // itself and runs without).
+    // With `staged`, the permuted records are left in the engine's stage
+    // buffer (*staged points there) instead of being copied back to d_po.
     int potential(const Tables &T, struct potential *d_po, const hp_options &o,
-                  bool permute = false) {
+                  bool permute = false, const struct potential **staged = nullptr) {
         st = hp_stats();
         st.records = T.nrec;
         st.halos = T.nhalos;
@@ -1079,11 +1087,16 @@ struct Engine {
             k_tree_records<<<blocks(T.nrec, 256), 256, 0, s>>>(d_po, F.as<float4>(), T.nrec,
                                                                stage.as<struct potential>());
             HP_LAUNCHED();
-            HP_TRY(hipMemcpyAsync(d_po, stage.p, sizeof(struct potential) * T.nrec,
-                                  hipMemcpyDeviceToDevice, s));
+            if (staged)
+                *staged = stage.as<struct potential>();
+            else
+                HP_TRY(hipMemcpyAsync(d_po, stage.p, sizeof(struct potential) * T.nrec,
+                                      hipMemcpyDeviceToDevice, s));
         }
+        // one readback for the build's and the walk's state when the build
+        // was not checked before the walk
         if (!build_checked && ((r = read_counters()) || (r = check_build()))) return r;
-        if ((r = check_walk())) return r;
+        if ((r = check_walk(build_checked))) return r;
         st.walk_groups = (int64_t)T.groups.size();
         st.walk_launches = T.groups.empty() ? 0 : 1;
         if (o.count_work && (r = read_work())) return r;
@@ -1762,9 +1775,10 @@ __attribute__((visibility("default"))) void compute_potential(struct potential *
         set_error(HP_EHIP, "copy of %lld records to the device failed", (long long)num_po);
         return fatal(HP_EHIP, __FILE__, __LINE__);
     }
-    if ((r = e->potential(T, dpo.as<struct potential>(), o, true)))
+    const struct potential *res = dpo.as<struct potential>();
+    if ((r = e->potential(T, dpo.as<struct potential>(), o, true, &res)))
         return fatal(r, __FILE__, __LINE__);
-    if (hipMemcpyAsync(po, dpo.p, sizeof(struct potential) * num_po, hipMemcpyDeviceToHost,
+    if (hipMemcpyAsync(po, res, sizeof(struct potential) * num_po, hipMemcpyDeviceToHost,
                        e->stream) != hipSuccess ||
         hipStreamSynchronize(e->stream) != hipSuccess) {
         set_error(HP_EHIP, "copy of %lld records from the device failed", (long long)num_po);

@@ -146,6 +146,32 @@ __global__ __launch_bounds__(1024) void k_scan_add(int32_t *__restrict__ v, int3
         if (base + k < m) v[base + k] += add;
 }
 
+// The wavefront subtrees' node ids (k_wave_need, the scan and k_wave_base)
+// in one launch, for a list of at most kScanBlock entries (a small batch or
+// one call): off[k] = the ids of the tasks before k.
+__global__ __launch_bounds__(1024) void k_wave_ids_one(const Task *__restrict__ tasks, BuildArgs a,
+                                                       int32_t *__restrict__ off, int32_t m) {
+    const int32_t ntasks = min(a.counters[kSmallNext], a.cap_small);
+    const int32_t base = threadIdx.x * 4;
+    int32_t x[4], s = 0;
+    for (int k = 0; k < 4; k++) {
+        const int32_t i = base + k;
+        x[k] = i < m && i < ntasks ? wave_ids(tasks[i].pad[1]) : 0;
+        s += x[k];
+    }
+    int32_t tot;
+    int32_t o = block_excl_scan_1024(s, &tot);
+    for (int k = 0; k < 4; k++) {
+        if (base + k < m) off[base + k] = o;
+        o += x[k];
+    }
+    if (threadIdx.x == 0) {
+        const int32_t base0 = a.counters[kNodeCount];
+        a.counters[kWaveIdBase] = base0;
+        a.counters[kNodeCount] = base0 + tot;
+    }
+}
+
 // stable scatter of one tile: its keys are loaded up front, then scattered
 // in rounds of 256 consecutive keys, wave-level multisplit by digit, running
 // per-digit bases across rounds
