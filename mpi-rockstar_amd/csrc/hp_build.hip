@@ -79,8 +79,12 @@ __global__ void k_fill_i32(int32_t *p, int32_t v, int64_t n) {
 // box keys per halo: [lo0 lo1 lo2 hi0 hi1 hi2] as order-preserving ints,
 // bad: non-finite count, pot: potential-only records (ke < 0) per call, and
 // pot[nhalos] > 0 when some call has any
-__global__ void k_root_init(int32_t *box, int32_t *bad, int32_t *pot, int nhalos) {
+// (and the build's counters: all zero but the node count, one id per call's
+// root; nothing reads them before k_roots)
+__global__ void k_root_init(int32_t *box, int32_t *bad, int32_t *pot, int nhalos,
+                            int32_t *counters) {
     int h = blockIdx.x * blockDim.x + threadIdx.x;
+    if (h < kNumCounters) counters[h] = h == kNodeCount ? nhalos : 0;
     if (h == 0) pot[nhalos] = 0;
     if (h >= nhalos) return;
     for (int k = 0; k < 3; k++) {

@@ -734,11 +734,10 @@ struct Engine {
     int enqueue_roots(const Tables &T, const struct potential *d_po) {
         const int32_t H = (int32_t)T.nhalos;
         hipStream_t s = stream;
-        if (H > 0) {
-            k_root_init<<<blocks(H, 256), 256, 0, s>>>(box.as<int32_t>(), bad.as<int32_t>(),
-                                                       pot.as<int32_t>(), H);
-            HP_LAUNCHED();
-        }
+        // the per-call state and the build's counters
+        k_root_init<<<blocks(std::max<int32_t>(H, kNumCounters), 256), 256, 0, s>>>(
+            box.as<int32_t>(), bad.as<int32_t>(), pot.as<int32_t>(), H, counters.as<int32_t>());
+        HP_LAUNCHED();
         if (!T.root_tiles.empty()) {
             k_gather<<<(int)T.root_tiles.size(), kTileThreads, 0, s>>>(
                 d_po, hfirst.as<int64_t>(), hcount.as<int64_t>(), root_tiles.as<RootTile>(),
@@ -746,10 +745,6 @@ struct Engine {
                 rkey.as<uint64_t>());
             HP_LAUNCHED();
         }
-        k_fill_i32<<<1, 64, 0, s>>>(counters.as<int32_t>(), 0, kNumCounters);
-        HP_LAUNCHED();
-        k_fill_i32<<<1, 1, 0, s>>>(counters.as<int32_t>() + kNodeCount, H, 1);
-        HP_LAUNCHED();
         if (H > 0) {
             k_roots<<<blocks(H, 256), 256, 0, s>>>(hfirst.as<int64_t>(), hcount.as<int64_t>(),
                                                    box.as<int32_t>(), bad.as<int32_t>(),
