@@ -1030,6 +1030,13 @@ struct Engine {
         return false;
     }
 
+    // the checks of a deferred pass, once h_counters holds its counters
+    int finish_checks() {
+        int r;
+        if ((r = check_build()) || (r = check_walk(false))) return r;
+        return HP_OK;
+    }
+
     int check_walk(bool read = true) {
         int r;
         if (read && (r = read_counters())) return r;
@@ -1046,8 +1053,13 @@ struct Engine {
     // itself and runs without).
     // With `staged`, the permuted records are left in the engine's stage
     // buffer (*staged points there) instead of being copied back to d_po.
+    // With `deferred`, a pass without split levels reads no counters back:
+    // *deferred is set, and the caller reads them with its own copy back
+    // and calls finish_checks().
     int potential(const Tables &T, struct potential *d_po, const hp_options &o,
-                  bool permute = false, const struct potential **staged = nullptr) {
+                  bool permute = false, const struct potential **staged = nullptr,
+                  bool *deferred = nullptr) {
+        if (deferred) *deferred = false;
         st = hp_stats();
         st.records = T.nrec;
         st.halos = T.nhalos;
@@ -1090,6 +1102,12 @@ struct Engine {
         }
         // one readback for the build's and the walk's state when the build
         // was not checked before the walk
+        if (!build_checked && deferred && !timing && !o.count_work) {
+            *deferred = true;
+            st.walk_groups = (int64_t)T.groups.size();
+            st.walk_launches = T.groups.empty() ? 0 : 1;
+            return HP_OK;
+        }
         if (!build_checked && ((r = read_counters()) || (r = check_build()))) return r;
         if ((r = check_walk(build_checked))) return r;
         st.walk_groups = (int64_t)T.groups.size();
@@ -1771,14 +1789,19 @@ __attribute__((visibility("default"))) void compute_potential(struct potential *
         return fatal(HP_EHIP, __FILE__, __LINE__);
     }
     const struct potential *res = dpo.as<struct potential>();
-    if ((r = e->potential(T, dpo.as<struct potential>(), o, true, &res)))
+    bool deferred = false;
+    if ((r = e->potential(T, dpo.as<struct potential>(), o, true, &res, &deferred)))
         return fatal(r, __FILE__, __LINE__);
+    // the records and, for a deferred pass, the counters, with one wait
     if (hipMemcpyAsync(po, res, sizeof(struct potential) * num_po, hipMemcpyDeviceToHost,
                        e->stream) != hipSuccess ||
+        (deferred && hipMemcpyAsync(e->h_counters, e->counters.p, sizeof(int32_t) * kNumCounters,
+                                    hipMemcpyDeviceToHost, e->stream) != hipSuccess) ||
         hipStreamSynchronize(e->stream) != hipSuccess) {
         set_error(HP_EHIP, "copy of %lld records from the device failed", (long long)num_po);
         return fatal(HP_EHIP, __FILE__, __LINE__);
     }
+    if (deferred && (r = e->finish_checks())) return fatal(r, __FILE__, __LINE__);
 }
 
 __attribute__((visibility("default"))) int hp_compute_potential_noalloc(struct potential *d_po,
