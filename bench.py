@@ -465,7 +465,8 @@ def main() -> int:
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU: this build has no multi-GPU path (asked for "
-              f"--gpus {args.gpus}); halos would shard across GPUs by call, see DESIGN.md",
+              f"--gpus {args.gpus}); the calls would be dealt to the GPUs whole, each "
+              f"GPU running this one-GPU step on its share",
               file=sys.stderr)
         return 2
 
