@@ -194,28 +194,44 @@ def run_drop_in(cat, seconds: float):
     lock = threading.Lock()
     nxt = [0]
 
+    # one pool for every pass, as the reference's OpenMP threads: each keeps
+    # its engine (and stream) from pass to pass
+    start, done = threading.Barrier(nt + 1), threading.Barrier(nt + 1)
+    stop, errors = [False], []
+
     def worker():
         while True:
-            with lock:
-                h = nxt[0]
-                nxt[0] += 1
-            if h >= cat.n_halos:
+            start.wait()
+            if stop[0]:
                 return
-            n = int(count[h])
-            if n <= 0:
-                continue
-            ptr = base + int(first[h]) * rec
-            if n > 1:
-                L.compute_potential(ptr, n)
-            L.compute_kinetic_energy(ptr, n, vbase + 12 * h, cbase + 12 * h, scale)
+            try:
+                while True:
+                    with lock:
+                        h = nxt[0]
+                        nxt[0] += 1
+                    if h >= cat.n_halos:
+                        break
+                    n = int(count[h])
+                    if n <= 0:
+                        continue
+                    ptr = base + int(first[h]) * rec
+                    if n > 1:
+                        L.compute_potential(ptr, n)
+                    L.compute_kinetic_energy(ptr, n, vbase + 12 * h, cbase + 12 * h, scale)
+            except Exception as exc:  # reported by the pass, after every thread is back
+                errors.append(exc)
+            done.wait()
+
+    ts = [threading.Thread(target=worker, daemon=True) for _ in range(nt)]
+    for t in ts:
+        t.start()
 
     def one():
         nxt[0] = 0
-        ts = [threading.Thread(target=worker) for _ in range(nt)]
-        for t in ts:
-            t.start()
-        for t in ts:
-            t.join()
+        start.wait()
+        done.wait()
+        if errors:
+            raise errors[0]
 
     one()  # warm: every thread's engine and scratch
     passes, busy = 0, 0.0
@@ -224,6 +240,10 @@ def run_drop_in(cat, seconds: float):
         one()
         busy += time.perf_counter() - t0
         passes += 1
+    stop[0] = True
+    start.wait()
+    for t in ts:
+        t.join()
     return {"value": passes * cat.n_records / busy, "unit": UNIT,
             "ms_per_pass": 1e3 * busy / passes, "passes": passes, "threads": nt,
             "calls": "compute_potential + compute_kinetic_energy per halo, host buffers"}
